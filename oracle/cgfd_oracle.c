@@ -1,0 +1,831 @@
+/*
+ * cgfd_oracle.c -- CPU restatement of the reference's isotropic elastic
+ * hot path.  TEST INFRASTRUCTURE ONLY (see cgfd_oracle.h).
+ *
+ * Every function follows the cited reference loop expression by expression,
+ * including where the reference evaluates in double and rounds to float, so
+ * that the FP32 build reproduces the reference bit for bit (it is compiled
+ * with -ffp-contract=off, as x86-64 -O3 code has no FMA).  The one deliberate
+ * difference: the traction-image mirror on the surface row with the forward
+ * z operator reads the real grid points below the window (the reference's
+ * own alternative formula, sv_curv_col_el.c:151-157) where the reference
+ * indexes veczt[-1] and veczt[-2] out of bounds (sv_curv_col_el.c:160-163).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "cgfd_oracle.h"
+
+#ifndef PI
+#define PI 3.14159265358979323846264338327950288419716939937510
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* scheme tables: fd_t.c:36-49 (RK4), :75-131 (one-sided ops), :185-236     */
+/* (pair schedule)                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* [layer][dir][n]; layer 3 is the interior DRP op, 0..2 the near-surface ones */
+static const int op_indx[4][2][5] = {
+  {{0, 1, 0, 0, 0}, {-1, 0, 0, 0, 0}},
+  {{0, 1, 0, 0, 0}, {-1, 0, 0, 0, 0}},
+  {{0, 1, 2, 0, 0}, {-2, -1, 0, 0, 0}},
+  {{-1, 0, 1, 2, 3}, {-3, -2, -1, 0, 1}}};
+static const double op_coef[4][2][5] = {
+  {{-1.0, 1.0, 0, 0, 0}, {-1.0, 1.0, 0, 0, 0}},
+  {{-1.0, 1.0, 0, 0, 0}, {-1.0, 1.0, 0, 0, 0}},
+  {{-7.0 / 6.0, 8.0 / 6.0, -1.0 / 6.0, 0, 0}, {1.0 / 6.0, -8.0 / 6.0, 7.0 / 6.0, 0, 0}},
+  {{-0.30874, -0.6326, 1.233, -0.3334, 0.04168},
+   {-0.04168, 0.3334, -1.233, 0.6326, 0.30874}}};
+static const int op_len[4] = {2, 2, 3, 5};
+/* forward(0)/backward(1) flag per pair for x,y,z; per stage dir=(flag+s)%2 */
+static const int pair_flags[8][3] = {
+  {0, 0, 0}, {1, 1, 0}, {1, 1, 1}, {0, 0, 1},
+  {0, 1, 0}, {1, 0, 0}, {1, 0, 1}, {0, 1, 1}};
+static const double rk_a_tab[3] = {0.5, 0.5, 1.0};
+static const double rk_b_tab[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+static const double rk_t_tab[4] = {0.0, 0.5, 0.5, 1.0};
+/* centred 7-point op for the metric, fd_t.c:161-171 (row 3) */
+static const double fdc_coef[7] = {-0.02084, 0.1667, -0.7709, 0.0, 0.7709, -0.1667, 0.02084};
+
+int orc_fd_ops(int *indx, float *coef, int *len, float *rk_a, float *rk_b, float *rk_t)
+{
+  for (int p = 0; p < 8; p++)
+    for (int s = 0; s < 4; s++)
+      for (int slot = 0; slot < 6; slot++) {
+        int d = slot < 2 ? slot : 2;
+        int dir = (pair_flags[p][d] + s) % 2;
+        int lay = slot < 2 ? 3 : slot - 2;
+        int base = (p * 4 + s) * 6 + slot;
+        len[base] = op_len[lay];
+        for (int n = 0; n < 5; n++) {
+          indx[base * 5 + n] = n < op_len[lay] ? op_indx[lay][dir][n] : 0;
+          coef[base * 5 + n] = n < op_len[lay] ? (float)op_coef[lay][dir][n] : 0.0f;
+        }
+      }
+  for (int s = 0; s < 4; s++) {
+    rk_b[s] = (float)rk_b_tab[s];
+    rk_t[s] = (float)rk_t_tab[s];
+    if (s < 3) rk_a[s] = (float)rk_a_tab[s];
+  }
+  return 8;
+}
+
+/* ------------------------------------------------------------------------ */
+/* setup: PML profile (bdry_t.c:237-269, 333-366), average spacing          */
+/* (bdry_t.c:399-503), regions (bdry_t.c:160-186)                           */
+/* ------------------------------------------------------------------------ */
+
+void orc_pml_profile(int num_layers, float L0, float dh, float vel,
+                     float amax, float bmax, int iside,
+                     float *A, float *B, float *D)
+{
+  int npoints = num_layers + 1;
+  float num_lay = npoints - 1;
+  float Rpp = (float)(pow(10, -((log10((double)num_lay) - 1.0) / log10(2.0) + 4.0)));
+  float dmax = (float)(-vel / (2.0 * L0) * log(Rpp) * (2.0f + 1.0));
+  for (int n = 0; n < npoints; n++) {
+    float L = n * dh;
+    int i = iside == 0 ? npoints - 1 - n : n;
+    D[i] = (L < 0) ? 0.0f : (float)(dmax * pow(L / L0, 2.0f));
+    A[i] = (L < 0) ? 0.0f : (float)(amax * (1.0 - pow(L / L0, 1.0f)));
+    B[i] = (L < 0) ? 1.0f : (float)(1.0 + (bmax - 1.0) * pow(L / L0, 2.0f));
+    D[i] /= B[i];
+    A[i] += D[i];
+    B[i] = 1.0 / B[i];
+  }
+}
+
+void orc_pml_region(int ni, int nj, int nk, const int pml_layers[6],
+                    const int pml_on[6], int region[6][6])
+{
+  int ni1 = ORC_NGHOST, ni2 = ni1 + ni - 1;
+  int nj1 = ORC_NGHOST, nj2 = nj1 + nj - 1;
+  int nk1 = ORC_NGHOST, nk2 = nk1 + nk - 1;
+  for (int f = 0; f < 6; f++) {
+    int L = pml_on[f] ? pml_layers[f] : 0;
+    int *r = region[f];
+    r[0] = ni1; r[1] = ni2; r[2] = nj1; r[3] = nj2; r[4] = nk1; r[5] = nk2;
+    if (f == 0) r[1] = ni1 + L;
+    if (f == 1) r[0] = ni2 - L;
+    if (f == 2) r[3] = nj1 + L;
+    if (f == 3) r[2] = nj2 - L;
+    if (f == 4) r[5] = nk1 + L;
+    if (f == 5) r[4] = nk2 - L;
+  }
+}
+
+float orc_pml_avg_dh(int ni, int nj, int nk, const float *x3d, const float *y3d,
+                     const float *z3d, const int r[6], int idim, float *L0)
+{
+  size_t nx = ni + 6, ny = nj + 6;
+  size_t sl = nx, ss = nx * ny;
+  double L = 0.0;
+  int num = 0;
+  /* walk each grid line of the region along idim, summing segment lengths */
+  int a1, a2, b1, b2, c1, c2;      /* outer, middle, along-line ranges */
+  size_t so, sm, sa;
+  if (idim == 0) { a1 = r[4]; a2 = r[5]; so = ss; b1 = r[2]; b2 = r[3]; sm = sl; c1 = r[0]; c2 = r[1]; sa = 1; }
+  else if (idim == 1) { a1 = r[4]; a2 = r[5]; so = ss; b1 = r[0]; b2 = r[1]; sm = 1; c1 = r[2]; c2 = r[3]; sa = sl; }
+  else { a1 = r[2]; a2 = r[3]; so = sl; b1 = r[0]; b2 = r[1]; sm = 1; c1 = r[4]; c2 = r[5]; sa = ss; }
+  for (int a = a1; a <= a2; a++)
+    for (int b = b1; b <= b2; b++) {
+      size_t p = (size_t)a * so + (size_t)b * sm + (size_t)c1 * sa;
+      double x0 = x3d[p], y0 = y3d[p], z0 = z3d[p];
+      for (int c = c1 + 1; c <= c2; c++) {
+        size_t q = (size_t)a * so + (size_t)b * sm + (size_t)c * sa;
+        double x1 = x3d[q], y1 = y3d[q], z1 = z3d[q];
+        L += sqrt((x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0) + (z1 - z0) * (z1 - z0));
+        x0 = x1; y0 = y1; z0 = z1;
+        num += 1;
+      }
+    }
+  float dh = (float)(L / num);
+  *L0 = dh * (c2 - c1);
+  return dh;
+}
+
+/* ------------------------------------------------------------------------ */
+/* setup: free-surface matrices, sv_curv_col_el_iso.c:1259-1375 with        */
+/* fdlib_math_invert3x3 / matmul3x3 (fdlib_math.c:7-49)                     */
+/* ------------------------------------------------------------------------ */
+
+static void inv3(float m[3][3])
+{
+  float inv[3][3], det;
+  inv[0][0] = m[1][1] * m[2][2] - m[2][1] * m[1][2];
+  inv[0][1] = m[2][1] * m[0][2] - m[0][1] * m[2][2];
+  inv[0][2] = m[0][1] * m[1][2] - m[0][2] * m[1][1];
+  inv[1][0] = m[1][2] * m[2][0] - m[1][0] * m[2][2];
+  inv[1][1] = m[0][0] * m[2][2] - m[2][0] * m[0][2];
+  inv[1][2] = m[1][0] * m[0][2] - m[0][0] * m[1][2];
+  inv[2][0] = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+  inv[2][1] = m[2][0] * m[0][1] - m[0][0] * m[2][1];
+  inv[2][2] = m[0][0] * m[1][1] - m[0][1] * m[1][0];
+  det = inv[0][0] * m[0][0] + inv[0][1] * m[1][0] + inv[0][2] * m[2][0];
+  det = 1.0f / det;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) m[i][j] = inv[i][j] * det;
+}
+
+static void mul3(float A[3][3], float B[3][3], float C[3][3])
+{
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      C[i][j] = 0.0;
+      for (int k = 0; k < 3; k++) C[i][j] += A[i][k] * B[k][j];
+    }
+}
+
+void orc_dvh2dvz_iso(int ni, int nj, int nk, const float *metric10,
+                     const float *med3, float *mats)
+{
+  size_t nx = ni + 6, ny = nj + 6, nz = nk + 6;
+  size_t vol = nx * ny * nz, ss = nx * ny;
+  const float *xi_x = metric10 + vol, *xi_y = metric10 + 2 * vol, *xi_z = metric10 + 3 * vol;
+  const float *et_x = metric10 + 4 * vol, *et_y = metric10 + 5 * vol, *et_z = metric10 + 6 * vol;
+  const float *zt_x = metric10 + 7 * vol, *zt_y = metric10 + 8 * vol, *zt_z = metric10 + 9 * vol;
+  const float *lam3d = med3, *mu3d = med3 + vol;
+  float *mVx = mats, *mVy = mats + ss * 9, *mF = mats + 2 * ss * 9;
+  size_t k = ORC_NGHOST + nk - 1;
+  for (size_t j = ORC_NGHOST; j < ORC_NGHOST + (size_t)nj; j++)
+    for (size_t i = ORC_NGHOST; i < ORC_NGHOST + (size_t)ni; i++) {
+      size_t p = i + j * nx + k * ss;
+      float e11 = xi_x[p], e12 = xi_y[p], e13 = xi_z[p];
+      float e21 = et_x[p], e22 = et_y[p], e23 = et_z[p];
+      float e31 = zt_x[p], e32 = zt_y[p], e33 = zt_z[p];
+      float lam = lam3d[p], mu = mu3d[p];
+      float lam2mu = lam + 2.0f * mu;
+      float A[3][3], B[3][3], C[3][3], AB[3][3], AC[3][3];
+      A[0][0] = lam2mu * e31 * e31 + mu * (e32 * e32 + e33 * e33);
+      A[0][1] = lam * e31 * e32 + mu * e32 * e31;
+      A[0][2] = lam * e31 * e33 + mu * e33 * e31;
+      A[1][0] = lam * e32 * e31 + mu * e31 * e32;
+      A[1][1] = lam2mu * e32 * e32 + mu * (e31 * e31 + e33 * e33);
+      A[1][2] = lam * e32 * e33 + mu * e33 * e32;
+      A[2][0] = lam * e33 * e31 + mu * e31 * e33;
+      A[2][1] = lam * e33 * e32 + mu * e32 * e33;
+      A[2][2] = lam2mu * e33 * e33 + mu * (e31 * e31 + e32 * e32);
+      inv3(A);
+      B[0][0] = -lam2mu * e31 * e11 - mu * (e32 * e12 + e33 * e13);
+      B[0][1] = -lam * e31 * e12 - mu * e32 * e11;
+      B[0][2] = -lam * e31 * e13 - mu * e33 * e11;
+      B[1][0] = -lam * e32 * e11 - mu * e31 * e12;
+      B[1][1] = -lam2mu * e32 * e12 - mu * (e31 * e11 + e33 * e13);
+      B[1][2] = -lam * e32 * e13 - mu * e33 * e12;
+      B[2][0] = -lam * e33 * e11 - mu * e31 * e13;
+      B[2][1] = -lam * e33 * e12 - mu * e32 * e13;
+      B[2][2] = -lam2mu * e33 * e13 - mu * (e31 * e11 + e32 * e12);
+      C[0][0] = -lam2mu * e31 * e21 - mu * (e32 * e22 + e33 * e23);
+      C[0][1] = -lam * e31 * e22 - mu * e32 * e21;
+      C[0][2] = -lam * e31 * e23 - mu * e33 * e21;
+      C[1][0] = -lam * e32 * e21 - mu * e31 * e22;
+      C[1][1] = -lam2mu * e32 * e22 - mu * (e31 * e21 + e33 * e23);
+      C[1][2] = -lam * e32 * e23 - mu * e33 * e22;
+      C[2][0] = -lam * e33 * e21 - mu * e31 * e23;
+      C[2][1] = -lam * e33 * e22 - mu * e32 * e23;
+      C[2][2] = -lam2mu * e33 * e23 - mu * (e31 * e21 + e32 * e22);
+      mul3(A, B, AB);
+      mul3(A, C, AC);
+      size_t ij = (j * nx + i) * 9;
+      for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+          mVx[ij + r * 3 + c] = AB[r][c];
+          mVy[ij + r * 3 + c] = AC[r][c];
+          mF[ij + r * 3 + c] = A[r][c];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* setup: metric, gd_t.c:191-402 (7-point centred op, cross products,       */
+/* ghost mirroring in x1,x2,y1,y2,z1,z2 order)                              */
+/* ------------------------------------------------------------------------ */
+
+void orc_metric(int ni, int nj, int nk, const float *xyz, float *metric10)
+{
+  size_t nx = ni + 6, ny = nj + 6, nz = nk + 6;
+  size_t vol = nx * ny * nz, sl = nx, ss = nx * ny;
+  const float *x3d = xyz, *y3d = xyz + vol, *z3d = xyz + 2 * vol;
+  float c[7];
+  for (int n = 0; n < 7; n++) c[n] = (float)fdc_coef[n];
+  size_t sh[3] = {1, sl, ss};
+  int ni1 = 3, ni2 = 3 + ni - 1, nj1 = 3, nj2 = 3 + nj - 1, nk1 = 3, nk2 = 3 + nk - 1;
+  for (size_t k = nk1; k <= (size_t)nk2; k++)
+    for (size_t j = nj1; j <= (size_t)nj2; j++)
+      for (size_t i = ni1; i <= (size_t)ni2; i++) {
+        size_t p = i + j * sl + k * ss;
+        float d[3][3]; /* d[axis][coord]: x_xi,y_xi,z_xi / x_et.. / x_zt.. */
+        for (int a = 0; a < 3; a++) {
+          const float *v[3] = {x3d, y3d, z3d};
+          for (int cc = 0; cc < 3; cc++) {
+            float s = c[0] * v[cc][p + (long)(-3) * (long)sh[a]];
+            for (int n = 1; n < 7; n++) s += c[n] * v[cc][p + (long)(n - 3) * (long)sh[a]];
+            d[a][cc] = s;
+          }
+        }
+        float *v1 = d[0], *v2 = d[1], *v3 = d[2], g[3];
+        g[0] = v1[1] * v2[2] - v1[2] * v2[1];
+        g[1] = v1[2] * v2[0] - v1[0] * v2[2];
+        g[2] = v1[0] * v2[1] - v1[1] * v2[0];
+        float jac = 0.0;
+        for (int n = 0; n < 3; n++) jac += g[n] * v3[n];
+        metric10[p] = jac;
+        g[0] = v2[1] * v3[2] - v2[2] * v3[1];
+        g[1] = v2[2] * v3[0] - v2[0] * v3[2];
+        g[2] = v2[0] * v3[1] - v2[1] * v3[0];
+        for (int n = 0; n < 3; n++) metric10[(1 + n) * vol + p] = g[n] / jac;
+        g[0] = v3[1] * v1[2] - v3[2] * v1[1];
+        g[1] = v3[2] * v1[0] - v3[0] * v1[2];
+        g[2] = v3[0] * v1[1] - v3[1] * v1[0];
+        for (int n = 0; n < 3; n++) metric10[(4 + n) * vol + p] = g[n] / jac;
+        g[0] = v1[1] * v2[2] - v1[2] * v2[1];
+        g[1] = v1[2] * v2[0] - v1[0] * v2[2];
+        g[2] = v1[0] * v2[1] - v1[1] * v2[0];
+        for (int n = 0; n < 3; n++) metric10[(7 + n) * vol + p] = g[n] / jac;
+      }
+  for (int m = 0; m < 10; m++) {
+    float *v = metric10 + m * vol;
+    for (size_t k = 0; k < nz; k++)
+      for (size_t j = 0; j < ny; j++) {
+        for (size_t i = 0; i < (size_t)ni1; i++) { size_t p = i + j * sl + k * ss; v[p] = v[p + (ni1 - i) * 2 - 1]; }
+        for (size_t i = ni2 + 1; i < nx; i++) { size_t p = i + j * sl + k * ss; v[p] = v[p - (i - ni2) * 2 + 1]; }
+      }
+    for (size_t k = 0; k < nz; k++) {
+      for (size_t j = 0; j < (size_t)nj1; j++)
+        for (size_t i = 0; i < nx; i++) { size_t p = i + j * sl + k * ss; v[p] = v[p + ((nj1 - j) * 2 - 1) * sl]; }
+      for (size_t j = nj2 + 1; j < ny; j++)
+        for (size_t i = 0; i < nx; i++) { size_t p = i + j * sl + k * ss; v[p] = v[p - ((j - nj2) * 2 - 1) * sl]; }
+    }
+    for (size_t k = 0; k < (size_t)nk1; k++)
+      for (size_t j = 0; j < ny; j++)
+        for (size_t i = 0; i < nx; i++) { size_t p = i + j * sl + k * ss; v[p] = v[p + ((nk1 - k) * 2 - 1) * ss]; }
+    for (size_t k = nk2 + 1; k < nz; k++)
+      for (size_t j = 0; j < ny; j++)
+        for (size_t i = 0; i < nx; i++) { size_t p = i + j * sl + k * ss; v[p] = v[p - ((k - nk2) * 2 - 1) * ss]; }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* setup: source pieces, src_t.c:2111-2151 (Gaussian weights), :2159-2166   */
+/* (Ricker), :2179-2184 (Gaussian), :2344-2383 (strike/dip/rake)            */
+/* ------------------------------------------------------------------------ */
+
+static float fgauss(float t, float a, float t0)
+{
+  float f = exp(-(t - t0) * (t - t0) / (a * a)) / (sqrtf(PI) * a);
+  return f;
+}
+
+void orc_gauss_weights(float *w, float x0, float y0, float z0, float a,
+                       int half, int half_below)
+{
+  int n = 0;
+  for (int k = -half; k <= half; k++)
+    for (int j = -half; j <= half; j++)
+      for (int i = -half; i <= half; i++) {
+        float D1 = fgauss(i - x0, a, 0.0);
+        float D2 = fgauss(j - y0, a, 0.0);
+        float D3 = fgauss(k - z0, a, 0.0);
+        w[n++] = D1 * D2 * D3;
+      }
+  float sum = 0.0;
+  n = 0;
+  for (int k = -half; k <= half_below; k++)
+    for (int j = -half; j <= half; j++)
+      for (int i = -half; i <= half; i++) sum += w[n++];
+  int len = 2 * half + 1;
+  for (int m = 0; m < len * len * len; m++) w[m] /= sum;
+}
+
+float orc_ricker(float t, float fc, float t0)
+{
+  float u = (t - t0) * 2.0 * PI * fc;
+  float v = (1 - u * u / 2.0) * exp(-u * u / 4.0);
+  return v;
+}
+
+void orc_angle2moment(float strike, float dip, float rake, float m[6])
+{
+  float dp = dip / 180.0 * PI, st = strike / 180.0 * PI, rk = rake / 180.0 * PI;
+  float M11 = -(sin(dp) * cos(rk) * sin(2.0 * st) + sin(2.0 * dp) * sin(rk) * sin(st) * sin(st));
+  float M22 = sin(dp) * cos(rk) * sin(2.0 * st) - sin(2.0 * dp) * sin(rk) * cos(st) * cos(st);
+  float M33 = -(M11 + M22);
+  float M12 = sin(dp) * cos(rk) * cos(2.0 * st) + 0.5 * sin(2.0 * dp) * sin(rk) * sin(2.0 * st);
+  float M13 = -(cos(dp) * cos(rk) * cos(st) + cos(2.0 * dp) * sin(rk) * sin(st));
+  float M23 = -(cos(dp) * cos(rk) * sin(st) - cos(2.0 * dp) * sin(rk) * cos(st));
+  m[0] = M22; m[1] = M11; m[2] = M33; m[3] = -M13; m[4] = -M23; m[5] = M12;
+}
+
+/* ------------------------------------------------------------------------ */
+/* solver state                                                             */
+/* ------------------------------------------------------------------------ */
+
+struct orc_s {
+  orc_cfg cfg;
+  int nx, ny, nz;
+  size_t vol, sl, ss;
+  int ni1, ni2, nj1, nj2, nk1, nk2;
+  real_t *metric;           /* 10 volumes */
+  real_t *med;              /* lambda, mu, slw */
+  real_t *mats;             /* matVx2Vz2, matVy2Vz2, matF2Vz2 */
+  int region[6][6];
+  size_t aux_off[6], aux_n[6], aux_total;
+  real_t *pA[6], *pB[6], *pD[6];
+  real_t *srcm;             /* 6 x nsrc*max_nt*4 */
+};
+
+size_t orc_aux_len(const orc_t *o) { return o->aux_total; }
+size_t orc_vol(const orc_t *o) { return o->vol; }
+
+orc_t *orc_create(const orc_cfg *cfg, const float *metric10, const float *med3)
+{
+  orc_t *o = (orc_t *)calloc(1, sizeof(orc_t));
+  o->cfg = *cfg;
+  o->nx = cfg->ni + 6; o->ny = cfg->nj + 6; o->nz = cfg->nk + 6;
+  o->sl = o->nx; o->ss = (size_t)o->nx * o->ny; o->vol = o->ss * o->nz;
+  o->ni1 = 3; o->ni2 = 3 + cfg->ni - 1;
+  o->nj1 = 3; o->nj2 = 3 + cfg->nj - 1;
+  o->nk1 = 3; o->nk2 = 3 + cfg->nk - 1;
+  o->metric = (real_t *)malloc(10 * o->vol * sizeof(real_t));
+  o->med = (real_t *)malloc(3 * o->vol * sizeof(real_t));
+  for (size_t p = 0; p < 10 * o->vol; p++) o->metric[p] = metric10[p];
+  for (size_t p = 0; p < 3 * o->vol; p++) o->med[p] = med3[p];
+  /* free-surface matrices, computed in the oracle's own precision */
+  size_t nm = o->ss * 9;
+  o->mats = (real_t *)calloc(3 * nm, sizeof(real_t));
+  if (cfg->free_top) {
+    float *m = (float *)calloc(3 * nm, sizeof(float));
+    orc_dvh2dvz_iso(cfg->ni, cfg->nj, cfg->nk, metric10, med3, m);
+    for (size_t p = 0; p < 3 * nm; p++) o->mats[p] = m[p];
+    free(m);
+  }
+  orc_pml_region(cfg->ni, cfg->nj, cfg->nk, cfg->pml_layers, cfg->pml_on, o->region);
+  size_t off = 0;
+  for (int f = 0; f < 6; f++) {
+    int *r = o->region[f];
+    o->aux_off[f] = off;
+    o->aux_n[f] = 0;
+    if (!cfg->pml_on[f]) continue;
+    size_t n = (size_t)(r[1] - r[0] + 1) * (r[3] - r[2] + 1) * (r[5] - r[4] + 1);
+    o->aux_n[f] = n;
+    off += 9 * n;
+    int np = cfg->pml_layers[f] + 1;
+    o->pA[f] = (real_t *)malloc(np * sizeof(real_t));
+    o->pB[f] = (real_t *)malloc(np * sizeof(real_t));
+    o->pD[f] = (real_t *)malloc(np * sizeof(real_t));
+    for (int q = 0; q < np; q++) {
+      o->pA[f][q] = cfg->pml_A[f][q];
+      o->pB[f][q] = cfg->pml_B[f][q];
+      o->pD[f][q] = cfg->pml_D[f][q];
+    }
+  }
+  o->aux_total = off;
+  if (cfg->nsrc > 0) {
+    size_t n = (size_t)cfg->nsrc * cfg->src_max_nt * 4;
+    o->srcm = (real_t *)malloc(6 * n * sizeof(real_t));
+    for (size_t p = 0; p < 6 * n; p++) o->srcm[p] = cfg->src_mij[p];
+  }
+  return o;
+}
+
+void orc_destroy(orc_t *o)
+{
+  if (!o) return;
+  free(o->metric); free(o->med); free(o->mats); free(o->srcm);
+  for (int f = 0; f < 6; f++) { free(o->pA[f]); free(o->pB[f]); free(o->pD[f]); }
+  free(o);
+}
+
+/* wavefield component slots (wav_t.c:93-136) */
+enum { VX = 0, VY, VZ, TXX, TYY, TZZ, TYZ, TXZ, TXY };
+
+typedef struct { int len; long sh[5]; real_t c[5]; } op_t;
+
+static void make_op(op_t *op, int lay, int dir, long stride)
+{
+  op->len = op_len[lay];
+  for (int n = 0; n < op->len; n++) {
+    op->sh[n] = op_indx[lay][dir][n] * stride;
+    op->c[n] = (real_t)(float)op_coef[lay][dir][n];
+  }
+#if ORC_REAL_IS_DOUBLE
+  for (int n = 0; n < op->len; n++) op->c[n] = op_coef[lay][dir][n];
+#endif
+}
+
+/* M_FD_SHIFT order: c0*v0, then += c_n*v_n (fd_t.h:11-31) */
+static inline real_t dop(const op_t *op, const real_t *v, size_t p)
+{
+  real_t d = op->c[0] * v[p + op->sh[0]];
+  for (int n = 1; n < op->len; n++) d += op->c[n] * v[p + op->sh[n]];
+  return d;
+}
+
+/* interior RHS, sv_curv_col_el_iso.c:209-441 */
+static void rhs_inner(orc_t *o, const real_t *w, real_t *h,
+                      const op_t *ox, const op_t *oy, const op_t *oz)
+{
+  size_t V = o->vol;
+  const real_t *M = o->metric, *md = o->med;
+#pragma omp parallel for schedule(static)
+  for (int k = o->nk1; k <= o->nk2; k++)
+    for (int j = o->nj1; j <= o->nj2; j++)
+      for (int i = o->ni1; i <= o->ni2; i++) {
+        size_t p = i + j * o->sl + k * o->ss;
+        real_t D[3][9];
+        for (int c = 0; c < 9; c++) {
+          D[0][c] = dop(ox, w + c * V, p);
+          D[1][c] = dop(oy, w + c * V, p);
+          D[2][c] = dop(oz, w + c * V, p);
+        }
+        real_t xix = M[V + p], xiy = M[2 * V + p], xiz = M[3 * V + p];
+        real_t etx = M[4 * V + p], ety = M[5 * V + p], etz = M[6 * V + p];
+        real_t ztx = M[7 * V + p], zty = M[8 * V + p], ztz = M[9 * V + p];
+        real_t lam = md[p], mu = md[V + p], slw = md[2 * V + p];
+        real_t lam2mu = lam + 2.0 * mu;
+        real_t (*X) = D[0], (*Y) = D[1], (*Z) = D[2];
+        h[VX * V + p] = slw * (xix * X[TXX] + xiy * X[TXY] + xiz * X[TXZ]
+                              + etx * Y[TXX] + ety * Y[TXY] + etz * Y[TXZ]
+                              + ztx * Z[TXX] + zty * Z[TXY] + ztz * Z[TXZ]);
+        h[VY * V + p] = slw * (xix * X[TXY] + xiy * X[TYY] + xiz * X[TYZ]
+                              + etx * Y[TXY] + ety * Y[TYY] + etz * Y[TYZ]
+                              + ztx * Z[TXY] + zty * Z[TYY] + ztz * Z[TYZ]);
+        h[VZ * V + p] = slw * (xix * X[TXZ] + xiy * X[TYZ] + xiz * X[TZZ]
+                              + etx * Y[TXZ] + ety * Y[TYZ] + etz * Y[TZZ]
+                              + ztx * Z[TXZ] + zty * Z[TYZ] + ztz * Z[TZZ]);
+        h[TXX * V + p] = lam2mu * (xix * X[VX] + etx * Y[VX] + ztx * Z[VX])
+                       + lam * (xiy * X[VY] + ety * Y[VY] + zty * Z[VY]
+                                + xiz * X[VZ] + etz * Y[VZ] + ztz * Z[VZ]);
+        h[TYY * V + p] = lam2mu * (xiy * X[VY] + ety * Y[VY] + zty * Z[VY])
+                       + lam * (xix * X[VX] + etx * Y[VX] + ztx * Z[VX]
+                                + xiz * X[VZ] + etz * Y[VZ] + ztz * Z[VZ]);
+        h[TZZ * V + p] = lam2mu * (xiz * X[VZ] + etz * Y[VZ] + ztz * Z[VZ])
+                       + lam * (xix * X[VX] + etx * Y[VX] + ztx * Z[VX]
+                                + xiy * X[VY] + ety * Y[VY] + zty * Z[VY]);
+        h[TXY * V + p] = mu * (xiy * X[VX] + xix * X[VY]
+                               + ety * Y[VX] + etx * Y[VY]
+                               + zty * Z[VX] + ztx * Z[VY]);
+        h[TXZ * V + p] = mu * (xiz * X[VX] + xix * X[VZ]
+                               + etz * Y[VX] + etx * Y[VZ]
+                               + ztz * Z[VX] + ztx * Z[VZ]);
+        h[TYZ * V + p] = mu * (xiz * X[VY] + xiy * X[VZ]
+                               + etz * Y[VY] + ety * Y[VZ]
+                               + ztz * Z[VY] + zty * Z[VZ]);
+      }
+}
+
+/* traction image for hV on the top layers, sv_curv_col_el.c:31-305.
+ * Surface traction (TxSrc..) is zero: no surface-force sources. */
+static void rhs_timg(orc_t *o, const real_t *w, real_t *h,
+                     const op_t *ox, const op_t *oy, const op_t *oz, const int *zidx)
+{
+  size_t V = o->vol;
+  const real_t *M = o->metric, *md = o->med;
+  const real_t *jac = M;
+  int zlen = oz->len;
+  int k_min = o->nk2 - zidx[zlen - 1];
+  /* traction rows: (Txx,Txy,Txz), (Txy,Tyy,Tyz), (Txz,Tyz,Tzz) */
+  static const int trow[3][3] = {{TXX, TXY, TXZ}, {TXY, TYY, TYZ}, {TXZ, TYZ, TZZ}};
+  for (int k = k_min; k <= o->nk2; k++) {
+    int n_free = o->nk2 - k - zidx[0];
+    for (int j = o->nj1; j <= o->nj2; j++)
+      for (int i = o->ni1; i <= o->ni2; i++) {
+        size_t p = i + j * o->sl + k * o->ss;
+        real_t slwjac = md[2 * V + p] / jac[p];
+        for (int comp = 0; comp < 3; comp++) {
+          const real_t *T0 = w + trow[comp][0] * V, *T1 = w + trow[comp][1] * V,
+                       *T2 = w + trow[comp][2] * V;
+          real_t vx[5], vy[5], vz[5];
+          for (int n = 0; n < ox->len; n++) {
+            size_t q = p + ox->sh[n];
+            vx[n] = jac[q] * (M[V + q] * T0[q] + M[2 * V + q] * T1[q] + M[3 * V + q] * T2[q]);
+          }
+          for (int n = 0; n < oy->len; n++) {
+            size_t q = p + oy->sh[n];
+            vy[n] = jac[q] * (M[4 * V + q] * T0[q] + M[5 * V + q] * T1[q] + M[6 * V + q] * T2[q]);
+          }
+          for (int n = 0; n < n_free; n++) {
+            size_t q = p + oz->sh[n];
+            vz[n] = jac[q] * (M[7 * V + q] * T0[q] + M[8 * V + q] * T1[q] + M[9 * V + q] * T2[q]);
+          }
+          vz[n_free] = 0.0;  /* surface traction */
+          for (int n = n_free + 1; n < zlen; n++) {
+            int m = n_free - (n - n_free);
+            real_t below;
+            if (m >= 0) {
+              below = vz[m];
+            } else {
+              /* mirror point lies below the operator window: evaluate it at
+               * its grid point k + zidx[n] - 2(n - n_free) */
+              size_t q = p + (long)(zidx[n] - 2 * (n - n_free)) * (long)o->ss;
+              below = jac[q] * (M[7 * V + q] * T0[q] + M[8 * V + q] * T1[q] + M[9 * V + q] * T2[q]);
+            }
+            vz[n] = 2.0 * 0.0f - below;
+          }
+          real_t DxT = ox->c[0] * vx[0], DyT = oy->c[0] * vy[0], DzT = oz->c[0] * vz[0];
+          for (int n = 1; n < ox->len; n++) DxT += ox->c[n] * vx[n];
+          for (int n = 1; n < oy->len; n++) DyT += oy->c[n] * vy[n];
+          for (int n = 1; n < zlen; n++) DzT += oz->c[n] * vz[n];
+          h[comp * V + p] = (DxT + DyT + DzT) * slwjac;
+        }
+      }
+  }
+}
+
+/* Hooke's law from 9 velocity derivatives (the expression of rhs_inner) */
+static inline void hooke(real_t *h, size_t V, size_t p, const real_t *m9,
+                         real_t lam, real_t lam2mu, real_t mu,
+                         const real_t *X, const real_t *Y, const real_t *Z)
+{
+  real_t xix = m9[0], xiy = m9[1], xiz = m9[2];
+  real_t etx = m9[3], ety = m9[4], etz = m9[5];
+  real_t ztx = m9[6], zty = m9[7], ztz = m9[8];
+  h[TXX * V + p] = lam2mu * (xix * X[0] + etx * Y[0] + ztx * Z[0])
+                 + lam * (xiy * X[1] + ety * Y[1] + zty * Z[1]
+                          + xiz * X[2] + etz * Y[2] + ztz * Z[2]);
+  h[TYY * V + p] = lam2mu * (xiy * X[1] + ety * Y[1] + zty * Z[1])
+                 + lam * (xix * X[0] + etx * Y[0] + ztx * Z[0]
+                          + xiz * X[2] + etz * Y[2] + ztz * Z[2]);
+  h[TZZ * V + p] = lam2mu * (xiz * X[2] + etz * Y[2] + ztz * Z[2])
+                 + lam * (xix * X[0] + etx * Y[0] + ztx * Z[0]
+                          + xiy * X[1] + ety * Y[1] + zty * Z[1]);
+  h[TXY * V + p] = mu * (xiy * X[0] + xix * X[1] + ety * Y[0] + etx * Y[1] + zty * Z[0] + ztx * Z[1]);
+  h[TXZ * V + p] = mu * (xiz * X[0] + xix * X[2] + etz * Y[0] + etx * Y[2] + ztz * Z[0] + ztx * Z[2]);
+  h[TYZ * V + p] = mu * (xiz * X[1] + xiy * X[2] + etz * Y[1] + ety * Y[2] + ztz * Z[1] + zty * Z[2]);
+}
+
+/* hT on the top 3 layers, sv_curv_col_el_iso.c:452-634 (no surface-force
+ * velocity term: VxSrc..VzSrc are zero without surface forces) */
+static void rhs_vlow(orc_t *o, const real_t *w, real_t *h,
+                     const op_t *ox, const op_t *oy, const op_t ozl[4])
+{
+  size_t V = o->vol;
+  const real_t *M = o->metric, *md = o->med;
+  const real_t *mVx = o->mats, *mVy = o->mats + o->ss * 9;
+  for (int n = 0; n < 3; n++) {
+    int k = o->nk2 - n;
+    for (int j = o->nj1; j <= o->nj2; j++)
+      for (int i = o->ni1; i <= o->ni2; i++) {
+        size_t p = i + j * o->sl + k * o->ss;
+        real_t m9[9];
+        for (int q = 0; q < 9; q++) m9[q] = M[(1 + q) * V + p];
+        real_t lam = md[p], mu = md[V + p];
+        real_t lam2mu = lam + 2.0 * mu;
+        real_t X[3], Y[3], Z[3];
+        for (int c = 0; c < 3; c++) {
+          X[c] = dop(ox, w + c * V, p);
+          Y[c] = dop(oy, w + c * V, p);
+        }
+        if (k == o->nk2) {
+          size_t ij = (i + j * o->sl) * 9;
+          for (int r = 0; r < 3; r++)
+            Z[r] = mVx[ij + 3 * r + 0] * X[0] + mVx[ij + 3 * r + 1] * X[1] + mVx[ij + 3 * r + 2] * X[2]
+                 + mVy[ij + 3 * r + 0] * Y[0] + mVy[ij + 3 * r + 1] * Y[1] + mVy[ij + 3 * r + 2] * Y[2];
+        } else {
+          for (int c = 0; c < 3; c++) Z[c] = dop(&ozl[n], w + c * V, p);
+        }
+        hooke(h, V, p, m9, lam, lam2mu, mu, X, Y, Z);
+      }
+  }
+}
+
+/* CFS-PML ADE, sv_curv_col_el_iso.c:645-1146 */
+static void rhs_pml(orc_t *o, const real_t *w, real_t *h, const real_t *aux,
+                    real_t *aux_h, const op_t *ox, const op_t *oy, const op_t *oz)
+{
+  size_t V = o->vol;
+  const real_t *M = o->metric, *md = o->med;
+  const real_t *mVx = o->mats, *mVy = o->mats + o->ss * 9;
+  for (int f = 0; f < 6; f++) {
+    if (!o->cfg.pml_on[f]) continue;
+    int idim = f / 2;
+    const int *r = o->region[f];
+    const op_t *op = idim == 0 ? ox : (idim == 1 ? oy : oz);
+    const real_t *mt = M + (1 + 3 * idim) * V;      /* xi_*, eta_* or zeta_* */
+    size_t na = o->aux_n[f];
+    const real_t *a = aux + o->aux_off[f];
+    real_t *ah = aux_h + o->aux_off[f];
+    size_t ia = 0;
+    for (int k = r[4]; k <= r[5]; k++)
+      for (int j = r[2]; j <= r[3]; j++)
+        for (int i = r[0]; i <= r[1]; i++, ia++) {
+          size_t p = i + j * o->sl + k * o->ss;
+          int ic = idim == 0 ? i - r[0] : (idim == 1 ? j - r[2] : k - r[4]);
+          real_t cD = o->pD[f][ic], cA = o->pA[f][ic], cB = o->pB[f][ic];
+          real_t cBm1 = cB - 1.0;
+          real_t e1 = mt[p], e2 = mt[V + p], e3 = mt[2 * V + p];
+          real_t lam = md[p], mu = md[V + p], slw = md[2 * V + p];
+          real_t lam2mu = lam + 2.0 * mu;
+          real_t D[9];
+          for (int c = 0; c < 9; c++) D[c] = dop(op, w + c * V, p);
+          real_t rr[9];
+          rr[VX] = slw * (e1 * D[TXX] + e2 * D[TXY] + e3 * D[TXZ]);
+          rr[VY] = slw * (e1 * D[TXY] + e2 * D[TYY] + e3 * D[TYZ]);
+          rr[VZ] = slw * (e1 * D[TXZ] + e2 * D[TYZ] + e3 * D[TZZ]);
+          rr[TXX] = lam2mu * e1 * D[VX] + lam * e2 * D[VY] + lam * e3 * D[VZ];
+          rr[TYY] = lam * e1 * D[VX] + lam2mu * e2 * D[VY] + lam * e3 * D[VZ];
+          rr[TZZ] = lam * e1 * D[VX] + lam * e2 * D[VY] + lam2mu * e3 * D[VZ];
+          rr[TXY] = mu * (e2 * D[VX] + e1 * D[VY]);
+          rr[TXZ] = mu * (e3 * D[VX] + e1 * D[VZ]);
+          rr[TYZ] = mu * (e3 * D[VY] + e2 * D[VZ]);
+          /* reference update order: Vx,Vy,Vz,Txx,Tyy,Tzz,Txz,Tyz,Txy */
+          static const int ord[9] = {VX, VY, VZ, TXX, TYY, TZZ, TXZ, TYZ, TXY};
+          for (int q = 0; q < 9; q++) {
+            int c = ord[q];
+            h[c * V + p] += cBm1 * rr[c] - cB * a[c * na + ia];
+          }
+          for (int q = 0; q < 9; q++) {
+            int c = ord[q];
+            ah[c * na + ia] = cD * rr[c] - cA * a[c * na + ia];
+          }
+          if (idim < 2 && o->cfg.free_top && k == o->nk2) {
+            size_t ij = (i + j * o->sl) * 9;
+            const real_t *mm = idim == 0 ? mVx : mVy;
+            real_t Dz[3];
+            for (int rr3 = 0; rr3 < 3; rr3++)
+              Dz[rr3] = mm[ij + 3 * rr3 + 0] * D[VX] + mm[ij + 3 * rr3 + 1] * D[VY]
+                      + mm[ij + 3 * rr3 + 2] * D[VZ];
+            real_t ztx = M[7 * V + p], zty = M[8 * V + p], ztz = M[9 * V + p];
+            real_t s[9];
+            s[TXX] = lam2mu * (ztx * Dz[0]) + lam * (zty * Dz[1] + ztz * Dz[2]);
+            s[TYY] = lam2mu * (zty * Dz[1]) + lam * (ztx * Dz[0] + ztz * Dz[2]);
+            s[TZZ] = lam2mu * (ztz * Dz[2]) + lam * (ztx * Dz[0] + zty * Dz[1]);
+            s[TXY] = mu * (zty * Dz[0] + ztx * Dz[1]);
+            s[TXZ] = mu * (ztz * Dz[0] + ztx * Dz[2]);
+            s[TYZ] = mu * (ztz * Dz[1] + zty * Dz[2]);
+            static const int ordT[6] = {TXX, TYY, TZZ, TXZ, TYZ, TXY};
+            for (int q = 0; q < 6; q++) {
+              int c = ordT[q];
+              h[c * V + p] += (cB - 1.0) * s[c];
+            }
+            for (int q = 0; q < 6; q++) {
+              int c = ordT[q];
+              ah[c * na + ia] += cD * s[c];
+            }
+          }
+        }
+  }
+}
+
+/* moment sources, sv_curv_col_el.c:312-479 (moment part) */
+static void rhs_src(orc_t *o, int it, int istage, real_t *h)
+{
+  const orc_cfg *c = &o->cfg;
+  size_t V = o->vol;
+  const real_t *jac = o->metric;
+  size_t nper = (size_t)c->nsrc * c->src_max_nt * 4;
+  int half = c->src_ext_half;
+  int len = 2 * half + 1;
+  float *ext = (float *)malloc(sizeof(float) * len * len * len);
+  for (int is = 0; is < c->nsrc; is++) {
+    if (!(it >= c->src_it_begin[is] && it <= c->src_it_end[is])) continue;
+    size_t ip = (size_t)is * c->src_max_nt * 4 + (size_t)(it - c->src_it_begin[is]) * 4 + istage;
+    real_t Mv[6];
+    for (int m = 0; m < 6; m++) Mv[m] = o->srcm[m * nper + ip];
+    int si = c->src_ijk[3 * is], sj = c->src_ijk[3 * is + 1], sk = c->src_ijk[3 * is + 2];
+    /* M order Mxx,Myy,Mzz,Mxz,Myz,Mxy onto slots Txx,Tyy,Tzz,Txz,Tyz,Txy */
+    static const int slot[6] = {TXX, TYY, TZZ, TXZ, TYZ, TXY};
+    if (!c->src_gauss) {
+      size_t p = si + sj * o->sl + sk * o->ss;
+      real_t rjac = 1.0 / jac[p];
+      for (int m = 0; m < 6; m++) h[slot[m] * V + p] -= Mv[m] * rjac;
+    } else {
+      int kb = (sk + half < o->nk2) ? half : o->nk2 - sk;
+      orc_gauss_weights(ext, c->src_inc[3 * is], c->src_inc[3 * is + 1], c->src_inc[3 * is + 2],
+                        c->src_ext_coef, half, kb);
+      int ie = 0;
+      for (int ke = -half; ke <= kb; ke++)
+        for (int je = -half; je <= half; je++)
+          for (int iee = -half; iee <= half; iee++, ie++) {
+            int i = si + iee, j = sj + je, k = sk + ke;
+            if (i < o->ni1 || i > o->ni2 || j < o->nj1 || j > o->nj2 || k < o->nk1 || k > o->nk2)
+              continue;
+            size_t p = i + j * o->sl + k * o->ss;
+            real_t coef = ext[ie];
+            real_t rjac = coef / jac[p];
+            for (int m = 0; m < 6; m++) h[slot[m] * V + p] -= Mv[m] * rjac;
+          }
+    }
+  }
+  free(ext);
+}
+
+void orc_onestage(orc_t *o, int it, int ipair, int istage,
+                  const real_t *w_cur, const real_t *aux_cur,
+                  real_t *rhs, real_t *aux_rhs)
+{
+  int dx = (pair_flags[ipair][0] + istage) % 2;
+  int dy = (pair_flags[ipair][1] + istage) % 2;
+  int dz = (pair_flags[ipair][2] + istage) % 2;
+  op_t ox, oy, oz, ozl[4];
+  make_op(&ox, 3, dx, 1);
+  make_op(&oy, 3, dy, (long)o->sl);
+  make_op(&oz, 3, dz, (long)o->ss);
+  for (int l = 0; l < 4; l++) make_op(&ozl[l], l, dz, (long)o->ss);
+  rhs_inner(o, w_cur, rhs, &ox, &oy, &oz);
+  if (o->cfg.free_top) {
+    rhs_timg(o, w_cur, rhs, &ox, &oy, &oz, op_indx[3][dz]);
+    rhs_vlow(o, w_cur, rhs, &ox, &oy, ozl);
+  }
+  if (o->aux_total) rhs_pml(o, w_cur, rhs, aux_cur, aux_rhs, &ox, &oy, &oz);
+  if (o->cfg.nsrc > 0) rhs_src(o, it, istage, rhs);
+}
+
+void orc_run(orc_t *o, float dt, int it0, int nt, real_t *w, real_t *aux,
+             int nrecv, const int *recv_idx, const float *recv_w,
+             int nt_total, real_t *seismo)
+{
+  size_t L = 9 * o->vol, LA = o->aux_total;
+  real_t *tmp = (real_t *)calloc(L, sizeof(real_t));
+  real_t *rhs = (real_t *)calloc(L, sizeof(real_t));
+  real_t *end = (real_t *)calloc(L, sizeof(real_t));
+  real_t *atmp = (real_t *)calloc(LA + 1, sizeof(real_t));
+  real_t *arhs = (real_t *)calloc(LA + 1, sizeof(real_t));
+  real_t *aend = (real_t *)calloc(LA + 1, sizeof(real_t));
+  real_t *pre = w, *apre = aux;
+  for (int n = 0; n < nt; n++) {
+    int it = it0 + n;
+    int ipair = it % 8;
+    for (int s = 0; s < 4; s++) {
+      const real_t *cur = s == 0 ? pre : tmp;
+      const real_t *acur = s == 0 ? apre : atmp;
+      orc_onestage(o, it, ipair, s, cur, acur, rhs, arhs);
+      real_t ca = s < 3 ? (real_t)((real_t)(float)rk_a_tab[s] * (real_t)dt) : 0;
+      real_t cb = (real_t)((real_t)(float)rk_b_tab[s] * (real_t)dt);
+#if ORC_REAL_IS_DOUBLE
+      ca = s < 3 ? rk_a_tab[s] * (double)dt : 0;
+      cb = rk_b_tab[s] * (double)dt;
+#endif
+      /* drv_rk_curv_col.c:291-446, whole level */
+#pragma omp parallel for schedule(static)
+      for (long p = 0; p < (long)L; p++) {
+        if (s < 3) tmp[p] = pre[p] + ca * rhs[p];
+        if (s == 0) end[p] = pre[p] + cb * rhs[p];
+        else end[p] += cb * rhs[p];
+      }
+      for (size_t p = 0; p < LA; p++) {
+        if (s < 3) atmp[p] = apre[p] + ca * arhs[p];
+        if (s == 0) aend[p] = apre[p] + cb * arhs[p];
+        else aend[p] += cb * arhs[p];
+      }
+    }
+    /* receivers, io_funcs.c:1585-1623 (indx1d[5] twice, indx1d[4] unused) */
+    for (int rcv = 0; rcv < nrecv; rcv++) {
+      const int *ix = recv_idx + 8 * rcv;
+      real_t Lx2 = recv_w[3 * rcv], Ly2 = recv_w[3 * rcv + 1], Lz2 = recv_w[3 * rcv + 2];
+      real_t Lx1 = 1.0 - Lx2, Ly1 = 1.0 - Ly2, Lz1 = 1.0 - Lz2;
+      for (int c = 0; c < 9; c++) {
+        const real_t *v = end + c * o->vol;
+        seismo[((size_t)rcv * 9 + c) * nt_total + it] =
+            v[ix[0]] * Lx1 * Ly1 * Lz1 + v[ix[1]] * Lx2 * Ly1 * Lz1
+          + v[ix[2]] * Lx1 * Ly2 * Lz1 + v[ix[3]] * Lx2 * Ly2 * Lz1
+          + v[ix[5]] * Lx1 * Ly1 * Lz2 + v[ix[5]] * Lx2 * Ly1 * Lz2
+          + v[ix[6]] * Lx1 * Ly2 * Lz2 + v[ix[7]] * Lx2 * Ly2 * Lz2;
+      }
+    }
+    memcpy(pre, end, L * sizeof(real_t));
+    if (LA) memcpy(apre, aend, LA * sizeof(real_t));
+  }
+  free(tmp); free(rhs); free(end); free(atmp); free(arhs); free(aend);
+}

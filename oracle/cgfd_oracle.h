@@ -1,0 +1,93 @@
+/*
+ * cgfd_oracle.h -- CPU restatement of CGFD3D's isotropic elastic
+ * time-stepping hot path.  TEST INFRASTRUCTURE ONLY: the product never links
+ * this; tests/, the smoke check and the CPU leg of the benchmark call it as
+ * the checker.
+ *
+ * Built twice by oracle/Makefile: liboracle.so with real_t = float (the
+ * reference's arithmetic, expression by expression, no FMA contraction) and
+ * liboracle64.so with real_t = double (the same scheme in FP64, used to state
+ * every FP32 result's error).
+ *
+ * Array layout is the reference's (gd_t.c:2890-2897, wav_t.c:93-136):
+ * x fastest, nx = ni+6, ny = nj+6, nz = nk+6, iptr = i + j*nx + k*nx*ny;
+ * wavefield components Vx,Vy,Vz,Txx,Tyy,Tzz,Tyz,Txz,Txy, one volume each;
+ * metric jac,xi_x,xi_y,xi_z,eta_x,eta_y,eta_z,zeta_x,zeta_y,zeta_z;
+ * isotropic medium lambda, mu, slowness (1/rho, md_t.c:602-623).
+ */
+#ifndef CGFD_ORACLE_H
+#define CGFD_ORACLE_H
+
+#include <stddef.h>
+
+#ifndef ORC_REAL
+#define ORC_REAL float
+#endif
+typedef ORC_REAL real_t;
+
+#define ORC_NGHOST 3
+
+/* face order matches the reference's idim*2+iside: x1,x2,y1,y2,z1,z2 */
+typedef struct {
+  int ni, nj, nk;
+  int free_top;            /* traction-image free surface on z2 */
+  int pml_on[6];
+  int pml_layers[6];
+  const float *pml_A[6];   /* per face, num_layers+1 values (bdry_t.c:246-269) */
+  const float *pml_B[6];
+  const float *pml_D[6];
+  /* moment sources (sv_curv_col_el.c:312-479) */
+  int nsrc;
+  int src_gauss;           /* 0: point, 1: Gaussian 7^3 smoothing */
+  int src_ext_half;        /* 3 for the Gaussian */
+  float src_ext_coef;      /* 1.5 (= ext_half/2, src_t.c:41-46) */
+  const int *src_ijk;      /* local indices with ghosts, 3 per source */
+  const float *src_inc;    /* si_inc,sj_inc,sk_inc, 3 per source */
+  const int *src_it_begin;
+  const int *src_it_end;
+  int src_max_nt;
+  const float *src_mij;    /* Mxx,Myy,Mzz,Mxz,Myz,Mxy, each [nsrc][max_nt][4] */
+} orc_cfg;
+
+typedef struct orc_s orc_t;
+
+/* ---- scheme tables (fd_t.c:18-304) ---- */
+/* slot 0=x, 1=y, 2..5 = z near-surface layers 0..3; 5 entries per slot */
+int orc_fd_ops(int *indx, float *coef, int *len,
+               float *rk_a, float *rk_b, float *rk_t);
+
+/* ---- setup pieces ---- */
+void orc_pml_profile(int num_layers, float L0, float dh, float vel,
+                     float amax, float bmax, int iside,
+                     float *A, float *B, float *D);
+float orc_pml_avg_dh(int ni, int nj, int nk, const float *x3d, const float *y3d,
+                     const float *z3d, const int region[6], int idim, float *L0);
+void orc_pml_region(int ni, int nj, int nk, const int pml_layers[6],
+                    const int pml_on[6], int region[6][6]);
+void orc_dvh2dvz_iso(int ni, int nj, int nk, const float *metric10,
+                     const float *med3, float *mats /* 3 x nx*ny*9 */);
+void orc_metric(int ni, int nj, int nk, const float *xyz /* 3 volumes */,
+                float *metric10);
+void orc_gauss_weights(float *w, float x0, float y0, float z0, float a,
+                       int half, int half_below);
+float orc_ricker(float t, float fc, float t0);
+void orc_angle2moment(float strike, float dip, float rake, float m[6]);
+
+/* ---- solver ---- */
+orc_t *orc_create(const orc_cfg *cfg, const float *metric10, const float *med3);
+void   orc_destroy(orc_t *o);
+size_t orc_aux_len(const orc_t *o);           /* floats of all PML aux, 1 level */
+size_t orc_vol(const orc_t *o);
+
+/* one RHS evaluation: rhs must be zero outside the physical region */
+void orc_onestage(orc_t *o, int it, int ipair, int istage,
+                  const real_t *w_cur, const real_t *aux_cur,
+                  real_t *rhs, real_t *aux_rhs);
+
+/* nt RK4 steps from level w (in/out) and aux (in/out, may be NULL if no PML);
+ * seismo[(r*9+icmp)*nt_total + it0+n] sampled after each step when nrecv>0. */
+void orc_run(orc_t *o, float dt, int it0, int nt, real_t *w, real_t *aux,
+             int nrecv, const int *recv_idx, const float *recv_w,
+             int nt_total, real_t *seismo);
+
+#endif

@@ -1,0 +1,163 @@
+"""ctypes access to the CPU oracle (liboracle.so / liboracle64.so) and, where
+it was built, to the reference harness (_ref/libcgfd3d_ref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the CPU
+baseline leg of bench.py, never by the product package.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_P = C.c_void_p
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+class OrcCfg(C.Structure):
+    _fields_ = [("ni", C.c_int), ("nj", C.c_int), ("nk", C.c_int),
+                ("free_top", C.c_int),
+                ("pml_on", C.c_int * 6), ("pml_layers", C.c_int * 6),
+                ("pml_A", _P * 6), ("pml_B", _P * 6), ("pml_D", _P * 6),
+                ("nsrc", C.c_int), ("src_gauss", C.c_int), ("src_ext_half", C.c_int),
+                ("src_ext_coef", C.c_float),
+                ("src_ijk", _P), ("src_inc", _P), ("src_it_begin", _P), ("src_it_end", _P),
+                ("src_max_nt", C.c_int), ("src_mij", _P)]
+
+
+class RefCfg(C.Structure):
+    _fields_ = [("ni", C.c_int), ("nj", C.c_int), ("nk", C.c_int), ("free_top", C.c_int),
+                ("pml_on", C.c_int * 6), ("pml_layers", C.c_int * 6),
+                ("pml_alpha", C.c_float * 6), ("pml_beta", C.c_float * 6),
+                ("pml_vel", C.c_float * 6),
+                ("nsrc", C.c_int), ("src_max_nt", C.c_int), ("src_ijk", _P),
+                ("src_it_begin", _P), ("src_it_end", _P), ("src_mij", _P)]
+
+
+def load_oracle(fp64=False):
+    path = os.path.join(HERE, "liboracle64.so" if fp64 else "liboracle.so")
+    lib = C.CDLL(path)
+    lib.orc_create.restype = _P
+    lib.orc_create.argtypes = [_P, _P, _P]
+    lib.orc_destroy.argtypes = [_P]
+    lib.orc_aux_len.restype = C.c_size_t
+    lib.orc_aux_len.argtypes = [_P]
+    lib.orc_vol.restype = C.c_size_t
+    lib.orc_vol.argtypes = [_P]
+    lib.orc_onestage.argtypes = [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]
+    lib.orc_run.argtypes = [_P, C.c_float, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P,
+                            C.c_int, _P]
+    lib.orc_fd_ops.argtypes = [_P] * 6
+    lib.orc_pml_profile.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
+                                    C.c_float, C.c_int, _P, _P, _P]
+    lib.orc_pml_avg_dh.restype = C.c_float
+    lib.orc_pml_avg_dh.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, C.c_int, _P]
+    lib.orc_pml_region.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P, _P]
+    lib.orc_dvh2dvz_iso.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P, _P]
+    lib.orc_metric.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P]
+    lib.orc_gauss_weights.argtypes = [_P, C.c_float, C.c_float, C.c_float, C.c_float,
+                                      C.c_int, C.c_int]
+    lib.orc_ricker.restype = C.c_float
+    lib.orc_ricker.argtypes = [C.c_float, C.c_float, C.c_float]
+    lib.orc_angle2moment.argtypes = [C.c_float, C.c_float, C.c_float, _P]
+    return lib
+
+
+def ref_available():
+    return os.path.exists(os.path.join(HERE, "_ref", "libcgfd3d_ref.so"))
+
+
+def load_ref():
+    lib = C.CDLL(os.path.join(HERE, "_ref", "libcgfd3d_ref.so"), mode=os.RTLD_LAZY)
+    lib.refh_fd_ops.restype = C.c_int
+    lib.refh_fd_ops.argtypes = [_P] * 6
+    lib.refh_pml.argtypes = [_P, _P, C.c_int, _P, _P, _P, _P]
+    lib.refh_dvh2dvz_iso.argtypes = [_P, _P, _P, _P]
+    lib.refh_iso_onestage.restype = C.c_long
+    lib.refh_iso_onestage.argtypes = [_P, _P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]
+    lib.refh_iso_run.argtypes = [_P, _P, _P, _P, C.c_float, C.c_int, _P, C.c_int, _P, _P,
+                                 _P, _P]
+    return lib
+
+
+class Oracle:
+    """One oracle solver instance over a fixed grid, medium and boundary set.
+
+    case: a dict built by tests (see tests/cases.py) holding ni,nj,nk, metric,
+    med, free_top, pml_on, pml_layers, pml A/B/D tables and the source tables.
+    """
+
+    def __init__(self, case, fp64=False):
+        self.lib = load_oracle(fp64)
+        self.dtype = np.float64 if fp64 else np.float32
+        self.case = case
+        cfg = OrcCfg()
+        cfg.ni, cfg.nj, cfg.nk = case["ni"], case["nj"], case["nk"]
+        cfg.free_top = int(case.get("free_top", 0))
+        self._keep = []
+        for f in range(6):
+            on = int(case.get("pml_on", [0] * 6)[f])
+            cfg.pml_on[f] = on
+            cfg.pml_layers[f] = int(case.get("pml_layers", [0] * 6)[f]) if on else 0
+            if on:
+                for name, fld in (("pml_A", cfg.pml_A), ("pml_B", cfg.pml_B),
+                                  ("pml_D", cfg.pml_D)):
+                    a = np.ascontiguousarray(case[name][f], np.float32)
+                    self._keep.append(a)
+                    fld[f] = a.ctypes.data
+        src = case.get("src")
+        if src is not None and src["nsrc"] > 0:
+            cfg.nsrc = src["nsrc"]
+            cfg.src_gauss = int(src.get("gauss", 0))
+            cfg.src_ext_half = int(src.get("ext_half", 3))
+            cfg.src_ext_coef = float(src.get("ext_coef", 1.5))
+            arrs = dict(ijk=np.ascontiguousarray(src["ijk"], np.int32),
+                        inc=np.ascontiguousarray(src.get("inc", np.zeros(3 * src["nsrc"])),
+                                                 np.float32),
+                        itb=np.ascontiguousarray(src["it_begin"], np.int32),
+                        ite=np.ascontiguousarray(src["it_end"], np.int32),
+                        mij=np.ascontiguousarray(src["mij"], np.float32))
+            self._keep += list(arrs.values())
+            cfg.src_ijk = arrs["ijk"].ctypes.data
+            cfg.src_inc = arrs["inc"].ctypes.data
+            cfg.src_it_begin = arrs["itb"].ctypes.data
+            cfg.src_it_end = arrs["ite"].ctypes.data
+            cfg.src_max_nt = int(src["max_nt"])
+            cfg.src_mij = arrs["mij"].ctypes.data
+        self.cfg = cfg
+        self.metric = np.ascontiguousarray(case["metric"], np.float32)
+        self.med = np.ascontiguousarray(case["med"], np.float32)
+        self.h = self.lib.orc_create(C.byref(cfg), ptr(self.metric), ptr(self.med))
+        self.vol = self.lib.orc_vol(self.h)
+        self.aux_len = self.lib.orc_aux_len(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    def onestage(self, it, ipair, istage, w_cur, aux_cur=None):
+        w = np.ascontiguousarray(w_cur, self.dtype)
+        rhs = np.zeros(9 * self.vol, self.dtype)
+        aux = np.ascontiguousarray(aux_cur if aux_cur is not None
+                                   else np.zeros(self.aux_len), self.dtype)
+        arhs = np.zeros(max(self.aux_len, 1), self.dtype)
+        self.lib.orc_onestage(self.h, it, ipair, istage, ptr(w), ptr(aux), ptr(rhs), ptr(arhs))
+        return rhs, arhs[: self.aux_len]
+
+    def run(self, dt, nt, w=None, aux=None, it0=0, recv_idx=None, recv_w=None, nt_total=None):
+        w = (np.zeros(9 * self.vol, self.dtype) if w is None
+             else np.array(w, self.dtype, copy=True))
+        aux = (np.zeros(max(self.aux_len, 1), self.dtype) if aux is None
+               else np.array(aux, self.dtype, copy=True))
+        nrecv = 0 if recv_idx is None else len(recv_idx)
+        nt_total = nt_total or (it0 + nt)
+        seismo = np.zeros((max(nrecv, 1), 9, nt_total), self.dtype)
+        ri = None if recv_idx is None else np.ascontiguousarray(recv_idx, np.int32)
+        rw = None if recv_w is None else np.ascontiguousarray(recv_w, np.float32)
+        self.lib.orc_run(self.h, dt, it0, nt, ptr(w), ptr(aux), nrecv, ptr(ri), ptr(rw),
+                         nt_total, ptr(seismo))
+        return w, aux[: self.aux_len], seismo[:nrecv]

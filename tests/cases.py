@@ -1,0 +1,199 @@
+"""Seeded synthetic inputs shared by the parity tests: small curvilinear grids
+with topography, heterogeneous isotropic media, PML tables and sources.
+
+Layout follows the reference (x fastest, 3 ghost points per side).  The metric
+and the PML profile tables are produced by the oracle's restatement of the
+reference setup code; the product's own setup code is checked against the
+same restatement in test_setup.py.
+"""
+import numpy as np
+
+from oracle import pyoracle
+
+
+def dims(ni, nj, nk):
+    return ni + 6, nj + 6, nk + 6
+
+
+def curvilinear_coords(ni, nj, nk, dh=100.0, seed=0, hill=0.3, wiggle=0.08):
+    """Body-fitted grid: flat bottom, Gaussian-hill top, mildly skewed columns.
+    Ghost coordinates extend the same analytic map, as a generated grid does."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = dims(ni, nj, nk)
+    i = np.arange(nx) - 3.0
+    j = np.arange(ny) - 3.0
+    k = np.arange(nz) - 3.0
+    K, J, I = np.meshgrid(k, j, i, indexing="ij")
+    ph = rng.uniform(0, 2 * np.pi, 4)
+    x = I * dh + wiggle * dh * np.sin(0.7 * J + ph[0]) * np.sin(0.5 * K + ph[1])
+    y = J * dh + wiggle * dh * np.sin(0.6 * I + ph[2]) * np.cos(0.4 * K + ph[3])
+    depth = (nk - 1) * dh
+    xc, yc = 0.5 * (ni - 1) * dh, 0.5 * (nj - 1) * dh
+    sig = 0.25 * min(ni, nj) * dh
+    top = hill * depth * np.exp(-((I * dh - xc) ** 2 + (J * dh - yc) ** 2) / (2 * sig ** 2))
+    frac = K / (nk - 1)
+    z = -depth + frac * (depth + top)
+    return np.stack([x, y, z]).astype(np.float32)
+
+
+def iso_medium(ni, nj, nk, seed=0, vp=6000.0, vs=3464.0, rho=2670.0, pert=0.05):
+    rng = np.random.default_rng(seed + 1)
+    nx, ny, nz = dims(ni, nj, nk)
+    f = 1.0 + pert * rng.standard_normal((3, nz, ny, nx))
+    vp_, vs_, rho_ = vp * f[0], vs * f[1], rho * f[2]
+    mu = (rho_ * vs_ ** 2).astype(np.float32)
+    lam = (rho_ * vp_ ** 2 - 2 * rho_ * vs_ ** 2).astype(np.float32)
+    slw = (1.0 / rho_).astype(np.float32)
+    return np.stack([lam, mu, slw]).astype(np.float32)
+
+
+def pml_tables(orc, ni, nj, nk, coords, pml_on, pml_layers, vel=6000.0,
+               alpha=None, beta=3.0):
+    """A/B/D per face with the reference's profile (bdry_t.c:215-272)."""
+    region = np.zeros((6, 6), np.int32)
+    on = np.array(pml_on, np.int32)
+    lay = np.array(pml_layers, np.int32)
+    orc.orc_pml_region(ni, nj, nk, pyoracle.ptr(lay), pyoracle.ptr(on), pyoracle.ptr(region))
+    nx, ny, nz = dims(ni, nj, nk)
+    vol = nx * ny * nz
+    c = np.ascontiguousarray(coords.reshape(3, vol))
+    A, B, D = [], [], []
+    for f in range(6):
+        if not pml_on[f]:
+            A.append(None); B.append(None); D.append(None)
+            continue
+        L0 = np.zeros(1, np.float32)
+        r = np.ascontiguousarray(region[f])
+        import ctypes
+        dh = orc.orc_pml_avg_dh(ni, nj, nk, pyoracle.ptr(c[0]), pyoracle.ptr(c[1]),
+                                pyoracle.ptr(c[2]), pyoracle.ptr(r), f // 2,
+                                pyoracle.ptr(L0))
+        n = pml_layers[f] + 1
+        a = np.zeros(n, np.float32); b = np.zeros(n, np.float32); d = np.zeros(n, np.float32)
+        am = alpha if alpha is not None else np.pi * 1.0  # ~ pi * fc
+        orc.orc_pml_profile(pml_layers[f], float(L0[0]), float(dh), vel, am, beta, f % 2,
+                            pyoracle.ptr(a), pyoracle.ptr(b), pyoracle.ptr(d))
+        A.append(a); B.append(b); D.append(d)
+    return A, B, D, region
+
+
+def make_case(ni=16, nj=14, nk=12, seed=0, free_top=True, pml=(1, 1, 1, 1, 1, 0),
+              layers=4, src=None, dh=100.0, hill=0.3):
+    orc = pyoracle.load_oracle()
+    nx, ny, nz = dims(ni, nj, nk)
+    coords = curvilinear_coords(ni, nj, nk, dh=dh, seed=seed, hill=hill)
+    metric = np.zeros((10, nz, ny, nx), np.float32)
+    orc.orc_metric(ni, nj, nk, pyoracle.ptr(np.ascontiguousarray(coords)), pyoracle.ptr(metric))
+    med = iso_medium(ni, nj, nk, seed=seed)
+    pml_on = list(pml)
+    pml_layers = [layers if p else 0 for p in pml_on]
+    A, B, D, region = pml_tables(orc, ni, nj, nk, coords, pml_on, pml_layers)
+    case = dict(ni=ni, nj=nj, nk=nk, coords=coords, metric=metric, med=med,
+                free_top=int(free_top), pml_on=pml_on, pml_layers=pml_layers,
+                pml_A=A, pml_B=B, pml_D=D, pml_region=region, src=src,
+                pml_vel=6000.0, pml_alpha=np.pi * 1.0, pml_beta=3.0)
+    return case
+
+
+def random_wavefield(case, seed=0, scale=1.0):
+    """Random values on physical points, zero ghosts (the reference keeps its
+    ghost layers at zero: wav_t.c:276-378 and the calloc'ed levels)."""
+    rng = np.random.default_rng(seed + 7)
+    ni, nj, nk = case["ni"], case["nj"], case["nk"]
+    nx, ny, nz = dims(ni, nj, nk)
+    w = np.zeros((9, nz, ny, nx), np.float32)
+    w[:, 3:3 + nk, 3:3 + nj, 3:3 + ni] = scale * rng.standard_normal((9, nk, nj, ni))
+    return w
+
+
+def aux_shapes(case):
+    """(nz_f, ny_f, nx_f) of each enabled PML face's aux block, face order."""
+    out = []
+    for f in range(6):
+        if not case["pml_on"][f]:
+            out.append(None)
+            continue
+        r = case["pml_region"][f]
+        out.append((r[5] - r[4] + 1, r[3] - r[2] + 1, r[1] - r[0] + 1))
+    return out
+
+
+def random_aux(case, seed=0, scale=1.0):
+    rng = np.random.default_rng(seed + 11)
+    n = sum(9 * int(np.prod(s)) for s in aux_shapes(case) if s is not None)
+    return (scale * rng.standard_normal(n)).astype(np.float32)
+
+
+def ricker_point_source(case, i, j, k, nt, dt, fc=2.0, t0=0.5, m0=1e16,
+                        mech=(1, 1, 1, 0, 0, 0), gauss=False, inc=(0.0, 0.0, 0.0)):
+    """Moment source with a Ricker STF table laid out as src_t.c:1047-1091:
+    M??[is][it][stage] at t = it*dt + rk_t[stage]*dt (wavelet_tstart = t0 of run = 0)."""
+    orc = pyoracle.load_oracle()
+    rk_t = [0.0, 0.5, 0.5, 1.0]
+    stf = np.zeros((nt, 4), np.float32)
+    for it in range(nt):
+        for s in range(4):
+            t = np.float32(np.float32(it * np.float32(dt)) + np.float32(np.float32(rk_t[s]) * np.float32(dt)))
+            stf[it, s] = orc.orc_ricker(float(t), fc, t0)
+    mij = np.zeros((6, 1, nt, 4), np.float32)
+    for m in range(6):
+        mij[m, 0] = (stf * np.float32(mech[m] * m0)).astype(np.float32)
+    return dict(nsrc=1, ijk=[i, j, k], inc=list(inc), it_begin=[0], it_end=[nt - 1],
+                max_nt=nt, mij=mij.reshape(-1), gauss=int(gauss), ext_half=3, ext_coef=1.5)
+
+
+def surface_receivers(case, n=4, seed=0):
+    """Receivers on the free surface at grid nodes (dk forced to 0 there,
+    io_funcs.c:366-368) with fractional di/dj."""
+    rng = np.random.default_rng(seed + 5)
+    ni, nj, nk = case["ni"], case["nj"], case["nk"]
+    nx, ny, _ = dims(ni, nj, nk)
+    idx, w = [], []
+    for _ in range(n):
+        i = int(rng.integers(3, 3 + ni - 1)); j = int(rng.integers(3, 3 + nj - 1))
+        k = 3 + nk - 1
+        di, dj, dk = float(rng.uniform(0, 1)), float(rng.uniform(0, 1)), 0.0
+        idx.append(corner_indices(i, j, k, nx, ny))
+        w.append((di, dj, dk))
+    return np.array(idx, np.int32), np.array(w, np.float32)
+
+
+def buried_receiver(case, i, j, k, di, dj, dk):
+    nx, ny, _ = dims(case["ni"], case["nj"], case["nk"])
+    return (np.array([corner_indices(i, j, k, nx, ny)], np.int32),
+            np.array([(di, dj, dk)], np.float32))
+
+
+def corner_indices(i, j, k, nx, ny):
+    """indx1d[0..7] of io_funcs.c:382-389."""
+    s = nx * ny
+    return [i + j * nx + k * s, i + 1 + j * nx + k * s,
+            i + (j + 1) * nx + k * s, i + 1 + (j + 1) * nx + k * s,
+            i + j * nx + (k + 1) * s, i + 1 + j * nx + (k + 1) * s,
+            i + (j + 1) * nx + (k + 1) * s, i + 1 + (j + 1) * nx + (k + 1) * s]
+
+
+def ref_cfg(case, src_arrays=None):
+    """The reference harness's config for the same case."""
+    c = pyoracle.RefCfg()
+    c.ni, c.nj, c.nk = case["ni"], case["nj"], case["nk"]
+    c.free_top = int(case["free_top"])
+    for f in range(6):
+        c.pml_on[f] = int(case["pml_on"][f])
+        c.pml_layers[f] = int(case["pml_layers"][f])
+        c.pml_alpha[f] = case["pml_alpha"]
+        c.pml_beta[f] = case["pml_beta"]
+        c.pml_vel[f] = case["pml_vel"]
+    keep = []
+    src = case.get("src")
+    if src is not None and src["nsrc"] > 0:
+        assert not src.get("gauss"), "the reference harness runs point sources only"
+        a = [np.ascontiguousarray(src["ijk"], np.int32),
+             np.ascontiguousarray(src["it_begin"], np.int32),
+             np.ascontiguousarray(src["it_end"], np.int32),
+             np.ascontiguousarray(src["mij"], np.float32)]
+        keep += a
+        c.nsrc = src["nsrc"]
+        c.src_max_nt = src["max_nt"]
+        c.src_ijk, c.src_it_begin, c.src_it_end, c.src_mij = [x.ctypes.data for x in a]
+    return c, keep
