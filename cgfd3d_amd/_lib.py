@@ -1,0 +1,73 @@
+"""ctypes binding of the C ABI in include/cgfdh.h (libcgfdh.so, built in-tree).
+
+There is no fallback: if the library is missing or fails to load, every entry
+point raises, so nothing silently runs off the GPU path.
+"""
+import ctypes as C
+import os
+
+_P = C.c_void_p
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libcgfdh.so")
+
+_lib = None
+
+
+class CgfdhDesc(C.Structure):
+    _fields_ = [("ni", C.c_int), ("nj", C.c_int), ("nk", C.c_int),
+                ("medium_type", C.c_int), ("free_top", C.c_int),
+                ("pml_on", C.c_int * 6), ("pml_layers", C.c_int * 6),
+                ("dt", C.c_float), ("nt_total", C.c_int)]
+
+
+# (name, restype, argtypes) of every symbol include/cgfdh.h declares
+SIGNATURES = [
+    ("cgfdh_create", C.c_int, [_P, _P]),
+    ("cgfdh_destroy", C.c_int, [_P]),
+    ("cgfdh_last_error", C.c_char_p, []),
+    ("cgfdh_set_metric", C.c_int, [_P, _P]),
+    ("cgfdh_set_medium_iso", C.c_int, [_P, _P, _P, _P]),
+    ("cgfdh_set_pml_face", C.c_int, [_P, C.c_int, _P, _P, _P]),
+    ("cgfdh_set_free_surface", C.c_int, [_P, _P, _P, _P]),
+    ("cgfdh_set_moment_sources", C.c_int,
+     [_P, C.c_int, _P, _P, _P, C.c_int, _P, C.c_int, C.c_int, _P]),
+    ("cgfdh_set_receivers", C.c_int, [_P, C.c_int, _P, _P]),
+    ("cgfdh_set_wavefield", C.c_int, [_P, _P]),
+    ("cgfdh_get_wavefield", C.c_int, [_P, _P]),
+    ("cgfdh_pml_aux_len", C.c_size_t, [_P]),
+    ("cgfdh_set_pml_aux", C.c_int, [_P, _P]),
+    ("cgfdh_get_pml_aux", C.c_int, [_P, _P]),
+    ("cgfdh_run", C.c_int, [_P, C.c_int, C.c_int]),
+    ("cgfdh_get_seismograms", C.c_int, [_P, _P]),
+    ("cgfdh_onestage", C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
+    ("cgfdh_last_timing", C.c_int, [_P, _P, _P, _P]),
+    ("cgfdh_set_kernel_variant", C.c_int, [_P, C.c_int]),
+]
+
+
+class CgfdhError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libcgfdh.so once; raise if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CgfdhError(f"{LIB_PATH} is missing: build it with "
+                         "`python -m cgfd3d_amd.build` (hipcc, gfx950)")
+    L = C.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib().cgfdh_last_error().decode(errors="replace")
+        raise CgfdhError(f"{what}: rc={rc}: {msg}")
+    return rc

@@ -1,0 +1,616 @@
+// cgfdh.hip -- gfx950 stage kernels and the C ABI declared in include/cgfdh.h.
+//
+// One RK4 stage is one fused launch: RHS (interior stencil, traction image,
+// vlow, CFS-PML with its auxiliary RK combine, moment sources) and the RK
+// combine of the wavefield, so the stage never writes an rhs level to HBM.
+// Levels: pre, tmpA, tmpB, end; the stage reading tmpA writes tmpB and the
+// other way round, because the stencil reads neighbours of the level it
+// replaces.  PML aux needs only pre/tmp/end: it is read at its own point.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "cgfdh.h"
+#include "stage_common.h"
+
+using namespace cgfdh;
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+
+// per-point kernel: one thread per grid point, every stencil operand read
+// from global memory.  The plain reference form of the stage.
+template <int DX, int DY, int DZ, int KIND>
+__global__ void __launch_bounds__(256) stage_point_kernel(StageParams P)
+{
+  const int i = P.ni1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = P.nj1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = P.nk1 + blockIdx.z;
+  if (i > P.ni2 || j > P.nj2) return;
+  const long p = (long)i + (long)j * P.nx + (long)k * P.ss;
+  const long V = P.vol;
+  float X[9], Y[9], Z[9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    const float* v = P.wcur + c * V;
+    X[c] = d5<DX>(v, p, 1);
+    Y[c] = d5<DY>(v, p, P.nx);
+    Z[c] = d5<DZ>(v, p, P.ss);
+  }
+  finish_point<DX, DY, DZ, KIND>(P, i, j, k, p, X, Y, Z);
+}
+
+// receivers: trilinear sample of w_end after a step (io_funcs.c:1585-1623),
+// indx1d[5] used twice and indx1d[4] not, as the reference does
+__global__ void recv_kernel(const float* __restrict__ w, long vol, int nrecv,
+                            const int* __restrict__ idx, const float* __restrict__ dijk,
+                            float* __restrict__ seismo, int nt_total, int it)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrecv * 9) return;
+  const int r = t / 9, c = t % 9;
+  const int* ix = idx + 8 * r;
+  const float Lx2 = dijk[3 * r], Ly2 = dijk[3 * r + 1], Lz2 = dijk[3 * r + 2];
+  const float Lx1 = (float)(1.0 - (double)Lx2);
+  const float Ly1 = (float)(1.0 - (double)Ly2);
+  const float Lz1 = (float)(1.0 - (double)Lz2);
+  const float* v = w + (long)c * vol;
+  const float s = v[ix[0]] * Lx1 * Ly1 * Lz1 + v[ix[1]] * Lx2 * Ly1 * Lz1 +
+                  v[ix[2]] * Lx1 * Ly2 * Lz1 + v[ix[3]] * Lx2 * Ly2 * Lz1 +
+                  v[ix[5]] * Lx1 * Ly1 * Lz2 + v[ix[5]] * Lx2 * Ly1 * Lz2 +
+                  v[ix[6]] * Lx1 * Ly2 * Lz2 + v[ix[7]] * Lx2 * Ly2 * Lz2;
+  seismo[((long)r * 9 + c) * nt_total + it] = s;
+}
+
+#include "stage_tiled.h"
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+static thread_local char g_err[512];
+
+static int fail(int code, const char* fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess)                                                          \
+      return fail(CGFDH_EDEVICE, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                             \
+  } while (0)
+
+struct FaceHost {
+  int on = 0;
+  int region[6] = {0};
+  int fnx = 0, fny = 0, fnz = 0;
+  long fn = 0;
+  float* A = nullptr;
+  float* B = nullptr;
+  float* D = nullptr;
+  float* aux[3] = {nullptr, nullptr, nullptr};   // pre, tmp, end
+  bool coefs_set = false;
+};
+
+struct cgfdh_solver {
+  cgfdh_desc d;
+  int nx, ny, nz;
+  long ss, vol;
+  int ni1, ni2, nj1, nj2, nk1, nk2;
+  hipStream_t stream = nullptr;
+  float* met = nullptr;
+  float* med = nullptr;          // lambda, mu, slw
+  float* lvl[4] = {nullptr, nullptr, nullptr, nullptr};   // pre, tmpA, tmpB, end
+  float* mats = nullptr;         // matVx2Vz2, matVy2Vz2, matF2Vz2
+  FaceHost face[6];
+  int nsrc = 0;
+  SrcDev* src = nullptr;
+  float* mij = nullptr;
+  long mij_stride = 0;
+  int src_max_nt = 0, src_spatial = 1, src_half = 0;
+  float* src_coef = nullptr;
+  int nrecv = 0;
+  int* recv_idx = nullptr;
+  float* recv_w = nullptr;
+  float* seismo = nullptr;
+  int variant = 1;
+  bool metric_set = false, medium_set = false, mats_set = false;
+  hipEvent_t ev_run[2];
+  std::vector<hipEvent_t> ev_stage;
+  double last_run_ms = 0, last_stage_ms = 0;
+  int last_launches = 0;
+};
+
+// operator-pair flags, fd_t.c:185-195; stage s uses dir (flag + s) % 2
+static const int kPairFlags[8][3] = {{0, 0, 0}, {1, 1, 0}, {1, 1, 1}, {0, 0, 1},
+                                     {0, 1, 0}, {1, 0, 0}, {1, 0, 1}, {0, 1, 1}};
+static const double kRkA[3] = {0.5, 0.5, 1.0};
+static const double kRkB[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+
+extern "C" const char* cgfdh_last_error(void) { return g_err; }
+
+static void face_region(const cgfdh_solver* s, int f, int L, int r[6])
+{
+  r[0] = s->ni1; r[1] = s->ni2; r[2] = s->nj1; r[3] = s->nj2; r[4] = s->nk1; r[5] = s->nk2;
+  if (f == 0) r[1] = s->ni1 + L;
+  if (f == 1) r[0] = s->ni2 - L;
+  if (f == 2) r[3] = s->nj1 + L;
+  if (f == 3) r[2] = s->nj2 - L;
+  if (f == 4) r[5] = s->nk1 + L;
+  if (f == 5) r[4] = s->nk2 - L;
+}
+
+static void free_all(cgfdh_solver* s)
+{
+  auto F = [](void* p) { if (p) (void)hipFree(p); };
+  F(s->met); F(s->med); F(s->mats);
+  for (auto& l : s->lvl) F(l);
+  for (auto& f : s->face) {
+    F(f.A); F(f.B); F(f.D);
+    for (auto& a : f.aux) F(a);
+  }
+  F(s->src); F(s->mij); F(s->src_coef);
+  F(s->recv_idx); F(s->recv_w); F(s->seismo);
+}
+
+extern "C" int cgfdh_create(const cgfdh_desc* d, cgfdh_solver** out)
+{
+  if (!d || !out) return fail(CGFDH_EINVAL, "cgfdh_create: null argument");
+  if (d->ni < 8 || d->nj < 8 || d->nk < 8)
+    return fail(CGFDH_EINVAL, "cgfdh_create: grid %dx%dx%d below the 8-point minimum", d->ni,
+                d->nj, d->nk);
+  if (d->medium_type != CGFDH_MEDIUM_ELASTIC_ISO)
+    return fail(CGFDH_EINVAL, "cgfdh_create: medium_type %d not supported", d->medium_type);
+  if (d->nt_total < 1) return fail(CGFDH_EINVAL, "cgfdh_create: nt_total %d", d->nt_total);
+  for (int f = 0; f < 6; f++) {
+    int n = f < 2 ? d->ni : (f < 4 ? d->nj : d->nk);
+    if (d->pml_on[f] && (d->pml_layers[f] < 1 || d->pml_layers[f] >= n))
+      return fail(CGFDH_EINVAL, "cgfdh_create: face %d has %d PML layers for %d points", f,
+                  d->pml_layers[f], n);
+  }
+  if (d->free_top && d->pml_on[5])
+    return fail(CGFDH_EINVAL, "cgfdh_create: z2 cannot be both free surface and PML");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(CGFDH_EDEVICE, "cgfdh_create: no HIP device visible");
+  cgfdh_solver* s = new cgfdh_solver();
+  s->d = *d;
+  s->nx = d->ni + 6; s->ny = d->nj + 6; s->nz = d->nk + 6;
+  s->ss = (long)s->nx * s->ny;
+  s->vol = s->ss * s->nz;
+  s->ni1 = 3; s->ni2 = 3 + d->ni - 1;
+  s->nj1 = 3; s->nj2 = 3 + d->nj - 1;
+  s->nk1 = 3; s->nk2 = 3 + d->nk - 1;
+  auto alloc = [&](float** p, size_t n) -> int {
+    if (hipMalloc((void**)p, n * sizeof(float)) != hipSuccess) return -1;
+    if (hipMemset(*p, 0, n * sizeof(float)) != hipSuccess) return -1;
+    return 0;
+  };
+  int bad = 0;
+  bad |= alloc(&s->met, 10 * s->vol);
+  bad |= alloc(&s->med, 3 * s->vol);
+  for (auto& l : s->lvl) bad |= alloc(&l, 9 * s->vol);
+  bad |= alloc(&s->mats, 3 * 9 * s->ss);
+  for (int f = 0; f < 6 && !bad; f++) {
+    FaceHost& F = s->face[f];
+    F.on = d->pml_on[f];
+    if (!F.on) continue;
+    face_region(s, f, d->pml_layers[f], F.region);
+    F.fnx = F.region[1] - F.region[0] + 1;
+    F.fny = F.region[3] - F.region[2] + 1;
+    F.fnz = F.region[5] - F.region[4] + 1;
+    F.fn = (long)F.fnx * F.fny * F.fnz;
+    int np = d->pml_layers[f] + 1;
+    bad |= alloc(&F.A, np) | alloc(&F.B, np) | alloc(&F.D, np);
+    for (auto& a : F.aux) bad |= alloc(&a, 9 * F.fn);
+  }
+  if (!bad && hipStreamCreate(&s->stream) != hipSuccess) bad = 1;
+  if (!bad && (hipEventCreate(&s->ev_run[0]) != hipSuccess ||
+               hipEventCreate(&s->ev_run[1]) != hipSuccess))
+    bad = 1;
+  if (bad) {
+    free_all(s);
+    delete s;
+    return fail(CGFDH_ENOMEM, "cgfdh_create: device allocation failed for %dx%dx%d", d->ni,
+                d->nj, d->nk);
+  }
+  *out = s;
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_destroy(cgfdh_solver* s)
+{
+  if (!s) return CGFDH_OK;
+  (void)hipStreamSynchronize(s->stream);
+  free_all(s);
+  for (auto e : s->ev_stage) (void)hipEventDestroy(e);
+  (void)hipEventDestroy(s->ev_run[0]);
+  (void)hipEventDestroy(s->ev_run[1]);
+  (void)hipStreamDestroy(s->stream);
+  delete s;
+  return CGFDH_OK;
+}
+
+static int h2d(cgfdh_solver* s, float* dst, const float* src, size_t n)
+{
+  HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_set_metric(cgfdh_solver* s, const float* m)
+{
+  if (!s || !m) return fail(CGFDH_EINVAL, "cgfdh_set_metric: null argument");
+  int rc = h2d(s, s->met, m, 10 * s->vol);
+  if (rc == 0) s->metric_set = true;
+  return rc;
+}
+
+extern "C" int cgfdh_set_medium_iso(cgfdh_solver* s, const float* lam, const float* mu,
+                                    const float* slw)
+{
+  if (!s || !lam || !mu || !slw) return fail(CGFDH_EINVAL, "cgfdh_set_medium_iso: null argument");
+  int rc = h2d(s, s->med, lam, s->vol);
+  if (!rc) rc = h2d(s, s->med + s->vol, mu, s->vol);
+  if (!rc) rc = h2d(s, s->med + 2 * s->vol, slw, s->vol);
+  if (!rc) s->medium_set = true;
+  return rc;
+}
+
+extern "C" int cgfdh_set_pml_face(cgfdh_solver* s, int f, const float* A, const float* B,
+                                  const float* D)
+{
+  if (!s || f < 0 || f > 5 || !A || !B || !D)
+    return fail(CGFDH_EINVAL, "cgfdh_set_pml_face: bad argument (face %d)", f);
+  if (!s->face[f].on) return fail(CGFDH_EINVAL, "cgfdh_set_pml_face: face %d has no PML", f);
+  int np = s->d.pml_layers[f] + 1;
+  int rc = h2d(s, s->face[f].A, A, np);
+  if (!rc) rc = h2d(s, s->face[f].B, B, np);
+  if (!rc) rc = h2d(s, s->face[f].D, D, np);
+  if (!rc) s->face[f].coefs_set = true;
+  return rc;
+}
+
+extern "C" int cgfdh_set_free_surface(cgfdh_solver* s, const float* mx, const float* my,
+                                      const float* mf)
+{
+  if (!s || !mx || !my || !mf) return fail(CGFDH_EINVAL, "cgfdh_set_free_surface: null argument");
+  size_t n = 9 * (size_t)s->ss;
+  int rc = h2d(s, s->mats, mx, n);
+  if (!rc) rc = h2d(s, s->mats + n, my, n);
+  if (!rc) rc = h2d(s, s->mats + 2 * n, mf, n);
+  if (!rc) s->mats_set = true;
+  return rc;
+}
+
+extern "C" int cgfdh_set_moment_sources(cgfdh_solver* s, int nsrc, const int* ijk,
+                                        const int* it_begin, const int* it_end, int max_nt,
+                                        const float* mij, int spatial, int ext_half,
+                                        const float* ext_coefs)
+{
+  if (!s || nsrc < 0) return fail(CGFDH_EINVAL, "cgfdh_set_moment_sources: bad argument");
+  if (s->src) { (void)hipFree(s->src); s->src = nullptr; }
+  if (s->mij) { (void)hipFree(s->mij); s->mij = nullptr; }
+  if (s->src_coef) { (void)hipFree(s->src_coef); s->src_coef = nullptr; }
+  s->nsrc = 0;
+  if (nsrc == 0) return CGFDH_OK;
+  if (!ijk || !it_begin || !it_end || !mij || max_nt < 1)
+    return fail(CGFDH_EINVAL, "cgfdh_set_moment_sources: missing tables");
+  if (spatial != CGFDH_SRC_POINT && spatial != CGFDH_SRC_GAUSSIAN)
+    return fail(CGFDH_EINVAL, "cgfdh_set_moment_sources: spatial type %d", spatial);
+  if (spatial == CGFDH_SRC_GAUSSIAN && (!ext_coefs || ext_half < 0 || ext_half > 8))
+    return fail(CGFDH_EINVAL, "cgfdh_set_moment_sources: Gaussian needs weights, half %d",
+                ext_half);
+  std::vector<SrcDev> hs(nsrc);
+  int L = 2 * ext_half + 1;
+  for (int is = 0; is < nsrc; is++) {
+    SrcDev& q = hs[is];
+    q.si = ijk[3 * is]; q.sj = ijk[3 * is + 1]; q.sk = ijk[3 * is + 2];
+    if (q.si < s->ni1 || q.si > s->ni2 || q.sj < s->nj1 || q.sj > s->nj2 || q.sk < s->nk1 ||
+        q.sk > s->nk2)
+      return fail(CGFDH_EINVAL, "cgfdh_set_moment_sources: source %d at (%d,%d,%d) outside the grid",
+                  is, q.si, q.sj, q.sk);
+    q.it_begin = it_begin[is];
+    q.it_end = it_end[is];
+    if (q.it_end - q.it_begin + 1 > max_nt)
+      return fail(CGFDH_EINVAL, "cgfdh_set_moment_sources: source %d spans more than max_nt", is);
+    q.kb = (q.sk + ext_half < s->nk2) ? ext_half : s->nk2 - q.sk;
+    q.coef_off = is * L * L * L;
+  }
+  HIPCHK(hipMalloc((void**)&s->src, nsrc * sizeof(SrcDev)));
+  HIPCHK(hipMemcpy(s->src, hs.data(), nsrc * sizeof(SrcDev), hipMemcpyHostToDevice));
+  s->mij_stride = (long)nsrc * max_nt * 4;
+  HIPCHK(hipMalloc((void**)&s->mij, 6 * s->mij_stride * sizeof(float)));
+  HIPCHK(hipMemcpy(s->mij, mij, 6 * s->mij_stride * sizeof(float), hipMemcpyHostToDevice));
+  if (spatial == CGFDH_SRC_GAUSSIAN) {
+    size_t n = (size_t)nsrc * L * L * L;
+    HIPCHK(hipMalloc((void**)&s->src_coef, n * sizeof(float)));
+    HIPCHK(hipMemcpy(s->src_coef, ext_coefs, n * sizeof(float), hipMemcpyHostToDevice));
+  }
+  s->nsrc = nsrc;
+  s->src_max_nt = max_nt;
+  s->src_spatial = spatial;
+  s->src_half = spatial == CGFDH_SRC_GAUSSIAN ? ext_half : 0;
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_set_receivers(cgfdh_solver* s, int nrecv, const int* idx, const float* dijk)
+{
+  if (!s || nrecv < 0) return fail(CGFDH_EINVAL, "cgfdh_set_receivers: bad argument");
+  if (s->recv_idx) { (void)hipFree(s->recv_idx); s->recv_idx = nullptr; }
+  if (s->recv_w) { (void)hipFree(s->recv_w); s->recv_w = nullptr; }
+  if (s->seismo) { (void)hipFree(s->seismo); s->seismo = nullptr; }
+  s->nrecv = 0;
+  if (nrecv == 0) return CGFDH_OK;
+  if (!idx || !dijk) return fail(CGFDH_EINVAL, "cgfdh_set_receivers: missing tables");
+  for (int r = 0; r < 8 * nrecv; r++)
+    if (idx[r] < 0 || idx[r] >= s->vol)
+      return fail(CGFDH_EINVAL, "cgfdh_set_receivers: corner index %d out of the volume", idx[r]);
+  HIPCHK(hipMalloc((void**)&s->recv_idx, 8 * nrecv * sizeof(int)));
+  HIPCHK(hipMemcpy(s->recv_idx, idx, 8 * nrecv * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc((void**)&s->recv_w, 3 * nrecv * sizeof(float)));
+  HIPCHK(hipMemcpy(s->recv_w, dijk, 3 * nrecv * sizeof(float), hipMemcpyHostToDevice));
+  size_t n = (size_t)nrecv * 9 * s->d.nt_total;
+  HIPCHK(hipMalloc((void**)&s->seismo, n * sizeof(float)));
+  HIPCHK(hipMemset(s->seismo, 0, n * sizeof(float)));
+  s->nrecv = nrecv;
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_set_wavefield(cgfdh_solver* s, const float* w)
+{
+  if (!s || !w) return fail(CGFDH_EINVAL, "cgfdh_set_wavefield: null argument");
+  return h2d(s, s->lvl[0], w, 9 * s->vol);
+}
+
+extern "C" int cgfdh_get_wavefield(cgfdh_solver* s, float* w)
+{
+  if (!s || !w) return fail(CGFDH_EINVAL, "cgfdh_get_wavefield: null argument");
+  HIPCHK(hipMemcpyAsync(w, s->lvl[0], 9 * s->vol * sizeof(float), hipMemcpyDeviceToHost,
+                        s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return CGFDH_OK;
+}
+
+extern "C" size_t cgfdh_pml_aux_len(cgfdh_solver* s)
+{
+  if (!s) return 0;
+  size_t n = 0;
+  for (auto& f : s->face)
+    if (f.on) n += 9 * f.fn;
+  return n;
+}
+
+static int aux_copy(cgfdh_solver* s, int level, float* host, bool to_dev)
+{
+  size_t off = 0;
+  for (auto& f : s->face) {
+    if (!f.on) continue;
+    size_t n = 9 * f.fn * sizeof(float);
+    if (to_dev)
+      HIPCHK(hipMemcpyAsync(f.aux[level], host + off, n, hipMemcpyHostToDevice, s->stream));
+    else
+      HIPCHK(hipMemcpyAsync(host + off, f.aux[level], n, hipMemcpyDeviceToHost, s->stream));
+    off += 9 * f.fn;
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_set_pml_aux(cgfdh_solver* s, const float* aux)
+{
+  if (!s || !aux) return fail(CGFDH_EINVAL, "cgfdh_set_pml_aux: null argument");
+  return aux_copy(s, 0, (float*)aux, true);
+}
+
+extern "C" int cgfdh_get_pml_aux(cgfdh_solver* s, float* aux)
+{
+  if (!s || !aux) return fail(CGFDH_EINVAL, "cgfdh_get_pml_aux: null argument");
+  return aux_copy(s, 0, aux, false);
+}
+
+extern "C" int cgfdh_get_seismograms(cgfdh_solver* s, float* out)
+{
+  if (!s || !out) return fail(CGFDH_EINVAL, "cgfdh_get_seismograms: null argument");
+  if (s->nrecv == 0) return CGFDH_OK;
+  HIPCHK(hipMemcpyAsync(out, s->seismo, (size_t)s->nrecv * 9 * s->d.nt_total * sizeof(float),
+                        hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_set_kernel_variant(cgfdh_solver* s, int v)
+{
+  if (!s || v < 0 || v > 1) return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: %d", v);
+  s->variant = v;
+  return CGFDH_OK;
+}
+
+static int ready(cgfdh_solver* s, const char* who)
+{
+  if (!s->metric_set || !s->medium_set)
+    return fail(CGFDH_ESTATE, "%s: metric and medium must be set first", who);
+  if (s->d.free_top && !s->mats_set)
+    return fail(CGFDH_ESTATE, "%s: free surface enabled but its matrices are not set", who);
+  for (int f = 0; f < 6; f++)
+    if (s->face[f].on && !s->face[f].coefs_set)
+      return fail(CGFDH_ESTATE, "%s: PML face %d coefficients not set", who, f);
+  return CGFDH_OK;
+}
+
+// fill the launch parameters of one stage; wcur/wtmp and aux levels chosen
+// by the caller
+static StageParams make_params(cgfdh_solver* s, int it, int istage, const float* wcur,
+                               float* wtmp, int aux_cur_level)
+{
+  StageParams P;
+  memset(&P, 0, sizeof P);
+  P.ni1 = s->ni1; P.ni2 = s->ni2; P.nj1 = s->nj1; P.nj2 = s->nj2; P.nk1 = s->nk1; P.nk2 = s->nk2;
+  P.nx = s->nx; P.ss = s->ss; P.vol = s->vol;
+  P.met = s->met;
+  P.lam = s->med; P.mu = s->med + s->vol; P.slw = s->med + 2 * s->vol;
+  P.wcur = wcur; P.wpre = s->lvl[0]; P.wtmp = wtmp; P.wend = s->lvl[3];
+  P.ca = istage < 3 ? (float)kRkA[istage] * s->d.dt : 0.0f;
+  P.cb = (float)kRkB[istage] * s->d.dt;
+  P.free_top = s->d.free_top;
+  P.matVx = s->mats; P.matVy = s->mats + 9 * s->ss;
+  int n = 0;
+  for (int f = 0; f < 6; f++) {
+    FaceHost& F = s->face[f];
+    if (!F.on) continue;
+    PmlDev& D = P.pml[n++];
+    D.idim = f / 2;
+    D.i1 = F.region[0]; D.i2 = F.region[1]; D.j1 = F.region[2]; D.j2 = F.region[3];
+    D.k1 = F.region[4]; D.k2 = F.region[5];
+    D.fnx = F.fnx; D.fny = F.fny; D.fn = F.fn;
+    D.A = F.A; D.B = F.B; D.D = F.D;
+    D.aux_cur = F.aux[aux_cur_level];
+    D.aux_pre = F.aux[0]; D.aux_tmp = F.aux[1]; D.aux_end = F.aux[2];
+  }
+  P.npml = n;
+  P.nsrc = s->nsrc; P.src = s->src; P.mij = s->mij; P.mij_stride = s->mij_stride;
+  P.src_max_nt = s->src_max_nt; P.src_spatial = s->src_spatial; P.src_half = s->src_half;
+  P.src_coef = s->src_coef;
+  P.it = it; P.istage = istage;
+  return P;
+}
+
+template <int KIND>
+static void launch_kind(cgfdh_solver* s, const StageParams& P, int dx, int dy, int dz)
+{
+  const int code = dx * 4 + dy * 2 + dz;
+  if (s->variant == 0) {
+    dim3 blk(64, 4, 1);
+    dim3 grd((s->d.ni + 63) / 64, (s->d.nj + 3) / 4, s->d.nk);
+#define CGFDH_PT(c, DX, DY, DZ) \
+  case c: hipLaunchKernelGGL((stage_point_kernel<DX, DY, DZ, KIND>), grd, blk, 0, s->stream, P); break;
+    switch (code) {
+      CGFDH_PT(0, 0, 0, 0) CGFDH_PT(1, 0, 0, 1) CGFDH_PT(2, 0, 1, 0) CGFDH_PT(3, 0, 1, 1)
+      CGFDH_PT(4, 1, 0, 0) CGFDH_PT(5, 1, 0, 1) CGFDH_PT(6, 1, 1, 0) CGFDH_PT(7, 1, 1, 1)
+    }
+#undef CGFDH_PT
+  } else {
+    launch_tiled<KIND>(s->stream, P, s->d.ni, s->d.nj, s->d.nk, code);
+  }
+}
+
+static void launch_stage(cgfdh_solver* s, const StageParams& P, int kind, int ipair, int istage)
+{
+  const int dx = (kPairFlags[ipair][0] + istage) % 2;
+  const int dy = (kPairFlags[ipair][1] + istage) % 2;
+  const int dz = (kPairFlags[ipair][2] + istage) % 2;
+  switch (kind) {
+    case KIND_FIRST: launch_kind<KIND_FIRST>(s, P, dx, dy, dz); break;
+    case KIND_MID: launch_kind<KIND_MID>(s, P, dx, dy, dz); break;
+    case KIND_LAST: launch_kind<KIND_LAST>(s, P, dx, dy, dz); break;
+    default: launch_kind<KIND_RHS>(s, P, dx, dy, dz); break;
+  }
+}
+
+extern "C" int cgfdh_run(cgfdh_solver* s, int it0, int nsteps)
+{
+  if (!s) return fail(CGFDH_EINVAL, "cgfdh_run: null solver");
+  int rc = ready(s, "cgfdh_run");
+  if (rc) return rc;
+  if (it0 < 0 || nsteps < 0 || it0 + nsteps > s->d.nt_total)
+    return fail(CGFDH_EINVAL, "cgfdh_run: steps %d..%d outside nt_total %d", it0, it0 + nsteps,
+                s->d.nt_total);
+  const int nev = 8 * nsteps;
+  while ((int)s->ev_stage.size() < nev) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    s->ev_stage.push_back(e);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(s->ev_run[0], s->stream));
+  for (int n = 0; n < nsteps; n++) {
+    const int it = it0 + n;
+    const int ipair = it % 8;
+    float* tA = s->lvl[1];
+    float* tB = s->lvl[2];
+    for (int st = 0; st < 4; st++) {
+      const float* wcur = st == 0 ? s->lvl[0] : (st % 2 == 1 ? tA : tB);
+      float* wtmp = st % 2 == 0 ? tA : tB;
+      StageParams P = make_params(s, it, st, wcur, wtmp, st == 0 ? 0 : 1);
+      const int kind = st == 0 ? KIND_FIRST : (st == 3 ? KIND_LAST : KIND_MID);
+      HIPCHK(hipEventRecord(s->ev_stage[8 * n + 2 * st], s->stream));
+      launch_stage(s, P, kind, ipair, st);
+      HIPCHK(hipEventRecord(s->ev_stage[8 * n + 2 * st + 1], s->stream));
+    }
+    if (s->nrecv > 0) {
+      int nt = s->nrecv * 9;
+      hipLaunchKernelGGL(recv_kernel, dim3((nt + 255) / 256), dim3(256), 0, s->stream,
+                         s->lvl[3], s->vol, s->nrecv, s->recv_idx, s->recv_w, s->seismo,
+                         s->d.nt_total, it);
+    }
+    std::swap(s->lvl[0], s->lvl[3]);
+    for (auto& f : s->face)
+      if (f.on) std::swap(f.aux[0], f.aux[2]);
+  }
+  HIPCHK(hipEventRecord(s->ev_run[1], s->stream));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->stream));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, s->ev_run[0], s->ev_run[1]));
+  s->last_run_ms = ms;
+  double st_ms = 0;
+  for (int e = 0; e < 4 * nsteps; e++) {
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, s->ev_stage[2 * e], s->ev_stage[2 * e + 1]));
+    st_ms += t;
+  }
+  s->last_stage_ms = st_ms;
+  s->last_launches = 4 * nsteps;
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_last_timing(cgfdh_solver* s, double* run_ms, double* stage_ms, int* n)
+{
+  if (!s) return fail(CGFDH_EINVAL, "cgfdh_last_timing: null solver");
+  if (run_ms) *run_ms = s->last_run_ms;
+  if (stage_ms) *stage_ms = s->last_stage_ms;
+  if (n) *n = s->last_launches;
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_onestage(cgfdh_solver* s, int it, int ipair, int istage, const float* w_cur,
+                              const float* aux_cur, float* rhs, float* aux_rhs)
+{
+  if (!s || !w_cur || !rhs) return fail(CGFDH_EINVAL, "cgfdh_onestage: null argument");
+  if (ipair < 0 || ipair > 7 || istage < 0 || istage > 3)
+    return fail(CGFDH_EINVAL, "cgfdh_onestage: pair %d stage %d", ipair, istage);
+  int rc = ready(s, "cgfdh_onestage");
+  if (rc) return rc;
+  // inputs go to tmpA / aux tmp, the rhs to tmpB / aux end
+  HIPCHK(hipMemcpyAsync(s->lvl[1], w_cur, 9 * s->vol * sizeof(float), hipMemcpyHostToDevice,
+                        s->stream));
+  HIPCHK(hipMemsetAsync(s->lvl[2], 0, 9 * s->vol * sizeof(float), s->stream));
+  if (aux_cur) {
+    rc = aux_copy(s, 1, (float*)aux_cur, true);
+    if (rc) return rc;
+  }
+  StageParams P = make_params(s, it, istage, s->lvl[1], nullptr, 1);
+  P.rhs_out = s->lvl[2];
+  int n = 0;
+  for (auto& f : s->face)
+    if (f.on) P.aux_rhs_out[n++] = f.aux[2];
+  launch_stage(s, P, KIND_RHS, ipair, istage);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(rhs, s->lvl[2], 9 * s->vol * sizeof(float), hipMemcpyDeviceToHost,
+                        s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (aux_rhs) return aux_copy(s, 2, aux_rhs, false);
+  return CGFDH_OK;
+}

@@ -1,0 +1,412 @@
+// stage_common.h -- device-side pieces shared by the stage kernels.
+//
+// Arithmetic follows the reference expression by expression and is compiled
+// with -ffp-contract=off (no FMA contraction), so every value matches the
+// reference's x86-64 FP32 evaluation.  Where the reference evaluates in
+// double and rounds to float (lam + 2.0*mu, coef_B - 1.0, 1.0/jac, ...),
+// the device code does the same.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace cgfdh {
+
+// wavefield component slots (wav_t.c:93-136)
+enum { VX = 0, VY, VZ, TXX, TYY, TZZ, TYZ, TXZ, TXY };
+
+// DRP MacCormack operators as float, fd_t.c:109-112 (row 3);
+// dir 0 = {-1..3} ("forward"), dir 1 = {-3..1}
+#define CGFDH_F0 ((float)-0.30874)
+#define CGFDH_F1 ((float)-0.6326)
+#define CGFDH_F2 ((float)1.233)
+#define CGFDH_F3 ((float)-0.3334)
+#define CGFDH_F4 ((float)0.04168)
+
+template <int DIR>
+__device__ __forceinline__ float d5(const float* __restrict__ v, long p, long s)
+{
+  if (DIR == 0)
+    return CGFDH_F0 * v[p - s] + CGFDH_F1 * v[p] + CGFDH_F2 * v[p + s] +
+           CGFDH_F3 * v[p + 2 * s] + CGFDH_F4 * v[p + 3 * s];
+  else
+    return -CGFDH_F4 * v[p - 3 * s] + -CGFDH_F3 * v[p - 2 * s] + -CGFDH_F2 * v[p - s] +
+           -CGFDH_F1 * v[p] + -CGFDH_F0 * v[p + s];
+}
+
+// the same operator applied to 5 values already gathered in window order
+template <int DIR>
+__device__ __forceinline__ float d5v(float a0, float a1, float a2, float a3, float a4)
+{
+  if (DIR == 0)
+    return CGFDH_F0 * a0 + CGFDH_F1 * a1 + CGFDH_F2 * a2 + CGFDH_F3 * a3 + CGFDH_F4 * a4;
+  else
+    return -CGFDH_F4 * a0 + -CGFDH_F3 * a1 + -CGFDH_F2 * a2 + -CGFDH_F1 * a3 + -CGFDH_F0 * a4;
+}
+
+template <int DIR> struct Win {
+  // offset of window entry 0 from the point
+  static constexpr int lo = DIR == 0 ? -1 : -3;
+};
+
+struct PmlDev {
+  int idim;                 // 0,1,2
+  int i1, i2, j1, j2, k1, k2;
+  int fnx, fny;
+  long fn;                  // points per component
+  const float* A;
+  const float* B;
+  const float* D;
+  const float* aux_cur;     // 9*fn, component order of the wavefield
+  const float* aux_pre;
+  float* aux_tmp;           // stage output (may alias aux_cur)
+  float* aux_end;
+};
+
+struct SrcDev {
+  int si, sj, sk;
+  int it_begin, it_end;
+  int kb;                   // k extent below the point (src k2_ext_below_surface)
+  int coef_off;             // offset of this source's weights in ext_coefs
+};
+
+struct StageParams {
+  int ni1, ni2, nj1, nj2, nk1, nk2;
+  int nx;
+  long ss, vol;
+  const float* met;         // 10 volumes
+  const float* lam;
+  const float* mu;
+  const float* slw;
+  const float* wcur;        // 9 volumes each
+  const float* wpre;
+  float* wtmp;
+  float* wend;
+  float* rhs_out;           // KIND_RHS only
+  float ca, cb;
+  int free_top;
+  const float* matVx;       // [ny][nx][3][3]
+  const float* matVy;
+  int npml;
+  PmlDev pml[6];
+  float* aux_rhs_out[6];    // KIND_RHS only, per enabled face
+  int nsrc;
+  const SrcDev* src;
+  const float* mij;         // 6 arrays of mij_stride
+  long mij_stride;
+  int src_max_nt;
+  int src_spatial;          // 1 point, 2 gaussian
+  int src_half;
+  const float* src_coef;
+  int it, istage;
+  // tiling of the z-marching kernel
+  int kchunk;
+  int tiles_x, tiles_y;
+};
+
+enum { KIND_FIRST = 0, KIND_MID = 1, KIND_LAST = 2, KIND_RHS = 3 };
+
+// Hooke's law from the 9 velocity derivatives (sv_curv_col_el_iso.c:409-435)
+__device__ __forceinline__ void hooke(float* r, const float* m, float lam, float lam2mu,
+                                      float mu, const float* X, const float* Y,
+                                      const float* Z)
+{
+  const float xix = m[0], xiy = m[1], xiz = m[2];
+  const float etx = m[3], ety = m[4], etz = m[5];
+  const float ztx = m[6], zty = m[7], ztz = m[8];
+  r[TXX] = lam2mu * (xix * X[0] + etx * Y[0] + ztx * Z[0]) +
+           lam * (xiy * X[1] + ety * Y[1] + zty * Z[1] + xiz * X[2] + etz * Y[2] + ztz * Z[2]);
+  r[TYY] = lam2mu * (xiy * X[1] + ety * Y[1] + zty * Z[1]) +
+           lam * (xix * X[0] + etx * Y[0] + ztx * Z[0] + xiz * X[2] + etz * Y[2] + ztz * Z[2]);
+  r[TZZ] = lam2mu * (xiz * X[2] + etz * Y[2] + ztz * Z[2]) +
+           lam * (xix * X[0] + etx * Y[0] + ztx * Z[0] + xiy * X[1] + ety * Y[1] + zty * Z[1]);
+  r[TXY] = mu * (xiy * X[0] + xix * X[1] + ety * Y[0] + etx * Y[1] + zty * Z[0] + ztx * Z[1]);
+  r[TXZ] = mu * (xiz * X[0] + xix * X[2] + etz * Y[0] + etx * Y[2] + ztz * Z[0] + ztx * Z[2]);
+  r[TYZ] = mu * (xiz * X[1] + xiy * X[2] + etz * Y[1] + ety * Y[2] + ztz * Z[1] + zty * Z[2]);
+}
+
+// momentum equation (sv_curv_col_el_iso.c:398-406); X,Y,Z hold the 9
+// derivatives in component-slot order
+__device__ __forceinline__ void momentum(float* r, const float* m, float slw,
+                                         const float* X, const float* Y, const float* Z)
+{
+  const float xix = m[0], xiy = m[1], xiz = m[2];
+  const float etx = m[3], ety = m[4], etz = m[5];
+  const float ztx = m[6], zty = m[7], ztz = m[8];
+  r[VX] = slw * (xix * X[TXX] + xiy * X[TXY] + xiz * X[TXZ] + etx * Y[TXX] + ety * Y[TXY] +
+                 etz * Y[TXZ] + ztx * Z[TXX] + zty * Z[TXY] + ztz * Z[TXZ]);
+  r[VY] = slw * (xix * X[TXY] + xiy * X[TYY] + xiz * X[TYZ] + etx * Y[TXY] + ety * Y[TYY] +
+                 etz * Y[TYZ] + ztx * Z[TXY] + zty * Z[TYY] + ztz * Z[TYZ]);
+  r[VZ] = slw * (xix * X[TXZ] + xiy * X[TYZ] + xiz * X[TZZ] + etx * Y[TXZ] + ety * Y[TYZ] +
+                 etz * Y[TZZ] + ztx * Z[TXZ] + zty * Z[TYZ] + ztz * Z[TZZ]);
+}
+
+// conservative traction vector jac*(e1*T0 + e2*T1 + e3*T2) at q
+__device__ __forceinline__ float tvec(const StageParams& P, long q, int e, const float* T0,
+                                      const float* T1, const float* T2)
+{
+  const float* M = P.met;
+  return M[q] * (M[e * P.vol + q] * T0[q] + M[(e + 1) * P.vol + q] * T1[q] +
+                 M[(e + 2) * P.vol + q] * T2[q]);
+}
+
+// traction-image hV on the top rows (sv_curv_col_el.c:31-305), zero surface
+// traction.  Mirror values come from the grid points they stand for.
+template <int DX, int DY, int DZ>
+__device__ void timg_hv(const StageParams& P, int k, long p, float* r)
+{
+  const float* w = P.wcur;
+  const long V = P.vol, sl = P.nx, ss = P.ss;
+  const int zlo = Win<DZ>::lo;
+  const int n_free = P.nk2 - k - zlo;
+  const float slwjac = P.slw[p] / P.met[p];
+  const int rows[3][3] = {{TXX, TXY, TXZ}, {TXY, TYY, TYZ}, {TXZ, TYZ, TZZ}};
+#pragma unroll
+  for (int comp = 0; comp < 3; comp++) {
+    const float* T0 = w + rows[comp][0] * V;
+    const float* T1 = w + rows[comp][1] * V;
+    const float* T2 = w + rows[comp][2] * V;
+    float vx[5], vy[5], vz[5];
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+      vx[n] = tvec(P, p + (Win<DX>::lo + n), 1, T0, T1, T2);
+      vy[n] = tvec(P, p + (long)(Win<DY>::lo + n) * sl, 4, T0, T1, T2);
+    }
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+      if (n < n_free) {
+        vz[n] = tvec(P, p + (long)(zlo + n) * ss, 7, T0, T1, T2);
+      } else if (n == n_free) {
+        vz[n] = 0.0f;
+      } else {
+        const long off = (zlo + n) - 2 * (n - n_free);
+        const float below = tvec(P, p + off * ss, 7, T0, T1, T2);
+        vz[n] = (float)(2.0 * 0.0f - (double)below);
+      }
+    }
+    const float DxT = d5v<DX>(vx[0], vx[1], vx[2], vx[3], vx[4]);
+    const float DyT = d5v<DY>(vy[0], vy[1], vy[2], vy[3], vy[4]);
+    const float DzT = d5v<DZ>(vz[0], vz[1], vz[2], vz[3], vz[4]);
+    r[comp] = (DxT + DyT + DzT) * slwjac;
+  }
+}
+
+// hT on the top 3 rows (sv_curv_col_el_iso.c:452-634): matrices at the
+// surface, the 2- and 3-point one-sided z operators below it
+template <int DZ>
+__device__ void vlow_ht(const StageParams& P, int i, int j, int k, long p, const float* m,
+                        float lam, float lam2mu, float mu, const float* X, const float* Y,
+                        float* r)
+{
+  float Z[3];
+  const long V = P.vol, ss = P.ss;
+  if (k == P.nk2) {
+    const long ij = ((long)i + (long)j * P.nx) * 9;
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+      Z[q] = P.matVx[ij + 3 * q + 0] * X[0] + P.matVx[ij + 3 * q + 1] * X[1] +
+             P.matVx[ij + 3 * q + 2] * X[2] + P.matVy[ij + 3 * q + 0] * Y[0] +
+             P.matVy[ij + 3 * q + 1] * Y[1] + P.matVy[ij + 3 * q + 2] * Y[2];
+  } else if (k == P.nk2 - 1) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const float* v = P.wcur + c * V;
+      float d;
+      if (DZ == 0) { d = -1.0f * v[p]; d += 1.0f * v[p + ss]; }
+      else { d = -1.0f * v[p - ss]; d += 1.0f * v[p]; }
+      Z[c] = d;
+    }
+  } else {
+    const float c0 = DZ == 0 ? (float)(-7.0 / 6.0) : (float)(1.0 / 6.0);
+    const float c1 = DZ == 0 ? (float)(8.0 / 6.0) : (float)(-8.0 / 6.0);
+    const float c2 = DZ == 0 ? (float)(-1.0 / 6.0) : (float)(7.0 / 6.0);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const float* v = P.wcur + c * V;
+      float d;
+      if (DZ == 0) { d = c0 * v[p]; d += c1 * v[p + ss]; d += c2 * v[p + 2 * ss]; }
+      else { d = c0 * v[p - 2 * ss]; d += c1 * v[p - ss]; d += c2 * v[p]; }
+      Z[c] = d;
+    }
+  }
+  hooke(r, m, lam, lam2mu, mu, X, Y, Z);
+}
+
+// CFS-PML correction of one face (sv_curv_col_el_iso.c:645-1146) and the
+// RK combine of its auxiliary variables
+template <int KIND>
+__device__ void pml_face(const StageParams& P, const PmlDev& F, int fslot, int i, int j, int k,
+                         long p, const float* Dd, float lam, float lam2mu, float mu,
+                         float slw, float* r)
+{
+  const long V = P.vol;
+  const int ic = F.idim == 0 ? i - F.i1 : (F.idim == 1 ? j - F.j1 : k - F.k1);
+  const float cD = F.D[ic], cA = F.A[ic], cB = F.B[ic];
+  const float cBm1 = (float)((double)cB - 1.0);
+  const float* M = P.met;
+  const int e = 1 + 3 * F.idim;
+  const float e1 = M[e * V + p], e2 = M[(e + 1) * V + p], e3 = M[(e + 2) * V + p];
+  float rr[9];
+  rr[VX] = slw * (e1 * Dd[TXX] + e2 * Dd[TXY] + e3 * Dd[TXZ]);
+  rr[VY] = slw * (e1 * Dd[TXY] + e2 * Dd[TYY] + e3 * Dd[TYZ]);
+  rr[VZ] = slw * (e1 * Dd[TXZ] + e2 * Dd[TYZ] + e3 * Dd[TZZ]);
+  rr[TXX] = lam2mu * e1 * Dd[VX] + lam * e2 * Dd[VY] + lam * e3 * Dd[VZ];
+  rr[TYY] = lam * e1 * Dd[VX] + lam2mu * e2 * Dd[VY] + lam * e3 * Dd[VZ];
+  rr[TZZ] = lam * e1 * Dd[VX] + lam * e2 * Dd[VY] + lam2mu * e3 * Dd[VZ];
+  rr[TXY] = mu * (e2 * Dd[VX] + e1 * Dd[VY]);
+  rr[TXZ] = mu * (e3 * Dd[VX] + e1 * Dd[VZ]);
+  rr[TYZ] = mu * (e3 * Dd[VY] + e2 * Dd[VZ]);
+  const long ia = (long)(i - F.i1) + (long)(j - F.j1) * F.fnx + (long)(k - F.k1) * F.fnx * F.fny;
+  float a[9], ah[9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) a[c] = F.aux_cur[c * F.fn + ia];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    r[c] += cBm1 * rr[c] - cB * a[c];
+    ah[c] = cD * rr[c] - cA * a[c];
+  }
+  if (F.idim < 2 && P.free_top && k == P.nk2) {
+    const long ij = ((long)i + (long)j * P.nx) * 9;
+    const float* mm = F.idim == 0 ? P.matVx : P.matVy;
+    float Dz[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+      Dz[q] = mm[ij + 3 * q + 0] * Dd[VX] + mm[ij + 3 * q + 1] * Dd[VY] +
+              mm[ij + 3 * q + 2] * Dd[VZ];
+    const float ztx = M[7 * V + p], zty = M[8 * V + p], ztz = M[9 * V + p];
+    float s[9];
+    s[TXX] = lam2mu * (ztx * Dz[0]) + lam * (zty * Dz[1] + ztz * Dz[2]);
+    s[TYY] = lam2mu * (zty * Dz[1]) + lam * (ztx * Dz[0] + ztz * Dz[2]);
+    s[TZZ] = lam2mu * (ztz * Dz[2]) + lam * (ztx * Dz[0] + zty * Dz[1]);
+    s[TXY] = mu * (zty * Dz[0] + ztx * Dz[1]);
+    s[TXZ] = mu * (ztz * Dz[0] + ztx * Dz[2]);
+    s[TYZ] = mu * (ztz * Dz[1] + zty * Dz[2]);
+#pragma unroll
+    for (int c = TXX; c <= TXY; c++) {
+      r[c] = (float)((double)r[c] + ((double)cB - 1.0) * (double)s[c]);
+      ah[c] += cD * s[c];
+    }
+  }
+  if (KIND == KIND_RHS) {
+#pragma unroll
+    for (int c = 0; c < 9; c++) P.aux_rhs_out[fslot][c * F.fn + ia] = ah[c];
+  } else if (KIND == KIND_FIRST) {
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+      const float pre = F.aux_pre[c * F.fn + ia];
+      F.aux_tmp[c * F.fn + ia] = pre + P.ca * ah[c];
+      F.aux_end[c * F.fn + ia] = pre + P.cb * ah[c];
+    }
+  } else if (KIND == KIND_MID) {
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+      const float pre = F.aux_pre[c * F.fn + ia];
+      const float end = F.aux_end[c * F.fn + ia];
+      F.aux_tmp[c * F.fn + ia] = pre + P.ca * ah[c];
+      F.aux_end[c * F.fn + ia] = end + P.cb * ah[c];
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+      const float end = F.aux_end[c * F.fn + ia];
+      F.aux_end[c * F.fn + ia] = end + P.cb * ah[c];
+    }
+  }
+}
+
+// moment sources (sv_curv_col_el.c:312-479, moment part)
+__device__ __forceinline__ void add_sources(const StageParams& P, int i, int j, int k, long p,
+                                            float* r)
+{
+  for (int is = 0; is < P.nsrc; is++) {
+    const SrcDev S = P.src[is];
+    if (P.it < S.it_begin || P.it > S.it_end) continue;
+    float coefv;
+    bool hit;
+    if (P.src_spatial == 1) {
+      hit = (i == S.si && j == S.sj && k == S.sk);
+      if (!hit) continue;
+      coefv = (float)(1.0 / (double)P.met[p]);
+    } else {
+      const int h = P.src_half;
+      const int di = i - S.si, dj = j - S.sj, dk = k - S.sk;
+      hit = di >= -h && di <= h && dj >= -h && dj <= h && dk >= -h && dk <= S.kb;
+      if (!hit) continue;
+      const int L = 2 * h + 1;
+      const float coef = P.src_coef[S.coef_off + ((dk + h) * L + (dj + h)) * L + (di + h)];
+      coefv = coef / P.met[p];
+    }
+    const long ip = (long)is * P.src_max_nt * 4 + (long)(P.it - S.it_begin) * 4 + P.istage;
+    const int slot[6] = {TXX, TYY, TZZ, TXZ, TYZ, TXY};
+#pragma unroll
+    for (int m = 0; m < 6; m++) r[slot[m]] -= P.mij[m * P.mij_stride + ip] * coefv;
+  }
+}
+
+// RK combine of the wavefield (drv_rk_curv_col.c:291-446)
+template <int KIND>
+__device__ __forceinline__ void combine(const StageParams& P, long p, const float* r)
+{
+  const long V = P.vol;
+  if (KIND == KIND_RHS) {
+#pragma unroll
+    for (int c = 0; c < 9; c++) P.rhs_out[c * V + p] = r[c];
+  } else if (KIND == KIND_FIRST) {
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+      const float pre = P.wpre[c * V + p];
+      P.wtmp[c * V + p] = pre + P.ca * r[c];
+      P.wend[c * V + p] = pre + P.cb * r[c];
+    }
+  } else if (KIND == KIND_MID) {
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+      const float pre = P.wpre[c * V + p];
+      const float end = P.wend[c * V + p];
+      P.wtmp[c * V + p] = pre + P.ca * r[c];
+      P.wend[c * V + p] = end + P.cb * r[c];
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+      const float end = P.wend[c * V + p];
+      P.wend[c * V + p] = end + P.cb * r[c];
+    }
+  }
+}
+
+// everything after the derivatives: inner formulas, surface overrides, PML,
+// sources, RK combine.  X/Y/Z: inner-operator derivatives of the 9 fields.
+template <int DX, int DY, int DZ, int KIND>
+__device__ __forceinline__ void finish_point(const StageParams& P, int i, int j, int k, long p,
+                                             const float* X, const float* Y, const float* Z)
+{
+  const long V = P.vol;
+  float m[9];
+#pragma unroll
+  for (int q = 0; q < 9; q++) m[q] = P.met[(1 + q) * V + p];
+  const float lam = P.lam[p], mu = P.mu[p], slw = P.slw[p];
+  const float lam2mu = (float)((double)lam + 2.0 * (double)mu);
+  float r[9];
+  momentum(r, m, slw, X, Y, Z);
+  if (P.free_top && k >= P.nk2 - (DZ == 0 ? 3 : 1)) {
+    // hV: traction image on rows nk2-3..nk2 (forward z op) or nk2-1..nk2
+    timg_hv<DX, DY, DZ>(P, k, p, r);
+  }
+  float Xv[3] = {X[VX], X[VY], X[VZ]}, Yv[3] = {Y[VX], Y[VY], Y[VZ]};
+  if (P.free_top && k >= P.nk2 - 2) {
+    vlow_ht<DZ>(P, i, j, k, p, m, lam, lam2mu, mu, Xv, Yv, r);
+  } else {
+    float Zv[3] = {Z[VX], Z[VY], Z[VZ]};
+    hooke(r, m, lam, lam2mu, mu, Xv, Yv, Zv);
+  }
+  for (int f = 0; f < P.npml; f++) {
+    const PmlDev& F = P.pml[f];
+    if (i >= F.i1 && i <= F.i2 && j >= F.j1 && j <= F.j2 && k >= F.k1 && k <= F.k2) {
+      const float* Dd = F.idim == 0 ? X : (F.idim == 1 ? Y : Z);
+      pml_face<KIND>(P, F, f, i, j, k, p, Dd, lam, lam2mu, mu, slw, r);
+    }
+  }
+  if (P.nsrc > 0) add_sources(P, i, j, k, p, r);
+  combine<KIND>(P, p, r);
+}
+
+}  // namespace cgfdh

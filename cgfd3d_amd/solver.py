@@ -1,0 +1,152 @@
+"""Python view of one engine instance (include/cgfdh.h): upload the static
+fields, advance steps, read back seismograms and state.
+
+The arrays use the reference layouts: grids with 3 ghost points per side,
+x fastest; wavefield components Vx,Vy,Vz,Txx,Tyy,Tzz,Tyz,Txz,Txy.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import CgfdhDesc, check, lib
+
+MEDIUM_ELASTIC_ISO = 2
+SRC_POINT = 1
+SRC_GAUSSIAN = 2
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Solver:
+    """Engine for one grid block: ni x nj x nk physical points."""
+
+    def __init__(self, ni, nj, nk, dt, nt_total, free_top=True, pml_on=(0,) * 6,
+                 pml_layers=(0,) * 6, medium_type=MEDIUM_ELASTIC_ISO):
+        d = CgfdhDesc()
+        d.ni, d.nj, d.nk = ni, nj, nk
+        d.medium_type = medium_type
+        d.free_top = int(free_top)
+        for f in range(6):
+            d.pml_on[f] = int(pml_on[f])
+            d.pml_layers[f] = int(pml_layers[f]) if pml_on[f] else 0
+        d.dt = dt
+        d.nt_total = nt_total
+        self.desc = d
+        self.ni, self.nj, self.nk = ni, nj, nk
+        self.nx, self.ny, self.nz = ni + 6, nj + 6, nk + 6
+        self.vol = self.nx * self.ny * self.nz
+        self.nt_total = nt_total
+        self.nrecv = 0
+        h = C.c_void_p()
+        check(lib().cgfdh_create(C.byref(d), C.byref(h)), "cgfdh_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cgfdh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- static inputs ----
+    def set_metric(self, metric10):
+        m = _f32(metric10)
+        assert m.size == 10 * self.vol
+        check(lib().cgfdh_set_metric(self.h, _p(m)), "set_metric")
+
+    def set_medium_iso(self, lam, mu, slw):
+        a, b, c = _f32(lam), _f32(mu), _f32(slw)
+        assert a.size == b.size == c.size == self.vol
+        check(lib().cgfdh_set_medium_iso(self.h, _p(a), _p(b), _p(c)), "set_medium_iso")
+
+    def set_pml_face(self, face, A, B, D):
+        a, b, d = _f32(A), _f32(B), _f32(D)
+        check(lib().cgfdh_set_pml_face(self.h, face, _p(a), _p(b), _p(d)), "set_pml_face")
+
+    def set_free_surface(self, mats):
+        m = _f32(mats).reshape(3, -1)
+        check(lib().cgfdh_set_free_surface(self.h, _p(m[0].copy()), _p(m[1].copy()),
+                                           _p(m[2].copy())), "set_free_surface")
+
+    def set_moment_sources(self, ijk, it_begin, it_end, max_nt, mij, spatial=SRC_POINT,
+                           ext_half=0, ext_coefs=None):
+        ijk = np.ascontiguousarray(ijk, np.int32).reshape(-1)
+        nsrc = ijk.size // 3
+        itb = np.ascontiguousarray(it_begin, np.int32)
+        ite = np.ascontiguousarray(it_end, np.int32)
+        m = _f32(mij)
+        ec = None if ext_coefs is None else _f32(ext_coefs)
+        check(lib().cgfdh_set_moment_sources(self.h, nsrc, _p(ijk), _p(itb), _p(ite), max_nt,
+                                             _p(m), spatial, ext_half,
+                                             None if ec is None else _p(ec)),
+              "set_moment_sources")
+
+    def set_receivers(self, indx1d, dijk):
+        idx = np.ascontiguousarray(indx1d, np.int32).reshape(-1)
+        w = _f32(dijk).reshape(-1)
+        self.nrecv = idx.size // 8
+        check(lib().cgfdh_set_receivers(self.h, self.nrecv, _p(idx), _p(w)), "set_receivers")
+
+    # ---- state ----
+    def set_wavefield(self, w9):
+        w = _f32(w9)
+        check(lib().cgfdh_set_wavefield(self.h, _p(w)), "set_wavefield")
+
+    def wavefield(self):
+        w = np.empty(9 * self.vol, np.float32)
+        check(lib().cgfdh_get_wavefield(self.h, _p(w)), "get_wavefield")
+        return w
+
+    def aux_len(self):
+        return int(lib().cgfdh_pml_aux_len(self.h))
+
+    def set_pml_aux(self, aux):
+        a = _f32(aux)
+        check(lib().cgfdh_set_pml_aux(self.h, _p(a)), "set_pml_aux")
+
+    def pml_aux(self):
+        a = np.empty(max(self.aux_len(), 1), np.float32)
+        check(lib().cgfdh_get_pml_aux(self.h, _p(a)), "get_pml_aux")
+        return a[: self.aux_len()]
+
+    # ---- time stepping ----
+    def run(self, it0, nsteps):
+        check(lib().cgfdh_run(self.h, it0, nsteps), "run")
+
+    def seismograms(self):
+        s = np.empty((max(self.nrecv, 1), 9, self.nt_total), np.float32)
+        check(lib().cgfdh_get_seismograms(self.h, _p(s)), "get_seismograms")
+        return s[: self.nrecv]
+
+    def onestage(self, it, ipair, istage, w_cur, aux_cur=None):
+        w = _f32(w_cur)
+        rhs = np.empty(9 * self.vol, np.float32)
+        n = self.aux_len()
+        arhs = np.empty(max(n, 1), np.float32)
+        a = None if aux_cur is None else _f32(aux_cur)
+        check(lib().cgfdh_onestage(self.h, it, ipair, istage, _p(w),
+                                   None if a is None else _p(a), _p(rhs), _p(arhs)),
+              "onestage")
+        return rhs, arhs[:n]
+
+    def last_timing(self):
+        run_ms, st_ms, n = C.c_double(), C.c_double(), C.c_int()
+        check(lib().cgfdh_last_timing(self.h, C.byref(run_ms), C.byref(st_ms), C.byref(n)),
+              "last_timing")
+        return run_ms.value, st_ms.value, n.value
+
+    def set_kernel_variant(self, v):
+        check(lib().cgfdh_set_kernel_variant(self.h, v), "set_kernel_variant")

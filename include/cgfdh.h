@@ -1,0 +1,138 @@
+/*
+ * cgfdh.h -- C ABI of the MI355X (gfx950) time-stepping engine for the
+ * curvilinear collocated-grid elastic solver.
+ *
+ * The reference's drop-in boundary for this path is the C function
+ *   drv_rk_curv_col_allstep(fd_t*, gd_t*, gdcurv_metric_t*, md_t*, src_t*,
+ *                           bdry_t*, wav_t*, mympi_t*, iorecv_t*, ioline_t*,
+ *                           ioslice_t*, iosnap_t*, float dt, int nt_total,
+ *                           float t0, char*, char*, int, int, int)
+ *   (forward/drv_rk_curv_col.h:17-37, called from main_curv_col_el_3d.c:849-858)
+ * together with the one-stage RHS it dispatches to,
+ *   sv_curv_col_el_iso_onestage(...)  (forward/sv_curv_col_el_iso.h:17-32).
+ * This ABI carries the same data with plain pointers and sizes, so a C shim
+ * that unpacks those structs into these calls replaces the reference's time
+ * loop (the shim is shown in INTEGRATION.md).  No torch or HIP types appear.
+ *
+ * Conventions (all follow the reference):
+ *  - grids carry 3 ghost points per side: nx=ni+6, ny=nj+6, nz=nk+6;
+ *    iptr = i + j*nx + k*nx*ny, x fastest (gd_t.c:2890-2897);
+ *  - wavefield components Vx,Vy,Vz,Txx,Tyy,Tzz,Tyz,Txz,Txy (wav_t.c:93-136);
+ *  - metric components jac,xi_x,xi_y,xi_z,eta_x,eta_y,eta_z,zeta_x,zeta_y,
+ *    zeta_z (gd_t.c:101-180);
+ *  - the medium's density is passed as slowness 1/rho (md_t.c:602-623);
+ *  - PML faces are numbered idim*2+iside: x1,x2,y1,y2,z1,z2 (bdry_t.h);
+ *  - time step it uses the operator pair it % 8 (drv_rk_curv_col.c:175).
+ * Every function returns 0 on success or a negative CGFDH_E* code; the
+ * message of the last failure on the calling thread is cgfdh_last_error().
+ * The library is single-caller per solver, like the reference's driver.
+ */
+#ifndef CGFDH_H
+#define CGFDH_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CGFDH_OK 0
+#define CGFDH_EINVAL (-1)
+#define CGFDH_ENOMEM (-2)
+#define CGFDH_EDEVICE (-3)
+#define CGFDH_ESTATE (-4)
+
+/* medium codes are the reference's (constants.h) */
+#define CGFDH_MEDIUM_ELASTIC_ISO 2
+
+#define CGFDH_SRC_POINT 1     /* CONST_SRC_SPATIAL_POINT */
+#define CGFDH_SRC_GAUSSIAN 2  /* CONST_SRC_SPATIAL_GAUSSIAN */
+
+typedef struct cgfdh_solver cgfdh_solver;
+
+typedef struct {
+  int ni, nj, nk;       /* physical grid points of the block */
+  int medium_type;      /* CGFDH_MEDIUM_ELASTIC_ISO */
+  int free_top;         /* 1: traction-image free surface on z2 */
+  int pml_on[6];        /* CFS-PML per face */
+  int pml_layers[6];    /* number_of_layers per face */
+  float dt;             /* size_of_time_step */
+  int nt_total;         /* number_of_time_steps + 1 (main_curv_col_el_3d.c:615) */
+} cgfdh_desc;
+
+/* Replaces the allocations of wav_init / bdry_pml_auxvar_init / bdry_free_set
+ * for the device (wav_t.c:15-179, bdry_t.c:291-327, bdry_t.c:49-112). */
+int cgfdh_create(const cgfdh_desc *desc, cgfdh_solver **out);
+int cgfdh_destroy(cgfdh_solver *s);
+const char *cgfdh_last_error(void);
+
+/* gdcurv_metric_t v4d, 10 volumes in the order above (gd_t.c:98-185). */
+int cgfdh_set_metric(cgfdh_solver *s, const float *metric10);
+
+/* md_t lambda, mu and rho-as-slowness (md_t.c:90-110, 602-623). */
+int cgfdh_set_medium_iso(cgfdh_solver *s, const float *lambda, const float *mu,
+                         const float *slowness);
+
+/* bdry_t A/B/D of one face, num_layers+1 values each, already converted as
+ * bdry_pml_set leaves them (D/=B, A+=D, B=1/B; bdry_t.c:259-268). */
+int cgfdh_set_pml_face(cgfdh_solver *s, int face, const float *A, const float *B,
+                       const float *D);
+
+/* bdry_t matVx2Vz2 / matVy2Vz2 / matF2Vz2, [ny][nx][3][3] each
+ * (sv_curv_col_el_iso_dvh2dvz, sv_curv_col_el_iso.c:1259-1375). */
+int cgfdh_set_free_surface(cgfdh_solver *s, const float *matVx2Vz2,
+                           const float *matVy2Vz2, const float *matF2Vz2);
+
+/* Moment sources as src_t holds them (src_t.h; src_t.c:985-1174):
+ *  ijk      local indices with ghosts (si,sj,sk), 3 per source;
+ *  it_begin/it_end per source; max_nt;
+ *  mij      Mxx,Myy,Mzz,Mxz,Myz,Mxy, each [nsrc][max_nt][4 stages];
+ *  spatial  CGFDH_SRC_POINT or CGFDH_SRC_GAUSSIAN;
+ *  ext_half half width of the Gaussian (src->ext_half_npoint);
+ *  ext_coefs the normalised (2*ext_half+1)^3 weights per source, as
+ *           src_cal_norm_delt3d_z2fre leaves them (NULL for point sources). */
+int cgfdh_set_moment_sources(cgfdh_solver *s, int nsrc, const int *ijk,
+                             const int *it_begin, const int *it_end, int max_nt,
+                             const float *mij, int spatial, int ext_half,
+                             const float *ext_coefs);
+
+/* Receivers as iorecv_one_t keeps them: indx1d[8] and di,dj,dk
+ * (io_funcs.c:358-389); samples follow io_recv_keep (io_funcs.c:1585-1623). */
+int cgfdh_set_receivers(cgfdh_solver *s, int nrecv, const int *indx1d,
+                        const float *dijk);
+
+/* Whole-state access (level "pre": the state at the start of step it0). */
+int cgfdh_set_wavefield(cgfdh_solver *s, const float *w9);
+int cgfdh_get_wavefield(cgfdh_solver *s, float *w9);
+size_t cgfdh_pml_aux_len(cgfdh_solver *s);   /* floats, all enabled faces */
+int cgfdh_set_pml_aux(cgfdh_solver *s, const float *aux);
+int cgfdh_get_pml_aux(cgfdh_solver *s, float *aux);
+
+/* Advance steps it0 .. it0+nsteps-1 (drv_rk_curv_col.c:167-544): four fused
+ * stage launches per step, receivers sampled after every step. */
+int cgfdh_run(cgfdh_solver *s, int it0, int nsteps);
+
+/* seismo[(r*9 + icmp)*nt_total + it], the layout of iorecv->recvone[r].seismo */
+int cgfdh_get_seismograms(cgfdh_solver *s, float *seismo);
+
+/* One RHS evaluation without the RK combine (sv_curv_col_el_iso_onestage):
+ * rhs and aux_rhs receive what the reference leaves in w_rhs and the PML
+ * aux rhs level.  Host buffers; for parity checks and drop-in use. */
+int cgfdh_onestage(cgfdh_solver *s, int it, int ipair, int istage,
+                   const float *w_cur, const float *aux_cur, float *rhs,
+                   float *aux_rhs);
+
+/* Device time of the last cgfdh_run, and the summed duration of its stage
+ * launches, in milliseconds (HIP events on the solver's stream). */
+int cgfdh_last_timing(cgfdh_solver *s, double *run_ms, double *stage_ms,
+                      int *stage_launches);
+
+/* Selects the stage kernel: 0 = per-point reference kernel, 1 = tiled
+ * z-marching kernel (default). */
+int cgfdh_set_kernel_variant(cgfdh_solver *s, int variant);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

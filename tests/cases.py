@@ -197,3 +197,66 @@ def ref_cfg(case, src_arrays=None):
         c.src_max_nt = src["max_nt"]
         c.src_ijk, c.src_it_begin, c.src_it_end, c.src_mij = [x.ctypes.data for x in a]
     return c, keep
+
+
+def make_solver(case, dt, nt_total, variant=1):
+    """The product engine loaded with a case's static fields."""
+    from cgfd3d_amd import Solver, SRC_GAUSSIAN, SRC_POINT
+    s = Solver(case["ni"], case["nj"], case["nk"], dt, nt_total, free_top=bool(case["free_top"]),
+               pml_on=case["pml_on"], pml_layers=case["pml_layers"])
+    s.set_kernel_variant(variant)
+    s.set_metric(case["metric"])
+    med = case["med"]
+    s.set_medium_iso(med[0], med[1], med[2])
+    for f in range(6):
+        if case["pml_on"][f]:
+            s.set_pml_face(f, case["pml_A"][f], case["pml_B"][f], case["pml_D"][f])
+    if case["free_top"]:
+        s.set_free_surface(free_surface_mats(case))
+    src = case.get("src")
+    if src is not None and src["nsrc"] > 0:
+        if src.get("gauss"):
+            coefs = gauss_coefs(case)
+            s.set_moment_sources(src["ijk"], src["it_begin"], src["it_end"], src["max_nt"],
+                                 src["mij"], SRC_GAUSSIAN, src.get("ext_half", 3), coefs)
+        else:
+            s.set_moment_sources(src["ijk"], src["it_begin"], src["it_end"], src["max_nt"],
+                                 src["mij"], SRC_POINT)
+    return s
+
+
+def free_surface_mats(case):
+    orc = pyoracle.load_oracle()
+    nx, ny, _ = dims(case["ni"], case["nj"], case["nk"])
+    mats = np.zeros(3 * nx * ny * 9, np.float32)
+    orc.orc_dvh2dvz_iso(case["ni"], case["nj"], case["nk"],
+                        pyoracle.ptr(np.ascontiguousarray(case["metric"])),
+                        pyoracle.ptr(np.ascontiguousarray(case["med"])), pyoracle.ptr(mats))
+    return mats
+
+
+def gauss_coefs(case):
+    """Normalised Gaussian weights per source (src_t.c:2111-2151)."""
+    orc = pyoracle.load_oracle()
+    src = case["src"]
+    half = src.get("ext_half", 3)
+    L = 2 * half + 1
+    nk2 = 3 + case["nk"] - 1
+    out = np.zeros((src["nsrc"], L ** 3), np.float32)
+    ijk = np.array(src["ijk"]).reshape(-1, 3)
+    inc = np.array(src["inc"], np.float32).reshape(-1, 3)
+    for s in range(src["nsrc"]):
+        sk = ijk[s, 2]
+        kb = half if sk + half < nk2 else nk2 - sk
+        w = np.zeros(L ** 3, np.float32)
+        orc.orc_gauss_weights(pyoracle.ptr(w), float(inc[s, 0]), float(inc[s, 1]),
+                              float(inc[s, 2]), src.get("ext_coef", 1.5), half, kb)
+        out[s] = w
+    return out.reshape(-1)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
