@@ -1,0 +1,159 @@
+"""Benchmark of the time-stepping hot path on one MI355X.
+
+Workload (the roofline configuration): isotropic elastic 600x600x300
+curvilinear grid (~108 M points), dx = 50 m, topography from 20 seeded
+Gaussian hills, 3-layer medium times a seeded Gaussian random field, seeded
+double-couple source with a 2 Hz Ricker, CFS-PML 20 layers on 5 faces,
+traction-image free surface, 256 surface receivers, FP32.
+
+A step is one full RK4 step (4 fused stage launches + receiver sampling) over
+the whole grid; inputs are resident in HBM before timing starts.  Prints one
+JSON line: grid-point RK4 updates/s, the HBM roofline of the stage kernel
+(algorithmic bytes per launch / HIP-event launch time) and a CPU baseline (the
+oracle, OpenMP over the host cores, on a bounded sub-block of the same
+workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def measured_traffic(workload_name):
+    """HBM bytes per stage launch from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", f"stage_tiled_{workload_name}_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(wl, budget_s=12.0):
+    """The oracle on a sub-block cut from the same workload (top of the grid,
+    free surface, PML on 5 faces, the source inside), timed on the host cores."""
+    import numpy as np
+    from oracle import pyoracle as po
+    from tests import cases as cs
+
+    ni, nj, nk = 160, 160, 80
+    i0 = (wl.ni - ni) // 2
+    j0 = (wl.nj - nj) // 2
+    k0 = wl.nk - nk
+    sl = (slice(k0, k0 + nk + 6), slice(j0, j0 + nj + 6), slice(i0, i0 + ni + 6))
+    nx, ny, nz = ni + 6, nj + 6, nk + 6
+    xyz = np.ascontiguousarray(wl.xyz.reshape(3, wl.nk + 6, wl.nj + 6, wl.ni + 6)[(slice(None),) + sl])
+    shp = (wl.nk + 6, wl.nj + 6, wl.ni + 6)
+    med = np.stack([wl.lam.reshape(shp)[sl], wl.mu.reshape(shp)[sl], wl.slw.reshape(shp)[sl]])
+    orc = po.load_oracle()
+    metric = np.zeros((10, nz, ny, nx), np.float32)
+    orc.orc_metric(ni, nj, nk, po.ptr(xyz), po.ptr(metric))
+    layers = 10
+    pml_on = [1, 1, 1, 1, 1, 0]
+    A, B, D, region = cs.pml_tables(orc, ni, nj, nk, xyz, pml_on, [layers] * 5 + [0],
+                                    vel=wl.pml_vel, alpha=wl.pml_alpha, beta=wl.pml_beta)
+    case = dict(ni=ni, nj=nj, nk=nk, metric=metric, med=np.ascontiguousarray(med),
+                free_top=1, pml_on=pml_on, pml_layers=[layers] * 5 + [0],
+                pml_A=A, pml_B=B, pml_D=D, pml_region=region, src=None)
+    src = dict(wl.src)
+    src.update(nsrc=1, ijk=[3 + ni // 2, 3 + nj // 2, 3 + nk - 1 - 20], inc=[0, 0, 0], gauss=0)
+    case["src"] = src
+    o = po.Oracle(case)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    t = time.perf_counter()
+    w, aux, _ = o.run(wl.dt, 1)
+    t1 = time.perf_counter() - t
+    n = max(1, min(50, int(budget_s / max(t1, 1e-3))))
+    t = time.perf_counter()
+    o.run(wl.dt, n, w=w, aux=aux, it0=1)
+    tn = time.perf_counter() - t
+    value = ni * nj * nk * n / tn
+    return {"value": value, "unit": "grid-point RK4 updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle (C restatement, OpenMP {threads} threads) on a {ni}x{nj}x{nk} "
+                      f"sub-block of the same workload with free surface, 10-layer PML on 5 "
+                      f"faces and the source, {n} RK4 steps in {tn:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--variant", type=int, default=1, help="0 per-point, 1 tiled kernel")
+    ap.add_argument("--size", type=str, default="600x600x300")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
+              "the path shards over x/y blocks with per-stage halo exchange, which is "
+              "not part of this build", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from cgfd3d_amd import workloads
+
+    ni, nj, nk = (int(v) for v in args.size.split("x"))
+    t = time.perf_counter()
+    wl = workloads.iso_topo_grf(ni=ni, nj=nj, nk=nk, nt_total=max(2001, args.steps + args.warmup))
+    log(f"workload {wl.name}: {wl.npoints / 1e6:.1f} M points, aux {wl.n_aux() / 1e6:.2f} M, "
+        f"built in {time.perf_counter() - t:.1f} s")
+    t = time.perf_counter()
+    s = wl.build_solver(variant=args.variant)
+    log(f"engine set up in {time.perf_counter() - t:.1f} s")
+    s.run(0, args.warmup)
+    torch.cuda.synchronize()
+    log(f"warmup {args.warmup} steps done")
+    t0 = time.perf_counter()
+    s.run(args.warmup, args.steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    run_ms, stage_ms, nlaunch = s.last_timing()
+    seis = s.seismograms()
+    s.close()
+    log(f"timed {args.steps} steps: wall {wall:.3f} s, device {run_ms:.1f} ms, "
+        f"stage launches {nlaunch} totalling {stage_ms:.1f} ms")
+    import numpy as np
+    if not np.isfinite(seis).all():
+        raise RuntimeError("non-finite seismograms")
+    N = wl.npoints
+    value = N * args.steps / wall
+    bstep = wl.algorithmic_bytes_per_step()
+    per_launch = bstep / 4.0
+    achieved = per_launch / (stage_ms / nlaunch / 1e3) / 1e9
+    traffic = measured_traffic(wl.name)
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "algorithmic_bytes_per_launch": per_launch,
+            "kernel": "stage_tiled_kernel" if args.variant == 1 else "stage_point_kernel"}
+    cpu = None
+    if not args.no_cpu:
+        t = time.perf_counter()
+        cpu = cpu_baseline(wl)
+        log(f"cpu baseline {cpu['value']:.3e} updates/s ({time.perf_counter() - t:.1f} s)")
+    out = {
+        "metric": "grid-point RK4 updates/s", "value": value, "unit": "grid-point RK4 updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{wl.name} iso elastic, 20 Gaussian hills, 3-layer+GRF, "
+                               f"DC Ricker 2 Hz, CFS-PML 20 x5, free surface, 256 receivers",
+                   "grid": [wl.ni, wl.nj, wl.nk], "dt": wl.dt, "parallelism": "single GPU"},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -34,9 +34,16 @@ def hipcc():
 
 
 def build_engine(verbose=False):
-    srcs = [os.path.join(CSRC, "cgfdh.hip")]
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC] + srcs + [
-        "-o", LIB + ".tmp"]
+    inc = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
+    obj = os.path.join(CSRC, "host_setup.o")
+    # host tables: plain C, no FMA contraction (the reference's x86-64 rounding)
+    ccmd = ["gcc", "-O2", "-fPIC", "-std=gnu99", "-ffp-contract=off", "-c",
+            os.path.join(CSRC, "host_setup.c"), "-o", obj] + inc
+    if verbose:
+        print(" ".join(ccmd), flush=True)
+    subprocess.run(ccmd, check=True)
+    srcs = [os.path.join(CSRC, "cgfdh.hip"), "-x", "none", obj]
+    cmd = [hipcc()] + HIPCC_FLAGS + inc + srcs + ["-o", LIB + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -68,6 +68,7 @@ __global__ void recv_kernel(const float* __restrict__ w, long vol, int nrecv,
 }
 
 #include "stage_tiled.h"
+#include "setup_kernels.h"
 
 // ---------------------------------------------------------------------------
 // host side
@@ -612,5 +613,67 @@ extern "C" int cgfdh_onestage(cgfdh_solver* s, int it, int ipair, int istage, co
                         s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   if (aux_rhs) return aux_copy(s, 2, aux_rhs, false);
+  return CGFDH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// setup on the device
+// ---------------------------------------------------------------------------
+
+extern "C" int cgfdh_compute_metric(cgfdh_solver* s, const float* xyz)
+{
+  if (!s || !xyz) return fail(CGFDH_EINVAL, "cgfdh_compute_metric: null argument");
+  float* d = nullptr;
+  HIPCHK(hipMalloc((void**)&d, 3 * s->vol * sizeof(float)));
+  hipError_t e = hipMemcpyAsync(d, xyz, 3 * s->vol * sizeof(float), hipMemcpyHostToDevice,
+                                s->stream);
+  if (e == hipSuccess) {
+    dim3 blk(256);
+    hipLaunchKernelGGL(metric_kernel, dim3((s->d.ni + 255) / 256, s->d.nj, s->d.nk), blk, 0,
+                       s->stream, d, s->met, s->d.ni, s->d.nj, s->d.nk, s->nx, s->ss, s->vol);
+    hipLaunchKernelGGL(metric_mirror_kernel, dim3((s->ny + 255) / 256, s->nz, 6), blk, 0,
+                       s->stream, s->met, 0, s->nx, s->ny, s->nz, s->ni1, s->ni2, s->nj1,
+                       s->nj2, s->nk1, s->nk2, s->ss, s->vol);
+    hipLaunchKernelGGL(metric_mirror_kernel, dim3((s->nx + 255) / 256, s->nz, 6), blk, 0,
+                       s->stream, s->met, 1, s->nx, s->ny, s->nz, s->ni1, s->ni2, s->nj1,
+                       s->nj2, s->nk1, s->nk2, s->ss, s->vol);
+    hipLaunchKernelGGL(metric_mirror_kernel, dim3((s->nx + 255) / 256, s->ny, 6), blk, 0,
+                       s->stream, s->met, 2, s->nx, s->ny, s->nz, s->ni1, s->ni2, s->nj1,
+                       s->nj2, s->nk1, s->nk2, s->ss, s->vol);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  }
+  (void)hipFree(d);
+  if (e != hipSuccess)
+    return fail(CGFDH_EDEVICE, "cgfdh_compute_metric: %s", hipGetErrorString(e));
+  s->metric_set = true;
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_compute_free_surface(cgfdh_solver* s)
+{
+  if (!s) return fail(CGFDH_EINVAL, "cgfdh_compute_free_surface: null solver");
+  if (!s->metric_set || !s->medium_set)
+    return fail(CGFDH_ESTATE, "cgfdh_compute_free_surface: metric and medium first");
+  hipLaunchKernelGGL(dvh2dvz_iso_kernel, dim3((s->d.ni + 255) / 256, s->d.nj), dim3(256), 0,
+                     s->stream, s->met, s->med, s->med + s->vol, s->mats, s->d.ni, s->d.nj,
+                     s->nx, s->nk2, s->ss, s->vol);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->mats_set = true;
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_get_metric(cgfdh_solver* s, float* m)
+{
+  if (!s || !m) return fail(CGFDH_EINVAL, "cgfdh_get_metric: null argument");
+  HIPCHK(hipMemcpy(m, s->met, 10 * s->vol * sizeof(float), hipMemcpyDeviceToHost));
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_get_free_surface(cgfdh_solver* s, float* m)
+{
+  if (!s || !m) return fail(CGFDH_EINVAL, "cgfdh_get_free_surface: null argument");
+  HIPCHK(hipMemcpy(m, s->mats, 27 * s->ss * sizeof(float), hipMemcpyDeviceToHost));
   return CGFDH_OK;
 }

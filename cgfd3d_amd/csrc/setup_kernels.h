@@ -1,0 +1,190 @@
+// setup_kernels.h -- one-off device kernels that build the solver's static
+// inputs: the curvilinear metric from grid coordinates and the free-surface
+// velocity-gradient matrices.  Same float arithmetic as the reference's host
+// code, so the results are bit-identical to it.
+#pragma once
+
+namespace cgfdh {
+
+// centred 7-point operator of the metric, fd_t.c:161-171 (row 3)
+#define CGFDH_C0 ((float)-0.02084)
+#define CGFDH_C1 ((float)0.1667)
+#define CGFDH_C2 ((float)-0.7709)
+#define CGFDH_C3 ((float)0.0)
+#define CGFDH_C4 ((float)0.7709)
+#define CGFDH_C5 ((float)-0.1667)
+#define CGFDH_C6 ((float)0.02084)
+
+__device__ __forceinline__ float dc7(const float* __restrict__ v, long p, long s)
+{
+  float d = CGFDH_C0 * v[p - 3 * s];
+  d += CGFDH_C1 * v[p - 2 * s];
+  d += CGFDH_C2 * v[p - s];
+  d += CGFDH_C3 * v[p];
+  d += CGFDH_C4 * v[p + s];
+  d += CGFDH_C5 * v[p + 2 * s];
+  d += CGFDH_C6 * v[p + 3 * s];
+  return d;
+}
+
+// gd_curv_metric_cal, physical points (gd_t.c:191-289)
+__global__ void metric_kernel(const float* __restrict__ xyz, float* __restrict__ met, int ni,
+                              int nj, int nk, int nx, long ss, long vol)
+{
+  const int i = 3 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 3 + blockIdx.y;
+  const int k = 3 + blockIdx.z;
+  if (i > 3 + ni - 1) return;
+  const long p = (long)i + (long)j * nx + (long)k * ss;
+  const float* X = xyz;
+  const float* Y = xyz + vol;
+  const float* Z = xyz + 2 * vol;
+  float v1[3] = {dc7(X, p, 1), dc7(Y, p, 1), dc7(Z, p, 1)};
+  float v2[3] = {dc7(X, p, nx), dc7(Y, p, nx), dc7(Z, p, nx)};
+  float v3[3] = {dc7(X, p, ss), dc7(Y, p, ss), dc7(Z, p, ss)};
+  float g[3];
+  g[0] = v1[1] * v2[2] - v1[2] * v2[1];
+  g[1] = v1[2] * v2[0] - v1[0] * v2[2];
+  g[2] = v1[0] * v2[1] - v1[1] * v2[0];
+  float jac = 0.0f;
+  jac += g[0] * v3[0];
+  jac += g[1] * v3[1];
+  jac += g[2] * v3[2];
+  met[p] = jac;
+  g[0] = v2[1] * v3[2] - v2[2] * v3[1];
+  g[1] = v2[2] * v3[0] - v2[0] * v3[2];
+  g[2] = v2[0] * v3[1] - v2[1] * v3[0];
+  met[1 * vol + p] = g[0] / jac;
+  met[2 * vol + p] = g[1] / jac;
+  met[3 * vol + p] = g[2] / jac;
+  g[0] = v3[1] * v1[2] - v3[2] * v1[1];
+  g[1] = v3[2] * v1[0] - v3[0] * v1[2];
+  g[2] = v3[0] * v1[1] - v3[1] * v1[0];
+  met[4 * vol + p] = g[0] / jac;
+  met[5 * vol + p] = g[1] / jac;
+  met[6 * vol + p] = g[2] / jac;
+  g[0] = v1[1] * v2[2] - v1[2] * v2[1];
+  g[1] = v1[2] * v2[0] - v1[0] * v2[2];
+  g[2] = v1[0] * v2[1] - v1[1] * v2[0];
+  met[7 * vol + p] = g[0] / jac;
+  met[8 * vol + p] = g[1] / jac;
+  met[9 * vol + p] = g[2] / jac;
+}
+
+// ghost mirroring of the metric (gd_t.c:291-401): pass 0 mirrors x ghosts
+// for every (j,k), pass 1 y ghosts for every (i,k), pass 2 z ghosts for every
+// (i,j); run in that order as the reference does
+__global__ void metric_mirror_kernel(float* __restrict__ met, int pass, int nx, int ny, int nz,
+                                     int ni1, int ni2, int nj1, int nj2, int nk1, int nk2,
+                                     long ss, long vol)
+{
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;   // fastest free index
+  const int b = blockIdx.y;                              // slower free index
+  int g = blockIdx.z;                                    // ghost slot 0..5
+  long dst, src;
+  if (pass == 0) {
+    if (a >= ny) return;
+    const int i = g < 3 ? g : ni2 + 1 + (g - 3);
+    const int is = g < 3 ? 2 * ni1 - i - 1 : 2 * ni2 - i + 1;
+    dst = (long)i + (long)a * nx + (long)b * ss;
+    src = (long)is + (long)a * nx + (long)b * ss;
+  } else if (pass == 1) {
+    if (a >= nx) return;
+    const int j = g < 3 ? g : nj2 + 1 + (g - 3);
+    const int js = g < 3 ? 2 * nj1 - j - 1 : 2 * nj2 - j + 1;
+    dst = (long)a + (long)j * nx + (long)b * ss;
+    src = (long)a + (long)js * nx + (long)b * ss;
+  } else {
+    if (a >= nx) return;
+    const int k = g < 3 ? g : nk2 + 1 + (g - 3);
+    const int ks = g < 3 ? 2 * nk1 - k - 1 : 2 * nk2 - k + 1;
+    dst = (long)a + (long)b * nx + (long)k * ss;
+    src = (long)a + (long)b * nx + (long)ks * ss;
+  }
+  for (int m = 0; m < 10; m++) met[m * vol + dst] = met[m * vol + src];
+}
+
+__device__ __forceinline__ void inv3(float m[3][3])
+{
+  float inv[3][3];
+  inv[0][0] = m[1][1] * m[2][2] - m[2][1] * m[1][2];
+  inv[0][1] = m[2][1] * m[0][2] - m[0][1] * m[2][2];
+  inv[0][2] = m[0][1] * m[1][2] - m[0][2] * m[1][1];
+  inv[1][0] = m[1][2] * m[2][0] - m[1][0] * m[2][2];
+  inv[1][1] = m[0][0] * m[2][2] - m[2][0] * m[0][2];
+  inv[1][2] = m[1][0] * m[0][2] - m[0][0] * m[1][2];
+  inv[2][0] = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+  inv[2][1] = m[2][0] * m[0][1] - m[0][0] * m[2][1];
+  inv[2][2] = m[0][0] * m[1][1] - m[0][1] * m[1][0];
+  float det = inv[0][0] * m[0][0] + inv[0][1] * m[1][0] + inv[0][2] * m[2][0];
+  det = 1.0f / det;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) m[i][j] = inv[i][j] * det;
+}
+
+__device__ __forceinline__ void mul3(const float A[3][3], const float B[3][3], float C[3][3])
+{
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      float c = 0.0f;
+      for (int k = 0; k < 3; k++) c += A[i][k] * B[k][j];
+      C[i][j] = c;
+    }
+}
+
+// sv_curv_col_el_iso_dvh2dvz (sv_curv_col_el_iso.c:1259-1375) at k = nk2
+__global__ void dvh2dvz_iso_kernel(const float* __restrict__ met, const float* __restrict__ lam3,
+                                   const float* __restrict__ mu3, float* __restrict__ mats,
+                                   int ni, int nj, int nx, int nk2, long ss, long vol)
+{
+  const int i = 3 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 3 + blockIdx.y;
+  if (i > 3 + ni - 1) return;
+  const long p = (long)i + (long)j * nx + (long)nk2 * ss;
+  const float e11 = met[1 * vol + p], e12 = met[2 * vol + p], e13 = met[3 * vol + p];
+  const float e21 = met[4 * vol + p], e22 = met[5 * vol + p], e23 = met[6 * vol + p];
+  const float e31 = met[7 * vol + p], e32 = met[8 * vol + p], e33 = met[9 * vol + p];
+  const float lam = lam3[p], mu = mu3[p];
+  const float lam2mu = lam + 2.0f * mu;
+  float A[3][3], B[3][3], C[3][3], AB[3][3], AC[3][3];
+  A[0][0] = lam2mu * e31 * e31 + mu * (e32 * e32 + e33 * e33);
+  A[0][1] = lam * e31 * e32 + mu * e32 * e31;
+  A[0][2] = lam * e31 * e33 + mu * e33 * e31;
+  A[1][0] = lam * e32 * e31 + mu * e31 * e32;
+  A[1][1] = lam2mu * e32 * e32 + mu * (e31 * e31 + e33 * e33);
+  A[1][2] = lam * e32 * e33 + mu * e33 * e32;
+  A[2][0] = lam * e33 * e31 + mu * e31 * e33;
+  A[2][1] = lam * e33 * e32 + mu * e32 * e33;
+  A[2][2] = lam2mu * e33 * e33 + mu * (e31 * e31 + e32 * e32);
+  inv3(A);
+  B[0][0] = -lam2mu * e31 * e11 - mu * (e32 * e12 + e33 * e13);
+  B[0][1] = -lam * e31 * e12 - mu * e32 * e11;
+  B[0][2] = -lam * e31 * e13 - mu * e33 * e11;
+  B[1][0] = -lam * e32 * e11 - mu * e31 * e12;
+  B[1][1] = -lam2mu * e32 * e12 - mu * (e31 * e11 + e33 * e13);
+  B[1][2] = -lam * e32 * e13 - mu * e33 * e12;
+  B[2][0] = -lam * e33 * e11 - mu * e31 * e13;
+  B[2][1] = -lam * e33 * e12 - mu * e32 * e13;
+  B[2][2] = -lam2mu * e33 * e13 - mu * (e31 * e11 + e32 * e12);
+  C[0][0] = -lam2mu * e31 * e21 - mu * (e32 * e22 + e33 * e23);
+  C[0][1] = -lam * e31 * e22 - mu * e32 * e21;
+  C[0][2] = -lam * e31 * e23 - mu * e33 * e21;
+  C[1][0] = -lam * e32 * e21 - mu * e31 * e22;
+  C[1][1] = -lam2mu * e32 * e22 - mu * (e31 * e21 + e33 * e23);
+  C[1][2] = -lam * e32 * e23 - mu * e33 * e22;
+  C[2][0] = -lam * e33 * e21 - mu * e31 * e23;
+  C[2][1] = -lam * e33 * e22 - mu * e32 * e23;
+  C[2][2] = -lam2mu * e33 * e23 - mu * (e31 * e21 + e32 * e22);
+  mul3(A, B, AB);
+  mul3(A, C, AC);
+  const long ij = ((long)j * nx + i) * 9;
+  const long n = 9 * ss;
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) {
+      mats[ij + r * 3 + c] = AB[r][c];
+      mats[n + ij + r * 3 + c] = AC[r][c];
+      mats[2 * n + ij + r * 3 + c] = A[r][c];
+    }
+}
+
+}  // namespace cgfdh

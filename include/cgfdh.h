@@ -131,6 +131,48 @@ int cgfdh_last_timing(cgfdh_solver *s, double *run_ms, double *stage_ms,
  * z-marching kernel (default). */
 int cgfdh_set_kernel_variant(cgfdh_solver *s, int variant);
 
+/* ---- setup on the device ------------------------------------------------ */
+
+/* Metric from grid coordinates x3d,y3d,z3d (3 volumes with ghosts), as
+ * gd_curv_metric_cal computes it, ghost mirroring included (gd_t.c:191-402). */
+int cgfdh_compute_metric(cgfdh_solver *s, const float *xyz);
+
+/* Free-surface matrices from the device metric and medium, as
+ * sv_curv_col_el_iso_dvh2dvz (sv_curv_col_el_iso.c:1259-1375), which the
+ * reference runs at the start of its time loop (drv_rk_curv_col.c:132-159). */
+int cgfdh_compute_free_surface(cgfdh_solver *s);
+
+int cgfdh_get_metric(cgfdh_solver *s, float *metric10);
+int cgfdh_get_free_surface(cgfdh_solver *s, float *mats /* 3 x [ny][nx][3][3] */);
+
+/* ---- host table builders ------------------------------------------------ */
+
+/* A/B/D of one PML face from the grid coordinates: bdry_pml_set with
+ * bdry_cal_abl_len_dh and bdry_pml_cal_{R,dmax,d,a,b} (bdry_t.c:115-506). */
+int cgfdh_pml_face_coefs(int ni, int nj, int nk, const float *xyz, int face, int num_layers,
+                         float velocity, float alpha_max, float beta_max,
+                         float *A, float *B, float *D);
+
+/* fun_ricker (src_t.c:2159-2166). */
+float cgfdh_ricker(float t, float fc, float t0);
+
+/* angle2moment + src_muDA_to_moment (src_t.c:2344-2409); m in the .src order
+ * Mxx,Myy,Mzz,Myz,Mxz,Mxy. */
+void cgfdh_moment_from_angles(float strike, float dip, float rake, float mu, float D,
+                              float A, float *m);
+
+/* Ricker moment-rate table of one source (src_t.c:550-555, 1029-1091):
+ * fills it_begin/it_end/max_nt and, if out != NULL, 6 arrays (engine order
+ * Mxx,Myy,Mzz,Mxz,Myz,Mxy) of [max_nt][4] at stride out_stride floats. */
+int cgfdh_ricker_moment_table(float t0, float dt, float stf_length, float wavelet_tstart,
+                              float fc, float tpeak, const float *m6, int *it_begin,
+                              int *it_end, int *max_nt, float *out, long out_stride);
+
+/* Normalised Gaussian source weights (src_cal_norm_delt3d_z2fre,
+ * src_t.c:2111-2151): (2*half+1)^3 values, normalised over k <= half_below. */
+int cgfdh_gauss_weights(float *w, float x0, float y0, float z0, float a, int half,
+                        int half_below);
+
 #ifdef __cplusplus
 }
 #endif

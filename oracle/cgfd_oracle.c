@@ -529,6 +529,7 @@ static void rhs_timg(orc_t *o, const real_t *w, real_t *h,
   static const int trow[3][3] = {{TXX, TXY, TXZ}, {TXY, TYY, TYZ}, {TXZ, TYZ, TZZ}};
   for (int k = k_min; k <= o->nk2; k++) {
     int n_free = o->nk2 - k - zidx[0];
+#pragma omp parallel for schedule(static)
     for (int j = o->nj1; j <= o->nj2; j++)
       for (int i = o->ni1; i <= o->ni2; i++) {
         size_t p = i + j * o->sl + k * o->ss;
@@ -605,6 +606,7 @@ static void rhs_vlow(orc_t *o, const real_t *w, real_t *h,
   const real_t *mVx = o->mats, *mVy = o->mats + o->ss * 9;
   for (int n = 0; n < 3; n++) {
     int k = o->nk2 - n;
+#pragma omp parallel for schedule(static)
     for (int j = o->nj1; j <= o->nj2; j++)
       for (int i = o->ni1; i <= o->ni2; i++) {
         size_t p = i + j * o->sl + k * o->ss;
@@ -646,10 +648,12 @@ static void rhs_pml(orc_t *o, const real_t *w, real_t *h, const real_t *aux,
     size_t na = o->aux_n[f];
     const real_t *a = aux + o->aux_off[f];
     real_t *ah = aux_h + o->aux_off[f];
-    size_t ia = 0;
+    size_t fnx = r[1] - r[0] + 1, fny = r[3] - r[2] + 1;
+#pragma omp parallel for schedule(static)
     for (int k = r[4]; k <= r[5]; k++)
       for (int j = r[2]; j <= r[3]; j++)
-        for (int i = r[0]; i <= r[1]; i++, ia++) {
+        for (int i = r[0]; i <= r[1]; i++) {
+          size_t ia = (i - r[0]) + (j - r[2]) * fnx + (k - r[4]) * fnx * fny;
           size_t p = i + j * o->sl + k * o->ss;
           int ic = idim == 0 ? i - r[0] : (idim == 1 ? j - r[2] : k - r[4]);
           real_t cD = o->pD[f][ic], cA = o->pA[f][ic], cB = o->pB[f][ic];
@@ -804,7 +808,8 @@ void orc_run(orc_t *o, float dt, int it0, int nt, real_t *w, real_t *aux,
         if (s == 0) end[p] = pre[p] + cb * rhs[p];
         else end[p] += cb * rhs[p];
       }
-      for (size_t p = 0; p < LA; p++) {
+#pragma omp parallel for schedule(static)
+      for (long p = 0; p < (long)LA; p++) {
         if (s < 3) atmp[p] = apre[p] + ca * arhs[p];
         if (s == 0) aend[p] = apre[p] + cb * arhs[p];
         else aend[p] += cb * arhs[p];
