@@ -68,6 +68,7 @@ __global__ void recv_kernel(const float* __restrict__ w, long vol, int nrecv,
 }
 
 #include "stage_tiled.h"
+#include "stage_interior.h"
 #include "setup_kernels.h"
 
 // ---------------------------------------------------------------------------
@@ -126,7 +127,7 @@ struct cgfdh_solver {
   int* recv_idx = nullptr;
   float* recv_w = nullptr;
   float* seismo = nullptr;
-  int variant = 1;
+  int variant = 2;
   bool metric_set = false, medium_set = false, mats_set = false;
   hipEvent_t ev_run[2];
   std::vector<hipEvent_t> ev_stage;
@@ -434,7 +435,7 @@ extern "C" int cgfdh_get_seismograms(cgfdh_solver* s, float* out)
 
 extern "C" int cgfdh_set_kernel_variant(cgfdh_solver* s, int v)
 {
-  if (!s || v < 0 || v > 1) return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: %d", v);
+  if (!s || v < 0 || v > 3) return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: %d", v);
   s->variant = v;
   return CGFDH_OK;
 }
@@ -502,8 +503,38 @@ static void launch_kind(cgfdh_solver* s, const StageParams& P, int dx, int dy, i
       CGFDH_PT(4, 1, 0, 0) CGFDH_PT(5, 1, 0, 1) CGFDH_PT(6, 1, 1, 0) CGFDH_PT(7, 1, 1, 1)
     }
 #undef CGFDH_PT
-  } else {
+  } else if (s->variant == 1) {
     launch_tiled<KIND>(s->stream, P, s->d.ni, s->d.nj, s->d.nk, code);
+  } else {
+    // interior box (no PML face, no free-surface row) on the lean tiled
+    // kernel; the shell around it on the general per-point kernel
+    const cgfdh_desc& d = s->d;
+    const int ilo = d.pml_on[0] ? s->ni1 + d.pml_layers[0] + 1 : s->ni1;
+    const int ihi = d.pml_on[1] ? s->ni2 - d.pml_layers[1] - 1 : s->ni2;
+    const int jlo = d.pml_on[2] ? s->nj1 + d.pml_layers[2] + 1 : s->nj1;
+    const int jhi = d.pml_on[3] ? s->nj2 - d.pml_layers[3] - 1 : s->nj2;
+    const int klo = d.pml_on[4] ? s->nk1 + d.pml_layers[4] + 1 : s->nk1;
+    const int khi = d.free_top ? s->nk2 - 4 : (d.pml_on[5] ? s->nk2 - d.pml_layers[5] - 1 : s->nk2);
+    auto box = [](int i0, int i1, int j0, int j1, int k0, int k1) {
+      BoxParams b;
+      memset(&b, 0, sizeof b);
+      b.i0 = i0; b.j0 = j0; b.k0 = k0;
+      b.ni = i1 - i0 + 1; b.nj = j1 - j0 + 1; b.nk = k1 - k0 + 1;
+      return b;
+    };
+    const BoxParams in = box(ilo, ihi, jlo, jhi, klo, khi);
+    if (in.ni > 0 && in.nj > 0 && in.nk > 0) {
+      if (s->variant == 3)
+        launch_interior_ty<KIND, 16>(s->stream, P, in, code);
+      else
+        launch_interior_ty<KIND, 8>(s->stream, P, in, code);
+    }
+    launch_box<KIND>(s->stream, P, box(s->ni1, s->ni2, s->nj1, s->nj2, khi + 1, s->nk2), code);
+    launch_box<KIND>(s->stream, P, box(s->ni1, s->ni2, s->nj1, s->nj2, s->nk1, klo - 1), code);
+    launch_box<KIND>(s->stream, P, box(s->ni1, ilo - 1, s->nj1, s->nj2, klo, khi), code);
+    launch_box<KIND>(s->stream, P, box(ihi + 1, s->ni2, s->nj1, s->nj2, klo, khi), code);
+    launch_box<KIND>(s->stream, P, box(ilo, ihi, s->nj1, jlo - 1, klo, khi), code);
+    launch_box<KIND>(s->stream, P, box(ilo, ihi, jhi + 1, s->nj2, klo, khi), code);
   }
 }
 

@@ -127,8 +127,10 @@ int cgfdh_onestage(cgfdh_solver *s, int it, int ipair, int istage,
 int cgfdh_last_timing(cgfdh_solver *s, double *run_ms, double *stage_ms,
                       int *stage_launches);
 
-/* Selects the stage kernel: 0 = per-point reference kernel, 1 = tiled
- * z-marching kernel (default). */
+/* Selects the stage kernels: 0 = per-point kernel everywhere, 1 = general
+ * tiled z-marching kernel everywhere, 2 (default) / 3 = lean tiled kernel on
+ * the interior box (64x8 / 64x16 tiles) with the per-point kernel on the
+ * PML and free-surface shell.  All variants produce identical bits. */
 int cgfdh_set_kernel_variant(cgfdh_solver *s, int variant);
 
 /* ---- setup on the device ------------------------------------------------ */

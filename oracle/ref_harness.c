@@ -350,7 +350,9 @@ int refh_pml(const refh_cfg *cfg, float *coords, int maxlen,
 int refh_dvh2dvz_iso(const refh_cfg *cfg, float *metric10, float *med3, float *mats_out)
 {
   refh_ctx c;
-  setup(&c, cfg, NULL, metric10, med3);
+  refh_cfg nopml = *cfg;   /* no coordinates here: the PML set-up needs them */
+  memset(nopml.pml_on, 0, sizeof nopml.pml_on);
+  setup(&c, &nopml, NULL, metric10, med3);
   sv_curv_col_el_iso_dvh2dvz(&c.gd, &c.metric, &c.md, &c.bdry, 0);
   size_t n = c.gd.siz_iz * 9;
   memcpy(mats_out, c.bdry.matVx2Vz2, n * sizeof(float));

@@ -1,0 +1,120 @@
+"""Writes the golden fixtures under tests/golden/ from the reference's own
+hot-path code (oracle/_ref/libcgfd3d_ref.so, built from /root/reference by
+oracle/Makefile).  Run in the development container, where the reference
+tree exists:
+
+    python tests/golden/make_golden.py
+
+Each fixture holds inputs' checksums (the inputs themselves are regenerated
+from seeds by tests/cases.py) and the reference outputs:
+  onestage_<name>.npz  one RHS evaluation (rhs and PML aux rhs) for every
+                       operator pair and stage on a small curvilinear grid
+  run_<name>.npz       seismograms of surface and buried receivers and the
+                       final surface plane after an RK4 run with a point
+                       moment source, CFS-PML and the free surface
+The reference's surface row of hV is re-evaluated by the harness on stages
+with the forward z operator (the reference reads out of bounds there; see
+oracle/ref_harness.c).
+"""
+import ctypes as C
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import pyoracle as po  # noqa: E402
+from tests import cases as cs  # noqa: E402
+
+ONESTAGE_CASES = {
+    "free_pml": dict(ni=10, nj=9, nk=8, seed=21, free_top=True, pml=(1, 1, 1, 1, 1, 0), layers=3),
+    "pml6": dict(ni=9, nj=8, nk=8, seed=22, free_top=False, pml=(1, 1, 1, 1, 1, 1), layers=3),
+}
+
+RUN_CASES = {
+    "free_pml_src": dict(case=dict(ni=22, nj=20, nk=18, seed=31, free_top=True,
+                                   pml=(1, 1, 1, 1, 1, 0), layers=5),
+                         nt=150, dt=0.004, fc=9.0, t0=0.12),
+}
+
+
+def checksum(*arrays):
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def input_checksum(case):
+    return checksum(case["metric"], case["med"],
+                    *[a for a in case["pml_A"] if a is not None],
+                    *[a for a in case["pml_B"] if a is not None],
+                    *[a for a in case["pml_D"] if a is not None])
+
+
+def run_receivers(case):
+    ridx, rw = cs.surface_receivers(case, n=6, seed=3)
+    bi, bw = cs.buried_receiver(case, 8, 9, 7, 0.3, 0.55, 0.4)
+    return np.concatenate([ridx, bi]), np.concatenate([rw, bw])
+
+
+def main():
+    if not po.ref_available():
+        sys.exit("oracle/_ref/libcgfd3d_ref.so not built (needs the reference tree)")
+    ref = po.load_ref()
+    for name, kw in ONESTAGE_CASES.items():
+        case = cs.make_case(**kw)
+        w = cs.random_wavefield(case, seed=kw["seed"])
+        aux = cs.random_aux(case, seed=kw["seed"])
+        cfg, keep = cs.ref_cfg(case)
+        co = np.ascontiguousarray(case["coords"])
+        rhs = np.zeros((8, 4, w.size), np.float32)
+        arhs = np.zeros((8, 4, max(aux.size, 1)), np.float32)
+        for ipair in range(8):
+            for istage in range(4):
+                r = np.zeros(w.size, np.float32)
+                a = np.zeros(max(aux.size, 1), np.float32)
+                ref.refh_iso_onestage(C.byref(cfg), po.ptr(co), po.ptr(case["metric"]),
+                                      po.ptr(case["med"]), 4, ipair, istage, 1, po.ptr(w),
+                                      po.ptr(aux), po.ptr(r), po.ptr(a))
+                rhs[ipair, istage] = r
+                arhs[ipair, istage] = a
+        # full values for two evaluations (forward and backward z operator),
+        # SHA-256 of the bytes of all 32 (pair, stage) evaluations
+        sha = np.array([[checksum(rhs[p, s], arhs[p, s, : aux.size]) for s in range(4)]
+                        for p in range(8)])
+        np.savez_compressed(os.path.join(HERE, f"onestage_{name}.npz"),
+                            rhs_p0s0=rhs[0, 0], aux_rhs_p0s0=arhs[0, 0, : aux.size],
+                            rhs_p2s0=rhs[2, 0], aux_rhs_p2s0=arhs[2, 0, : aux.size],
+                            sha256=sha, input_sha256=input_checksum(case),
+                            w_sha256=checksum(w), aux_sha256=checksum(aux))
+        print("wrote", name)
+    for name, spec in RUN_CASES.items():
+        case = cs.make_case(**spec["case"])
+        ni, nj, nk = case["ni"], case["nj"], case["nk"]
+        nt, dt = spec["nt"], spec["dt"]
+        case["src"] = cs.ricker_point_source(case, 3 + ni // 2, 3 + nj // 2, 3 + nk // 2, nt, dt,
+                                             fc=spec["fc"], t0=spec["t0"])
+        ridx, rw = run_receivers(case)
+        cfg, keep = cs.ref_cfg(case)
+        co = np.ascontiguousarray(case["coords"])
+        nx, ny, nz = ni + 6, nj + 6, nk + 6
+        wfin = np.zeros(9 * nx * ny * nz, np.float32)
+        seis = np.zeros((len(ridx), 9, nt), np.float32)
+        ref.refh_iso_run(C.byref(cfg), po.ptr(co), po.ptr(case["metric"]), po.ptr(case["med"]),
+                         dt, nt, None, len(ridx), po.ptr(ridx), po.ptr(rw), po.ptr(wfin),
+                         po.ptr(seis))
+        W = wfin.reshape(9, nz, ny, nx)
+        np.savez_compressed(os.path.join(HERE, f"run_{name}.npz"), seismo=seis,
+                            surface=W[:, 3 + nk - 1, 3:3 + nj, 3:3 + ni],
+                            wave_sha256=checksum(wfin), input_sha256=input_checksum(case),
+                            src_sha256=checksum(case["src"]["mij"]))
+        print("wrote", name)
+
+
+if __name__ == "__main__":
+    main()

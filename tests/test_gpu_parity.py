@@ -19,7 +19,7 @@ CONFIGS = [
 ]
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 @pytest.mark.parametrize("name,free,pml", CONFIGS)
 def test_onestage_all_pairs(name, free, pml, variant):
     case = cs.make_case(18, 15, 13, seed=2, free_top=free, pml=pml, layers=4)
@@ -43,7 +43,7 @@ def test_onestage_all_pairs(name, free, pml, variant):
     assert mism == 0, "expected bit-identical RHS"
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 @pytest.mark.parametrize("gauss", [False, True])
 def test_run_matches_oracle(variant, gauss):
     ni, nj, nk, nt, dt = 26, 22, 20, 60, 0.004
