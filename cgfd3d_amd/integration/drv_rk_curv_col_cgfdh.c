@@ -1,0 +1,137 @@
+/*
+ * drv_rk_curv_col_cgfdh.c -- drop-in replacement of drv_rk_curv_col_allstep
+ * (forward/drv_rk_curv_col.h:17-37) for the reference build: the same C
+ * signature, the reference's structs unpacked into the engine's C ABI
+ * (include/cgfdh.h).  Compile it into the reference's main program in place
+ * of forward/drv_rk_curv_col.c and link cgfd3d_amd/libcgfdh.so.
+ *
+ * Covered: isotropic elastic medium, CFS-PML faces, traction-image free
+ * surface, point and Gaussian moment sources, receivers.  Anything else the
+ * par file asks for (other media, force sources, dd sources, receiver lines,
+ * slices, snapshots, ablexp, several MPI ranks) stops the run with a message,
+ * the reference's fatal-error convention.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <mpi.h>
+
+#include "drv_rk_curv_col.h"
+#include "cgfdh.h"
+
+static void cgfdh_fatal(const char *what, int rc)
+{
+  fprintf(stderr, "ERROR: %s failed (%d): %s\n", what, rc, cgfdh_last_error());
+  MPI_Abort(MPI_COMM_WORLD, 1);
+}
+
+#define CHK(call) do { int rc_ = (call); if (rc_) cgfdh_fatal(#call, rc_); } while (0)
+
+void
+drv_rk_curv_col_allstep(
+  fd_t            *fd,
+  gd_t        *gd,
+  gdcurv_metric_t *metric,
+  md_t      *md,
+  src_t     *src,
+  bdry_t    *bdry,
+  wav_t     *wav,
+  mympi_t    *mympi,
+  iorecv_t   *iorecv,
+  ioline_t   *ioline,
+  ioslice_t  *ioslice,
+  iosnap_t   *iosnap,
+  float dt, int nt_total, float t0,
+  char *output_fname_part,
+  char *output_dir,
+  int qc_check_nan_num_of_step,
+  const int output_all,
+  const int verbose)
+{
+  for (int n = 0; n < 4; n++)
+    if (mympi->neighid[n] != MPI_PROC_NULL) {
+      fprintf(stderr, "ERROR: the GPU time loop runs one block (1 MPI rank)\n");
+      MPI_Abort(MPI_COMM_WORLD, 1);
+    }
+  if (md->medium_type != CONST_MEDIUM_ELASTIC_ISO || src->force_actived ||
+      src->dd_is_valid || ioline->num_of_lines > 0 || bdry->is_enable_ablexp ||
+      ioslice->num_of_slice_x + ioslice->num_of_slice_y + ioslice->num_of_slice_z > 0 ||
+      iosnap->num_of_snap > 0 || output_all) {
+    fprintf(stderr, "ERROR: par file asks for a feature the GPU time loop lacks\n");
+    MPI_Abort(MPI_COMM_WORLD, 1);
+  }
+
+  cgfdh_desc d;
+  memset(&d, 0, sizeof d);
+  d.ni = gd->ni; d.nj = gd->nj; d.nk = gd->nk;
+  d.medium_type = CGFDH_MEDIUM_ELASTIC_ISO;
+  d.free_top = bdry->is_sides_free[2][1];
+  for (int f = 0; f < 6; f++) {
+    d.pml_on[f] = bdry->is_sides_pml[f / 2][f % 2];
+    d.pml_layers[f] = d.pml_on[f] ? bdry->num_of_layers[f / 2][f % 2] : 0;
+  }
+  d.dt = dt;
+  d.nt_total = nt_total;
+
+  cgfdh_solver *s = NULL;
+  CHK(cgfdh_create(&d, &s));
+  CHK(cgfdh_set_metric(s, metric->v4d));
+  CHK(cgfdh_set_medium_iso(s, md->lambda, md->mu, md->rho));  /* rho already 1/rho */
+  for (int f = 0; f < 6; f++)
+    if (d.pml_on[f])
+      CHK(cgfdh_set_pml_face(s, f, bdry->A[f / 2][f % 2], bdry->B[f / 2][f % 2],
+                             bdry->D[f / 2][f % 2]));
+  if (d.free_top) CHK(cgfdh_compute_free_surface(s));   /* drv_rk_curv_col.c:132-159 */
+
+  int ns = src->total_number;
+  if (ns > 0 && src->moment_actived) {
+    size_t per = (size_t)ns * src->max_nt * src->max_stage;
+    int gauss = src->itype_spatial_ext != CONST_SRC_SPATIAL_POINT;
+    int half = src->ext_half_npoint, len = 2 * half + 1;
+    int *ijk = malloc(3 * ns * sizeof(int));
+    float *mij = malloc(6 * per * sizeof(float));
+    float *w = gauss ? malloc((size_t)ns * len * len * len * sizeof(float)) : NULL;
+    memcpy(mij + 0 * per, src->Mxx, per * sizeof(float));
+    memcpy(mij + 1 * per, src->Myy, per * sizeof(float));
+    memcpy(mij + 2 * per, src->Mzz, per * sizeof(float));
+    memcpy(mij + 3 * per, src->Mxz, per * sizeof(float));
+    memcpy(mij + 4 * per, src->Myz, per * sizeof(float));
+    memcpy(mij + 5 * per, src->Mxy, per * sizeof(float));
+    for (int is = 0; is < ns; is++) {
+      ijk[3 * is] = src->si[is]; ijk[3 * is + 1] = src->sj[is]; ijk[3 * is + 2] = src->sk[is];
+      if (gauss) {   /* the weights sv_curv_col_el_rhs_src computes per stage */
+        int kb = src->sk[is] + half < gd->nk2 ? half : gd->nk2 - src->sk[is];
+        CHK(cgfdh_gauss_weights(w + (size_t)is * len * len * len, src->si_inc[is],
+                                src->sj_inc[is], src->sk_inc[is], src->ext_func_coef, half, kb));
+      }
+    }
+    CHK(cgfdh_set_moment_sources(s, ns, ijk, src->it_begin, src->it_end, src->max_nt, mij,
+                                 gauss ? CGFDH_SRC_GAUSSIAN : CGFDH_SRC_POINT, half, w));
+    free(ijk); free(mij); free(w);
+  }
+
+  int nr = iorecv->total_number;
+  if (nr > 0) {
+    int *idx = malloc(8 * nr * sizeof(int));
+    float *dijk = malloc(3 * nr * sizeof(float));
+    for (int r = 0; r < nr; r++) {
+      iorecv_one_t *o = iorecv->recvone + r;
+      memcpy(idx + 8 * r, o->indx1d, 8 * sizeof(int));
+      dijk[3 * r] = o->di; dijk[3 * r + 1] = o->dj; dijk[3 * r + 2] = o->dk;
+    }
+    CHK(cgfdh_set_receivers(s, nr, idx, dijk));
+    free(idx); free(dijk);
+  }
+
+  CHK(cgfdh_run(s, 0, nt_total));
+
+  if (nr > 0) {   /* iorecv->recvone[r].seismo[icmp*max_nt + it] */
+    float *seis = malloc((size_t)nr * 9 * nt_total * sizeof(float));
+    CHK(cgfdh_get_seismograms(s, seis));
+    for (int r = 0; r < nr; r++)
+      memcpy(iorecv->recvone[r].seismo, seis + (size_t)r * 9 * nt_total,
+             (size_t)9 * nt_total * sizeof(float));
+    free(seis);
+  }
+  CHK(cgfdh_destroy(s));
+}

@@ -129,3 +129,16 @@ def test_ricker_moment_table_matches_reference_layout():
     # engine order Mxx,Myy,Mzz,Mxz,Myz,Mxy from the .src order Mxx,Myy,Mzz,Myz,Mxz,Mxy
     for m, src_slot in enumerate([0, 1, 2, 4, 3, 5]):
         assert np.array_equal(table[m], (ref[m] * m6[src_slot]).astype(np.float32))
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference/forward"), reason="reference headers absent")
+def test_reference_shim_compiles():
+    """The drop-in drv_rk_curv_col_allstep compiles against the reference's
+    own headers (the structs it unpacks)."""
+    import subprocess
+    shim = os.path.join(ROOT, "cgfd3d_amd", "integration", "drv_rk_curv_col_cgfdh.c")
+    r = subprocess.run(["gcc", "-fsyntax-only", "-std=gnu99", "-Wall", "-Werror",
+                        "-I/opt/conda/include", "-I/root/reference/lib",
+                        "-I/root/reference/forward", "-I" + os.path.join(ROOT, "include"), shim],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
