@@ -148,7 +148,7 @@ def main():
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "algorithmic_bytes_per_launch": per_launch,
-            "kernel": "stage_tiled_kernel" if args.variant == 1 else "stage_point_kernel"}
+            "kernel": ["stage_point_kernel", "stage_tiled_kernel", "stage_interior_kernel<TY=8> + stage_box_kernel", "stage_interior_kernel<TY=16> + stage_box_kernel", "stage_interior_point_kernel<BY=4> + stage_box_kernel", "stage_interior_point_kernel<BY=2> + stage_box_kernel"][args.variant]}
     cpu = None
     if not args.no_cpu:
         t = time.perf_counter()

@@ -67,6 +67,43 @@ __global__ void recv_kernel(const float* __restrict__ w, long vol, int nrecv,
   seismo[((long)r * 9 + c) * nt_total + it] = s;
 }
 
+// peak ground motion on the free surface after a step (PG_calcu,
+// wav_t.c:380-455): maps PGV,PGVh,PGVx,PGVy,PGVz, PGA.., PGD.. and the
+// trapezoid displacement, from w_end and the state before the step
+__global__ void pg_kernel(const float* __restrict__ w1, const float* __restrict__ w0,
+                          float* __restrict__ PG, float* __restrict__ Dis, int ni, int nj,
+                          int nx, long ss, long vol, int nk2, float dt)
+{
+  const int i = 3 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 3 + blockIdx.y;
+  if (i > 3 + ni - 1) return;
+  const long q = (long)i + (long)j * nx;
+  const long p = q + (long)nk2 * ss;
+  const float vx1 = w1[p], vy1 = w1[vol + p], vz1 = w1[2 * vol + p];
+  const float vx0 = w0[p], vy0 = w0[vol + p], vz0 = w0[2 * vol + p];
+  const float Ax = fabsf((vx1 - vx0) / dt);
+  const float Ay = fabsf((vy1 - vy0) / dt);
+  const float Az = fabsf((vz1 - vz0) / dt);
+  float* Dx = Dis;
+  float* Dy = Dis + ss;
+  float* Dz = Dis + 2 * ss;
+  const float dx = (float)((double)Dx[q] + 0.5 * (double)(vx1 + vx0) * (double)dt);
+  const float dy = (float)((double)Dy[q] + 0.5 * (double)(vy1 + vy0) * (double)dt);
+  const float dz = (float)((double)Dz[q] + 0.5 * (double)(vz1 + vz0) * (double)dt);
+  Dx[q] = dx; Dy[q] = dy; Dz[q] = dz;
+  const float V = (float)sqrt((double)(vx1 * vx1 + vy1 * vy1 + vz1 * vz1));
+  const float Vh = (float)sqrt((double)(vx1 * vx1 + vy1 * vy1));
+  const float A = (float)sqrt((double)(Ax * Ax + Ay * Ay + Az * Az));
+  const float Ah = (float)sqrt((double)(Ax * Ax + Ay * Ay));
+  const float D = (float)sqrt((double)(dx * dx + dy * dy + dz * dz));
+  const float Dh = (float)sqrt((double)(dx * dx + dy * dy));
+  const float val[15] = {V, Vh, fabsf(vx1), fabsf(vy1), fabsf(vz1), A, Ah, Ax, Ay, Az,
+                         D, Dh, fabsf(dx), fabsf(dy), fabsf(dz)};
+#pragma unroll
+  for (int m = 0; m < 15; m++)
+    if (PG[m * ss + q] < val[m]) PG[m * ss + q] = val[m];
+}
+
 #include "stage_tiled.h"
 #include "stage_interior.h"
 #include "setup_kernels.h"
@@ -127,6 +164,7 @@ struct cgfdh_solver {
   int* recv_idx = nullptr;
   float* recv_w = nullptr;
   float* seismo = nullptr;
+  float* pg = nullptr;           // 15 peak maps + 3 displacement maps, [ny][nx] each
   int variant = 2;
   bool metric_set = false, medium_set = false, mats_set = false;
   hipEvent_t ev_run[2];
@@ -164,7 +202,7 @@ static void free_all(cgfdh_solver* s)
     for (auto& a : f.aux) F(a);
   }
   F(s->src); F(s->mij); F(s->src_coef);
-  F(s->recv_idx); F(s->recv_w); F(s->seismo);
+  F(s->recv_idx); F(s->recv_w); F(s->seismo); F(s->pg);
 }
 
 extern "C" int cgfdh_create(const cgfdh_desc* d, cgfdh_solver** out)
@@ -205,6 +243,7 @@ extern "C" int cgfdh_create(const cgfdh_desc* d, cgfdh_solver** out)
   bad |= alloc(&s->med, 3 * s->vol);
   for (auto& l : s->lvl) bad |= alloc(&l, 9 * s->vol);
   bad |= alloc(&s->mats, 3 * 9 * s->ss);
+  if (d->free_top) bad |= alloc(&s->pg, 18 * s->ss);
   for (int f = 0; f < 6 && !bad; f++) {
     FaceHost& F = s->face[f];
     F.on = d->pml_on[f];
@@ -435,7 +474,7 @@ extern "C" int cgfdh_get_seismograms(cgfdh_solver* s, float* out)
 
 extern "C" int cgfdh_set_kernel_variant(cgfdh_solver* s, int v)
 {
-  if (!s || v < 0 || v > 3) return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: %d", v);
+  if (!s || v < 0 || v > 5) return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: %d", v);
   s->variant = v;
   return CGFDH_OK;
 }
@@ -526,6 +565,10 @@ static void launch_kind(cgfdh_solver* s, const StageParams& P, int dx, int dy, i
     if (in.ni > 0 && in.nj > 0 && in.nk > 0) {
       if (s->variant == 3)
         launch_interior_ty<KIND, 16>(s->stream, P, in, code);
+      else if (s->variant == 4)
+        launch_interior_point<KIND, 4>(s->stream, P, in, code);
+      else if (s->variant == 5)
+        launch_interior_point<KIND, 2>(s->stream, P, in, code);
       else
         launch_interior_ty<KIND, 8>(s->stream, P, in, code);
     }
@@ -581,6 +624,10 @@ extern "C" int cgfdh_run(cgfdh_solver* s, int it0, int nsteps)
       launch_stage(s, P, kind, ipair, st);
       HIPCHK(hipEventRecord(s->ev_stage[8 * n + 2 * st + 1], s->stream));
     }
+    if (s->pg)   // drv_rk_curv_col.c:492-495, before the pre/end swap
+      hipLaunchKernelGGL(pg_kernel, dim3((s->d.ni + 255) / 256, s->d.nj), dim3(256), 0,
+                         s->stream, s->lvl[3], s->lvl[0], s->pg, s->pg + 15 * s->ss, s->d.ni,
+                         s->d.nj, s->nx, s->ss, s->vol, s->nk2, s->d.dt);
     if (s->nrecv > 0) {
       int nt = s->nrecv * 9;
       hipLaunchKernelGGL(recv_kernel, dim3((nt + 255) / 256), dim3(256), 0, s->stream,
@@ -706,5 +753,15 @@ extern "C" int cgfdh_get_free_surface(cgfdh_solver* s, float* m)
 {
   if (!s || !m) return fail(CGFDH_EINVAL, "cgfdh_get_free_surface: null argument");
   HIPCHK(hipMemcpy(m, s->mats, 27 * s->ss * sizeof(float), hipMemcpyDeviceToHost));
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_get_peak_ground_motion(cgfdh_solver* s, float* pg, float* dis)
+{
+  if (!s) return fail(CGFDH_EINVAL, "cgfdh_get_peak_ground_motion: null solver");
+  if (!s->pg) return fail(CGFDH_ESTATE, "cgfdh_get_peak_ground_motion: no free surface");
+  if (pg) HIPCHK(hipMemcpy(pg, s->pg, 15 * s->ss * sizeof(float), hipMemcpyDeviceToHost));
+  if (dis)
+    HIPCHK(hipMemcpy(dis, s->pg + 15 * s->ss, 3 * s->ss * sizeof(float), hipMemcpyDeviceToHost));
   return CGFDH_OK;
 }

@@ -153,6 +153,69 @@ static void launch_interior_ty(hipStream_t st, const StageParams& P, BoxParams B
 #undef CGFDH_IN
 }
 
+// lean per-point kernel for the interior box: one thread per point, stencil
+// operands straight from global memory (L1/L2 serve the overlap), no LDS and
+// no barrier; block ids remapped so each XCD owns a contiguous run of
+// (x,y,z) blocks.  BY rows of 64 points per block.
+template <int DX, int DY, int DZ, int KIND, int BY>
+__global__ void __launch_bounds__(64 * BY)
+stage_interior_point_kernel(StageParams P, BoxParams B)
+{
+  const int nb8 = gridDim.x / 8;
+  int b = (blockIdx.x % 8) * nb8 + blockIdx.x / 8;
+  if (b >= B.nblocks) return;
+  const int bz = b / (B.tiles_x * B.tiles_y);
+  b -= bz * B.tiles_x * B.tiles_y;
+  const int by = b / B.tiles_x;
+  const int bx = b - by * B.tiles_x;
+  const int i = B.i0 + bx * 64 + threadIdx.x;
+  const int j = B.j0 + by * BY + threadIdx.y;
+  const int k = B.k0 + bz;
+  if (i >= B.i0 + B.ni || j >= B.j0 + B.nj) return;
+  const long p = (long)i + (long)j * P.nx + (long)k * P.ss;
+  const long V = P.vol;
+  float X[9], Y[9], Z[9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    const float* v = P.wcur + c * V;
+    X[c] = d5<DX>(v, p, 1);
+    Y[c] = d5<DY>(v, p, P.nx);
+    Z[c] = d5<DZ>(v, p, P.ss);
+  }
+  float m[9];
+#pragma unroll
+  for (int n = 0; n < 9; n++) m[n] = P.met[(1 + n) * V + p];
+  const float lam = P.lam[p], mu = P.mu[p], slw = P.slw[p];
+  const float lam2mu = (float)((double)lam + 2.0 * (double)mu);
+  float r[9];
+  momentum(r, m, slw, X, Y, Z);
+  float Xv[3] = {X[VX], X[VY], X[VZ]}, Yv[3] = {Y[VX], Y[VY], Y[VZ]},
+        Zv[3] = {Z[VX], Z[VY], Z[VZ]};
+  hooke(r, m, lam, lam2mu, mu, Xv, Yv, Zv);
+  if (P.nsrc > 0) add_sources(P, i, j, k, p, r);
+  combine<KIND>(P, p, r);
+}
+
+template <int KIND, int BY>
+static void launch_interior_point(hipStream_t st, const StageParams& P, BoxParams B, int code)
+{
+  B.tiles_x = (B.ni + 63) / 64;
+  B.tiles_y = (B.nj + BY - 1) / BY;
+  B.nblocks = B.tiles_x * B.tiles_y * B.nk;
+  const int grid = (B.nblocks + 7) / 8 * 8;
+  dim3 blk(64, BY, 1);
+#define CGFDH_IP(c, DX, DY, DZ)                                                          \
+  case c:                                                                                \
+    hipLaunchKernelGGL((stage_interior_point_kernel<DX, DY, DZ, KIND, BY>), dim3(grid), blk, \
+                       0, st, P, B);                                                     \
+    break;
+  switch (code) {
+    CGFDH_IP(0, 0, 0, 0) CGFDH_IP(1, 0, 0, 1) CGFDH_IP(2, 0, 1, 0) CGFDH_IP(3, 0, 1, 1)
+    CGFDH_IP(4, 1, 0, 0) CGFDH_IP(5, 1, 0, 1) CGFDH_IP(6, 1, 1, 0) CGFDH_IP(7, 1, 1, 1)
+  }
+#undef CGFDH_IP
+}
+
 // general per-point kernel restricted to a box (the boundary shell)
 template <int DX, int DY, int DZ, int KIND>
 __global__ void __launch_bounds__(256) stage_box_kernel(StageParams P, BoxParams B)

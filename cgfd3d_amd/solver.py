@@ -131,6 +131,13 @@ class Solver:
         check(lib().cgfdh_get_seismograms(self.h, _p(s)), "get_seismograms")
         return s[: self.nrecv]
 
+    def peak_ground_motion(self):
+        """(15, ny, nx) peak maps and (3, ny, nx) accumulated displacement."""
+        pg = np.empty((15, self.ny, self.nx), np.float32)
+        dis = np.empty((3, self.ny, self.nx), np.float32)
+        check(lib().cgfdh_get_peak_ground_motion(self.h, _p(pg), _p(dis)), "peak_ground_motion")
+        return pg, dis
+
     def onestage(self, it, ipair, istage, w_cur, aux_cur=None):
         w = _f32(w_cur)
         rhs = np.empty(9 * self.vol, np.float32)

@@ -115,6 +115,13 @@ int cgfdh_run(cgfdh_solver *s, int it0, int nsteps);
 /* seismo[(r*9 + icmp)*nt_total + it], the layout of iorecv->recvone[r].seismo */
 int cgfdh_get_seismograms(cgfdh_solver *s, float *seismo);
 
+/* Peak ground motion on the free surface, updated after every step as
+ * PG_calcu does (wav_t.c:380-455, called at drv_rk_curv_col.c:492-495):
+ * pg = 15 maps [ny][nx] in the order PGV,PGVh,PGVx,PGVy,PGVz,PGA,PGAh,PGAx,
+ * PGAy,PGAz,PGD,PGDh,PGDx,PGDy,PGDz (the PG buffer PG_slice_output writes);
+ * dis = the 3 accumulated displacement maps.  Either pointer may be NULL. */
+int cgfdh_get_peak_ground_motion(cgfdh_solver *s, float *pg, float *dis);
+
 /* One RHS evaluation without the RK combine (sv_curv_col_el_iso_onestage):
  * rhs and aux_rhs receive what the reference leaves in w_rhs and the PML
  * aux rhs level.  Host buffers; for parity checks and drop-in use. */

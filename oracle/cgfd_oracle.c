@@ -373,6 +373,7 @@ struct orc_s {
   size_t aux_off[6], aux_n[6], aux_total;
   real_t *pA[6], *pB[6], *pD[6];
   real_t *srcm;             /* 6 x nsrc*max_nt*4 */
+  real_t *pg, *dis;         /* peak ground motion maps, free surface only */
 };
 
 size_t orc_aux_len(const orc_t *o) { return o->aux_total; }
@@ -399,6 +400,8 @@ orc_t *orc_create(const orc_cfg *cfg, const float *metric10, const float *med3)
     orc_dvh2dvz_iso(cfg->ni, cfg->nj, cfg->nk, metric10, med3, m);
     for (size_t p = 0; p < 3 * nm; p++) o->mats[p] = m[p];
     free(m);
+    o->pg = (real_t *)calloc(15 * o->ss, sizeof(real_t));
+    o->dis = (real_t *)calloc(3 * o->ss, sizeof(real_t));
   }
   orc_pml_region(cfg->ni, cfg->nj, cfg->nk, cfg->pml_layers, cfg->pml_on, o->region);
   size_t off = 0;
@@ -432,7 +435,7 @@ orc_t *orc_create(const orc_cfg *cfg, const float *metric10, const float *med3)
 void orc_destroy(orc_t *o)
 {
   if (!o) return;
-  free(o->metric); free(o->med); free(o->mats); free(o->srcm);
+  free(o->metric); free(o->med); free(o->mats); free(o->srcm); free(o->pg); free(o->dis);
   for (int f = 0; f < 6; f++) { free(o->pA[f]); free(o->pB[f]); free(o->pD[f]); }
   free(o);
 }
@@ -776,6 +779,57 @@ void orc_onestage(orc_t *o, int it, int ipair, int istage,
   if (o->cfg.nsrc > 0) rhs_src(o, it, istage, rhs);
 }
 
+/* peak ground motion on the free surface, PG_calcu (wav_t.c:380-455):
+ * 15 maps PGV,PGVh,PGVx,PGVy,PGVz,PGA,PGAh,PGAx..z,PGD,PGDh,PGDx..z and the
+ * trapezoid displacement accumulator, from w_end and the previous step */
+static void pg_calcu(orc_t *o, const real_t *w1, const real_t *w0, float dt)
+{
+  size_t V = o->vol, S = o->ss;
+  const real_t *Vx1 = w1, *Vy1 = w1 + V, *Vz1 = w1 + 2 * V;
+  const real_t *Vx0 = w0, *Vy0 = w0 + V, *Vz0 = w0 + 2 * V;
+  real_t *PG = o->pg, *Dx = o->dis, *Dy = o->dis + S, *Dz = o->dis + 2 * S;
+  for (int j = o->nj1; j <= o->nj2; j++)
+    for (int i = o->ni1; i <= o->ni2; i++) {
+      size_t p = i + j * o->sl + (size_t)o->nk2 * S, q = i + j * o->sl;
+      real_t Vv, Vh, D, Dh, A, Ah, Ax, Ay, Az;
+      Ax = fabs((Vx1[p] - Vx0[p]) / dt);
+      Ay = fabs((Vy1[p] - Vy0[p]) / dt);
+      Az = fabs((Vz1[p] - Vz0[p]) / dt);
+      Dx[q] += 0.5 * (Vx1[p] + Vx0[p]) * dt;
+      Dy[q] += 0.5 * (Vy1[p] + Vy0[p]) * dt;
+      Dz[q] += 0.5 * (Vz1[p] + Vz0[p]) * dt;
+      Vv = sqrt(Vx1[p] * Vx1[p] + Vy1[p] * Vy1[p] + Vz1[p] * Vz1[p]);
+      Vh = sqrt(Vx1[p] * Vx1[p] + Vy1[p] * Vy1[p]);
+      A = sqrt(Ax * Ax + Ay * Ay + Az * Az);
+      Ah = sqrt(Ax * Ax + Ay * Ay);
+      D = sqrt(Dx[q] * Dx[q] + Dy[q] * Dy[q] + Dz[q] * Dz[q]);
+      Dh = sqrt(Dx[q] * Dx[q] + Dy[q] * Dy[q]);
+      real_t *m = PG;
+      if (m[2 * S + q] < fabs(Vx1[p])) m[2 * S + q] = fabs(Vx1[p]);
+      if (m[3 * S + q] < fabs(Vy1[p])) m[3 * S + q] = fabs(Vy1[p]);
+      if (m[4 * S + q] < fabs(Vz1[p])) m[4 * S + q] = fabs(Vz1[p]);
+      if (m[7 * S + q] < Ax) m[7 * S + q] = Ax;
+      if (m[8 * S + q] < Ay) m[8 * S + q] = Ay;
+      if (m[9 * S + q] < Az) m[9 * S + q] = Az;
+      if (m[12 * S + q] < fabs(Dx[q])) m[12 * S + q] = fabs(Dx[q]);
+      if (m[13 * S + q] < fabs(Dy[q])) m[13 * S + q] = fabs(Dy[q]);
+      if (m[14 * S + q] < fabs(Dz[q])) m[14 * S + q] = fabs(Dz[q]);
+      if (m[0 * S + q] < Vv) m[0 * S + q] = Vv;
+      if (m[1 * S + q] < Vh) m[1 * S + q] = Vh;
+      if (m[5 * S + q] < A) m[5 * S + q] = A;
+      if (m[6 * S + q] < Ah) m[6 * S + q] = Ah;
+      if (m[10 * S + q] < D) m[10 * S + q] = D;
+      if (m[11 * S + q] < Dh) m[11 * S + q] = Dh;
+    }
+}
+
+void orc_get_pg(const orc_t *o, real_t *pg, real_t *dis)
+{
+  if (!o->pg) return;
+  memcpy(pg, o->pg, 15 * o->ss * sizeof(real_t));
+  memcpy(dis, o->dis, 3 * o->ss * sizeof(real_t));
+}
+
 void orc_run(orc_t *o, float dt, int it0, int nt, real_t *w, real_t *aux,
              int nrecv, const int *recv_idx, const float *recv_w,
              int nt_total, real_t *seismo)
@@ -829,6 +883,7 @@ void orc_run(orc_t *o, float dt, int it0, int nt, real_t *w, real_t *aux,
           + v[ix[6]] * Lx1 * Ly2 * Lz2 + v[ix[7]] * Lx2 * Ly2 * Lz2;
       }
     }
+    if (o->pg) pg_calcu(o, end, pre, dt);   /* drv_rk_curv_col.c:492-495 */
     memcpy(pre, end, L * sizeof(real_t));
     if (LA) memcpy(apre, aend, LA * sizeof(real_t));
   }

@@ -86,6 +86,7 @@ void orc_onestage(orc_t *o, int it, int ipair, int istage,
 
 /* nt RK4 steps from level w (in/out) and aux (in/out, may be NULL if no PML);
  * seismo[(r*9+icmp)*nt_total + it0+n] sampled after each step when nrecv>0. */
+void orc_get_pg(const orc_t *o, real_t *pg, real_t *dis);
 void orc_run(orc_t *o, float dt, int it0, int nt, real_t *w, real_t *aux,
              int nrecv, const int *recv_idx, const float *recv_w,
              int nt_total, real_t *seismo);

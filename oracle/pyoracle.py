@@ -79,7 +79,7 @@ def load_ref():
     lib.refh_iso_onestage.restype = C.c_long
     lib.refh_iso_onestage.argtypes = [_P, _P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]
     lib.refh_iso_run.argtypes = [_P, _P, _P, _P, C.c_float, C.c_int, _P, C.c_int, _P, _P,
-                                 _P, _P]
+                                 _P, _P, _P, _P]
     return lib
 
 
@@ -161,3 +161,15 @@ class Oracle:
         self.lib.orc_run(self.h, dt, it0, nt, ptr(w), ptr(aux), nrecv, ptr(ri), ptr(rw),
                          nt_total, ptr(seismo))
         return w, aux[: self.aux_len], seismo[:nrecv]
+
+
+def oracle_pg(orc):
+    """Peak ground motion maps (15, ny, nx) and displacement (3, ny, nx) of an
+    Oracle after run() (free surface only)."""
+    lib = orc.lib
+    lib.orc_get_pg.argtypes = [_P, _P, _P]
+    nx, ny = orc.case["ni"] + 6, orc.case["nj"] + 6
+    pg = np.zeros((15, ny, nx), orc.dtype)
+    dis = np.zeros((3, ny, nx), orc.dtype)
+    lib.orc_get_pg(orc.h, ptr(pg), ptr(dis))
+    return pg, dis

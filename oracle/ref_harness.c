@@ -415,7 +415,7 @@ long refh_iso_onestage(const refh_cfg *cfg, float *coords, float *metric10, floa
 int refh_iso_run(const refh_cfg *cfg, float *coords, float *metric10, float *med3,
                  float dt, int nt, const float *w_init,
                  int nrecv, const int *recv_idx, const float *recv_w,
-                 float *w_final, float *seismo_out)
+                 float *w_final, float *seismo_out, float *pg_out, float *dis_out)
 {
   refh_ctx c;
   setup(&c, cfg, coords, metric10, med3);
@@ -424,6 +424,11 @@ int refh_iso_run(const refh_cfg *cfg, float *coords, float *metric10, float *med
   float *w_pre = c.wav.v5d, *w_tmp = c.wav.v5d + L, *w_rhs = c.wav.v5d + 2 * L,
         *w_end = c.wav.v5d + 3 * L, *w_cur;
   if (w_init) memcpy(w_pre, w_init, L * sizeof(float));
+  float *pg = NULL, *dis = NULL;
+  if (cfg->free_top && pg_out) {
+    pg = (float *)calloc(15 * c.gd.siz_iz, sizeof(float));
+    dis = (float *)calloc(3 * c.gd.siz_iz, sizeof(float));
+  }
   for (int f = 0; f < 6; f++) {
     bdrypml_auxvar_t *a = &c.bdry.auxvar[f / 2][f % 2];
     if (!c.bdry.is_sides_pml[f / 2][f % 2]) continue;
@@ -480,6 +485,9 @@ int refh_iso_run(const refh_cfg *cfg, float *coords, float *metric10, float *med
         seismo_out[((size_t)r * 9 + icmp) * nt + it] = v;
       }
     }
+    /* peak ground motion from w_end and w_pre before the swap
+     * (drv_rk_curv_col.c:492-495), the reference's own PG_calcu */
+    if (cfg->free_top && pg) PG_calcu(w_end, w_pre, &c.gd, pg, dis, dt);
     wav_zero_edge(&c.gd, &c.wav, w_rhs);
     float *t = w_pre; w_pre = w_end; w_end = t;
     for (int f = 0; f < 6; f++) {
@@ -489,6 +497,11 @@ int refh_iso_run(const refh_cfg *cfg, float *coords, float *metric10, float *med
     }
   }
   if (w_final) memcpy(w_final, w_pre, L * sizeof(float));
+  if (pg) {
+    memcpy(pg_out, pg, 15 * c.gd.siz_iz * sizeof(float));
+    if (dis_out) memcpy(dis_out, dis, 3 * c.gd.siz_iz * sizeof(float));
+    free(pg); free(dis);
+  }
   teardown(&c);
   return 0;
 }

@@ -107,7 +107,7 @@ def main():
         seis = np.zeros((len(ridx), 9, nt), np.float32)
         ref.refh_iso_run(C.byref(cfg), po.ptr(co), po.ptr(case["metric"]), po.ptr(case["med"]),
                          dt, nt, None, len(ridx), po.ptr(ridx), po.ptr(rw), po.ptr(wfin),
-                         po.ptr(seis))
+                         po.ptr(seis), None, None)
         W = wfin.reshape(9, nz, ny, nx)
         np.savez_compressed(os.path.join(HERE, f"run_{name}.npz"), seismo=seis,
                             surface=W[:, 3 + nk - 1, 3:3 + nj, 3:3 + ni],

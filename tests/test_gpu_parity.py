@@ -19,7 +19,7 @@ CONFIGS = [
 ]
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("name,free,pml", CONFIGS)
 def test_onestage_all_pairs(name, free, pml, variant):
     case = cs.make_case(18, 15, 13, seed=2, free_top=free, pml=pml, layers=4)
@@ -43,7 +43,7 @@ def test_onestage_all_pairs(name, free, pml, variant):
     assert mism == 0, "expected bit-identical RHS"
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("gauss", [False, True])
 def test_run_matches_oracle(variant, gauss):
     ni, nj, nk, nt, dt = 26, 22, 20, 60, 0.004
@@ -56,13 +56,125 @@ def test_run_matches_oracle(variant, gauss):
     ridx = np.concatenate([ridx, bi]); rw = np.concatenate([rw, bw])
     orc = po.Oracle(case)
     w_o, a_o, s_o = orc.run(dt, nt, recv_idx=ridx, recv_w=rw)
+    pg_o, dis_o = po.oracle_pg(orc)
     s = cs.make_solver(case, dt, nt, variant=variant)
     s.set_receivers(ridx, rw)
     s.run(0, nt)
     w_g, a_g, s_g = s.wavefield(), s.pml_aux(), s.seismograms()
+    pg_g, dis_g = s.peak_ground_motion()
     s.close()
+    assert pg_o.max() > 0
+    assert np.array_equal(pg_g, pg_o) and np.array_equal(dis_g, dis_o), "peak ground motion"
     e_w, e_s, e_a = cs.rel_l2(w_g, w_o), cs.rel_l2(s_g, s_o), cs.rel_l2(a_g, a_o)
     print(f"gauss={gauss} variant={variant}: rel L2 wave {e_w:.2e} seis {e_s:.2e} aux {e_a:.2e}")
     assert np.abs(s_o).max() > 0
     assert e_w <= 1e-5 and e_s <= 1e-5 and e_a <= 1e-5
     assert np.array_equal(w_g, w_o) and np.array_equal(s_g, s_o)
+
+
+def test_device_setup_matches_oracle():
+    """Metric (gd_curv_metric_cal) and free-surface matrices
+    (sv_curv_col_el_iso_dvh2dvz) built on the device equal the oracle's."""
+    import ctypes as C
+    from cgfd3d_amd import Solver, workloads
+    case = cs.make_case(21, 17, 15, seed=3, hill=0.4)
+    L = workloads._setup_sigs()
+    s = Solver(21, 17, 15, 0.004, 4, free_top=True)
+    xyz = np.ascontiguousarray(case["coords"])
+    assert L.cgfdh_compute_metric(s.h, po.ptr(xyz)) == 0
+    m = np.zeros_like(case["metric"])
+    assert L.cgfdh_get_metric(s.h, po.ptr(m)) == 0
+    assert np.array_equal(m, case["metric"])
+    s.set_medium_iso(case["med"][0], case["med"][1], case["med"][2])
+    assert L.cgfdh_compute_free_surface(s.h) == 0
+    mats = np.zeros(3 * 27 * 23 * 9, np.float32)
+    assert L.cgfdh_get_free_surface(s.h, po.ptr(mats)) == 0
+    assert np.array_equal(mats, cs.free_surface_mats(case))
+    s.close()
+
+
+def test_engine_matches_reference_golden():
+    """The committed fixture written by the reference's own code."""
+    import os
+    from tests.golden.make_golden import RUN_CASES, run_receivers
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "run_free_pml_src.npz"))
+    spec = RUN_CASES["free_pml_src"]
+    case = cs.make_case(**spec["case"])
+    ni, nj, nk = case["ni"], case["nj"], case["nk"]
+    case["src"] = cs.ricker_point_source(case, 3 + ni // 2, 3 + nj // 2, 3 + nk // 2, spec["nt"],
+                                         spec["dt"], fc=spec["fc"], t0=spec["t0"])
+    ridx, rw = run_receivers(case)
+    s = cs.make_solver(case, spec["dt"], spec["nt"])
+    s.set_receivers(ridx, rw)
+    s.run(0, spec["nt"])
+    seis, w = s.seismograms(), s.wavefield().reshape(9, nk + 6, nj + 6, ni + 6)
+    s.close()
+    print(f"golden: rel L2 seismograms {cs.rel_l2(seis, g['seismo']):.2e}")
+    assert cs.rel_l2(seis, g["seismo"]) <= 1e-5
+    assert np.array_equal(seis, g["seismo"])
+    assert np.array_equal(w[:, 3 + nk - 1, 3:3 + nj, 3:3 + ni], g["surface"])
+
+
+def test_midsize_run_matches_oracle():
+    """120x110x80 grid, 12 steps of the benchmark's kind of workload (topography,
+    random medium, double couple, PML 10 on 5 faces), engine vs oracle."""
+    from cgfd3d_amd import workloads
+    wl = workloads.iso_topo_grf(ni=120, nj=110, nk=80, nt_total=40, dt=0.004, pml=10, n_side=6)
+    s = wl.build_solver()
+    s.run(0, 12)
+    w_g, seis_g = s.wavefield(), s.seismograms()
+    s.close()
+    orc = po.load_oracle()
+    metric = np.zeros((10, 86, 116, 126), np.float32)
+    orc.orc_metric(120, 110, 80, po.ptr(wl.xyz), po.ptr(metric))
+    A, B, D, region = cs.pml_tables(orc, 120, 110, 80, wl.xyz, wl.pml_on, wl.pml_layers,
+                                    vel=wl.pml_vel, alpha=wl.pml_alpha, beta=wl.pml_beta)
+    case = dict(ni=120, nj=110, nk=80, metric=metric,
+                med=np.stack([wl.lam, wl.mu, wl.slw]), free_top=1, pml_on=wl.pml_on,
+                pml_layers=wl.pml_layers, pml_A=A, pml_B=B, pml_D=D, pml_region=region)
+    src = dict(wl.src)
+    src.update(nsrc=1, inc=[0, 0, 0], gauss=0)
+    case["src"] = src
+    w_o, _, seis_o = po.Oracle(case).run(wl.dt, 12, recv_idx=wl.recv_idx, recv_w=wl.recv_w,
+                                          nt_total=40)
+    e = cs.rel_l2(w_g, w_o)
+    print(f"midsize: rel L2 wavefield {e:.2e}")
+    assert np.abs(w_o).max() > 0 and e <= 1e-5
+    assert np.array_equal(w_g, w_o) and np.array_equal(seis_g, seis_o)
+
+
+def test_full_size_linearity():
+    """600x600x300 benchmark workload: doubling the source doubles every value
+    of the wavefield and every seismogram sample exactly (all operations are
+    linear and scaling by 2 is exact in binary floating point) except where
+    products fall into the subnormal range, whose rounding has fewer bits; a
+    run without a source stays exactly zero."""
+    from cgfd3d_amd import workloads
+    wl = workloads.iso_topo_grf(nt_total=10)
+    print("full-size workload built", flush=True)
+    s = wl.build_solver()
+    s.run(0, 6)
+    w1, r1 = s.wavefield(), s.seismograms()
+    s.close()
+    print("run 1 done", flush=True)
+    wl.src = dict(wl.src)
+    wl.src["mij"] = (wl.src["mij"] * np.float32(2.0)).astype(np.float32)
+    s = wl.build_solver()
+    s.run(0, 6)
+    w2, r2 = s.wavefield(), s.seismograms()
+    s.close()
+    print("run 2 done", flush=True)
+    assert np.abs(w1).max() > 0
+    differ = w2 != 2.0 * w1
+    print(f"values not exactly doubled: {int(differ.sum())}, largest |w| among them "
+          f"{float(np.abs(w1[differ]).max()) if differ.any() else 0.0:.3e}")
+    # 2^-100: values this small sit at the subnormal edge after a few products
+    assert not differ.any() or np.abs(w1[differ]).max() < 2.0 ** -100
+    assert cs.rel_l2(w2, 2.0 * w1) <= 1e-12
+    rdiff = r2 != 2.0 * r1
+    assert not rdiff.any() or np.abs(r1[rdiff]).max() < 2.0 ** -100
+    wl.src = None
+    s = wl.build_solver()
+    s.run(0, 3)
+    assert not np.any(s.wavefield())
+    s.close()

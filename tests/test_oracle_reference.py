@@ -122,17 +122,25 @@ def test_run_matches_reference():
     ridx, rw = cs.surface_receivers(case, n=4)
     bi, bw = cs.buried_receiver(case, 7, 8, 6, 0.2, 0.7, 0.35)
     ridx = np.concatenate([ridx, bi]); rw = np.concatenate([rw, bw])
-    w_o, a_o, s_o = po.Oracle(case).run(dt, nt, recv_idx=ridx, recv_w=rw)
+    orc = po.Oracle(case)
+    w_o, a_o, s_o = orc.run(dt, nt, recv_idx=ridx, recv_w=rw)
+    pg_o, dis_o = po.oracle_pg(orc)
     ref = po.load_ref()
     cfg, keep = cs.ref_cfg(case)
     w_r = np.zeros_like(w_o)
     s_r = np.zeros((len(ridx), 9, nt), np.float32)
+    pg_r = np.zeros_like(pg_o)
+    dis_r = np.zeros_like(dis_o)
     ref.refh_iso_run(C.byref(cfg), po.ptr(np.ascontiguousarray(case["coords"])),
                      po.ptr(case["metric"]), po.ptr(case["med"]), dt, nt, None, len(ridx),
-                     po.ptr(ridx), po.ptr(rw), po.ptr(w_r), po.ptr(s_r))
+                     po.ptr(ridx), po.ptr(rw), po.ptr(w_r), po.ptr(s_r), po.ptr(pg_r),
+                     po.ptr(dis_r))
     assert np.abs(s_r).max() > 0
     assert np.array_equal(w_o, w_r)
     assert np.array_equal(s_o, s_r)
+    # peak ground motion: the reference's own PG_calcu (wav_t.c:380-455)
+    assert pg_r.max() > 0
+    assert np.array_equal(pg_o, pg_r) and np.array_equal(dis_o, dis_r)
 
 
 # ---- golden fixtures (written by tests/golden/make_golden.py) ------------
