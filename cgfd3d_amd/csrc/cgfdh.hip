@@ -572,12 +572,16 @@ static void launch_kind(cgfdh_solver* s, const StageParams& P, int dx, int dy, i
       else
         launch_interior_ty<KIND, 8>(s->stream, P, in, code);
     }
-    launch_box<KIND>(s->stream, P, box(s->ni1, s->ni2, s->nj1, s->nj2, khi + 1, s->nk2), code);
-    launch_box<KIND>(s->stream, P, box(s->ni1, s->ni2, s->nj1, s->nj2, s->nk1, klo - 1), code);
-    launch_box<KIND>(s->stream, P, box(s->ni1, ilo - 1, s->nj1, s->nj2, klo, khi), code);
-    launch_box<KIND>(s->stream, P, box(ihi + 1, s->ni2, s->nj1, s->nj2, klo, khi), code);
-    launch_box<KIND>(s->stream, P, box(ilo, ihi, s->nj1, jlo - 1, klo, khi), code);
-    launch_box<KIND>(s->stream, P, box(ilo, ihi, jhi + 1, s->nj2, klo, khi), code);
+    // only the top slab holds free-surface rows (the others end at khi)
+    constexpr int SURF = HAS_SURF | HAS_PML;
+    launch_box<KIND, SURF>(s->stream, P, box(s->ni1, s->ni2, s->nj1, s->nj2, khi + 1, s->nk2),
+                           code);
+    launch_box<KIND, HAS_PML>(s->stream, P,
+                              box(s->ni1, s->ni2, s->nj1, s->nj2, s->nk1, klo - 1), code);
+    launch_box<KIND, HAS_PML>(s->stream, P, box(s->ni1, ilo - 1, s->nj1, s->nj2, klo, khi), code);
+    launch_box<KIND, HAS_PML>(s->stream, P, box(ihi + 1, s->ni2, s->nj1, s->nj2, klo, khi), code);
+    launch_box<KIND, HAS_PML>(s->stream, P, box(ilo, ihi, s->nj1, jlo - 1, klo, khi), code);
+    launch_box<KIND, HAS_PML>(s->stream, P, box(ilo, ihi, jhi + 1, s->nj2, klo, khi), code);
   }
 }
 

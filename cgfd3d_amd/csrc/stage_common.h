@@ -373,9 +373,14 @@ __device__ __forceinline__ void combine(const StageParams& P, long p, const floa
   }
 }
 
+// which boundary treatments a kernel instance has to handle
+enum { HAS_SURF = 1, HAS_PML = 2 };
+
 // everything after the derivatives: inner formulas, surface overrides, PML,
 // sources, RK combine.  X/Y/Z: inner-operator derivatives of the 9 fields.
-template <int DX, int DY, int DZ, int KIND>
+// FLAGS prunes the surface and PML code from instances whose points never
+// need them (the result is the same either way).
+template <int DX, int DY, int DZ, int KIND, int FLAGS = HAS_SURF | HAS_PML>
 __device__ __forceinline__ void finish_point(const StageParams& P, int i, int j, int k, long p,
                                              const float* X, const float* Y, const float* Z)
 {
@@ -387,18 +392,18 @@ __device__ __forceinline__ void finish_point(const StageParams& P, int i, int j,
   const float lam2mu = (float)((double)lam + 2.0 * (double)mu);
   float r[9];
   momentum(r, m, slw, X, Y, Z);
-  if (P.free_top && k >= P.nk2 - (DZ == 0 ? 3 : 1)) {
+  if ((FLAGS & HAS_SURF) && P.free_top && k >= P.nk2 - (DZ == 0 ? 3 : 1)) {
     // hV: traction image on rows nk2-3..nk2 (forward z op) or nk2-1..nk2
     timg_hv<DX, DY, DZ>(P, k, p, r);
   }
   float Xv[3] = {X[VX], X[VY], X[VZ]}, Yv[3] = {Y[VX], Y[VY], Y[VZ]};
-  if (P.free_top && k >= P.nk2 - 2) {
+  if ((FLAGS & HAS_SURF) && P.free_top && k >= P.nk2 - 2) {
     vlow_ht<DZ>(P, i, j, k, p, m, lam, lam2mu, mu, Xv, Yv, r);
   } else {
     float Zv[3] = {Z[VX], Z[VY], Z[VZ]};
     hooke(r, m, lam, lam2mu, mu, Xv, Yv, Zv);
   }
-  for (int f = 0; f < P.npml; f++) {
+  for (int f = 0; (FLAGS & HAS_PML) && f < P.npml; f++) {
     const PmlDev& F = P.pml[f];
     if (i >= F.i1 && i <= F.i2 && j >= F.j1 && j <= F.j2 && k >= F.k1 && k <= F.k2) {
       const float* Dd = F.idim == 0 ? X : (F.idim == 1 ? Y : Z);

@@ -216,8 +216,9 @@ static void launch_interior_point(hipStream_t st, const StageParams& P, BoxParam
 #undef CGFDH_IP
 }
 
-// general per-point kernel restricted to a box (the boundary shell)
-template <int DX, int DY, int DZ, int KIND>
+// general per-point kernel restricted to a box (the boundary shell); FLAGS
+// says whether the box touches the free-surface rows
+template <int DX, int DY, int DZ, int KIND, int FLAGS>
 __global__ void __launch_bounds__(256) stage_box_kernel(StageParams P, BoxParams B)
 {
   const int i = B.i0 + blockIdx.x * 64 + threadIdx.x;
@@ -234,18 +235,18 @@ __global__ void __launch_bounds__(256) stage_box_kernel(StageParams P, BoxParams
     Y[c] = d5<DY>(v, p, P.nx);
     Z[c] = d5<DZ>(v, p, P.ss);
   }
-  finish_point<DX, DY, DZ, KIND>(P, i, j, k, p, X, Y, Z);
+  finish_point<DX, DY, DZ, KIND, FLAGS>(P, i, j, k, p, X, Y, Z);
 }
 
-template <int KIND>
+template <int KIND, int FLAGS>
 static void launch_box(hipStream_t st, const StageParams& P, BoxParams B, int code)
 {
   if (B.ni <= 0 || B.nj <= 0 || B.nk <= 0) return;
   dim3 blk(64, 4, 1);
   dim3 grd((B.ni + 63) / 64, (B.nj + 3) / 4, B.nk);
-#define CGFDH_BX(c, DX, DY, DZ)                                                          \
-  case c:                                                                                \
-    hipLaunchKernelGGL((stage_box_kernel<DX, DY, DZ, KIND>), grd, blk, 0, st, P, B); \
+#define CGFDH_BX(c, DX, DY, DZ)                                                                \
+  case c:                                                                                      \
+    hipLaunchKernelGGL((stage_box_kernel<DX, DY, DZ, KIND, FLAGS>), grd, blk, 0, st, P, B); \
     break;
   switch (code) {
     CGFDH_BX(0, 0, 0, 0) CGFDH_BX(1, 0, 0, 1) CGFDH_BX(2, 0, 1, 0) CGFDH_BX(3, 0, 1, 1)
