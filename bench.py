@@ -25,19 +25,32 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 
+# per variant: (profile name, the kernels of one stage launch)
+KERNELS = [
+    ("stage_point", "stage_point_kernel"),
+    ("stage_tiled", "stage_tiled_kernel"),
+    ("stage_interior_ty8", "stage_interior_kernel<TY=8> + stage_box_kernel"),
+    ("stage_interior_ty16", "stage_interior_kernel<TY=16> + stage_box_kernel"),
+    ("stage_interior_point_by4", "stage_interior_point_kernel<BY=4> + stage_box_kernel"),
+    ("stage_interior_point_by2", "stage_interior_point_kernel<BY=2> + stage_box_kernel"),
+]
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measured_traffic(workload_name):
+def measured_traffic(kernel, workload_name):
     """HBM bytes per stage launch from the committed rocprofv3 PMC summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
-    path = os.path.join(ROOT, "profiles", f"stage_tiled_{workload_name}_traffic.json")
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None
+    when no summary for this kernel and workload is committed."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload_name}_traffic.md")
     try:
         with open(path) as f:
-            return json.load(f).get("bytes_per_launch")
-    except (OSError, ValueError):
+            m = re.search(r"^traffic_bytes_per_stage_launch:\s*([0-9.eE+]+)", f.read(), re.M)
+        return float(m.group(1)) if m else None
+    except OSError:
         return None
 
 
@@ -92,7 +105,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--variant", type=int, default=1, help="0 per-point, 1 tiled kernel")
+    ap.add_argument("--variant", type=int, default=4,
+                    help="stage kernels: 0 per-point, 1 tiled, 2/3 z-marching interior + "
+                         "shell, 4/5 per-point interior (64x4 / 64x2) + shell")
     ap.add_argument("--size", type=str, default="600x600x300")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -144,11 +159,11 @@ def main():
     bstep = wl.algorithmic_bytes_per_step()
     per_launch = bstep / 4.0
     achieved = per_launch / (stage_ms / nlaunch / 1e3) / 1e9
-    traffic = measured_traffic(wl.name)
+    kernel = KERNELS[args.variant]
+    traffic = measured_traffic(kernel[0], wl.name)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "algorithmic_bytes_per_launch": per_launch,
-            "kernel": ["stage_point_kernel", "stage_tiled_kernel", "stage_interior_kernel<TY=8> + stage_box_kernel", "stage_interior_kernel<TY=16> + stage_box_kernel", "stage_interior_point_kernel<BY=4> + stage_box_kernel", "stage_interior_point_kernel<BY=2> + stage_box_kernel"][args.variant]}
+            "algorithmic_bytes_per_launch": per_launch, "kernel": kernel[1]}
     cpu = None
     if not args.no_cpu:
         t = time.perf_counter()

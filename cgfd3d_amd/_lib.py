@@ -27,6 +27,8 @@ SIGNATURES = [
     ("cgfdh_last_error", C.c_char_p, []),
     ("cgfdh_set_metric", C.c_int, [_P, _P]),
     ("cgfdh_set_medium_iso", C.c_int, [_P, _P, _P, _P]),
+    ("cgfdh_set_medium_vti", C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    ("cgfdh_set_medium_aniso", C.c_int, [_P, _P, _P]),
     ("cgfdh_set_pml_face", C.c_int, [_P, C.c_int, _P, _P, _P]),
     ("cgfdh_set_free_surface", C.c_int, [_P, _P, _P, _P]),
     ("cgfdh_set_moment_sources", C.c_int,

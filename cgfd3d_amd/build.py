@@ -19,7 +19,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libcgfdh.so")
 
 HIPCC_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
+    "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
     "-ffp-contract=off", "-fno-gpu-flush-denormals-to-zero",
     "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -33,20 +33,40 @@ def hipcc():
     raise RuntimeError("hipcc not found: the gfx950 engine cannot be built")
 
 
+# one translation unit per medium's stage kernels, compiled in parallel
+UNITS = ["cgfdh.hip", "stage_iso.hip", "stage_vti.hip", "stage_aniso.hip"]
+
+
 def build_engine(verbose=False):
+    from concurrent.futures import ThreadPoolExecutor
     inc = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
-    obj = os.path.join(CSRC, "host_setup.o")
+    objdir = os.path.join(CSRC, "_obj")
+    os.makedirs(objdir, exist_ok=True)
+    obj = os.path.join(objdir, "host_setup.o")
     # host tables: plain C, no FMA contraction (the reference's x86-64 rounding)
     ccmd = ["gcc", "-O2", "-fPIC", "-std=gnu99", "-ffp-contract=off", "-c",
             os.path.join(CSRC, "host_setup.c"), "-o", obj] + inc
-    if verbose:
-        print(" ".join(ccmd), flush=True)
-    subprocess.run(ccmd, check=True)
-    srcs = [os.path.join(CSRC, "cgfdh.hip"), "-x", "none", obj]
-    cmd = [hipcc()] + HIPCC_FLAGS + inc + srcs + ["-o", LIB + ".tmp"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
+    cmds = [ccmd]
+    objs = [obj]
+    for u in UNITS:
+        o = os.path.join(objdir, u.replace(".hip", ".o"))
+        cmds.append([hipcc()] + HIPCC_FLAGS + inc + ["-c", os.path.join(CSRC, u), "-o", o])
+        objs.append(o)
+
+    def run(cmd):
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        return subprocess.run(cmd, capture_output=not verbose, text=True)
+
+    with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
+        results = list(ex.map(run, cmds))
+    for cmd, r in zip(cmds, results):
+        if r.returncode != 0:
+            raise RuntimeError(f"build failed: {' '.join(cmd)}\n{r.stderr or ''}")
+    link = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", LIB + ".tmp"]
+    r = run(link)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(link)}\n{r.stderr or ''}")
     os.replace(LIB + ".tmp", LIB)
     return LIB
 

@@ -15,35 +15,13 @@
 #include <vector>
 
 #include "cgfdh.h"
-#include "stage_common.h"
+#include "stage_launch.h"
 
 using namespace cgfdh;
 
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-
-// per-point kernel: one thread per grid point, every stencil operand read
-// from global memory.  The plain reference form of the stage.
-template <int DX, int DY, int DZ, int KIND>
-__global__ void __launch_bounds__(256) stage_point_kernel(StageParams P)
-{
-  const int i = P.ni1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = P.nj1 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = P.nk1 + blockIdx.z;
-  if (i > P.ni2 || j > P.nj2) return;
-  const long p = (long)i + (long)j * P.nx + (long)k * P.ss;
-  const long V = P.vol;
-  float X[9], Y[9], Z[9];
-#pragma unroll
-  for (int c = 0; c < 9; c++) {
-    const float* v = P.wcur + c * V;
-    X[c] = d5<DX>(v, p, 1);
-    Y[c] = d5<DY>(v, p, P.nx);
-    Z[c] = d5<DZ>(v, p, P.ss);
-  }
-  finish_point<DX, DY, DZ, KIND>(P, i, j, k, p, X, Y, Z);
-}
 
 // receivers: trilinear sample of w_end after a step (io_funcs.c:1585-1623),
 // indx1d[5] used twice and indx1d[4] not, as the reference does
@@ -104,8 +82,6 @@ __global__ void pg_kernel(const float* __restrict__ w1, const float* __restrict_
     if (PG[m * ss + q] < val[m]) PG[m * ss + q] = val[m];
 }
 
-#include "stage_tiled.h"
-#include "stage_interior.h"
 #include "setup_kernels.h"
 
 // ---------------------------------------------------------------------------
@@ -150,7 +126,8 @@ struct cgfdh_solver {
   int ni1, ni2, nj1, nj2, nk1, nk2;
   hipStream_t stream = nullptr;
   float* met = nullptr;
-  float* med = nullptr;          // lambda, mu, slw
+  int ncmp = 3;                  // medium volumes (medium.h), slowness last
+  float* med = nullptr;
   float* lvl[4] = {nullptr, nullptr, nullptr, nullptr};   // pre, tmpA, tmpB, end
   float* mats = nullptr;         // matVx2Vz2, matVy2Vz2, matF2Vz2
   FaceHost face[6];
@@ -165,7 +142,7 @@ struct cgfdh_solver {
   float* recv_w = nullptr;
   float* seismo = nullptr;
   float* pg = nullptr;           // 15 peak maps + 3 displacement maps, [ny][nx] each
-  int variant = 2;
+  int variant = 4;
   bool metric_set = false, medium_set = false, mats_set = false;
   hipEvent_t ev_run[2];
   std::vector<hipEvent_t> ev_stage;
@@ -211,7 +188,8 @@ extern "C" int cgfdh_create(const cgfdh_desc* d, cgfdh_solver** out)
   if (d->ni < 8 || d->nj < 8 || d->nk < 8)
     return fail(CGFDH_EINVAL, "cgfdh_create: grid %dx%dx%d below the 8-point minimum", d->ni,
                 d->nj, d->nk);
-  if (d->medium_type != CGFDH_MEDIUM_ELASTIC_ISO)
+  if (d->medium_type != CGFDH_MEDIUM_ELASTIC_ISO && d->medium_type != CGFDH_MEDIUM_ELASTIC_VTI &&
+      d->medium_type != CGFDH_MEDIUM_ELASTIC_ANISO)
     return fail(CGFDH_EINVAL, "cgfdh_create: medium_type %d not supported", d->medium_type);
   if (d->nt_total < 1) return fail(CGFDH_EINVAL, "cgfdh_create: nt_total %d", d->nt_total);
   for (int f = 0; f < 6; f++) {
@@ -240,7 +218,10 @@ extern "C" int cgfdh_create(const cgfdh_desc* d, cgfdh_solver** out)
   };
   int bad = 0;
   bad |= alloc(&s->met, 10 * s->vol);
-  bad |= alloc(&s->med, 3 * s->vol);
+  s->ncmp = d->medium_type == CGFDH_MEDIUM_ELASTIC_VTI     ? Medium<MEDIUM_VTI>::ncmp
+            : d->medium_type == CGFDH_MEDIUM_ELASTIC_ANISO ? Medium<MEDIUM_ANISO>::ncmp
+                                                          : Medium<MEDIUM_ISO>::ncmp;
+  bad |= alloc(&s->med, (size_t)s->ncmp * s->vol);
   for (auto& l : s->lvl) bad |= alloc(&l, 9 * s->vol);
   bad |= alloc(&s->mats, 3 * 9 * s->ss);
   if (d->free_top) bad |= alloc(&s->pg, 18 * s->ss);
@@ -303,9 +284,38 @@ extern "C" int cgfdh_set_medium_iso(cgfdh_solver* s, const float* lam, const flo
                                     const float* slw)
 {
   if (!s || !lam || !mu || !slw) return fail(CGFDH_EINVAL, "cgfdh_set_medium_iso: null argument");
+  if (s->d.medium_type != CGFDH_MEDIUM_ELASTIC_ISO)
+    return fail(CGFDH_ESTATE, "cgfdh_set_medium_iso: solver medium_type is %d", s->d.medium_type);
   int rc = h2d(s, s->med, lam, s->vol);
   if (!rc) rc = h2d(s, s->med + s->vol, mu, s->vol);
   if (!rc) rc = h2d(s, s->med + 2 * s->vol, slw, s->vol);
+  if (!rc) s->medium_set = true;
+  return rc;
+}
+
+extern "C" int cgfdh_set_medium_vti(cgfdh_solver* s, const float* c11, const float* c13,
+                                    const float* c33, const float* c55, const float* c66,
+                                    const float* slw)
+{
+  if (!s || !c11 || !c13 || !c33 || !c55 || !c66 || !slw)
+    return fail(CGFDH_EINVAL, "cgfdh_set_medium_vti: null argument");
+  if (s->d.medium_type != CGFDH_MEDIUM_ELASTIC_VTI)
+    return fail(CGFDH_ESTATE, "cgfdh_set_medium_vti: solver medium_type is %d", s->d.medium_type);
+  const float* v[6] = {c11, c13, c33, c55, c66, slw};
+  int rc = 0;
+  for (int q = 0; q < 6 && !rc; q++) rc = h2d(s, s->med + q * s->vol, v[q], s->vol);
+  if (!rc) s->medium_set = true;
+  return rc;
+}
+
+extern "C" int cgfdh_set_medium_aniso(cgfdh_solver* s, const float* cij, const float* slw)
+{
+  if (!s || !cij || !slw) return fail(CGFDH_EINVAL, "cgfdh_set_medium_aniso: null argument");
+  if (s->d.medium_type != CGFDH_MEDIUM_ELASTIC_ANISO)
+    return fail(CGFDH_ESTATE, "cgfdh_set_medium_aniso: solver medium_type is %d",
+                s->d.medium_type);
+  int rc = h2d(s, s->med, cij, 21 * s->vol);
+  if (!rc) rc = h2d(s, s->med + 21 * s->vol, slw, s->vol);
   if (!rc) s->medium_set = true;
   return rc;
 }
@@ -475,6 +485,9 @@ extern "C" int cgfdh_get_seismograms(cgfdh_solver* s, float* out)
 extern "C" int cgfdh_set_kernel_variant(cgfdh_solver* s, int v)
 {
   if (!s || v < 0 || v > 5) return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: %d", v);
+  if (s->d.medium_type != CGFDH_MEDIUM_ELASTIC_ISO && v >= 1 && v <= 3)
+    return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: variant %d is built for the isotropic "
+                "medium only (VTI / anisotropic: 0, 4, 5)", v);
   s->variant = v;
   return CGFDH_OK;
 }
@@ -501,7 +514,7 @@ static StageParams make_params(cgfdh_solver* s, int it, int istage, const float*
   P.ni1 = s->ni1; P.ni2 = s->ni2; P.nj1 = s->nj1; P.nj2 = s->nj2; P.nk1 = s->nk1; P.nk2 = s->nk2;
   P.nx = s->nx; P.ss = s->ss; P.vol = s->vol;
   P.met = s->met;
-  P.lam = s->med; P.mu = s->med + s->vol; P.slw = s->med + 2 * s->vol;
+  P.med = s->med; P.slw = s->med + (s->ncmp - 1) * s->vol;
   P.wcur = wcur; P.wpre = s->lvl[0]; P.wtmp = wtmp; P.wend = s->lvl[3];
   P.ca = istage < 3 ? (float)kRkA[istage] * s->d.dt : 0.0f;
   P.cb = (float)kRkB[istage] * s->d.dt;
@@ -528,74 +541,27 @@ static StageParams make_params(cgfdh_solver* s, int it, int istage, const float*
   return P;
 }
 
-template <int KIND>
-static void launch_kind(cgfdh_solver* s, const StageParams& P, int dx, int dy, int dz)
-{
-  const int code = dx * 4 + dy * 2 + dz;
-  if (s->variant == 0) {
-    dim3 blk(64, 4, 1);
-    dim3 grd((s->d.ni + 63) / 64, (s->d.nj + 3) / 4, s->d.nk);
-#define CGFDH_PT(c, DX, DY, DZ) \
-  case c: hipLaunchKernelGGL((stage_point_kernel<DX, DY, DZ, KIND>), grd, blk, 0, s->stream, P); break;
-    switch (code) {
-      CGFDH_PT(0, 0, 0, 0) CGFDH_PT(1, 0, 0, 1) CGFDH_PT(2, 0, 1, 0) CGFDH_PT(3, 0, 1, 1)
-      CGFDH_PT(4, 1, 0, 0) CGFDH_PT(5, 1, 0, 1) CGFDH_PT(6, 1, 1, 0) CGFDH_PT(7, 1, 1, 1)
-    }
-#undef CGFDH_PT
-  } else if (s->variant == 1) {
-    launch_tiled<KIND>(s->stream, P, s->d.ni, s->d.nj, s->d.nk, code);
-  } else {
-    // interior box (no PML face, no free-surface row) on the lean tiled
-    // kernel; the shell around it on the general per-point kernel
-    const cgfdh_desc& d = s->d;
-    const int ilo = d.pml_on[0] ? s->ni1 + d.pml_layers[0] + 1 : s->ni1;
-    const int ihi = d.pml_on[1] ? s->ni2 - d.pml_layers[1] - 1 : s->ni2;
-    const int jlo = d.pml_on[2] ? s->nj1 + d.pml_layers[2] + 1 : s->nj1;
-    const int jhi = d.pml_on[3] ? s->nj2 - d.pml_layers[3] - 1 : s->nj2;
-    const int klo = d.pml_on[4] ? s->nk1 + d.pml_layers[4] + 1 : s->nk1;
-    const int khi = d.free_top ? s->nk2 - 4 : (d.pml_on[5] ? s->nk2 - d.pml_layers[5] - 1 : s->nk2);
-    auto box = [](int i0, int i1, int j0, int j1, int k0, int k1) {
-      BoxParams b;
-      memset(&b, 0, sizeof b);
-      b.i0 = i0; b.j0 = j0; b.k0 = k0;
-      b.ni = i1 - i0 + 1; b.nj = j1 - j0 + 1; b.nk = k1 - k0 + 1;
-      return b;
-    };
-    const BoxParams in = box(ilo, ihi, jlo, jhi, klo, khi);
-    if (in.ni > 0 && in.nj > 0 && in.nk > 0) {
-      if (s->variant == 3)
-        launch_interior_ty<KIND, 16>(s->stream, P, in, code);
-      else if (s->variant == 4)
-        launch_interior_point<KIND, 4>(s->stream, P, in, code);
-      else if (s->variant == 5)
-        launch_interior_point<KIND, 2>(s->stream, P, in, code);
-      else
-        launch_interior_ty<KIND, 8>(s->stream, P, in, code);
-    }
-    // only the top slab holds free-surface rows (the others end at khi)
-    constexpr int SURF = HAS_SURF | HAS_PML;
-    launch_box<KIND, SURF>(s->stream, P, box(s->ni1, s->ni2, s->nj1, s->nj2, khi + 1, s->nk2),
-                           code);
-    launch_box<KIND, HAS_PML>(s->stream, P,
-                              box(s->ni1, s->ni2, s->nj1, s->nj2, s->nk1, klo - 1), code);
-    launch_box<KIND, HAS_PML>(s->stream, P, box(s->ni1, ilo - 1, s->nj1, s->nj2, klo, khi), code);
-    launch_box<KIND, HAS_PML>(s->stream, P, box(ihi + 1, s->ni2, s->nj1, s->nj2, klo, khi), code);
-    launch_box<KIND, HAS_PML>(s->stream, P, box(ilo, ihi, s->nj1, jlo - 1, klo, khi), code);
-    launch_box<KIND, HAS_PML>(s->stream, P, box(ilo, ihi, jhi + 1, s->nj2, klo, khi), code);
-  }
-}
-
 static void launch_stage(cgfdh_solver* s, const StageParams& P, int kind, int ipair, int istage)
 {
   const int dx = (kPairFlags[ipair][0] + istage) % 2;
   const int dy = (kPairFlags[ipair][1] + istage) % 2;
   const int dz = (kPairFlags[ipair][2] + istage) % 2;
-  switch (kind) {
-    case KIND_FIRST: launch_kind<KIND_FIRST>(s, P, dx, dy, dz); break;
-    case KIND_MID: launch_kind<KIND_MID>(s, P, dx, dy, dz); break;
-    case KIND_LAST: launch_kind<KIND_LAST>(s, P, dx, dy, dz); break;
-    default: launch_kind<KIND_RHS>(s, P, dx, dy, dz); break;
+  LaunchCtx L;
+  L.stream = s->stream;
+  L.variant = s->variant;
+  L.ni = s->d.ni; L.nj = s->d.nj; L.nk = s->d.nk;
+  L.ni1 = s->ni1; L.ni2 = s->ni2; L.nj1 = s->nj1; L.nj2 = s->nj2; L.nk1 = s->nk1; L.nk2 = s->nk2;
+  L.free_top = s->d.free_top;
+  for (int f = 0; f < 6; f++) {
+    L.pml_on[f] = s->d.pml_on[f];
+    L.pml_layers[f] = s->d.pml_layers[f];
   }
+  if (s->d.medium_type == CGFDH_MEDIUM_ELASTIC_VTI)
+    launch_stage_vti(L, P, kind, dx, dy, dz);
+  else if (s->d.medium_type == CGFDH_MEDIUM_ELASTIC_ANISO)
+    launch_stage_aniso(L, P, kind, dx, dy, dz);
+  else
+    launch_stage_iso(L, P, kind, dx, dy, dz);
 }
 
 extern "C" int cgfdh_run(cgfdh_solver* s, int it0, int nsteps)
@@ -737,9 +703,16 @@ extern "C" int cgfdh_compute_free_surface(cgfdh_solver* s)
   if (!s) return fail(CGFDH_EINVAL, "cgfdh_compute_free_surface: null solver");
   if (!s->metric_set || !s->medium_set)
     return fail(CGFDH_ESTATE, "cgfdh_compute_free_surface: metric and medium first");
-  hipLaunchKernelGGL(dvh2dvz_iso_kernel, dim3((s->d.ni + 255) / 256, s->d.nj), dim3(256), 0,
-                     s->stream, s->met, s->med, s->med + s->vol, s->mats, s->d.ni, s->d.nj,
-                     s->nx, s->nk2, s->ss, s->vol);
+  const dim3 grd((s->d.ni + 255) / 256, s->d.nj), blk(256);
+  if (s->d.medium_type == CGFDH_MEDIUM_ELASTIC_VTI)
+    hipLaunchKernelGGL(dvh2dvz_aniso_kernel<MEDIUM_VTI>, grd, blk, 0, s->stream, s->met, s->med,
+                       s->mats, s->d.ni, s->d.nj, s->nx, s->nk2, s->ss, s->vol);
+  else if (s->d.medium_type == CGFDH_MEDIUM_ELASTIC_ANISO)
+    hipLaunchKernelGGL(dvh2dvz_aniso_kernel<MEDIUM_ANISO>, grd, blk, 0, s->stream, s->met,
+                       s->med, s->mats, s->d.ni, s->d.nj, s->nx, s->nk2, s->ss, s->vol);
+  else
+    hipLaunchKernelGGL(dvh2dvz_iso_kernel, grd, blk, 0, s->stream, s->met, s->med,
+                       s->med + s->vol, s->mats, s->d.ni, s->d.nj, s->nx, s->nk2, s->ss, s->vol);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->stream));
   s->mats_set = true;

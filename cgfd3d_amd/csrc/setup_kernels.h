@@ -187,4 +187,87 @@ __global__ void dvh2dvz_iso_kernel(const float* __restrict__ met, const float* _
     }
 }
 
+// VTI / general anisotropic traction-free matrices at k = nk2
+// (sv_curv_col_el_vti.c:1085-1203, sv_curv_col_el_aniso.c:1267-1417).
+// K[t][b] is the coefficient of dV_b in stress row t (medium.h); rows of the
+// traction on the zeta surface: (Txx,Txy,Txz), (Txy,Tyy,Tyz), (Txz,Tyz,Tzz)
+// against zeta_x, zeta_y, zeta_z.  A sums K_zeta[row][b] * zeta_m, B and C
+// the same with K_xi and K_eta; stored as -A^-1 B and -A^-1 C.  VTI has no
+// term where kVtiHas is false; the third matrix is not written.
+__device__ constexpr bool kVtiHas[6][3] = {{1, 1, 1}, {1, 1, 1}, {1, 1, 1},
+                                           {0, 1, 1}, {1, 0, 1}, {1, 1, 0}};
+// VTI term c * e: index into {c11, c12, c13, c33, c55, c66} and into (e1,e2,e3)
+__device__ constexpr int kVtiC[6][3] = {{0, 1, 2}, {1, 0, 2}, {2, 2, 3},
+                                        {0, 4, 4}, {4, 0, 4}, {5, 5, 0}};
+__device__ constexpr int kVtiE[6][3] = {{0, 1, 2}, {0, 1, 2}, {0, 1, 2},
+                                        {0, 2, 1}, {2, 0, 0}, {1, 0, 0}};
+__device__ constexpr int kTracRow[3][3] = {{0, 5, 4}, {5, 1, 3}, {4, 3, 2}};
+
+template <int MT>
+__device__ __forceinline__ float coupling_k(const float* cf, int t, int b, const float* e)
+{
+  if (MT == MEDIUM_VTI) return cf[kVtiC[t][b]] * e[kVtiE[t][b]];
+  return cf[kAnisoIx[t][b][0]] * e[0] + cf[kAnisoIx[t][b][1]] * e[1] +
+         cf[kAnisoIx[t][b][2]] * e[2];
+}
+
+template <int MT>
+__global__ void dvh2dvz_aniso_kernel(const float* __restrict__ met, const float* __restrict__ med,
+                                     float* __restrict__ mats, int ni, int nj, int nx, int nk2,
+                                     long ss, long vol)
+{
+  const int i = 3 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 3 + blockIdx.y;
+  if (i > 3 + ni - 1) return;
+  const long p = (long)i + (long)j * nx + (long)nk2 * ss;
+  float xi[3], et[3], zt[3];
+#pragma unroll
+  for (int m = 0; m < 3; m++) {
+    xi[m] = met[(1 + m) * vol + p];
+    et[m] = met[(4 + m) * vol + p];
+    zt[m] = met[(7 + m) * vol + p];
+  }
+  float cf[21];
+  if (MT == MEDIUM_VTI) {
+    const float c11 = med[p], c66 = med[4 * vol + p];
+    cf[0] = c11;
+    cf[1] = (float)((double)c11 - 2.0 * (double)c66);
+    cf[2] = med[vol + p];
+    cf[3] = med[2 * vol + p];
+    cf[4] = med[3 * vol + p];
+    cf[5] = c66;
+  } else {
+#pragma unroll
+    for (int q = 0; q < 21; q++) cf[q] = med[q * vol + p];
+  }
+  float A[3][3], B[3][3], C[3][3], AB[3][3], AC[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      bool first = true;
+#pragma unroll
+      for (int m = 0; m < 3; m++) {
+        const int t = kTracRow[r][m];
+        if (MT == MEDIUM_VTI && !kVtiHas[t][b]) continue;
+        const float a = coupling_k<MT>(cf, t, b, zt) * zt[m];
+        const float bb = coupling_k<MT>(cf, t, b, xi) * zt[m];
+        const float cc = coupling_k<MT>(cf, t, b, et) * zt[m];
+        if (first) { A[r][b] = a; B[r][b] = bb; C[r][b] = cc; }
+        else { A[r][b] += a; B[r][b] += bb; C[r][b] += cc; }
+        first = false;
+      }
+    }
+  inv3(A);
+  mul3(A, B, AB);
+  mul3(A, C, AC);
+  const long ij = ((long)j * nx + i) * 9;
+  const long n = 9 * ss;
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) {
+      mats[ij + r * 3 + c] = -1.0f * AB[r][c];
+      mats[n + ij + r * 3 + c] = -1.0f * AC[r][c];
+    }
+}
+
 }  // namespace cgfdh

@@ -8,10 +8,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-namespace cgfdh {
+#include "medium.h"
 
-// wavefield component slots (wav_t.c:93-136)
-enum { VX = 0, VY, VZ, TXX, TYY, TZZ, TYZ, TXZ, TXY };
+namespace cgfdh {
 
 // DRP MacCormack operators as float, fd_t.c:109-112 (row 3);
 // dir 0 = {-1..3} ("forward"), dir 1 = {-3..1}
@@ -73,9 +72,8 @@ struct StageParams {
   int nx;
   long ss, vol;
   const float* met;         // 10 volumes
-  const float* lam;
-  const float* mu;
-  const float* slw;
+  const float* med;         // the medium's volumes (see medium.h)
+  const float* slw;         // its last volume
   const float* wcur;        // 9 volumes each
   const float* wpre;
   float* wtmp;
@@ -103,25 +101,6 @@ struct StageParams {
 };
 
 enum { KIND_FIRST = 0, KIND_MID = 1, KIND_LAST = 2, KIND_RHS = 3 };
-
-// Hooke's law from the 9 velocity derivatives (sv_curv_col_el_iso.c:409-435)
-__device__ __forceinline__ void hooke(float* r, const float* m, float lam, float lam2mu,
-                                      float mu, const float* X, const float* Y,
-                                      const float* Z)
-{
-  const float xix = m[0], xiy = m[1], xiz = m[2];
-  const float etx = m[3], ety = m[4], etz = m[5];
-  const float ztx = m[6], zty = m[7], ztz = m[8];
-  r[TXX] = lam2mu * (xix * X[0] + etx * Y[0] + ztx * Z[0]) +
-           lam * (xiy * X[1] + ety * Y[1] + zty * Z[1] + xiz * X[2] + etz * Y[2] + ztz * Z[2]);
-  r[TYY] = lam2mu * (xiy * X[1] + ety * Y[1] + zty * Z[1]) +
-           lam * (xix * X[0] + etx * Y[0] + ztx * Z[0] + xiz * X[2] + etz * Y[2] + ztz * Z[2]);
-  r[TZZ] = lam2mu * (xiz * X[2] + etz * Y[2] + ztz * Z[2]) +
-           lam * (xix * X[0] + etx * Y[0] + ztx * Z[0] + xiy * X[1] + ety * Y[1] + zty * Z[1]);
-  r[TXY] = mu * (xiy * X[0] + xix * X[1] + ety * Y[0] + etx * Y[1] + zty * Z[0] + ztx * Z[1]);
-  r[TXZ] = mu * (xiz * X[0] + xix * X[2] + etz * Y[0] + etx * Y[2] + ztz * Z[0] + ztx * Z[2]);
-  r[TYZ] = mu * (xiz * X[1] + xiy * X[2] + etz * Y[1] + ety * Y[2] + ztz * Z[1] + zty * Z[2]);
-}
 
 // momentum equation (sv_curv_col_el_iso.c:398-406); X,Y,Z hold the 9
 // derivatives in component-slot order
@@ -189,12 +168,12 @@ __device__ void timg_hv(const StageParams& P, int k, long p, float* r)
   }
 }
 
-// hT on the top 3 rows (sv_curv_col_el_iso.c:452-634): matrices at the
-// surface, the 2- and 3-point one-sided z operators below it
-template <int DZ>
+// hT on the top 3 rows (sv_curv_col_el_iso.c:452-634 and the VTI / general
+// anisotropic equivalents): matrices at the surface, the 2- and 3-point
+// one-sided z operators below it
+template <int DZ, int MT>
 __device__ void vlow_ht(const StageParams& P, int i, int j, int k, long p, const float* m,
-                        float lam, float lam2mu, float mu, const float* X, const float* Y,
-                        float* r)
+                        const Medium<MT>& md, const float* X, const float* Y, float* r)
 {
   float Z[3];
   const long V = P.vol, ss = P.ss;
@@ -227,15 +206,14 @@ __device__ void vlow_ht(const StageParams& P, int i, int j, int k, long p, const
       Z[c] = d;
     }
   }
-  hooke(r, m, lam, lam2mu, mu, X, Y, Z);
+  md.hooke(r, m, X, Y, Z);
 }
 
 // CFS-PML correction of one face (sv_curv_col_el_iso.c:645-1146) and the
 // RK combine of its auxiliary variables
-template <int KIND>
+template <int KIND, int MT>
 __device__ void pml_face(const StageParams& P, const PmlDev& F, int fslot, int i, int j, int k,
-                         long p, const float* Dd, float lam, float lam2mu, float mu,
-                         float slw, float* r)
+                         long p, const float* Dd, const Medium<MT>& md, float slw, float* r)
 {
   const long V = P.vol;
   const int ic = F.idim == 0 ? i - F.i1 : (F.idim == 1 ? j - F.j1 : k - F.k1);
@@ -248,12 +226,7 @@ __device__ void pml_face(const StageParams& P, const PmlDev& F, int fslot, int i
   rr[VX] = slw * (e1 * Dd[TXX] + e2 * Dd[TXY] + e3 * Dd[TXZ]);
   rr[VY] = slw * (e1 * Dd[TXY] + e2 * Dd[TYY] + e3 * Dd[TYZ]);
   rr[VZ] = slw * (e1 * Dd[TXZ] + e2 * Dd[TYZ] + e3 * Dd[TZZ]);
-  rr[TXX] = lam2mu * e1 * Dd[VX] + lam * e2 * Dd[VY] + lam * e3 * Dd[VZ];
-  rr[TYY] = lam * e1 * Dd[VX] + lam2mu * e2 * Dd[VY] + lam * e3 * Dd[VZ];
-  rr[TZZ] = lam * e1 * Dd[VX] + lam * e2 * Dd[VY] + lam2mu * e3 * Dd[VZ];
-  rr[TXY] = mu * (e2 * Dd[VX] + e1 * Dd[VY]);
-  rr[TXZ] = mu * (e3 * Dd[VX] + e1 * Dd[VZ]);
-  rr[TYZ] = mu * (e3 * Dd[VY] + e2 * Dd[VZ]);
+  md.one_dir(rr, e1, e2, e3, Dd);
   const long ia = (long)(i - F.i1) + (long)(j - F.j1) * F.fnx + (long)(k - F.k1) * F.fnx * F.fny;
   float a[9], ah[9];
 #pragma unroll
@@ -273,12 +246,7 @@ __device__ void pml_face(const StageParams& P, const PmlDev& F, int fslot, int i
               mm[ij + 3 * q + 2] * Dd[VZ];
     const float ztx = M[7 * V + p], zty = M[8 * V + p], ztz = M[9 * V + p];
     float s[9];
-    s[TXX] = lam2mu * (ztx * Dz[0]) + lam * (zty * Dz[1] + ztz * Dz[2]);
-    s[TYY] = lam2mu * (zty * Dz[1]) + lam * (ztx * Dz[0] + ztz * Dz[2]);
-    s[TZZ] = lam2mu * (ztz * Dz[2]) + lam * (ztx * Dz[0] + zty * Dz[1]);
-    s[TXY] = mu * (zty * Dz[0] + ztx * Dz[1]);
-    s[TXZ] = mu * (ztz * Dz[0] + ztx * Dz[2]);
-    s[TYZ] = mu * (ztz * Dz[1] + zty * Dz[2]);
+    md.surf(s, ztx, zty, ztz, Dz);
 #pragma unroll
     for (int c = TXX; c <= TXY; c++) {
       r[c] = (float)((double)r[c] + ((double)cB - 1.0) * (double)s[c]);
@@ -373,6 +341,30 @@ __device__ __forceinline__ void combine(const StageParams& P, long p, const floa
   }
 }
 
+// the interior form of everything after the derivatives: momentum, Hooke's
+// law, sources and the RK combine, for points with no PML face and no
+// free-surface row
+template <int KIND, int MT>
+__device__ __forceinline__ void interior_finish(const StageParams& P, int i, int j, int k,
+                                                long p, const float* X, const float* Y,
+                                                const float* Z)
+{
+  const long V = P.vol;
+  float m[9];
+#pragma unroll
+  for (int n = 0; n < 9; n++) m[n] = P.met[(1 + n) * V + p];
+  Medium<MT> md;
+  md.load(P.med, V, p);
+  const float slw = P.slw[p];
+  float r[9];
+  momentum(r, m, slw, X, Y, Z);
+  float Xv[3] = {X[VX], X[VY], X[VZ]}, Yv[3] = {Y[VX], Y[VY], Y[VZ]},
+        Zv[3] = {Z[VX], Z[VY], Z[VZ]};
+  md.hooke(r, m, Xv, Yv, Zv);
+  if (P.nsrc > 0) add_sources(P, i, j, k, p, r);
+  combine<KIND>(P, p, r);
+}
+
 // which boundary treatments a kernel instance has to handle
 enum { HAS_SURF = 1, HAS_PML = 2 };
 
@@ -380,7 +372,7 @@ enum { HAS_SURF = 1, HAS_PML = 2 };
 // sources, RK combine.  X/Y/Z: inner-operator derivatives of the 9 fields.
 // FLAGS prunes the surface and PML code from instances whose points never
 // need them (the result is the same either way).
-template <int DX, int DY, int DZ, int KIND, int FLAGS = HAS_SURF | HAS_PML>
+template <int DX, int DY, int DZ, int KIND, int MT, int FLAGS = HAS_SURF | HAS_PML>
 __device__ __forceinline__ void finish_point(const StageParams& P, int i, int j, int k, long p,
                                              const float* X, const float* Y, const float* Z)
 {
@@ -388,8 +380,9 @@ __device__ __forceinline__ void finish_point(const StageParams& P, int i, int j,
   float m[9];
 #pragma unroll
   for (int q = 0; q < 9; q++) m[q] = P.met[(1 + q) * V + p];
-  const float lam = P.lam[p], mu = P.mu[p], slw = P.slw[p];
-  const float lam2mu = (float)((double)lam + 2.0 * (double)mu);
+  Medium<MT> md;
+  md.load(P.med, V, p);
+  const float slw = P.slw[p];
   float r[9];
   momentum(r, m, slw, X, Y, Z);
   if ((FLAGS & HAS_SURF) && P.free_top && k >= P.nk2 - (DZ == 0 ? 3 : 1)) {
@@ -398,16 +391,16 @@ __device__ __forceinline__ void finish_point(const StageParams& P, int i, int j,
   }
   float Xv[3] = {X[VX], X[VY], X[VZ]}, Yv[3] = {Y[VX], Y[VY], Y[VZ]};
   if ((FLAGS & HAS_SURF) && P.free_top && k >= P.nk2 - 2) {
-    vlow_ht<DZ>(P, i, j, k, p, m, lam, lam2mu, mu, Xv, Yv, r);
+    vlow_ht<DZ, MT>(P, i, j, k, p, m, md, Xv, Yv, r);
   } else {
     float Zv[3] = {Z[VX], Z[VY], Z[VZ]};
-    hooke(r, m, lam, lam2mu, mu, Xv, Yv, Zv);
+    md.hooke(r, m, Xv, Yv, Zv);
   }
   for (int f = 0; (FLAGS & HAS_PML) && f < P.npml; f++) {
     const PmlDev& F = P.pml[f];
     if (i >= F.i1 && i <= F.i2 && j >= F.j1 && j <= F.j2 && k >= F.k1 && k <= F.k2) {
       const float* Dd = F.idim == 0 ? X : (F.idim == 1 ? Y : Z);
-      pml_face<KIND>(P, F, f, i, j, k, p, Dd, lam, lam2mu, mu, slw, r);
+      pml_face<KIND, MT>(P, F, f, i, j, k, p, Dd, md, slw, r);
     }
   }
   if (P.nsrc > 0) add_sources(P, i, j, k, p, r);

@@ -26,7 +26,7 @@ template <int TY> struct LdsTile {
   static constexpr int PLANE = W * H;
 };
 
-template <int DX, int DY, int DZ, int KIND, int TY>
+template <int DX, int DY, int DZ, int KIND, int MT, int TY>
 __global__ void __launch_bounds__(64 * TY)
 stage_interior_kernel(StageParams P, BoxParams B)
 {
@@ -110,25 +110,14 @@ stage_interior_kernel(StageParams P, BoxParams B)
                        s[colb + 4 * T::W]);
         Z[c] = d5v<DZ>(q[c][0], q[c][1], q[c][2], q[c][3], q[c][4]);
       }
-      float m[9];
-#pragma unroll
-      for (int n = 0; n < 9; n++) m[n] = P.met[(1 + n) * V + p];
-      const float lam = P.lam[p], mu = P.mu[p], slw = P.slw[p];
-      const float lam2mu = (float)((double)lam + 2.0 * (double)mu);
-      float r[9];
-      momentum(r, m, slw, X, Y, Z);
-      float Xv[3] = {X[VX], X[VY], X[VZ]}, Yv[3] = {Y[VX], Y[VY], Y[VZ]},
-            Zv[3] = {Z[VX], Z[VY], Z[VZ]};
-      hooke(r, m, lam, lam2mu, mu, Xv, Yv, Zv);
-      if (P.nsrc > 0) add_sources(P, i, j, k, p, r);
-      combine<KIND>(P, p, r);
+      interior_finish<KIND, MT>(P, i, j, k, p, X, Y, Z);
     }
     // the other LDS buffer is written next; this one is reused two planes on,
     // after the next barrier, so one barrier per plane suffices
   }
 }
 
-template <int KIND, int TY>
+template <int KIND, int MT, int TY>
 static void launch_interior_ty(hipStream_t st, const StageParams& P, BoxParams B, int code)
 {
   B.tiles_x = (B.ni + 63) / 64;
@@ -143,7 +132,7 @@ static void launch_interior_ty(hipStream_t st, const StageParams& P, BoxParams B
   dim3 blk(64, TY, 1);
 #define CGFDH_IN(c, DX, DY, DZ)                                                              \
   case c:                                                                                    \
-    hipLaunchKernelGGL((stage_interior_kernel<DX, DY, DZ, KIND, TY>), dim3(grid), blk, 0, st, \
+    hipLaunchKernelGGL((stage_interior_kernel<DX, DY, DZ, KIND, MT, TY>), dim3(grid), blk, 0, st, \
                        P, B);                                                                \
     break;
   switch (code) {
@@ -157,7 +146,7 @@ static void launch_interior_ty(hipStream_t st, const StageParams& P, BoxParams B
 // operands straight from global memory (L1/L2 serve the overlap), no LDS and
 // no barrier; block ids remapped so each XCD owns a contiguous run of
 // (x,y,z) blocks.  BY rows of 64 points per block.
-template <int DX, int DY, int DZ, int KIND, int BY>
+template <int DX, int DY, int DZ, int KIND, int MT, int BY>
 __global__ void __launch_bounds__(64 * BY)
 stage_interior_point_kernel(StageParams P, BoxParams B)
 {
@@ -182,21 +171,10 @@ stage_interior_point_kernel(StageParams P, BoxParams B)
     Y[c] = d5<DY>(v, p, P.nx);
     Z[c] = d5<DZ>(v, p, P.ss);
   }
-  float m[9];
-#pragma unroll
-  for (int n = 0; n < 9; n++) m[n] = P.met[(1 + n) * V + p];
-  const float lam = P.lam[p], mu = P.mu[p], slw = P.slw[p];
-  const float lam2mu = (float)((double)lam + 2.0 * (double)mu);
-  float r[9];
-  momentum(r, m, slw, X, Y, Z);
-  float Xv[3] = {X[VX], X[VY], X[VZ]}, Yv[3] = {Y[VX], Y[VY], Y[VZ]},
-        Zv[3] = {Z[VX], Z[VY], Z[VZ]};
-  hooke(r, m, lam, lam2mu, mu, Xv, Yv, Zv);
-  if (P.nsrc > 0) add_sources(P, i, j, k, p, r);
-  combine<KIND>(P, p, r);
+  interior_finish<KIND, MT>(P, i, j, k, p, X, Y, Z);
 }
 
-template <int KIND, int BY>
+template <int KIND, int MT, int BY>
 static void launch_interior_point(hipStream_t st, const StageParams& P, BoxParams B, int code)
 {
   B.tiles_x = (B.ni + 63) / 64;
@@ -206,7 +184,7 @@ static void launch_interior_point(hipStream_t st, const StageParams& P, BoxParam
   dim3 blk(64, BY, 1);
 #define CGFDH_IP(c, DX, DY, DZ)                                                          \
   case c:                                                                                \
-    hipLaunchKernelGGL((stage_interior_point_kernel<DX, DY, DZ, KIND, BY>), dim3(grid), blk, \
+    hipLaunchKernelGGL((stage_interior_point_kernel<DX, DY, DZ, KIND, MT, BY>), dim3(grid), blk, \
                        0, st, P, B);                                                     \
     break;
   switch (code) {
@@ -218,7 +196,7 @@ static void launch_interior_point(hipStream_t st, const StageParams& P, BoxParam
 
 // general per-point kernel restricted to a box (the boundary shell); FLAGS
 // says whether the box touches the free-surface rows
-template <int DX, int DY, int DZ, int KIND, int FLAGS>
+template <int DX, int DY, int DZ, int KIND, int MT, int FLAGS>
 __global__ void __launch_bounds__(256) stage_box_kernel(StageParams P, BoxParams B)
 {
   const int i = B.i0 + blockIdx.x * 64 + threadIdx.x;
@@ -235,10 +213,10 @@ __global__ void __launch_bounds__(256) stage_box_kernel(StageParams P, BoxParams
     Y[c] = d5<DY>(v, p, P.nx);
     Z[c] = d5<DZ>(v, p, P.ss);
   }
-  finish_point<DX, DY, DZ, KIND, FLAGS>(P, i, j, k, p, X, Y, Z);
+  finish_point<DX, DY, DZ, KIND, MT, FLAGS>(P, i, j, k, p, X, Y, Z);
 }
 
-template <int KIND, int FLAGS>
+template <int KIND, int MT, int FLAGS>
 static void launch_box(hipStream_t st, const StageParams& P, BoxParams B, int code)
 {
   if (B.ni <= 0 || B.nj <= 0 || B.nk <= 0) return;
@@ -246,7 +224,7 @@ static void launch_box(hipStream_t st, const StageParams& P, BoxParams B, int co
   dim3 grd((B.ni + 63) / 64, (B.nj + 3) / 4, B.nk);
 #define CGFDH_BX(c, DX, DY, DZ)                                                                \
   case c:                                                                                      \
-    hipLaunchKernelGGL((stage_box_kernel<DX, DY, DZ, KIND, FLAGS>), grd, blk, 0, st, P, B); \
+    hipLaunchKernelGGL((stage_box_kernel<DX, DY, DZ, KIND, MT, FLAGS>), grd, blk, 0, st, P, B); \
     break;
   switch (code) {
     CGFDH_BX(0, 0, 0, 0) CGFDH_BX(1, 0, 0, 1) CGFDH_BX(2, 0, 1, 0) CGFDH_BX(3, 0, 1, 1)

@@ -1,0 +1,138 @@
+// stage_launch.h -- launch of one fused RK stage for one medium type.
+//
+// The kernels are templated on the medium (medium.h); each medium's
+// instances are compiled in a translation unit of their own
+// (stage_iso.hip, stage_vti.hip, stage_aniso.hip), which defines
+// launch_stage_<medium> from launch_stage_mt below.
+//
+// Variants: 0 the per-point kernel over the whole grid; 1 the z-marching
+// tiled kernel over the whole grid; 2/3 the z-marching interior kernel
+// (64x8 / 64x16 tiles) plus the per-point kernel on the boundary shell;
+// 4/5 the lean per-point interior kernel (64x4 / 64x2 blocks) plus the
+// shell.  The VTI and anisotropic media build variants 0, 4 and 5.
+#pragma once
+#include <algorithm>
+#include <cstring>
+
+#include "stage_common.h"
+#include "stage_tiled.h"
+#include "stage_interior.h"
+
+namespace cgfdh {
+
+struct LaunchCtx {
+  hipStream_t stream;
+  int variant;
+  int ni, nj, nk;
+  int ni1, ni2, nj1, nj2, nk1, nk2;
+  int free_top;
+  int pml_on[6], pml_layers[6];
+};
+
+// per-point kernel: one thread per grid point, every stencil operand read
+// from global memory.  The plain reference form of the stage.
+template <int DX, int DY, int DZ, int KIND, int MT>
+__global__ void __launch_bounds__(256) stage_point_kernel(StageParams P)
+{
+  const int i = P.ni1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = P.nj1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = P.nk1 + blockIdx.z;
+  if (i > P.ni2 || j > P.nj2) return;
+  const long p = (long)i + (long)j * P.nx + (long)k * P.ss;
+  const long V = P.vol;
+  float X[9], Y[9], Z[9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    const float* v = P.wcur + c * V;
+    X[c] = d5<DX>(v, p, 1);
+    Y[c] = d5<DY>(v, p, P.nx);
+    Z[c] = d5<DZ>(v, p, P.ss);
+  }
+  finish_point<DX, DY, DZ, KIND, MT>(P, i, j, k, p, X, Y, Z);
+}
+
+template <int KIND, int MT>
+static void launch_kind(const LaunchCtx& L, const StageParams& P, int dx, int dy, int dz)
+{
+  const int code = dx * 4 + dy * 2 + dz;
+  hipStream_t st = L.stream;
+  if (L.variant == 0) {
+    dim3 blk(64, 4, 1);
+    dim3 grd((L.ni + 63) / 64, (L.nj + 3) / 4, L.nk);
+#define CGFDH_PT(c, DX, DY, DZ)                                                              \
+  case c:                                                                                    \
+    hipLaunchKernelGGL((stage_point_kernel<DX, DY, DZ, KIND, MT>), grd, blk, 0, st, P); \
+    break;
+    switch (code) {
+      CGFDH_PT(0, 0, 0, 0) CGFDH_PT(1, 0, 0, 1) CGFDH_PT(2, 0, 1, 0) CGFDH_PT(3, 0, 1, 1)
+      CGFDH_PT(4, 1, 0, 0) CGFDH_PT(5, 1, 0, 1) CGFDH_PT(6, 1, 1, 0) CGFDH_PT(7, 1, 1, 1)
+    }
+#undef CGFDH_PT
+    return;
+  }
+  if constexpr (MT == MEDIUM_ISO) {
+    if (L.variant == 1) {
+      launch_tiled<KIND, MT>(st, P, L.ni, L.nj, L.nk, code);
+      return;
+    }
+  }
+  // interior box (no PML face, no free-surface row) on a lean kernel; the
+  // shell around it on the general per-point kernel
+  const int ilo = L.pml_on[0] ? L.ni1 + L.pml_layers[0] + 1 : L.ni1;
+  const int ihi = L.pml_on[1] ? L.ni2 - L.pml_layers[1] - 1 : L.ni2;
+  const int jlo = L.pml_on[2] ? L.nj1 + L.pml_layers[2] + 1 : L.nj1;
+  const int jhi = L.pml_on[3] ? L.nj2 - L.pml_layers[3] - 1 : L.nj2;
+  const int klo = L.pml_on[4] ? L.nk1 + L.pml_layers[4] + 1 : L.nk1;
+  const int khi = L.free_top ? L.nk2 - 4 : (L.pml_on[5] ? L.nk2 - L.pml_layers[5] - 1 : L.nk2);
+  auto box = [](int i0, int i1, int j0, int j1, int k0, int k1) {
+    BoxParams b;
+    memset(&b, 0, sizeof b);
+    b.i0 = i0; b.j0 = j0; b.k0 = k0;
+    b.ni = i1 - i0 + 1; b.nj = j1 - j0 + 1; b.nk = k1 - k0 + 1;
+    return b;
+  };
+  const BoxParams in = box(ilo, ihi, jlo, jhi, klo, khi);
+  if (in.ni > 0 && in.nj > 0 && in.nk > 0) {
+    if (L.variant == 5) {
+      launch_interior_point<KIND, MT, 2>(st, P, in, code);
+    } else if (L.variant == 4) {
+      launch_interior_point<KIND, MT, 4>(st, P, in, code);
+    } else {
+      if constexpr (MT == MEDIUM_ISO) {
+        if (L.variant == 3)
+          launch_interior_ty<KIND, MT, 16>(st, P, in, code);
+        else
+          launch_interior_ty<KIND, MT, 8>(st, P, in, code);
+      } else {
+        launch_interior_point<KIND, MT, 4>(st, P, in, code);
+      }
+    }
+  }
+  // only the top slab holds free-surface rows (the others end at khi)
+  constexpr int SURF = HAS_SURF | HAS_PML;
+  launch_box<KIND, MT, SURF>(st, P, box(L.ni1, L.ni2, L.nj1, L.nj2, khi + 1, L.nk2), code);
+  launch_box<KIND, MT, HAS_PML>(st, P, box(L.ni1, L.ni2, L.nj1, L.nj2, L.nk1, klo - 1), code);
+  launch_box<KIND, MT, HAS_PML>(st, P, box(L.ni1, ilo - 1, L.nj1, L.nj2, klo, khi), code);
+  launch_box<KIND, MT, HAS_PML>(st, P, box(ihi + 1, L.ni2, L.nj1, L.nj2, klo, khi), code);
+  launch_box<KIND, MT, HAS_PML>(st, P, box(ilo, ihi, L.nj1, jlo - 1, klo, khi), code);
+  launch_box<KIND, MT, HAS_PML>(st, P, box(ilo, ihi, jhi + 1, L.nj2, klo, khi), code);
+}
+
+template <int MT>
+void launch_stage_mt(const LaunchCtx& L, const StageParams& P, int kind, int dx, int dy, int dz)
+{
+  switch (kind) {
+    case KIND_FIRST: launch_kind<KIND_FIRST, MT>(L, P, dx, dy, dz); break;
+    case KIND_MID: launch_kind<KIND_MID, MT>(L, P, dx, dy, dz); break;
+    case KIND_LAST: launch_kind<KIND_LAST, MT>(L, P, dx, dy, dz); break;
+    default: launch_kind<KIND_RHS, MT>(L, P, dx, dy, dz); break;
+  }
+}
+
+// defined in stage_iso.hip, stage_vti.hip and stage_aniso.hip
+void launch_stage_iso(const LaunchCtx& L, const StageParams& P, int kind, int dx, int dy, int dz);
+void launch_stage_vti(const LaunchCtx& L, const StageParams& P, int kind, int dx, int dy, int dz);
+void launch_stage_aniso(const LaunchCtx& L, const StageParams& P, int kind, int dx, int dy,
+                        int dz);
+
+}  // namespace cgfdh

@@ -19,7 +19,7 @@ constexpr int LDS_W = TILE_X + HALO;          // 68
 constexpr int LDS_H = TILE_Y + HALO;          // 12
 constexpr int LDS_PLANE = LDS_W * LDS_H;
 
-template <int DX, int DY, int DZ, int KIND>
+template <int DX, int DY, int DZ, int KIND, int MT>
 __global__ void __launch_bounds__(TILE_X * TILE_Y)
 stage_tiled_kernel(StageParams P)
 {
@@ -109,12 +109,12 @@ stage_tiled_kernel(StageParams P)
                        s[colb + 4 * LDS_W]);
         Z[c] = d5v<DZ>(q[c][0], q[c][1], q[c][2], q[c][3], q[c][4]);
       }
-      finish_point<DX, DY, DZ, KIND>(P, i, j, k, pij + pk, X, Y, Z);
+      finish_point<DX, DY, DZ, KIND, MT>(P, i, j, k, pij + pk, X, Y, Z);
     }
   }
 }
 
-template <int KIND>
+template <int KIND, int MT>
 static void launch_tiled(hipStream_t st, StageParams P, int ni, int nj, int nk, int code)
 {
   P.tiles_x = (ni + TILE_X - 1) / TILE_X;
@@ -129,7 +129,7 @@ static void launch_tiled(hipStream_t st, StageParams P, int ni, int nj, int nk, 
   dim3 blk(TILE_X, TILE_Y, 1);
   dim3 grd(ntile * nchunk, 1, 1);
 #define CGFDH_TL(c, DX, DY, DZ) \
-  case c: hipLaunchKernelGGL((stage_tiled_kernel<DX, DY, DZ, KIND>), grd, blk, 0, st, P); break;
+  case c: hipLaunchKernelGGL((stage_tiled_kernel<DX, DY, DZ, KIND, MT>), grd, blk, 0, st, P); break;
   switch (code) {
     CGFDH_TL(0, 0, 0, 0) CGFDH_TL(1, 0, 0, 1) CGFDH_TL(2, 0, 1, 0) CGFDH_TL(3, 0, 1, 1)
     CGFDH_TL(4, 1, 0, 0) CGFDH_TL(5, 1, 0, 1) CGFDH_TL(6, 1, 1, 0) CGFDH_TL(7, 1, 1, 1)

@@ -11,6 +11,8 @@ import numpy as np
 from ._lib import CgfdhDesc, check, lib
 
 MEDIUM_ELASTIC_ISO = 2
+MEDIUM_ELASTIC_VTI = 3
+MEDIUM_ELASTIC_ANISO = 4
 SRC_POINT = 1
 SRC_GAUSSIAN = 2
 
@@ -71,6 +73,26 @@ class Solver:
         a, b, c = _f32(lam), _f32(mu), _f32(slw)
         assert a.size == b.size == c.size == self.vol
         check(lib().cgfdh_set_medium_iso(self.h, _p(a), _p(b), _p(c)), "set_medium_iso")
+
+    def set_medium_vti(self, c11, c13, c33, c55, c66, slw):
+        v = [_f32(x) for x in (c11, c13, c33, c55, c66, slw)]
+        assert all(x.size == self.vol for x in v)
+        check(lib().cgfdh_set_medium_vti(self.h, *[_p(x) for x in v]), "set_medium_vti")
+
+    def set_medium_aniso(self, cij21, slw):
+        c, s = _f32(cij21), _f32(slw)
+        assert c.size == 21 * self.vol and s.size == self.vol
+        check(lib().cgfdh_set_medium_aniso(self.h, _p(c), _p(s)), "set_medium_aniso")
+
+    def set_medium(self, medium_type, volumes):
+        """volumes: the medium's arrays in include/cgfdh.h order, slowness last."""
+        v = _f32(volumes).reshape(-1, self.vol)
+        if medium_type == MEDIUM_ELASTIC_VTI:
+            self.set_medium_vti(*v)
+        elif medium_type == MEDIUM_ELASTIC_ANISO:
+            self.set_medium_aniso(v[:21], v[21])
+        else:
+            self.set_medium_iso(*v)
 
     def set_pml_face(self, face, A, B, D):
         a, b, d = _f32(A), _f32(B), _f32(D)

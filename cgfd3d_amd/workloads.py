@@ -191,7 +191,7 @@ class Workload:
         nsurf = self.ni * self.nj if self.free_top else 0
         return 768.0 * self.npoints + 576.0 * self.n_aux() + 352.0 * nsurf
 
-    def build_solver(self, variant=1):
+    def build_solver(self, variant=4):
         L = _setup_sigs()
         s = Solver(self.ni, self.nj, self.nk, self.dt, self.nt_total, free_top=self.free_top,
                    pml_on=self.pml_on, pml_layers=self.pml_layers)

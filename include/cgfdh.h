@@ -9,7 +9,9 @@
  *                           float t0, char*, char*, int, int, int)
  *   (forward/drv_rk_curv_col.h:17-37, called from main_curv_col_el_3d.c:849-858)
  * together with the one-stage RHS it dispatches to,
- *   sv_curv_col_el_iso_onestage(...)  (forward/sv_curv_col_el_iso.h:17-32).
+ *   sv_curv_col_el_iso_onestage(...)    (forward/sv_curv_col_el_iso.h:17-32),
+ *   sv_curv_col_el_vti_onestage(...)    (forward/sv_curv_col_el_vti.h:17-31),
+ *   sv_curv_col_el_aniso_onestage(...)  (forward/sv_curv_col_el_aniso.h:17-32).
  * This ABI carries the same data with plain pointers and sizes, so a C shim
  * that unpacks those structs into these calls replaces the reference's time
  * loop (the shim is shown in INTEGRATION.md).  No torch or HIP types appear.
@@ -44,6 +46,8 @@ extern "C" {
 
 /* medium codes are the reference's (constants.h) */
 #define CGFDH_MEDIUM_ELASTIC_ISO 2
+#define CGFDH_MEDIUM_ELASTIC_VTI 3
+#define CGFDH_MEDIUM_ELASTIC_ANISO 4
 
 #define CGFDH_SRC_POINT 1     /* CONST_SRC_SPATIAL_POINT */
 #define CGFDH_SRC_GAUSSIAN 2  /* CONST_SRC_SPATIAL_GAUSSIAN */
@@ -52,7 +56,7 @@ typedef struct cgfdh_solver cgfdh_solver;
 
 typedef struct {
   int ni, nj, nk;       /* physical grid points of the block */
-  int medium_type;      /* CGFDH_MEDIUM_ELASTIC_ISO */
+  int medium_type;      /* CGFDH_MEDIUM_ELASTIC_ISO, _VTI or _ANISO */
   int free_top;         /* 1: traction-image free surface on z2 */
   int pml_on[6];        /* CFS-PML per face */
   int pml_layers[6];    /* number_of_layers per face */
@@ -73,13 +77,28 @@ int cgfdh_set_metric(cgfdh_solver *s, const float *metric10);
 int cgfdh_set_medium_iso(cgfdh_solver *s, const float *lambda, const float *mu,
                          const float *slowness);
 
+/* md_t c11, c13, c33, c55, c66 of a VTI medium (md_t.c:112-134) and
+ * rho-as-slowness; c12 = c11 - 2 c66 is formed per point as
+ * sv_curv_col_el_vti.c:351 does. */
+int cgfdh_set_medium_vti(cgfdh_solver *s, const float *c11, const float *c13,
+                         const float *c33, const float *c55, const float *c66,
+                         const float *slowness);
+
+/* md_t c11,c12,c13,c14,c15,c16,c22,c23,c24,c25,c26,c33,c34,c35,c36,c44,c45,
+ * c46,c55,c56,c66 of a general anisotropic medium (md_t.c:136-221), 21
+ * volumes back to back, and rho-as-slowness. */
+int cgfdh_set_medium_aniso(cgfdh_solver *s, const float *cij21,
+                           const float *slowness);
+
 /* bdry_t A/B/D of one face, num_layers+1 values each, already converted as
  * bdry_pml_set leaves them (D/=B, A+=D, B=1/B; bdry_t.c:259-268). */
 int cgfdh_set_pml_face(cgfdh_solver *s, int face, const float *A, const float *B,
                        const float *D);
 
 /* bdry_t matVx2Vz2 / matVy2Vz2 / matF2Vz2, [ny][nx][3][3] each
- * (sv_curv_col_el_iso_dvh2dvz, sv_curv_col_el_iso.c:1259-1375). */
+ * (sv_curv_col_el_iso_dvh2dvz, sv_curv_col_el_iso.c:1259-1375; the VTI and
+ * anisotropic versions, sv_curv_col_el_vti.c:1085-1203 and
+ * sv_curv_col_el_aniso.c:1267-1417, leave matF2Vz2 zero). */
 int cgfdh_set_free_surface(cgfdh_solver *s, const float *matVx2Vz2,
                            const float *matVy2Vz2, const float *matF2Vz2);
 
@@ -135,9 +154,11 @@ int cgfdh_last_timing(cgfdh_solver *s, double *run_ms, double *stage_ms,
                       int *stage_launches);
 
 /* Selects the stage kernels: 0 = per-point kernel everywhere, 1 = general
- * tiled z-marching kernel everywhere, 2 (default) / 3 = lean tiled kernel on
- * the interior box (64x8 / 64x16 tiles) with the per-point kernel on the
- * PML and free-surface shell.  All variants produce identical bits. */
+ * tiled z-marching kernel everywhere, 2 / 3 = lean z-marching kernel on the
+ * interior box (64x8 / 64x16 tiles), 4 / 5 = lean per-point kernel on the
+ * interior box (64x4 / 64x2 blocks); 2..5 run the per-point kernel on the PML
+ * and free-surface shell.  Default 4.  Variants 1..3 exist for the isotropic
+ * medium only.  All variants produce identical bits. */
 int cgfdh_set_kernel_variant(cgfdh_solver *s, int variant);
 
 /* ---- setup on the device ------------------------------------------------ */
@@ -147,8 +168,9 @@ int cgfdh_set_kernel_variant(cgfdh_solver *s, int variant);
 int cgfdh_compute_metric(cgfdh_solver *s, const float *xyz);
 
 /* Free-surface matrices from the device metric and medium, as
- * sv_curv_col_el_iso_dvh2dvz (sv_curv_col_el_iso.c:1259-1375), which the
- * reference runs at the start of its time loop (drv_rk_curv_col.c:132-159). */
+ * sv_curv_col_el_{iso,vti,aniso}_dvh2dvz (sv_curv_col_el_iso.c:1259-1375,
+ * sv_curv_col_el_vti.c:1085-1203, sv_curv_col_el_aniso.c:1267-1417), which
+ * the reference runs at the start of its time loop (drv_rk_curv_col.c:132-159). */
 int cgfdh_compute_free_surface(cgfdh_solver *s);
 
 int cgfdh_get_metric(cgfdh_solver *s, float *metric10);

@@ -1,6 +1,7 @@
 /*
- * cgfd_oracle.c -- CPU restatement of the reference's isotropic elastic
- * hot path.  TEST INFRASTRUCTURE ONLY (see cgfd_oracle.h).
+ * cgfd_oracle.c -- CPU restatement of the reference's elastic hot path
+ * (isotropic, VTI and general anisotropic media).  TEST INFRASTRUCTURE ONLY
+ * (see cgfd_oracle.h).
  *
  * Every function follows the cited reference loop expression by expression,
  * including where the reference evaluates in double and rounds to float, so
@@ -174,6 +175,141 @@ static void mul3(float A[3][3], float B[3][3], float C[3][3])
     for (int j = 0; j < 3; j++) {
       C[i][j] = 0.0;
       for (int k = 0; k < 3; k++) C[i][j] += A[i][k] * B[k][j];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* VTI and general anisotropic stress-velocity coupling                     */
+/* ------------------------------------------------------------------------ */
+
+/* Stress rows in Voigt order Txx,Tyy,Tzz,Tyz,Txz,Txy (wavefield slots 3..8).
+ * For one direction with metric (e1,e2,e3), the rate of stress row t holds
+ * the derivative of V_b times a coefficient K[t][b]:
+ *   anisotropic: K = c[a0]*e1 + c[a1]*e2 + c[a2]*e3 with a = aniso_ix[t][b]
+ *     into c11,c12,c13,c14,c15,c16,c22,c23,c24,c25,c26,c33,c34,c35,c36,c44,
+ *     c45,c46,c55,c56,c66 (sv_curv_col_el_aniso.c:468-495);
+ *   VTI: K = c[vti_c]*e[vti_e] into {c11, c12 = c11 - 2 c66, c13, c33, c55,
+ *     c66}, no term where vti_c < 0 (sv_curv_col_el_vti.c:453-475). */
+static const int aniso_ix[6][3][3] = {
+  {{0, 5, 4}, {5, 1, 3}, {4, 3, 2}},
+  {{1, 10, 9}, {10, 6, 8}, {9, 8, 7}},
+  {{2, 14, 13}, {14, 7, 12}, {13, 12, 11}},
+  {{3, 17, 16}, {17, 8, 15}, {16, 15, 12}},
+  {{4, 19, 18}, {19, 9, 16}, {18, 16, 13}},
+  {{5, 20, 19}, {20, 10, 17}, {19, 17, 14}}};
+static const int vti_c[6][3] = {{0, 1, 2}, {1, 0, 2}, {2, 2, 3}, {-1, 4, 4}, {4, -1, 4}, {5, 5, -1}};
+static const int vti_e[6][3] = {{0, 1, 2}, {0, 1, 2}, {0, 1, 2}, {0, 2, 1}, {2, 0, 0}, {1, 0, 0}};
+/* rows of the traction on a zeta surface: (Txx,Txy,Txz), (Txy,Tyy,Tyz),
+ * (Txz,Tyz,Tzz) against zeta_x, zeta_y, zeta_z */
+static const int trac_row[3][3] = {{0, 5, 4}, {5, 1, 3}, {4, 3, 2}};
+
+static inline int has_term(int mt, int t, int b) { return mt != ORC_MEDIUM_VTI || vti_c[t][b] >= 0; }
+
+int orc_medium_ncmp(int mt)
+{
+  return mt == ORC_MEDIUM_VTI ? 6 : (mt == ORC_MEDIUM_ANISO ? 22 : 3);
+}
+
+#define DEFINE_COUPLING(NAME, T)                                                  \
+  static void NAME(int mt, const T *c, const T e[3], T K[6][3])                   \
+  {                                                                               \
+    for (int t = 0; t < 6; t++)                                                   \
+      for (int b = 0; b < 3; b++) {                                               \
+        if (mt == ORC_MEDIUM_VTI) {                                               \
+          K[t][b] = vti_c[t][b] < 0 ? (T)0 : c[vti_c[t][b]] * e[vti_e[t][b]];     \
+        } else {                                                                  \
+          const int *a = aniso_ix[t][b];                                          \
+          K[t][b] = c[a[0]] * e[0] + c[a[1]] * e[1] + c[a[2]] * e[2];             \
+        }                                                                         \
+      }                                                                           \
+  }                                                                               \
+  /* point medium: VTI {c11,c12,c13,c33,c55,c66}, anisotropic the 21 Cij */       \
+  static void NAME##_medium(int mt, const T *med, size_t V, size_t p, T *c)       \
+  {                                                                               \
+    if (mt == ORC_MEDIUM_VTI) {                                                   \
+      T c11 = med[p], c13 = med[V + p], c33 = med[2 * V + p];                     \
+      T c55 = med[3 * V + p], c66 = med[4 * V + p];                               \
+      c[0] = c11; c[1] = c11 - 2.0 * c66; c[2] = c13;                             \
+      c[3] = c33; c[4] = c55; c[5] = c66;                                         \
+    } else {                                                                      \
+      for (int q = 0; q < 21; q++) c[q] = med[q * V + p];                         \
+    }                                                                             \
+  }
+DEFINE_COUPLING(coupling, real_t)
+DEFINE_COUPLING(coupling_f, float)
+
+/* the six stress rates: one left-to-right sum over directions d and velocity
+ * components b of K_d[t][b] * D[d][b], the order the reference writes them */
+static void stress_rates(int mt, const real_t *c, int ndir, const real_t (*e)[3],
+                         const real_t (*D)[3], real_t out[6])
+{
+  real_t K[3][6][3];
+  for (int d = 0; d < ndir; d++) coupling(mt, c, e[d], K[d]);
+  for (int t = 0; t < 6; t++) {
+    real_t acc = 0;
+    int first = 1;
+    for (int d = 0; d < ndir; d++)
+      for (int b = 0; b < 3; b++) {
+        if (!has_term(mt, t, b)) continue;
+        real_t term = K[d][t][b] * D[d][b];
+        acc = first ? term : acc + term;
+        first = 0;
+      }
+    out[t] = acc;
+  }
+}
+
+/* traction-free matrices for VTI / anisotropic media: A[i][j] sums
+ * K_zeta[row][j] * zeta_m, B and C the same with K_xi and K_eta; stored as
+ * -A^-1 B and -A^-1 C (sv_curv_col_el_vti.c:1159-1199,
+ * sv_curv_col_el_aniso.c:1377-1413) */
+void orc_dvh2dvz(int mt, int ni, int nj, int nk, const float *metric10,
+                 const float *med, float *mats)
+{
+  if (mt != ORC_MEDIUM_VTI && mt != ORC_MEDIUM_ANISO) {
+    orc_dvh2dvz_iso(ni, nj, nk, metric10, med, mats);
+    return;
+  }
+  size_t nx = ni + 6, ny = nj + 6, nz = nk + 6;
+  size_t vol = nx * ny * nz, ss = nx * ny;
+  float *mVx = mats, *mVy = mats + ss * 9;
+  size_t k = ORC_NGHOST + nk - 1;
+  for (size_t j = ORC_NGHOST; j < ORC_NGHOST + (size_t)nj; j++)
+    for (size_t i = ORC_NGHOST; i < ORC_NGHOST + (size_t)ni; i++) {
+      size_t p = i + j * nx + k * ss;
+      float c[21], xi[3], et[3], zt[3];
+      coupling_f_medium(mt, med, vol, p, c);
+      for (int m = 0; m < 3; m++) {
+        xi[m] = metric10[(1 + m) * vol + p];
+        et[m] = metric10[(4 + m) * vol + p];
+        zt[m] = metric10[(7 + m) * vol + p];
+      }
+      float Kx[6][3], Ky[6][3], Kz[6][3];
+      coupling_f(mt, c, xi, Kx);
+      coupling_f(mt, c, et, Ky);
+      coupling_f(mt, c, zt, Kz);
+      float A[3][3], B[3][3], C[3][3], AB[3][3], AC[3][3];
+      for (int r = 0; r < 3; r++)
+        for (int b = 0; b < 3; b++) {
+          int first = 1;
+          for (int m = 0; m < 3; m++) {
+            int t = trac_row[r][m];
+            if (!has_term(mt, t, b)) continue;
+            float a = Kz[t][b] * zt[m], bb = Kx[t][b] * zt[m], cc = Ky[t][b] * zt[m];
+            if (first) { A[r][b] = a; B[r][b] = bb; C[r][b] = cc; }
+            else { A[r][b] += a; B[r][b] += bb; C[r][b] += cc; }
+            first = 0;
+          }
+        }
+      inv3(A);
+      mul3(A, B, AB);
+      mul3(A, C, AC);
+      size_t ij = (j * nx + i) * 9;
+      for (int r = 0; r < 3; r++)
+        for (int b = 0; b < 3; b++) {
+          mVx[ij + r * 3 + b] = -1.0f * AB[r][b];
+          mVy[ij + r * 3 + b] = -1.0f * AC[r][b];
+        }
     }
 }
 
@@ -367,7 +503,9 @@ struct orc_s {
   size_t vol, sl, ss;
   int ni1, ni2, nj1, nj2, nk1, nk2;
   real_t *metric;           /* 10 volumes */
-  real_t *med;              /* lambda, mu, slw */
+  int mt, ncmp;             /* medium type, volumes per medium */
+  real_t *med;              /* ncmp volumes, slowness last */
+  const real_t *slw;
   real_t *mats;             /* matVx2Vz2, matVy2Vz2, matF2Vz2 */
   int region[6][6];
   size_t aux_off[6], aux_n[6], aux_total;
@@ -379,25 +517,28 @@ struct orc_s {
 size_t orc_aux_len(const orc_t *o) { return o->aux_total; }
 size_t orc_vol(const orc_t *o) { return o->vol; }
 
-orc_t *orc_create(const orc_cfg *cfg, const float *metric10, const float *med3)
+orc_t *orc_create(const orc_cfg *cfg, const float *metric10, const float *med)
 {
   orc_t *o = (orc_t *)calloc(1, sizeof(orc_t));
   o->cfg = *cfg;
+  o->mt = cfg->medium_type ? cfg->medium_type : ORC_MEDIUM_ISO;
+  o->ncmp = orc_medium_ncmp(o->mt);
   o->nx = cfg->ni + 6; o->ny = cfg->nj + 6; o->nz = cfg->nk + 6;
   o->sl = o->nx; o->ss = (size_t)o->nx * o->ny; o->vol = o->ss * o->nz;
   o->ni1 = 3; o->ni2 = 3 + cfg->ni - 1;
   o->nj1 = 3; o->nj2 = 3 + cfg->nj - 1;
   o->nk1 = 3; o->nk2 = 3 + cfg->nk - 1;
   o->metric = (real_t *)malloc(10 * o->vol * sizeof(real_t));
-  o->med = (real_t *)malloc(3 * o->vol * sizeof(real_t));
+  o->med = (real_t *)malloc(o->ncmp * o->vol * sizeof(real_t));
   for (size_t p = 0; p < 10 * o->vol; p++) o->metric[p] = metric10[p];
-  for (size_t p = 0; p < 3 * o->vol; p++) o->med[p] = med3[p];
+  for (size_t p = 0; p < o->ncmp * o->vol; p++) o->med[p] = med[p];
+  o->slw = o->med + (o->ncmp - 1) * o->vol;
   /* free-surface matrices, computed in the oracle's own precision */
   size_t nm = o->ss * 9;
   o->mats = (real_t *)calloc(3 * nm, sizeof(real_t));
   if (cfg->free_top) {
     float *m = (float *)calloc(3 * nm, sizeof(float));
-    orc_dvh2dvz_iso(cfg->ni, cfg->nj, cfg->nk, metric10, med3, m);
+    orc_dvh2dvz(o->mt, cfg->ni, cfg->nj, cfg->nk, metric10, med, m);
     for (size_t p = 0; p < 3 * nm; p++) o->mats[p] = m[p];
     free(m);
     o->pg = (real_t *)calloc(15 * o->ss, sizeof(real_t));
@@ -485,9 +626,17 @@ static void rhs_inner(orc_t *o, const real_t *w, real_t *h,
         real_t xix = M[V + p], xiy = M[2 * V + p], xiz = M[3 * V + p];
         real_t etx = M[4 * V + p], ety = M[5 * V + p], etz = M[6 * V + p];
         real_t ztx = M[7 * V + p], zty = M[8 * V + p], ztz = M[9 * V + p];
-        real_t lam = md[p], mu = md[V + p], slw = md[2 * V + p];
-        real_t lam2mu = lam + 2.0 * mu;
+        real_t slw = o->slw[p];
         real_t (*X) = D[0], (*Y) = D[1], (*Z) = D[2];
+        if (o->mt != ORC_MEDIUM_ISO) {
+          /* sv_curv_col_el_vti.c:453-475, sv_curv_col_el_aniso.c:455-491 */
+          real_t c[21], e[3][3], Dv[3][3], out[6];
+          coupling_medium(o->mt, md, V, p, c);
+          for (int d = 0; d < 3; d++)
+            for (int m = 0; m < 3; m++) { e[d][m] = M[(1 + 3 * d + m) * V + p]; Dv[d][m] = D[d][m]; }
+          stress_rates(o->mt, c, 3, (const real_t (*)[3])e, (const real_t (*)[3])Dv, out);
+          for (int t = 0; t < 6; t++) h[(TXX + t) * V + p] = out[t];
+        }
         h[VX * V + p] = slw * (xix * X[TXX] + xiy * X[TXY] + xiz * X[TXZ]
                               + etx * Y[TXX] + ety * Y[TXY] + etz * Y[TXZ]
                               + ztx * Z[TXX] + zty * Z[TXY] + ztz * Z[TXZ]);
@@ -497,6 +646,9 @@ static void rhs_inner(orc_t *o, const real_t *w, real_t *h,
         h[VZ * V + p] = slw * (xix * X[TXZ] + xiy * X[TYZ] + xiz * X[TZZ]
                               + etx * Y[TXZ] + ety * Y[TYZ] + etz * Y[TZZ]
                               + ztx * Z[TXZ] + zty * Z[TYZ] + ztz * Z[TZZ]);
+        if (o->mt != ORC_MEDIUM_ISO) continue;
+        real_t lam = md[p], mu = md[V + p];
+        real_t lam2mu = lam + 2.0 * mu;
         h[TXX * V + p] = lam2mu * (xix * X[VX] + etx * Y[VX] + ztx * Z[VX])
                        + lam * (xiy * X[VY] + ety * Y[VY] + zty * Z[VY]
                                 + xiz * X[VZ] + etz * Y[VZ] + ztz * Z[VZ]);
@@ -524,7 +676,7 @@ static void rhs_timg(orc_t *o, const real_t *w, real_t *h,
                      const op_t *ox, const op_t *oy, const op_t *oz, const int *zidx)
 {
   size_t V = o->vol;
-  const real_t *M = o->metric, *md = o->med;
+  const real_t *M = o->metric;
   const real_t *jac = M;
   int zlen = oz->len;
   int k_min = o->nk2 - zidx[zlen - 1];
@@ -536,7 +688,7 @@ static void rhs_timg(orc_t *o, const real_t *w, real_t *h,
     for (int j = o->nj1; j <= o->nj2; j++)
       for (int i = o->ni1; i <= o->ni2; i++) {
         size_t p = i + j * o->sl + k * o->ss;
-        real_t slwjac = md[2 * V + p] / jac[p];
+        real_t slwjac = o->slw[p] / jac[p];
         for (int comp = 0; comp < 3; comp++) {
           const real_t *T0 = w + trow[comp][0] * V, *T1 = w + trow[comp][1] * V,
                        *T2 = w + trow[comp][2] * V;
@@ -615,8 +767,6 @@ static void rhs_vlow(orc_t *o, const real_t *w, real_t *h,
         size_t p = i + j * o->sl + k * o->ss;
         real_t m9[9];
         for (int q = 0; q < 9; q++) m9[q] = M[(1 + q) * V + p];
-        real_t lam = md[p], mu = md[V + p];
-        real_t lam2mu = lam + 2.0 * mu;
         real_t X[3], Y[3], Z[3];
         for (int c = 0; c < 3; c++) {
           X[c] = dop(ox, w + c * V, p);
@@ -630,7 +780,18 @@ static void rhs_vlow(orc_t *o, const real_t *w, real_t *h,
         } else {
           for (int c = 0; c < 3; c++) Z[c] = dop(&ozl[n], w + c * V, p);
         }
-        hooke(h, V, p, m9, lam, lam2mu, mu, X, Y, Z);
+        if (o->mt == ORC_MEDIUM_ISO) {
+          real_t lam = md[p], mu = md[V + p];
+          real_t lam2mu = lam + 2.0 * mu;
+          hooke(h, V, p, m9, lam, lam2mu, mu, X, Y, Z);
+        } else {
+          /* sv_curv_col_el_vti.c:566-586, sv_curv_col_el_aniso.c:676-702 */
+          real_t c[21], Dv[3][3], out[6];
+          coupling_medium(o->mt, md, V, p, c);
+          for (int m = 0; m < 3; m++) { Dv[0][m] = X[m]; Dv[1][m] = Y[m]; Dv[2][m] = Z[m]; }
+          stress_rates(o->mt, c, 3, (const real_t (*)[3])m9, (const real_t (*)[3])Dv, out);
+          for (int t = 0; t < 6; t++) h[(TXX + t) * V + p] = out[t];
+        }
       }
   }
 }
@@ -662,20 +823,32 @@ static void rhs_pml(orc_t *o, const real_t *w, real_t *h, const real_t *aux,
           real_t cD = o->pD[f][ic], cA = o->pA[f][ic], cB = o->pB[f][ic];
           real_t cBm1 = cB - 1.0;
           real_t e1 = mt[p], e2 = mt[V + p], e3 = mt[2 * V + p];
-          real_t lam = md[p], mu = md[V + p], slw = md[2 * V + p];
-          real_t lam2mu = lam + 2.0 * mu;
+          real_t slw = o->slw[p];
+          real_t lam = 0, mu = 0, lam2mu = 0, cm[21];
+          if (o->mt == ORC_MEDIUM_ISO) {
+            lam = md[p]; mu = md[V + p];
+            lam2mu = lam + 2.0 * mu;
+          } else {
+            coupling_medium(o->mt, md, V, p, cm);
+          }
           real_t D[9];
           for (int c = 0; c < 9; c++) D[c] = dop(op, w + c * V, p);
           real_t rr[9];
           rr[VX] = slw * (e1 * D[TXX] + e2 * D[TXY] + e3 * D[TXZ]);
           rr[VY] = slw * (e1 * D[TXY] + e2 * D[TYY] + e3 * D[TYZ]);
           rr[VZ] = slw * (e1 * D[TXZ] + e2 * D[TYZ] + e3 * D[TZZ]);
-          rr[TXX] = lam2mu * e1 * D[VX] + lam * e2 * D[VY] + lam * e3 * D[VZ];
-          rr[TYY] = lam * e1 * D[VX] + lam2mu * e2 * D[VY] + lam * e3 * D[VZ];
-          rr[TZZ] = lam * e1 * D[VX] + lam * e2 * D[VY] + lam2mu * e3 * D[VZ];
-          rr[TXY] = mu * (e2 * D[VX] + e1 * D[VY]);
-          rr[TXZ] = mu * (e3 * D[VX] + e1 * D[VZ]);
-          rr[TYZ] = mu * (e3 * D[VY] + e2 * D[VZ]);
+          if (o->mt == ORC_MEDIUM_ISO) {
+            rr[TXX] = lam2mu * e1 * D[VX] + lam * e2 * D[VY] + lam * e3 * D[VZ];
+            rr[TYY] = lam * e1 * D[VX] + lam2mu * e2 * D[VY] + lam * e3 * D[VZ];
+            rr[TZZ] = lam * e1 * D[VX] + lam * e2 * D[VY] + lam2mu * e3 * D[VZ];
+            rr[TXY] = mu * (e2 * D[VX] + e1 * D[VY]);
+            rr[TXZ] = mu * (e3 * D[VX] + e1 * D[VZ]);
+            rr[TYZ] = mu * (e3 * D[VY] + e2 * D[VZ]);
+          } else {
+            /* one direction of the Hooke sums (sv_curv_col_el_vti.c:757-765) */
+            real_t e[1][3] = {{e1, e2, e3}}, Dv[1][3] = {{D[VX], D[VY], D[VZ]}};
+            stress_rates(o->mt, cm, 1, (const real_t (*)[3])e, (const real_t (*)[3])Dv, rr + TXX);
+          }
           /* reference update order: Vx,Vy,Vz,Txx,Tyy,Tzz,Txz,Tyz,Txy */
           static const int ord[9] = {VX, VY, VZ, TXX, TYY, TZZ, TXZ, TYZ, TXY};
           for (int q = 0; q < 9; q++) {
@@ -695,12 +868,18 @@ static void rhs_pml(orc_t *o, const real_t *w, real_t *h, const real_t *aux,
                       + mm[ij + 3 * rr3 + 2] * D[VZ];
             real_t ztx = M[7 * V + p], zty = M[8 * V + p], ztz = M[9 * V + p];
             real_t s[9];
-            s[TXX] = lam2mu * (ztx * Dz[0]) + lam * (zty * Dz[1] + ztz * Dz[2]);
-            s[TYY] = lam2mu * (zty * Dz[1]) + lam * (ztx * Dz[0] + ztz * Dz[2]);
-            s[TZZ] = lam2mu * (ztz * Dz[2]) + lam * (ztx * Dz[0] + zty * Dz[1]);
-            s[TXY] = mu * (zty * Dz[0] + ztx * Dz[1]);
-            s[TXZ] = mu * (ztz * Dz[0] + ztx * Dz[2]);
-            s[TYZ] = mu * (ztz * Dz[1] + zty * Dz[2]);
+            if (o->mt == ORC_MEDIUM_ISO) {
+              s[TXX] = lam2mu * (ztx * Dz[0]) + lam * (zty * Dz[1] + ztz * Dz[2]);
+              s[TYY] = lam2mu * (zty * Dz[1]) + lam * (ztx * Dz[0] + ztz * Dz[2]);
+              s[TZZ] = lam2mu * (ztz * Dz[2]) + lam * (ztx * Dz[0] + zty * Dz[1]);
+              s[TXY] = mu * (zty * Dz[0] + ztx * Dz[1]);
+              s[TXZ] = mu * (ztz * Dz[0] + ztx * Dz[2]);
+              s[TYZ] = mu * (ztz * Dz[1] + zty * Dz[2]);
+            } else {
+              /* sv_curv_col_el_vti.c:817-822 */
+              real_t e[1][3] = {{ztx, zty, ztz}}, Dv[1][3] = {{Dz[0], Dz[1], Dz[2]}};
+              stress_rates(o->mt, cm, 1, (const real_t (*)[3])e, (const real_t (*)[3])Dv, s + TXX);
+            }
             static const int ordT[6] = {TXX, TYY, TZZ, TXZ, TYZ, TXY};
             for (int q = 0; q < 6; q++) {
               int c = ordT[q];

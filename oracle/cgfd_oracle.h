@@ -13,7 +13,12 @@
  * x fastest, nx = ni+6, ny = nj+6, nz = nk+6, iptr = i + j*nx + k*nx*ny;
  * wavefield components Vx,Vy,Vz,Txx,Tyy,Tzz,Tyz,Txz,Txy, one volume each;
  * metric jac,xi_x,xi_y,xi_z,eta_x,eta_y,eta_z,zeta_x,zeta_y,zeta_z;
- * isotropic medium lambda, mu, slowness (1/rho, md_t.c:602-623).
+ * medium volumes (md_t.c:28-260, density replaced by slowness 1/rho as
+ * md_rho_to_slow does, md_t.c:602-623):
+ *   isotropic (2):  lambda, mu, slw
+ *   VTI (3):        c11, c13, c33, c55, c66, slw
+ *   anisotropic (4): c11,c12,c13,c14,c15,c16,c22,c23,c24,c25,c26,c33,c34,c35,
+ *                    c36,c44,c45,c46,c55,c56,c66, slw
  */
 #ifndef CGFD_ORACLE_H
 #define CGFD_ORACLE_H
@@ -26,6 +31,11 @@
 typedef ORC_REAL real_t;
 
 #define ORC_NGHOST 3
+
+/* medium types, constants.h:22-26 */
+#define ORC_MEDIUM_ISO   2
+#define ORC_MEDIUM_VTI   3
+#define ORC_MEDIUM_ANISO 4
 
 /* face order matches the reference's idim*2+iside: x1,x2,y1,y2,z1,z2 */
 typedef struct {
@@ -47,6 +57,7 @@ typedef struct {
   const int *src_it_end;
   int src_max_nt;
   const float *src_mij;    /* Mxx,Myy,Mzz,Mxz,Myz,Mxy, each [nsrc][max_nt][4] */
+  int medium_type;         /* ORC_MEDIUM_*; 0 is read as isotropic */
 } orc_cfg;
 
 typedef struct orc_s orc_t;
@@ -66,6 +77,11 @@ void orc_pml_region(int ni, int nj, int nk, const int pml_layers[6],
                     const int pml_on[6], int region[6][6]);
 void orc_dvh2dvz_iso(int ni, int nj, int nk, const float *metric10,
                      const float *med3, float *mats /* 3 x nx*ny*9 */);
+/* VTI and anisotropic matrices (sv_curv_col_el_vti.c:1085-1203,
+ * sv_curv_col_el_aniso.c:1267-1417); the third matrix is left untouched */
+void orc_dvh2dvz(int medium_type, int ni, int nj, int nk, const float *metric10,
+                 const float *med, float *mats);
+int orc_medium_ncmp(int medium_type);
 void orc_metric(int ni, int nj, int nk, const float *xyz /* 3 volumes */,
                 float *metric10);
 void orc_gauss_weights(float *w, float x0, float y0, float z0, float a,
@@ -74,7 +90,7 @@ float orc_ricker(float t, float fc, float t0);
 void orc_angle2moment(float strike, float dip, float rake, float m[6]);
 
 /* ---- solver ---- */
-orc_t *orc_create(const orc_cfg *cfg, const float *metric10, const float *med3);
+orc_t *orc_create(const orc_cfg *cfg, const float *metric10, const float *med);
 void   orc_destroy(orc_t *o);
 size_t orc_aux_len(const orc_t *o);           /* floats of all PML aux, 1 level */
 size_t orc_vol(const orc_t *o);

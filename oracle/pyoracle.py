@@ -25,7 +25,11 @@ class OrcCfg(C.Structure):
                 ("nsrc", C.c_int), ("src_gauss", C.c_int), ("src_ext_half", C.c_int),
                 ("src_ext_coef", C.c_float),
                 ("src_ijk", _P), ("src_inc", _P), ("src_it_begin", _P), ("src_it_end", _P),
-                ("src_max_nt", C.c_int), ("src_mij", _P)]
+                ("src_max_nt", C.c_int), ("src_mij", _P), ("medium_type", C.c_int)]
+
+
+MEDIUM_ISO, MEDIUM_VTI, MEDIUM_ANISO = 2, 3, 4
+MEDIUM_NCMP = {MEDIUM_ISO: 3, MEDIUM_VTI: 6, MEDIUM_ANISO: 22}
 
 
 class RefCfg(C.Structure):
@@ -34,7 +38,8 @@ class RefCfg(C.Structure):
                 ("pml_alpha", C.c_float * 6), ("pml_beta", C.c_float * 6),
                 ("pml_vel", C.c_float * 6),
                 ("nsrc", C.c_int), ("src_max_nt", C.c_int), ("src_ijk", _P),
-                ("src_it_begin", _P), ("src_it_end", _P), ("src_mij", _P)]
+                ("src_it_begin", _P), ("src_it_end", _P), ("src_mij", _P),
+                ("medium_type", C.c_int)]
 
 
 def load_oracle(fp64=False):
@@ -57,6 +62,7 @@ def load_oracle(fp64=False):
     lib.orc_pml_avg_dh.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, C.c_int, _P]
     lib.orc_pml_region.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P, _P]
     lib.orc_dvh2dvz_iso.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P, _P]
+    lib.orc_dvh2dvz.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P]
     lib.orc_metric.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P]
     lib.orc_gauss_weights.argtypes = [_P, C.c_float, C.c_float, C.c_float, C.c_float,
                                       C.c_int, C.c_int]
@@ -75,11 +81,11 @@ def load_ref():
     lib.refh_fd_ops.restype = C.c_int
     lib.refh_fd_ops.argtypes = [_P] * 6
     lib.refh_pml.argtypes = [_P, _P, C.c_int, _P, _P, _P, _P]
-    lib.refh_dvh2dvz_iso.argtypes = [_P, _P, _P, _P]
-    lib.refh_iso_onestage.restype = C.c_long
-    lib.refh_iso_onestage.argtypes = [_P, _P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]
-    lib.refh_iso_run.argtypes = [_P, _P, _P, _P, C.c_float, C.c_int, _P, C.c_int, _P, _P,
-                                 _P, _P, _P, _P]
+    lib.refh_dvh2dvz.argtypes = [_P, _P, _P, _P]
+    lib.refh_onestage.restype = C.c_long
+    lib.refh_onestage.argtypes = [_P, _P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]
+    lib.refh_run.argtypes = [_P, _P, _P, _P, C.c_float, C.c_int, _P, C.c_int, _P, _P,
+                             _P, _P, _P, _P]
     return lib
 
 
@@ -87,7 +93,7 @@ class Oracle:
     """One oracle solver instance over a fixed grid, medium and boundary set.
 
     case: a dict built by tests (see tests/cases.py) holding ni,nj,nk, metric,
-    med, free_top, pml_on, pml_layers, pml A/B/D tables and the source tables.
+    medium_type and med (its volumes, slowness last), free_top, pml_on, pml_layers, pml A/B/D tables and the source tables.
     """
 
     def __init__(self, case, fp64=False):
@@ -97,6 +103,7 @@ class Oracle:
         cfg = OrcCfg()
         cfg.ni, cfg.nj, cfg.nk = case["ni"], case["nj"], case["nk"]
         cfg.free_top = int(case.get("free_top", 0))
+        cfg.medium_type = int(case.get("medium_type", MEDIUM_ISO))
         self._keep = []
         for f in range(6):
             on = int(case.get("pml_on", [0] * 6)[f])

@@ -1,6 +1,7 @@
 /*
  * ref_harness.c -- test-only driver around the reference's own hot-path
- * sources (fd_t.c, sv_curv_col_el.c, sv_curv_col_el_iso.c, bdry_t.c, wav_t.c,
+ * sources (fd_t.c, sv_curv_col_el.c, sv_curv_col_el_iso.c,
+ * sv_curv_col_el_vti.c, sv_curv_col_el_aniso.c, bdry_t.c, wav_t.c,
  * fdlib_math.c, fdlib_mem.c), compiled unchanged from /root/reference by
  * oracle/Makefile into oracle/_ref/libcgfd3d_ref.so.
  *
@@ -33,6 +34,8 @@
 #include "bdry_t.h"
 #include "sv_curv_col_el.h"
 #include "sv_curv_col_el_iso.h"
+#include "sv_curv_col_el_vti.h"
+#include "sv_curv_col_el_aniso.h"
 
 typedef struct {
   int ni, nj, nk;
@@ -49,6 +52,10 @@ typedef struct {
   const int *src_it_begin;
   const int *src_it_end;
   const float *src_mij;   /* 6 arrays back to back: Mxx,Myy,Mzz,Mxz,Myz,Mxy */
+  /* CONST_MEDIUM_ELASTIC_ISO (or 0), _VTI or _ANISO; the medium array holds
+   * lambda,mu,slw / c11,c13,c33,c55,c66,slw / the 21 Cij (c11,c12,..,c66) and
+   * slw, one volume each */
+  int medium_type;
 } refh_cfg;
 
 typedef struct {
@@ -80,8 +87,13 @@ static void fill_gd(gd_t *gd, int ni, int nj, int nk,
   gd->x3d = x3d; gd->y3d = y3d; gd->z3d = z3d;
 }
 
+static int medium_of(const refh_cfg *cfg)
+{
+  return cfg->medium_type ? cfg->medium_type : CONST_MEDIUM_ELASTIC_ISO;
+}
+
 static void setup(refh_ctx *c, const refh_cfg *cfg, float *coords,
-                  float *metric10, float *med3)
+                  float *metric10, float *med)
 {
   memset(c, 0, sizeof(*c));
   fd_set_macdrp(&c->fd);
@@ -102,8 +114,21 @@ static void setup(refh_ctx *c, const refh_cfg *cfg, float *coords,
   md_t *md = &c->md;
   md->nx = gd->nx; md->ny = gd->ny; md->nz = gd->nz;
   md->siz_iy = gd->siz_iy; md->siz_iz = gd->siz_iz; md->siz_icmp = gd->siz_icmp;
-  md->medium_type = CONST_MEDIUM_ELASTIC_ISO;
-  md->lambda = med3; md->mu = med3 + vol; md->rho = med3 + 2 * vol;
+  md->medium_type = medium_of(cfg);
+  if (md->medium_type == CONST_MEDIUM_ELASTIC_VTI) {
+    float **f[5] = {&md->c11, &md->c13, &md->c33, &md->c55, &md->c66};
+    for (int q = 0; q < 5; q++) *f[q] = med + q * vol;
+    md->rho = med + 5 * vol;
+  } else if (md->medium_type == CONST_MEDIUM_ELASTIC_ANISO) {
+    float **f[21] = {&md->c11, &md->c12, &md->c13, &md->c14, &md->c15, &md->c16,
+                     &md->c22, &md->c23, &md->c24, &md->c25, &md->c26,
+                     &md->c33, &md->c34, &md->c35, &md->c36,
+                     &md->c44, &md->c45, &md->c46, &md->c55, &md->c56, &md->c66};
+    for (int q = 0; q < 21; q++) *f[q] = med + q * vol;
+    md->rho = med + 21 * vol;
+  } else {
+    md->lambda = med; md->mu = med + vol; md->rho = med + 2 * vol;
+  }
 
   wav_init(gd, &c->wav, 4, 0, 0);
 
@@ -228,6 +253,15 @@ static void patch_timg_surface_row(refh_ctx *c, float *w_cur, float *rhs, fd_op_
     }
 }
 
+#define METRIC9(m) (m)->xi_x, (m)->xi_y, (m)->xi_z, (m)->eta_x, (m)->eta_y, (m)->eta_z, \
+                   (m)->zeta_x, (m)->zeta_y, (m)->zeta_z
+#define VTI_C(md) (md)->c11, (md)->c13, (md)->c33, (md)->c55, (md)->c66
+#define ANISO_C(md) (md)->c11, (md)->c12, (md)->c13, (md)->c14, (md)->c15, (md)->c16, \
+                    (md)->c22, (md)->c23, (md)->c24, (md)->c25, (md)->c26, (md)->c33, \
+                    (md)->c34, (md)->c35, (md)->c36, (md)->c44, (md)->c45, (md)->c46, \
+                    (md)->c55, (md)->c56, (md)->c66
+
+/* sv_curv_col_el_{iso,vti,aniso}_onestage: inner -> timg -> vlow -> cfspml -> src */
 static void harness_onestage(refh_ctx *c, float *w_cur, float *rhs, int ipair, int istage,
                              int patch)
 {
@@ -246,14 +280,32 @@ static void harness_onestage(refh_ctx *c, float *w_cur, float *rhs, int ipair, i
   float *hTxx = rhs + wav->Txx_pos, *hTyy = rhs + wav->Tyy_pos, *hTzz = rhs + wav->Tzz_pos;
   float *hTxz = rhs + wav->Txz_pos, *hTyz = rhs + wav->Tyz_pos, *hTxy = rhs + wav->Txy_pos;
   size_t sl = gd->siz_line, ss = gd->siz_slice;
-  sv_curv_col_el_iso_rhs_inner(Vx, Vy, Vz, Txx, Tyy, Tzz, Txz, Tyz, Txy,
-                               hVx, hVy, hVz, hTxx, hTyy, hTzz, hTxz, hTyz, hTxy,
-                               m->xi_x, m->xi_y, m->xi_z, m->eta_x, m->eta_y, m->eta_z,
-                               m->zeta_x, m->zeta_y, m->zeta_z, md->lambda, md->mu, md->rho,
-                               gd->ni1, gd->ni2, gd->nj1, gd->nj2, gd->nk1, gd->nk2, sl, ss,
-                               fx->total_len, fx->indx, fx->coef,
-                               fy->total_len, fy->indx, fy->coef,
-                               fz->total_len, fz->indx, fz->coef, 0, 0);
+  int mt = md->medium_type;
+  if (mt == CONST_MEDIUM_ELASTIC_VTI)
+    sv_curv_col_el_vti_rhs_inner(Vx, Vy, Vz, Txx, Tyy, Tzz, Txz, Tyz, Txy,
+                                 hVx, hVy, hVz, hTxx, hTyy, hTzz, hTxz, hTyz, hTxy, METRIC9(m),
+                                 VTI_C(md), md->rho,
+                                 gd->ni1, gd->ni2, gd->nj1, gd->nj2, gd->nk1, gd->nk2, sl, ss,
+                                 fx->total_len, fx->indx, fx->coef,
+                                 fy->total_len, fy->indx, fy->coef,
+                                 fz->total_len, fz->indx, fz->coef, 0, 0);
+  else if (mt == CONST_MEDIUM_ELASTIC_ANISO)
+    sv_curv_col_el_aniso_rhs_inner(Vx, Vy, Vz, Txx, Tyy, Tzz, Txz, Tyz, Txy,
+                                   hVx, hVy, hVz, hTxx, hTyy, hTzz, hTxz, hTyz, hTxy, METRIC9(m),
+                                   ANISO_C(md), md->rho,
+                                   gd->ni1, gd->ni2, gd->nj1, gd->nj2, gd->nk1, gd->nk2, sl, ss,
+                                   fx->total_len, fx->indx, fx->coef,
+                                   fy->total_len, fy->indx, fy->coef,
+                                   fz->total_len, fz->indx, fz->coef, 0, 0);
+  else
+    sv_curv_col_el_iso_rhs_inner(Vx, Vy, Vz, Txx, Tyy, Tzz, Txz, Tyz, Txy,
+                                 hVx, hVy, hVz, hTxx, hTyy, hTzz, hTxz, hTyz, hTxy,
+                                 m->xi_x, m->xi_y, m->xi_z, m->eta_x, m->eta_y, m->eta_z,
+                                 m->zeta_x, m->zeta_y, m->zeta_z, md->lambda, md->mu, md->rho,
+                                 gd->ni1, gd->ni2, gd->nj1, gd->nj2, gd->nk1, gd->nk2, sl, ss,
+                                 fx->total_len, fx->indx, fx->coef,
+                                 fy->total_len, fy->indx, fy->coef,
+                                 fz->total_len, fz->indx, fz->coef, 0, 0);
   if (c->bdry.is_sides_free[2][1] == 1) {
     sv_curv_col_el_rhs_timg_z2(Txx, Tyy, Tzz, Txz, Tyz, Txy, hVx, hVy, hVz,
                                m->xi_x, m->xi_y, m->xi_z, m->eta_x, m->eta_y, m->eta_z,
@@ -264,16 +316,46 @@ static void harness_onestage(refh_ctx *c, float *w_cur, float *rhs, int ipair, i
                                fy->total_len, fy->indx, fy->coef,
                                fz->total_len, fz->indx, fz->coef, 0, 0);
     if (patch && fz->indx[0] == -1) patch_timg_surface_row(c, w_cur, rhs, fx, fy, fz);
-    sv_curv_col_el_iso_rhs_vlow_z2(Vx, Vy, Vz, hTxx, hTyy, hTzz, hTxz, hTyz, hTxy,
-                                   m->xi_x, m->xi_y, m->xi_z, m->eta_x, m->eta_y, m->eta_z,
-                                   m->zeta_x, m->zeta_y, m->zeta_z, md->lambda, md->mu, md->rho,
-                                   c->bdry.matVx2Vz2, c->bdry.matVy2Vz2, c->bdry.matF2Vz2,
-                                   c->src.VxSrc, c->src.VySrc, c->src.VzSrc,
-                                   gd->ni1, gd->ni2, gd->nj1, gd->nj2, gd->nk1, gd->nk2, sl, ss,
-                                   fx->total_len, fx->indx, fx->coef,
-                                   fy->total_len, fy->indx, fy->coef,
-                                   c->fd.num_of_fdz_op, fzl, c->fd.fdz_max_len, 0, 0);
+    if (mt == CONST_MEDIUM_ELASTIC_VTI)
+      sv_curv_col_el_vti_rhs_vlow_z2(Vx, Vy, Vz, hTxx, hTyy, hTzz, hTxz, hTyz, hTxy, METRIC9(m),
+                                     VTI_C(md), md->rho, c->bdry.matVx2Vz2, c->bdry.matVy2Vz2,
+                                     gd->ni1, gd->ni2, gd->nj1, gd->nj2, gd->nk1, gd->nk2, sl, ss,
+                                     fx->total_len, fx->indx, fx->coef,
+                                     fy->total_len, fy->indx, fy->coef,
+                                     c->fd.num_of_fdz_op, fzl, c->fd.fdz_max_len, 0, 0);
+    else if (mt == CONST_MEDIUM_ELASTIC_ANISO)
+      sv_curv_col_el_aniso_rhs_vlow_z2(Vx, Vy, Vz, hTxx, hTyy, hTzz, hTxz, hTyz, hTxy, METRIC9(m),
+                                       ANISO_C(md), md->rho, c->bdry.matVx2Vz2, c->bdry.matVy2Vz2,
+                                       gd->ni1, gd->ni2, gd->nj1, gd->nj2, gd->nk1, gd->nk2, sl, ss,
+                                       fx->total_len, fx->indx, fx->coef,
+                                       fy->total_len, fy->indx, fy->coef,
+                                       c->fd.num_of_fdz_op, fzl, c->fd.fdz_max_len, 0, 0);
+    else
+      sv_curv_col_el_iso_rhs_vlow_z2(Vx, Vy, Vz, hTxx, hTyy, hTzz, hTxz, hTyz, hTxy,
+                                     m->xi_x, m->xi_y, m->xi_z, m->eta_x, m->eta_y, m->eta_z,
+                                     m->zeta_x, m->zeta_y, m->zeta_z, md->lambda, md->mu, md->rho,
+                                     c->bdry.matVx2Vz2, c->bdry.matVy2Vz2, c->bdry.matF2Vz2,
+                                     c->src.VxSrc, c->src.VySrc, c->src.VzSrc,
+                                     gd->ni1, gd->ni2, gd->nj1, gd->nj2, gd->nk1, gd->nk2, sl, ss,
+                                     fx->total_len, fx->indx, fx->coef,
+                                     fy->total_len, fy->indx, fy->coef,
+                                     c->fd.num_of_fdz_op, fzl, c->fd.fdz_max_len, 0, 0);
   }
+  if (c->bdry.is_enable_pml == 1 && mt == CONST_MEDIUM_ELASTIC_VTI)
+    sv_curv_col_el_vti_rhs_cfspml(Vx, Vy, Vz, Txx, Tyy, Tzz, Txz, Tyz, Txy,
+                                  hVx, hVy, hVz, hTxx, hTyy, hTzz, hTxz, hTyz, hTxy, METRIC9(m),
+                                  VTI_C(md), md->rho, gd->nk2, sl, ss,
+                                  fx->total_len, fx->indx, fx->coef,
+                                  fy->total_len, fy->indx, fy->coef,
+                                  fz->total_len, fz->indx, fz->coef, &c->bdry, 0, 0);
+  else if (c->bdry.is_enable_pml == 1 && mt == CONST_MEDIUM_ELASTIC_ANISO)
+    sv_curv_col_el_aniso_rhs_cfspml(Vx, Vy, Vz, Txx, Tyy, Tzz, Txz, Tyz, Txy,
+                                    hVx, hVy, hVz, hTxx, hTyy, hTzz, hTxz, hTyz, hTxy, METRIC9(m),
+                                    ANISO_C(md), md->rho, gd->nk2, sl, ss,
+                                    fx->total_len, fx->indx, fx->coef,
+                                    fy->total_len, fy->indx, fy->coef,
+                                    fz->total_len, fz->indx, fz->coef, &c->bdry, 0, 0);
+  else
   if (c->bdry.is_enable_pml == 1)
     sv_curv_col_el_iso_rhs_cfspml(Vx, Vy, Vz, Txx, Tyy, Tzz, Txz, Tyz, Txy,
                                   hVx, hVy, hVz, hTxx, hTyy, hTzz, hTxz, hTyz, hTxy,
@@ -345,15 +427,26 @@ int refh_pml(const refh_cfg *cfg, float *coords, int maxlen,
   return 0;
 }
 
-/* ---- free-surface matrices (sv_curv_col_el_iso_dvh2dvz) ----------------- */
+static void ref_dvh2dvz(refh_ctx *c)
+{
+  int mt = c->md.medium_type;
+  if (mt == CONST_MEDIUM_ELASTIC_VTI)
+    sv_curv_col_el_vti_dvh2dvz(&c->gd, &c->metric, &c->md, &c->bdry, 0);
+  else if (mt == CONST_MEDIUM_ELASTIC_ANISO)
+    sv_curv_col_el_aniso_dvh2dvz(&c->gd, &c->metric, &c->md, &c->bdry, 0);
+  else
+    sv_curv_col_el_iso_dvh2dvz(&c->gd, &c->metric, &c->md, &c->bdry, 0);
+}
+
+/* ---- free-surface matrices (sv_curv_col_el_{iso,vti,aniso}_dvh2dvz) ----- */
 /* mats_out: 3 arrays of nx*ny*9 (matVx2Vz2, matVy2Vz2, matF2Vz2) */
-int refh_dvh2dvz_iso(const refh_cfg *cfg, float *metric10, float *med3, float *mats_out)
+int refh_dvh2dvz(const refh_cfg *cfg, float *metric10, float *med3, float *mats_out)
 {
   refh_ctx c;
   refh_cfg nopml = *cfg;   /* no coordinates here: the PML set-up needs them */
   memset(nopml.pml_on, 0, sizeof nopml.pml_on);
   setup(&c, &nopml, NULL, metric10, med3);
-  sv_curv_col_el_iso_dvh2dvz(&c.gd, &c.metric, &c.md, &c.bdry, 0);
+  ref_dvh2dvz(&c);
   size_t n = c.gd.siz_iz * 9;
   memcpy(mats_out, c.bdry.matVx2Vz2, n * sizeof(float));
   memcpy(mats_out + n, c.bdry.matVy2Vz2, n * sizeof(float));
@@ -370,17 +463,17 @@ static size_t aux_total(refh_ctx *c)
   return t;
 }
 
-/* ---- one RHS evaluation (sv_curv_col_el_iso_onestage) ------------------- */
+/* ---- one RHS evaluation (sv_curv_col_el_*_onestage) --------------------- */
 /* aux_cur / aux_rhs_out: the enabled PML faces' 9-component aux arrays, face
  * after face in x1,x2,y1,y2,z1,z2 order.  Returns the aux length in floats. */
-long refh_iso_onestage(const refh_cfg *cfg, float *coords, float *metric10, float *med3,
-                       int it, int ipair, int istage, int patch,
-                       const float *w_cur, const float *aux_cur,
-                       float *rhs_out, float *aux_rhs_out)
+long refh_onestage(const refh_cfg *cfg, float *coords, float *metric10, float *med3,
+                   int it, int ipair, int istage, int patch,
+                   const float *w_cur, const float *aux_cur,
+                   float *rhs_out, float *aux_rhs_out)
 {
   refh_ctx c;
   setup(&c, cfg, coords, metric10, med3);
-  if (cfg->free_top) sv_curv_col_el_iso_dvh2dvz(&c.gd, &c.metric, &c.md, &c.bdry, 0);
+  if (cfg->free_top) ref_dvh2dvz(&c);
   size_t L = c.wav.siz_ilevel;
   float *wcur = c.wav.v5d;          /* level 0 */
   float *rhs = c.wav.v5d + 2 * L;   /* level 2, zero */
@@ -412,14 +505,14 @@ long refh_iso_onestage(const refh_cfg *cfg, float *coords, float *metric10, floa
 /* ---- the time loop around the reference RHS ----------------------------- */
 /* recv_idx: 8 corner 1-D indices per receiver (io_funcs.c:382-389 order);
  * recv_w: di,dj,dk per receiver.  seismo_out: [nrecv][9][nt]. */
-int refh_iso_run(const refh_cfg *cfg, float *coords, float *metric10, float *med3,
-                 float dt, int nt, const float *w_init,
-                 int nrecv, const int *recv_idx, const float *recv_w,
-                 float *w_final, float *seismo_out, float *pg_out, float *dis_out)
+int refh_run(const refh_cfg *cfg, float *coords, float *metric10, float *med3,
+              float dt, int nt, const float *w_init,
+              int nrecv, const int *recv_idx, const float *recv_w,
+              float *w_final, float *seismo_out, float *pg_out, float *dis_out)
 {
   refh_ctx c;
   setup(&c, cfg, coords, metric10, med3);
-  if (cfg->free_top) sv_curv_col_el_iso_dvh2dvz(&c.gd, &c.metric, &c.md, &c.bdry, 0);
+  if (cfg->free_top) ref_dvh2dvz(&c);
   size_t L = c.wav.siz_ilevel, V = c.wav.siz_icmp;
   float *w_pre = c.wav.v5d, *w_tmp = c.wav.v5d + L, *w_rhs = c.wav.v5d + 2 * L,
         *w_end = c.wav.v5d + 3 * L, *w_cur;

@@ -1,5 +1,6 @@
 """Seeded synthetic inputs shared by the parity tests: small curvilinear grids
-with topography, heterogeneous isotropic media, PML tables and sources.
+with topography, heterogeneous isotropic, VTI and general anisotropic media,
+PML tables and sources.
 
 Layout follows the reference (x fastest, 3 ghost points per side).  The metric
 and the PML profile tables are produced by the oracle's restatement of the
@@ -47,6 +48,93 @@ def iso_medium(ni, nj, nk, seed=0, vp=6000.0, vs=3464.0, rho=2670.0, pert=0.05):
     return np.stack([lam, mu, slw]).astype(np.float32)
 
 
+def vti_medium(ni, nj, nk, seed=0, vp=6000.0, vs=3464.0, rho=2670.0, eps=0.12,
+               delta=0.06, gamma=0.09, pert=0.05):
+    """c11,c13,c33,c55,c66,slw from Thomsen parameters with seeded per-point
+    perturbations (vertical axis of symmetry)."""
+    rng = np.random.default_rng(seed + 2)
+    nx, ny, nz = dims(ni, nj, nk)
+    f = 1.0 + pert * rng.standard_normal((6, nz, ny, nx))
+    vp_, vs_, rho_ = vp * f[0], vs * f[1], rho * f[2]
+    e, d, g = eps * f[3], delta * f[4], gamma * f[5]
+    c33 = rho_ * vp_ ** 2
+    c55 = rho_ * vs_ ** 2
+    c11 = c33 * (1 + 2 * e)
+    c66 = c55 * (1 + 2 * g)
+    c13 = np.sqrt((c33 - c55) * (2 * d * c33 + c33 - c55)) - c55
+    return np.stack([c11, c13, c33, c55, c66, 1.0 / rho_]).astype(np.float32)
+
+
+# Voigt index pairs and the 21 upper-triangle entries c11,c12,..,c66
+_VOIGT = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]
+_UPPER = [(a, b) for a in range(6) for b in range(a, 6)]
+
+
+def aniso_medium(ni, nj, nk, seed=0, pert=0.03):
+    """The 21 Cij (c11,c12,..,c66) and slw of a triclinic medium: a VTI
+    stiffness rotated by a seeded per-point rotation, then perturbed
+    symmetrically (small enough to stay positive definite)."""
+    rng = np.random.default_rng(seed + 3)
+    nx, ny, nz = dims(ni, nj, nk)
+    n = nx * ny * nz
+    v = vti_medium(ni, nj, nk, seed=seed).reshape(6, n).astype(np.float64)
+    c11, c13, c33, c55, c66, slw = v
+    C = np.zeros((n, 6, 6))
+    C[:, 0, 0] = C[:, 1, 1] = c11
+    C[:, 0, 1] = C[:, 1, 0] = c11 - 2 * c66
+    C[:, 0, 2] = C[:, 2, 0] = C[:, 1, 2] = C[:, 2, 1] = c13
+    C[:, 2, 2] = c33
+    C[:, 3, 3] = C[:, 4, 4] = c55
+    C[:, 5, 5] = c66
+    # 4th-order tensor, rotate, back to Voigt
+    T = np.zeros((n, 3, 3, 3, 3))
+    for a, (i, j) in enumerate(_VOIGT):
+        for b, (k, l) in enumerate(_VOIGT):
+            for (p, q) in {(i, j), (j, i)}:
+                for (r, s) in {(k, l), (l, k)}:
+                    T[:, p, q, r, s] = C[:, a, b]
+    ang = rng.uniform(-0.6, 0.6, (3, n))
+    ca, sa = np.cos(ang), np.sin(ang)
+    Rz = np.zeros((n, 3, 3)); Rz[:, 0, 0] = ca[0]; Rz[:, 0, 1] = -sa[0]; Rz[:, 1, 0] = sa[0]
+    Rz[:, 1, 1] = ca[0]; Rz[:, 2, 2] = 1
+    Ry = np.zeros((n, 3, 3)); Ry[:, 0, 0] = ca[1]; Ry[:, 0, 2] = sa[1]; Ry[:, 2, 0] = -sa[1]
+    Ry[:, 2, 2] = ca[1]; Ry[:, 1, 1] = 1
+    Rx = np.zeros((n, 3, 3)); Rx[:, 1, 1] = ca[2]; Rx[:, 1, 2] = -sa[2]; Rx[:, 2, 1] = sa[2]
+    Rx[:, 2, 2] = ca[2]; Rx[:, 0, 0] = 1
+    R = Rz @ Ry @ Rx
+    T = np.einsum("nia,njb,nkc,nld,nabcd->nijkl", R, R, R, R, T, optimize=True)
+    out = np.zeros((22, n))
+    scale = np.abs(C).max(axis=(1, 2))
+    noise = pert * rng.standard_normal((21, n)) * scale
+    for m, (a, b) in enumerate(_UPPER):
+        (i, j), (k, l) = _VOIGT[a], _VOIGT[b]
+        out[m] = T[:, i, j, k, l] + (noise[m] if a != b else abs(noise[m]))
+    out[21] = slw
+    return out.reshape(22, nz, ny, nx).astype(np.float32)
+
+
+def vti_as_aniso(vti):
+    """The 21 Cij of a VTI medium written the way the reference's own test
+    medium does (md_t.c:544-590): c12 = c11 - 2 c66, c22 = c11, c23 = c13,
+    c44 = c55, the rest zero."""
+    c11, c13, c33, c55, c66, slw = vti.astype(np.float32)
+    z = np.zeros_like(c11)
+    c12 = (c11.astype(np.float64) - 2.0 * c66.astype(np.float64)).astype(np.float32)
+    comp = dict(c11=c11, c12=c12, c13=c13, c22=c11, c23=c13, c33=c33, c44=c55, c55=c55,
+                c66=c66)
+    names = ["c%d%d" % (a + 1, b + 1) for a, b in _UPPER]
+    return np.stack([comp.get(nm, z) for nm in names] + [slw]).astype(np.float32)
+
+
+def make_medium(kind, ni, nj, nk, seed=0):
+    """(medium_type, volumes) for kind 'iso', 'vti' or 'aniso'."""
+    if kind == "vti":
+        return pyoracle.MEDIUM_VTI, vti_medium(ni, nj, nk, seed=seed)
+    if kind == "aniso":
+        return pyoracle.MEDIUM_ANISO, aniso_medium(ni, nj, nk, seed=seed)
+    return pyoracle.MEDIUM_ISO, iso_medium(ni, nj, nk, seed=seed)
+
+
 def pml_tables(orc, ni, nj, nk, coords, pml_on, pml_layers, vel=6000.0,
                alpha=None, beta=3.0):
     """A/B/D per face with the reference's profile (bdry_t.c:215-272)."""
@@ -78,17 +166,17 @@ def pml_tables(orc, ni, nj, nk, coords, pml_on, pml_layers, vel=6000.0,
 
 
 def make_case(ni=16, nj=14, nk=12, seed=0, free_top=True, pml=(1, 1, 1, 1, 1, 0),
-              layers=4, src=None, dh=100.0, hill=0.3):
+              layers=4, src=None, dh=100.0, hill=0.3, medium="iso"):
     orc = pyoracle.load_oracle()
     nx, ny, nz = dims(ni, nj, nk)
     coords = curvilinear_coords(ni, nj, nk, dh=dh, seed=seed, hill=hill)
     metric = np.zeros((10, nz, ny, nx), np.float32)
     orc.orc_metric(ni, nj, nk, pyoracle.ptr(np.ascontiguousarray(coords)), pyoracle.ptr(metric))
-    med = iso_medium(ni, nj, nk, seed=seed)
+    mtype, med = make_medium(medium, ni, nj, nk, seed=seed)
     pml_on = list(pml)
     pml_layers = [layers if p else 0 for p in pml_on]
     A, B, D, region = pml_tables(orc, ni, nj, nk, coords, pml_on, pml_layers)
-    case = dict(ni=ni, nj=nj, nk=nk, coords=coords, metric=metric, med=med,
+    case = dict(ni=ni, nj=nj, nk=nk, coords=coords, metric=metric, med=med, medium_type=mtype,
                 free_top=int(free_top), pml_on=pml_on, pml_layers=pml_layers,
                 pml_A=A, pml_B=B, pml_D=D, pml_region=region, src=src,
                 pml_vel=6000.0, pml_alpha=np.pi * 1.0, pml_beta=3.0)
@@ -178,6 +266,7 @@ def ref_cfg(case, src_arrays=None):
     c = pyoracle.RefCfg()
     c.ni, c.nj, c.nk = case["ni"], case["nj"], case["nk"]
     c.free_top = int(case["free_top"])
+    c.medium_type = int(case.get("medium_type", pyoracle.MEDIUM_ISO))
     for f in range(6):
         c.pml_on[f] = int(case["pml_on"][f])
         c.pml_layers[f] = int(case["pml_layers"][f])
@@ -199,15 +288,15 @@ def ref_cfg(case, src_arrays=None):
     return c, keep
 
 
-def make_solver(case, dt, nt_total, variant=1):
+def make_solver(case, dt, nt_total, variant=4):
     """The product engine loaded with a case's static fields."""
     from cgfd3d_amd import Solver, SRC_GAUSSIAN, SRC_POINT
+    mtype = int(case.get("medium_type", pyoracle.MEDIUM_ISO))
     s = Solver(case["ni"], case["nj"], case["nk"], dt, nt_total, free_top=bool(case["free_top"]),
-               pml_on=case["pml_on"], pml_layers=case["pml_layers"])
+               pml_on=case["pml_on"], pml_layers=case["pml_layers"], medium_type=mtype)
     s.set_kernel_variant(variant)
     s.set_metric(case["metric"])
-    med = case["med"]
-    s.set_medium_iso(med[0], med[1], med[2])
+    s.set_medium(mtype, case["med"])
     for f in range(6):
         if case["pml_on"][f]:
             s.set_pml_face(f, case["pml_A"][f], case["pml_B"][f], case["pml_D"][f])
@@ -229,9 +318,10 @@ def free_surface_mats(case):
     orc = pyoracle.load_oracle()
     nx, ny, _ = dims(case["ni"], case["nj"], case["nk"])
     mats = np.zeros(3 * nx * ny * 9, np.float32)
-    orc.orc_dvh2dvz_iso(case["ni"], case["nj"], case["nk"],
-                        pyoracle.ptr(np.ascontiguousarray(case["metric"])),
-                        pyoracle.ptr(np.ascontiguousarray(case["med"])), pyoracle.ptr(mats))
+    orc.orc_dvh2dvz(int(case.get("medium_type", pyoracle.MEDIUM_ISO)),
+                    case["ni"], case["nj"], case["nk"],
+                    pyoracle.ptr(np.ascontiguousarray(case["metric"])),
+                    pyoracle.ptr(np.ascontiguousarray(case["med"])), pyoracle.ptr(mats))
     return mats
 
 

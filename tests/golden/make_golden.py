@@ -78,7 +78,7 @@ def main():
             for istage in range(4):
                 r = np.zeros(w.size, np.float32)
                 a = np.zeros(max(aux.size, 1), np.float32)
-                ref.refh_iso_onestage(C.byref(cfg), po.ptr(co), po.ptr(case["metric"]),
+                ref.refh_onestage(C.byref(cfg), po.ptr(co), po.ptr(case["metric"]),
                                       po.ptr(case["med"]), 4, ipair, istage, 1, po.ptr(w),
                                       po.ptr(aux), po.ptr(r), po.ptr(a))
                 rhs[ipair, istage] = r
@@ -105,7 +105,7 @@ def main():
         nx, ny, nz = ni + 6, nj + 6, nk + 6
         wfin = np.zeros(9 * nx * ny * nz, np.float32)
         seis = np.zeros((len(ridx), 9, nt), np.float32)
-        ref.refh_iso_run(C.byref(cfg), po.ptr(co), po.ptr(case["metric"]), po.ptr(case["med"]),
+        ref.refh_run(C.byref(cfg), po.ptr(co), po.ptr(case["metric"]), po.ptr(case["med"]),
                          dt, nt, None, len(ridx), po.ptr(ridx), po.ptr(rw), po.ptr(wfin),
                          po.ptr(seis), None, None)
         W = wfin.reshape(9, nz, ny, nx)

@@ -19,10 +19,16 @@ CONFIGS = [
 ]
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
+# (medium, kernel variant): every variant of the isotropic medium, the
+# per-point and lean-interior variants of the VTI and anisotropic media
+MEDIUM_VARIANTS = ([("iso", v) for v in range(6)] +
+                   [(m, v) for m in ("vti", "aniso") for v in (0, 4, 5)])
+
+
+@pytest.mark.parametrize("medium,variant", MEDIUM_VARIANTS)
 @pytest.mark.parametrize("name,free,pml", CONFIGS)
-def test_onestage_all_pairs(name, free, pml, variant):
-    case = cs.make_case(18, 15, 13, seed=2, free_top=free, pml=pml, layers=4)
+def test_onestage_all_pairs(name, free, pml, medium, variant):
+    case = cs.make_case(18, 15, 13, seed=2, free_top=free, pml=pml, layers=4, medium=medium)
     w = cs.random_wavefield(case, seed=4).reshape(-1)
     aux = cs.random_aux(case, seed=4) if any(pml) else None
     orc = po.Oracle(case)
@@ -38,16 +44,21 @@ def test_onestage_all_pairs(name, free, pml, variant):
                 worst = max(worst, cs.rel_l2(a_g, a_o))
                 mism += int(np.count_nonzero(a_g != a_o))
     s.close()
-    print(f"{name} variant={variant}: worst rel L2 {worst:.3e}, differing values {mism}")
+    print(f"{name} {medium} variant={variant}: worst rel L2 {worst:.3e}, differing values {mism}")
     assert worst <= 1e-6
     assert mism == 0, "expected bit-identical RHS"
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
+RUN_VARIANTS = ([("iso", v) for v in range(6)] +
+                [(m, v) for m in ("vti", "aniso") for v in (0, 4)])
+
+
+@pytest.mark.parametrize("medium,variant", RUN_VARIANTS)
 @pytest.mark.parametrize("gauss", [False, True])
-def test_run_matches_oracle(variant, gauss):
+def test_run_matches_oracle(medium, variant, gauss):
     ni, nj, nk, nt, dt = 26, 22, 20, 60, 0.004
-    case = cs.make_case(ni, nj, nk, seed=5, free_top=True, pml=(1, 1, 1, 1, 1, 0), layers=5)
+    case = cs.make_case(ni, nj, nk, seed=5, free_top=True, pml=(1, 1, 1, 1, 1, 0), layers=5,
+                        medium=medium)
     case["src"] = cs.ricker_point_source(case, 3 + ni // 2, 3 + nj // 2, 3 + nk // 2, nt, dt,
                                          fc=10.0, t0=0.12, gauss=gauss,
                                          inc=(0.2, -0.3, 0.1) if gauss else (0, 0, 0))
@@ -66,26 +77,28 @@ def test_run_matches_oracle(variant, gauss):
     assert pg_o.max() > 0
     assert np.array_equal(pg_g, pg_o) and np.array_equal(dis_g, dis_o), "peak ground motion"
     e_w, e_s, e_a = cs.rel_l2(w_g, w_o), cs.rel_l2(s_g, s_o), cs.rel_l2(a_g, a_o)
-    print(f"gauss={gauss} variant={variant}: rel L2 wave {e_w:.2e} seis {e_s:.2e} aux {e_a:.2e}")
+    print(f"{medium} gauss={gauss} variant={variant}: rel L2 wave {e_w:.2e} seis {e_s:.2e} "
+          f"aux {e_a:.2e}")
     assert np.abs(s_o).max() > 0
     assert e_w <= 1e-5 and e_s <= 1e-5 and e_a <= 1e-5
     assert np.array_equal(w_g, w_o) and np.array_equal(s_g, s_o)
 
 
-def test_device_setup_matches_oracle():
+@pytest.mark.parametrize("medium", ["iso", "vti", "aniso"])
+def test_device_setup_matches_oracle(medium):
     """Metric (gd_curv_metric_cal) and free-surface matrices
-    (sv_curv_col_el_iso_dvh2dvz) built on the device equal the oracle's."""
-    import ctypes as C
+    (sv_curv_col_el_{iso,vti,aniso}_dvh2dvz) built on the device equal the
+    oracle's."""
     from cgfd3d_amd import Solver, workloads
-    case = cs.make_case(21, 17, 15, seed=3, hill=0.4)
+    case = cs.make_case(21, 17, 15, seed=3, hill=0.4, medium=medium)
     L = workloads._setup_sigs()
-    s = Solver(21, 17, 15, 0.004, 4, free_top=True)
+    s = Solver(21, 17, 15, 0.004, 4, free_top=True, medium_type=case["medium_type"])
     xyz = np.ascontiguousarray(case["coords"])
     assert L.cgfdh_compute_metric(s.h, po.ptr(xyz)) == 0
     m = np.zeros_like(case["metric"])
     assert L.cgfdh_get_metric(s.h, po.ptr(m)) == 0
     assert np.array_equal(m, case["metric"])
-    s.set_medium_iso(case["med"][0], case["med"][1], case["med"][2])
+    s.set_medium(case["medium_type"], case["med"])
     assert L.cgfdh_compute_free_surface(s.h) == 0
     mats = np.zeros(3 * 27 * 23 * 9, np.float32)
     assert L.cgfdh_get_free_surface(s.h, po.ptr(mats)) == 0
@@ -165,14 +178,21 @@ def test_full_size_linearity():
     s.close()
     print("run 2 done", flush=True)
     assert np.abs(w1).max() > 0
-    differ = w2 != 2.0 * w1
-    print(f"values not exactly doubled: {int(differ.sum())}, largest |w| among them "
-          f"{float(np.abs(w1[differ]).max()) if differ.any() else 0.0:.3e}")
-    # 2^-100: values this small sit at the subnormal edge after a few products
-    assert not differ.any() or np.abs(w1[differ]).max() < 2.0 ** -100
+    # Products that fall into the subnormal range round with fewer bits, and
+    # the stress-velocity coupling (lambda * metric * dt ~ 1e7 per stage)
+    # carries those last-bit differences upward through the steps; they stay
+    # many orders of magnitude below each component's peak.
+    w1c, w2c = w1.reshape(9, -1), w2.reshape(9, -1)
+    for c in range(9):
+        peak = float(np.abs(w1c[c]).max())
+        differ = w2c[c] != 2.0 * w1c[c]
+        worst = float(np.abs(w1c[c][differ]).max()) if differ.any() else 0.0
+        print(f"component {c}: peak {peak:.3e}, not exactly doubled {int(differ.sum())}, "
+              f"largest among them {worst:.3e}")
+        assert worst <= 1e-15 * peak
     assert cs.rel_l2(w2, 2.0 * w1) <= 1e-12
     rdiff = r2 != 2.0 * r1
-    assert not rdiff.any() or np.abs(r1[rdiff]).max() < 2.0 ** -100
+    assert not rdiff.any() or np.abs(r1[rdiff]).max() <= 1e-15 * np.abs(r1).max()
     wl.src = None
     s = wl.build_solver()
     s.run(0, 3)

@@ -50,23 +50,28 @@ def test_pml_profiles_match_reference(layers):
         assert np.array_equal(D[f], case["pml_D"][f])
 
 
+MEDIA = ["iso", "vti", "aniso"]
+
+
 @needs_ref
-def test_free_surface_matrices_match_reference():
-    case = cs.make_case(13, 12, 10, seed=4, free_top=True)
+@pytest.mark.parametrize("medium", MEDIA)
+def test_free_surface_matrices_match_reference(medium):
+    case = cs.make_case(13, 12, 10, seed=4, free_top=True, medium=medium)
     ref = po.load_ref()
     cfg, keep = cs.ref_cfg(case)
     nx, ny = case["ni"] + 6, case["nj"] + 6
     m_ref = np.zeros(3 * nx * ny * 9, np.float32)
-    ref.refh_dvh2dvz_iso(C.byref(cfg), po.ptr(case["metric"]), po.ptr(case["med"]), po.ptr(m_ref))
+    ref.refh_dvh2dvz(C.byref(cfg), po.ptr(case["metric"]), po.ptr(case["med"]), po.ptr(m_ref))
     assert np.abs(m_ref).max() > 0
     assert np.array_equal(cs.free_surface_mats(case), m_ref)
 
 
 @needs_ref
+@pytest.mark.parametrize("medium", MEDIA)
 @pytest.mark.parametrize("free,pml", [(False, (0,) * 6), (False, (1,) * 6), (True, (0,) * 6),
                                       (True, (1, 1, 1, 1, 1, 0))])
-def test_onestage_matches_reference(free, pml):
-    case = cs.make_case(13, 12, 11, seed=6, free_top=free, pml=pml, layers=3)
+def test_onestage_matches_reference(free, pml, medium):
+    case = cs.make_case(13, 12, 11, seed=6, free_top=free, pml=pml, layers=3, medium=medium)
     case["src"] = cs.ricker_point_source(case, 9, 8, 7, 10, 0.004, fc=10.0, t0=0.05)
     w = cs.random_wavefield(case, seed=1)
     aux = cs.random_aux(case, seed=1)
@@ -79,7 +84,7 @@ def test_onestage_matches_reference(free, pml):
             r_o, a_o = orc.onestage(3, ipair, istage, w.reshape(-1), aux)
             r_r = np.zeros_like(r_o)
             a_r = np.zeros(max(len(a_o), 1), np.float32)
-            ref.refh_iso_onestage(C.byref(cfg), po.ptr(co), po.ptr(case["metric"]),
+            ref.refh_onestage(C.byref(cfg), po.ptr(co), po.ptr(case["metric"]),
                                   po.ptr(case["med"]), 3, ipair, istage, 1, po.ptr(w),
                                   po.ptr(aux), po.ptr(r_r), po.ptr(a_r))
             assert np.array_equal(r_o, r_r), (ipair, istage)
@@ -104,7 +109,7 @@ def test_reference_patch_touches_only_the_undefined_row():
         for patch in (0, 1):
             r = np.zeros(w.size, np.float32)
             a = np.zeros(aux.size, np.float32)
-            ref.refh_iso_onestage(C.byref(cfg), po.ptr(co), po.ptr(case["metric"]),
+            ref.refh_onestage(C.byref(cfg), po.ptr(co), po.ptr(case["metric"]),
                                   po.ptr(case["med"]), 0, ipair, istage, patch, po.ptr(w),
                                   po.ptr(aux), po.ptr(r), po.ptr(a))
             out.append((r.reshape(9, nz, ny, nx), a))
@@ -115,9 +120,11 @@ def test_reference_patch_touches_only_the_undefined_row():
 
 
 @needs_ref
-def test_run_matches_reference():
+@pytest.mark.parametrize("medium", MEDIA)
+def test_run_matches_reference(medium):
     ni, nj, nk, nt, dt = 20, 18, 16, 80, 0.004
-    case = cs.make_case(ni, nj, nk, seed=12, free_top=True, pml=(1, 1, 1, 1, 1, 0), layers=4)
+    case = cs.make_case(ni, nj, nk, seed=12, free_top=True, pml=(1, 1, 1, 1, 1, 0), layers=4,
+                        medium=medium)
     case["src"] = cs.ricker_point_source(case, 13, 12, 10, nt, dt, fc=9.0, t0=0.12)
     ridx, rw = cs.surface_receivers(case, n=4)
     bi, bw = cs.buried_receiver(case, 7, 8, 6, 0.2, 0.7, 0.35)
@@ -131,7 +138,7 @@ def test_run_matches_reference():
     s_r = np.zeros((len(ridx), 9, nt), np.float32)
     pg_r = np.zeros_like(pg_o)
     dis_r = np.zeros_like(dis_o)
-    ref.refh_iso_run(C.byref(cfg), po.ptr(np.ascontiguousarray(case["coords"])),
+    ref.refh_run(C.byref(cfg), po.ptr(np.ascontiguousarray(case["coords"])),
                      po.ptr(case["metric"]), po.ptr(case["med"]), dt, nt, None, len(ridx),
                      po.ptr(ridx), po.ptr(rw), po.ptr(w_r), po.ptr(s_r), po.ptr(pg_r),
                      po.ptr(dis_r))
