@@ -54,7 +54,7 @@ def measured_traffic(kernel, workload_name):
         return None
 
 
-def cpu_baseline(wl, budget_s=12.0):
+def cpu_baseline(wl, budget_s=15.0):
     """The oracle on a sub-block cut from the same workload (top of the grid,
     free surface, PML on 5 faces, the source inside), timed on the host cores."""
     import numpy as np
@@ -88,7 +88,7 @@ def cpu_baseline(wl, budget_s=12.0):
     t = time.perf_counter()
     w, aux, _ = o.run(wl.dt, 1)
     t1 = time.perf_counter() - t
-    n = max(1, min(50, int(budget_s / max(t1, 1e-3))))
+    n = max(1, min(1000, int(budget_s / max(t1, 1e-3))))
     t = time.perf_counter()
     o.run(wl.dt, n, w=w, aux=aux, it0=1)
     tn = time.perf_counter() - t

@@ -484,10 +484,10 @@ extern "C" int cgfdh_get_seismograms(cgfdh_solver* s, float* out)
 
 extern "C" int cgfdh_set_kernel_variant(cgfdh_solver* s, int v)
 {
-  if (!s || v < 0 || v > 5) return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: %d", v);
+  if (!s || v < 0 || v > 7) return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: %d", v);
   if (s->d.medium_type != CGFDH_MEDIUM_ELASTIC_ISO && v >= 1 && v <= 3)
     return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: variant %d is built for the isotropic "
-                "medium only (VTI / anisotropic: 0, 4, 5)", v);
+                "medium only (VTI / anisotropic: 0, 4..7)", v);
   s->variant = v;
   return CGFDH_OK;
 }

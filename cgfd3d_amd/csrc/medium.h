@@ -15,6 +15,12 @@
 
 namespace cgfdh {
 
+// Streaming accesses: values a stage reads or writes once per point (metric,
+// medium, the RK levels other than the stencil's input, PML auxiliaries) are
+// marked non-temporal so they do not push the stencil's reused lines out of L2.
+__device__ __forceinline__ float ld_stream(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void st_stream(float* p, float v) { __builtin_nontemporal_store(v, p); }
+
 enum { MEDIUM_ISO = 2, MEDIUM_VTI = 3, MEDIUM_ANISO = 4 };
 
 template <int MT> struct Medium;
@@ -26,11 +32,18 @@ template <> struct Medium<MEDIUM_ISO> {
   static constexpr int ncmp = 3;
   float lam, mu, lam2mu;
 
+  // from the coefficient volumes' values at one point (lambda, mu)
+  __device__ __forceinline__ void from(const float* v)
+  {
+    lam = v[0];
+    mu = v[1];
+    lam2mu = (float)((double)lam + 2.0 * (double)mu);
+  }
+
   __device__ __forceinline__ void load(const float* __restrict__ med, long V, long p)
   {
-    lam = med[p];
-    mu = med[V + p];
-    lam2mu = (float)((double)lam + 2.0 * (double)mu);
+    const float v[2] = {ld_stream(med + p), ld_stream(med + V + p)};
+    from(v);
   }
 
   // stress rates from the velocity derivatives along xi (X), eta (Y) and
@@ -84,14 +97,23 @@ template <> struct Medium<MEDIUM_VTI> {
   static constexpr int ncmp = 6;
   float c11, c12, c13, c33, c55, c66;
 
+  // from c11, c13, c33, c55, c66 at one point
+  __device__ __forceinline__ void from(const float* v)
+  {
+    c11 = v[0];
+    c13 = v[1];
+    c33 = v[2];
+    c55 = v[3];
+    c66 = v[4];
+    c12 = (float)((double)c11 - 2.0 * (double)c66);
+  }
+
   __device__ __forceinline__ void load(const float* __restrict__ med, long V, long p)
   {
-    c11 = med[p];
-    c13 = med[V + p];
-    c33 = med[2 * V + p];
-    c55 = med[3 * V + p];
-    c66 = med[4 * V + p];
-    c12 = (float)((double)c11 - 2.0 * (double)c66);
+    float v[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) v[q] = ld_stream(med + q * V + p);
+    from(v);
   }
 
   // sv_curv_col_el_vti.c:453-475
@@ -151,10 +173,16 @@ template <> struct Medium<MEDIUM_ANISO> {
   static constexpr int ncmp = 22;
   float c[21];
 
+  __device__ __forceinline__ void from(const float* v)
+  {
+#pragma unroll
+    for (int q = 0; q < 21; q++) c[q] = v[q];
+  }
+
   __device__ __forceinline__ void load(const float* __restrict__ med, long V, long p)
   {
 #pragma unroll
-    for (int q = 0; q < 21; q++) c[q] = med[q * V + p];
+    for (int q = 0; q < 21; q++) c[q] = ld_stream(med + q * V + p);
   }
 
   __device__ __forceinline__ float K(int t, int b, float e1, float e2, float e3) const

@@ -230,7 +230,7 @@ __device__ void pml_face(const StageParams& P, const PmlDev& F, int fslot, int i
   const long ia = (long)(i - F.i1) + (long)(j - F.j1) * F.fnx + (long)(k - F.k1) * F.fnx * F.fny;
   float a[9], ah[9];
 #pragma unroll
-  for (int c = 0; c < 9; c++) a[c] = F.aux_cur[c * F.fn + ia];
+  for (int c = 0; c < 9; c++) a[c] = ld_stream(F.aux_cur + c * F.fn + ia);
 #pragma unroll
   for (int c = 0; c < 9; c++) {
     r[c] += cBm1 * rr[c] - cB * a[c];
@@ -259,23 +259,23 @@ __device__ void pml_face(const StageParams& P, const PmlDev& F, int fslot, int i
   } else if (KIND == KIND_FIRST) {
 #pragma unroll
     for (int c = 0; c < 9; c++) {
-      const float pre = F.aux_pre[c * F.fn + ia];
-      F.aux_tmp[c * F.fn + ia] = pre + P.ca * ah[c];
-      F.aux_end[c * F.fn + ia] = pre + P.cb * ah[c];
+      const float pre = ld_stream(F.aux_pre + c * F.fn + ia);
+      st_stream(F.aux_tmp + c * F.fn + ia, pre + P.ca * ah[c]);
+      st_stream(F.aux_end + c * F.fn + ia, pre + P.cb * ah[c]);
     }
   } else if (KIND == KIND_MID) {
 #pragma unroll
     for (int c = 0; c < 9; c++) {
-      const float pre = F.aux_pre[c * F.fn + ia];
-      const float end = F.aux_end[c * F.fn + ia];
-      F.aux_tmp[c * F.fn + ia] = pre + P.ca * ah[c];
-      F.aux_end[c * F.fn + ia] = end + P.cb * ah[c];
+      const float pre = ld_stream(F.aux_pre + c * F.fn + ia);
+      const float end = ld_stream(F.aux_end + c * F.fn + ia);
+      st_stream(F.aux_tmp + c * F.fn + ia, pre + P.ca * ah[c]);
+      st_stream(F.aux_end + c * F.fn + ia, end + P.cb * ah[c]);
     }
   } else {
 #pragma unroll
     for (int c = 0; c < 9; c++) {
-      const float end = F.aux_end[c * F.fn + ia];
-      F.aux_end[c * F.fn + ia] = end + P.cb * ah[c];
+      const float end = ld_stream(F.aux_end + c * F.fn + ia);
+      st_stream(F.aux_end + c * F.fn + ia, end + P.cb * ah[c]);
     }
   }
 }
@@ -320,23 +320,23 @@ __device__ __forceinline__ void combine(const StageParams& P, long p, const floa
   } else if (KIND == KIND_FIRST) {
 #pragma unroll
     for (int c = 0; c < 9; c++) {
-      const float pre = P.wpre[c * V + p];
-      P.wtmp[c * V + p] = pre + P.ca * r[c];
-      P.wend[c * V + p] = pre + P.cb * r[c];
+      const float pre = ld_stream(P.wpre + c * V + p);
+      st_stream(P.wtmp + c * V + p, pre + P.ca * r[c]);
+      st_stream(P.wend + c * V + p, pre + P.cb * r[c]);
     }
   } else if (KIND == KIND_MID) {
 #pragma unroll
     for (int c = 0; c < 9; c++) {
-      const float pre = P.wpre[c * V + p];
-      const float end = P.wend[c * V + p];
-      P.wtmp[c * V + p] = pre + P.ca * r[c];
-      P.wend[c * V + p] = end + P.cb * r[c];
+      const float pre = ld_stream(P.wpre + c * V + p);
+      const float end = ld_stream(P.wend + c * V + p);
+      st_stream(P.wtmp + c * V + p, pre + P.ca * r[c]);
+      st_stream(P.wend + c * V + p, end + P.cb * r[c]);
     }
   } else {
 #pragma unroll
     for (int c = 0; c < 9; c++) {
-      const float end = P.wend[c * V + p];
-      P.wend[c * V + p] = end + P.cb * r[c];
+      const float end = ld_stream(P.wend + c * V + p);
+      st_stream(P.wend + c * V + p, end + P.cb * r[c]);
     }
   }
 }
@@ -352,10 +352,10 @@ __device__ __forceinline__ void interior_finish(const StageParams& P, int i, int
   const long V = P.vol;
   float m[9];
 #pragma unroll
-  for (int n = 0; n < 9; n++) m[n] = P.met[(1 + n) * V + p];
+  for (int n = 0; n < 9; n++) m[n] = ld_stream(P.met + (1 + n) * V + p);
   Medium<MT> md;
   md.load(P.med, V, p);
-  const float slw = P.slw[p];
+  const float slw = ld_stream(P.slw + p);
   float r[9];
   momentum(r, m, slw, X, Y, Z);
   float Xv[3] = {X[VX], X[VY], X[VZ]}, Yv[3] = {Y[VX], Y[VY], Y[VZ]},
@@ -379,10 +379,10 @@ __device__ __forceinline__ void finish_point(const StageParams& P, int i, int j,
   const long V = P.vol;
   float m[9];
 #pragma unroll
-  for (int q = 0; q < 9; q++) m[q] = P.met[(1 + q) * V + p];
+  for (int q = 0; q < 9; q++) m[q] = ld_stream(P.met + (1 + q) * V + p);
   Medium<MT> md;
   md.load(P.med, V, p);
-  const float slw = P.slw[p];
+  const float slw = ld_stream(P.slw + p);
   float r[9];
   momentum(r, m, slw, X, Y, Z);
   if ((FLAGS & HAS_SURF) && P.free_top && k >= P.nk2 - (DZ == 0 ? 3 : 1)) {

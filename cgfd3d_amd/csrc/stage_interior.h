@@ -6,7 +6,9 @@
 // Block = 64 x TY threads over a 64 x TY tile of one z chunk; the block walks
 // its k planes keeping the 5-value z window of all 9 fields in registers and
 // stages the current plane (plus its x/y halo) in a double-buffered LDS tile,
-// so each plane costs one barrier.  Block ids are remapped so that each XCD
+// so each plane costs one barrier.  The newest window entry and the halo of
+// the next plane are loaded one plane ahead, so their HBM latency overlaps
+// the current plane's arithmetic.  Block ids are remapped so that each XCD
 // owns a contiguous run of tiles (x fastest, then y, then chunk): tiles that
 // share cache lines and halo rows meet in the same L2.
 #pragma once
@@ -18,7 +20,59 @@ struct BoxParams {
   int ni, nj, nk;           // extent
   int tiles_x, tiles_y, nchunk, kchunk;
   int nblocks;              // real blocks (grid is padded to a multiple of 8)
+  int gx, gy;               // per-point kernels: tiles per block group in x, y
+  int groups_x;             // groups along x
+  int tw;                   // box kernel: tile width in x (tile = tw x 256/tw)
 };
+
+// Tile-group shape of the per-point interior kernel, "GXxGY" tiles, from
+// CGFDH_TILE_GROUP when set (a tuning knob), else 9x8.
+inline void tile_group(int& gx, int& gy)
+{
+  static int cgx = 0, cgy = 0;
+  if (!cgx) {
+    cgx = 9; cgy = 8;
+    if (const char* e = getenv("CGFDH_TILE_GROUP")) {
+      int a = 0, b = 0;
+      if (sscanf(e, "%dx%d", &a, &b) == 2 && a > 0 && b > 0) { cgx = a; cgy = b; }
+    }
+  }
+  gx = cgx; gy = cgy;
+}
+
+// block of a grouped, XCD-remapped 1-D grid over a box tiled 64 x by: the
+// tile (bx, by) and plane k it covers, or false for padding blocks
+__device__ __forceinline__ bool grouped_block(const BoxParams& B, int& bx, int& by, int& k)
+{
+  const int nb8 = gridDim.x / 8;
+  const int b = (blockIdx.x % 8) * nb8 + blockIdx.x / 8;
+  if (b >= B.nblocks) return false;
+  const int gt = B.gx * B.gy;
+  const int per_group = gt * B.nk;
+  const int g = b / per_group;
+  const int r = b - g * per_group;
+  const int kk = r / gt;
+  const int t = r - kk * gt;
+  const int gyi = g / B.groups_x;
+  bx = (g - gyi * B.groups_x) * B.gx + t % B.gx;
+  by = gyi * B.gy + t / B.gx;
+  k = B.k0 + kk;
+  return bx < B.tiles_x && by < B.tiles_y;
+}
+
+// the grouped grid of a box tiled tw x by
+inline int grouped_grid(BoxParams& B, int by, int tw = 64)
+{
+  B.tiles_x = (B.ni + tw - 1) / tw;
+  B.tiles_y = (B.nj + by - 1) / by;
+  tile_group(B.gx, B.gy);
+  B.gx = std::min(B.gx, B.tiles_x);
+  B.gy = std::min(B.gy, B.tiles_y);
+  B.groups_x = (B.tiles_x + B.gx - 1) / B.gx;
+  const int groups_y = (B.tiles_y + B.gy - 1) / B.gy;
+  B.nblocks = B.groups_x * groups_y * B.gx * B.gy * B.nk;
+  return (B.nblocks + 7) / 8 * 8;
+}
 
 template <int TY> struct LdsTile {
   static constexpr int W = 64 + 4;
@@ -27,7 +81,7 @@ template <int TY> struct LdsTile {
 };
 
 template <int DX, int DY, int DZ, int KIND, int MT, int TY>
-__global__ void __launch_bounds__(64 * TY)
+__global__ void __launch_bounds__(64 * TY, TY <= 8 ? 2 : 4)
 stage_interior_kernel(StageParams P, BoxParams B)
 {
   using T = LdsTile<TY>;
@@ -48,57 +102,73 @@ stage_interior_kernel(StageParams P, BoxParams B)
   const int kbeg = B.k0 + chunk * B.kchunk;
   const int kend = min(B.k0 + B.nk - 1, kbeg + B.kchunk - 1);
   const int nx = P.nx;
+  const int ny = (int)(P.ss / nx);
   const long V = P.vol, ss = P.ss;
   // tiles may run past the box (and past the allocation's rows): such
   // threads load zeros or in-bounds neighbours and compute nothing
   const bool valid = (i < B.i0 + B.ni) && (j < B.j0 + B.nj);
-  const bool inrow = (i < nx) && (j < (int)(ss / nx));
+  const bool inrow = (i < nx) && (j < ny);
   const long pij = (long)i + (long)j * nx;
   constexpr int xl = -Win<DX>::lo;
   constexpr int yl = -Win<DY>::lo;
   constexpr int zlo = Win<DZ>::lo;
   constexpr int zc = -zlo;
 
-  float q[9][5];
+  // halo duty: threads ty < 4 load one y-halo row each, threads of row 4
+  // with tx < 4*TY one x-halo column entry; hidx is the LDS slot, hp the
+  // point (minus the plane offset), hok whether it lies in the allocation
+  int hslot = -1;
+  long hp = 0;
+  bool hok = false;
+  if (ty < 4) {
+    const int rrel = DY == 0 ? (ty == 0 ? -1 : TY + ty - 1) : (ty < 3 ? ty - 3 : TY);
+    const int jj = j0 + rrel;
+    hslot = (rrel + yl) * T::W + (tx + xl);
+    hp = (long)i + (long)jj * nx;
+    hok = i < nx && jj < ny;
+  } else if (ty == 4 && tx < 4 * TY) {
+    const int row = tx / 4, hc = tx % 4;
+    const int crel = DX == 0 ? (hc == 0 ? -1 : 64 + hc - 1) : (hc < 3 ? hc - 3 : 64);
+    const int ii = i0 + crel;
+    hslot = (row + yl) * T::W + (crel + xl);
+    hp = (long)ii + (long)(j0 + row) * nx;
+    hok = ii < nx && j0 + row < ny;
+  }
+
+  // software pipeline: the z window of plane k sits in q, the newest window
+  // entry and the halo of plane k+1 are loaded one iteration ahead
+  float q[9][5], qn[9], hv[9];
 #pragma unroll
-  for (int c = 0; c < 9; c++)
+  for (int c = 0; c < 9; c++) {
 #pragma unroll
-    for (int n = 1; n < 5; n++)
-      q[c][n] = inrow ? P.wcur[c * V + pij + (long)(kbeg + zlo + n - 1) * ss] : 0.0f;
+    for (int n = 0; n < 4; n++)
+      q[c][n + 1] = inrow ? P.wcur[c * V + pij + (long)(kbeg + zlo + n) * ss] : 0.0f;
+    qn[c] = inrow ? P.wcur[c * V + pij + (long)(kbeg + zlo + 4) * ss] : 0.0f;
+    hv[c] = hok ? P.wcur[c * V + hp + (long)kbeg * ss] : 0.0f;
+  }
 
   int buf = 0;
   for (int k = kbeg; k <= kend; k++, buf ^= 1) {
+    float* s0 = sm[buf];
 #pragma unroll
     for (int c = 0; c < 9; c++) {
 #pragma unroll
       for (int n = 0; n < 4; n++) q[c][n] = q[c][n + 1];
-      q[c][4] = inrow ? P.wcur[c * V + pij + (long)(k + zlo + 4) * ss] : 0.0f;
+      q[c][4] = qn[c];
+      s0[c * T::PLANE + (ty + yl) * T::W + (tx + xl)] = q[c][zc];
+      if (hslot >= 0) s0[c * T::PLANE + hslot] = hv[c];
     }
-    float* s0 = sm[buf];
+    if (k < kend) {
+      const long pn = (long)(k + 1) * ss;
 #pragma unroll
-    for (int c = 0; c < 9; c++) s0[c * T::PLANE + (ty + yl) * T::W + (tx + xl)] = q[c][zc];
-    const long pk = (long)k * ss;
-    if (ty < 4) {
-      const int rrel = DY == 0 ? (ty == 0 ? -1 : TY + ty - 1) : (ty < 3 ? ty - 3 : TY);
-      const int jj = j0 + rrel;
-      const long pp = (long)i + (long)jj * nx + pk;
-      const bool ok = i < nx && jj < (int)(ss / nx);
-#pragma unroll
-      for (int c = 0; c < 9; c++)
-        s0[c * T::PLANE + (rrel + yl) * T::W + (tx + xl)] = ok ? P.wcur[c * V + pp] : 0.0f;
-    } else if (ty == 4 && tx < 4 * TY) {
-      const int row = tx / 4, hc = tx % 4;
-      const int crel = DX == 0 ? (hc == 0 ? -1 : 64 + hc - 1) : (hc < 3 ? hc - 3 : 64);
-      const int ii = i0 + crel;
-      const long pp = (long)ii + (long)(j0 + row) * nx + pk;
-      const bool ok = ii < nx && j0 + row < (int)(ss / nx);
-#pragma unroll
-      for (int c = 0; c < 9; c++)
-        s0[c * T::PLANE + (row + yl) * T::W + (crel + xl)] = ok ? P.wcur[c * V + pp] : 0.0f;
+      for (int c = 0; c < 9; c++) {
+        qn[c] = inrow ? P.wcur[c * V + pij + (long)(k + 1 + zlo + 4) * ss] : 0.0f;
+        hv[c] = hok ? P.wcur[c * V + hp + pn] : 0.0f;
+      }
     }
     __syncthreads();
     if (valid) {
-      const long p = pij + pk;
+      const long p = pij + (long)k * ss;
       float X[9], Y[9], Z[9];
 #pragma unroll
       for (int c = 0; c < 9; c++) {
@@ -144,22 +214,19 @@ static void launch_interior_ty(hipStream_t st, const StageParams& P, BoxParams B
 
 // lean per-point kernel for the interior box: one thread per point, stencil
 // operands straight from global memory (L1/L2 serve the overlap), no LDS and
-// no barrier; block ids remapped so each XCD owns a contiguous run of
-// (x,y,z) blocks.  BY rows of 64 points per block.
+// no barrier.  BY rows of 64 points per block.  Block order: groups of
+// gx x gy tiles, k planes inside a group, tiles fastest; block ids remapped
+// so each XCD owns a contiguous run.  The blocks an XCD runs together then
+// cover a compact brick of tiles and planes, so the z and y neighbours one
+// block reads are mostly still in that XCD's L2 when its neighbours read them.
 template <int DX, int DY, int DZ, int KIND, int MT, int BY>
 __global__ void __launch_bounds__(64 * BY)
 stage_interior_point_kernel(StageParams P, BoxParams B)
 {
-  const int nb8 = gridDim.x / 8;
-  int b = (blockIdx.x % 8) * nb8 + blockIdx.x / 8;
-  if (b >= B.nblocks) return;
-  const int bz = b / (B.tiles_x * B.tiles_y);
-  b -= bz * B.tiles_x * B.tiles_y;
-  const int by = b / B.tiles_x;
-  const int bx = b - by * B.tiles_x;
+  int bx, by, k;
+  if (!grouped_block(B, bx, by, k)) return;
   const int i = B.i0 + bx * 64 + threadIdx.x;
   const int j = B.j0 + by * BY + threadIdx.y;
-  const int k = B.k0 + bz;
   if (i >= B.i0 + B.ni || j >= B.j0 + B.nj) return;
   const long p = (long)i + (long)j * P.nx + (long)k * P.ss;
   const long V = P.vol;
@@ -177,10 +244,7 @@ stage_interior_point_kernel(StageParams P, BoxParams B)
 template <int KIND, int MT, int BY>
 static void launch_interior_point(hipStream_t st, const StageParams& P, BoxParams B, int code)
 {
-  B.tiles_x = (B.ni + 63) / 64;
-  B.tiles_y = (B.nj + BY - 1) / BY;
-  B.nblocks = B.tiles_x * B.tiles_y * B.nk;
-  const int grid = (B.nblocks + 7) / 8 * 8;
+  const int grid = grouped_grid(B, BY);
   dim3 blk(64, BY, 1);
 #define CGFDH_IP(c, DX, DY, DZ)                                                          \
   case c:                                                                                \
@@ -195,13 +259,21 @@ static void launch_interior_point(hipStream_t st, const StageParams& P, BoxParam
 }
 
 // general per-point kernel restricted to a box (the boundary shell); FLAGS
-// says whether the box touches the free-surface rows
+// says whether the box touches the free-surface rows.  256-thread blocks over
+// tiles tw points wide and 256/tw rows deep (tw = the box width when that is
+// under 64, so the thin PML slabs along x keep whole waves busy), in the
+// grouped block order of the interior kernel.
 template <int DX, int DY, int DZ, int KIND, int MT, int FLAGS>
 __global__ void __launch_bounds__(256) stage_box_kernel(StageParams P, BoxParams B)
 {
-  const int i = B.i0 + blockIdx.x * 64 + threadIdx.x;
-  const int j = B.j0 + blockIdx.y * 4 + threadIdx.y;
-  const int k = B.k0 + blockIdx.z;
+  int bx, by, k;
+  if (!grouped_block(B, bx, by, k)) return;
+  const int rows = 256 / B.tw;
+  const int t = threadIdx.x;
+  const int ty = t / B.tw;
+  if (ty >= rows) return;
+  const int i = B.i0 + bx * B.tw + (t - ty * B.tw);
+  const int j = B.j0 + by * rows + ty;
   if (i >= B.i0 + B.ni || j >= B.j0 + B.nj) return;
   const long p = (long)i + (long)j * P.nx + (long)k * P.ss;
   const long V = P.vol;
@@ -220,8 +292,9 @@ template <int KIND, int MT, int FLAGS>
 static void launch_box(hipStream_t st, const StageParams& P, BoxParams B, int code)
 {
   if (B.ni <= 0 || B.nj <= 0 || B.nk <= 0) return;
-  dim3 blk(64, 4, 1);
-  dim3 grd((B.ni + 63) / 64, (B.nj + 3) / 4, B.nk);
+  B.tw = std::min(64, B.ni);
+  dim3 blk(256, 1, 1);
+  dim3 grd(grouped_grid(B, 256 / B.tw, B.tw), 1, 1);
 #define CGFDH_BX(c, DX, DY, DZ)                                                                \
   case c:                                                                                      \
     hipLaunchKernelGGL((stage_box_kernel<DX, DY, DZ, KIND, MT, FLAGS>), grd, blk, 0, st, P, B); \

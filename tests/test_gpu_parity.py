@@ -20,9 +20,10 @@ CONFIGS = [
 
 
 # (medium, kernel variant): every variant of the isotropic medium, the
-# per-point and lean-interior variants of the VTI and anisotropic media
-MEDIUM_VARIANTS = ([("iso", v) for v in range(6)] +
-                   [(m, v) for m in ("vti", "aniso") for v in (0, 4, 5)])
+# per-point, lean-interior and vectorised variants of the VTI and
+# anisotropic media
+MEDIUM_VARIANTS = ([("iso", v) for v in range(8)] +
+                   [(m, v) for m in ("vti", "aniso") for v in (0, 4, 5, 6, 7)])
 
 
 @pytest.mark.parametrize("medium,variant", MEDIUM_VARIANTS)
@@ -49,8 +50,8 @@ def test_onestage_all_pairs(name, free, pml, medium, variant):
     assert mism == 0, "expected bit-identical RHS"
 
 
-RUN_VARIANTS = ([("iso", v) for v in range(6)] +
-                [(m, v) for m in ("vti", "aniso") for v in (0, 4)])
+RUN_VARIANTS = ([("iso", v) for v in range(8)] +
+                [(m, v) for m in ("vti", "aniso") for v in (0, 4, 6)])
 
 
 @pytest.mark.parametrize("medium,variant", RUN_VARIANTS)
