@@ -33,6 +33,8 @@ KERNELS = [
     ("stage_interior_ty16", "stage_interior_kernel<TY=16> + stage_box_kernel"),
     ("stage_interior_point_by4", "stage_interior_point_kernel<BY=4> + stage_box_kernel"),
     ("stage_interior_point_by2", "stage_interior_point_kernel<BY=2> + stage_box_kernel"),
+    ("stage_interior_vec2", "stage_interior_vec_kernel<NQ=2> + stage_box_kernel"),
+    ("stage_interior_vec4", "stage_interior_vec_kernel<NQ=4> + stage_box_kernel"),
 ]
 
 
@@ -107,7 +109,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--variant", type=int, default=4,
                     help="stage kernels: 0 per-point, 1 tiled, 2/3 z-marching interior + "
-                         "shell, 4/5 per-point interior (64x4 / 64x2) + shell")
+                         "shell, 4/5 per-point interior (64x4 / 64x2) + shell, 6/7 "
+                         "x-vectorised interior (2 / 4 points per thread) + shell")
     ap.add_argument("--size", type=str, default="600x600x300")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

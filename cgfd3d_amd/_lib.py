@@ -17,7 +17,7 @@ class CgfdhDesc(C.Structure):
     _fields_ = [("ni", C.c_int), ("nj", C.c_int), ("nk", C.c_int),
                 ("medium_type", C.c_int), ("free_top", C.c_int),
                 ("pml_on", C.c_int * 6), ("pml_layers", C.c_int * 6),
-                ("dt", C.c_float), ("nt_total", C.c_int)]
+                ("dt", C.c_float), ("nt_total", C.c_int), ("nmaxwell", C.c_int)]
 
 
 # (name, restype, argtypes) of every symbol include/cgfdh.h declares
@@ -29,6 +29,9 @@ SIGNATURES = [
     ("cgfdh_set_medium_iso", C.c_int, [_P, _P, _P, _P]),
     ("cgfdh_set_medium_vti", C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     ("cgfdh_set_medium_aniso", C.c_int, [_P, _P, _P]),
+    ("cgfdh_set_medium_visco", C.c_int, [_P, _P, _P, _P, _P, _P]),
+    ("cgfdh_set_free_surface_matD", C.c_int, [_P, _P]),
+    ("cgfdh_get_free_surface_matD", C.c_int, [_P, _P]),
     ("cgfdh_set_pml_face", C.c_int, [_P, C.c_int, _P, _P, _P]),
     ("cgfdh_set_free_surface", C.c_int, [_P, _P, _P, _P]),
     ("cgfdh_set_moment_sources", C.c_int,

@@ -34,7 +34,7 @@ def hipcc():
 
 
 # one translation unit per medium's stage kernels, compiled in parallel
-UNITS = ["cgfdh.hip", "stage_iso.hip", "stage_vti.hip", "stage_aniso.hip"]
+UNITS = ["cgfdh.hip", "stage_iso.hip", "stage_vti.hip", "stage_aniso.hip", "stage_visco.hip"]
 
 
 def build_engine(verbose=False):

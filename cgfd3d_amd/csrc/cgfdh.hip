@@ -25,13 +25,13 @@ using namespace cgfdh;
 
 // receivers: trilinear sample of w_end after a step (io_funcs.c:1585-1623),
 // indx1d[5] used twice and indx1d[4] not, as the reference does
-__global__ void recv_kernel(const float* __restrict__ w, long vol, int nrecv,
+__global__ void recv_kernel(const float* __restrict__ w, long vol, int nrecv, int nw,
                             const int* __restrict__ idx, const float* __restrict__ dijk,
                             float* __restrict__ seismo, int nt_total, int it)
 {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nrecv * 9) return;
-  const int r = t / 9, c = t % 9;
+  if (t >= nrecv * nw) return;
+  const int r = t / nw, c = t % nw;
   const int* ix = idx + 8 * r;
   const float Lx2 = dijk[3 * r], Ly2 = dijk[3 * r + 1], Lz2 = dijk[3 * r + 2];
   const float Lx1 = (float)(1.0 - (double)Lx2);
@@ -42,7 +42,61 @@ __global__ void recv_kernel(const float* __restrict__ w, long vol, int nrecv,
                   v[ix[2]] * Lx1 * Ly2 * Lz1 + v[ix[3]] * Lx2 * Ly2 * Lz1 +
                   v[ix[5]] * Lx1 * Ly1 * Lz2 + v[ix[5]] * Lx2 * Ly1 * Lz2 +
                   v[ix[6]] * Lx1 * Ly2 * Lz2 + v[ix[7]] * Lx2 * Ly2 * Lz2;
-  seismo[((long)r * 9 + c) * nt_total + it] = s;
+  seismo[((long)r * nw + c) * nt_total + it] = s;
+}
+
+// visco-elastic free surface after the last stage: the surface-row stresses
+// rotated into the surface frame with the zeta-normal rows zeroed and back
+// (sv_curv_col_vis_iso_free, sv_curv_col_vis_iso.c:510-623)
+__global__ void vis_free_kernel(float* __restrict__ w, const float* __restrict__ matD, int ni,
+                                int nj, int nx, long ss, long vol, int nk2)
+{
+  const int i = 3 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 3 + blockIdx.y;
+  if (i > 3 + ni - 1) return;
+  const long p = (long)i + (long)j * nx + (long)nk2 * ss;
+  const long ij = ((long)i + (long)j * nx) * 9;
+  float* Txx = w + TXX * vol;
+  float* Tyy = w + TYY * vol;
+  float* Tzz = w + TZZ * vol;
+  float* Txy = w + TXY * vol;
+  float* Txz = w + TXZ * vol;
+  float* Tyz = w + TYZ * vol;
+  float D[3][3], DT[3][3], Tl[3][3], DTl[3][3], Tg[3][3];
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) {
+      D[a][b] = matD[ij + 3 * a + b];
+      DT[b][a] = D[a][b];
+    }
+  const float d11 = D[0][0], d12 = D[0][1], d13 = D[0][2];
+  const float d21 = D[1][0], d22 = D[1][1], d23 = D[1][2];
+  const float txx = Txx[p], tyy = Tyy[p], tzz = Tzz[p], txy = Txy[p], txz = Txz[p], tyz = Tyz[p];
+  Tl[0][0] = d11 * d11 * txx + d12 * d12 * tyy + d13 * d13 * tzz +
+             2 * (d11 * d12 * txy + d11 * d13 * txz + d12 * d13 * tyz);
+  Tl[0][1] = d11 * d21 * txx + d12 * d22 * tyy + d13 * d23 * tzz + (d11 * d22 + d12 * d21) * txy +
+             (d11 * d23 + d21 * d13) * txz + (d12 * d23 + d22 * d13) * tyz;
+  Tl[1][1] = d21 * d21 * txx + d22 * d22 * tyy + d23 * d23 * tzz +
+             2 * (d21 * d22 * txy + d21 * d23 * txz + d22 * d23 * tyz);
+  Tl[1][0] = Tl[0][1];
+  Tl[0][2] = 0.0f; Tl[1][2] = 0.0f; Tl[2][0] = 0.0f; Tl[2][1] = 0.0f; Tl[2][2] = 0.0f;
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) {
+      float c = 0.0f;
+      for (int q = 0; q < 3; q++) c += DT[a][q] * Tl[q][b];
+      DTl[a][b] = c;
+    }
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) {
+      float c = 0.0f;
+      for (int q = 0; q < 3; q++) c += DTl[a][q] * D[q][b];
+      Tg[a][b] = c;
+    }
+  Txx[p] = Tg[0][0];
+  Tyy[p] = Tg[1][1];
+  Tzz[p] = Tg[2][2];
+  Txy[p] = Tg[0][1];
+  Txz[p] = Tg[0][2];
+  Tyz[p] = Tg[1][2];
 }
 
 // peak ground motion on the free surface after a step (PG_calcu,
@@ -126,10 +180,12 @@ struct cgfdh_solver {
   int ni1, ni2, nj1, nj2, nk1, nk2;
   hipStream_t stream = nullptr;
   float* met = nullptr;
-  int ncmp = 3;                  // medium volumes (medium.h), slowness last
+  int ncmp = 3;                  // medium volumes (medium.h)
+  int nw = 9;                    // wavefield components (9 + 6 per Maxwell body)
+  float wl[MAXWELL_MAX] = {0};   // visco-elastic relaxation frequencies
   float* med = nullptr;
   float* lvl[4] = {nullptr, nullptr, nullptr, nullptr};   // pre, tmpA, tmpB, end
-  float* mats = nullptr;         // matVx2Vz2, matVy2Vz2, matF2Vz2
+  float* mats = nullptr;         // matVx2Vz2, matVy2Vz2, matF2Vz2, matD
   FaceHost face[6];
   int nsrc = 0;
   SrcDev* src = nullptr;
@@ -189,8 +245,11 @@ extern "C" int cgfdh_create(const cgfdh_desc* d, cgfdh_solver** out)
     return fail(CGFDH_EINVAL, "cgfdh_create: grid %dx%dx%d below the 8-point minimum", d->ni,
                 d->nj, d->nk);
   if (d->medium_type != CGFDH_MEDIUM_ELASTIC_ISO && d->medium_type != CGFDH_MEDIUM_ELASTIC_VTI &&
-      d->medium_type != CGFDH_MEDIUM_ELASTIC_ANISO)
+      d->medium_type != CGFDH_MEDIUM_ELASTIC_ANISO && d->medium_type != CGFDH_MEDIUM_VISCO_ISO)
     return fail(CGFDH_EINVAL, "cgfdh_create: medium_type %d not supported", d->medium_type);
+  if (d->medium_type == CGFDH_MEDIUM_VISCO_ISO && (d->nmaxwell < 1 || d->nmaxwell > MAXWELL_MAX))
+    return fail(CGFDH_EINVAL, "cgfdh_create: nmaxwell %d outside 1..%d", d->nmaxwell,
+                MAXWELL_MAX);
   if (d->nt_total < 1) return fail(CGFDH_EINVAL, "cgfdh_create: nt_total %d", d->nt_total);
   for (int f = 0; f < 6; f++) {
     int n = f < 2 ? d->ni : (f < 4 ? d->nj : d->nk);
@@ -218,12 +277,15 @@ extern "C" int cgfdh_create(const cgfdh_desc* d, cgfdh_solver** out)
   };
   int bad = 0;
   bad |= alloc(&s->met, 10 * s->vol);
-  s->ncmp = d->medium_type == CGFDH_MEDIUM_ELASTIC_VTI     ? Medium<MEDIUM_VTI>::ncmp
-            : d->medium_type == CGFDH_MEDIUM_ELASTIC_ANISO ? Medium<MEDIUM_ANISO>::ncmp
-                                                          : Medium<MEDIUM_ISO>::ncmp;
+  // coefficient volumes + slowness (+ Ylam_n, Ymu_n for the visco-elastic medium)
+  s->ncmp = d->medium_type == CGFDH_MEDIUM_ELASTIC_VTI     ? Medium<MEDIUM_VTI>::ncoef + 1
+            : d->medium_type == CGFDH_MEDIUM_ELASTIC_ANISO ? Medium<MEDIUM_ANISO>::ncoef + 1
+            : d->medium_type == CGFDH_MEDIUM_VISCO_ISO     ? 3 + 2 * d->nmaxwell
+                                                          : Medium<MEDIUM_ISO>::ncoef + 1;
+  s->nw = d->medium_type == CGFDH_MEDIUM_VISCO_ISO ? 9 + 6 * d->nmaxwell : 9;
   bad |= alloc(&s->med, (size_t)s->ncmp * s->vol);
-  for (auto& l : s->lvl) bad |= alloc(&l, 9 * s->vol);
-  bad |= alloc(&s->mats, 3 * 9 * s->ss);
+  for (auto& l : s->lvl) bad |= alloc(&l, (size_t)s->nw * s->vol);
+  bad |= alloc(&s->mats, 4 * 9 * s->ss);
   if (d->free_top) bad |= alloc(&s->pg, 18 * s->ss);
   for (int f = 0; f < 6 && !bad; f++) {
     FaceHost& F = s->face[f];
@@ -320,6 +382,29 @@ extern "C" int cgfdh_set_medium_aniso(cgfdh_solver* s, const float* cij, const f
   return rc;
 }
 
+extern "C" int cgfdh_set_medium_visco(cgfdh_solver* s, const float* lambda, const float* mu,
+                                      const float* slw, const float* ylam_ymu, const float* wl)
+{
+  if (!s || !lambda || !mu || !slw || !ylam_ymu || !wl)
+    return fail(CGFDH_EINVAL, "cgfdh_set_medium_visco: null argument");
+  if (s->d.medium_type != CGFDH_MEDIUM_VISCO_ISO)
+    return fail(CGFDH_ESTATE, "cgfdh_set_medium_visco: solver medium_type is %d",
+                s->d.medium_type);
+  int rc = h2d(s, s->med, lambda, s->vol);
+  if (!rc) rc = h2d(s, s->med + s->vol, mu, s->vol);
+  if (!rc) rc = h2d(s, s->med + 2 * s->vol, slw, s->vol);
+  if (!rc) rc = h2d(s, s->med + 3 * s->vol, ylam_ymu, 2 * (size_t)s->d.nmaxwell * s->vol);
+  for (int n = 0; n < s->d.nmaxwell; n++) s->wl[n] = wl[n];
+  if (!rc) s->medium_set = true;
+  return rc;
+}
+
+extern "C" int cgfdh_set_free_surface_matD(cgfdh_solver* s, const float* matD)
+{
+  if (!s || !matD) return fail(CGFDH_EINVAL, "cgfdh_set_free_surface_matD: null argument");
+  return h2d(s, s->mats + 27 * s->ss, matD, 9 * (size_t)s->ss);
+}
+
 extern "C" int cgfdh_set_pml_face(cgfdh_solver* s, int f, const float* A, const float* B,
                                   const float* D)
 {
@@ -413,7 +498,7 @@ extern "C" int cgfdh_set_receivers(cgfdh_solver* s, int nrecv, const int* idx, c
   HIPCHK(hipMemcpy(s->recv_idx, idx, 8 * nrecv * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(hipMalloc((void**)&s->recv_w, 3 * nrecv * sizeof(float)));
   HIPCHK(hipMemcpy(s->recv_w, dijk, 3 * nrecv * sizeof(float), hipMemcpyHostToDevice));
-  size_t n = (size_t)nrecv * 9 * s->d.nt_total;
+  size_t n = (size_t)nrecv * s->nw * s->d.nt_total;
   HIPCHK(hipMalloc((void**)&s->seismo, n * sizeof(float)));
   HIPCHK(hipMemset(s->seismo, 0, n * sizeof(float)));
   s->nrecv = nrecv;
@@ -423,13 +508,13 @@ extern "C" int cgfdh_set_receivers(cgfdh_solver* s, int nrecv, const int* idx, c
 extern "C" int cgfdh_set_wavefield(cgfdh_solver* s, const float* w)
 {
   if (!s || !w) return fail(CGFDH_EINVAL, "cgfdh_set_wavefield: null argument");
-  return h2d(s, s->lvl[0], w, 9 * s->vol);
+  return h2d(s, s->lvl[0], w, (size_t)s->nw * s->vol);
 }
 
 extern "C" int cgfdh_get_wavefield(cgfdh_solver* s, float* w)
 {
   if (!s || !w) return fail(CGFDH_EINVAL, "cgfdh_get_wavefield: null argument");
-  HIPCHK(hipMemcpyAsync(w, s->lvl[0], 9 * s->vol * sizeof(float), hipMemcpyDeviceToHost,
+  HIPCHK(hipMemcpyAsync(w, s->lvl[0], (size_t)s->nw * s->vol * sizeof(float), hipMemcpyDeviceToHost,
                         s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   return CGFDH_OK;
@@ -476,7 +561,7 @@ extern "C" int cgfdh_get_seismograms(cgfdh_solver* s, float* out)
 {
   if (!s || !out) return fail(CGFDH_EINVAL, "cgfdh_get_seismograms: null argument");
   if (s->nrecv == 0) return CGFDH_OK;
-  HIPCHK(hipMemcpyAsync(out, s->seismo, (size_t)s->nrecv * 9 * s->d.nt_total * sizeof(float),
+  HIPCHK(hipMemcpyAsync(out, s->seismo, (size_t)s->nrecv * s->nw * s->d.nt_total * sizeof(float),
                         hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   return CGFDH_OK;
@@ -487,7 +572,10 @@ extern "C" int cgfdh_set_kernel_variant(cgfdh_solver* s, int v)
   if (!s || v < 0 || v > 7) return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: %d", v);
   if (s->d.medium_type != CGFDH_MEDIUM_ELASTIC_ISO && v >= 1 && v <= 3)
     return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: variant %d is built for the isotropic "
-                "medium only (VTI / anisotropic: 0, 4..7)", v);
+                "medium only (VTI / anisotropic: 0, 4..7; visco-elastic: 0, 4, 5)", v);
+  if (s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO && v >= 6)
+    return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: variant %d is not built for the "
+                "visco-elastic medium (0, 4, 5)", v);
   s->variant = v;
   return CGFDH_OK;
 }
@@ -514,7 +602,14 @@ static StageParams make_params(cgfdh_solver* s, int it, int istage, const float*
   P.ni1 = s->ni1; P.ni2 = s->ni2; P.nj1 = s->nj1; P.nj2 = s->nj2; P.nk1 = s->nk1; P.nk2 = s->nk2;
   P.nx = s->nx; P.ss = s->ss; P.vol = s->vol;
   P.met = s->met;
-  P.med = s->med; P.slw = s->med + (s->ncmp - 1) * s->vol;
+  const bool iso_like = s->d.medium_type == CGFDH_MEDIUM_ELASTIC_ISO ||
+                        s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO;
+  P.med = s->med;
+  P.slw = s->med + (iso_like ? 2 : s->ncmp - 1) * s->vol;
+  P.nw = s->nw;
+  P.nmax = s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO ? s->d.nmaxwell : 0;
+  for (int n = 0; n < MAXWELL_MAX; n++) P.wl[n] = s->wl[n];
+  P.ylam = s->med + 3 * s->vol;
   P.wcur = wcur; P.wpre = s->lvl[0]; P.wtmp = wtmp; P.wend = s->lvl[3];
   P.ca = istage < 3 ? (float)kRkA[istage] * s->d.dt : 0.0f;
   P.cb = (float)kRkB[istage] * s->d.dt;
@@ -560,6 +655,8 @@ static void launch_stage(cgfdh_solver* s, const StageParams& P, int kind, int ip
     launch_stage_vti(L, P, kind, dx, dy, dz);
   else if (s->d.medium_type == CGFDH_MEDIUM_ELASTIC_ANISO)
     launch_stage_aniso(L, P, kind, dx, dy, dz);
+  else if (s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO)
+    launch_stage_visco(L, P, kind, dx, dy, dz);
   else
     launch_stage_iso(L, P, kind, dx, dy, dz);
 }
@@ -594,15 +691,19 @@ extern "C" int cgfdh_run(cgfdh_solver* s, int it0, int nsteps)
       launch_stage(s, P, kind, ipair, st);
       HIPCHK(hipEventRecord(s->ev_stage[8 * n + 2 * st + 1], s->stream));
     }
+    if (s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO && s->d.free_top)   // drv_rk_curv_col.c:440-445
+      hipLaunchKernelGGL(vis_free_kernel, dim3((s->d.ni + 255) / 256, s->d.nj), dim3(256), 0,
+                         s->stream, s->lvl[3], s->mats + 27 * s->ss, s->d.ni, s->d.nj, s->nx,
+                         s->ss, s->vol, s->nk2);
     if (s->pg)   // drv_rk_curv_col.c:492-495, before the pre/end swap
       hipLaunchKernelGGL(pg_kernel, dim3((s->d.ni + 255) / 256, s->d.nj), dim3(256), 0,
                          s->stream, s->lvl[3], s->lvl[0], s->pg, s->pg + 15 * s->ss, s->d.ni,
                          s->d.nj, s->nx, s->ss, s->vol, s->nk2, s->d.dt);
     if (s->nrecv > 0) {
-      int nt = s->nrecv * 9;
+      int nt = s->nrecv * s->nw;
       hipLaunchKernelGGL(recv_kernel, dim3((nt + 255) / 256), dim3(256), 0, s->stream,
-                         s->lvl[3], s->vol, s->nrecv, s->recv_idx, s->recv_w, s->seismo,
-                         s->d.nt_total, it);
+                         s->lvl[3], s->vol, s->nrecv, s->nw, s->recv_idx, s->recv_w,
+                         s->seismo, s->d.nt_total, it);
     }
     std::swap(s->lvl[0], s->lvl[3]);
     for (auto& f : s->face)
@@ -643,9 +744,10 @@ extern "C" int cgfdh_onestage(cgfdh_solver* s, int it, int ipair, int istage, co
   int rc = ready(s, "cgfdh_onestage");
   if (rc) return rc;
   // inputs go to tmpA / aux tmp, the rhs to tmpB / aux end
-  HIPCHK(hipMemcpyAsync(s->lvl[1], w_cur, 9 * s->vol * sizeof(float), hipMemcpyHostToDevice,
+  const size_t lv = (size_t)s->nw * s->vol;
+  HIPCHK(hipMemcpyAsync(s->lvl[1], w_cur, lv * sizeof(float), hipMemcpyHostToDevice,
                         s->stream));
-  HIPCHK(hipMemsetAsync(s->lvl[2], 0, 9 * s->vol * sizeof(float), s->stream));
+  HIPCHK(hipMemsetAsync(s->lvl[2], 0, lv * sizeof(float), s->stream));
   if (aux_cur) {
     rc = aux_copy(s, 1, (float*)aux_cur, true);
     if (rc) return rc;
@@ -657,7 +759,7 @@ extern "C" int cgfdh_onestage(cgfdh_solver* s, int it, int ipair, int istage, co
     if (f.on) P.aux_rhs_out[n++] = f.aux[2];
   launch_stage(s, P, KIND_RHS, ipair, istage);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(rhs, s->lvl[2], 9 * s->vol * sizeof(float), hipMemcpyDeviceToHost,
+  HIPCHK(hipMemcpyAsync(rhs, s->lvl[2], lv * sizeof(float), hipMemcpyDeviceToHost,
                         s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   if (aux_rhs) return aux_copy(s, 2, aux_rhs, false);
@@ -688,6 +790,11 @@ extern "C" int cgfdh_compute_metric(cgfdh_solver* s, const float* xyz)
     hipLaunchKernelGGL(metric_mirror_kernel, dim3((s->nx + 255) / 256, s->ny, 6), blk, 0,
                        s->stream, s->met, 2, s->nx, s->ny, s->nz, s->ni1, s->ni2, s->nj1,
                        s->nj2, s->nk1, s->nk2, s->ss, s->vol);
+    // the visco-elastic surface rotation needs the coordinates: build it now
+    if (s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO && s->d.free_top)
+      hipLaunchKernelGGL(vis_matd_kernel, dim3((s->d.ni + 255) / 256, s->d.nj), blk, 0,
+                         s->stream, d, s->met, s->mats + 27 * s->ss, s->d.ni, s->d.nj, s->nx,
+                         s->nk2, s->ss, s->vol);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   }
@@ -712,7 +819,8 @@ extern "C" int cgfdh_compute_free_surface(cgfdh_solver* s)
                        s->med, s->mats, s->d.ni, s->d.nj, s->nx, s->nk2, s->ss, s->vol);
   else
     hipLaunchKernelGGL(dvh2dvz_iso_kernel, grd, blk, 0, s->stream, s->met, s->med,
-                       s->med + s->vol, s->mats, s->d.ni, s->d.nj, s->nx, s->nk2, s->ss, s->vol);
+                       s->med + s->vol, s->mats, s->d.ni, s->d.nj, s->nx, s->nk2, s->ss, s->vol,
+                       s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO ? 0 : 1);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->stream));
   s->mats_set = true;
@@ -730,6 +838,13 @@ extern "C" int cgfdh_get_free_surface(cgfdh_solver* s, float* m)
 {
   if (!s || !m) return fail(CGFDH_EINVAL, "cgfdh_get_free_surface: null argument");
   HIPCHK(hipMemcpy(m, s->mats, 27 * s->ss * sizeof(float), hipMemcpyDeviceToHost));
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_get_free_surface_matD(cgfdh_solver* s, float* m)
+{
+  if (!s || !m) return fail(CGFDH_EINVAL, "cgfdh_get_free_surface_matD: null argument");
+  HIPCHK(hipMemcpy(m, s->mats + 27 * s->ss, 9 * s->ss * sizeof(float), hipMemcpyDeviceToHost));
   return CGFDH_OK;
 }
 

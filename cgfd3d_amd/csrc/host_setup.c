@@ -176,3 +176,127 @@ int cgfdh_gauss_weights(float *w, float x0, float y0, float z0, float a, int hal
   for (int m = 0; m < len * len * len; m++) w[m] /= sum;
   return CGFDH_OK;
 }
+
+/* ------------------------------------------------------------------------ */
+/* GMB visco-elastic coefficients: md_vis_GMB_cal_Y (md_t.c:643-792) with its */
+/* least-squares fit md_visco_LS / md_visco_LS_mat_inv (md_t.c:795-914)       */
+/* ------------------------------------------------------------------------ */
+
+#define GMB_N CGFDH_MAXWELL_MAX
+#define GMB_M (2 * CGFDH_MAXWELL_MAX)
+
+/* inverse of the n x n normal matrix a (destroyed) by Gauss-Jordan, swapping
+ * in a later row when a pivot is exactly zero */
+static int gmb_invert(float a[GMB_N][GMB_N], float b[GMB_N][GMB_N], int n)
+{
+  for (int r = 0; r < n; r++)
+    for (int c = 0; c < n; c++) b[r][c] = r == c ? 1.0f : 0.0f;
+  for (int k = 0; k < n; k++) {
+    if (a[k][k] == 0) {
+      int sw = k + 1;
+      while (sw < n && a[sw][k] == 0) sw++;
+      if (sw >= n) return -1;
+      for (int c = 0; c < n; c++) {
+        float t = a[k][c]; a[k][c] = a[sw][c]; a[sw][c] = t;
+        t = b[k][c]; b[k][c] = b[sw][c]; b[sw][c] = t;
+      }
+    }
+    float piv = a[k][k];
+    for (int c = 0; c < n; c++) {
+      a[k][c] = a[k][c] / piv;
+      b[k][c] = b[k][c] / piv;
+    }
+    for (int r = 0; r < n; r++) {
+      if (r == k) continue;
+      float f = a[r][k];
+      for (int c = 0; c < n; c++) {
+        a[r][c] = a[r][c] - a[k][c] * f;
+        b[r][c] = b[r][c] - b[k][c] * f;
+      }
+    }
+  }
+  return 0;
+}
+
+/* least squares y = (G^T G)^-1 G^T d for a right-hand side constant d */
+static int gmb_fit(float G[GMB_M][GMB_N], float d, int m, int n, float *y)
+{
+  float gtg[GMB_N][GMB_N], inv[GMB_N][GMB_N], x[GMB_N][GMB_M];
+  for (int r = 0; r < n; r++)
+    for (int c = 0; c < n; c++) {
+      float acc = 0;
+      for (int k = 0; k < m; k++) acc = acc + G[k][r] * G[k][c];
+      gtg[r][c] = acc;
+    }
+  if (gmb_invert(gtg, inv, n)) return -1;
+  for (int r = 0; r < n; r++)
+    for (int c = 0; c < m; c++) {
+      float acc = 0;
+      for (int k = 0; k < n; k++) acc = acc + inv[r][k] * G[c][k];
+      x[r][c] = acc;
+    }
+  for (int r = 0; r < n; r++) {
+    float acc = 0.0;
+    for (int c = 0; c < m; c++) acc = acc + x[r][c] * d;
+    y[r] = acc;
+  }
+  return 0;
+}
+
+/* phase terms of a GMB modulus at the reference frequency: theta1 = 1 -
+ * sum Y/(1+(wr/wl)^2), theta2 = sum Y (wr/wl)/(1+(wr/wl)^2), R = |theta| */
+static void gmb_theta(const float *Y, const float *wl, int n, float wr, float *t1, float *R)
+{
+  float acc = 0.0;
+  for (int q = 0; q < n; q++) acc = acc + Y[q] / (1 + pow(wr / wl[q], 2));
+  *t1 = 1 - acc;
+  acc = 0.0;
+  for (int q = 0; q < n; q++) acc = acc + Y[q] * wr / wl[q] / (1 + pow(wr / wl[q], 2));
+  const float t2 = acc;
+  *R = sqrt(pow(*t1, 2) + pow(t2, 2));
+}
+
+int cgfdh_visco_gmb(long npts, int nmaxwell, float freq, float fmin, float fmax,
+                    float *lambda, float *mu, const float *Qp, const float *Qs,
+                    float *ylam_ymu, float *wl)
+{
+  if (npts < 0 || nmaxwell < 1 || nmaxwell > CGFDH_MAXWELL_MAX || !lambda || !mu || !Qp ||
+      !Qs || !ylam_ymu || !wl || !(fmin > 0) || !(fmax > fmin))
+    return CGFDH_EINVAL;
+  const int m = 2 * nmaxwell - 1;
+  const float wr = 2 * PI * freq, w0 = 2 * PI * fmin, w1 = 2 * PI * fmax;
+  const float ratio = w1 / w0;
+  float wk[GMB_M];
+  for (int k = 0; k < m; k++) wk[k] = w0 * pow(ratio, k / (float)(m - 1));
+  for (int q = 0; q < nmaxwell; q++) wl[q] = wk[2 * q];
+  int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+  for (long p = 0; p < npts; p++) {
+    float GP[GMB_M][GMB_N], GS[GMB_M][GMB_N], YP[GMB_N], YS[GMB_N];
+    const float qp1 = 1.0 / round(Qp[p]), qs1 = 1.0 / round(Qs[p]);
+    for (int k = 0; k < m; k++)
+      for (int q = 0; q < nmaxwell; q++) {
+        GP[k][q] = (wl[q] * wk[k] + pow(wl[q], 2) * qp1) / (pow(wl[q], 2) + pow(wk[k], 2));
+        GS[k][q] = (wl[q] * wk[k] + pow(wl[q], 2) * qs1) / (pow(wl[q], 2) + pow(wk[k], 2));
+      }
+    if (gmb_fit(GP, qp1, m, nmaxwell, YP) || gmb_fit(GS, qs1, m, nmaxwell, YS)) {
+      bad = 1;
+      continue;
+    }
+    const float lam = lambda[p], muu = mu[p];
+    /* P modulus lambda + 2 mu and S modulus mu scaled to unrelaxed values:
+     * M_u = M (R + theta1) / (2 R^2) */
+    float t1, R;
+    gmb_theta(YP, wl, nmaxwell, wr, &t1, &R);
+    const float kappa = (lam + 2 * muu) * (R + t1) / (2 * pow(R, 2));
+    gmb_theta(YS, wl, nmaxwell, wr, &t1, &R);
+    const float mu_u = muu * (R + t1) / (2 * pow(R, 2));
+    mu[p] = mu_u;
+    lambda[p] = kappa - 2 * mu_u;
+    for (int q = 0; q < nmaxwell; q++) {
+      ylam_ymu[(2 * q) * npts + p] = (1 + 2 * mu[p] / lambda[p]) * YP[q] - 2 * mu[p] / lambda[p] * YS[q];
+      ylam_ymu[(2 * q + 1) * npts + p] = YS[q];
+    }
+  }
+  return bad ? CGFDH_EINVAL : CGFDH_OK;
+}

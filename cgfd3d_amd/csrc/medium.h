@@ -4,9 +4,13 @@
 //   Medium<2>  isotropic: lambda, mu          (sv_curv_col_el_iso.c)
 //   Medium<3>  VTI: c11, c13, c33, c55, c66   (sv_curv_col_el_vti.c)
 //   Medium<4>  general anisotropic: 21 Cij    (sv_curv_col_el_aniso.c)
+//   Medium<5>  visco-elastic GMB: the isotropic coupling with unrelaxed
+//              lambda, mu; its memory variables are handled by atten()
+//              in stage_common.h               (sv_curv_col_vis_iso.c)
 //
-// Each medium lives in HBM as ncmp volumes (slowness last, read separately):
-// lambda, mu, slw / c11, c13, c33, c55, c66, slw / c11, c12, ..., c66, slw.
+// Each medium lives in HBM as volumes (ncoef coefficient volumes first, read
+// by load(), then the slowness): lambda, mu, slw / c11, c13, c33, c55, c66,
+// slw / c11, c12, ..., c66, slw / lambda, mu, slw, Ylam_0, Ymu_0, Ylam_1, ...
 // Every expression keeps the reference's written order; the kernels are
 // compiled without FMA contraction, so results match the reference's FP32
 // evaluation bit for bit.
@@ -16,12 +20,13 @@
 namespace cgfdh {
 
 // Streaming accesses: values a stage reads or writes once per point (metric,
-// medium, the RK levels other than the stencil's input, PML auxiliaries) are
-// marked non-temporal so they do not push the stencil's reused lines out of L2.
-__device__ __forceinline__ float ld_stream(const float* p) { return __builtin_nontemporal_load(p); }
-__device__ __forceinline__ void st_stream(float* p, float v) { __builtin_nontemporal_store(v, p); }
+// medium, the RK levels other than the stencil's input, PML auxiliaries).
+// Plain loads: marking them non-temporal measured slower (it also keeps them
+// out of the last-level cache that serves part of the stencil's re-reads).
+__device__ __forceinline__ float ld_stream(const float* p) { return *p; }
+__device__ __forceinline__ void st_stream(float* p, float v) { *p = v; }
 
-enum { MEDIUM_ISO = 2, MEDIUM_VTI = 3, MEDIUM_ANISO = 4 };
+enum { MEDIUM_ISO = 2, MEDIUM_VTI = 3, MEDIUM_ANISO = 4, MEDIUM_VISCO = 5 };
 
 template <int MT> struct Medium;
 
@@ -29,7 +34,7 @@ template <int MT> struct Medium;
 enum { VX = 0, VY, VZ, TXX, TYY, TZZ, TYZ, TXZ, TXY };
 
 template <> struct Medium<MEDIUM_ISO> {
-  static constexpr int ncmp = 3;
+  static constexpr int ncoef = 2;
   float lam, mu, lam2mu;
 
   // from the coefficient volumes' values at one point (lambda, mu)
@@ -94,7 +99,7 @@ template <> struct Medium<MEDIUM_ISO> {
 
 // VTI: every term is one coefficient product (c * e) times a derivative.
 template <> struct Medium<MEDIUM_VTI> {
-  static constexpr int ncmp = 6;
+  static constexpr int ncoef = 5;
   float c11, c12, c13, c33, c55, c66;
 
   // from c11, c13, c33, c55, c66 at one point
@@ -170,7 +175,7 @@ __device__ constexpr int kAnisoIx[6][3][3] = {
     {{4, 19, 18}, {19, 9, 16}, {18, 16, 13}}, {{5, 20, 19}, {20, 10, 17}, {19, 17, 14}}};
 
 template <> struct Medium<MEDIUM_ANISO> {
-  static constexpr int ncmp = 22;
+  static constexpr int ncoef = 21;
   float c[21];
 
   __device__ __forceinline__ void from(const float* v)
@@ -226,5 +231,8 @@ template <> struct Medium<MEDIUM_ANISO> {
     one_dir(s, ztx, zty, ztz, Dz);
   }
 };
+
+// visco-elastic GMB: the stress-velocity coupling is the isotropic one
+template <> struct Medium<MEDIUM_VISCO> : Medium<MEDIUM_ISO> {};
 
 }  // namespace cgfdh

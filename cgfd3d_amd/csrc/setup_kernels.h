@@ -133,9 +133,12 @@ __device__ __forceinline__ void mul3(const float A[3][3], const float B[3][3], f
 }
 
 // sv_curv_col_el_iso_dvh2dvz (sv_curv_col_el_iso.c:1259-1375) at k = nk2
+// (the visco-elastic version, sv_curv_col_vis_iso.c:354-506, computes the same
+// two matrices and leaves matF2Vz2 unset: write_f = 0)
 __global__ void dvh2dvz_iso_kernel(const float* __restrict__ met, const float* __restrict__ lam3,
                                    const float* __restrict__ mu3, float* __restrict__ mats,
-                                   int ni, int nj, int nx, int nk2, long ss, long vol)
+                                   int ni, int nj, int nx, int nk2, long ss, long vol,
+                                   int write_f)
 {
   const int i = 3 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 3 + blockIdx.y;
@@ -183,8 +186,38 @@ __global__ void dvh2dvz_iso_kernel(const float* __restrict__ met, const float* _
     for (int c = 0; c < 3; c++) {
       mats[ij + r * 3 + c] = AB[r][c];
       mats[n + ij + r * 3 + c] = AC[r][c];
-      mats[2 * n + ij + r * 3 + c] = A[r][c];
+      if (write_f) mats[2 * n + ij + r * 3 + c] = A[r][c];
     }
+}
+
+// visco-elastic surface rotation matrix matD (sv_curv_col_vis_iso.c:415-501):
+// rows = the surface's tangent pair and normal, from the eta derivatives of
+// the coordinates (centred 7-point operator) and the zeta metric
+__global__ void vis_matd_kernel(const float* __restrict__ xyz, const float* __restrict__ met,
+                                float* __restrict__ matD, int ni, int nj, int nx, int nk2,
+                                long ss, long vol)
+{
+  const int i = 3 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 3 + blockIdx.y;
+  if (i > 3 + ni - 1) return;
+  const long p = (long)i + (long)j * nx + (long)nk2 * ss;
+  const float xet = dc7(xyz, p, nx), yet = dc7(xyz + vol, p, nx), zet = dc7(xyz + 2 * vol, p, nx);
+  const float ztx = met[7 * vol + p], zty = met[8 * vol + p], ztz = met[9 * vol + p];
+  const float e_n = (float)(1.0 / sqrt((double)(xet * xet + yet * yet + zet * zet)));
+  const float e_m = (float)(1.0 / sqrt((double)(ztx * ztx + zty * zty + ztz * ztz)));
+  const float e_nm = e_n * e_m;
+  float D[9];
+  D[0] = (yet * ztz - zet * zty) * e_nm;
+  D[1] = (zet * ztx - xet * ztz) * e_nm;
+  D[2] = (xet * zty - yet * ztx) * e_nm;
+  D[3] = xet * e_n;
+  D[4] = yet * e_n;
+  D[5] = zet * e_n;
+  D[6] = ztx * e_m;
+  D[7] = zty * e_m;
+  D[8] = ztz * e_m;
+  const long ij = ((long)j * nx + i) * 9;
+  for (int q = 0; q < 9; q++) matD[ij + q] = D[q];
 }
 
 // VTI / general anisotropic traction-free matrices at k = nk2

@@ -67,6 +67,10 @@ struct SrcDev {
   int coef_off;             // offset of this source's weights in ext_coefs
 };
 
+// visco-elastic: at most this many Maxwell bodies, 6 memory variables each
+constexpr int MAXWELL_MAX = 4;
+constexpr int NW_MAX = 9 + 6 * MAXWELL_MAX;
+
 struct StageParams {
   int ni1, ni2, nj1, nj2, nk1, nk2;
   int nx;
@@ -95,6 +99,11 @@ struct StageParams {
   int src_half;
   const float* src_coef;
   int it, istage;
+  // visco-elastic GMB: bodies, wavefield components (9 + 6 nmax), relaxation
+  // frequencies, the Ylam_n / Ymu_n volumes (interleaved, from ylam)
+  int nmax, nw;
+  float wl[MAXWELL_MAX];
+  const float* ylam;
   // tiling of the z-marching kernel
   int kchunk;
   int tiles_x, tiles_y;
@@ -309,24 +318,28 @@ __device__ __forceinline__ void add_sources(const StageParams& P, int i, int j, 
   }
 }
 
-// RK combine of the wavefield (drv_rk_curv_col.c:291-446)
-template <int KIND>
+// RK combine of the wavefield (drv_rk_curv_col.c:291-446); NW > 9 when the
+// visco-elastic memory variables follow (P.nw of them are live)
+template <int KIND, int NW = 9>
 __device__ __forceinline__ void combine(const StageParams& P, long p, const float* r)
 {
   const long V = P.vol;
   if (KIND == KIND_RHS) {
 #pragma unroll
-    for (int c = 0; c < 9; c++) P.rhs_out[c * V + p] = r[c];
+    for (int c = 0; c < NW; c++)
+      if (c < 9 || c < P.nw) P.rhs_out[c * V + p] = r[c];
   } else if (KIND == KIND_FIRST) {
 #pragma unroll
-    for (int c = 0; c < 9; c++) {
+    for (int c = 0; c < NW; c++) {
+      if (c >= 9 && c >= P.nw) break;
       const float pre = ld_stream(P.wpre + c * V + p);
       st_stream(P.wtmp + c * V + p, pre + P.ca * r[c]);
       st_stream(P.wend + c * V + p, pre + P.cb * r[c]);
     }
   } else if (KIND == KIND_MID) {
 #pragma unroll
-    for (int c = 0; c < 9; c++) {
+    for (int c = 0; c < NW; c++) {
+      if (c >= 9 && c >= P.nw) break;
       const float pre = ld_stream(P.wpre + c * V + p);
       const float end = ld_stream(P.wend + c * V + p);
       st_stream(P.wtmp + c * V + p, pre + P.ca * r[c]);
@@ -334,11 +347,63 @@ __device__ __forceinline__ void combine(const StageParams& P, long p, const floa
     }
   } else {
 #pragma unroll
-    for (int c = 0; c < 9; c++) {
+    for (int c = 0; c < NW; c++) {
+      if (c >= 9 && c >= P.nw) break;
       const float end = ld_stream(P.wend + c * V + p);
       st_stream(P.wend + c * V + p, end + P.cb * r[c]);
     }
   }
+}
+
+// GMB attenuation after the PML terms (sv_curv_col_vis_iso.c:251-342): the
+// memory-variable rates into r[9 + 6n + {xx,yy,zz,yz,xz,xy}] and the memory
+// terms subtracted from the stress rates, with the reference's double
+// intermediates
+__device__ __forceinline__ void atten(const StageParams& P, long p, float lam, float mu, float* r)
+{
+  const long V = P.vol;
+  const float hTxx = r[TXX], hTyy = r[TYY], hTzz = r[TZZ];
+  const float hTxy = r[TXY], hTxz = r[TXZ], hTyz = r[TYZ];
+  const float sum_hxyz = (hTxx + hTyy + hTzz) / (3 * lam + 2 * mu);
+  const double mu2 = (double)(2 * mu);
+  const float EVxx = (float)(((2.0 * hTxx - hTyy - hTzz) / mu2 + sum_hxyz) / 3);
+  const float EVyy = (float)(((2.0 * hTyy - hTxx - hTzz) / mu2 + sum_hxyz) / 3);
+  const float EVzz = (float)(((2.0 * hTzz - hTxx - hTyy) / mu2 + sum_hxyz) / 3);
+  const float EVxy = (float)((double)(hTxy / mu) * 0.5);
+  const float EVxz = (float)((double)(hTxz / mu) * 0.5);
+  const float EVyz = (float)((double)(hTyz / mu) * 0.5);
+  float sJxyz = 0.0f, sJxx = 0.0f, sJyy = 0.0f, sJzz = 0.0f, sJxy = 0.0f, sJxz = 0.0f,
+        sJyz = 0.0f;
+#pragma unroll
+  for (int n = 0; n < MAXWELL_MAX; n++) {
+    if (n >= P.nmax) break;
+    const float* J = P.wcur + (9 + 6 * n) * V + p;
+    const float Jxx = J[0], Jyy = J[V], Jzz = J[2 * V], Jyz = J[3 * V], Jxz = J[4 * V],
+                Jxy = J[5 * V];
+    const float wl = P.wl[n];
+    float* h = r + 9 + 6 * n;
+    h[0] = wl * (EVxx - Jxx);
+    h[1] = wl * (EVyy - Jyy);
+    h[2] = wl * (EVzz - Jzz);
+    h[3] = wl * (EVyz - Jyz);
+    h[4] = wl * (EVxz - Jxz);
+    h[5] = wl * (EVxy - Jxy);
+    const float Yl = ld_stream(P.ylam + (2 * n) * V + p);
+    const float Ym = ld_stream(P.ylam + (2 * n + 1) * V + p);
+    sJxyz += Yl * (Jxx + Jyy + Jzz);
+    sJxx += Ym * Jxx;
+    sJyy += Ym * Jyy;
+    sJzz += Ym * Jzz;
+    sJxy += Ym * Jxy;
+    sJxz += Ym * Jxz;
+    sJyz += Ym * Jyz;
+  }
+  r[TXX] = hTxx - (float)(lam * sJxyz + 2.0 * mu * sJxx);
+  r[TYY] = hTyy - (float)(lam * sJxyz + 2.0 * mu * sJyy);
+  r[TZZ] = hTzz - (float)(lam * sJxyz + 2.0 * mu * sJzz);
+  r[TXY] = hTxy - (float)(2.0 * mu * sJxy);
+  r[TXZ] = hTxz - (float)(2.0 * mu * sJxz);
+  r[TYZ] = hTyz - (float)(2.0 * mu * sJyz);
 }
 
 // the interior form of everything after the derivatives: momentum, Hooke's
@@ -356,13 +421,14 @@ __device__ __forceinline__ void interior_finish(const StageParams& P, int i, int
   Medium<MT> md;
   md.load(P.med, V, p);
   const float slw = ld_stream(P.slw + p);
-  float r[9];
+  float r[MT == MEDIUM_VISCO ? NW_MAX : 9];
   momentum(r, m, slw, X, Y, Z);
   float Xv[3] = {X[VX], X[VY], X[VZ]}, Yv[3] = {Y[VX], Y[VY], Y[VZ]},
         Zv[3] = {Z[VX], Z[VY], Z[VZ]};
   md.hooke(r, m, Xv, Yv, Zv);
+  if constexpr (MT == MEDIUM_VISCO) atten(P, p, md.lam, md.mu, r);
   if (P.nsrc > 0) add_sources(P, i, j, k, p, r);
-  combine<KIND>(P, p, r);
+  combine<KIND, MT == MEDIUM_VISCO ? NW_MAX : 9>(P, p, r);
 }
 
 // which boundary treatments a kernel instance has to handle
@@ -383,7 +449,7 @@ __device__ __forceinline__ void finish_point(const StageParams& P, int i, int j,
   Medium<MT> md;
   md.load(P.med, V, p);
   const float slw = ld_stream(P.slw + p);
-  float r[9];
+  float r[MT == MEDIUM_VISCO ? NW_MAX : 9];
   momentum(r, m, slw, X, Y, Z);
   if ((FLAGS & HAS_SURF) && P.free_top && k >= P.nk2 - (DZ == 0 ? 3 : 1)) {
     // hV: traction image on rows nk2-3..nk2 (forward z op) or nk2-1..nk2
@@ -403,8 +469,9 @@ __device__ __forceinline__ void finish_point(const StageParams& P, int i, int j,
       pml_face<KIND, MT>(P, F, f, i, j, k, p, Dd, md, slw, r);
     }
   }
+  if constexpr (MT == MEDIUM_VISCO) atten(P, p, md.lam, md.mu, r);
   if (P.nsrc > 0) add_sources(P, i, j, k, p, r);
-  combine<KIND>(P, p, r);
+  combine<KIND, MT == MEDIUM_VISCO ? NW_MAX : 9>(P, p, r);
 }
 
 }  // namespace cgfdh

@@ -26,12 +26,12 @@ struct BoxParams {
 };
 
 // Tile-group shape of the per-point interior kernel, "GXxGY" tiles, from
-// CGFDH_TILE_GROUP when set (a tuning knob), else 9x8.
+// CGFDH_TILE_GROUP when set (a tuning knob), else 9x2.
 inline void tile_group(int& gx, int& gy)
 {
   static int cgx = 0, cgy = 0;
   if (!cgx) {
-    cgx = 9; cgy = 8;
+    cgx = 9; cgy = 2;
     if (const char* e = getenv("CGFDH_TILE_GROUP")) {
       int a = 0, b = 0;
       if (sscanf(e, "%dx%d", &a, &b) == 2 && a > 0 && b > 0) { cgx = a; cgy = b; }

@@ -10,8 +10,9 @@
 // (64x8 / 64x16 tiles) plus the per-point kernel on the boundary shell;
 // 4/5 the lean per-point interior kernel (64x4 / 64x2 blocks) plus the
 // shell; 6/7 the x-vectorised interior kernel (2 points per thread on 64x4
-// threads / 4 points per thread on 64x2 threads) plus the shell.  The VTI and anisotropic media build
-// variants 0 and 4..7.
+// threads / 4 points per thread on 64x2 threads) plus the shell.  The VTI
+// and anisotropic media build variants 0 and 4..7, the visco-elastic medium
+// 0, 4 and 5.
 #pragma once
 #include <algorithm>
 #include <cstdio>
@@ -97,17 +98,19 @@ static void launch_kind(const LaunchCtx& L, const StageParams& P, int dx, int dy
     return b;
   };
   const BoxParams in = box(ilo, ihi, jlo, jhi, klo, khi);
-  if (in.ni > 0 && in.nj > 0 && in.nk > 0 && L.variant >= 6) {
+  if (MT != MEDIUM_VISCO && in.ni > 0 && in.nj > 0 && in.nk > 0 && L.variant >= 6) {
     // whole point groups on the vectorised kernel, the last columns on the
     // per-point box kernel
     BoxParams vb = in;
     const int nq = L.variant == 7 ? 4 : 2;
     vb.ni = in.ni / nq * nq;
-    if (vb.ni > 0) {
-      if (L.variant == 7)
-        launch_interior_vec<KIND, MT, 2, 4>(st, P, vb, code);
-      else
-        launch_interior_vec<KIND, MT, 4, 2>(st, P, vb, code);
+    if constexpr (MT != MEDIUM_VISCO) {
+      if (vb.ni > 0) {
+        if (L.variant == 7)
+          launch_interior_vec<KIND, MT, 2, 4>(st, P, vb, code);
+        else
+          launch_interior_vec<KIND, MT, 4, 2>(st, P, vb, code);
+      }
     }
     if (in.ni > vb.ni)
       launch_box<KIND, MT, HAS_PML>(st, P, box(ilo + vb.ni, ihi, jlo, jhi, klo, khi), code);
@@ -152,6 +155,8 @@ void launch_stage_mt(const LaunchCtx& L, const StageParams& P, int kind, int dx,
 void launch_stage_iso(const LaunchCtx& L, const StageParams& P, int kind, int dx, int dy, int dz);
 void launch_stage_vti(const LaunchCtx& L, const StageParams& P, int kind, int dx, int dy, int dz);
 void launch_stage_aniso(const LaunchCtx& L, const StageParams& P, int kind, int dx, int dy,
+                        int dz);
+void launch_stage_visco(const LaunchCtx& L, const StageParams& P, int kind, int dx, int dy,
                         int dz);
 
 }  // namespace cgfdh

@@ -33,9 +33,7 @@ __device__ __forceinline__ void ldq(const float* p, float* o)
 template <int NQ>
 __device__ __forceinline__ void ldq_stream(const float* p, float* o)
 {
-  const typename Vec<NQ>::T t = __builtin_nontemporal_load((const typename Vec<NQ>::U*)p);
-#pragma unroll
-  for (int q = 0; q < NQ; q++) o[q] = t[q];
+  ldq<NQ>(p, o);
 }
 template <int NQ>
 __device__ __forceinline__ void stq_stream(float* p, const float* v)
@@ -43,7 +41,7 @@ __device__ __forceinline__ void stq_stream(float* p, const float* v)
   typename Vec<NQ>::T t;
 #pragma unroll
   for (int q = 0; q < NQ; q++) t[q] = v[q];
-  __builtin_nontemporal_store(t, (typename Vec<NQ>::U*)p);
+  *(typename Vec<NQ>::U*)p = t;
 }
 
 // keeps the scheduler from hoisting the next phase's loads above this point,
@@ -135,7 +133,7 @@ stage_interior_vec_kernel(StageParams P, BoxParams B)
   const long p = (long)i + (long)j * P.nx + (long)k * P.ss;
   const long V = P.vol;
   const float* w = P.wcur + p;
-  constexpr int NM = Medium<MT>::ncmp - 1;
+  constexpr int NM = Medium<MT>::ncoef;
   float m4[9][NQ];
 #pragma unroll
   for (int n = 0; n < 9; n++) ldq_stream<NQ>(P.met + (1 + n) * V + p, m4[n]);

@@ -13,6 +13,7 @@ from ._lib import CgfdhDesc, check, lib
 MEDIUM_ELASTIC_ISO = 2
 MEDIUM_ELASTIC_VTI = 3
 MEDIUM_ELASTIC_ANISO = 4
+MEDIUM_VISCO_ISO = 5
 SRC_POINT = 1
 SRC_GAUSSIAN = 2
 
@@ -29,10 +30,13 @@ class Solver:
     """Engine for one grid block: ni x nj x nk physical points."""
 
     def __init__(self, ni, nj, nk, dt, nt_total, free_top=True, pml_on=(0,) * 6,
-                 pml_layers=(0,) * 6, medium_type=MEDIUM_ELASTIC_ISO):
+                 pml_layers=(0,) * 6, medium_type=MEDIUM_ELASTIC_ISO, nmaxwell=0):
         d = CgfdhDesc()
         d.ni, d.nj, d.nk = ni, nj, nk
         d.medium_type = medium_type
+        d.nmaxwell = nmaxwell
+        # wavefield components: 9, plus 6 memory variables per Maxwell body
+        self.ncomp = 9 + 6 * nmaxwell if medium_type == MEDIUM_VISCO_ISO else 9
         d.free_top = int(free_top)
         for f in range(6):
             d.pml_on[f] = int(pml_on[f])
@@ -84,10 +88,19 @@ class Solver:
         assert c.size == 21 * self.vol and s.size == self.vol
         check(lib().cgfdh_set_medium_aniso(self.h, _p(c), _p(s)), "set_medium_aniso")
 
-    def set_medium(self, medium_type, volumes):
-        """volumes: the medium's arrays in include/cgfdh.h order, slowness last."""
+    def set_medium_visco(self, lam, mu, slw, ylam_ymu, wl):
+        a, b, c = _f32(lam), _f32(mu), _f32(slw)
+        y, w = _f32(ylam_ymu), _f32(wl)
+        check(lib().cgfdh_set_medium_visco(self.h, _p(a), _p(b), _p(c), _p(y), _p(w)),
+              "set_medium_visco")
+
+    def set_medium(self, medium_type, volumes, wl=None):
+        """volumes: the medium's arrays in include/cgfdh.h order (for the
+        visco-elastic medium lambda, mu, slw, Ylam_0, Ymu_0, ..., with wl)."""
         v = _f32(volumes).reshape(-1, self.vol)
-        if medium_type == MEDIUM_ELASTIC_VTI:
+        if medium_type == MEDIUM_VISCO_ISO:
+            self.set_medium_visco(v[0], v[1], v[2], v[3:], wl)
+        elif medium_type == MEDIUM_ELASTIC_VTI:
             self.set_medium_vti(*v)
         elif medium_type == MEDIUM_ELASTIC_ANISO:
             self.set_medium_aniso(v[:21], v[21])
@@ -99,9 +112,13 @@ class Solver:
         check(lib().cgfdh_set_pml_face(self.h, face, _p(a), _p(b), _p(d)), "set_pml_face")
 
     def set_free_surface(self, mats):
-        m = _f32(mats).reshape(3, -1)
+        """matVx2Vz2, matVy2Vz2, matF2Vz2 (and matD as a 4th array)."""
+        m = _f32(mats).reshape(-1, 9 * self.nx * self.ny)
         check(lib().cgfdh_set_free_surface(self.h, _p(m[0].copy()), _p(m[1].copy()),
                                            _p(m[2].copy())), "set_free_surface")
+        if len(m) > 3:
+            check(lib().cgfdh_set_free_surface_matD(self.h, _p(m[3].copy())),
+                  "set_free_surface_matD")
 
     def set_moment_sources(self, ijk, it_begin, it_end, max_nt, mij, spatial=SRC_POINT,
                            ext_half=0, ext_coefs=None):
@@ -125,10 +142,11 @@ class Solver:
     # ---- state ----
     def set_wavefield(self, w9):
         w = _f32(w9)
+        assert w.size == self.ncomp * self.vol
         check(lib().cgfdh_set_wavefield(self.h, _p(w)), "set_wavefield")
 
     def wavefield(self):
-        w = np.empty(9 * self.vol, np.float32)
+        w = np.empty(self.ncomp * self.vol, np.float32)
         check(lib().cgfdh_get_wavefield(self.h, _p(w)), "get_wavefield")
         return w
 
@@ -149,7 +167,7 @@ class Solver:
         check(lib().cgfdh_run(self.h, it0, nsteps), "run")
 
     def seismograms(self):
-        s = np.empty((max(self.nrecv, 1), 9, self.nt_total), np.float32)
+        s = np.empty((max(self.nrecv, 1), self.ncomp, self.nt_total), np.float32)
         check(lib().cgfdh_get_seismograms(self.h, _p(s)), "get_seismograms")
         return s[: self.nrecv]
 
@@ -162,7 +180,7 @@ class Solver:
 
     def onestage(self, it, ipair, istage, w_cur, aux_cur=None):
         w = _f32(w_cur)
-        rhs = np.empty(9 * self.vol, np.float32)
+        rhs = np.empty(self.ncomp * self.vol, np.float32)
         n = self.aux_len()
         arhs = np.empty(max(n, 1), np.float32)
         a = None if aux_cur is None else _f32(aux_cur)

@@ -11,7 +11,8 @@
  * together with the one-stage RHS it dispatches to,
  *   sv_curv_col_el_iso_onestage(...)    (forward/sv_curv_col_el_iso.h:17-32),
  *   sv_curv_col_el_vti_onestage(...)    (forward/sv_curv_col_el_vti.h:17-31),
- *   sv_curv_col_el_aniso_onestage(...)  (forward/sv_curv_col_el_aniso.h:17-32).
+ *   sv_curv_col_el_aniso_onestage(...)  (forward/sv_curv_col_el_aniso.h:17-32),
+ *   sv_curv_col_vis_iso_onestage(...)   (forward/sv_curv_col_vis_iso.h:17-33).
  * This ABI carries the same data with plain pointers and sizes, so a C shim
  * that unpacks those structs into these calls replaces the reference's time
  * loop (the shim is shown in INTEGRATION.md).  No torch or HIP types appear.
@@ -19,7 +20,10 @@
  * Conventions (all follow the reference):
  *  - grids carry 3 ghost points per side: nx=ni+6, ny=nj+6, nz=nk+6;
  *    iptr = i + j*nx + k*nx*ny, x fastest (gd_t.c:2890-2897);
- *  - wavefield components Vx,Vy,Vz,Txx,Tyy,Tzz,Tyz,Txz,Txy (wav_t.c:93-136);
+ *  - wavefield components Vx,Vy,Vz,Txx,Tyy,Tzz,Tyz,Txz,Txy (wav_t.c:93-136),
+ *    followed for the visco-elastic medium by Jxx,Jyy,Jzz,Jyz,Jxz,Jxy of
+ *    each Maxwell body (wav_t.c:138-171): ncomp = 9 + 6*nmaxwell components
+ *    in every wavefield array and seismogram of this ABI;
  *  - metric components jac,xi_x,xi_y,xi_z,eta_x,eta_y,eta_z,zeta_x,zeta_y,
  *    zeta_z (gd_t.c:101-180);
  *  - the medium's density is passed as slowness 1/rho (md_t.c:602-623);
@@ -48,6 +52,8 @@ extern "C" {
 #define CGFDH_MEDIUM_ELASTIC_ISO 2
 #define CGFDH_MEDIUM_ELASTIC_VTI 3
 #define CGFDH_MEDIUM_ELASTIC_ANISO 4
+#define CGFDH_MEDIUM_VISCO_ISO 5       /* visco-elastic, GMB memory variables */
+#define CGFDH_MAXWELL_MAX 4
 
 #define CGFDH_SRC_POINT 1     /* CONST_SRC_SPATIAL_POINT */
 #define CGFDH_SRC_GAUSSIAN 2  /* CONST_SRC_SPATIAL_GAUSSIAN */
@@ -56,12 +62,13 @@ typedef struct cgfdh_solver cgfdh_solver;
 
 typedef struct {
   int ni, nj, nk;       /* physical grid points of the block */
-  int medium_type;      /* CGFDH_MEDIUM_ELASTIC_ISO, _VTI or _ANISO */
+  int medium_type;      /* CGFDH_MEDIUM_ELASTIC_ISO, _VTI, _ANISO or _VISCO_ISO */
   int free_top;         /* 1: traction-image free surface on z2 */
   int pml_on[6];        /* CFS-PML per face */
   int pml_layers[6];    /* number_of_layers per face */
   float dt;             /* size_of_time_step */
   int nt_total;         /* number_of_time_steps + 1 (main_curv_col_el_3d.c:615) */
+  int nmaxwell;         /* visco-elastic: Maxwell bodies, 1..CGFDH_MAXWELL_MAX */
 } cgfdh_desc;
 
 /* Replaces the allocations of wav_init / bdry_pml_auxvar_init / bdry_free_set
@@ -90,6 +97,14 @@ int cgfdh_set_medium_vti(cgfdh_solver *s, const float *c11, const float *c13,
 int cgfdh_set_medium_aniso(cgfdh_solver *s, const float *cij21,
                            const float *slowness);
 
+/* md_t of the GMB visco-elastic medium after md_vis_GMB_cal_Y (md_t.c:643-792;
+ * cgfdh_visco_gmb builds it): unrelaxed lambda, mu, rho-as-slowness, then
+ * Ylam_n, Ymu_n interleaved per Maxwell body (2*nmaxwell volumes, the md_t
+ * order, md_t.c:240-249), and the bodies' relaxation frequencies wl. */
+int cgfdh_set_medium_visco(cgfdh_solver *s, const float *lambda, const float *mu,
+                           const float *slowness, const float *ylam_ymu,
+                           const float *wl);
+
 /* bdry_t A/B/D of one face, num_layers+1 values each, already converted as
  * bdry_pml_set leaves them (D/=B, A+=D, B=1/B; bdry_t.c:259-268). */
 int cgfdh_set_pml_face(cgfdh_solver *s, int face, const float *A, const float *B,
@@ -101,6 +116,13 @@ int cgfdh_set_pml_face(cgfdh_solver *s, int face, const float *A, const float *B
  * sv_curv_col_el_aniso.c:1267-1417, leave matF2Vz2 zero). */
 int cgfdh_set_free_surface(cgfdh_solver *s, const float *matVx2Vz2,
                            const float *matVy2Vz2, const float *matF2Vz2);
+
+/* bdry_t matD [ny][nx][3][3] of the visco-elastic free surface
+ * (sv_curv_col_vis_iso.c:485-501), the rotation applied to the surface-row
+ * stresses after every step (sv_curv_col_vis_iso_free, drv_rk_curv_col.c:440-445);
+ * cgfdh_compute_metric builds it on the device for such a solver. */
+int cgfdh_set_free_surface_matD(cgfdh_solver *s, const float *matD);
+int cgfdh_get_free_surface_matD(cgfdh_solver *s, float *matD);
 
 /* Moment sources as src_t holds them (src_t.h; src_t.c:985-1174):
  *  ijk      local indices with ghosts (si,sj,sk), 3 per source;
@@ -120,7 +142,8 @@ int cgfdh_set_moment_sources(cgfdh_solver *s, int nsrc, const int *ijk,
 int cgfdh_set_receivers(cgfdh_solver *s, int nrecv, const int *indx1d,
                         const float *dijk);
 
-/* Whole-state access (level "pre": the state at the start of step it0). */
+/* Whole-state access (level "pre": the state at the start of step it0),
+ * ncomp volumes. */
 int cgfdh_set_wavefield(cgfdh_solver *s, const float *w9);
 int cgfdh_get_wavefield(cgfdh_solver *s, float *w9);
 size_t cgfdh_pml_aux_len(cgfdh_solver *s);   /* floats, all enabled faces */
@@ -131,7 +154,8 @@ int cgfdh_get_pml_aux(cgfdh_solver *s, float *aux);
  * stage launches per step, receivers sampled after every step. */
 int cgfdh_run(cgfdh_solver *s, int it0, int nsteps);
 
-/* seismo[(r*9 + icmp)*nt_total + it], the layout of iorecv->recvone[r].seismo */
+/* seismo[(r*ncomp + icmp)*nt_total + it], the layout of
+ * iorecv->recvone[r].seismo (all ncomp components, io_funcs.c:1598-1612) */
 int cgfdh_get_seismograms(cgfdh_solver *s, float *seismo);
 
 /* Peak ground motion on the free surface, updated after every step as
@@ -200,6 +224,16 @@ void cgfdh_moment_from_angles(float strike, float dip, float rake, float mu, flo
 int cgfdh_ricker_moment_table(float t0, float dt, float stf_length, float wavelet_tstart,
                               float fc, float tpeak, const float *m6, int *it_begin,
                               int *it_end, int *max_nt, float *out, long out_stride);
+
+/* GMB visco-elastic coefficients (md_vis_GMB_cal_Y, md_t.c:643-792, with the
+ * least-squares fit md_visco_LS, md_t.c:795-914): relaxation frequencies wl
+ * log-spaced over [fmin, fmax], Y fitted so 1/Q is flat at 1/round(Qp|Qs);
+ * lambda, mu (relaxed, npts values) are replaced in place by the unrelaxed
+ * moduli at the reference frequency freq; ylam_ymu gets the 2*nmaxwell
+ * volumes Ylam_n, Ymu_n of npts values in cgfdh_set_medium_visco's order. */
+int cgfdh_visco_gmb(long npts, int nmaxwell, float freq, float fmin, float fmax,
+                    float *lambda, float *mu, const float *Qp, const float *Qs,
+                    float *ylam_ymu, float *wl);
 
 /* Normalised Gaussian source weights (src_cal_norm_delt3d_z2fre,
  * src_t.c:2111-2151): (2*half+1)^3 values, normalised over k <= half_below. */

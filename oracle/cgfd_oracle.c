@@ -313,6 +313,164 @@ void orc_dvh2dvz(int mt, int ni, int nj, int nk, const float *metric10,
     }
 }
 
+/* visco-elastic surface matrices, sv_curv_col_vis_iso.c:354-506 */
+void orc_dvh2dvz_vis(int ni, int nj, int nk, const float *metric10,
+                     const float *med, const float *xyz, float *mats4)
+{
+  size_t nx = ni + 6, ny = nj + 6, nz = nk + 6;
+  size_t vol = nx * ny * nz, ss = nx * ny;
+  const float *x3d = xyz, *y3d = xyz + vol, *z3d = xyz + 2 * vol;
+  float *mVx = mats4, *mVy = mats4 + ss * 9, *mD = mats4 + 3 * ss * 9;
+  /* the isotropic matrices: the same expressions (:435-468) */
+  float *tmp = (float *)calloc(3 * ss * 9, sizeof(float));
+  orc_dvh2dvz_iso(ni, nj, nk, metric10, med, tmp);
+  size_t k = ORC_NGHOST + nk - 1;
+  float fc[7];
+  for (int n = 0; n < 7; n++) fc[n] = (float)fdc_coef[n];
+  for (size_t j = ORC_NGHOST; j < ORC_NGHOST + (size_t)nj; j++)
+    for (size_t i = ORC_NGHOST; i < ORC_NGHOST + (size_t)ni; i++) {
+      size_t p = i + j * nx + k * ss, ij = (j * nx + i) * 9;
+      float ztx = metric10[7 * vol + p], zty = metric10[8 * vol + p], ztz = metric10[9 * vol + p];
+      /* M_FD_SHIFT along eta with the centred op (fd->fdc, fd_t.c:161-171) */
+      float xet = fc[0] * x3d[p - 3 * nx], yet = fc[0] * y3d[p - 3 * nx], zet = fc[0] * z3d[p - 3 * nx];
+      for (int n = 1; n < 7; n++) {
+        long q = (long)p + (long)(n - 3) * (long)nx;
+        xet += fc[n] * x3d[q];
+        yet += fc[n] * y3d[q];
+        zet += fc[n] * z3d[q];
+      }
+      float e_n = 1.0 / sqrt(xet * xet + yet * yet + zet * zet);
+      float e_m = 1.0 / sqrt(ztx * ztx + zty * zty + ztz * ztz);
+      float e_nm = e_n * e_m;
+      float D[9];
+      D[0] = (yet * ztz - zet * zty) * e_nm;
+      D[1] = (zet * ztx - xet * ztz) * e_nm;
+      D[2] = (xet * zty - yet * ztx) * e_nm;
+      D[3] = xet * e_n;
+      D[4] = yet * e_n;
+      D[5] = zet * e_n;
+      D[6] = ztx * e_m;
+      D[7] = zty * e_m;
+      D[8] = ztz * e_m;
+      for (int q = 0; q < 9; q++) {
+        mVx[ij + q] = tmp[ij + q];
+        mVy[ij + q] = tmp[ss * 9 + ij + q];
+        mD[ij + q] = D[q];
+      }
+    }
+  free(tmp);
+}
+
+/* md_visco_LS_mat_inv (md_t.c:857-914): Gauss-Jordan with row swap on a zero
+ * pivot */
+static void ls_mat_inv(float m[][ORC_MAXWELL_MAX], float inv[][ORC_MAXWELL_MAX], int n)
+{
+  float tmp;
+  int j = 0;
+  for (int k = 0; k < n; k++) {
+    if (m[k][k] == 0) {
+      for (int jj = k + 1; jj < n; jj++) {
+        j = jj;
+        if (m[j][k] != 0) break;
+      }
+      for (int i = 0; i < n; i++) {
+        tmp = m[k][i]; m[k][i] = m[j][i]; m[j][i] = tmp;
+        tmp = inv[k][i]; inv[k][i] = inv[j][i]; inv[j][i] = tmp;
+      }
+    }
+    tmp = m[k][k];
+    for (int jj = 0; jj < n; jj++) {
+      m[k][jj] = m[k][jj] / tmp;
+      inv[k][jj] = inv[k][jj] / tmp;
+    }
+    for (int i = 0; i < n; i++) {
+      if (i == k) continue;
+      tmp = m[i][k];
+      for (int jj = 0; jj < n; jj++) {
+        m[i][jj] = m[i][jj] - m[k][jj] * tmp;
+        inv[i][jj] = inv[i][jj] - inv[k][jj] * tmp;
+      }
+    }
+  }
+}
+
+/* md_visco_LS (md_t.c:795-852): y = (G^T G)^-1 G^T d for a constant d */
+static void visco_ls(float G[][ORC_MAXWELL_MAX], float *y, float d, int m, int n)
+{
+  float tr[ORC_MAXWELL_MAX][2 * ORC_MAXWELL_MAX], mu[ORC_MAXWELL_MAX][ORC_MAXWELL_MAX];
+  float inv[ORC_MAXWELL_MAX][ORC_MAXWELL_MAX], pr[ORC_MAXWELL_MAX][2 * ORC_MAXWELL_MAX];
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) inv[i][j] = i == j ? 1 : 0;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < m; j++) tr[i][j] = G[j][i];
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      mu[i][j] = 0;
+      for (int k = 0; k < m; k++) mu[i][j] = mu[i][j] + tr[i][k] * G[k][j];
+    }
+  ls_mat_inv(mu, inv, n);
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < m; j++) {
+      pr[i][j] = 0;
+      for (int k = 0; k < n; k++) pr[i][j] = pr[i][j] + inv[i][k] * tr[k][j];
+    }
+  for (int i = 0; i < n; i++) {
+    float sum = 0.0;
+    for (int j = 0; j < m; j++) sum = sum + pr[i][j] * d;
+    y[i] = sum;
+  }
+}
+
+void orc_visco_gmb(long npts, int nmaxwell, float freq, float fmin, float fmax,
+                   float *lambda, float *mu, const float *Qp, const float *Qs,
+                   float *Ylam, float *Ymu, float *wl)
+{
+  int kmax = 2 * nmaxwell - 1;
+  float wr = 2 * PI * freq, wmin = 2 * PI * fmin, wmax = 2 * PI * fmax;
+  float wratio = wmax / wmin;
+  float wk[2 * ORC_MAXWELL_MAX];
+  for (int k = 0; k < kmax; k++) wk[k] = wmin * pow(wratio, k / (float)(kmax - 1));
+  for (int n = 0; n < nmaxwell; n++) wl[n] = wk[2 * n];
+#pragma omp parallel for schedule(static)
+  for (long p = 0; p < npts; p++) {
+    float GP[2 * ORC_MAXWELL_MAX][ORC_MAXWELL_MAX], GS[2 * ORC_MAXWELL_MAX][ORC_MAXWELL_MAX];
+    float YP[ORC_MAXWELL_MAX], YS[ORC_MAXWELL_MAX];
+    float QP1 = 1.0 / round(Qp[p]), QS1 = 1.0 / round(Qs[p]);
+    float lam = lambda[p], muu = mu[p];
+    for (int m = 0; m < kmax; m++)
+      for (int n = 0; n < nmaxwell; n++) {
+        GP[m][n] = (wl[n] * wk[m] + pow(wl[n], 2) * QP1) / (pow(wl[n], 2) + pow(wk[m], 2));
+        GS[m][n] = (wl[n] * wk[m] + pow(wl[n], 2) * QS1) / (pow(wl[n], 2) + pow(wk[m], 2));
+      }
+    visco_ls(GP, YP, QP1, kmax, nmaxwell);
+    visco_ls(GS, YS, QS1, kmax, nmaxwell);
+    float theta1, theta2, thetatmp = 0.0, R;
+    for (int n = 0; n < nmaxwell; n++) thetatmp = thetatmp + YP[n] / (1 + pow(wr / wl[n], 2));
+    theta1 = 1 - thetatmp;
+    thetatmp = 0.0;
+    for (int n = 0; n < nmaxwell; n++)
+      thetatmp = thetatmp + YP[n] * wr / wl[n] / (1 + pow(wr / wl[n], 2));
+    theta2 = thetatmp;
+    R = sqrt(pow(theta1, 2) + pow(theta2, 2));
+    float kappa = (lam + 2 * muu) * (R + theta1) / (2 * pow(R, 2));
+    thetatmp = 0.0;
+    for (int n = 0; n < nmaxwell; n++) thetatmp = thetatmp + YS[n] / (1 + pow(wr / wl[n], 2));
+    theta1 = 1 - thetatmp;
+    thetatmp = 0.0;
+    for (int n = 0; n < nmaxwell; n++)
+      thetatmp = thetatmp + YS[n] * wr / wl[n] / (1 + pow(wr / wl[n], 2));
+    theta2 = thetatmp;
+    R = sqrt(pow(theta1, 2) + pow(theta2, 2));
+    float muunrelax = muu * (R + theta1) / (2 * pow(R, 2));
+    mu[p] = muunrelax;
+    lambda[p] = kappa - 2 * muunrelax;
+    for (int n = 0; n < nmaxwell; n++) {
+      Ylam[p * nmaxwell + n] = (1 + 2 * mu[p] / lambda[p]) * YP[n] - 2 * mu[p] / lambda[p] * YS[n];
+      Ymu[p * nmaxwell + n] = YS[n];
+    }
+  }
+}
+
 void orc_dvh2dvz_iso(int ni, int nj, int nk, const float *metric10,
                      const float *med3, float *mats)
 {
@@ -497,6 +655,9 @@ void orc_angle2moment(float strike, float dip, float rake, float m[6])
 /* solver state                                                             */
 /* ------------------------------------------------------------------------ */
 
+/* the visco-elastic medium runs the isotropic elastic RHS (lambda, mu) */
+#define ISO_LIKE(o) ((o)->mt == ORC_MEDIUM_ISO || (o)->mt == ORC_MEDIUM_VISCO)
+
 struct orc_s {
   orc_cfg cfg;
   int nx, ny, nz;
@@ -504,9 +665,12 @@ struct orc_s {
   int ni1, ni2, nj1, nj2, nk1, nk2;
   real_t *metric;           /* 10 volumes */
   int mt, ncmp;             /* medium type, volumes per medium */
+  int nw;                   /* wavefield components: 9, + 6 per Maxwell body */
+  int nmax;                 /* visco-elastic: Maxwell bodies */
+  real_t wl[ORC_MAXWELL_MAX];
   real_t *med;              /* ncmp volumes, slowness last */
   const real_t *slw;
-  real_t *mats;             /* matVx2Vz2, matVy2Vz2, matF2Vz2 */
+  real_t *mats;             /* matVx2Vz2, matVy2Vz2, matF2Vz2, matD */
   int region[6][6];
   size_t aux_off[6], aux_n[6], aux_total;
   real_t *pA[6], *pB[6], *pD[6];
@@ -516,6 +680,7 @@ struct orc_s {
 
 size_t orc_aux_len(const orc_t *o) { return o->aux_total; }
 size_t orc_vol(const orc_t *o) { return o->vol; }
+int orc_ncomp(const orc_t *o) { return o->nw; }
 
 orc_t *orc_create(const orc_cfg *cfg, const float *metric10, const float *med)
 {
@@ -523,6 +688,13 @@ orc_t *orc_create(const orc_cfg *cfg, const float *metric10, const float *med)
   o->cfg = *cfg;
   o->mt = cfg->medium_type ? cfg->medium_type : ORC_MEDIUM_ISO;
   o->ncmp = orc_medium_ncmp(o->mt);
+  o->nw = 9;
+  if (o->mt == ORC_MEDIUM_VISCO) {
+    o->nmax = cfg->nmaxwell;
+    o->ncmp = 3 + 2 * o->nmax;
+    o->nw = 9 + 6 * o->nmax;
+    for (int n = 0; n < o->nmax; n++) o->wl[n] = cfg->visco_wl[n];
+  }
   o->nx = cfg->ni + 6; o->ny = cfg->nj + 6; o->nz = cfg->nk + 6;
   o->sl = o->nx; o->ss = (size_t)o->nx * o->ny; o->vol = o->ss * o->nz;
   o->ni1 = 3; o->ni2 = 3 + cfg->ni - 1;
@@ -532,14 +704,17 @@ orc_t *orc_create(const orc_cfg *cfg, const float *metric10, const float *med)
   o->med = (real_t *)malloc(o->ncmp * o->vol * sizeof(real_t));
   for (size_t p = 0; p < 10 * o->vol; p++) o->metric[p] = metric10[p];
   for (size_t p = 0; p < o->ncmp * o->vol; p++) o->med[p] = med[p];
-  o->slw = o->med + (o->ncmp - 1) * o->vol;
+  o->slw = o->med + (ISO_LIKE(o) ? 2 : o->ncmp - 1) * o->vol;
   /* free-surface matrices, computed in the oracle's own precision */
   size_t nm = o->ss * 9;
-  o->mats = (real_t *)calloc(3 * nm, sizeof(real_t));
+  o->mats = (real_t *)calloc(4 * nm, sizeof(real_t));
   if (cfg->free_top) {
-    float *m = (float *)calloc(3 * nm, sizeof(float));
-    orc_dvh2dvz(o->mt, cfg->ni, cfg->nj, cfg->nk, metric10, med, m);
-    for (size_t p = 0; p < 3 * nm; p++) o->mats[p] = m[p];
+    float *m = (float *)calloc(4 * nm, sizeof(float));
+    if (o->mt == ORC_MEDIUM_VISCO)
+      orc_dvh2dvz_vis(cfg->ni, cfg->nj, cfg->nk, metric10, med, cfg->xyz, m);
+    else
+      orc_dvh2dvz(o->mt, cfg->ni, cfg->nj, cfg->nk, metric10, med, m);
+    for (size_t p = 0; p < 4 * nm; p++) o->mats[p] = m[p];
     free(m);
     o->pg = (real_t *)calloc(15 * o->ss, sizeof(real_t));
     o->dis = (real_t *)calloc(3 * o->ss, sizeof(real_t));
@@ -583,6 +758,7 @@ void orc_destroy(orc_t *o)
 
 /* wavefield component slots (wav_t.c:93-136) */
 enum { VX = 0, VY, VZ, TXX, TYY, TZZ, TYZ, TXZ, TXY };
+
 
 typedef struct { int len; long sh[5]; real_t c[5]; } op_t;
 
@@ -628,7 +804,7 @@ static void rhs_inner(orc_t *o, const real_t *w, real_t *h,
         real_t ztx = M[7 * V + p], zty = M[8 * V + p], ztz = M[9 * V + p];
         real_t slw = o->slw[p];
         real_t (*X) = D[0], (*Y) = D[1], (*Z) = D[2];
-        if (o->mt != ORC_MEDIUM_ISO) {
+        if (!ISO_LIKE(o)) {
           /* sv_curv_col_el_vti.c:453-475, sv_curv_col_el_aniso.c:455-491 */
           real_t c[21], e[3][3], Dv[3][3], out[6];
           coupling_medium(o->mt, md, V, p, c);
@@ -646,7 +822,7 @@ static void rhs_inner(orc_t *o, const real_t *w, real_t *h,
         h[VZ * V + p] = slw * (xix * X[TXZ] + xiy * X[TYZ] + xiz * X[TZZ]
                               + etx * Y[TXZ] + ety * Y[TYZ] + etz * Y[TZZ]
                               + ztx * Z[TXZ] + zty * Z[TYZ] + ztz * Z[TZZ]);
-        if (o->mt != ORC_MEDIUM_ISO) continue;
+        if (!ISO_LIKE(o)) continue;
         real_t lam = md[p], mu = md[V + p];
         real_t lam2mu = lam + 2.0 * mu;
         h[TXX * V + p] = lam2mu * (xix * X[VX] + etx * Y[VX] + ztx * Z[VX])
@@ -780,7 +956,7 @@ static void rhs_vlow(orc_t *o, const real_t *w, real_t *h,
         } else {
           for (int c = 0; c < 3; c++) Z[c] = dop(&ozl[n], w + c * V, p);
         }
-        if (o->mt == ORC_MEDIUM_ISO) {
+        if (ISO_LIKE(o)) {
           real_t lam = md[p], mu = md[V + p];
           real_t lam2mu = lam + 2.0 * mu;
           hooke(h, V, p, m9, lam, lam2mu, mu, X, Y, Z);
@@ -825,7 +1001,7 @@ static void rhs_pml(orc_t *o, const real_t *w, real_t *h, const real_t *aux,
           real_t e1 = mt[p], e2 = mt[V + p], e3 = mt[2 * V + p];
           real_t slw = o->slw[p];
           real_t lam = 0, mu = 0, lam2mu = 0, cm[21];
-          if (o->mt == ORC_MEDIUM_ISO) {
+          if (ISO_LIKE(o)) {
             lam = md[p]; mu = md[V + p];
             lam2mu = lam + 2.0 * mu;
           } else {
@@ -837,7 +1013,7 @@ static void rhs_pml(orc_t *o, const real_t *w, real_t *h, const real_t *aux,
           rr[VX] = slw * (e1 * D[TXX] + e2 * D[TXY] + e3 * D[TXZ]);
           rr[VY] = slw * (e1 * D[TXY] + e2 * D[TYY] + e3 * D[TYZ]);
           rr[VZ] = slw * (e1 * D[TXZ] + e2 * D[TYZ] + e3 * D[TZZ]);
-          if (o->mt == ORC_MEDIUM_ISO) {
+          if (ISO_LIKE(o)) {
             rr[TXX] = lam2mu * e1 * D[VX] + lam * e2 * D[VY] + lam * e3 * D[VZ];
             rr[TYY] = lam * e1 * D[VX] + lam2mu * e2 * D[VY] + lam * e3 * D[VZ];
             rr[TZZ] = lam * e1 * D[VX] + lam * e2 * D[VY] + lam2mu * e3 * D[VZ];
@@ -868,7 +1044,7 @@ static void rhs_pml(orc_t *o, const real_t *w, real_t *h, const real_t *aux,
                       + mm[ij + 3 * rr3 + 2] * D[VZ];
             real_t ztx = M[7 * V + p], zty = M[8 * V + p], ztz = M[9 * V + p];
             real_t s[9];
-            if (o->mt == ORC_MEDIUM_ISO) {
+            if (ISO_LIKE(o)) {
               s[TXX] = lam2mu * (ztx * Dz[0]) + lam * (zty * Dz[1] + ztz * Dz[2]);
               s[TYY] = lam2mu * (zty * Dz[1]) + lam * (ztx * Dz[0] + ztz * Dz[2]);
               s[TZZ] = lam2mu * (ztz * Dz[2]) + lam * (ztx * Dz[0] + zty * Dz[1]);
@@ -937,6 +1113,115 @@ static void rhs_src(orc_t *o, int it, int istage, real_t *h)
   free(ext);
 }
 
+/* GMB attenuation after the PML terms, sv_curv_col_vis_iso.c:251-342: memory
+ * variable rates and the memory terms subtracted from the stress rates.
+ * J slots of body n: 9+6n + {Jxx, Jyy, Jzz, Jyz, Jxz, Jxy} */
+static void rhs_atten(orc_t *o, const real_t *w, real_t *h)
+{
+  size_t V = o->vol;
+  const real_t *md = o->med;
+  int nm = o->nmax;
+#pragma omp parallel for schedule(static)
+  for (int k = o->nk1; k <= o->nk2; k++)
+    for (int j = o->nj1; j <= o->nj2; j++)
+      for (int i = o->ni1; i <= o->ni2; i++) {
+        size_t p = i + j * o->sl + k * o->ss;
+        real_t lam = md[p], mu = md[V + p];
+        real_t hTxx = h[TXX * V + p], hTyy = h[TYY * V + p], hTzz = h[TZZ * V + p];
+        real_t hTxy = h[TXY * V + p], hTxz = h[TXZ * V + p], hTyz = h[TYZ * V + p];
+        real_t sum_hxyz = (hTxx + hTyy + hTzz) / (3 * lam + 2 * mu);
+        real_t EVxx = ((2.0 * hTxx - hTyy - hTzz) / (2 * mu) + sum_hxyz) / 3;
+        real_t EVyy = ((2.0 * hTyy - hTxx - hTzz) / (2 * mu) + sum_hxyz) / 3;
+        real_t EVzz = ((2.0 * hTzz - hTxx - hTyy) / (2 * mu) + sum_hxyz) / 3;
+        real_t EVxy = hTxy / mu * 0.5;
+        real_t EVxz = hTxz / mu * 0.5;
+        real_t EVyz = hTyz / mu * 0.5;
+        real_t sum_Jxyz = 0.0, sum_Jxx = 0.0, sum_Jyy = 0.0, sum_Jzz = 0.0;
+        real_t sum_Jxy = 0.0, sum_Jxz = 0.0, sum_Jyz = 0.0;
+        for (int n = 0; n < nm; n++) {
+          size_t b = (size_t)(9 + 6 * n) * V + p;
+          real_t Jxx = w[b], Jyy = w[b + V], Jzz = w[b + 2 * V];
+          real_t Jyz = w[b + 3 * V], Jxz = w[b + 4 * V], Jxy = w[b + 5 * V];
+          real_t wl = o->wl[n];
+          h[b] = wl * (EVxx - Jxx);
+          h[b + V] = wl * (EVyy - Jyy);
+          h[b + 2 * V] = wl * (EVzz - Jzz);
+          h[b + 5 * V] = wl * (EVxy - Jxy);
+          h[b + 4 * V] = wl * (EVxz - Jxz);
+          h[b + 3 * V] = wl * (EVyz - Jyz);
+        }
+        for (int n = 0; n < nm; n++) {
+          size_t b = (size_t)(9 + 6 * n) * V + p;
+          real_t Jxx = w[b], Jyy = w[b + V], Jzz = w[b + 2 * V];
+          real_t Jyz = w[b + 3 * V], Jxz = w[b + 4 * V], Jxy = w[b + 5 * V];
+          real_t Yl = md[(3 + 2 * n) * V + p], Ym = md[(4 + 2 * n) * V + p];
+          sum_Jxyz += Yl * (Jxx + Jyy + Jzz);
+          sum_Jxx += Ym * Jxx;
+          sum_Jyy += Ym * Jyy;
+          sum_Jzz += Ym * Jzz;
+          sum_Jxy += Ym * Jxy;
+          sum_Jxz += Ym * Jxz;
+          sum_Jyz += Ym * Jyz;
+        }
+        real_t mem_Txx = lam * sum_Jxyz + 2.0 * mu * sum_Jxx;
+        real_t mem_Tyy = lam * sum_Jxyz + 2.0 * mu * sum_Jyy;
+        real_t mem_Tzz = lam * sum_Jxyz + 2.0 * mu * sum_Jzz;
+        real_t mem_Txy = 2.0 * mu * sum_Jxy;
+        real_t mem_Txz = 2.0 * mu * sum_Jxz;
+        real_t mem_Tyz = 2.0 * mu * sum_Jyz;
+        h[TXX * V + p] = hTxx - mem_Txx;
+        h[TYY * V + p] = hTyy - mem_Tyy;
+        h[TZZ * V + p] = hTzz - mem_Tzz;
+        h[TXY * V + p] = hTxy - mem_Txy;
+        h[TXZ * V + p] = hTxz - mem_Txz;
+        h[TYZ * V + p] = hTyz - mem_Tyz;
+      }
+}
+
+/* rotation of the surface-row stresses after the last stage,
+ * sv_curv_col_vis_iso_free (sv_curv_col_vis_iso.c:510-623) */
+static void vis_free(orc_t *o, real_t *w)
+{
+  size_t V = o->vol;
+  const real_t *mD = o->mats + 3 * o->ss * 9;
+  real_t *Txx = w + TXX * V, *Tyy = w + TYY * V, *Tzz = w + TZZ * V;
+  real_t *Txy = w + TXY * V, *Txz = w + TXZ * V, *Tyz = w + TYZ * V;
+  for (int j = o->nj1; j <= o->nj2; j++)
+    for (int i = o->ni1; i <= o->ni2; i++) {
+      size_t p = i + j * o->sl + (size_t)o->nk2 * o->ss, ij = (i + j * o->sl) * 9;
+      real_t D[3][3], DT[3][3], Tl[3][3], DTl[3][3], Tg[3][3];
+      for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) { D[r][c] = mD[ij + 3 * r + c]; DT[c][r] = D[r][c]; }
+      real_t d11 = D[0][0], d12 = D[0][1], d13 = D[0][2];
+      real_t d21 = D[1][0], d22 = D[1][1], d23 = D[1][2];
+      Tl[0][0] = d11 * d11 * Txx[p] + d12 * d12 * Tyy[p] + d13 * d13 * Tzz[p]
+               + 2 * (d11 * d12 * Txy[p] + d11 * d13 * Txz[p] + d12 * d13 * Tyz[p]);
+      Tl[0][1] = d11 * d21 * Txx[p] + d12 * d22 * Tyy[p] + d13 * d23 * Tzz[p]
+               + (d11 * d22 + d12 * d21) * Txy[p] + (d11 * d23 + d21 * d13) * Txz[p]
+               + (d12 * d23 + d22 * d13) * Tyz[p];
+      Tl[1][1] = d21 * d21 * Txx[p] + d22 * d22 * Tyy[p] + d23 * d23 * Tzz[p]
+               + 2 * (d21 * d22 * Txy[p] + d21 * d23 * Txz[p] + d22 * d23 * Tyz[p]);
+      Tl[1][0] = Tl[0][1];
+      Tl[0][2] = 0.0; Tl[1][2] = 0.0; Tl[2][0] = 0.0; Tl[2][1] = 0.0; Tl[2][2] = 0.0;
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+          DTl[a][b] = 0.0;
+          for (int c = 0; c < 3; c++) DTl[a][b] += DT[a][c] * Tl[c][b];
+        }
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+          Tg[a][b] = 0.0;
+          for (int c = 0; c < 3; c++) Tg[a][b] += DTl[a][c] * D[c][b];
+        }
+      Txx[p] = Tg[0][0];
+      Tyy[p] = Tg[1][1];
+      Tzz[p] = Tg[2][2];
+      Txy[p] = Tg[0][1];
+      Txz[p] = Tg[0][2];
+      Tyz[p] = Tg[1][2];
+    }
+}
+
 void orc_onestage(orc_t *o, int it, int ipair, int istage,
                   const real_t *w_cur, const real_t *aux_cur,
                   real_t *rhs, real_t *aux_rhs)
@@ -955,6 +1240,7 @@ void orc_onestage(orc_t *o, int it, int ipair, int istage,
     rhs_vlow(o, w_cur, rhs, &ox, &oy, ozl);
   }
   if (o->aux_total) rhs_pml(o, w_cur, rhs, aux_cur, aux_rhs, &ox, &oy, &oz);
+  if (o->mt == ORC_MEDIUM_VISCO) rhs_atten(o, w_cur, rhs);
   if (o->cfg.nsrc > 0) rhs_src(o, it, istage, rhs);
 }
 
@@ -1013,7 +1299,7 @@ void orc_run(orc_t *o, float dt, int it0, int nt, real_t *w, real_t *aux,
              int nrecv, const int *recv_idx, const float *recv_w,
              int nt_total, real_t *seismo)
 {
-  size_t L = 9 * o->vol, LA = o->aux_total;
+  size_t L = (size_t)o->nw * o->vol, LA = o->aux_total;
   real_t *tmp = (real_t *)calloc(L, sizeof(real_t));
   real_t *rhs = (real_t *)calloc(L, sizeof(real_t));
   real_t *end = (real_t *)calloc(L, sizeof(real_t));
@@ -1047,15 +1333,17 @@ void orc_run(orc_t *o, float dt, int it0, int nt, real_t *w, real_t *aux,
         if (s == 0) aend[p] = apre[p] + cb * arhs[p];
         else aend[p] += cb * arhs[p];
       }
+      /* drv_rk_curv_col.c:440-445, after the last stage's combine */
+      if (s == 3 && o->mt == ORC_MEDIUM_VISCO && o->cfg.free_top) vis_free(o, end);
     }
     /* receivers, io_funcs.c:1585-1623 (indx1d[5] twice, indx1d[4] unused) */
     for (int rcv = 0; rcv < nrecv; rcv++) {
       const int *ix = recv_idx + 8 * rcv;
       real_t Lx2 = recv_w[3 * rcv], Ly2 = recv_w[3 * rcv + 1], Lz2 = recv_w[3 * rcv + 2];
       real_t Lx1 = 1.0 - Lx2, Ly1 = 1.0 - Ly2, Lz1 = 1.0 - Lz2;
-      for (int c = 0; c < 9; c++) {
+      for (int c = 0; c < o->nw; c++) {
         const real_t *v = end + c * o->vol;
-        seismo[((size_t)rcv * 9 + c) * nt_total + it] =
+        seismo[((size_t)rcv * o->nw + c) * nt_total + it] =
             v[ix[0]] * Lx1 * Ly1 * Lz1 + v[ix[1]] * Lx2 * Ly1 * Lz1
           + v[ix[2]] * Lx1 * Ly2 * Lz1 + v[ix[3]] * Lx2 * Ly2 * Lz1
           + v[ix[5]] * Lx1 * Ly1 * Lz2 + v[ix[5]] * Lx2 * Ly1 * Lz2

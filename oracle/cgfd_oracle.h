@@ -19,6 +19,10 @@
  *   VTI (3):        c11, c13, c33, c55, c66, slw
  *   anisotropic (4): c11,c12,c13,c14,c15,c16,c22,c23,c24,c25,c26,c33,c34,c35,
  *                    c36,c44,c45,c46,c55,c56,c66, slw
+ *   visco-elastic GMB (5): lambda, mu (unrelaxed), slw, then Ylam_n, Ymu_n for
+ *                    each of the nmaxwell bodies (md_t.c:222-249); the
+ *                    wavefield then carries 6 memory variables per body after
+ *                    the 9 fields, Jxx,Jyy,Jzz,Jyz,Jxz,Jxy (wav_t.c:138-171)
  */
 #ifndef CGFD_ORACLE_H
 #define CGFD_ORACLE_H
@@ -36,6 +40,8 @@ typedef ORC_REAL real_t;
 #define ORC_MEDIUM_ISO   2
 #define ORC_MEDIUM_VTI   3
 #define ORC_MEDIUM_ANISO 4
+#define ORC_MEDIUM_VISCO 5
+#define ORC_MAXWELL_MAX 8
 
 /* face order matches the reference's idim*2+iside: x1,x2,y1,y2,z1,z2 */
 typedef struct {
@@ -58,6 +64,10 @@ typedef struct {
   int src_max_nt;
   const float *src_mij;    /* Mxx,Myy,Mzz,Mxz,Myz,Mxy, each [nsrc][max_nt][4] */
   int medium_type;         /* ORC_MEDIUM_*; 0 is read as isotropic */
+  /* visco-elastic only */
+  int nmaxwell;            /* memory-variable bodies */
+  const float *visco_wl;   /* relaxation frequencies, nmaxwell values */
+  const float *xyz;        /* coordinates (3 volumes), for the surface matD */
 } orc_cfg;
 
 typedef struct orc_s orc_t;
@@ -82,6 +92,18 @@ void orc_dvh2dvz_iso(int ni, int nj, int nk, const float *metric10,
 void orc_dvh2dvz(int medium_type, int ni, int nj, int nk, const float *metric10,
                  const float *med, float *mats);
 int orc_medium_ncmp(int medium_type);
+/* visco-elastic free-surface matrices, sv_curv_col_vis_iso_dvh2dvz
+ * (sv_curv_col_vis_iso.c:354-506): matVx2Vz2, matVy2Vz2 as the isotropic
+ * ones, matD (the 4th [ny][nx][3][3] array) from the eta derivatives of the
+ * coordinates with the centred 7-point operator; matF2Vz2 left untouched */
+void orc_dvh2dvz_vis(int ni, int nj, int nk, const float *metric10,
+                     const float *med, const float *xyz, float *mats4);
+/* GMB memory-variable coefficients, md_vis_GMB_cal_Y (md_t.c:643-792) with
+ * md_visco_LS (:795-852): from relaxed lambda, mu and Qp, Qs at npts points,
+ * in place to unrelaxed lambda, mu, plus Ylam/Ymu [npts][nmaxwell] and wl */
+void orc_visco_gmb(long npts, int nmaxwell, float freq, float fmin, float fmax,
+                   float *lambda, float *mu, const float *Qp, const float *Qs,
+                   float *Ylam, float *Ymu, float *wl);
 void orc_metric(int ni, int nj, int nk, const float *xyz /* 3 volumes */,
                 float *metric10);
 void orc_gauss_weights(float *w, float x0, float y0, float z0, float a,
@@ -94,6 +116,7 @@ orc_t *orc_create(const orc_cfg *cfg, const float *metric10, const float *med);
 void   orc_destroy(orc_t *o);
 size_t orc_aux_len(const orc_t *o);           /* floats of all PML aux, 1 level */
 size_t orc_vol(const orc_t *o);
+int    orc_ncomp(const orc_t *o);             /* wavefield components */
 
 /* one RHS evaluation: rhs must be zero outside the physical region */
 void orc_onestage(orc_t *o, int it, int ipair, int istage,

@@ -25,11 +25,11 @@ class OrcCfg(C.Structure):
                 ("nsrc", C.c_int), ("src_gauss", C.c_int), ("src_ext_half", C.c_int),
                 ("src_ext_coef", C.c_float),
                 ("src_ijk", _P), ("src_inc", _P), ("src_it_begin", _P), ("src_it_end", _P),
-                ("src_max_nt", C.c_int), ("src_mij", _P), ("medium_type", C.c_int)]
+                ("src_max_nt", C.c_int), ("src_mij", _P), ("medium_type", C.c_int),
+                ("nmaxwell", C.c_int), ("visco_wl", _P), ("xyz", _P)]
 
 
-MEDIUM_ISO, MEDIUM_VTI, MEDIUM_ANISO = 2, 3, 4
-MEDIUM_NCMP = {MEDIUM_ISO: 3, MEDIUM_VTI: 6, MEDIUM_ANISO: 22}
+MEDIUM_ISO, MEDIUM_VTI, MEDIUM_ANISO, MEDIUM_VISCO = 2, 3, 4, 5
 
 
 class RefCfg(C.Structure):
@@ -39,7 +39,7 @@ class RefCfg(C.Structure):
                 ("pml_vel", C.c_float * 6),
                 ("nsrc", C.c_int), ("src_max_nt", C.c_int), ("src_ijk", _P),
                 ("src_it_begin", _P), ("src_it_end", _P), ("src_mij", _P),
-                ("medium_type", C.c_int)]
+                ("medium_type", C.c_int), ("nmaxwell", C.c_int), ("visco_wl", _P)]
 
 
 def load_oracle(fp64=False):
@@ -63,6 +63,11 @@ def load_oracle(fp64=False):
     lib.orc_pml_region.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P, _P]
     lib.orc_dvh2dvz_iso.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P, _P]
     lib.orc_dvh2dvz.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P]
+    lib.orc_dvh2dvz_vis.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]
+    lib.orc_visco_gmb.argtypes = [C.c_long, C.c_int, C.c_float, C.c_float, C.c_float,
+                                  _P, _P, _P, _P, _P, _P, _P]
+    lib.orc_ncomp.restype = C.c_int
+    lib.orc_ncomp.argtypes = [_P]
     lib.orc_metric.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P]
     lib.orc_gauss_weights.argtypes = [_P, C.c_float, C.c_float, C.c_float, C.c_float,
                                       C.c_int, C.c_int]
@@ -82,6 +87,7 @@ def load_ref():
     lib.refh_fd_ops.argtypes = [_P] * 6
     lib.refh_pml.argtypes = [_P, _P, C.c_int, _P, _P, _P, _P]
     lib.refh_dvh2dvz.argtypes = [_P, _P, _P, _P]
+    lib.refh_dvh2dvz_vis.argtypes = [_P, _P, _P, _P, _P]
     lib.refh_onestage.restype = C.c_long
     lib.refh_onestage.argtypes = [_P, _P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]
     lib.refh_run.argtypes = [_P, _P, _P, _P, C.c_float, C.c_int, _P, C.c_int, _P, _P,
@@ -105,6 +111,13 @@ class Oracle:
         cfg.free_top = int(case.get("free_top", 0))
         cfg.medium_type = int(case.get("medium_type", MEDIUM_ISO))
         self._keep = []
+        if cfg.medium_type == MEDIUM_VISCO:
+            wl = np.ascontiguousarray(case["visco_wl"], np.float32)
+            xyz = np.ascontiguousarray(case["coords"], np.float32)
+            self._keep += [wl, xyz]
+            cfg.nmaxwell = len(wl)
+            cfg.visco_wl = wl.ctypes.data
+            cfg.xyz = xyz.ctypes.data
         for f in range(6):
             on = int(case.get("pml_on", [0] * 6)[f])
             cfg.pml_on[f] = on
@@ -139,6 +152,7 @@ class Oracle:
         self.med = np.ascontiguousarray(case["med"], np.float32)
         self.h = self.lib.orc_create(C.byref(cfg), ptr(self.metric), ptr(self.med))
         self.vol = self.lib.orc_vol(self.h)
+        self.ncomp = self.lib.orc_ncomp(self.h)
         self.aux_len = self.lib.orc_aux_len(self.h)
 
     def __del__(self):
@@ -148,7 +162,7 @@ class Oracle:
 
     def onestage(self, it, ipair, istage, w_cur, aux_cur=None):
         w = np.ascontiguousarray(w_cur, self.dtype)
-        rhs = np.zeros(9 * self.vol, self.dtype)
+        rhs = np.zeros(self.ncomp * self.vol, self.dtype)
         aux = np.ascontiguousarray(aux_cur if aux_cur is not None
                                    else np.zeros(self.aux_len), self.dtype)
         arhs = np.zeros(max(self.aux_len, 1), self.dtype)
@@ -156,13 +170,13 @@ class Oracle:
         return rhs, arhs[: self.aux_len]
 
     def run(self, dt, nt, w=None, aux=None, it0=0, recv_idx=None, recv_w=None, nt_total=None):
-        w = (np.zeros(9 * self.vol, self.dtype) if w is None
+        w = (np.zeros(self.ncomp * self.vol, self.dtype) if w is None
              else np.array(w, self.dtype, copy=True))
         aux = (np.zeros(max(self.aux_len, 1), self.dtype) if aux is None
                else np.array(aux, self.dtype, copy=True))
         nrecv = 0 if recv_idx is None else len(recv_idx)
         nt_total = nt_total or (it0 + nt)
-        seismo = np.zeros((max(nrecv, 1), 9, nt_total), self.dtype)
+        seismo = np.zeros((max(nrecv, 1), self.ncomp, nt_total), self.dtype)
         ri = None if recv_idx is None else np.ascontiguousarray(recv_idx, np.int32)
         rw = None if recv_w is None else np.ascontiguousarray(recv_w, np.float32)
         self.lib.orc_run(self.h, dt, it0, nt, ptr(w), ptr(aux), nrecv, ptr(ri), ptr(rw),

@@ -1,7 +1,7 @@
 /*
  * ref_harness.c -- test-only driver around the reference's own hot-path
  * sources (fd_t.c, sv_curv_col_el.c, sv_curv_col_el_iso.c,
- * sv_curv_col_el_vti.c, sv_curv_col_el_aniso.c, bdry_t.c, wav_t.c,
+ * sv_curv_col_el_vti.c, sv_curv_col_el_aniso.c, sv_curv_col_vis_iso.c, bdry_t.c, wav_t.c,
  * fdlib_math.c, fdlib_mem.c), compiled unchanged from /root/reference by
  * oracle/Makefile into oracle/_ref/libcgfd3d_ref.so.
  *
@@ -36,6 +36,7 @@
 #include "sv_curv_col_el_iso.h"
 #include "sv_curv_col_el_vti.h"
 #include "sv_curv_col_el_aniso.h"
+#include "sv_curv_col_vis_iso.h"
 
 typedef struct {
   int ni, nj, nk;
@@ -56,6 +57,10 @@ typedef struct {
    * lambda,mu,slw / c11,c13,c33,c55,c66,slw / the 21 Cij (c11,c12,..,c66) and
    * slw, one volume each */
   int medium_type;
+  /* CONST_MEDIUM_VISCOELASTIC_ISO: lambda, mu, slw, then Ylam_n, Ymu_n per
+   * Maxwell body; wl the bodies' relaxation frequencies */
+  int nmaxwell;
+  const float *visco_wl;
 } refh_cfg;
 
 typedef struct {
@@ -67,6 +72,7 @@ typedef struct {
   bdry_t bdry;
   src_t src;
   float *T0;   /* zero surface traction / velocity-source slices */
+  float *Ylam[16], *Ymu[16], wl[16];
 } refh_ctx;
 
 static void fill_gd(gd_t *gd, int ni, int nj, int nk,
@@ -129,8 +135,21 @@ static void setup(refh_ctx *c, const refh_cfg *cfg, float *coords,
   } else {
     md->lambda = med; md->mu = med + vol; md->rho = med + 2 * vol;
   }
+  int visco = md->medium_type == CONST_MEDIUM_VISCOELASTIC_ISO;
+  if (visco) {
+    md->nmaxwell = cfg->nmaxwell;
+    md->visco_type = CONST_VISCO_GMB;
+    for (int n = 0; n < cfg->nmaxwell; n++) {
+      c->Ylam[n] = med + (3 + 2 * n) * vol;
+      c->Ymu[n] = med + (4 + 2 * n) * vol;
+      c->wl[n] = cfg->visco_wl[n];
+    }
+    md->Ylam = c->Ylam;
+    md->Ymu = c->Ymu;
+    md->wl = c->wl;
+  }
 
-  wav_init(gd, &c->wav, 4, 0, 0);
+  wav_init(gd, &c->wav, 4, visco ? CONST_VISCO_GMB : 0, visco ? cfg->nmaxwell : 0);
 
   int neighid[6] = {MPI_PROC_NULL, MPI_PROC_NULL, MPI_PROC_NULL,
                     MPI_PROC_NULL, MPI_PROC_NULL, MPI_PROC_NULL};
@@ -190,6 +209,13 @@ static void teardown(refh_ctx *c)
     }
   free(c->bdry.matVx2Vz2); free(c->bdry.matVy2Vz2);
   free(c->bdry.matF2Vz2); free(c->bdry.matD);
+  free(c->wav.cmp_pos);
+  if (c->wav.Jxx_pos) {
+    free(c->wav.Jxx_pos); free(c->wav.Jyy_pos); free(c->wav.Jzz_pos);
+    free(c->wav.Jxy_pos); free(c->wav.Jxz_pos); free(c->wav.Jyz_pos);
+    free(c->wav.Jxx_seq); free(c->wav.Jyy_seq); free(c->wav.Jzz_seq);
+    free(c->wav.Jxy_seq); free(c->wav.Jxz_seq); free(c->wav.Jyz_seq);
+  }
   free(c->T0);
   free(c->src.si); free(c->src.sj); free(c->src.sk);
   free(c->src.it_begin); free(c->src.it_end);
@@ -365,6 +391,22 @@ static void harness_onestage(refh_ctx *c, float *w_cur, float *rhs, int ipair, i
                                   fx->total_len, fx->indx, fx->coef,
                                   fy->total_len, fy->indx, fy->coef,
                                   fz->total_len, fz->indx, fz->coef, &c->bdry, 0, 0);
+  if (mt == CONST_MEDIUM_VISCOELASTIC_ISO) {
+    /* sv_curv_col_vis_iso.c:207-215 */
+    int nm = md->nmaxwell;
+    float *J[6][16], *hJ[6][16];
+    for (int n = 0; n < nm; n++) {
+      const size_t pos[6] = {wav->Jxx_pos[n], wav->Jyy_pos[n], wav->Jzz_pos[n],
+                             wav->Jxy_pos[n], wav->Jxz_pos[n], wav->Jyz_pos[n]};
+      for (int q = 0; q < 6; q++) { J[q][n] = w_cur + pos[q]; hJ[q][n] = rhs + pos[q]; }
+    }
+    sv_curv_col_vis_iso_atten(hTxx, hTyy, hTzz, hTxz, hTyz, hTxy,
+                              J[0], J[1], J[2], J[3], J[4], J[5],
+                              hJ[0], hJ[1], hJ[2], hJ[3], hJ[4], hJ[5],
+                              md->lambda, md->mu, md->rho, md->wl, md->Ylam, md->Ymu,
+                              gd->ni1, gd->ni2, gd->nj1, gd->nj2, gd->nk1, gd->nk2, sl, ss,
+                              nm, 0, 0);
+  }
   if (c->src.total_number > 0)
     sv_curv_col_el_rhs_src(hVx, hVy, hVz, hTxx, hTyy, hTzz, hTxz, hTyz, hTxy,
                            m->jac, md->rho, gd, &c->src, 0, 0);
@@ -434,6 +476,9 @@ static void ref_dvh2dvz(refh_ctx *c)
     sv_curv_col_el_vti_dvh2dvz(&c->gd, &c->metric, &c->md, &c->bdry, 0);
   else if (mt == CONST_MEDIUM_ELASTIC_ANISO)
     sv_curv_col_el_aniso_dvh2dvz(&c->gd, &c->metric, &c->md, &c->bdry, 0);
+  else if (mt == CONST_MEDIUM_VISCOELASTIC_ISO)
+    sv_curv_col_vis_iso_dvh2dvz(&c->gd, &c->metric, &c->md, &c->bdry, c->fd.fdc_len,
+                                c->fd.fdc_indx, c->fd.fdc_coef, 0);
   else
     sv_curv_col_el_iso_dvh2dvz(&c->gd, &c->metric, &c->md, &c->bdry, 0);
 }
@@ -455,11 +500,30 @@ int refh_dvh2dvz(const refh_cfg *cfg, float *metric10, float *med3, float *mats_
   return 0;
 }
 
+/* visco-elastic matrices (sv_curv_col_vis_iso_dvh2dvz): mats_out holds 4
+ * arrays of nx*ny*9, matVx2Vz2, matVy2Vz2, matF2Vz2, matD */
+int refh_dvh2dvz_vis(const refh_cfg *cfg, float *coords, float *metric10, float *med,
+                     float *mats_out)
+{
+  refh_ctx c;
+  refh_cfg nopml = *cfg;
+  memset(nopml.pml_on, 0, sizeof nopml.pml_on);
+  setup(&c, &nopml, coords, metric10, med);
+  ref_dvh2dvz(&c);
+  size_t n = c.gd.siz_iz * 9;
+  memcpy(mats_out, c.bdry.matVx2Vz2, n * sizeof(float));
+  memcpy(mats_out + n, c.bdry.matVy2Vz2, n * sizeof(float));
+  memcpy(mats_out + 2 * n, c.bdry.matF2Vz2, n * sizeof(float));
+  memcpy(mats_out + 3 * n, c.bdry.matD, n * sizeof(float));
+  teardown(&c);
+  return 0;
+}
+
 static size_t aux_total(refh_ctx *c)
 {
   size_t t = 0;
   for (int f = 0; f < 6; f++)
-    if (c->bdry.is_sides_pml[f / 2][f % 2]) t += c->bdry.auxvar[f / 2][f % 2].siz_ilevel;
+    if (c->bdry.is_sides_pml[f / 2][f % 2]) t += 9 * c->bdry.auxvar[f / 2][f % 2].siz_icmp;
   return t;
 }
 
@@ -484,8 +548,10 @@ long refh_onestage(const refh_cfg *cfg, float *coords, float *metric10, float *m
     if (!c.bdry.is_sides_pml[f / 2][f % 2]) continue;
     a->cur = a->var;
     a->rhs = a->var + 2 * a->siz_ilevel;
-    if (aux_cur) memcpy(a->cur, aux_cur + off, a->siz_ilevel * sizeof(float));
-    off += a->siz_ilevel;
+    /* the 9 elastic components; a visco-elastic wavefield's memory variables
+     * have aux slots too (bdry_t.c:300), which stay zero */
+    if (aux_cur) memcpy(a->cur, aux_cur + off, 9 * a->siz_icmp * sizeof(float));
+    off += 9 * a->siz_icmp;
   }
   c.src.it = it;
   c.src.istage = istage;
@@ -495,8 +561,8 @@ long refh_onestage(const refh_cfg *cfg, float *coords, float *metric10, float *m
   for (int f = 0; f < 6; f++) {
     bdrypml_auxvar_t *a = &c.bdry.auxvar[f / 2][f % 2];
     if (!c.bdry.is_sides_pml[f / 2][f % 2]) continue;
-    if (aux_rhs_out) memcpy(aux_rhs_out + off, a->rhs, a->siz_ilevel * sizeof(float));
-    off += a->siz_ilevel;
+    if (aux_rhs_out) memcpy(aux_rhs_out + off, a->rhs, 9 * a->siz_icmp * sizeof(float));
+    off += 9 * a->siz_icmp;
   }
   teardown(&c);
   return (long)off;
@@ -558,12 +624,15 @@ int refh_run(const refh_cfg *cfg, float *coords, float *metric10, float *med3,
         else
           for (size_t p = 0; p < n; p++) a->end[p] += cb * a->rhs[p];
       }
+      /* drv_rk_curv_col.c:440-445 */
+      if (istage == 3 && c.md.medium_type == CONST_MEDIUM_VISCOELASTIC_ISO && cfg->free_top)
+        sv_curv_col_vis_iso_free(w_end, &c.wav, &c.gd, &c.metric, &c.md, &c.bdry, 0, 0);
     }
     for (int r = 0; r < nrecv; r++) {
       const int *ix = recv_idx + 8 * r;
       float Lx2 = recv_w[3 * r], Ly2 = recv_w[3 * r + 1], Lz2 = recv_w[3 * r + 2];
       float Lx1 = 1.0 - Lx2, Ly1 = 1.0 - Ly2, Lz1 = 1.0 - Lz2;
-      for (int icmp = 0; icmp < 9; icmp++) {
+      for (int icmp = 0; icmp < c.wav.ncmp; icmp++) {
         float *var = w_end + icmp * V;
         /* same expression as the reference receiver sampler, including the
          * repeated indx1d[5] term and the absent indx1d[4] term */
@@ -575,7 +644,7 @@ int refh_run(const refh_cfg *cfg, float *coords, float *metric10, float *med3,
                 + var[ix[5]] * Lx2 * Ly1 * Lz2
                 + var[ix[6]] * Lx1 * Ly2 * Lz2
                 + var[ix[7]] * Lx2 * Ly2 * Lz2;
-        seismo_out[((size_t)r * 9 + icmp) * nt + it] = v;
+        seismo_out[((size_t)r * c.wav.ncmp + icmp) * nt + it] = v;
       }
     }
     /* peak ground motion from w_end and w_pre before the swap

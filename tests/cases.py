@@ -126,6 +126,30 @@ def vti_as_aniso(vti):
     return np.stack([comp.get(nm, z) for nm in names] + [slw]).astype(np.float32)
 
 
+def visco_medium(ni, nj, nk, seed=0, nmaxwell=3, qp=100.0, qs=50.0, freq=1.0, fmin=0.5,
+                 fmax=5.0, pert=0.1):
+    """GMB visco-elastic medium: the isotropic medium with seeded Qp, Qs around
+    qp, qs, converted by the oracle's restatement of md_vis_GMB_cal_Y.
+    Returns (volumes lambda, mu, slw, Ylam_0, Ymu_0, ..., wl)."""
+    rng = np.random.default_rng(seed + 4)
+    lam, mu, slw = iso_medium(ni, nj, nk, seed=seed)
+    shp = lam.shape
+    Qp = (qp * (1 + pert * rng.standard_normal(shp))).astype(np.float32)
+    Qs = (qs * (1 + pert * rng.standard_normal(shp))).astype(np.float32)
+    n = lam.size
+    lam = np.ascontiguousarray(lam.reshape(-1)); mu = np.ascontiguousarray(mu.reshape(-1))
+    Yl = np.zeros((n, nmaxwell), np.float32); Ym = np.zeros((n, nmaxwell), np.float32)
+    wl = np.zeros(nmaxwell, np.float32)
+    pyoracle.load_oracle().orc_visco_gmb(n, nmaxwell, freq, fmin, fmax, pyoracle.ptr(lam),
+                                         pyoracle.ptr(mu), pyoracle.ptr(Qp.reshape(-1)),
+                                         pyoracle.ptr(Qs.reshape(-1)), pyoracle.ptr(Yl),
+                                         pyoracle.ptr(Ym), pyoracle.ptr(wl))
+    vols = [lam.reshape(shp), mu.reshape(shp), slw]
+    for b in range(nmaxwell):
+        vols += [Yl[:, b].reshape(shp), Ym[:, b].reshape(shp)]
+    return np.stack(vols).astype(np.float32), wl
+
+
 def make_medium(kind, ni, nj, nk, seed=0):
     """(medium_type, volumes) for kind 'iso', 'vti' or 'aniso'."""
     if kind == "vti":
@@ -133,6 +157,11 @@ def make_medium(kind, ni, nj, nk, seed=0):
     if kind == "aniso":
         return pyoracle.MEDIUM_ANISO, aniso_medium(ni, nj, nk, seed=seed)
     return pyoracle.MEDIUM_ISO, iso_medium(ni, nj, nk, seed=seed)
+
+
+def ncomp(case):
+    """Wavefield components: 9, plus 6 memory variables per Maxwell body."""
+    return 9 + 6 * len(case.get("visco_wl", ()))
 
 
 def pml_tables(orc, ni, nj, nk, coords, pml_on, pml_layers, vel=6000.0,
@@ -172,7 +201,12 @@ def make_case(ni=16, nj=14, nk=12, seed=0, free_top=True, pml=(1, 1, 1, 1, 1, 0)
     coords = curvilinear_coords(ni, nj, nk, dh=dh, seed=seed, hill=hill)
     metric = np.zeros((10, nz, ny, nx), np.float32)
     orc.orc_metric(ni, nj, nk, pyoracle.ptr(np.ascontiguousarray(coords)), pyoracle.ptr(metric))
-    mtype, med = make_medium(medium, ni, nj, nk, seed=seed)
+    visco_wl = None
+    if medium == "visco":
+        mtype = pyoracle.MEDIUM_VISCO
+        med, visco_wl = visco_medium(ni, nj, nk, seed=seed)
+    else:
+        mtype, med = make_medium(medium, ni, nj, nk, seed=seed)
     pml_on = list(pml)
     pml_layers = [layers if p else 0 for p in pml_on]
     A, B, D, region = pml_tables(orc, ni, nj, nk, coords, pml_on, pml_layers)
@@ -180,6 +214,8 @@ def make_case(ni=16, nj=14, nk=12, seed=0, free_top=True, pml=(1, 1, 1, 1, 1, 0)
                 free_top=int(free_top), pml_on=pml_on, pml_layers=pml_layers,
                 pml_A=A, pml_B=B, pml_D=D, pml_region=region, src=src,
                 pml_vel=6000.0, pml_alpha=np.pi * 1.0, pml_beta=3.0)
+    if visco_wl is not None:
+        case["visco_wl"] = visco_wl
     return case
 
 
@@ -189,8 +225,9 @@ def random_wavefield(case, seed=0, scale=1.0):
     rng = np.random.default_rng(seed + 7)
     ni, nj, nk = case["ni"], case["nj"], case["nk"]
     nx, ny, nz = dims(ni, nj, nk)
-    w = np.zeros((9, nz, ny, nx), np.float32)
-    w[:, 3:3 + nk, 3:3 + nj, 3:3 + ni] = scale * rng.standard_normal((9, nk, nj, ni))
+    nc = ncomp(case)
+    w = np.zeros((nc, nz, ny, nx), np.float32)
+    w[:, 3:3 + nk, 3:3 + nj, 3:3 + ni] = scale * rng.standard_normal((nc, nk, nj, ni))
     return w
 
 
@@ -267,13 +304,18 @@ def ref_cfg(case, src_arrays=None):
     c.ni, c.nj, c.nk = case["ni"], case["nj"], case["nk"]
     c.free_top = int(case["free_top"])
     c.medium_type = int(case.get("medium_type", pyoracle.MEDIUM_ISO))
+    keep = []
+    if "visco_wl" in case:
+        wl = np.ascontiguousarray(case["visco_wl"], np.float32)
+        keep.append(wl)
+        c.nmaxwell = len(wl)
+        c.visco_wl = wl.ctypes.data
     for f in range(6):
         c.pml_on[f] = int(case["pml_on"][f])
         c.pml_layers[f] = int(case["pml_layers"][f])
         c.pml_alpha[f] = case["pml_alpha"]
         c.pml_beta[f] = case["pml_beta"]
         c.pml_vel[f] = case["pml_vel"]
-    keep = []
     src = case.get("src")
     if src is not None and src["nsrc"] > 0:
         assert not src.get("gauss"), "the reference harness runs point sources only"
@@ -292,11 +334,13 @@ def make_solver(case, dt, nt_total, variant=4):
     """The product engine loaded with a case's static fields."""
     from cgfd3d_amd import Solver, SRC_GAUSSIAN, SRC_POINT
     mtype = int(case.get("medium_type", pyoracle.MEDIUM_ISO))
+    wl = case.get("visco_wl")
     s = Solver(case["ni"], case["nj"], case["nk"], dt, nt_total, free_top=bool(case["free_top"]),
-               pml_on=case["pml_on"], pml_layers=case["pml_layers"], medium_type=mtype)
+               pml_on=case["pml_on"], pml_layers=case["pml_layers"], medium_type=mtype,
+               nmaxwell=0 if wl is None else len(wl))
     s.set_kernel_variant(variant)
     s.set_metric(case["metric"])
-    s.set_medium(mtype, case["med"])
+    s.set_medium(mtype, case["med"], wl=wl)
     for f in range(6):
         if case["pml_on"][f]:
             s.set_pml_face(f, case["pml_A"][f], case["pml_B"][f], case["pml_D"][f])
@@ -315,8 +359,16 @@ def make_solver(case, dt, nt_total, variant=4):
 
 
 def free_surface_mats(case):
+    """matVx2Vz2, matVy2Vz2, matF2Vz2 (and matD for the visco-elastic medium)."""
     orc = pyoracle.load_oracle()
     nx, ny, _ = dims(case["ni"], case["nj"], case["nk"])
+    if case.get("medium_type") == pyoracle.MEDIUM_VISCO:
+        mats = np.zeros(4 * nx * ny * 9, np.float32)
+        orc.orc_dvh2dvz_vis(case["ni"], case["nj"], case["nk"],
+                            pyoracle.ptr(np.ascontiguousarray(case["metric"])),
+                            pyoracle.ptr(np.ascontiguousarray(case["med"])),
+                            pyoracle.ptr(np.ascontiguousarray(case["coords"])), pyoracle.ptr(mats))
+        return mats
     mats = np.zeros(3 * nx * ny * 9, np.float32)
     orc.orc_dvh2dvz(int(case.get("medium_type", pyoracle.MEDIUM_ISO)),
                     case["ni"], case["nj"], case["nk"],

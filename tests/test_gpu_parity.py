@@ -21,9 +21,11 @@ CONFIGS = [
 
 # (medium, kernel variant): every variant of the isotropic medium, the
 # per-point, lean-interior and vectorised variants of the VTI and
-# anisotropic media
+# anisotropic media, the per-point and lean-interior ones of the
+# visco-elastic medium
 MEDIUM_VARIANTS = ([("iso", v) for v in range(8)] +
-                   [(m, v) for m in ("vti", "aniso") for v in (0, 4, 5, 6, 7)])
+                   [(m, v) for m in ("vti", "aniso") for v in (0, 4, 5, 6, 7)] +
+                   [("visco", v) for v in (0, 4, 5)])
 
 
 @pytest.mark.parametrize("medium,variant", MEDIUM_VARIANTS)
@@ -51,7 +53,8 @@ def test_onestage_all_pairs(name, free, pml, medium, variant):
 
 
 RUN_VARIANTS = ([("iso", v) for v in range(8)] +
-                [(m, v) for m in ("vti", "aniso") for v in (0, 4, 6)])
+                [(m, v) for m in ("vti", "aniso") for v in (0, 4, 6)] +
+                [("visco", v) for v in (0, 4)])
 
 
 @pytest.mark.parametrize("medium,variant", RUN_VARIANTS)
@@ -85,7 +88,7 @@ def test_run_matches_oracle(medium, variant, gauss):
     assert np.array_equal(w_g, w_o) and np.array_equal(s_g, s_o)
 
 
-@pytest.mark.parametrize("medium", ["iso", "vti", "aniso"])
+@pytest.mark.parametrize("medium", ["iso", "vti", "aniso", "visco"])
 def test_device_setup_matches_oracle(medium):
     """Metric (gd_curv_metric_cal) and free-surface matrices
     (sv_curv_col_el_{iso,vti,aniso}_dvh2dvz) built on the device equal the
@@ -93,17 +96,26 @@ def test_device_setup_matches_oracle(medium):
     from cgfd3d_amd import Solver, workloads
     case = cs.make_case(21, 17, 15, seed=3, hill=0.4, medium=medium)
     L = workloads._setup_sigs()
-    s = Solver(21, 17, 15, 0.004, 4, free_top=True, medium_type=case["medium_type"])
+    wl = case.get("visco_wl")
+    s = Solver(21, 17, 15, 0.004, 4, free_top=True, medium_type=case["medium_type"],
+               nmaxwell=0 if wl is None else len(wl))
     xyz = np.ascontiguousarray(case["coords"])
     assert L.cgfdh_compute_metric(s.h, po.ptr(xyz)) == 0
     m = np.zeros_like(case["metric"])
     assert L.cgfdh_get_metric(s.h, po.ptr(m)) == 0
     assert np.array_equal(m, case["metric"])
-    s.set_medium(case["medium_type"], case["med"])
+    s.set_medium(case["medium_type"], case["med"], wl=wl)
     assert L.cgfdh_compute_free_surface(s.h) == 0
     mats = np.zeros(3 * 27 * 23 * 9, np.float32)
     assert L.cgfdh_get_free_surface(s.h, po.ptr(mats)) == 0
-    assert np.array_equal(mats, cs.free_surface_mats(case))
+    want = cs.free_surface_mats(case)
+    assert np.array_equal(mats, want[: mats.size])
+    if wl is not None:
+        # matD, built from the coordinates by cgfdh_compute_metric
+        matD = np.zeros(27 * 23 * 9, np.float32)
+        from cgfd3d_amd._lib import lib
+        assert lib().cgfdh_get_free_surface_matD(s.h, po.ptr(matD)) == 0
+        assert np.array_equal(matD, want[mats.size:])
     s.close()
 
 

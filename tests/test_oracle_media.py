@@ -1,9 +1,11 @@
-"""Checks of the oracle's VTI and general anisotropic media that do not depend
-on the reference build: Hooke's law against the stiffness tensor contracted
-in float64, the traction-free condition the surface matrices must satisfy,
-the reference's own way of writing a VTI medium as a general anisotropic one
-(md_t.c:544-590), and the isotropic limit.  A transposed or mistyped entry of
-the Cij index tables fails the first two."""
+"""Checks of the oracle's VTI, general anisotropic and GMB visco-elastic media
+that do not depend on the reference build: Hooke's law against the stiffness
+tensor contracted in float64, the traction-free condition the surface
+matrices must satisfy, the reference's own way of writing a VTI medium as a
+general anisotropic one (md_t.c:544-590), the isotropic limits, the GMB
+model's closed forms (flat Q over the band, the relaxed phase velocity at the
+reference frequency) and the traction-free surface rotation.  A transposed or
+mistyped entry of the Cij index tables fails the first two."""
 import numpy as np
 import pytest
 
@@ -180,3 +182,137 @@ def test_fp32_run_error_against_fp64(medium):
     e = cs.rel_l2(s32, s64)
     print(f"{medium}: FP32 vs FP64 seismograms rel L2 {e:.2e}")
     assert np.abs(s64).max() > 0 and e < 1e-4
+
+
+# ---- visco-elastic GMB (sv_curv_col_vis_iso.c, md_t.c:643-914) -----------
+
+def _gmb(n=3, npts=40, seed=0, qp=100.0, qs=50.0, freq=1.0, fmin=0.5, fmax=5.0):
+    rng = np.random.default_rng(seed)
+    lam = rng.uniform(2e10, 4e10, npts).astype(np.float32)
+    mu = rng.uniform(1e10, 3e10, npts).astype(np.float32)
+    Qp = (qp * rng.uniform(0.8, 1.2, npts)).astype(np.float32)
+    Qs = (qs * rng.uniform(0.8, 1.2, npts)).astype(np.float32)
+    return lam, mu, Qp, Qs
+
+
+def test_gmb_host_builder_matches_oracle():
+    """The engine's host builder and the oracle's restatement agree bit for
+    bit (unrelaxed moduli, Ylam, Ymu, wl)."""
+    import ctypes as C
+    from cgfd3d_amd import _lib
+    L = C.CDLL(_lib.LIB_PATH)
+    L.cgfdh_visco_gmb.argtypes = [C.c_long, C.c_int, C.c_float, C.c_float, C.c_float] + \
+        [C.c_void_p] * 6
+    lam, mu, Qp, Qs = _gmb()
+    n, npts = 3, len(lam)
+    l1, m1 = lam.copy(), mu.copy()
+    ym = np.zeros((2 * n, npts), np.float32)
+    wl1 = np.zeros(n, np.float32)
+    assert L.cgfdh_visco_gmb(npts, n, 1.0, 0.5, 5.0, po.ptr(l1), po.ptr(m1), po.ptr(Qp),
+                             po.ptr(Qs), po.ptr(ym), po.ptr(wl1)) == 0
+    l2, m2 = lam.copy(), mu.copy()
+    Yl = np.zeros((npts, n), np.float32); Ym = np.zeros((npts, n), np.float32)
+    wl2 = np.zeros(n, np.float32)
+    po.load_oracle().orc_visco_gmb(npts, n, 1.0, 0.5, 5.0, po.ptr(l2), po.ptr(m2), po.ptr(Qp),
+                                   po.ptr(Qs), po.ptr(Yl), po.ptr(Ym), po.ptr(wl2))
+    assert np.array_equal(l1, l2) and np.array_equal(m1, m2) and np.array_equal(wl1, wl2)
+    assert np.array_equal(ym[0::2].T, Yl) and np.array_equal(ym[1::2].T, Ym)
+
+
+def _modulus(Y, wl, w):
+    """GMB modulus over its unrelaxed value: 1 - sum Y_n wl_n / (wl_n + i w)."""
+    return 1 - np.sum(Y[None, :] * wl[None, :] / (wl[None, :] + 1j * w[:, None]), axis=1)
+
+
+@pytest.mark.parametrize("n", [2, 3, 4])
+def test_gmb_fit_gives_flat_q_and_keeps_the_reference_velocity(n):
+    """Closed forms of the generalized Maxwell body: Q^-1 = Im M / Re M stays
+    near the target over [fmin, fmax], and the unrelaxed moduli give the
+    relaxed (input) phase velocity at the reference frequency, for P
+    (Y_P = (lam Ylam + 2 mu Ymu)/(lam + 2 mu)) and S (Y_S = Ymu)."""
+    lam, mu, Qp, Qs = _gmb(n=n, npts=12, seed=n)
+    rho = 2500.0
+    l2, m2 = lam.copy(), mu.copy()
+    Yl = np.zeros((len(lam), n), np.float32); Ym = np.zeros((len(lam), n), np.float32)
+    wl = np.zeros(n, np.float32)
+    po.load_oracle().orc_visco_gmb(len(lam), n, 1.0, 0.5, 5.0, po.ptr(l2), po.ptr(m2),
+                                   po.ptr(Qp), po.ptr(Qs), po.ptr(Yl), po.ptr(Ym), po.ptr(wl))
+    wl = wl.astype(np.float64)
+    band = 2 * np.pi * np.geomspace(0.5, 5.0, 25)
+    wr = np.array([2 * np.pi * 1.0])
+    worst_q, worst_v = 0.0, 0.0
+    for p in range(len(lam)):
+        lu, mu_u = float(l2[p]), float(m2[p])
+        YP = (lu * Yl[p].astype(np.float64) + 2 * mu_u * Ym[p]) / (lu + 2 * mu_u)
+        YS = Ym[p].astype(np.float64)
+        for Y, Q, Mu, Mr in ((YP, Qp[p], lu + 2 * mu_u, float(lam[p]) + 2 * float(mu[p])),
+                             (YS, Qs[p], mu_u, float(mu[p]))):
+            M = _modulus(Y, wl, band)
+            qinv = M.imag / M.real
+            worst_q = max(worst_q, float(np.max(np.abs(qinv * round(float(Q)) - 1))))
+            k = 1 / np.sqrt(Mu * _modulus(Y, wl, wr) / rho)
+            v = 1 / k.real[0]
+            worst_v = max(worst_v, abs(v / np.sqrt(Mr / rho) - 1))
+    print(f"n={n}: worst |Q/Q0 - 1| over the band {worst_q:.3f}, velocity at fr {worst_v:.1e}")
+    assert worst_q < (0.25 if n == 2 else 0.1)
+    assert worst_v < 1e-5
+
+
+def test_visco_surface_rotation_leaves_no_traction():
+    """sv_curv_col_vis_iso_free rotates the surface-row stresses into the
+    frame (tangent x normal, tangent, normal) of matD, zeroes the rows of the
+    normal and rotates back: after every step the traction T.n on the
+    surface vanishes, and matD is orthonormal to FP32 precision."""
+    ni, nj, nk, nt, dt = 14, 13, 12, 6, 0.004
+    case = cs.make_case(ni, nj, nk, seed=9, free_top=True, pml=(0,) * 6, medium="visco",
+                        hill=0.4)
+    nx, ny, nz = ni + 6, nj + 6, nk + 6
+    mats = cs.free_surface_mats(case).reshape(4, ny, nx, 3, 3)
+    D = mats[3, 3:3 + nj, 3:3 + ni].astype(np.float64)
+    eye = np.einsum("jiab,jicb->jiac", D, D)
+    assert np.abs(eye - np.eye(3)).max() < 1e-5
+    w = cs.random_wavefield(case, seed=3).reshape(-1)
+    orc = po.Oracle(case, fp64=False)
+    w1, _, _ = orc.run(dt, nt, w=w)
+    W = w1.reshape(cs.ncomp(case), nz, ny, nx)[:, 3 + nk - 1, 3:3 + nj, 3:3 + ni].astype(np.float64)
+    T = np.stack([np.stack([W[3], W[8], W[7]]), np.stack([W[8], W[4], W[6]]),
+                  np.stack([W[7], W[6], W[5]])])           # [a][b][j][i]
+    n = D[:, :, 2, :]                                        # unit normal rows
+    t = np.einsum("abji,jib->jia", T, n)
+    rel = np.abs(t).max() / np.abs(T).max()
+    print(f"surface traction / stress after {nt} steps: {rel:.1e}")
+    assert rel < 1e-5
+
+
+def test_visco_high_q_limit():
+    """Q -> 1e8: the memory variables' couplings scale with 1/Q, so the
+    visco-elastic RHS of the elastic fields (and the PML auxiliaries) matches
+    the isotropic one to ~1e-6 (its unrelaxed moduli equal the relaxed ones to
+    FP32 rounding); the attenuation terms themselves are pinned bit for bit
+    by the reference build."""
+    case_i = cs.make_case(13, 12, 11, seed=6, free_top=True, pml=(1, 1, 1, 1, 1, 0), layers=3)
+    lam, mu, slw = (v.copy() for v in case_i["med"])
+    shp = lam.shape
+    n, npts = 3, lam.size
+    l2, m2 = np.ascontiguousarray(lam.reshape(-1)), np.ascontiguousarray(mu.reshape(-1))
+    Q = np.full(npts, 1e8, np.float32)
+    Yl = np.zeros((npts, n), np.float32); Ym = np.zeros((npts, n), np.float32)
+    wl = np.zeros(n, np.float32)
+    po.load_oracle().orc_visco_gmb(npts, n, 1.0, 0.5, 5.0, po.ptr(l2), po.ptr(m2), po.ptr(Q),
+                                   po.ptr(Q), po.ptr(Yl), po.ptr(Ym), po.ptr(wl))
+    vols = [l2.reshape(shp), m2.reshape(shp), slw]
+    for b in range(n):
+        vols += [Yl[:, b].reshape(shp), Ym[:, b].reshape(shp)]
+    case_v = dict(case_i, medium_type=po.MEDIUM_VISCO, med=np.stack(vols).astype(np.float32),
+                  visco_wl=wl)
+    wv = cs.random_wavefield(case_v, seed=2).reshape(-1)
+    wi = wv[: 9 * lam.size]
+    aux = cs.random_aux(case_i, seed=2)
+    oi, ov = po.Oracle(case_i), po.Oracle(case_v)
+    worst = 0.0
+    for ipair, istage in [(0, 0), (3, 1), (6, 2)]:
+        ri, ai = oi.onestage(2, ipair, istage, wi, aux)
+        rv, av = ov.onestage(2, ipair, istage, wv, aux)
+        worst = max(worst, cs.rel_l2(rv[: ri.size], ri), cs.rel_l2(av, ai))
+    print(f"high-Q visco vs isotropic RHS rel L2 {worst:.1e}")
+    assert worst < 1e-5

@@ -50,11 +50,11 @@ def test_pml_profiles_match_reference(layers):
         assert np.array_equal(D[f], case["pml_D"][f])
 
 
-MEDIA = ["iso", "vti", "aniso"]
+MEDIA = ["iso", "vti", "aniso", "visco"]
 
 
 @needs_ref
-@pytest.mark.parametrize("medium", MEDIA)
+@pytest.mark.parametrize("medium", ["iso", "vti", "aniso"])
 def test_free_surface_matrices_match_reference(medium):
     case = cs.make_case(13, 12, 10, seed=4, free_top=True, medium=medium)
     ref = po.load_ref()
@@ -63,6 +63,20 @@ def test_free_surface_matrices_match_reference(medium):
     m_ref = np.zeros(3 * nx * ny * 9, np.float32)
     ref.refh_dvh2dvz(C.byref(cfg), po.ptr(case["metric"]), po.ptr(case["med"]), po.ptr(m_ref))
     assert np.abs(m_ref).max() > 0
+    assert np.array_equal(cs.free_surface_mats(case), m_ref)
+
+
+@needs_ref
+def test_visco_free_surface_matrices_match_reference():
+    """sv_curv_col_vis_iso_dvh2dvz, including matD from the coordinates."""
+    case = cs.make_case(13, 12, 10, seed=4, free_top=True, medium="visco")
+    ref = po.load_ref()
+    cfg, keep = cs.ref_cfg(case)
+    nx, ny = case["ni"] + 6, case["nj"] + 6
+    m_ref = np.zeros(4 * nx * ny * 9, np.float32)
+    ref.refh_dvh2dvz_vis(C.byref(cfg), po.ptr(np.ascontiguousarray(case["coords"])),
+                         po.ptr(case["metric"]), po.ptr(case["med"]), po.ptr(m_ref))
+    assert np.abs(m_ref[3 * nx * ny * 9:]).max() > 0
     assert np.array_equal(cs.free_surface_mats(case), m_ref)
 
 
@@ -135,7 +149,7 @@ def test_run_matches_reference(medium):
     ref = po.load_ref()
     cfg, keep = cs.ref_cfg(case)
     w_r = np.zeros_like(w_o)
-    s_r = np.zeros((len(ridx), 9, nt), np.float32)
+    s_r = np.zeros((len(ridx), cs.ncomp(case), nt), np.float32)
     pg_r = np.zeros_like(pg_o)
     dis_r = np.zeros_like(dis_o)
     ref.refh_run(C.byref(cfg), po.ptr(np.ascontiguousarray(case["coords"])),
