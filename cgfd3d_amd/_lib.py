@@ -48,6 +48,13 @@ SIGNATURES = [
     ("cgfdh_onestage", C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
     ("cgfdh_last_timing", C.c_int, [_P, _P, _P, _P]),
     ("cgfdh_set_kernel_variant", C.c_int, [_P, C.c_int]),
+    ("cgfdh_set_tile_group", C.c_int, [_P, C.c_int, C.c_int]),
+    ("cgfdh_get_kernel_choice", C.c_int, [_P, _P, _P, _P]),
+    ("cgfdh_batch_create", C.c_int, [C.c_int, _P, _P]),
+    ("cgfdh_batch_destroy", C.c_int, [_P]),
+    ("cgfdh_batch_autotune", C.c_int, [_P, C.c_int]),
+    ("cgfdh_batch_run", C.c_int, [_P, C.c_int, C.c_int]),
+    ("cgfdh_batch_last_timing", C.c_int, [_P, _P]),
 ]
 
 

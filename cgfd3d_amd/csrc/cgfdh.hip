@@ -27,10 +27,12 @@ using namespace cgfdh;
 // indx1d[5] used twice and indx1d[4] not, as the reference does
 __global__ void recv_kernel(const float* __restrict__ w, long vol, int nrecv, int nw,
                             const int* __restrict__ idx, const float* __restrict__ dijk,
-                            float* __restrict__ seismo, int nt_total, int it)
+                            float* __restrict__ seismo, int nt_total, int it,
+                            const int* __restrict__ it_off)
 {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nrecv * nw) return;
+  if (it_off) it += *it_off;
   const int r = t / nw, c = t % nw;
   const int* ix = idx + 8 * r;
   const float Lx2 = dijk[3 * r], Ly2 = dijk[3 * r + 1], Lz2 = dijk[3 * r + 2];
@@ -199,6 +201,7 @@ struct cgfdh_solver {
   float* seismo = nullptr;
   float* pg = nullptr;           // 15 peak maps + 3 displacement maps, [ny][nx] each
   int variant = 4;
+  int tgx = 0, tgy = 0;          // tile group (0: the default)
   bool metric_set = false, medium_set = false, mats_set = false;
   hipEvent_t ev_run[2];
   std::vector<hipEvent_t> ev_stage;
@@ -580,6 +583,14 @@ extern "C" int cgfdh_set_kernel_variant(cgfdh_solver* s, int v)
   return CGFDH_OK;
 }
 
+extern "C" int cgfdh_set_tile_group(cgfdh_solver* s, int gx, int gy)
+{
+  if (!s || gx < 0 || gy < 0) return fail(CGFDH_EINVAL, "cgfdh_set_tile_group: bad argument");
+  s->tgx = gx;
+  s->tgy = gy;
+  return CGFDH_OK;
+}
+
 static int ready(cgfdh_solver* s, const char* who)
 {
   if (!s->metric_set || !s->medium_set)
@@ -636,17 +647,20 @@ static StageParams make_params(cgfdh_solver* s, int it, int istage, const float*
   return P;
 }
 
-static void launch_stage(cgfdh_solver* s, const StageParams& P, int kind, int ipair, int istage)
+static void launch_stage(cgfdh_solver* s, hipStream_t stream, const StageParams& P, int kind,
+                         int ipair, int istage)
 {
   const int dx = (kPairFlags[ipair][0] + istage) % 2;
   const int dy = (kPairFlags[ipair][1] + istage) % 2;
   const int dz = (kPairFlags[ipair][2] + istage) % 2;
   LaunchCtx L;
-  L.stream = s->stream;
+  L.stream = stream;
   L.variant = s->variant;
   L.ni = s->d.ni; L.nj = s->d.nj; L.nk = s->d.nk;
   L.ni1 = s->ni1; L.ni2 = s->ni2; L.nj1 = s->nj1; L.nj2 = s->nj2; L.nk1 = s->nk1; L.nk2 = s->nk2;
   L.free_top = s->d.free_top;
+  L.tgx = s->tgx;
+  L.tgy = s->tgy;
   for (int f = 0; f < 6; f++) {
     L.pml_on[f] = s->d.pml_on[f];
     L.pml_layers[f] = s->d.pml_layers[f];
@@ -659,6 +673,47 @@ static void launch_stage(cgfdh_solver* s, const StageParams& P, int kind, int ip
     launch_stage_visco(L, P, kind, dx, dy, dz);
   else
     launch_stage_iso(L, P, kind, dx, dy, dz);
+}
+
+// one RK4 step of solver s on stream st (drv_rk_curv_col.c:167-544): four fused
+// stages, the visco-elastic surface rotation, peak ground motion, receivers,
+// then the pre/end swap.  ev (optional) gets a record before and after each
+// stage; it_off (optional) is a device step offset added to it, for graph
+// replays of a captured cycle.
+static int enqueue_step(cgfdh_solver* s, hipStream_t st, int it, const int* it_off,
+                        hipEvent_t* ev)
+{
+  const int ipair = it % 8;
+  float* tA = s->lvl[1];
+  float* tB = s->lvl[2];
+  for (int sg = 0; sg < 4; sg++) {
+    const float* wcur = sg == 0 ? s->lvl[0] : (sg % 2 == 1 ? tA : tB);
+    float* wtmp = sg % 2 == 0 ? tA : tB;
+    StageParams P = make_params(s, it, sg, wcur, wtmp, sg == 0 ? 0 : 1);
+    P.it_off = it_off;
+    const int kind = sg == 0 ? KIND_FIRST : (sg == 3 ? KIND_LAST : KIND_MID);
+    if (ev) HIPCHK(hipEventRecord(ev[2 * sg], st));
+    launch_stage(s, st, P, kind, ipair, sg);
+    if (ev) HIPCHK(hipEventRecord(ev[2 * sg + 1], st));
+  }
+  if (s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO && s->d.free_top)   // drv_rk_curv_col.c:440-445
+    hipLaunchKernelGGL(vis_free_kernel, dim3((s->d.ni + 255) / 256, s->d.nj), dim3(256), 0, st,
+                       s->lvl[3], s->mats + 27 * s->ss, s->d.ni, s->d.nj, s->nx, s->ss, s->vol,
+                       s->nk2);
+  if (s->pg)   // drv_rk_curv_col.c:492-495, before the pre/end swap
+    hipLaunchKernelGGL(pg_kernel, dim3((s->d.ni + 255) / 256, s->d.nj), dim3(256), 0, st,
+                       s->lvl[3], s->lvl[0], s->pg, s->pg + 15 * s->ss, s->d.ni, s->d.nj, s->nx,
+                       s->ss, s->vol, s->nk2, s->d.dt);
+  if (s->nrecv > 0) {
+    const int nt = s->nrecv * s->nw;
+    hipLaunchKernelGGL(recv_kernel, dim3((nt + 255) / 256), dim3(256), 0, st, s->lvl[3], s->vol,
+                       s->nrecv, s->nw, s->recv_idx, s->recv_w, s->seismo, s->d.nt_total, it,
+                       it_off);
+  }
+  std::swap(s->lvl[0], s->lvl[3]);
+  for (auto& f : s->face)
+    if (f.on) std::swap(f.aux[0], f.aux[2]);
+  return CGFDH_OK;
 }
 
 extern "C" int cgfdh_run(cgfdh_solver* s, int it0, int nsteps)
@@ -678,36 +733,8 @@ extern "C" int cgfdh_run(cgfdh_solver* s, int it0, int nsteps)
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(s->ev_run[0], s->stream));
   for (int n = 0; n < nsteps; n++) {
-    const int it = it0 + n;
-    const int ipair = it % 8;
-    float* tA = s->lvl[1];
-    float* tB = s->lvl[2];
-    for (int st = 0; st < 4; st++) {
-      const float* wcur = st == 0 ? s->lvl[0] : (st % 2 == 1 ? tA : tB);
-      float* wtmp = st % 2 == 0 ? tA : tB;
-      StageParams P = make_params(s, it, st, wcur, wtmp, st == 0 ? 0 : 1);
-      const int kind = st == 0 ? KIND_FIRST : (st == 3 ? KIND_LAST : KIND_MID);
-      HIPCHK(hipEventRecord(s->ev_stage[8 * n + 2 * st], s->stream));
-      launch_stage(s, P, kind, ipair, st);
-      HIPCHK(hipEventRecord(s->ev_stage[8 * n + 2 * st + 1], s->stream));
-    }
-    if (s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO && s->d.free_top)   // drv_rk_curv_col.c:440-445
-      hipLaunchKernelGGL(vis_free_kernel, dim3((s->d.ni + 255) / 256, s->d.nj), dim3(256), 0,
-                         s->stream, s->lvl[3], s->mats + 27 * s->ss, s->d.ni, s->d.nj, s->nx,
-                         s->ss, s->vol, s->nk2);
-    if (s->pg)   // drv_rk_curv_col.c:492-495, before the pre/end swap
-      hipLaunchKernelGGL(pg_kernel, dim3((s->d.ni + 255) / 256, s->d.nj), dim3(256), 0,
-                         s->stream, s->lvl[3], s->lvl[0], s->pg, s->pg + 15 * s->ss, s->d.ni,
-                         s->d.nj, s->nx, s->ss, s->vol, s->nk2, s->d.dt);
-    if (s->nrecv > 0) {
-      int nt = s->nrecv * s->nw;
-      hipLaunchKernelGGL(recv_kernel, dim3((nt + 255) / 256), dim3(256), 0, s->stream,
-                         s->lvl[3], s->vol, s->nrecv, s->nw, s->recv_idx, s->recv_w,
-                         s->seismo, s->d.nt_total, it);
-    }
-    std::swap(s->lvl[0], s->lvl[3]);
-    for (auto& f : s->face)
-      if (f.on) std::swap(f.aux[0], f.aux[2]);
+    rc = enqueue_step(s, s->stream, it0 + n, nullptr, s->ev_stage.data() + 8 * n);
+    if (rc) return rc;
   }
   HIPCHK(hipEventRecord(s->ev_run[1], s->stream));
   HIPCHK(hipGetLastError());
@@ -757,7 +784,7 @@ extern "C" int cgfdh_onestage(cgfdh_solver* s, int it, int ipair, int istage, co
   int n = 0;
   for (auto& f : s->face)
     if (f.on) P.aux_rhs_out[n++] = f.aux[2];
-  launch_stage(s, P, KIND_RHS, ipair, istage);
+  launch_stage(s, s->stream, P, KIND_RHS, ipair, istage);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(rhs, s->lvl[2], lv * sizeof(float), hipMemcpyDeviceToHost,
                         s->stream));
@@ -855,5 +882,239 @@ extern "C" int cgfdh_get_peak_ground_motion(cgfdh_solver* s, float* pg, float* d
   if (pg) HIPCHK(hipMemcpy(pg, s->pg, 15 * s->ss * sizeof(float), hipMemcpyDeviceToHost));
   if (dis)
     HIPCHK(hipMemcpy(dis, s->pg + 15 * s->ss, 3 * s->ss * sizeof(float), hipMemcpyDeviceToHost));
+  return CGFDH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched runs: independent solvers advanced together.  A cycle of 8 steps
+// (one full operator-pair schedule, an even number of level swaps, so every
+// pointer returns to where it started) of all solvers is captured once into
+// a hipGraph, each solver on its own stream so the graph's branches run
+// concurrently, and replayed; a device step offset shifts the captured step
+// numbers of sources and receivers by 8 per replay.
+// ---------------------------------------------------------------------------
+
+__global__ void set_int_kernel(int* p, int v) { *p = v; }
+
+struct cgfdh_batch {
+  std::vector<cgfdh_solver*> sv;
+  hipStream_t stream = nullptr;
+  int* it_off = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int cap_it0 = -1;                 // step the captured cycle starts at
+  std::vector<hipEvent_t> ev;       // fork / join
+  hipEvent_t t[2];
+  double last_ms = 0;
+};
+
+static void batch_drop_graph(cgfdh_batch* b)
+{
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  b->exec = nullptr;
+  b->cap_it0 = -1;
+}
+
+extern "C" int cgfdh_batch_create(int n, cgfdh_solver* const* solvers, cgfdh_batch** out)
+{
+  if (n < 1 || !solvers || !out) return fail(CGFDH_EINVAL, "cgfdh_batch_create: bad argument");
+  for (int i = 0; i < n; i++)
+    if (!solvers[i]) return fail(CGFDH_EINVAL, "cgfdh_batch_create: solver %d is null", i);
+  cgfdh_batch* b = new cgfdh_batch();
+  b->sv.assign(solvers, solvers + n);
+  bool bad = hipStreamCreate(&b->stream) != hipSuccess ||
+             hipMalloc((void**)&b->it_off, sizeof(int)) != hipSuccess ||
+             hipEventCreate(&b->t[0]) != hipSuccess || hipEventCreate(&b->t[1]) != hipSuccess;
+  for (int i = 0; i < 2 * n + 1 && !bad; i++) {
+    hipEvent_t e;
+    bad = hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess;
+    if (!bad) b->ev.push_back(e);
+  }
+  if (bad) {
+    delete b;
+    return fail(CGFDH_EDEVICE, "cgfdh_batch_create: stream / event / allocation failed");
+  }
+  *out = b;
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_batch_destroy(cgfdh_batch* b)
+{
+  if (!b) return CGFDH_OK;
+  (void)hipStreamSynchronize(b->stream);
+  batch_drop_graph(b);
+  for (auto e : b->ev) (void)hipEventDestroy(e);
+  (void)hipEventDestroy(b->t[0]);
+  (void)hipEventDestroy(b->t[1]);
+  (void)hipFree(b->it_off);
+  (void)hipStreamDestroy(b->stream);
+  delete b;
+  return CGFDH_OK;
+}
+
+// capture steps it0..it0+7 of every solver into one graph
+static int batch_capture(cgfdh_batch* b, int it0)
+{
+  batch_drop_graph(b);
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeGlobal));
+  int rc = CGFDH_OK;
+  const int n = (int)b->sv.size();
+  hipError_t e = hipEventRecord(b->ev[2 * n], b->stream);
+  for (int i = 0; i < n && e == hipSuccess && !rc; i++) {
+    cgfdh_solver* s = b->sv[i];
+    e = hipStreamWaitEvent(s->stream, b->ev[2 * n], 0);
+    for (int q = 0; q < 8 && e == hipSuccess && !rc; q++)
+      rc = enqueue_step(s, s->stream, it0 + q, b->it_off, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(b->ev[i], s->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->ev[i], 0);
+  }
+  hipError_t ee = hipStreamEndCapture(b->stream, &g);
+  if (e == hipSuccess) e = ee;
+  if (e == hipSuccess && !rc) e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  if (rc) return rc;
+  if (e != hipSuccess) {
+    b->exec = nullptr;
+    return fail(CGFDH_EDEVICE, "cgfdh_batch: graph capture failed: %s", hipGetErrorString(e));
+  }
+  b->cap_it0 = it0;
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_batch_run(cgfdh_batch* b, int it0, int nsteps)
+{
+  if (!b || it0 < 0 || nsteps < 0) return fail(CGFDH_EINVAL, "cgfdh_batch_run: bad argument");
+  for (size_t i = 0; i < b->sv.size(); i++) {
+    cgfdh_solver* s = b->sv[i];
+    int rc = ready(s, "cgfdh_batch_run");
+    if (rc) return rc;
+    if (it0 + nsteps > s->d.nt_total)
+      return fail(CGFDH_EINVAL, "cgfdh_batch_run: steps %d..%d outside solver %zu's nt_total %d",
+                  it0, it0 + nsteps, i, s->d.nt_total);
+  }
+  for (auto s : b->sv) HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipEventRecord(b->t[0], b->stream));
+  int it = it0;
+  const int cycles = nsteps / 8;
+  if (cycles > 0) {
+    // a captured cycle serves every start step congruent to it mod 8
+    if (!b->exec || (it - b->cap_it0) % 8 != 0 || it < b->cap_it0) {
+      int rc = batch_capture(b, it);
+      if (rc) return rc;
+    }
+    for (int c = 0; c < cycles; c++, it += 8) {
+      hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, b->stream, b->it_off,
+                         it - b->cap_it0);
+      HIPCHK(hipGraphLaunch(b->exec, b->stream));
+    }
+  }
+  // the remaining 0..7 steps, eagerly, solver after solver on the batch stream
+  for (; it < it0 + nsteps; it++)
+    for (auto s : b->sv) {
+      int rc = enqueue_step(s, b->stream, it, nullptr, nullptr);
+      if (rc) return rc;
+    }
+  HIPCHK(hipEventRecord(b->t[1], b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipGetLastError());
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, b->t[0], b->t[1]));
+  b->last_ms = ms;
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_batch_last_timing(cgfdh_batch* b, double* ms)
+{
+  if (!b || !ms) return fail(CGFDH_EINVAL, "cgfdh_batch_last_timing: bad argument");
+  *ms = b->last_ms;
+  return CGFDH_OK;
+}
+
+// autotune: per solver, time one step of each candidate kernel variant and
+// tile group from the current state (restored after every trial) and keep
+// the fastest.  Restores the state, so the caller's run starts unchanged.
+extern "C" int cgfdh_batch_autotune(cgfdh_batch* b, int it0)
+{
+  if (!b) return fail(CGFDH_EINVAL, "cgfdh_batch_autotune: null batch");
+  batch_drop_graph(b);
+  for (size_t i = 0; i < b->sv.size(); i++) {
+    cgfdh_solver* s = b->sv[i];
+    int rc = ready(s, "cgfdh_batch_autotune");
+    if (rc) return rc;
+    if (it0 + 1 > s->d.nt_total) return fail(CGFDH_EINVAL, "cgfdh_batch_autotune: it0 %d", it0);
+    // saved state: the pre level, the PML aux pre levels, peak ground motion
+    const size_t lv = (size_t)s->nw * s->vol;
+    size_t naux = 0;
+    for (auto& f : s->face)
+      if (f.on) naux += 9 * f.fn;
+    const size_t npg = s->pg ? 18 * (size_t)s->ss : 0;
+    float* save = nullptr;
+    HIPCHK(hipMalloc((void**)&save, (lv + naux + npg + 1) * sizeof(float)));
+    auto copy_state = [&](bool to_save) -> hipError_t {
+      hipError_t e = to_save ? hipMemcpyAsync(save, s->lvl[0], lv * 4, hipMemcpyDeviceToDevice,
+                                              s->stream)
+                             : hipMemcpyAsync(s->lvl[0], save, lv * 4, hipMemcpyDeviceToDevice,
+                                              s->stream);
+      size_t off = lv;
+      for (auto& f : s->face) {
+        if (!f.on || e != hipSuccess) continue;
+        e = to_save ? hipMemcpyAsync(save + off, f.aux[0], 9 * f.fn * 4,
+                                     hipMemcpyDeviceToDevice, s->stream)
+                    : hipMemcpyAsync(f.aux[0], save + off, 9 * f.fn * 4,
+                                     hipMemcpyDeviceToDevice, s->stream);
+        off += 9 * f.fn;
+      }
+      if (npg && e == hipSuccess)
+        e = to_save ? hipMemcpyAsync(save + off, s->pg, npg * 4, hipMemcpyDeviceToDevice, s->stream)
+                    : hipMemcpyAsync(s->pg, save + off, npg * 4, hipMemcpyDeviceToDevice, s->stream);
+      return e;
+    };
+    hipError_t e = copy_state(true);
+    const bool iso = s->d.medium_type == CGFDH_MEDIUM_ELASTIC_ISO;
+    const bool visco = s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO;
+    std::vector<int> variants = visco ? std::vector<int>{4, 5} : std::vector<int>{4, 6, 7};
+    if (iso) variants.push_back(3);
+    const int groups[][2] = {{9, 2}, {3, 4}, {9, 8}};
+    float best = 1e30f;
+    int bv = s->variant, bgx = s->tgx, bgy = s->tgy;
+    hipEvent_t t0 = b->t[0], t1 = b->t[1];
+    for (int v : variants)
+      for (auto& g : groups) {
+        if (e != hipSuccess) break;
+        s->variant = v;
+        s->tgx = g[0];
+        s->tgy = g[1];
+        float ms = 0;
+        for (int rep = 0; rep < 2 && e == hipSuccess; rep++) {   // first run warms up
+          e = copy_state(false);
+          if (e == hipSuccess) e = hipEventRecord(t0, s->stream);
+          if (e == hipSuccess && enqueue_step(s, s->stream, it0, nullptr, nullptr)) e = hipErrorUnknown;
+          if (e == hipSuccess) e = hipEventRecord(t1, s->stream);
+          if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+          if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+          std::swap(s->lvl[0], s->lvl[3]);      // undo the step's swaps
+          for (auto& f : s->face)
+            if (f.on) std::swap(f.aux[0], f.aux[2]);
+        }
+        if (e == hipSuccess && ms < best) { best = ms; bv = v; bgx = g[0]; bgy = g[1]; }
+      }
+    if (e == hipSuccess) e = copy_state(false);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(save);
+    s->variant = bv;
+    s->tgx = bgx;
+    s->tgy = bgy;
+    if (e != hipSuccess)
+      return fail(CGFDH_EDEVICE, "cgfdh_batch_autotune: %s", hipGetErrorString(e));
+  }
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_get_kernel_choice(cgfdh_solver* s, int* variant, int* gx, int* gy)
+{
+  if (!s) return fail(CGFDH_EINVAL, "cgfdh_get_kernel_choice: null solver");
+  if (variant) *variant = s->variant;
+  if (gx) *gx = s->tgx;
+  if (gy) *gy = s->tgy;
   return CGFDH_OK;
 }

@@ -99,6 +99,7 @@ struct StageParams {
   int src_half;
   const float* src_coef;
   int it, istage;
+  const int* it_off;        // if set, a device step offset added to it (graph replays)
   // visco-elastic GMB: bodies, wavefield components (9 + 6 nmax), relaxation
   // frequencies, the Ylam_n / Ymu_n volumes (interleaved, from ylam)
   int nmax, nw;
@@ -293,9 +294,10 @@ __device__ void pml_face(const StageParams& P, const PmlDev& F, int fslot, int i
 __device__ __forceinline__ void add_sources(const StageParams& P, int i, int j, int k, long p,
                                             float* r)
 {
+  const int it = P.it + (P.it_off ? *P.it_off : 0);
   for (int is = 0; is < P.nsrc; is++) {
     const SrcDev S = P.src[is];
-    if (P.it < S.it_begin || P.it > S.it_end) continue;
+    if (it < S.it_begin || it > S.it_end) continue;
     float coefv;
     bool hit;
     if (P.src_spatial == 1) {
@@ -311,7 +313,7 @@ __device__ __forceinline__ void add_sources(const StageParams& P, int i, int j, 
       const float coef = P.src_coef[S.coef_off + ((dk + h) * L + (dj + h)) * L + (di + h)];
       coefv = coef / P.met[p];
     }
-    const long ip = (long)is * P.src_max_nt * 4 + (long)(P.it - S.it_begin) * 4 + P.istage;
+    const long ip = (long)is * P.src_max_nt * 4 + (long)(it - S.it_begin) * 4 + P.istage;
     const int slot[6] = {TXX, TYY, TZZ, TXZ, TYZ, TXY};
 #pragma unroll
     for (int m = 0; m < 6; m++) r[slot[m]] -= P.mij[m * P.mij_stride + ip] * coefv;

@@ -65,7 +65,7 @@ inline int grouped_grid(BoxParams& B, int by, int tw = 64)
 {
   B.tiles_x = (B.ni + tw - 1) / tw;
   B.tiles_y = (B.nj + by - 1) / by;
-  tile_group(B.gx, B.gy);
+  if (B.gx <= 0 || B.gy <= 0) tile_group(B.gx, B.gy);
   B.gx = std::min(B.gx, B.tiles_x);
   B.gy = std::min(B.gy, B.tiles_y);
   B.groups_x = (B.tiles_x + B.gx - 1) / B.gx;

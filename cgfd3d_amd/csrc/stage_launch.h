@@ -33,6 +33,7 @@ struct LaunchCtx {
   int ni1, ni2, nj1, nj2, nk1, nk2;
   int free_top;
   int pml_on[6], pml_layers[6];
+  int tgx, tgy;            // tile group of the grouped block order (0: default)
 };
 
 // per-point kernel: one thread per grid point, every stencil operand read
@@ -90,9 +91,10 @@ static void launch_kind(const LaunchCtx& L, const StageParams& P, int dx, int dy
   const int jhi = L.pml_on[3] ? L.nj2 - L.pml_layers[3] - 1 : L.nj2;
   const int klo = L.pml_on[4] ? L.nk1 + L.pml_layers[4] + 1 : L.nk1;
   const int khi = L.free_top ? L.nk2 - 4 : (L.pml_on[5] ? L.nk2 - L.pml_layers[5] - 1 : L.nk2);
-  auto box = [](int i0, int i1, int j0, int j1, int k0, int k1) {
+  auto box = [&L](int i0, int i1, int j0, int j1, int k0, int k1) {
     BoxParams b;
     memset(&b, 0, sizeof b);
+    b.gx = L.tgx; b.gy = L.tgy;
     b.i0 = i0; b.j0 = j0; b.k0 = k0;
     b.ni = i1 - i0 + 1; b.nj = j1 - j0 + 1; b.nk = k1 - k0 + 1;
     return b;

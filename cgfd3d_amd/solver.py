@@ -197,3 +197,43 @@ class Solver:
 
     def set_kernel_variant(self, v):
         check(lib().cgfdh_set_kernel_variant(self.h, v), "set_kernel_variant")
+
+    def set_tile_group(self, gx, gy):
+        check(lib().cgfdh_set_tile_group(self.h, gx, gy), "set_tile_group")
+
+    def kernel_choice(self):
+        v, gx, gy = C.c_int(), C.c_int(), C.c_int()
+        check(lib().cgfdh_get_kernel_choice(self.h, C.byref(v), C.byref(gx), C.byref(gy)),
+              "get_kernel_choice")
+        return v.value, gx.value, gy.value
+
+
+class Batch:
+    """Independent solvers advanced together (cgfdh_batch_*): 8-step cycles of
+    all of them captured once into a hipGraph and replayed."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        arr = (C.c_void_p * len(self.solvers))(*[s.h.value for s in self.solvers])
+        h = C.c_void_p()
+        check(lib().cgfdh_batch_create(len(self.solvers), arr, C.byref(h)), "batch_create")
+        self.h = h
+
+    def autotune(self, it0=0):
+        check(lib().cgfdh_batch_autotune(self.h, it0), "batch_autotune")
+
+    def run(self, it0, nsteps):
+        check(lib().cgfdh_batch_run(self.h, it0, nsteps), "batch_run")
+
+    def last_ms(self):
+        ms = C.c_double()
+        check(lib().cgfdh_batch_last_timing(self.h, C.byref(ms)), "batch_last_timing")
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cgfdh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

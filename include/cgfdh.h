@@ -187,6 +187,27 @@ int cgfdh_last_timing(cgfdh_solver *s, double *run_ms, double *stage_ms,
  * bits. */
 int cgfdh_set_kernel_variant(cgfdh_solver *s, int variant);
 
+/* Tile group of the grouped block order (gx x gy tiles per group, 0 0 = the
+ * default 9x2) and the solver's current kernel choice. */
+int cgfdh_set_tile_group(cgfdh_solver *s, int gx, int gy);
+int cgfdh_get_kernel_choice(cgfdh_solver *s, int *variant, int *gx, int *gy);
+
+/* Batched runs of independent solvers (no reference counterpart: the
+ * reference runs one grid per process; this serves many-source studies on
+ * one GPU).  Every solver is set up with the calls above; cgfdh_batch_run
+ * advances them all over the same steps, capturing 8-step cycles of all
+ * solvers once into a hipGraph (one stream per solver, concurrent branches)
+ * and replaying it; results equal cgfdh_run of each solver bit for bit.
+ * cgfdh_batch_autotune times each solver's kernel variants and tile groups
+ * for one step from the current state (restored afterwards) and keeps the
+ * fastest. */
+typedef struct cgfdh_batch cgfdh_batch;
+int cgfdh_batch_create(int n, cgfdh_solver *const *solvers, cgfdh_batch **out);
+int cgfdh_batch_destroy(cgfdh_batch *b);
+int cgfdh_batch_autotune(cgfdh_batch *b, int it0);
+int cgfdh_batch_run(cgfdh_batch *b, int it0, int nsteps);
+int cgfdh_batch_last_timing(cgfdh_batch *b, double *ms);
+
 /* ---- setup on the device ------------------------------------------------ */
 
 /* Metric from grid coordinates x3d,y3d,z3d (3 volumes with ghosts), as

@@ -1,0 +1,109 @@
+"""GPU: batched runs of independent solvers (cgfdh_batch_*).  The batch
+captures 8-step cycles of all solvers into one hipGraph and replays them; its
+results must equal each solver's own cgfdh_run bit for bit, and through that
+the oracle's."""
+import numpy as np
+import pytest
+
+from tests import cases as cs
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+NT, DT = 40, 0.004
+
+
+def _cases():
+    """Three uneven grids with different media, boundaries and sources."""
+    out = []
+    for n, (dims, medium, pml, seed) in enumerate([((22, 19, 17), "iso", (1, 1, 1, 1, 1, 0), 1),
+                                                   ((18, 20, 16), "vti", (1, 1, 0, 1, 1, 0), 2),
+                                                   ((17, 16, 18), "visco", (1, 0, 1, 1, 1, 0), 3)]):
+        ni, nj, nk = dims
+        case = cs.make_case(ni, nj, nk, seed=seed, free_top=True, pml=pml, layers=4,
+                            medium=medium)
+        case["src"] = cs.ricker_point_source(case, 3 + ni // 2, 3 + nj // 2, 3 + nk // 2, NT, DT,
+                                             fc=10.0, t0=0.1)
+        ridx, rw = cs.surface_receivers(case, n=3, seed=seed)
+        out.append((case, ridx, rw))
+    return out
+
+
+def _solver(case, ridx, rw, variant=4):
+    s = cs.make_solver(case, DT, NT, variant=variant)
+    s.set_receivers(ridx, rw)
+    return s
+
+
+def _state(s):
+    pg, dis = s.peak_ground_motion()
+    return s.wavefield(), s.pml_aux(), s.seismograms(), pg, dis
+
+
+def _same(a, b):
+    return all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+def test_batch_equals_individual_runs():
+    """19 steps (2 graph cycles + 3 eager steps) then 16 more from step 19
+    (a cycle captured at another start step): identical to cgfdh_run."""
+    from cgfd3d_amd import Batch
+    cases = _cases()
+    ref = []
+    for case, ridx, rw in cases:
+        s = _solver(case, ridx, rw)
+        s.run(0, 19)
+        s.run(19, 16)
+        ref.append(_state(s))
+        s.close()
+    sv = [_solver(case, ridx, rw) for case, ridx, rw in cases]
+    b = Batch(sv)
+    b.run(0, 19)
+    b.run(19, 16)
+    print(f"batch: last call {b.last_ms():.2f} ms")
+    for n, s in enumerate(sv):
+        assert _same(_state(s), ref[n]), f"solver {n}"
+    b.close()
+    for s in sv:
+        s.close()
+
+
+def test_batch_matches_oracle():
+    """One batch member against the oracle (wavefield, aux, seismograms)."""
+    from cgfd3d_amd import Batch
+    case, ridx, rw = _cases()[0]
+    s = _solver(case, ridx, rw)
+    b = Batch([s])
+    b.run(0, 24)
+    w_g, a_g, s_g = s.wavefield(), s.pml_aux(), s.seismograms()
+    b.close()
+    s.close()
+    w_o, a_o, s_o = po.Oracle(case).run(DT, 24, recv_idx=ridx, recv_w=rw, nt_total=NT)
+    print(f"batch vs oracle: rel L2 {cs.rel_l2(w_g, w_o):.1e}")
+    assert np.abs(s_o).max() > 0
+    assert np.array_equal(w_g, w_o) and np.array_equal(a_g, a_o) and np.array_equal(s_g, s_o)
+
+
+def test_batch_autotune_leaves_the_state():
+    """Autotuning runs trial steps but restores the state: the run after it
+    equals a run without it, whatever kernels were picked."""
+    from cgfd3d_amd import Batch
+    cases = _cases()
+    ref = []
+    for case, ridx, rw in cases:
+        s = _solver(case, ridx, rw)
+        s.run(0, 5)
+        s.run(5, 11)
+        ref.append(_state(s))
+        s.close()
+    sv = [_solver(case, ridx, rw) for case, ridx, rw in cases]
+    b = Batch(sv)
+    b.run(0, 5)
+    b.autotune(5)
+    print("autotuned:", [s.kernel_choice() for s in sv])
+    b.run(5, 11)
+    for n, s in enumerate(sv):
+        assert _same(_state(s), ref[n]), f"solver {n}"
+    b.close()
+    for s in sv:
+        s.close()

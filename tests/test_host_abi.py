@@ -49,9 +49,14 @@ def test_bad_descriptor_rejected_before_the_device():
     h = C.c_void_p()
     assert lib().cgfdh_create(C.byref(d), C.byref(h)) == -1
     assert b"minimum" in lib().cgfdh_last_error()
-    d.ni, d.medium_type = 20, 5     # visco-elastic: not part of this engine
+    d.ni, d.medium_type = 20, 6     # no medium with this code
     assert lib().cgfdh_create(C.byref(d), C.byref(h)) == -1
-    d.medium_type = 2
+    assert b"medium" in lib().cgfdh_last_error()
+    d.medium_type = 5               # visco-elastic needs 1..4 Maxwell bodies
+    for nm in (0, 5):
+        d.nmaxwell = nm
+        assert lib().cgfdh_create(C.byref(d), C.byref(h)) == -1
+    d.medium_type, d.nmaxwell = 2, 0
     d.pml_on[0], d.pml_layers[0] = 1, 25
     assert lib().cgfdh_create(C.byref(d), C.byref(h)) == -1
     d.pml_layers[0] = 5
