@@ -33,8 +33,10 @@ def hipcc():
     raise RuntimeError("hipcc not found: the gfx950 engine cannot be built")
 
 
-# one translation unit per medium's stage kernels, compiled in parallel
-UNITS = ["cgfdh.hip", "stage_iso.hip", "stage_vti.hip", "stage_aniso.hip", "stage_visco.hip"]
+# one translation unit per medium's stage kernels plus the surrogate network's
+# kernels, compiled in parallel
+UNITS = ["cgfdh.hip", "stage_iso.hip", "stage_vti.hip", "stage_aniso.hip", "stage_visco.hip",
+         "surrogate.hip"]
 
 
 def build_engine(verbose=False):

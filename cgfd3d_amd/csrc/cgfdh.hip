@@ -155,6 +155,9 @@ static int fail(int code, const char* fmt, ...)
   return code;
 }
 
+// error reporting for the other translation units of the library
+int cgfdh_internal_fail(int code, const char* msg) { return fail(code, "%s", msg); }
+
 #define HIPCHK(x)                                                                  \
   do {                                                                             \
     hipError_t e_ = (x);                                                           \

@@ -1,0 +1,189 @@
+"""Training and evaluation data of the seismogram surrogate, produced by the
+batched forward solver on one fixed grid and medium.
+
+Grid: 160 x 160 x 100 curvilinear, 100 m horizontal spacing, a flat bottom
+9.9 km below datum, seeded Gaussian-hill topography (6 hills, <= 600 m), the
+3-layer model times a seeded Gaussian random field (5 % std, 500 m
+correlation), CFS-PML 10 layers on 5 faces, traction-image free surface,
+dt = 8 ms, 1024 recorded steps.  Receivers: an 8 x 8 grid of surface nodes.
+
+Sources: point moment tensors at grid nodes of the interior (x, y at least
+20 nodes from the sides), depth 1-8 km below the local surface, strike /
+dip / rake uniform in [0, 360) / [0, 90] / [-180, 180), Ricker f0 uniform in
+1-3 Hz with its peak at 1.2 / f0.  Target of one source: the 3 velocity
+components at the 64 receivers over 1024 steps.
+"""
+import math
+
+import numpy as np
+
+from . import workloads as wk
+from .solver import Batch
+
+NI, NJ, NK, DH = 160, 160, 100, 100.0
+NT, DT = 1024, 0.008
+N_SIDE = 8
+PML = 10
+MARGIN = 20
+RANGES = dict(depth=(1000.0, 8000.0), strike=(0.0, 360.0), dip=(0.0, 90.0),
+              rake=(-180.0, 180.0), f0=(1.0, 3.0))
+
+
+class SurrogateGrid:
+    """The fixed grid, medium and receivers, built once."""
+
+    def __init__(self, seed=77, ni=NI, nj=NJ, nk=NK, dh=DH, nt=NT, dt=DT, n_side=N_SIDE,
+                 pml=PML):
+        self.ni, self.nj, self.nk, self.dh, self.nt, self.dt = ni, nj, nk, dh, nt, dt
+        top = wk.topography(ni, nj, dh, 6, seed, amp_max=600.0, sig_min=1500.0, sig_max=4000.0)
+        self.top = top
+        self.depth = (nk - 1) * dh
+        xyz = wk.layered_coords(ni, nj, nk, dh, top, self.depth)
+        grf = wk.gaussian_random_field((nk + 6, nj + 6, ni + 6), dh, 500.0, 0.05, seed + 1)
+        layers = [(1e9, 4500.0, 2600.0, 2400.0), (-1500.0, 6000.0, 3464.0, 2670.0),
+                  (-6000.0, 6800.0, 3900.0, 2900.0)]
+        lam, mu, slw, vpmax = wk.layered_medium(xyz[2], layers, grf)
+        ridx, rw = wk.surface_receiver_grid(ni, nj, nk, n_side)
+        self.nrecv = n_side * n_side
+        self.wl = wk.Workload(f"surrogate{ni}x{nj}x{nk}", ni, nj, nk, dt, nt, xyz, lam, mu, slw,
+                              [pml] * 5 + [0], 7000.0, 3.14, 2.0, True, None, ridx, rw)
+        self.x_range = ((3 + MARGIN - 3) * dh, (ni - 1 - MARGIN) * dh)
+        self.y_range = ((3 + MARGIN - 3) * dh, (nj - 1 - MARGIN) * dh)
+
+    def sample_sources(self, n, seed):
+        """n seeded sources: rows (x, y, depth, strike, dip, rake, f0) with x,
+        y, depth those of the grid node the source sits on, and its (i,j,k)."""
+        rng = np.random.default_rng(seed)
+        xyz = self.wl.xyz.reshape(3, self.nk + 6, self.nj + 6, self.ni + 6)
+        params = np.zeros((n, 7), np.float64)
+        ijk = np.zeros((n, 3), np.int32)
+        ksurf = 3 + self.nk - 1
+        for s in range(n):
+            i = int(rng.integers(3 + MARGIN, 3 + self.ni - MARGIN))
+            j = int(rng.integers(3 + MARGIN, 3 + self.nj - MARGIN))
+            want = rng.uniform(*RANGES["depth"])
+            col = xyz[2, :, j, i]
+            k = int(np.argmin(np.abs((col[ksurf] - col[3:ksurf + 1]) - want))) + 3
+            params[s] = (xyz[0, k, j, i], xyz[1, k, j, i], col[ksurf] - col[k],
+                         rng.uniform(*RANGES["strike"]), rng.uniform(*RANGES["dip"]),
+                         rng.uniform(*RANGES["rake"]), rng.uniform(*RANGES["f0"]))
+            ijk[s] = (i, j, k)
+        return params, ijk
+
+    def source_tables(self, p, ijk):
+        """Moment-source tables of one source (Ricker f0, peak at 1.2 / f0)."""
+        L = wk._setup_sigs()
+        m6 = np.zeros(6, np.float32)
+        L.cgfdh_moment_from_angles(p[3], p[4], p[5], 3.0e10, 1.0, 1.0e6, wk._p(m6))
+        f0 = float(p[6])
+        tpeak = 1.2 / f0
+        return wk.moment_source(self.ni, self.nj, self.nk, int(ijk[0]), int(ijk[1]), int(ijk[2]),
+                                self.dt, 0.0, m6, f0, tpeak, 2.0 * tpeak)
+
+
+def features(params, grid):
+    """Network input rows (float32, 32 wide): the 7 parameters scaled to
+    [-1, 1], the unit moment tensor of the mechanism (6 values) and the sines
+    and cosines of strike, dip and rake; zero padded."""
+    p = np.asarray(params, np.float64)
+    n = p.shape[0]
+    x = np.zeros((n, 32), np.float32)
+
+    def sc(v, lo, hi):
+        return 2.0 * (v - lo) / (hi - lo) - 1.0
+    x[:, 0] = sc(p[:, 0], *grid.x_range)
+    x[:, 1] = sc(p[:, 1], *grid.y_range)
+    x[:, 2] = sc(p[:, 2], *RANGES["depth"])
+    x[:, 3] = sc(p[:, 3], *RANGES["strike"])
+    x[:, 4] = sc(p[:, 4], *RANGES["dip"])
+    x[:, 5] = sc(p[:, 5], *RANGES["rake"])
+    x[:, 6] = sc(p[:, 6], *RANGES["f0"])
+    s, d, r = np.radians(p[:, 3]), np.radians(p[:, 4]), np.radians(p[:, 5])
+    # double-couple moment tensor (Aki & Richards, x north / y east / z down)
+    mxx = -(np.sin(d) * np.cos(r) * np.sin(2 * s) + np.sin(2 * d) * np.sin(r) * np.sin(s) ** 2)
+    mxy = np.sin(d) * np.cos(r) * np.cos(2 * s) + 0.5 * np.sin(2 * d) * np.sin(r) * np.sin(2 * s)
+    mxz = -(np.cos(d) * np.cos(r) * np.cos(s) + np.cos(2 * d) * np.sin(r) * np.sin(s))
+    myy = np.sin(d) * np.cos(r) * np.sin(2 * s) - np.sin(2 * d) * np.sin(r) * np.cos(s) ** 2
+    myz = -(np.cos(d) * np.cos(r) * np.sin(s) - np.cos(2 * d) * np.sin(r) * np.cos(s))
+    mzz = np.sin(2 * d) * np.sin(r)
+    x[:, 7:13] = np.stack([mxx, myy, mzz, myz, mxz, mxy], axis=1)
+    x[:, 13:19] = np.stack([np.sin(s), np.cos(s), np.sin(d), np.cos(d), np.sin(r), np.cos(r)],
+                           axis=1)
+    return x
+
+
+def to_target_layout(seis):
+    """[n, nrecv, 3, nt] solver seismograms -> [n, nt, 3 * nrecv] (the
+    network's channels-last output: channel = component * nrecv + receiver)."""
+    n, nr, nc, nt = seis.shape
+    return np.ascontiguousarray(seis.transpose(0, 3, 2, 1).reshape(n, nt, nc * nr))
+
+
+def from_target_layout(y, nrecv):
+    """[n, nt, 3 * nrecv] -> [n, nrecv, 3, nt]."""
+    n, nt, c = y.shape
+    return np.ascontiguousarray(y.reshape(n, nt, c // nrecv, nrecv).transpose(0, 3, 2, 1))
+
+
+class FdGenerator:
+    """Runs batches of sources on the fixed grid with the batched solver.
+    Solvers are built once per batch slot and reused: before each batch the
+    wavefield and PML memory are reset and the sources replaced."""
+
+    def __init__(self, grid, batch, variant=4, autotune=True):
+        self.grid, self.batch, self.variant, self.autotune = grid, batch, variant, autotune
+        self.solvers = []
+        self._zero_w = None
+        self._zero_aux = None
+        self.fd_seconds = 0.0
+        self.runs = 0
+
+    def _slots(self, n):
+        while len(self.solvers) < n:
+            self.solvers.append(self.grid.wl.build_solver(variant=self.variant))
+        return self.solvers[:n]
+
+    def run(self, params, ijk):
+        """Seismograms [n, nrecv, 3, nt] of the given sources."""
+        import time
+        g = self.grid
+        out = np.zeros((len(params), g.nrecv, 3, g.nt), np.float32)
+        for c0 in range(0, len(params), self.batch):
+            idx = range(c0, min(len(params), c0 + self.batch))
+            sv = self._slots(len(idx))
+            for s, q in zip(sv, idx):
+                if self._zero_w is None:
+                    self._zero_w = np.zeros(s.ncomp * s.nx * s.ny * s.nz, np.float32)
+                    self._zero_aux = np.zeros(s.aux_len(), np.float32)
+                s.set_wavefield(self._zero_w)
+                if self._zero_aux.size:
+                    s.set_pml_aux(self._zero_aux)
+                s.set_moment_sources(**g.source_tables(params[q], ijk[q]))
+            b = Batch(sv)
+            if self.autotune and c0 == 0:
+                b.autotune(0)
+                for s in sv:
+                    s.set_wavefield(self._zero_w)
+                    if self._zero_aux.size:
+                        s.set_pml_aux(self._zero_aux)
+            t = time.perf_counter()
+            b.run(0, g.nt)
+            self.fd_seconds += b.last_ms() / 1e3
+            for s, q in zip(sv, idx):
+                out[q] = s.seismograms()[:, :3, : g.nt]
+            b.close()
+            self.runs += len(idx)
+            _ = t
+        return out
+
+    def close(self):
+        for s in self.solvers:
+            s.close()
+        self.solvers = []
+
+
+def relative_misfit(pred, ref):
+    """Per-source relative L2 misfit |pred - ref| / |ref| over all traces."""
+    n = ref.shape[0]
+    d = (pred.reshape(n, -1).astype(np.float64) - ref.reshape(n, -1))
+    return np.sqrt((d ** 2).sum(axis=1) / (ref.reshape(n, -1).astype(np.float64) ** 2).sum(axis=1))

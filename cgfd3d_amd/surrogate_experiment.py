@@ -1,0 +1,180 @@
+"""End-to-end surrogate experiment: generate FD runs with the batched solver,
+train the network on the GPU with the library's BF16 kernels, and evaluate
+it on held-out sources against FD.
+
+    python -m cgfd3d_amd.surrogate_experiment --train 768 --test 128
+
+Prints one JSON line: FD and surrogate seismogram sets per second (a set is
+the 3 x nrecv x nt traces of one source), held-out relative L2 misfits of
+the surrogate and of the baselines without it (nearest training source in
+parameter space, and the untrained network), and the training record.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def train(net, x, y, tn2, epochs, batch, lr_max, seed, time_budget=None, log_every=None):
+    """Minibatch Adam with a cosine learning-rate decay; y and tn2 are device
+    tensors, x a host array.  Returns the per-epoch mean training loss."""
+    import torch
+    n = x.shape[0]
+    rng = np.random.default_rng(seed)
+    xd = torch.from_numpy(np.ascontiguousarray(x)).to("cuda:0")
+    steps_per_epoch = n // batch
+    total = epochs * steps_per_epoch
+    hist = []
+    t0 = time.perf_counter()
+    step = 0
+    for ep in range(epochs):
+        perm = torch.from_numpy(rng.permutation(n)).to("cuda:0")
+        acc = torch.zeros((), device="cuda:0")
+        for s in range(steps_per_epoch):
+            idx = perm[s * batch:(s + 1) * batch]
+            lr = lr_max * 0.5 * (1.0 + math.cos(math.pi * step / max(1, total)))
+            loss = net.train_step(xd[idx], y[idx], tn2[idx], lr)
+            acc += loss.sum()
+            step += 1
+        hist.append(float(acc) / (steps_per_epoch * batch))
+        if log_every and (ep % log_every == 0 or ep == epochs - 1):
+            log(f"epoch {ep}: train loss {hist[-1]:.4f} ({time.perf_counter() - t0:.1f} s)")
+        if time_budget and time.perf_counter() - t0 > time_budget:
+            log(f"training time budget reached after {ep + 1} epochs")
+            break
+    return hist
+
+
+def predict(net, x, nsamp_per_call=64):
+    """Host predictions [n, nt, channels] of the network for feature rows x."""
+    outs = []
+    n = x.shape[0]
+    for c0 in range(0, n, nsamp_per_call):
+        xb = x[c0:c0 + nsamp_per_call]
+        m = xb.shape[0]
+        pad = (-m) % 8
+        if pad:
+            xb = np.concatenate([xb, np.zeros((pad, xb.shape[1]), xb.dtype)])
+        y = net.forward(xb).float().cpu().numpy().reshape(m + pad, -1)[:m]
+        outs.append(y)
+    return np.concatenate(outs)
+
+
+def inference_rate(net, x, nsamp=256, reps=10):
+    """Surrogate seismogram sets per second: forward passes of nsamp sources
+    timed with HIP events on the launch stream."""
+    import torch
+    xb = np.resize(x, (nsamp, x.shape[1])).astype(np.float32)
+    xd = torch.from_numpy(xb).to("cuda:0")
+    net.forward(xd)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        net.forward(xd)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return nsamp / (ms / 1e3), ms
+
+
+def nearest_neighbour(xtr, ytr, xte):
+    """Baseline without the network: the seismograms of the training source
+    nearest in the scaled parameter space (the 7 scaled parameters)."""
+    d = ((xte[:, None, :7] - xtr[None, :, :7]) ** 2).sum(axis=2)
+    return ytr[np.argmin(d, axis=1)]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train", type=int, default=768)
+    ap.add_argument("--test", type=int, default=128)
+    ap.add_argument("--fd-batch", type=int, default=32)
+    ap.add_argument("--epochs", type=int, default=400)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--train-seconds", type=float, default=240.0)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--save", type=str, default="")
+    ap.add_argument("--small", action="store_true",
+                    help="a 64x64x48 grid with 256 steps (pipeline check)")
+    args = ap.parse_args()
+    import torch
+    from . import surrogate as sg
+    from . import surrogate_data as sd
+
+    t = time.perf_counter()
+    if args.small:
+        grid = sd.SurrogateGrid(ni=64, nj=64, nk=48, nt=256, n_side=4)
+    else:
+        grid = sd.SurrogateGrid()
+    log(f"grid {grid.wl.name} built in {time.perf_counter() - t:.1f} s")
+    ptr, ijk_tr = grid.sample_sources(args.train, args.seed)
+    pte, ijk_te = grid.sample_sources(args.test, args.seed + 1000)
+    gen = sd.FdGenerator(grid, args.fd_batch)
+    t = time.perf_counter()
+    seis_tr = gen.run(ptr, ijk_tr)
+    log(f"{args.train} training runs in {time.perf_counter() - t:.1f} s "
+        f"(device {gen.fd_seconds:.1f} s)")
+    fd_dev_s, fd_runs = gen.fd_seconds, gen.runs
+    t = time.perf_counter()
+    seis_te = gen.run(pte, ijk_te)
+    fd_dev_s, fd_runs = gen.fd_seconds, gen.runs
+    gen.close()
+    log(f"{args.test} held-out runs in {time.perf_counter() - t:.1f} s")
+    fd_rate = fd_runs / fd_dev_s
+
+    xtr, xte = sd.features(ptr, grid), sd.features(pte, grid)
+    ytr, yte = sd.to_target_layout(seis_tr), sd.to_target_layout(seis_te)
+    scale = 1.0 / float(np.sqrt((ytr.astype(np.float64) ** 2).mean()))
+    layers = sg.default_layers(grid.nrecv, grid.nt)
+    net = sg.SurrogateNet(layers, seed=args.seed)
+    log(f"network: {net.n_params() / 1e6:.2f} M parameters")
+    yd = torch.from_numpy((ytr * scale).astype(np.float32)).to("cuda:0")
+    tn2 = torch.from_numpy(((ytr * scale).reshape(len(ytr), -1).astype(np.float64) ** 2)
+                           .sum(axis=1).astype(np.float32)).to("cuda:0")
+
+    untrained = sd.relative_misfit(predict(net, xte) / scale, yte.reshape(len(yte), -1))
+    t = time.perf_counter()
+    hist = train(net, xtr, yd, tn2, args.epochs, args.batch, args.lr, args.seed,
+                 time_budget=args.train_seconds, log_every=25)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t
+    pred = predict(net, xte) / scale
+    mis = sd.relative_misfit(pred, yte.reshape(len(yte), -1))
+    mis_tr = sd.relative_misfit(predict(net, xtr[:128]) / scale, ytr[:128].reshape(128, -1))
+    nn = sd.relative_misfit(nearest_neighbour(xtr, ytr.reshape(len(ytr), -1), xte),
+                            yte.reshape(len(yte), -1))
+    rate, ms = inference_rate(net, xte)
+    out = {
+        "metric": "surrogate vs FD seismogram sets/s",
+        "surrogate_sets_per_s": rate, "surrogate_ms_per_256": ms,
+        "fd_sets_per_s": fd_rate, "speedup": rate / fd_rate,
+        "grid": [grid.ni, grid.nj, grid.nk], "nt": grid.nt, "nrecv": grid.nrecv,
+        "train_runs": args.train, "test_runs": args.test,
+        "heldout_misfit_median": float(np.median(mis)), "heldout_misfit_mean": float(mis.mean()),
+        "heldout_frac_le_5pct": float((mis <= 0.05).mean()),
+        "train_misfit_median": float(np.median(mis_tr)),
+        "nearest_neighbour_misfit_median": float(np.median(nn)),
+        "untrained_misfit_median": float(np.median(untrained)),
+        "epochs_run": len(hist), "train_seconds": train_s,
+        "final_train_loss": hist[-1] if hist else None,
+        "params": net.n_params(), "dtype": "bf16 weights/activations, fp32 accumulation",
+    }
+    if args.save:
+        os.makedirs(os.path.dirname(os.path.abspath(args.save)), exist_ok=True)
+        np.savez(args.save, misfit=mis, nn_misfit=nn, hist=np.array(hist), params_test=pte,
+                 pred0=pred[:2], ref0=yte[:2])
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -21,15 +21,27 @@ def _lib():
     return workloads._setup_sigs()
 
 
+def _declared(header):
+    hdr = open(os.path.join(ROOT, "include", header)).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    return sorted(set(re.findall(r"\b(cgfd[hs]_[a-z0-9_]+)\s*\(", hdr)))
+
+
 def test_library_exports_every_declared_symbol():
     from cgfd3d_amd._lib import LIB_PATH
-    hdr = open(os.path.join(ROOT, "include", "cgfdh.h")).read()
-    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
-    names = sorted(set(re.findall(r"\b(cgfdh_[a-z0-9_]+)\s*\(", hdr)))
-    assert len(names) >= 25
+    headers = sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h"))
+    assert headers == ["cgfdh.h", "cgfds.h"]
+    names = [n for h in headers for n in _declared(h)]
+    assert len(names) >= 30
     lib = C.CDLL(LIB_PATH)
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
+
+
+def test_surrogate_binding_covers_its_header():
+    from cgfd3d_amd._lib import SIGNATURES
+    bound = {s[0] for s in SIGNATURES}
+    assert set(_declared("cgfds.h")) <= bound
 
 
 def test_python_binding_covers_the_header():
