@@ -1,6 +1,6 @@
 """Benchmark of the time-stepping hot path on one MI355X.
 
-Workload (the roofline configuration): isotropic elastic 600x600x300
+Default workload (the roofline configuration): isotropic elastic 600x600x300
 curvilinear grid (~108 M points), dx = 50 m, topography from 20 seeded
 Gaussian hills, 3-layer medium times a seeded Gaussian random field, seeded
 double-couple source with a 2 Hz Ricker, CFS-PML 20 layers on 5 faces,
@@ -12,8 +12,18 @@ JSON line: grid-point RK4 updates/s, the HBM roofline of the stage kernel
 (algorithmic bytes per launch / HIP-event launch time) and a CPU baseline (the
 oracle, OpenMP over the host cores, on a bounded sub-block of the same
 workload).
+
+--workload vti | aniso | visco: the same topography and layered GRF model on
+400x400x200 with a VTI medium (Thomsen eps, delta, gamma seeded per layer in
+[0, 0.2]), its seeded random rotation (21 Cij), or the GMB visco-elastic
+medium (3 Maxwell bodies, Qp 100, Qs 50 over 0.5-5 Hz).
+--workload batch: 64 independent grids (24 at 128^3, 24 at 192x192x128, 16
+at 256x256x160; own topography, medium, source position, mechanism and 1-3 Hz
+dominant frequency each) advanced together by one replayed hipGraph with
+autotuned kernels.
 """
 import argparse
+import collections
 import json
 import os
 import sys
@@ -35,6 +45,8 @@ KERNELS = [
     ("stage_interior_point_by2", "stage_interior_point_kernel<BY=2> + stage_box_kernel"),
     ("stage_interior_vec2", "stage_interior_vec_kernel<NQ=2> + stage_box_kernel"),
     ("stage_interior_vec4", "stage_interior_vec_kernel<NQ=4> + stage_box_kernel"),
+    ("stage_interior_lds8", "stage_interior_lds_kernel<TY=8> + stage_box_kernel"),
+    ("stage_interior_lds16", "stage_interior_lds_kernel<TY=16> + stage_box_kernel"),
 ]
 
 
@@ -58,7 +70,8 @@ def measured_traffic(kernel, workload_name):
 
 def cpu_baseline(wl, budget_s=15.0):
     """The oracle on a sub-block cut from the same workload (top of the grid,
-    free surface, PML on 5 faces, the source inside), timed on the host cores."""
+    free surface, PML on 5 faces, the source inside, the workload's medium),
+    timed on the host cores."""
     import numpy as np
     from oracle import pyoracle as po
     from tests import cases as cs
@@ -71,7 +84,8 @@ def cpu_baseline(wl, budget_s=15.0):
     nx, ny, nz = ni + 6, nj + 6, nk + 6
     xyz = np.ascontiguousarray(wl.xyz.reshape(3, wl.nk + 6, wl.nj + 6, wl.ni + 6)[(slice(None),) + sl])
     shp = (wl.nk + 6, wl.nj + 6, wl.ni + 6)
-    med = np.stack([wl.lam.reshape(shp)[sl], wl.mu.reshape(shp)[sl], wl.slw.reshape(shp)[sl]])
+    vols = [wl.lam, wl.mu, wl.slw] if wl.med_vols is None else wl.med_vols
+    med = np.stack([np.asarray(v).reshape(shp)[sl] for v in vols])
     orc = po.load_oracle()
     metric = np.zeros((10, nz, ny, nx), np.float32)
     orc.orc_metric(ni, nj, nk, po.ptr(xyz), po.ptr(metric))
@@ -81,7 +95,10 @@ def cpu_baseline(wl, budget_s=15.0):
                                     vel=wl.pml_vel, alpha=wl.pml_alpha, beta=wl.pml_beta)
     case = dict(ni=ni, nj=nj, nk=nk, metric=metric, med=np.ascontiguousarray(med),
                 free_top=1, pml_on=pml_on, pml_layers=[layers] * 5 + [0],
-                pml_A=A, pml_B=B, pml_D=D, pml_region=region, src=None)
+                pml_A=A, pml_B=B, pml_D=D, pml_region=region, src=None,
+                medium_type=wl.medium_type, coords=xyz)
+    if wl.wl is not None:
+        case["visco_wl"] = wl.wl
     src = dict(wl.src)
     src.update(nsrc=1, ijk=[3 + ni // 2, 3 + nj // 2, 3 + nk - 1 - 20], inc=[0, 0, 0], gauss=0)
     case["src"] = src
@@ -98,8 +115,69 @@ def cpu_baseline(wl, budget_s=15.0):
     return {"value": value, "unit": "grid-point RK4 updates/s", "cores": threads,
             "kind": "port",
             "sample": f"oracle (C restatement, OpenMP {threads} threads) on a {ni}x{nj}x{nk} "
+                      f"{wl.name.rstrip('0123456789x')} "
                       f"sub-block of the same workload with free surface, 10-layer PML on 5 "
                       f"faces and the source, {n} RK4 steps in {tn:.1f} s"}
+
+
+MEDIUM_NAMES = {2: "iso elastic", 3: "VTI elastic (Thomsen eps/delta/gamma in [0, 0.2])",
+                4: "general anisotropic (rotated VTI, 21 Cij)",
+                5: "visco-elastic GMB (3 Maxwell bodies, Qp 100, Qs 50, 0.5-5 Hz)"}
+
+
+def bench_batch(args):
+    """The batched configuration: every grid built, uploaded and freed on the
+    host one at a time, then all advanced together."""
+    import torch
+    from cgfd3d_amd import Batch, workloads
+    t = time.perf_counter()
+    solvers, npts, bstep = [], 0, 0.0
+    nt_total = max(1001, args.steps + args.warmup + 16)
+    for wl in workloads.batch_workloads(nt_total=nt_total):
+        solvers.append(wl.build_solver(variant=args.variant))
+        npts += wl.npoints
+        bstep += wl.algorithmic_bytes_per_step()
+        del wl
+    log(f"{len(solvers)} grids, {npts / 1e6:.1f} M points, set up in "
+        f"{time.perf_counter() - t:.1f} s")
+    b = Batch(solvers)
+    t = time.perf_counter()
+    b.autotune(0)
+    choices = collections.Counter(tuple(s.kernel_choice()) for s in solvers)
+    log(f"autotuned in {time.perf_counter() - t:.1f} s: {dict(choices)}")
+    b.run(0, args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.run(args.warmup, args.steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    dev_ms = b.last_ms()
+    import numpy as np
+    for s in solvers:
+        if not np.isfinite(s.seismograms()).all():
+            raise RuntimeError("non-finite seismograms")
+    b.close()
+    for s in solvers:
+        s.close()
+    achieved = bstep * args.steps / (dev_ms / 1e3) / 1e9
+    out = {
+        "metric": "grid-point RK4 updates/s", "value": npts * args.steps / wall,
+        "unit": "grid-point RK4 updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batch of 64 iso elastic grids (24x128^3, 24x192x192x128, "
+                               "16x256x256x160), own topography/GRF/source each, fc 1-3 Hz, "
+                               "CFS-PML 20 x5, free surface, one replayed hipGraph per 8 steps, "
+                               "autotuned kernels",
+                   "grids": len(solvers), "points": npts, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_step": bstep,
+                     "kernel": "all stage kernels of the batch (graph replay device time)"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -110,8 +188,11 @@ def main():
     ap.add_argument("--variant", type=int, default=4,
                     help="stage kernels: 0 per-point, 1 tiled, 2/3 z-marching interior + "
                          "shell, 4/5 per-point interior (64x4 / 64x2) + shell, 6/7 "
-                         "x-vectorised interior (2 / 4 points per thread) + shell")
-    ap.add_argument("--size", type=str, default="600x600x300")
+                         "x-vectorised interior (2 / 4 points per thread) + shell, 8/9 "
+                         "LDS-tiled interior (64x8 / 64x16 plane tiles) + shell")
+    ap.add_argument("--workload", default="iso", choices=["iso", "vti", "aniso", "visco", "batch"])
+    ap.add_argument("--size", type=str, default=None,
+                    help="grid NIxNJxNK (default 600x600x300 iso, 400x400x200 media)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device copies of MIB MiB (PMC counter calibration)")
@@ -134,9 +215,17 @@ def main():
     import torch
     from cgfd3d_amd import workloads
 
-    ni, nj, nk = (int(v) for v in args.size.split("x"))
+    if args.workload == "batch":
+        return bench_batch(args)
+    size = args.size or ("600x600x300" if args.workload == "iso" else "400x400x200")
+    ni, nj, nk = (int(v) for v in size.split("x"))
     t = time.perf_counter()
-    wl = workloads.iso_topo_grf(ni=ni, nj=nj, nk=nk, nt_total=max(2001, args.steps + args.warmup))
+    if args.workload == "iso":
+        wl = workloads.iso_topo_grf(ni=ni, nj=nj, nk=nk,
+                                    nt_total=max(2001, args.steps + args.warmup))
+    else:
+        wl = workloads.media_topo_grf(args.workload, ni=ni, nj=nj, nk=nk,
+                                      nt_total=max(1501, args.steps + args.warmup))
     log(f"workload {wl.name}: {wl.npoints / 1e6:.1f} M points, aux {wl.n_aux() / 1e6:.2f} M, "
         f"built in {time.perf_counter() - t:.1f} s")
     t = time.perf_counter()
@@ -162,6 +251,8 @@ def main():
     bstep = wl.algorithmic_bytes_per_step()
     per_launch = bstep / 4.0
     achieved = per_launch / (stage_ms / nlaunch / 1e3) / 1e9
+    if wl.medium_type == 5 and args.variant in (6, 7):
+        args.variant = 4
     kernel = KERNELS[args.variant]
     traffic = measured_traffic(kernel[0], wl.name)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -177,9 +268,11 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{wl.name} iso elastic, 20 Gaussian hills, 3-layer+GRF, "
-                               f"DC Ricker 2 Hz, CFS-PML 20 x5, free surface, 256 receivers",
-                   "grid": [wl.ni, wl.nj, wl.nk], "dt": wl.dt, "parallelism": "single GPU"},
+        "config": {"workload": f"{wl.name} {MEDIUM_NAMES[wl.medium_type]}, 20 Gaussian hills, "
+                               f"3-layer+GRF, DC Ricker 2 Hz, CFS-PML 20 x5, free surface, "
+                               f"256 receivers",
+                   "grid": [wl.ni, wl.nj, wl.nk], "dt": wl.dt, "parallelism": "single GPU",
+                   "bytes_per_point_step": wl.bytes_per_point()},
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

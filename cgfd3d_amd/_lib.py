@@ -61,6 +61,9 @@ SIGNATURES = [
     ("cgfds_colsum_bf16", C.c_int, [C.c_int, C.c_int, _P, C.c_long, _P, C.c_int, _P, _P]),
     ("cgfds_loss_grad", C.c_int, [C.c_int, C.c_long, _P, _P, _P, C.c_float, _P, _P, _P]),
     ("cgfds_adam", C.c_int, [C.c_long, _P, _P, _P, _P, _P] + [C.c_float] * 8 + [_P]),
+    ("cgfds_mt_contract", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
+    ("cgfds_mt_loss_grad", C.c_int,
+     [C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, C.c_float, _P, _P, _P]),
     ("cgfds_f32_to_bf16", C.c_int, [C.c_long, _P, _P, _P]),
 ]
 

@@ -575,13 +575,13 @@ extern "C" int cgfdh_get_seismograms(cgfdh_solver* s, float* out)
 
 extern "C" int cgfdh_set_kernel_variant(cgfdh_solver* s, int v)
 {
-  if (!s || v < 0 || v > 7) return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: %d", v);
+  if (!s || v < 0 || v > 9) return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: %d", v);
   if (s->d.medium_type != CGFDH_MEDIUM_ELASTIC_ISO && v >= 1 && v <= 3)
     return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: variant %d is built for the isotropic "
-                "medium only (VTI / anisotropic: 0, 4..7; visco-elastic: 0, 4, 5)", v);
-  if (s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO && v >= 6)
+                "medium only (VTI / anisotropic: 0, 4..9; visco-elastic: 0, 4, 5, 8, 9)", v);
+  if (s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO && (v == 6 || v == 7))
     return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: variant %d is not built for the "
-                "visco-elastic medium (0, 4, 5)", v);
+                "visco-elastic medium (0, 4, 5, 8, 9)", v);
   s->variant = v;
   return CGFDH_OK;
 }
@@ -1075,7 +1075,8 @@ extern "C" int cgfdh_batch_autotune(cgfdh_batch* b, int it0)
     hipError_t e = copy_state(true);
     const bool iso = s->d.medium_type == CGFDH_MEDIUM_ELASTIC_ISO;
     const bool visco = s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO;
-    std::vector<int> variants = visco ? std::vector<int>{4, 5} : std::vector<int>{4, 6, 7};
+    std::vector<int> variants = visco ? std::vector<int>{4, 5, 8, 9}
+                                      : std::vector<int>{4, 6, 7, 8, 9};
     if (iso) variants.push_back(3);
     const int groups[][2] = {{9, 2}, {3, 4}, {9, 8}};
     float best = 1e30f;

@@ -258,6 +258,80 @@ static void launch_interior_point(hipStream_t st, const StageParams& P, BoxParam
 #undef CGFDH_IP
 }
 
+// interior box, one plane of a 64 x TY tile per block: the x and y stencil
+// operands of all 9 fields come from an LDS copy of the tile plus its 4-wide
+// x/y halo (each value loaded once per block instead of once per reader),
+// the z operands straight from global memory, issued before the barrier.
+// Same grouped block order as the per-point kernel.
+template <int DX, int DY, int DZ, int KIND, int MT, int TY>
+__global__ void __launch_bounds__(64 * TY)
+stage_interior_lds_kernel(StageParams P, BoxParams B)
+{
+  using T = LdsTile<TY>;
+  __shared__ float sm[9 * T::PLANE];
+  int bx, by, k;
+  // block-uniform exit: every thread of a block takes the same branch
+  if (!grouped_block(B, bx, by, k)) return;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i0 = B.i0 + bx * 64, j0 = B.j0 + by * TY;
+  const int i = i0 + tx, j = j0 + ty;
+  const int nx = P.nx;
+  const int ny = (int)(P.ss / nx);
+  const long V = P.vol, ss = P.ss;
+  const long pk = (long)k * ss;
+  constexpr int xl = -Win<DX>::lo;
+  constexpr int yl = -Win<DY>::lo;
+  const bool valid = i < B.i0 + B.ni && j < B.j0 + B.nj;
+  const long p = (long)i + (long)j * nx + pk;
+  float Z[9];
+  if (valid) {
+#pragma unroll
+    for (int c = 0; c < 9; c++) Z[c] = d5<DZ>(P.wcur + c * V, p, ss);
+  }
+  // tile entry e = row * W + col holds point (i0 - xl + col, j0 - yl + row);
+  // the interior box lies >= 3 points inside the allocation, so only the
+  // right/back ends of a partial tile can leave a row (those read zeros and
+  // feed no valid point)
+  for (int e = ty * 64 + tx; e < T::PLANE; e += 64 * TY) {
+    const int r = e / T::W, col = e - r * T::W;
+    const int gi = i0 - xl + col, gj = j0 - yl + r;
+    const bool ok = gi < nx && gj < ny;
+    const long g = (long)gi + (long)gj * nx + pk;
+#pragma unroll
+    for (int c = 0; c < 9; c++) sm[c * T::PLANE + e] = ok ? P.wcur[c * V + g] : 0.0f;
+  }
+  __syncthreads();
+  if (!valid) return;
+  float X[9], Y[9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    const float* s = sm + c * T::PLANE;
+    const int rowb = (ty + yl) * T::W + tx;
+    X[c] = d5v<DX>(s[rowb], s[rowb + 1], s[rowb + 2], s[rowb + 3], s[rowb + 4]);
+    const int colb = ty * T::W + tx + xl;
+    Y[c] = d5v<DY>(s[colb], s[colb + T::W], s[colb + 2 * T::W], s[colb + 3 * T::W],
+                   s[colb + 4 * T::W]);
+  }
+  interior_finish<KIND, MT>(P, i, j, k, p, X, Y, Z);
+}
+
+template <int KIND, int MT, int TY>
+static void launch_interior_lds(hipStream_t st, const StageParams& P, BoxParams B, int code)
+{
+  const int grid = grouped_grid(B, TY);
+  dim3 blk(64, TY, 1);
+#define CGFDH_IL(c, DX, DY, DZ)                                                          \
+  case c:                                                                                \
+    hipLaunchKernelGGL((stage_interior_lds_kernel<DX, DY, DZ, KIND, MT, TY>), dim3(grid), blk, \
+                       0, st, P, B);                                                     \
+    break;
+  switch (code) {
+    CGFDH_IL(0, 0, 0, 0) CGFDH_IL(1, 0, 0, 1) CGFDH_IL(2, 0, 1, 0) CGFDH_IL(3, 0, 1, 1)
+    CGFDH_IL(4, 1, 0, 0) CGFDH_IL(5, 1, 0, 1) CGFDH_IL(6, 1, 1, 0) CGFDH_IL(7, 1, 1, 1)
+  }
+#undef CGFDH_IL
+}
+
 // general per-point kernel restricted to a box (the boundary shell); FLAGS
 // says whether the box touches the free-surface rows.  256-thread blocks over
 // tiles tw points wide and 256/tw rows deep (tw = the box width when that is

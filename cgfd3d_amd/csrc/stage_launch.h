@@ -10,9 +10,10 @@
 // (64x8 / 64x16 tiles) plus the per-point kernel on the boundary shell;
 // 4/5 the lean per-point interior kernel (64x4 / 64x2 blocks) plus the
 // shell; 6/7 the x-vectorised interior kernel (2 points per thread on 64x4
-// threads / 4 points per thread on 64x2 threads) plus the shell.  The VTI
-// and anisotropic media build variants 0 and 4..7, the visco-elastic medium
-// 0, 4 and 5.
+// threads / 4 points per thread on 64x2 threads) plus the shell; 8/9 the
+// LDS-tiled interior kernel (one 64x8 / 64x16 plane tile per block, x/y
+// operands from LDS) plus the shell.  The VTI and anisotropic media build
+// variants 0 and 4..9, the visco-elastic medium 0, 4, 5, 8 and 9.
 #pragma once
 #include <algorithm>
 #include <cstdio>
@@ -100,7 +101,8 @@ static void launch_kind(const LaunchCtx& L, const StageParams& P, int dx, int dy
     return b;
   };
   const BoxParams in = box(ilo, ihi, jlo, jhi, klo, khi);
-  if (MT != MEDIUM_VISCO && in.ni > 0 && in.nj > 0 && in.nk > 0 && L.variant >= 6) {
+  if (MT != MEDIUM_VISCO && in.ni > 0 && in.nj > 0 && in.nk > 0 &&
+      (L.variant == 6 || L.variant == 7)) {
     // whole point groups on the vectorised kernel, the last columns on the
     // per-point box kernel
     BoxParams vb = in;
@@ -117,7 +119,11 @@ static void launch_kind(const LaunchCtx& L, const StageParams& P, int dx, int dy
     if (in.ni > vb.ni)
       launch_box<KIND, MT, HAS_PML>(st, P, box(ilo + vb.ni, ihi, jlo, jhi, klo, khi), code);
   } else if (in.ni > 0 && in.nj > 0 && in.nk > 0) {
-    if (L.variant == 5) {
+    if (L.variant == 8) {
+      launch_interior_lds<KIND, MT, 8>(st, P, in, code);
+    } else if (L.variant == 9) {
+      launch_interior_lds<KIND, MT, 16>(st, P, in, code);
+    } else if (L.variant == 5) {
       launch_interior_point<KIND, MT, 2>(st, P, in, code);
     } else if (L.variant == 4) {
       launch_interior_point<KIND, MT, 4>(st, P, in, code);

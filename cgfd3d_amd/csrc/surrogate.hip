@@ -246,6 +246,66 @@ __global__ void adam_kernel(long n, float* w, float* m, float* v, const float* g
   if (w16) w16[i] = f2bf(wi);
 }
 
+// moment-tensor contraction of basis waveforms, one block per sample:
+// y[b][t][c] = sum_k m[b][k] g[b][t][k][c] (k in increasing order)
+constexpr int MT_MAX = 8;
+
+__global__ __launch_bounds__(256) void mt_contract_kernel(int nt, int nc, int nk, const float* m,
+                                                         const float* g, float* y)
+{
+  const int b = blockIdx.x;
+  float mk[MT_MAX];
+  for (int k = 0; k < nk; ++k) mk[k] = m[b * nk + k];
+  const long n = (long)nt * nc;
+  const float* gb = g + (long)b * n * nk;
+  for (long e = threadIdx.x; e < n; e += 256) {
+    const long t = e / nc, c = e - t * nc;
+    const float* q = gb + t * nk * nc + c;
+    float v = mk[0] * q[0];
+    for (int k = 1; k < nk; ++k) v = v + mk[k] * q[(long)k * nc];
+    y[(long)b * n + e] = v;
+  }
+}
+
+// the contraction fused with the per-sample relative misfit and its gradient
+// with respect to the basis waveforms: dg[b][t][k][c] = bf16(m[b][k] dy[b][t][c]),
+// dy = 2 (y - T) / (nsamp (tn2[b] + eps))
+__global__ __launch_bounds__(256) void mt_loss_grad_kernel(int nt, int nc, int nk, const float* m,
+                                                          const float* g, const float* T,
+                                                          const float* tn2, float eps, int nsamp,
+                                                          uint16_t* dg, float* loss)
+{
+  __shared__ float red[256];
+  const int b = blockIdx.x;
+  float mk[MT_MAX];
+  for (int k = 0; k < nk; ++k) mk[k] = m[b * nk + k];
+  const float den = tn2[b] + eps;
+  const float scale = 2.0f / ((float)nsamp * den);
+  const long n = (long)nt * nc;
+  const float* gb = g + (long)b * n * nk;
+  uint16_t* db = dg ? dg + (long)b * n * nk : nullptr;
+  float s = 0.f;
+  for (long e = threadIdx.x; e < n; e += 256) {
+    const long t = e / nc, c = e - t * nc;
+    const long o = t * nk * nc + c;
+    float v = mk[0] * gb[o];
+    for (int k = 1; k < nk; ++k) v = v + mk[k] * gb[o + (long)k * nc];
+    const float d = v - T[(long)b * n + e];
+    s = s + d * d;
+    if (db) {
+      const float dy = d * scale;
+      for (int k = 0; k < nk; ++k) db[o + (long)k * nc] = f2bf(mk[k] * dy);
+    }
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss[b] = red[0] / den;
+}
+
 __global__ void to_bf16_kernel(long n, const float* src, uint16_t* dst)
 {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -345,6 +405,28 @@ extern "C" int cgfds_adam(long n, float* w, float* m, float* v, const float* g, 
                      (hipStream_t)stream, n, w, m, v, g, (uint16_t*)w16, lr, b1, b2, c1, c2, bc1,
                      bc2, eps);
   return launch_check("cgfds_adam");
+}
+
+extern "C" int cgfds_mt_contract(int nsamp, int nt, int nc, int nk, const float* m,
+                                 const float* g, float* y, void* stream)
+{
+  if (nsamp <= 0 || nt <= 0 || nc <= 0 || nk < 1 || nk > MT_MAX || !m || !g || !y)
+    return sfail(CGFDH_EINVAL, "cgfds_mt_contract: bad argument (nk must be 1..8)");
+  hipLaunchKernelGGL(mt_contract_kernel, dim3(nsamp), dim3(256), 0, (hipStream_t)stream, nt, nc,
+                     nk, m, g, y);
+  return launch_check("cgfds_mt_contract");
+}
+
+extern "C" int cgfds_mt_loss_grad(int nsamp, int nt, int nc, int nk, const float* m,
+                                  const float* g, const float* T, const float* tnorm2, float eps,
+                                  void* dg16, float* loss, void* stream)
+{
+  if (nsamp <= 0 || nt <= 0 || nc <= 0 || nk < 1 || nk > MT_MAX || !m || !g || !T || !tnorm2 ||
+      !loss)
+    return sfail(CGFDH_EINVAL, "cgfds_mt_loss_grad: bad argument (nk must be 1..8)");
+  hipLaunchKernelGGL(mt_loss_grad_kernel, dim3(nsamp), dim3(256), 0, (hipStream_t)stream, nt, nc,
+                     nk, m, g, T, tnorm2, eps, nsamp, (uint16_t*)dg16, loss);
+  return launch_check("cgfds_mt_loss_grad");
 }
 
 extern "C" int cgfds_f32_to_bf16(long n, const float* src, void* dst, void* stream)

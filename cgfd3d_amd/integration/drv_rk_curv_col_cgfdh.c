@@ -5,9 +5,10 @@
  * (include/cgfdh.h).  Compile it into the reference's main program in place
  * of forward/drv_rk_curv_col.c and link cgfd3d_amd/libcgfdh.so.
  *
- * Covered: isotropic elastic medium, CFS-PML faces, traction-image free
- * surface, point and Gaussian moment sources, receivers.  Anything else the
- * par file asks for (other media, force sources, dd sources, receiver lines,
+ * Covered: isotropic, VTI and general anisotropic elastic media and the GMB
+ * visco-elastic medium, CFS-PML faces, traction-image free surface, point
+ * and Gaussian moment sources, receivers.  Anything else the par file asks
+ * for (acoustic medium, Graves Qs, force sources, dd sources, receiver lines,
  * slices, snapshots, ablexp, several MPI ranks) stops the run with a message,
  * the reference's fatal-error convention.
  */
@@ -53,7 +54,11 @@ drv_rk_curv_col_allstep(
       fprintf(stderr, "ERROR: the GPU time loop runs one block (1 MPI rank)\n");
       MPI_Abort(MPI_COMM_WORLD, 1);
     }
-  if (md->medium_type != CONST_MEDIUM_ELASTIC_ISO || src->force_actived ||
+  int mt = md->medium_type;
+  int visco = mt == CONST_MEDIUM_VISCOELASTIC_ISO;
+  if ((mt != CONST_MEDIUM_ELASTIC_ISO && mt != CONST_MEDIUM_ELASTIC_VTI &&
+       mt != CONST_MEDIUM_ELASTIC_ANISO && !visco) ||
+      (visco && md->visco_type != CONST_VISCO_GMB) || src->force_actived ||
       src->dd_is_valid || ioline->num_of_lines > 0 || bdry->is_enable_ablexp ||
       ioslice->num_of_slice_x + ioslice->num_of_slice_y + ioslice->num_of_slice_z > 0 ||
       iosnap->num_of_snap > 0 || output_all) {
@@ -64,7 +69,8 @@ drv_rk_curv_col_allstep(
   cgfdh_desc d;
   memset(&d, 0, sizeof d);
   d.ni = gd->ni; d.nj = gd->nj; d.nk = gd->nk;
-  d.medium_type = CGFDH_MEDIUM_ELASTIC_ISO;
+  d.medium_type = mt;              /* the engine's codes are the reference's (constants.h) */
+  d.nmaxwell = visco ? md->nmaxwell : 0;
   d.free_top = bdry->is_sides_free[2][1];
   for (int f = 0; f < 6; f++) {
     d.pml_on[f] = bdry->is_sides_pml[f / 2][f % 2];
@@ -75,8 +81,32 @@ drv_rk_curv_col_allstep(
 
   cgfdh_solver *s = NULL;
   CHK(cgfdh_create(&d, &s));
-  CHK(cgfdh_set_metric(s, metric->v4d));
-  CHK(cgfdh_set_medium_iso(s, md->lambda, md->mu, md->rho));  /* rho already 1/rho */
+  size_t vol = (size_t)gd->nx * gd->ny * gd->nz;
+  if (visco && d.free_top)         /* device metric build also forms matD from the coords */
+    CHK(cgfdh_compute_metric(s, gd->x3d));
+  CHK(cgfdh_set_metric(s, metric->v4d));   /* the reference's own metric */
+  /* md->rho holds 1/rho here (md_rho_to_slow, main_curv_col_el_3d.c:843) */
+  if (mt == CONST_MEDIUM_ELASTIC_ISO) {
+    CHK(cgfdh_set_medium_iso(s, md->lambda, md->mu, md->rho));
+  } else if (mt == CONST_MEDIUM_ELASTIC_VTI) {
+    CHK(cgfdh_set_medium_vti(s, md->c11, md->c13, md->c33, md->c55, md->c66, md->rho));
+  } else if (mt == CONST_MEDIUM_ELASTIC_ANISO) {
+    const float *c[21] = {md->c11, md->c12, md->c13, md->c14, md->c15, md->c16, md->c22,
+                          md->c23, md->c24, md->c25, md->c26, md->c33, md->c34, md->c35,
+                          md->c36, md->c44, md->c45, md->c46, md->c55, md->c56, md->c66};
+    float *cij = malloc(21 * vol * sizeof(float));
+    for (int n = 0; n < 21; n++) memcpy(cij + n * vol, c[n], vol * sizeof(float));
+    CHK(cgfdh_set_medium_aniso(s, cij, md->rho));
+    free(cij);
+  } else {
+    float *y = malloc(2 * (size_t)md->nmaxwell * vol * sizeof(float));
+    for (int n = 0; n < md->nmaxwell; n++) {
+      memcpy(y + (2 * n) * vol, md->Ylam[n], vol * sizeof(float));
+      memcpy(y + (2 * n + 1) * vol, md->Ymu[n], vol * sizeof(float));
+    }
+    CHK(cgfdh_set_medium_visco(s, md->lambda, md->mu, md->rho, y, md->wl));
+    free(y);
+  }
   for (int f = 0; f < 6; f++)
     if (d.pml_on[f])
       CHK(cgfdh_set_pml_face(s, f, bdry->A[f / 2][f % 2], bdry->B[f / 2][f % 2],
@@ -126,11 +156,12 @@ drv_rk_curv_col_allstep(
   CHK(cgfdh_run(s, 0, nt_total));
 
   if (nr > 0) {   /* iorecv->recvone[r].seismo[icmp*max_nt + it] */
-    float *seis = malloc((size_t)nr * 9 * nt_total * sizeof(float));
+    int nc = wav->ncmp;            /* 9, or 9 + 6 nmaxwell (wav_t.c:26-33) */
+    float *seis = malloc((size_t)nr * nc * nt_total * sizeof(float));
     CHK(cgfdh_get_seismograms(s, seis));
     for (int r = 0; r < nr; r++)
-      memcpy(iorecv->recvone[r].seismo, seis + (size_t)r * 9 * nt_total,
-             (size_t)9 * nt_total * sizeof(float));
+      memcpy(iorecv->recvone[r].seismo, seis + (size_t)r * nc * nt_total,
+             (size_t)nc * nt_total * sizeof(float));
     free(seis);
   }
   CHK(cgfdh_destroy(s));

@@ -5,7 +5,11 @@ velocity seismograms at the surface receivers of one fixed grid and medium.
 Network: a 4-layer MLP of width 512 (ReLU) and a 1-D transposed-convolution
 decoder: a dense layer to a 16-sample sequence of 256 channels, then three
 transposed convolutions with kernel = stride = 4 (16 -> 64 -> 256 -> 1024
-samples; 256 -> 128 -> 128 -> 3*nrecv channels).  In the channels-last layout
+samples; 256 -> 128 -> 128 -> 3*nrecv channels).  Optionally (mt_terms = 6)
+the last layer emits 6 basis waveform sets and a fused kernel contracts them
+with the source's moment tensor: the seismograms are linear in it, so the
+network only has to learn the nonlinear dependence on position, depth and
+frequency.  In the channels-last layout
 [sample, time, channel] every layer is one GEMM whose output rows are reread
 as the next layer's input rows, so the whole forward and backward pass runs
 on the BF16 matrix-core GEMM of libcgfdh (cgfds_gemm_bf16): BF16 weights and
@@ -27,14 +31,16 @@ EPI_NONE, EPI_RELU, EPI_MASK = 0, 1, 2
 IN_FEATURES = 32        # input row width (7 parameters and derived features, zero padded)
 
 
-def default_layers(nrecv=64, nt=1024):
+def default_layers(nrecv=64, nt=1024, mt_terms=0):
     """(K_in, N_out, relu) per layer.  The decoder upsamples by 4 per
-    transposed convolution from nt/64 samples."""
+    transposed convolution from nt/64 samples.  With mt_terms > 0 the last
+    layer emits that many basis waveform sets (one per moment-tensor
+    component), contracted with the source's moment tensor."""
     t0 = nt // 64
     assert t0 * 64 == nt
     return [(IN_FEATURES, 512, 1), (512, 512, 1), (512, 512, 1), (512, 512, 1),
             (512, t0 * 256, 1),
-            (256, 4 * 128, 1), (128, 4 * 128, 1), (128, 4 * 3 * nrecv, 0)]
+            (256, 4 * 128, 1), (128, 4 * 128, 1), (128, 4 * 3 * nrecv * max(1, mt_terms), 0)]
 
 
 def layer_rows(layers, nsamp):
@@ -89,6 +95,16 @@ def adam(w, m, v, g, w16, lr, t, b1=0.9, b2=0.999, eps=1e-8):
           "adam")
 
 
+def mt_contract(nsamp, nt, nc, nk, m, g, y):
+    check(lib().cgfds_mt_contract(nsamp, nt, nc, nk, _ptr(m), _ptr(g), _ptr(y), _stream()),
+          "mt_contract")
+
+
+def mt_loss_grad(nsamp, nt, nc, nk, m, g, T, tn2, eps, dg16, loss):
+    check(lib().cgfds_mt_loss_grad(nsamp, nt, nc, nk, _ptr(m), _ptr(g), _ptr(T), _ptr(tn2), eps,
+                                   _ptr(dg16), _ptr(loss), _stream()), "mt_loss_grad")
+
+
 def to_bf16(src, dst):
     check(lib().cgfds_f32_to_bf16(src.numel(), _ptr(src), _ptr(dst), _stream()), "f32_to_bf16")
 
@@ -104,11 +120,18 @@ def _splitk(M, N, K):
 class SurrogateNet:
     """Parameters, Adam state and activation buffers of one network on cuda:0."""
 
-    def __init__(self, layers=None, seed=0, init_params=None):
+    def __init__(self, layers=None, seed=0, init_params=None, mt_terms=0, out_shape=None):
+        """mt_terms > 0: the last layer's output per sample is [nt, mt_terms,
+        nc] basis waveforms (out_shape = (nt, nc)), contracted with each
+        sample's moment tensor into the [nt, nc] prediction."""
         torch = _torch()
         self.torch = torch
         self.dev = torch.device("cuda:0")
         self.layers = list(layers or default_layers())
+        self.mt_terms = int(mt_terms)
+        self.out_shape = out_shape
+        if self.mt_terms and out_shape is None:
+            raise ValueError("a moment-tensor output needs out_shape = (nt, nc)")
         self.w32, self.w16, self.b32 = [], [], []
         rng = np.random.default_rng(seed)
         for li, (k, n, act) in enumerate(self.layers):
@@ -162,16 +185,37 @@ class SurrogateNet:
         work = torch.empty(wk, dtype=torch.float32, device=self.dev)
         x32 = torch.empty((nsamp, self.layers[0][0]), dtype=torch.float32, device=self.dev)
         loss = torch.empty(nsamp, dtype=torch.float32, device=self.dev)
-        self._bufs[nsamp] = dict(rows=rows, h=h, y=y, g=g, work=work, x32=x32, loss=loss)
+        yc = None
+        if self.mt_terms:
+            nt, nc = self.out_shape
+            if y.numel() != nsamp * nt * nc * self.mt_terms:
+                raise ValueError("the last layer does not match out_shape and mt_terms")
+            yc = torch.empty((nsamp * nt, nc), dtype=torch.float32, device=self.dev)
+        m = torch.zeros((nsamp, max(1, self.mt_terms)), dtype=torch.float32, device=self.dev)
+        self._bufs[nsamp] = dict(rows=rows, h=h, y=y, g=g, work=work, x32=x32, loss=loss,
+                                 yc=yc, m=m)
         return self._bufs[nsamp]
 
-    def forward(self, x, nsamp=None):
-        """x: [nsamp, IN_FEATURES] float32 (host numpy or device tensor).
-        Returns the FP32 output [rows_last, N_last] (a view of an internal
-        buffer, valid until the next call with the same nsamp)."""
+    def _set_m(self, b, m):
+        if not self.mt_terms:
+            return
+        if m is None:
+            raise ValueError("this network needs each sample's moment tensor")
+        if isinstance(m, np.ndarray):
+            b["m"].copy_(self.torch.from_numpy(np.ascontiguousarray(m, np.float32)))
+        else:
+            b["m"].copy_(m)
+
+    def forward(self, x, nsamp=None, m=None):
+        """x: [nsamp, IN_FEATURES] float32 (host numpy or device tensor); m:
+        [nsamp, mt_terms] moment tensors when the network has that output.
+        Returns the FP32 prediction (a view of an internal buffer, valid
+        until the next call with the same nsamp): [rows_last, N_last], or
+        [nsamp * nt, nc] after the moment-tensor contraction."""
         torch = self.torch
         nsamp = nsamp or x.shape[0]
         b = self._buffers(nsamp)
+        self._set_m(b, m)
         if isinstance(x, np.ndarray):
             b["x32"].copy_(torch.from_numpy(np.ascontiguousarray(x, np.float32)))
         else:
@@ -185,6 +229,10 @@ class SurrogateNet:
                  epi=EPI_RELU if act else EPI_NONE,
                  C32=b["y"] if li == last else None, ldc32=n,
                  C16=out16, ldc16=n)
+        if self.mt_terms:
+            nt, nc = self.out_shape
+            mt_contract(nsamp, nt, nc, self.mt_terms, b["m"], b["y"], b["yc"])
+            return b["yc"]
         return b["y"]
 
     def backward(self, nsamp):
@@ -213,22 +261,28 @@ class SurrogateNet:
             adam(self.w32[li], self.mw[li], self.vw[li], self.gw[li], self.w16[li], lr, self.t)
             adam(self.b32[li], self.mb[li], self.vb[li], self.gb[li], None, lr, self.t)
 
-    def train_step(self, x, t, tn2, lr, eps=1e-12):
-        """One step on a batch: x [nsamp, IN_FEATURES], t [nsamp, ...] FP32
-        target with the output's layout, tn2 [nsamp] squared target norms.
-        Returns the device tensor of per-sample losses (before the update)."""
+    def _loss(self, x, t, tn2, eps, m, grad):
         nsamp = x.shape[0]
-        y = self.forward(x, nsamp)
+        y = self.forward(x, nsamp, m)
         b = self._bufs[nsamp]
-        loss_grad(nsamp, y.numel() // nsamp, y, t, tn2, eps, b["g"][-1], b["loss"])
+        g16 = b["g"][-1] if grad else None
+        if self.mt_terms:
+            nt, nc = self.out_shape
+            mt_loss_grad(nsamp, nt, nc, self.mt_terms, b["m"], b["y"], t, tn2, eps, g16, b["loss"])
+        else:
+            loss_grad(nsamp, y.numel() // nsamp, y, t, tn2, eps, g16, b["loss"])
+        return nsamp, b
+
+    def train_step(self, x, t, tn2, lr, eps=1e-12, m=None):
+        """One step on a batch: x [nsamp, IN_FEATURES], t [nsamp, ...] FP32
+        target with the prediction's layout, tn2 [nsamp] squared target
+        norms, m the moment tensors (moment-tensor output only).  Returns the
+        device tensor of per-sample losses (before the update)."""
+        nsamp, b = self._loss(x, t, tn2, eps, m, True)
         self.backward(nsamp)
         self.adam_step(lr)
         return b["loss"]
 
-    def loss(self, x, t, tn2, eps=1e-12):
+    def loss(self, x, t, tn2, eps=1e-12, m=None):
         """Per-sample relative squared misfit without an update (device tensor)."""
-        nsamp = x.shape[0]
-        y = self.forward(x, nsamp)
-        b = self._bufs[nsamp]
-        loss_grad(nsamp, y.numel() // nsamp, y, t, tn2, eps, None, b["loss"])
-        return b["loss"]
+        return self._loss(x, t, tn2, eps, m, False)[1]["loss"]

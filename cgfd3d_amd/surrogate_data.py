@@ -49,6 +49,10 @@ class SurrogateGrid:
                               [pml] * 5 + [0], 7000.0, 3.14, 2.0, True, None, ridx, rw)
         self.x_range = ((3 + MARGIN - 3) * dh, (ni - 1 - MARGIN) * dh)
         self.y_range = ((3 + MARGIN - 3) * dh, (nj - 1 - MARGIN) * dh)
+        # sources stay 3 nodes above the bottom PML
+        self.kmin = 3 + pml + 3
+        self.depth_range = (RANGES["depth"][0],
+                            min(RANGES["depth"][1], (nk - 1 - (self.kmin - 3)) * dh * 0.95))
 
     def sample_sources(self, n, seed):
         """n seeded sources: rows (x, y, depth, strike, dip, rake, f0) with x,
@@ -61,9 +65,9 @@ class SurrogateGrid:
         for s in range(n):
             i = int(rng.integers(3 + MARGIN, 3 + self.ni - MARGIN))
             j = int(rng.integers(3 + MARGIN, 3 + self.nj - MARGIN))
-            want = rng.uniform(*RANGES["depth"])
+            want = rng.uniform(*self.depth_range)
             col = xyz[2, :, j, i]
-            k = int(np.argmin(np.abs((col[ksurf] - col[3:ksurf + 1]) - want))) + 3
+            k = int(np.argmin(np.abs((col[ksurf] - col[self.kmin:ksurf + 1]) - want))) + self.kmin
             params[s] = (xyz[0, k, j, i], xyz[1, k, j, i], col[ksurf] - col[k],
                          rng.uniform(*RANGES["strike"]), rng.uniform(*RANGES["dip"]),
                          rng.uniform(*RANGES["rake"]), rng.uniform(*RANGES["f0"]))
@@ -81,10 +85,26 @@ class SurrogateGrid:
                                 self.dt, 0.0, m6, f0, tpeak, 2.0 * tpeak)
 
 
-def features(params, grid):
+def moment_tensors(params):
+    """[n, 6] moment tensors of the sources' mechanisms, exactly as the FD
+    source tables take them (cgfdh_moment_from_angles), divided by the
+    scalar moment."""
+    L = wk._setup_sigs()
+    out = np.zeros((len(params), 6), np.float32)
+    m6 = np.zeros(6, np.float32)
+    m0 = 3.0e10 * 1.0 * 1.0e6
+    for n, p in enumerate(params):
+        L.cgfdh_moment_from_angles(p[3], p[4], p[5], 3.0e10, 1.0, 1.0e6, wk._p(m6))
+        out[n] = m6 / m0
+    return out
+
+
+def features(params, grid, mechanism=True):
     """Network input rows (float32, 32 wide): the 7 parameters scaled to
     [-1, 1], the unit moment tensor of the mechanism (6 values) and the sines
-    and cosines of strike, dip and rake; zero padded."""
+    and cosines of strike, dip and rake; zero padded.  mechanism=False zeroes
+    every mechanism entry (a moment-tensor network takes the mechanism
+    through its output contraction only)."""
     p = np.asarray(params, np.float64)
     n = p.shape[0]
     x = np.zeros((n, 32), np.float32)
@@ -93,7 +113,7 @@ def features(params, grid):
         return 2.0 * (v - lo) / (hi - lo) - 1.0
     x[:, 0] = sc(p[:, 0], *grid.x_range)
     x[:, 1] = sc(p[:, 1], *grid.y_range)
-    x[:, 2] = sc(p[:, 2], *RANGES["depth"])
+    x[:, 2] = sc(p[:, 2], *grid.depth_range)
     x[:, 3] = sc(p[:, 3], *RANGES["strike"])
     x[:, 4] = sc(p[:, 4], *RANGES["dip"])
     x[:, 5] = sc(p[:, 5], *RANGES["rake"])
@@ -109,6 +129,9 @@ def features(params, grid):
     x[:, 7:13] = np.stack([mxx, myy, mzz, myz, mxz, mxy], axis=1)
     x[:, 13:19] = np.stack([np.sin(s), np.cos(s), np.sin(d), np.cos(d), np.sin(r), np.cos(r)],
                            axis=1)
+    if not mechanism:
+        x[:, 3:6] = 0.0
+        x[:, 7:19] = 0.0
     return x
 
 

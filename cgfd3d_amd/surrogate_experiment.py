@@ -23,13 +23,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def train(net, x, y, tn2, epochs, batch, lr_max, seed, time_budget=None, log_every=None):
+def train(net, x, y, tn2, epochs, batch, lr_max, seed, time_budget=None, log_every=None, m=None):
     """Minibatch Adam with a cosine learning-rate decay; y and tn2 are device
-    tensors, x a host array.  Returns the per-epoch mean training loss."""
+    tensors, x (and m, the moment tensors of a moment-tensor network) host
+    arrays.  Returns the per-epoch mean training loss."""
     import torch
     n = x.shape[0]
     rng = np.random.default_rng(seed)
     xd = torch.from_numpy(np.ascontiguousarray(x)).to("cuda:0")
+    md = None if m is None else torch.from_numpy(np.ascontiguousarray(m, np.float32)).to("cuda:0")
     steps_per_epoch = n // batch
     total = epochs * steps_per_epoch
     hist = []
@@ -41,7 +43,8 @@ def train(net, x, y, tn2, epochs, batch, lr_max, seed, time_budget=None, log_eve
         for s in range(steps_per_epoch):
             idx = perm[s * batch:(s + 1) * batch]
             lr = lr_max * 0.5 * (1.0 + math.cos(math.pi * step / max(1, total)))
-            loss = net.train_step(xd[idx], y[idx], tn2[idx], lr)
+            loss = net.train_step(xd[idx], y[idx], tn2[idx], lr,
+                                  m=None if md is None else md[idx])
             acc += loss.sum()
             step += 1
         hist.append(float(acc) / (steps_per_epoch * batch))
@@ -53,33 +56,39 @@ def train(net, x, y, tn2, epochs, batch, lr_max, seed, time_budget=None, log_eve
     return hist
 
 
-def predict(net, x, nsamp_per_call=64):
-    """Host predictions [n, nt, channels] of the network for feature rows x."""
+def predict(net, x, m=None, nsamp_per_call=64):
+    """Host predictions [n, nt * channels] of the network for feature rows x
+    (and moment tensors m of a moment-tensor network)."""
     outs = []
     n = x.shape[0]
     for c0 in range(0, n, nsamp_per_call):
         xb = x[c0:c0 + nsamp_per_call]
-        m = xb.shape[0]
-        pad = (-m) % 8
+        mb = None if m is None else m[c0:c0 + nsamp_per_call]
+        cnt = xb.shape[0]
+        pad = (-cnt) % 8
         if pad:
             xb = np.concatenate([xb, np.zeros((pad, xb.shape[1]), xb.dtype)])
-        y = net.forward(xb).float().cpu().numpy().reshape(m + pad, -1)[:m]
+            if mb is not None:
+                mb = np.concatenate([mb, np.zeros((pad, mb.shape[1]), mb.dtype)])
+        y = net.forward(xb, m=mb).float().cpu().numpy().reshape(cnt + pad, -1)[:cnt]
         outs.append(y)
     return np.concatenate(outs)
 
 
-def inference_rate(net, x, nsamp=256, reps=10):
+def inference_rate(net, x, m=None, nsamp=256, reps=10):
     """Surrogate seismogram sets per second: forward passes of nsamp sources
     timed with HIP events on the launch stream."""
     import torch
     xb = np.resize(x, (nsamp, x.shape[1])).astype(np.float32)
     xd = torch.from_numpy(xb).to("cuda:0")
-    net.forward(xd)
+    md = None if m is None else torch.from_numpy(
+        np.resize(m, (nsamp, m.shape[1])).astype(np.float32)).to("cuda:0")
+    net.forward(xd, m=md)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(reps):
-        net.forward(xd)
+        net.forward(xd, m=md)
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
@@ -104,6 +113,8 @@ def main():
     ap.add_argument("--train-seconds", type=float, default=240.0)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--save", type=str, default="")
+    ap.add_argument("--no-mt", action="store_true",
+                    help="plain output layer (no moment-tensor contraction)")
     ap.add_argument("--small", action="store_true",
                     help="a 64x64x48 grid with 256 steps (pipeline check)")
     args = ap.parse_args()
@@ -132,28 +143,35 @@ def main():
     log(f"{args.test} held-out runs in {time.perf_counter() - t:.1f} s")
     fd_rate = fd_runs / fd_dev_s
 
-    xtr, xte = sd.features(ptr, grid), sd.features(pte, grid)
+    mt = 0 if args.no_mt else 6
+    xtr, xte = sd.features(ptr, grid, not mt), sd.features(pte, grid, not mt)
+    mtr = sd.moment_tensors(ptr) if mt else None
+    mte = sd.moment_tensors(pte) if mt else None
     ytr, yte = sd.to_target_layout(seis_tr), sd.to_target_layout(seis_te)
     scale = 1.0 / float(np.sqrt((ytr.astype(np.float64) ** 2).mean()))
-    layers = sg.default_layers(grid.nrecv, grid.nt)
-    net = sg.SurrogateNet(layers, seed=args.seed)
+    layers = sg.default_layers(grid.nrecv, grid.nt, mt)
+    net = sg.SurrogateNet(layers, seed=args.seed, mt_terms=mt,
+                          out_shape=(grid.nt, 3 * grid.nrecv))
     log(f"network: {net.n_params() / 1e6:.2f} M parameters")
     yd = torch.from_numpy((ytr * scale).astype(np.float32)).to("cuda:0")
     tn2 = torch.from_numpy(((ytr * scale).reshape(len(ytr), -1).astype(np.float64) ** 2)
                            .sum(axis=1).astype(np.float32)).to("cuda:0")
 
-    untrained = sd.relative_misfit(predict(net, xte) / scale, yte.reshape(len(yte), -1))
+    untrained = sd.relative_misfit(predict(net, xte, mte) / scale, yte.reshape(len(yte), -1))
     t = time.perf_counter()
     hist = train(net, xtr, yd, tn2, args.epochs, args.batch, args.lr, args.seed,
-                 time_budget=args.train_seconds, log_every=25)
+                 time_budget=args.train_seconds, log_every=25, m=mtr)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t
-    pred = predict(net, xte) / scale
+    pred = predict(net, xte, mte) / scale
     mis = sd.relative_misfit(pred, yte.reshape(len(yte), -1))
-    mis_tr = sd.relative_misfit(predict(net, xtr[:128]) / scale, ytr[:128].reshape(128, -1))
-    nn = sd.relative_misfit(nearest_neighbour(xtr, ytr.reshape(len(ytr), -1), xte),
+    ntr = min(128, len(xtr))
+    mis_tr = sd.relative_misfit(predict(net, xtr[:ntr], None if mtr is None else mtr[:ntr]) / scale,
+                                ytr[:ntr].reshape(ntr, -1))
+    nn = sd.relative_misfit(nearest_neighbour(sd.features(ptr, grid), ytr.reshape(len(ytr), -1),
+                                              sd.features(pte, grid)),
                             yte.reshape(len(yte), -1))
-    rate, ms = inference_rate(net, xte)
+    rate, ms = inference_rate(net, xte, mte)
     out = {
         "metric": "surrogate vs FD seismogram sets/s",
         "surrogate_sets_per_s": rate, "surrogate_ms_per_256": ms,
@@ -167,7 +185,8 @@ def main():
         "untrained_misfit_median": float(np.median(untrained)),
         "epochs_run": len(hist), "train_seconds": train_s,
         "final_train_loss": hist[-1] if hist else None,
-        "params": net.n_params(), "dtype": "bf16 weights/activations, fp32 accumulation",
+        "params": net.n_params(), "moment_tensor_output": bool(mt),
+        "dtype": "bf16 weights/activations, fp32 accumulation",
     }
     if args.save:
         os.makedirs(os.path.dirname(os.path.abspath(args.save)), exist_ok=True)

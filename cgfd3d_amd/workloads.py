@@ -12,7 +12,11 @@ import math
 import numpy as np
 
 from ._lib import check, lib
-from .solver import SRC_POINT, Solver
+from .solver import (MEDIUM_ELASTIC_ANISO, MEDIUM_ELASTIC_ISO, MEDIUM_ELASTIC_VTI,
+                     MEDIUM_VISCO_ISO, SRC_POINT, Solver)
+
+# medium coefficient volumes per point (besides the 9 metric volumes)
+_NCOEF = {MEDIUM_ELASTIC_ISO: 3, MEDIUM_ELASTIC_VTI: 6, MEDIUM_ELASTIC_ANISO: 22}
 
 _P = C.c_void_p
 
@@ -42,6 +46,9 @@ def _setup_sigs():
     L.cgfdh_ricker_moment_table.restype = C.c_int
     L.cgfdh_gauss_weights.argtypes = [_P] + [C.c_float] * 4 + [C.c_int, C.c_int]
     L.cgfdh_gauss_weights.restype = C.c_int
+    L.cgfdh_visco_gmb.argtypes = [C.c_long, C.c_int, C.c_float, C.c_float, C.c_float,
+                                  _P, _P, _P, _P, _P, _P]
+    L.cgfdh_visco_gmb.restype = C.c_int
     return L
 
 
@@ -115,9 +122,9 @@ def gaussian_random_field(shape, dh, corr_len, std, seed):
     return f * (std / f.std())
 
 
-def layered_medium(z3d, layers, grf):
-    """lambda, mu, slowness from a layer table [(z_top_of_layer, vp, vs, rho)]
-    ordered from the top down, scaled by (1 + grf)."""
+def layered_velocity(z3d, layers, grf):
+    """vp, vs, rho from a layer table [(z_top_of_layer, vp, vs, rho)] ordered
+    from the top down, scaled by (1 + grf)."""
     vp = np.empty(z3d.shape, np.float32)
     vs = np.empty_like(vp)
     rho = np.empty_like(vp)
@@ -128,6 +135,12 @@ def layered_medium(z3d, layers, grf):
     if grf is not None:
         s = 1.0 + grf
         vp *= s; vs *= s; rho *= s
+    return vp, vs, rho
+
+
+def layered_medium(z3d, layers, grf):
+    """lambda, mu, slowness (and the largest vp) of the layered model."""
+    vp, vs, rho = layered_velocity(z3d, layers, grf)
     mu = (rho * vs * vs).astype(np.float32)
     lam = (rho * vp * vp - 2.0 * mu).astype(np.float32)
     slw = np.where(rho > 1e-10, 1.0 / rho.astype(np.float64), 0.0).astype(np.float32)
@@ -159,8 +172,12 @@ class Workload:
     """Everything one run of the engine needs, as host arrays."""
 
     def __init__(self, name, ni, nj, nk, dt, nt_total, xyz, lam, mu, slw, pml_layers,
-                 pml_vel, pml_alpha, pml_beta, free_top, src, recv_idx, recv_w):
+                 pml_vel, pml_alpha, pml_beta, free_top, src, recv_idx, recv_w,
+                 medium_type=MEDIUM_ELASTIC_ISO, med_vols=None, wl=None, nmaxwell=0):
+        """lam, mu, slw: the isotropic medium; other media pass medium_type
+        and med_vols (the volumes in cgfdh_set_medium_* order; visco with wl)."""
         self.name = name
+        self.medium_type, self.med_vols, self.wl, self.nmaxwell = medium_type, med_vols, wl, nmaxwell
         self.ni, self.nj, self.nk = ni, nj, nk
         self.dt, self.nt_total = dt, nt_total
         self.xyz, self.lam, self.mu, self.slw = xyz, lam, mu, slw
@@ -185,19 +202,38 @@ class Workload:
             n += dims[0] * dims[1] * dims[2]
         return n
 
+    def bytes_per_point(self):
+        """Minimum FP32 traffic per point and RK4 step: 16 wavefield
+        components' worth per component (stage 1 reads pre and writes tmp,
+        end; stages 2-3 read tmp, pre, end and write tmp, end; stage 4 reads
+        tmp, end and writes end) plus the 9 metric and the medium volumes
+        once per stage: 4 (16 nw + 4 ncoef) bytes.  Iso 768, VTI 816, aniso
+        1072, visco with 3 Maxwell bodies 2016."""
+        nw = 9 + 6 * self.nmaxwell
+        ncoef = 9 + (3 + 2 * self.nmaxwell if self.medium_type == MEDIUM_VISCO_ISO
+                     else _NCOEF[self.medium_type])
+        return 4.0 * (16 * nw + 4 * ncoef)
+
     def algorithmic_bytes_per_step(self):
-        """Minimum HBM traffic of one RK4 step of the isotropic path (FP32):
-        768 B per point, 576 B per PML aux point, 352 B per surface column."""
+        """Minimum HBM traffic of one RK4 step (FP32): bytes_per_point per
+        point, 576 B per PML aux point (9 components), 352 B per free-surface
+        column (18 matrix floats and 4 jacobian values per stage; +36 B of
+        the visco surface rotation)."""
         nsurf = self.ni * self.nj if self.free_top else 0
-        return 768.0 * self.npoints + 576.0 * self.n_aux() + 352.0 * nsurf
+        surf = 352.0 + (36.0 if self.medium_type == MEDIUM_VISCO_ISO else 0.0)
+        return self.bytes_per_point() * self.npoints + 576.0 * self.n_aux() + surf * nsurf
 
     def build_solver(self, variant=4):
         L = _setup_sigs()
         s = Solver(self.ni, self.nj, self.nk, self.dt, self.nt_total, free_top=self.free_top,
-                   pml_on=self.pml_on, pml_layers=self.pml_layers)
+                   pml_on=self.pml_on, pml_layers=self.pml_layers,
+                   medium_type=self.medium_type, nmaxwell=self.nmaxwell)
         s.set_kernel_variant(variant)
         check(L.cgfdh_compute_metric(s.h, _p(self.xyz)), "compute_metric")
-        s.set_medium_iso(self.lam, self.mu, self.slw)
+        if self.medium_type == MEDIUM_ELASTIC_ISO:
+            s.set_medium_iso(self.lam, self.mu, self.slw)
+        else:
+            s.set_medium(self.medium_type, self.med_vols, self.wl)
         for f in range(6):
             n = self.pml_layers[f]
             if n <= 0:
@@ -259,3 +295,161 @@ def iso_topo_grf(ni=600, nj=600, nk=300, dh=50.0, nt_total=2001, dt=0.004, seed=
     ridx, rw = surface_receiver_grid(ni, nj, nk, n_side)
     return Workload(f"iso{ni}x{nj}x{nk}", ni, nj, nk, dt, nt_total, xyz, lam, mu, slw,
                     [pml, pml, pml, pml, pml, 0], 7000.0, 3.14, 2.0, True, src, ridx, rw)
+
+
+def _layer_index(z3d, layers):
+    """Layer number (0 = top) of every point for a layer table ordered from the top."""
+    idx = np.zeros(z3d.shape, np.int8)
+    for n, (ztop, *_r) in enumerate(layers[1:], start=1):
+        idx[z3d < ztop] = n
+    return idx
+
+
+def _bond_matrix(R):
+    """6x6 Bond matrix M of a rotation R (Voigt order 11,22,33,23,13,12) with
+    C' = M C M^T for the stiffness in Voigt notation."""
+    M = np.zeros((6, 6))
+    pairs = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]
+    for a, (i, j) in enumerate(pairs):
+        for b, (k, l) in enumerate(pairs):
+            if b < 3:
+                M[a, b] = R[i, k] * R[j, l]
+            else:
+                M[a, b] = R[i, k] * R[j, l] + R[i, l] * R[j, k]
+    return M
+
+
+def _random_rotation(rng):
+    """Uniformly distributed rotation matrix (QR of a Gaussian matrix)."""
+    q, r = np.linalg.qr(rng.standard_normal((3, 3)))
+    q *= np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def media_topo_grf(kind, ni=400, nj=400, nk=200, dh=50.0, nt_total=1501, dt=0.004, seed=2025,
+                   nhills=20, pml=20, n_side=16, nmaxwell=3, qp=100.0, qs=50.0,
+                   fmin=0.5, fmax=5.0, fref=1.0):
+    """The anisotropy / attenuation workloads on the topography + layered GRF
+    model of iso_topo_grf: kind 'vti' (Thomsen eps, delta, gamma seeded per
+    layer in [0, 0.2]), 'aniso' (the VTI stiffness under one seeded random
+    rotation: 21 Cij) or 'visco' (GMB with nmaxwell bodies, constant Qp, Qs
+    over fmin..fmax Hz)."""
+    top = topography(ni, nj, dh, nhills, seed)
+    depth = (nk - 1) * dh
+    xyz = layered_coords(ni, nj, nk, dh, top, depth)
+    grf = gaussian_random_field((nk + 6, nj + 6, ni + 6), dh, 500.0, 0.05, seed + 1)
+    layers = [(1e9, 4500.0, 2600.0, 2400.0), (-1500.0, 6000.0, 3464.0, 2670.0),
+              (-6000.0, 6800.0, 3900.0, 2900.0)]
+    vp, vs, rho = layered_velocity(xyz[2], layers, grf)
+    lid = _layer_index(xyz[2], layers)
+    del grf
+    rng = np.random.default_rng(seed + 5)
+    slw = np.where(rho > 1e-10, 1.0 / rho.astype(np.float64), 0.0).astype(np.float32)
+    wl = None
+    if kind in ("vti", "aniso"):
+        th = rng.uniform(0.0, 0.2, (len(layers), 3))        # eps, delta, gamma per layer
+        e, d, g = th[lid, 0], th[lid, 1], th[lid, 2]
+        c33 = rho.astype(np.float64) * vp * vp
+        c55 = rho.astype(np.float64) * vs * vs
+        c11 = c33 * (1 + 2 * e)
+        c66 = c55 * (1 + 2 * g)
+        c13 = np.sqrt((c33 - c55) * (2 * d * c33 + c33 - c55)) - c55
+        del e, d, g
+        if kind == "vti":
+            vols = [c.astype(np.float32) for c in (c11, c13, c33, c55, c66)] + [slw]
+            mtype = MEDIUM_ELASTIC_VTI
+        else:
+            M = _bond_matrix(_random_rotation(rng))
+            n = c11.size
+            cij = np.empty((21, n), np.float32)
+            ut = [(a, b) for a in range(6) for b in range(a, 6)]
+            chunk = 1 << 20
+            f = [c.reshape(-1) for c in (c11, c13, c33, c55, c66)]
+            for s0 in range(0, n, chunk):
+                sl = slice(s0, min(n, s0 + chunk))
+                m = sl.stop - sl.start
+                C = np.zeros((m, 6, 6))
+                C[:, 0, 0] = C[:, 1, 1] = f[0][sl]
+                C[:, 0, 1] = C[:, 1, 0] = f[0][sl] - 2.0 * f[4][sl]
+                C[:, 0, 2] = C[:, 2, 0] = C[:, 1, 2] = C[:, 2, 1] = f[1][sl]
+                C[:, 2, 2] = f[2][sl]
+                C[:, 3, 3] = C[:, 4, 4] = f[3][sl]
+                C[:, 5, 5] = f[4][sl]
+                Cr = M @ C @ M.T
+                for q, (a, b) in enumerate(ut):
+                    cij[q, sl] = Cr[:, a, b]
+            vols = [cij.reshape((21,) + c11.shape)[q] for q in range(21)] + [slw]
+            mtype = MEDIUM_ELASTIC_ANISO
+        del c11, c13, c33, c55, c66
+    elif kind == "visco":
+        mu = (rho * vs * vs).astype(np.float32).reshape(-1)
+        lam = (rho * vp * vp - 2.0 * mu.reshape(rho.shape)).astype(np.float32).reshape(-1)
+        n = lam.size
+        Qp = np.full(n, qp, np.float32)
+        Qs = np.full(n, qs, np.float32)
+        y = np.zeros((2 * nmaxwell, n), np.float32)
+        wl = np.zeros(nmaxwell, np.float32)
+        L = _setup_sigs()
+        check(L.cgfdh_visco_gmb(n, nmaxwell, fref, fmin, fmax, _p(lam), _p(mu), _p(Qp), _p(Qs),
+                                _p(y), _p(wl)), "visco_gmb")
+        shp = rho.shape
+        vols = [lam.reshape(shp), mu.reshape(shp), slw] + [y[q].reshape(shp)
+                                                           for q in range(2 * nmaxwell)]
+        mtype = MEDIUM_VISCO_ISO
+    else:
+        raise ValueError(f"unknown medium kind {kind!r}")
+    vpmax = float(vp.max())
+    del vp, vs, rho
+    strike, dip, rake = rng.uniform(0, 360), rng.uniform(10, 90), rng.uniform(-180, 180)
+    L = _setup_sigs()
+    m6 = np.zeros(6, np.float32)
+    L.cgfdh_moment_from_angles(strike, dip, rake, 3.0e10, 1.0, 1.0e6, _p(m6))
+    si, sj = 3 + ni // 2, 3 + nj // 2
+    sk = 3 + nk - 1 - min(int(round(5000.0 / dh)), int(0.4 * nk))
+    src = moment_source(ni, nj, nk, si, sj, sk, dt, 0.0, m6, 2.0, 0.5, 2.0)
+    ridx, rw = surface_receiver_grid(ni, nj, nk, n_side)
+    wlo = Workload(f"{kind}{ni}x{nj}x{nk}", ni, nj, nk, dt, nt_total, xyz, None, None, None,
+                   [pml, pml, pml, pml, pml, 0], 7000.0, 3.14, 2.0, True, src, ridx, rw,
+                   medium_type=mtype, med_vols=vols, wl=wl,
+                   nmaxwell=nmaxwell if mtype == MEDIUM_VISCO_ISO else 0)
+    _ = vpmax
+    return wlo
+
+
+def batch_workloads(seed=4242, dh=50.0, dt=0.004, nt_total=1001, pml=20, n_side=4,
+                    sizes=((24, (128, 128, 128)), (24, (192, 192, 128)), (16, (256, 256, 160)))):
+    """Yields the independent runs of the batched configuration one at a time
+    (64 by default: 24 at 128^3, 24 at 192x192x128, 16 at 256x256x160), each
+    with its own seeded topography, GRF medium, source position, mechanism
+    and dominant frequency in 1-3 Hz."""
+    rng = np.random.default_rng(seed)
+    n = 0
+    for count, (ni, nj, nk) in sizes:
+        for _ in range(count):
+            s = int(rng.integers(1 << 30))
+            r = np.random.default_rng(s)
+            top = topography(ni, nj, dh, int(r.integers(3, 9)), s, amp_max=600.0,
+                             sig_min=600.0, sig_max=2500.0)
+            depth = (nk - 1) * dh
+            xyz = layered_coords(ni, nj, nk, dh, top, depth)
+            grf = gaussian_random_field((nk + 6, nj + 6, ni + 6), dh, 500.0, 0.05, s + 1)
+            layers = [(1e9, 4500.0, 2600.0, 2400.0), (-1500.0, 6000.0, 3464.0, 2670.0),
+                      (-4000.0, 6800.0, 3900.0, 2900.0)]
+            lam, mu, slw, _ = layered_medium(xyz[2], layers, grf)
+            del grf
+            strike, dip, rake = r.uniform(0, 360), r.uniform(10, 90), r.uniform(-180, 180)
+            fc = r.uniform(1.0, 3.0)
+            L = _setup_sigs()
+            m6 = np.zeros(6, np.float32)
+            L.cgfdh_moment_from_angles(strike, dip, rake, 3.0e10, 1.0, 1.0e6, _p(m6))
+            si = 3 + int(r.integers(pml + 5, ni - pml - 5))
+            sj = 3 + int(r.integers(pml + 5, nj - pml - 5))
+            sk = 3 + int(r.integers(pml + 5, nk - 10))
+            tpeak = 1.2 / fc
+            src = moment_source(ni, nj, nk, si, sj, sk, dt, 0.0, m6, fc, tpeak, 2.0 * tpeak)
+            ridx, rw = surface_receiver_grid(ni, nj, nk, n_side)
+            yield Workload(f"batch{n}_{ni}x{nj}x{nk}", ni, nj, nk, dt, nt_total, xyz, lam, mu,
+                           slw, [pml] * 5 + [0], 7000.0, 3.14, 2.0, True, src, ridx, rw)
+            n += 1

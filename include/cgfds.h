@@ -56,6 +56,19 @@ int cgfds_adam(long n, float *w, float *m, float *v, const float *g, void *w16, 
                float b1, float b2, float c1, float c2, float bc1, float bc2, float eps,
                void *stream);
 
+/* Moment-tensor contraction of basis waveforms (the seismograms are linear
+ * in the moment tensor): y[b][t][c] = sum_k m[b][k] g[b][t][k][c] for
+ * nsamp samples, nt times, nc channels, nk <= 8 basis terms. */
+int cgfds_mt_contract(int nsamp, int nt, int nc, int nk, const float *m, const float *g,
+                      float *y, void *stream);
+
+/* The contraction fused with cgfds_loss_grad's per-sample misfit against T
+ * [nsamp][nt][nc]; dg16 (optional) receives the bf16 gradient with respect
+ * to g: m[b][k] times the misfit gradient with respect to y. */
+int cgfds_mt_loss_grad(int nsamp, int nt, int nc, int nk, const float *m, const float *g,
+                       const float *T, const float *tnorm2, float eps, void *dg16, float *loss,
+                       void *stream);
+
 /* dst[i] = bf16(src[i]) */
 int cgfds_f32_to_bf16(long n, const float *src, void *dst, void *stream);
 

@@ -123,3 +123,31 @@ def adam(w, m, v, g, t, lr, b1=0.9, b2=0.999, eps=1e-8):
     vh = v / bc2
     w = w - f(lr) * (mh / (np.sqrt(vh) + f(eps)))
     return w.astype(np.float32), m.astype(np.float32), v.astype(np.float32)
+
+
+def mt_contract(m, g, nt, nc):
+    """y[b][t][c] = sum_k m[b][k] g[b][t][k][c] in float32, k summed in
+    increasing order (the device's order)."""
+    m = np.asarray(m, np.float32)
+    ns, nk = m.shape
+    g = np.asarray(g, np.float32).reshape(ns, nt, nk, nc)
+    y = m[:, None, 0, None] * g[:, :, 0, :]
+    for k in range(1, nk):
+        y = y + m[:, None, k, None] * g[:, :, k, :]
+    return y
+
+
+def mt_loss_grad(m, g, t, tn2, eps, nt, nc):
+    """Contraction, per-sample relative misfit and the bf16 gradient with
+    respect to the basis waveforms g ([ns, nt, nk, nc])."""
+    m = np.asarray(m, np.float32)
+    ns, nk = m.shape
+    f = np.float32
+    y = mt_contract(m, g, nt, nc)
+    d = y - np.asarray(t, np.float32).reshape(ns, nt, nc)
+    den = np.asarray(tn2, f) + f(eps)
+    loss = (d.astype(np.float64) ** 2).sum(axis=(1, 2)) / den
+    scale = f(2.0) / (f(ns) * den)
+    dy = d * scale[:, None, None]
+    dg = bf16_round(m[:, None, :, None] * dy[:, :, None, :])
+    return loss, dg

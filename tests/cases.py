@@ -402,3 +402,29 @@ def rel_l2(a, b):
     b = np.asarray(b, np.float64).ravel()
     nb = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def workload_case(wl):
+    """The oracle case of a whole Workload of cgfd3d_amd.workloads (metric from
+    the oracle's own restatement, PML tables, medium volumes, point source)."""
+    ni, nj, nk = wl.ni, wl.nj, wl.nk
+    orc = pyoracle.load_oracle()
+    nx, ny, nz = dims(ni, nj, nk)
+    xyz = np.ascontiguousarray(wl.xyz.reshape(3, nz, ny, nx), np.float32)
+    metric = np.zeros((10, nz, ny, nx), np.float32)
+    orc.orc_metric(ni, nj, nk, pyoracle.ptr(xyz), pyoracle.ptr(metric))
+    A, B, D, region = pml_tables(orc, ni, nj, nk, xyz, wl.pml_on, wl.pml_layers,
+                                 vel=wl.pml_vel, alpha=wl.pml_alpha, beta=wl.pml_beta)
+    vols = [wl.lam, wl.mu, wl.slw] if wl.med_vols is None else wl.med_vols
+    med = np.ascontiguousarray(np.stack([np.asarray(v).reshape(nz, ny, nx) for v in vols]),
+                               np.float32)
+    case = dict(ni=ni, nj=nj, nk=nk, metric=metric, med=med, medium_type=wl.medium_type,
+                free_top=int(wl.free_top), pml_on=list(wl.pml_on), pml_layers=list(wl.pml_layers),
+                pml_A=A, pml_B=B, pml_D=D, pml_region=region, coords=xyz)
+    if wl.wl is not None:
+        case["visco_wl"] = np.asarray(wl.wl, np.float32)
+    if wl.src is not None:
+        src = dict(wl.src)
+        src.update(nsrc=1, inc=[0, 0, 0], gauss=0)
+        case["src"] = src
+    return case

@@ -189,3 +189,55 @@ def test_training_reduces_the_misfit():
           f"(training loss {hist[0]:.3f} -> {hist[-1]:.4f})")
     # predicting zeros scores exactly 1
     assert after < 0.06 and after < 0.1 * before
+
+
+def test_mt_contract_and_loss_grad_match_oracle():
+    from cgfd3d_amd import surrogate as sg
+    torch = _torch()
+    rng = np.random.default_rng(11)
+    ns, nt, nk, nc = 8, 40, 6, 24
+    m = rng.standard_normal((ns, nk)).astype(np.float32)
+    g = rng.standard_normal((ns, nt, nk, nc)).astype(np.float32)
+    t = rng.standard_normal((ns, nt, nc)).astype(np.float32)
+    tn2 = (t.reshape(ns, -1).astype(np.float64) ** 2).sum(axis=1).astype(np.float32)
+    y = torch.empty((ns, nt, nc), device="cuda:0")
+    sg.mt_contract(ns, nt, nc, nk, _dev(m), _dev(g), y)
+    assert np.array_equal(_host(y), so.mt_contract(m, g, nt, nc))    # same FP32 operations
+    dg = torch.empty((ns, nt, nk, nc), dtype=torch.bfloat16, device="cuda:0")
+    loss = torch.empty(ns, device="cuda:0")
+    sg.mt_loss_grad(ns, nt, nc, nk, _dev(m), _dev(g), _dev(t), _dev(tn2), 1e-12, dg, loss)
+    l_o, dg_o = so.mt_loss_grad(m, g, t, tn2, 1e-12, nt, nc)
+    assert np.array_equal(_host(dg), dg_o)
+    assert np.allclose(_host(loss), l_o, rtol=1e-5)
+
+
+def test_mt_network_forward_backward_match_oracle():
+    from cgfd3d_amd.surrogate import SurrogateNet
+    layers = [(32, 64, 1), (64, 64, 1), (64, 128, 1), (32, 64, 1), (16, 6 * 48, 0)]
+    ns, nt, nc = 16, 64, 12    # last layer rows 16 per sample x (4 taps x 6 terms x 12 channels)
+    net = SurrogateNet(layers, seed=4, mt_terms=6, out_shape=(nt, nc))
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal((ns, 32)).astype(np.float32)
+    m = rng.standard_normal((ns, 6)).astype(np.float32)
+    y = _host(net.forward(x, m=m))
+    params = [(so.bf16_round(w), b) for w, b in net.params_numpy()]
+    hs, g_o = so.forward(params, x, layers)
+    y_o = so.mt_contract(m, g_o, nt, nc)
+    e = rel_l2(y.reshape(ns, nt, nc), y_o)
+    print(f"mt forward rel L2 {e:.1e}")
+    assert e <= 2e-3
+    t = rng.standard_normal((ns, nt, nc)).astype(np.float32)
+    tn2 = (t.reshape(ns, -1).astype(np.float64) ** 2).sum(axis=1).astype(np.float32)
+    _, dg = so.mt_loss_grad(m, g_o, t, tn2, 1e-12, nt, nc)
+    grads = so.backward(params, hs, dg, layers)
+    from cgfd3d_amd import surrogate as sg
+    b = net._bufs[ns]
+    net.forward(x, m=m)
+    sg.mt_loss_grad(ns, nt, nc, 6, b["m"], b["y"], _dev(t), _dev(tn2), 1e-12, b["g"][-1],
+                    b["loss"])
+    net.backward(ns)
+    for li in range(len(layers)):
+        ew = rel_l2(_host(net.gw[li]), grads[li][0])
+        eb = rel_l2(_host(net.gb[li]), grads[li][1])
+        print(f"layer {li}: dW rel L2 {ew:.1e}, db rel L2 {eb:.1e}")
+        assert ew <= 2e-3 and eb <= 2e-3

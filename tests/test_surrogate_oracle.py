@@ -112,3 +112,22 @@ def test_adam_closed_form_first_step():
     w1, m1, v1 = so.adam(w, z, z, g, 1, 1e-3)
     assert np.allclose(w - w1, 1e-3 * np.sign(g), rtol=1e-4, atol=1e-9)
     assert np.allclose(m1, 0.1 * g, rtol=1e-6) and np.allclose(v1, 0.001 * g * g, rtol=1e-5)
+
+
+def test_mt_contraction_is_the_linear_combination():
+    rng = np.random.default_rng(10)
+    ns, nt, nk, nc = 3, 5, 6, 4
+    m = rng.standard_normal((ns, nk)).astype(np.float32)
+    g = rng.standard_normal((ns, nt, nk, nc)).astype(np.float32)
+    y = so.mt_contract(m, g, nt, nc)
+    assert np.allclose(y, np.einsum("bk,btkc->btc", m.astype(np.float64), g), rtol=1e-5,
+                       atol=1e-6)
+    # gradient of the contracted misfit with respect to g: finite differences (float64
+    # through the exact path: d loss / d g = m * d loss / d y)
+    t = rng.standard_normal((ns, nt, nc))
+    tn2 = (t.reshape(ns, -1) ** 2).sum(axis=1)
+    loss, dg = so.mt_loss_grad(m, g, t, tn2, 0.0, nt, nc)
+    _, gy = so.loss_grad(np.einsum("bk,btkc->btc", m.astype(np.float64), g), t, tn2, 0.0,
+                         exact=True)
+    want = m[:, None, :, None] * gy.reshape(ns, nt, 1, nc)
+    assert np.allclose(dg, want, rtol=1e-2, atol=1e-6)   # dg is bf16
