@@ -184,6 +184,8 @@ struct cgfdh_solver {
   long ss, vol;
   int ni1, ni2, nj1, nj2, nk1, nk2;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr; // shell boxes, when they run beside the interior
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   float* met = nullptr;
   int ncmp = 3;                  // medium volumes (medium.h)
   int nw = 9;                    // wavefield components (9 + 6 per Maxwell body)
@@ -307,6 +309,10 @@ extern "C" int cgfdh_create(const cgfdh_desc* d, cgfdh_solver** out)
     for (auto& a : F.aux) bad |= alloc(&a, 9 * F.fn);
   }
   if (!bad && hipStreamCreate(&s->stream) != hipSuccess) bad = 1;
+  if (!bad && (hipStreamCreate(&s->stream2) != hipSuccess ||
+               hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
+               hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming) != hipSuccess))
+    bad = 1;
   if (!bad && (hipEventCreate(&s->ev_run[0]) != hipSuccess ||
                hipEventCreate(&s->ev_run[1]) != hipSuccess))
     bad = 1;
@@ -328,6 +334,10 @@ extern "C" int cgfdh_destroy(cgfdh_solver* s)
   for (auto e : s->ev_stage) (void)hipEventDestroy(e);
   (void)hipEventDestroy(s->ev_run[0]);
   (void)hipEventDestroy(s->ev_run[1]);
+  (void)hipStreamSynchronize(s->stream2);
+  if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
+  if (s->ev_join) (void)hipEventDestroy(s->ev_join);
+  (void)hipStreamDestroy(s->stream2);
   (void)hipStreamDestroy(s->stream);
   delete s;
   return CGFDH_OK;
@@ -650,6 +660,31 @@ static StageParams make_params(cgfdh_solver* s, int it, int istage, const float*
   return P;
 }
 
+// whether the PML slabs of the shell go in one launch: CGFDH_MERGE_SHELL=0/1
+// (a tuning knob), default on
+static int merge_shell()
+{
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("CGFDH_MERGE_SHELL");
+    v = (e && e[0] == '0') ? 0 : 1;
+  }
+  return v;
+}
+
+// whether the shell boxes run on a second stream beside the interior kernel
+// (eager runs only; a captured batch keeps one stream per solver):
+// CGFDH_SHELL_STREAM=0/1 (a tuning knob), default off
+static int shell_stream()
+{
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("CGFDH_SHELL_STREAM");
+    v = (e && e[0] == '1') ? 1 : 0;
+  }
+  return v;
+}
+
 static void launch_stage(cgfdh_solver* s, hipStream_t stream, const StageParams& P, int kind,
                          int ipair, int istage)
 {
@@ -664,9 +699,21 @@ static void launch_stage(cgfdh_solver* s, hipStream_t stream, const StageParams&
   L.free_top = s->d.free_top;
   L.tgx = s->tgx;
   L.tgy = s->tgy;
+  L.merge_shell = merge_shell();
   for (int f = 0; f < 6; f++) {
     L.pml_on[f] = s->d.pml_on[f];
     L.pml_layers[f] = s->d.pml_layers[f];
+  }
+  L.shell_stream = stream;
+  bool two = false;
+  if (shell_stream() && s->variant >= 2) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    two = hipStreamIsCapturing(stream, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone;
+  }
+  if (two) {
+    (void)hipEventRecord(s->ev_fork, stream);
+    (void)hipStreamWaitEvent(s->stream2, s->ev_fork, 0);
+    L.shell_stream = s->stream2;
   }
   if (s->d.medium_type == CGFDH_MEDIUM_ELASTIC_VTI)
     launch_stage_vti(L, P, kind, dx, dy, dz);
@@ -676,6 +723,10 @@ static void launch_stage(cgfdh_solver* s, hipStream_t stream, const StageParams&
     launch_stage_visco(L, P, kind, dx, dy, dz);
   else
     launch_stage_iso(L, P, kind, dx, dy, dz);
+  if (two) {
+    (void)hipEventRecord(s->ev_join, s->stream2);
+    (void)hipStreamWaitEvent(stream, s->ev_join, 0);
+  }
 }
 
 // one RK4 step of solver s on stream st (drv_rk_curv_col.c:167-544): four fused

@@ -60,6 +60,27 @@ __device__ __forceinline__ bool grouped_block(const BoxParams& B, int& bx, int& 
   return bx < B.tiles_x && by < B.tiles_y;
 }
 
+// grouped_block for block bidx of a sub-grid of nblk blocks (nblk a
+// multiple of 8 starting at a multiple of 8, so bidx % 8 is still the XCD)
+__device__ __forceinline__ bool grouped_block_at(const BoxParams& B, int bidx, int nblk, int& bx,
+                                                 int& by, int& k)
+{
+  const int nb8 = nblk / 8;
+  const int b = (bidx % 8) * nb8 + bidx / 8;
+  if (b >= B.nblocks) return false;
+  const int gt = B.gx * B.gy;
+  const int per_group = gt * B.nk;
+  const int g = b / per_group;
+  const int r = b - g * per_group;
+  const int kk = r / gt;
+  const int t = r - kk * gt;
+  const int gyi = g / B.groups_x;
+  bx = (g - gyi * B.groups_x) * B.gx + t % B.gx;
+  by = gyi * B.gy + t / B.gx;
+  k = B.k0 + kk;
+  return bx < B.tiles_x && by < B.tiles_y;
+}
+
 // the grouped grid of a box tiled tw x by
 inline int grouped_grid(BoxParams& B, int by, int tw = 64)
 {
@@ -360,6 +381,76 @@ __global__ void __launch_bounds__(256) stage_box_kernel(StageParams P, BoxParams
     Z[c] = d5<DZ>(v, p, P.ss);
   }
   finish_point<DX, DY, DZ, KIND, MT, FLAGS>(P, i, j, k, p, X, Y, Z);
+}
+
+// several shell boxes of one FLAGS class in one launch: block ranges
+// start[q] .. start[q+1]-1 (multiples of 8) belong to box q
+constexpr int SHELL_MAX = 5;
+struct ShellBoxes {
+  BoxParams b[SHELL_MAX];
+  int start[SHELL_MAX + 1];
+  int n;
+};
+
+template <int DX, int DY, int DZ, int KIND, int MT, int FLAGS>
+__global__ void __launch_bounds__(256) stage_shell_kernel(StageParams P, ShellBoxes S)
+{
+  int q = 0;
+  while (q + 1 < S.n && (int)blockIdx.x >= S.start[q + 1]) ++q;
+  const BoxParams& B = S.b[q];
+  int bx, by, k;
+  if (!grouped_block_at(B, blockIdx.x - S.start[q], S.start[q + 1] - S.start[q], bx, by, k))
+    return;
+  const int rows = 256 / B.tw;
+  const int t = threadIdx.x;
+  const int ty = t / B.tw;
+  if (ty >= rows) return;
+  const int i = B.i0 + bx * B.tw + (t - ty * B.tw);
+  const int j = B.j0 + by * rows + ty;
+  if (i >= B.i0 + B.ni || j >= B.j0 + B.nj) return;
+  const long p = (long)i + (long)j * P.nx + (long)k * P.ss;
+  const long V = P.vol;
+  float X[9], Y[9], Z[9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    const float* v = P.wcur + c * V;
+    X[c] = d5<DX>(v, p, 1);
+    Y[c] = d5<DY>(v, p, P.nx);
+    Z[c] = d5<DZ>(v, p, P.ss);
+  }
+  finish_point<DX, DY, DZ, KIND, MT, FLAGS>(P, i, j, k, p, X, Y, Z);
+}
+
+template <int KIND, int MT, int FLAGS>
+static void launch_shell(hipStream_t st, const StageParams& P, const BoxParams* boxes, int n,
+                         int code)
+{
+  ShellBoxes S;
+  memset(&S, 0, sizeof S);
+  int total = 0;
+  for (int q = 0; q < n && S.n < SHELL_MAX; q++) {
+    BoxParams B = boxes[q];
+    if (B.ni <= 0 || B.nj <= 0 || B.nk <= 0) continue;
+    B.tw = std::min(64, B.ni);
+    const int g = grouped_grid(B, 256 / B.tw, B.tw);
+    S.b[S.n] = B;
+    S.start[S.n] = total;
+    total += g;
+    S.n++;
+  }
+  if (!S.n) return;
+  S.start[S.n] = total;
+  dim3 blk(256, 1, 1);
+#define CGFDH_SH(c, DX, DY, DZ)                                                                  \
+  case c:                                                                                        \
+    hipLaunchKernelGGL((stage_shell_kernel<DX, DY, DZ, KIND, MT, FLAGS>), dim3(total), blk, 0, st, \
+                       P, S);                                                                    \
+    break;
+  switch (code) {
+    CGFDH_SH(0, 0, 0, 0) CGFDH_SH(1, 0, 0, 1) CGFDH_SH(2, 0, 1, 0) CGFDH_SH(3, 0, 1, 1)
+    CGFDH_SH(4, 1, 0, 0) CGFDH_SH(5, 1, 0, 1) CGFDH_SH(6, 1, 1, 0) CGFDH_SH(7, 1, 1, 1)
+  }
+#undef CGFDH_SH
 }
 
 template <int KIND, int MT, int FLAGS>

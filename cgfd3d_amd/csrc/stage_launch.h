@@ -35,6 +35,8 @@ struct LaunchCtx {
   int free_top;
   int pml_on[6], pml_layers[6];
   int tgx, tgy;            // tile group of the grouped block order (0: default)
+  int merge_shell;         // 1: the five PML slabs of the shell in one launch
+  hipStream_t shell_stream;  // stream of the shell boxes (= stream, or one beside it)
 };
 
 // per-point kernel: one thread per grid point, every stencil operand read
@@ -140,12 +142,19 @@ static void launch_kind(const LaunchCtx& L, const StageParams& P, int dx, int dy
   }
   // only the top slab holds free-surface rows (the others end at khi)
   constexpr int SURF = HAS_SURF | HAS_PML;
+  st = L.shell_stream;
   launch_box<KIND, MT, SURF>(st, P, box(L.ni1, L.ni2, L.nj1, L.nj2, khi + 1, L.nk2), code);
-  launch_box<KIND, MT, HAS_PML>(st, P, box(L.ni1, L.ni2, L.nj1, L.nj2, L.nk1, klo - 1), code);
-  launch_box<KIND, MT, HAS_PML>(st, P, box(L.ni1, ilo - 1, L.nj1, L.nj2, klo, khi), code);
-  launch_box<KIND, MT, HAS_PML>(st, P, box(ihi + 1, L.ni2, L.nj1, L.nj2, klo, khi), code);
-  launch_box<KIND, MT, HAS_PML>(st, P, box(ilo, ihi, L.nj1, jlo - 1, klo, khi), code);
-  launch_box<KIND, MT, HAS_PML>(st, P, box(ilo, ihi, jhi + 1, L.nj2, klo, khi), code);
+  const BoxParams faces[5] = {box(L.ni1, L.ni2, L.nj1, L.nj2, L.nk1, klo - 1),
+                              box(L.ni1, ilo - 1, L.nj1, L.nj2, klo, khi),
+                              box(ihi + 1, L.ni2, L.nj1, L.nj2, klo, khi),
+                              box(ilo, ihi, L.nj1, jlo - 1, klo, khi),
+                              box(ilo, ihi, jhi + 1, L.nj2, klo, khi)};
+  if (L.merge_shell) {
+    // the five side and bottom slabs in one launch
+    launch_shell<KIND, MT, HAS_PML>(st, P, faces, 5, code);
+  } else {
+    for (const BoxParams& f : faces) launch_box<KIND, MT, HAS_PML>(st, P, f, code);
+  }
 }
 
 template <int MT>

@@ -31,16 +31,18 @@ EPI_NONE, EPI_RELU, EPI_MASK = 0, 1, 2
 IN_FEATURES = 32        # input row width (7 parameters and derived features, zero padded)
 
 
-def default_layers(nrecv=64, nt=1024, mt_terms=0):
+def default_layers(nrecv=64, nt=1024, mt_terms=0, width=1):
     """(K_in, N_out, relu) per layer.  The decoder upsamples by 4 per
-    transposed convolution from nt/64 samples.  With mt_terms > 0 the last
-    layer emits that many basis waveform sets (one per moment-tensor
-    component), contracted with the source's moment tensor."""
+    transposed convolution from nt/64 samples (channels 256, 128, 128 times
+    width).  With mt_terms > 0 the last layer emits that many basis waveform
+    sets (one per moment-tensor component), contracted with the source's
+    moment tensor."""
     t0 = nt // 64
     assert t0 * 64 == nt
+    c0, c1 = 256 * width, 128 * width
     return [(IN_FEATURES, 512, 1), (512, 512, 1), (512, 512, 1), (512, 512, 1),
-            (512, t0 * 256, 1),
-            (256, 4 * 128, 1), (128, 4 * 128, 1), (128, 4 * 3 * nrecv * max(1, mt_terms), 0)]
+            (512, t0 * c0, 1),
+            (c0, 4 * c1, 1), (c1, 4 * c1, 1), (c1, 4 * 3 * nrecv * max(1, mt_terms), 0)]
 
 
 def layer_rows(layers, nsamp):

@@ -13,7 +13,8 @@ dip / rake uniform in [0, 360) / [0, 90] / [-180, 180), Ricker f0 uniform in
 1-3 Hz with its peak at 1.2 / f0.  Target of one source: the 3 velocity
 components at the 64 receivers over 1024 steps.
 """
-import math
+import sys
+import time
 
 import numpy as np
 
@@ -100,11 +101,14 @@ def moment_tensors(params):
 
 
 def features(params, grid, mechanism=True):
-    """Network input rows (float32, 32 wide): the 7 parameters scaled to
-    [-1, 1], the unit moment tensor of the mechanism (6 values) and the sines
-    and cosines of strike, dip and rake; zero padded.  mechanism=False zeroes
-    every mechanism entry (a moment-tensor network takes the mechanism
-    through its output contraction only)."""
+    """Network input rows (float32, 32 wide).  mechanism=True: the 7
+    parameters scaled to [-1, 1], the unit moment tensor of the mechanism (6
+    values) and the sines and cosines of strike, dip and rake; zero padded.
+    mechanism=False (a moment-tensor network, which takes the mechanism
+    through its output contraction): x, y, depth and f0 scaled to [-1, 1]
+    and their sines and cosines at 1, 2 and 4 half-periods over the range
+    (28 values), which lets the MLP represent arrival times that move with
+    the source."""
     p = np.asarray(params, np.float64)
     n = p.shape[0]
     x = np.zeros((n, 32), np.float32)
@@ -130,8 +134,12 @@ def features(params, grid, mechanism=True):
     x[:, 13:19] = np.stack([np.sin(s), np.cos(s), np.sin(d), np.cos(d), np.sin(r), np.cos(r)],
                            axis=1)
     if not mechanism:
-        x[:, 3:6] = 0.0
-        x[:, 7:19] = 0.0
+        base = np.stack([x[:, 0], x[:, 1], x[:, 2], x[:, 6]], axis=1).astype(np.float64)
+        enc = [base]
+        for f in (1.0, 2.0, 4.0):
+            enc += [np.sin(np.pi * f * base), np.cos(np.pi * f * base)]
+        x = np.zeros((n, 32), np.float32)
+        x[:, :28] = np.concatenate(enc, axis=1)
     return x
 
 
@@ -153,8 +161,9 @@ class FdGenerator:
     Solvers are built once per batch slot and reused: before each batch the
     wavefield and PML memory are reset and the sources replaced."""
 
-    def __init__(self, grid, batch, variant=4, autotune=True):
+    def __init__(self, grid, batch, variant=4, autotune=True, progress=False):
         self.grid, self.batch, self.variant, self.autotune = grid, batch, variant, autotune
+        self.progress = progress
         self.solvers = []
         self._zero_w = None
         self._zero_aux = None
@@ -168,7 +177,6 @@ class FdGenerator:
 
     def run(self, params, ijk):
         """Seismograms [n, nrecv, 3, nt] of the given sources."""
-        import time
         g = self.grid
         out = np.zeros((len(params), g.nrecv, 3, g.nt), np.float32)
         for c0 in range(0, len(params), self.batch):
@@ -196,7 +204,9 @@ class FdGenerator:
                 out[q] = s.seismograms()[:, :3, : g.nt]
             b.close()
             self.runs += len(idx)
-            _ = t
+            if self.progress:
+                print(f"FD runs {idx.stop}/{len(params)}: {time.perf_counter() - t:.1f} s for "
+                      f"{len(idx)} sources", file=sys.stderr, flush=True)
         return out
 
     def close(self):

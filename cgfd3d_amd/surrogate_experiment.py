@@ -113,6 +113,13 @@ def main():
     ap.add_argument("--train-seconds", type=float, default=240.0)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--save", type=str, default="")
+    ap.add_argument("--subsets", type=str, default="",
+                    help="comma-separated training-set sizes (nested prefixes of the "
+                         "training runs): one network each, reported under 'subsets'")
+    ap.add_argument("--width", type=int, default=1, help="decoder channel multiplier")
+    ap.add_argument("--sweep", type=str, default="",
+                    help="extra trainings on the same data, ';'-separated key=value lists "
+                         "over lr, epochs, batch, width (e.g. 'lr=3e-3;width=2,lr=1e-3')")
     ap.add_argument("--no-mt", action="store_true",
                     help="plain output layer (no moment-tensor contraction)")
     ap.add_argument("--small", action="store_true",
@@ -130,7 +137,7 @@ def main():
     log(f"grid {grid.wl.name} built in {time.perf_counter() - t:.1f} s")
     ptr, ijk_tr = grid.sample_sources(args.train, args.seed)
     pte, ijk_te = grid.sample_sources(args.test, args.seed + 1000)
-    gen = sd.FdGenerator(grid, args.fd_batch)
+    gen = sd.FdGenerator(grid, args.fd_batch, progress=True)
     t = time.perf_counter()
     seis_tr = gen.run(ptr, ijk_tr)
     log(f"{args.train} training runs in {time.perf_counter() - t:.1f} s "
@@ -149,48 +156,73 @@ def main():
     mte = sd.moment_tensors(pte) if mt else None
     ytr, yte = sd.to_target_layout(seis_tr), sd.to_target_layout(seis_te)
     scale = 1.0 / float(np.sqrt((ytr.astype(np.float64) ** 2).mean()))
-    layers = sg.default_layers(grid.nrecv, grid.nt, mt)
-    net = sg.SurrogateNet(layers, seed=args.seed, mt_terms=mt,
-                          out_shape=(grid.nt, 3 * grid.nrecv))
-    log(f"network: {net.n_params() / 1e6:.2f} M parameters")
-    yd = torch.from_numpy((ytr * scale).astype(np.float32)).to("cuda:0")
-    tn2 = torch.from_numpy(((ytr * scale).reshape(len(ytr), -1).astype(np.float64) ** 2)
-                           .sum(axis=1).astype(np.float32)).to("cuda:0")
-
-    untrained = sd.relative_misfit(predict(net, xte, mte) / scale, yte.reshape(len(yte), -1))
-    t = time.perf_counter()
-    hist = train(net, xtr, yd, tn2, args.epochs, args.batch, args.lr, args.seed,
-                 time_budget=args.train_seconds, log_every=25, m=mtr)
-    torch.cuda.synchronize()
-    train_s = time.perf_counter() - t
-    pred = predict(net, xte, mte) / scale
-    mis = sd.relative_misfit(pred, yte.reshape(len(yte), -1))
-    ntr = min(128, len(xtr))
-    mis_tr = sd.relative_misfit(predict(net, xtr[:ntr], None if mtr is None else mtr[:ntr]) / scale,
-                                ytr[:ntr].reshape(ntr, -1))
     nn = sd.relative_misfit(nearest_neighbour(sd.features(ptr, grid), ytr.reshape(len(ytr), -1),
                                               sd.features(pte, grid)),
                             yte.reshape(len(yte), -1))
+
+    def fit(ntrain, lr=args.lr, epochs=args.epochs, batch=args.batch, width=args.width):
+        layers = sg.default_layers(grid.nrecv, grid.nt, mt, width)
+        net = sg.SurrogateNet(layers, seed=args.seed, mt_terms=mt,
+                              out_shape=(grid.nt, 3 * grid.nrecv))
+        yd = torch.from_numpy((ytr[:ntrain] * scale).astype(np.float32)).to("cuda:0")
+        tn2 = torch.from_numpy(((ytr[:ntrain] * scale).reshape(ntrain, -1).astype(np.float64) ** 2)
+                               .sum(axis=1).astype(np.float32)).to("cuda:0")
+        untrained = sd.relative_misfit(predict(net, xte, mte) / scale, yte.reshape(len(yte), -1))
+        t = time.perf_counter()
+        hist = train(net, xtr[:ntrain], yd, tn2, epochs, batch, lr, args.seed,
+                     time_budget=args.train_seconds, log_every=50,
+                     m=None if mtr is None else mtr[:ntrain])
+        torch.cuda.synchronize()
+        train_s = time.perf_counter() - t
+        pred = predict(net, xte, mte) / scale
+        mis = sd.relative_misfit(pred, yte.reshape(len(yte), -1))
+        ntr = min(128, ntrain)
+        mis_tr = sd.relative_misfit(
+            predict(net, xtr[:ntr], None if mtr is None else mtr[:ntr]) / scale,
+            ytr[:ntr].reshape(ntr, -1))
+        del yd, tn2
+        return net, dict(train_runs=ntrain, lr=lr, epochs=epochs, batch=batch, width=width,
+                         heldout_misfit_median=float(np.median(mis)),
+                         heldout_misfit_mean=float(mis.mean()),
+                         heldout_frac_le_5pct=float((mis <= 0.05).mean()),
+                         train_misfit_median=float(np.median(mis_tr)),
+                         untrained_misfit_median=float(np.median(untrained)),
+                         epochs_run=len(hist), train_seconds=train_s,
+                         final_train_loss=hist[-1] if hist else None), mis, pred
+
+    subsets = []
+    for n_sub in [int(v) for v in args.subsets.split(",") if v]:
+        if n_sub < len(xtr):
+            _, rec, _, _ = fit(n_sub)
+            log(f"subset {n_sub}: held-out median misfit {rec['heldout_misfit_median']:.3f}")
+            subsets.append(rec)
+    sweep = []
+    for item in [v for v in args.sweep.split(";") if v]:
+        kw = {}
+        for kv in item.split(","):
+            key, val = kv.split("=")
+            kw[key.strip()] = float(val) if key.strip() == "lr" else int(val)
+        _, rec, _, _ = fit(len(xtr), **kw)
+        log(f"sweep {kw}: held-out median misfit {rec['heldout_misfit_median']:.3f}, "
+            f"train {rec['train_misfit_median']:.3f}")
+        sweep.append(rec)
+    net, rec, mis, pred = fit(len(xtr))
+    log(f"network: {net.n_params() / 1e6:.2f} M parameters")
     rate, ms = inference_rate(net, xte, mte)
     out = {
         "metric": "surrogate vs FD seismogram sets/s",
         "surrogate_sets_per_s": rate, "surrogate_ms_per_256": ms,
         "fd_sets_per_s": fd_rate, "speedup": rate / fd_rate,
         "grid": [grid.ni, grid.nj, grid.nk], "nt": grid.nt, "nrecv": grid.nrecv,
-        "train_runs": args.train, "test_runs": args.test,
-        "heldout_misfit_median": float(np.median(mis)), "heldout_misfit_mean": float(mis.mean()),
-        "heldout_frac_le_5pct": float((mis <= 0.05).mean()),
-        "train_misfit_median": float(np.median(mis_tr)),
+        "test_runs": args.test, **rec,
         "nearest_neighbour_misfit_median": float(np.median(nn)),
-        "untrained_misfit_median": float(np.median(untrained)),
-        "epochs_run": len(hist), "train_seconds": train_s,
-        "final_train_loss": hist[-1] if hist else None,
+        "subsets": subsets, "sweep": sweep,
         "params": net.n_params(), "moment_tensor_output": bool(mt),
         "dtype": "bf16 weights/activations, fp32 accumulation",
     }
     if args.save:
         os.makedirs(os.path.dirname(os.path.abspath(args.save)), exist_ok=True)
-        np.savez(args.save, misfit=mis, nn_misfit=nn, hist=np.array(hist), params_test=pte,
+        np.savez(args.save, misfit=mis, nn_misfit=nn, params_test=pte,
                  pred0=pred[:2], ref0=yte[:2])
     print(json.dumps(out), flush=True)
 

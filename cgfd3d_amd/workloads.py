@@ -444,9 +444,10 @@ def batch_workloads(seed=4242, dh=50.0, dt=0.004, nt_total=1001, pml=20, n_side=
             L = _setup_sigs()
             m6 = np.zeros(6, np.float32)
             L.cgfdh_moment_from_angles(strike, dip, rake, 3.0e10, 1.0, 1.0e6, _p(m6))
-            si = 3 + int(r.integers(pml + 5, ni - pml - 5))
-            sj = 3 + int(r.integers(pml + 5, nj - pml - 5))
-            sk = 3 + int(r.integers(pml + 5, nk - 10))
+            # inside the PML frame, at least 3 nodes below the surface
+            si = 3 + int(r.integers(pml + 3, max(pml + 4, ni - pml - 3)))
+            sj = 3 + int(r.integers(pml + 3, max(pml + 4, nj - pml - 3)))
+            sk = 3 + int(r.integers(pml + 3, max(pml + 4, nk - 4)))
             tpeak = 1.2 / fc
             src = moment_source(ni, nj, nk, si, sj, sk, dt, 0.0, m6, fc, tpeak, 2.0 * tpeak)
             ridx, rw = surface_receiver_grid(ni, nj, nk, n_side)
