@@ -47,6 +47,8 @@ KERNELS = [
     ("stage_interior_vec4", "stage_interior_vec_kernel<NQ=4> + stage_box_kernel"),
     ("stage_interior_lds8", "stage_interior_lds_kernel<TY=8> + stage_box_kernel"),
     ("stage_interior_lds16", "stage_interior_lds_kernel<TY=16> + stage_box_kernel"),
+    ("stage_interior_pointz2", "stage_interior_pointz_kernel<KZ=2> + stage_shell_kernel"),
+    ("stage_interior_pointz4", "stage_interior_pointz_kernel<KZ=4> + stage_shell_kernel"),
 ]
 
 
@@ -189,7 +191,8 @@ def main():
                     help="stage kernels: 0 per-point, 1 tiled, 2/3 z-marching interior + "
                          "shell, 4/5 per-point interior (64x4 / 64x2) + shell, 6/7 "
                          "x-vectorised interior (2 / 4 points per thread) + shell, 8/9 "
-                         "LDS-tiled interior (64x8 / 64x16 plane tiles) + shell")
+                         "LDS-tiled interior (64x8 / 64x16 plane tiles) + shell, 10/11 "
+                         "per-point interior marching 2 / 4 planes per block + shell")
     ap.add_argument("--workload", default="iso", choices=["iso", "vti", "aniso", "visco", "batch"])
     ap.add_argument("--size", type=str, default=None,
                     help="grid NIxNJxNK (default 600x600x300 iso, 400x400x200 media)")

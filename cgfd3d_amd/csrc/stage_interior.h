@@ -279,6 +279,55 @@ static void launch_interior_point(hipStream_t st, const StageParams& P, BoxParam
 #undef CGFDH_IP
 }
 
+// the per-point kernel marching KZ consecutive planes per block, so that the
+// z neighbours one plane reads are read again by the same block moments
+// later (L1/L2 hits) instead of by blocks far apart in time.  The box's nk
+// holds the number of KZ-plane chunks and nchunk the true plane count.
+template <int DX, int DY, int DZ, int KIND, int MT, int BY, int KZ>
+__global__ void __launch_bounds__(64 * BY)
+stage_interior_pointz_kernel(StageParams P, BoxParams B)
+{
+  int bx, by, kc;
+  if (!grouped_block(B, bx, by, kc)) return;
+  const int i = B.i0 + bx * 64 + threadIdx.x;
+  const int j = B.j0 + by * BY + threadIdx.y;
+  if (i >= B.i0 + B.ni || j >= B.j0 + B.nj) return;
+  const int kbeg = B.k0 + (kc - B.k0) * KZ;
+  const int kend = min(B.k0 + B.nchunk, kbeg + KZ);
+  const long V = P.vol;
+  for (int k = kbeg; k < kend; k++) {
+    const long p = (long)i + (long)j * P.nx + (long)k * P.ss;
+    float X[9], Y[9], Z[9];
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+      const float* v = P.wcur + c * V;
+      X[c] = d5<DX>(v, p, 1);
+      Y[c] = d5<DY>(v, p, P.nx);
+      Z[c] = d5<DZ>(v, p, P.ss);
+    }
+    interior_finish<KIND, MT>(P, i, j, k, p, X, Y, Z);
+  }
+}
+
+template <int KIND, int MT, int BY, int KZ>
+static void launch_interior_pointz(hipStream_t st, const StageParams& P, BoxParams B, int code)
+{
+  B.nchunk = B.nk;
+  B.nk = (B.nk + KZ - 1) / KZ;
+  const int grid = grouped_grid(B, BY);
+  dim3 blk(64, BY, 1);
+#define CGFDH_IZ(c, DX, DY, DZ)                                                               \
+  case c:                                                                                     \
+    hipLaunchKernelGGL((stage_interior_pointz_kernel<DX, DY, DZ, KIND, MT, BY, KZ>), dim3(grid), \
+                       blk, 0, st, P, B);                                                     \
+    break;
+  switch (code) {
+    CGFDH_IZ(0, 0, 0, 0) CGFDH_IZ(1, 0, 0, 1) CGFDH_IZ(2, 0, 1, 0) CGFDH_IZ(3, 0, 1, 1)
+    CGFDH_IZ(4, 1, 0, 0) CGFDH_IZ(5, 1, 0, 1) CGFDH_IZ(6, 1, 1, 0) CGFDH_IZ(7, 1, 1, 1)
+  }
+#undef CGFDH_IZ
+}
+
 // interior box, one plane of a 64 x TY tile per block: the x and y stencil
 // operands of all 9 fields come from an LDS copy of the tile plus its 4-wide
 // x/y halo (each value loaded once per block instead of once per reader),

@@ -12,8 +12,10 @@
 // shell; 6/7 the x-vectorised interior kernel (2 points per thread on 64x4
 // threads / 4 points per thread on 64x2 threads) plus the shell; 8/9 the
 // LDS-tiled interior kernel (one 64x8 / 64x16 plane tile per block, x/y
-// operands from LDS) plus the shell.  The VTI and anisotropic media build
-// variants 0 and 4..9, the visco-elastic medium 0, 4, 5, 8 and 9.
+// operands from LDS) plus the shell; 10/11 the per-point interior kernel
+// marching 2 / 4 planes per block plus the shell.  The VTI and anisotropic
+// media build variants 0 and 4..11, the visco-elastic medium all but 1..3,
+// 6 and 7.
 #pragma once
 #include <algorithm>
 #include <cstdio>
@@ -121,7 +123,11 @@ static void launch_kind(const LaunchCtx& L, const StageParams& P, int dx, int dy
     if (in.ni > vb.ni)
       launch_box<KIND, MT, HAS_PML>(st, P, box(ilo + vb.ni, ihi, jlo, jhi, klo, khi), code);
   } else if (in.ni > 0 && in.nj > 0 && in.nk > 0) {
-    if (L.variant == 8) {
+    if (L.variant == 10) {
+      launch_interior_pointz<KIND, MT, 4, 2>(st, P, in, code);
+    } else if (L.variant == 11) {
+      launch_interior_pointz<KIND, MT, 4, 4>(st, P, in, code);
+    } else if (L.variant == 8) {
       launch_interior_lds<KIND, MT, 8>(st, P, in, code);
     } else if (L.variant == 9) {
       launch_interior_lds<KIND, MT, 16>(st, P, in, code);

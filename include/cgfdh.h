@@ -182,10 +182,11 @@ int cgfdh_last_timing(cgfdh_solver *s, double *run_ms, double *stage_ms,
  * interior box (64x8 / 64x16 tiles), 4 / 5 = lean per-point kernel on the
  * interior box (64x4 / 64x2 blocks), 6 / 7 = x-vectorised kernel on the
  * interior box (2 / 4 points per thread), 8 / 9 = LDS-tiled plane kernel on
- * the interior box (64x8 / 64x16 tiles); 2..9 run the per-point kernel on
- * the PML and free-surface shell.  Default 4.  Variants 1..3 exist for the
- * isotropic medium only, 6 / 7 not for the visco-elastic medium.  All
- * variants produce identical bits. */
+ * the interior box (64x8 / 64x16 tiles), 10 / 11 = per-point kernel
+ * marching 2 / 4 planes per block; 2..11 run the per-point kernel on the PML
+ * and free-surface shell.  Default 4.  Variants 1..3 exist for the isotropic
+ * medium only, 6 / 7 not for the visco-elastic medium.  All variants produce
+ * identical bits. */
 int cgfdh_set_kernel_variant(cgfdh_solver *s, int variant);
 
 /* Tile group of the grouped block order (gx x gy tiles per group, 0 0 = the
