@@ -35,20 +35,26 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 
-# per variant: (profile name, the kernels of one stage launch)
+# per variant: (profile name, the kernels of one stage launch); the shell
+# kernels are stage_shell_kernel (the five PML slabs) and stage_box_kernel
+# (the free-surface slab)
 KERNELS = [
     ("stage_point", "stage_point_kernel"),
     ("stage_tiled", "stage_tiled_kernel"),
-    ("stage_interior_ty8", "stage_interior_kernel<TY=8> + stage_box_kernel"),
-    ("stage_interior_ty16", "stage_interior_kernel<TY=16> + stage_box_kernel"),
-    ("stage_interior_point_by4", "stage_interior_point_kernel<BY=4> + stage_box_kernel"),
-    ("stage_interior_point_by2", "stage_interior_point_kernel<BY=2> + stage_box_kernel"),
-    ("stage_interior_vec2", "stage_interior_vec_kernel<NQ=2> + stage_box_kernel"),
-    ("stage_interior_vec4", "stage_interior_vec_kernel<NQ=4> + stage_box_kernel"),
-    ("stage_interior_lds8", "stage_interior_lds_kernel<TY=8> + stage_box_kernel"),
-    ("stage_interior_lds16", "stage_interior_lds_kernel<TY=16> + stage_box_kernel"),
-    ("stage_interior_pointz2", "stage_interior_pointz_kernel<KZ=2> + stage_shell_kernel"),
-    ("stage_interior_pointz4", "stage_interior_pointz_kernel<KZ=4> + stage_shell_kernel"),
+    ("stage_interior_ty8", "stage_interior_kernel<TY=8> + shell kernels"),
+    ("stage_interior_ty16", "stage_interior_kernel<TY=16> + shell kernels"),
+    ("stage_interior_point_by4", "stage_interior_point_kernel<BY=4> + shell kernels"),
+    ("stage_interior_point_by2", "stage_interior_point_kernel<BY=2> + shell kernels"),
+    ("stage_interior_vec2", "stage_interior_vec_kernel<NQ=2> + shell kernels"),
+    ("stage_interior_vec4", "stage_interior_vec_kernel<NQ=4> + shell kernels"),
+    ("stage_interior_lds8", "stage_interior_lds_kernel<TY=8> + shell kernels"),
+    ("stage_interior_lds16", "stage_interior_lds_kernel<TY=16> + shell kernels"),
+    ("stage_interior_pointz2", "stage_interior_pointz_kernel<KZ=2> + shell kernels"),
+    ("stage_interior_pointz4", "stage_interior_pointz_kernel<KZ=4> + shell kernels"),
+    ("stage_interior_phased", "stage_interior_phased_kernel<BY=4> + shell kernels"),
+    ("stage_interior_phased_late", "stage_interior_phased_kernel<BY=4, combine loads last> + "
+                                   "shell kernels"),
+    ("stage_interior_ldsp8", "stage_interior_ldsp_kernel<TY=8> + shell kernels"),
 ]
 
 
@@ -187,12 +193,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--variant", type=int, default=4,
-                    help="stage kernels: 0 per-point, 1 tiled, 2/3 z-marching interior + "
+    ap.add_argument("--variant", type=int, default=None,
+                    help="stage kernels (default: the engine's choice for the medium, 12 "
+                         "elastic, 4 visco-elastic): 0 per-point, 1 tiled, 2/3 z-marching interior + "
                          "shell, 4/5 per-point interior (64x4 / 64x2) + shell, 6/7 "
                          "x-vectorised interior (2 / 4 points per thread) + shell, 8/9 "
                          "LDS-tiled interior (64x8 / 64x16 plane tiles) + shell, 10/11 "
-                         "per-point interior marching 2 / 4 planes per block + shell")
+                         "per-point interior marching 2 / 4 planes per block + shell, 12 "
+                         "two-phase per-point interior + shell, 13 the same with the RK "
+                         "combine loads last, 14 LDS-tiled interior with phased loads + shell")
     ap.add_argument("--workload", default="iso", choices=["iso", "vti", "aniso", "visco", "batch"])
     ap.add_argument("--size", type=str, default=None,
                     help="grid NIxNJxNK (default 600x600x300 iso, 400x400x200 media)")
@@ -242,6 +251,7 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     run_ms, stage_ms, nlaunch = s.last_timing()
+    used_variant = s.kernel_choice()[0]
     seis = s.seismograms()
     s.close()
     log(f"timed {args.steps} steps: wall {wall:.3f} s, device {run_ms:.1f} ms, "
@@ -254,9 +264,7 @@ def main():
     bstep = wl.algorithmic_bytes_per_step()
     per_launch = bstep / 4.0
     achieved = per_launch / (stage_ms / nlaunch / 1e3) / 1e9
-    if wl.medium_type == 5 and args.variant in (6, 7):
-        args.variant = 4
-    kernel = KERNELS[args.variant]
+    kernel = KERNELS[used_variant]
     traffic = measured_traffic(kernel[0], wl.name)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

@@ -205,7 +205,7 @@ struct cgfdh_solver {
   float* recv_w = nullptr;
   float* seismo = nullptr;
   float* pg = nullptr;           // 15 peak maps + 3 displacement maps, [ny][nx] each
-  int variant = 4;
+  int variant = 12;              // set per medium in cgfdh_create
   int tgx = 0, tgy = 0;          // tile group (0: the default)
   bool metric_set = false, medium_set = false, mats_set = false;
   hipEvent_t ev_run[2];
@@ -308,7 +308,10 @@ extern "C" int cgfdh_create(const cgfdh_desc* d, cgfdh_solver** out)
     bad |= alloc(&F.A, np) | alloc(&F.B, np) | alloc(&F.D, np);
     for (auto& a : F.aux) bad |= alloc(&a, 9 * F.fn);
   }
-  if (!bad && hipStreamCreate(&s->stream) != hipSuccess) bad = 1;
+    // default interior kernel: the two-phase per-point kernel for the elastic
+  // media, the per-point kernel for the visco-elastic medium
+  s->variant = d->medium_type == CGFDH_MEDIUM_VISCO_ISO ? 4 : 12;
+if (!bad && hipStreamCreate(&s->stream) != hipSuccess) bad = 1;
   if (!bad && (hipStreamCreate(&s->stream2) != hipSuccess ||
                hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
                hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming) != hipSuccess))
@@ -585,11 +588,11 @@ extern "C" int cgfdh_get_seismograms(cgfdh_solver* s, float* out)
 
 extern "C" int cgfdh_set_kernel_variant(cgfdh_solver* s, int v)
 {
-  if (!s || v < 0 || v > 11) return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: %d", v);
+  if (!s || v < 0 || v > 14) return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: %d", v);
   if (s->d.medium_type != CGFDH_MEDIUM_ELASTIC_ISO && v >= 1 && v <= 3)
     return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: variant %d is built for the isotropic "
-                "medium only (VTI / anisotropic: 0, 4..11; visco-elastic: 0, 4, 5, 8..11)", v);
-  if (s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO && (v == 6 || v == 7))
+                "medium only (VTI / anisotropic: 0, 4..14; visco-elastic: 0, 4, 5, 8..11)", v);
+  if (s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO && (v == 6 || v == 7 || v >= 12))
     return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: variant %d is not built for the "
                 "visco-elastic medium (0, 4, 5, 8..11)", v);
   s->variant = v;
@@ -1127,7 +1130,7 @@ extern "C" int cgfdh_batch_autotune(cgfdh_batch* b, int it0)
     const bool iso = s->d.medium_type == CGFDH_MEDIUM_ELASTIC_ISO;
     const bool visco = s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO;
     std::vector<int> variants = visco ? std::vector<int>{4, 5, 8, 9}
-                                      : std::vector<int>{4, 6, 7, 8, 9};
+                                      : std::vector<int>{4, 6, 7, 8, 12, 14};
     if (iso) variants.push_back(3);
     const int groups[][2] = {{9, 2}, {3, 4}, {9, 8}};
     float best = 1e30f;

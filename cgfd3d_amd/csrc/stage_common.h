@@ -357,6 +357,40 @@ __device__ __forceinline__ void combine(const StageParams& P, long p, const floa
   }
 }
 
+// combine() of the 9 elastic fields split in two: the loads of the levels
+// the stage kind reads, and the stores with the same arithmetic
+template <int KIND>
+__device__ __forceinline__ void combine_load(const StageParams& P, long p, float* pre, float* end)
+{
+  const long V = P.vol;
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    if (KIND == KIND_FIRST || KIND == KIND_MID) pre[c] = ld_stream(P.wpre + c * V + p);
+    if (KIND == KIND_MID || KIND == KIND_LAST) end[c] = ld_stream(P.wend + c * V + p);
+  }
+}
+
+template <int KIND>
+__device__ __forceinline__ void combine_store(const StageParams& P, long p, const float* r,
+                                              const float* pre, const float* end)
+{
+  const long V = P.vol;
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    if (KIND == KIND_RHS) {
+      P.rhs_out[c * V + p] = r[c];
+    } else if (KIND == KIND_FIRST) {
+      st_stream(P.wtmp + c * V + p, pre[c] + P.ca * r[c]);
+      st_stream(P.wend + c * V + p, pre[c] + P.cb * r[c]);
+    } else if (KIND == KIND_MID) {
+      st_stream(P.wtmp + c * V + p, pre[c] + P.ca * r[c]);
+      st_stream(P.wend + c * V + p, end[c] + P.cb * r[c]);
+    } else {
+      st_stream(P.wend + c * V + p, end[c] + P.cb * r[c]);
+    }
+  }
+}
+
 // GMB attenuation after the PML terms (sv_curv_col_vis_iso.c:251-342): the
 // memory-variable rates into r[9 + 6n + {xx,yy,zz,yz,xz,xy}] and the memory
 // terms subtracted from the stress rates, with the reference's double
@@ -423,6 +457,11 @@ __device__ __forceinline__ void interior_finish(const StageParams& P, int i, int
   Medium<MT> md;
   md.load(P.med, V, p);
   const float slw = ld_stream(P.slw + p);
+  // the elastic media's RK combine inputs are loaded here, with the
+  // coefficients, so their latency overlaps the arithmetic below instead of
+  // following the source branch
+  float cpre[9], cend[9];
+  if constexpr (MT != MEDIUM_VISCO) combine_load<KIND>(P, p, cpre, cend);
   float r[MT == MEDIUM_VISCO ? NW_MAX : 9];
   momentum(r, m, slw, X, Y, Z);
   float Xv[3] = {X[VX], X[VY], X[VZ]}, Yv[3] = {Y[VX], Y[VY], Y[VZ]},
@@ -430,7 +469,10 @@ __device__ __forceinline__ void interior_finish(const StageParams& P, int i, int
   md.hooke(r, m, Xv, Yv, Zv);
   if constexpr (MT == MEDIUM_VISCO) atten(P, p, md.lam, md.mu, r);
   if (P.nsrc > 0) add_sources(P, i, j, k, p, r);
-  combine<KIND, MT == MEDIUM_VISCO ? NW_MAX : 9>(P, p, r);
+  if constexpr (MT != MEDIUM_VISCO)
+    combine_store<KIND>(P, p, r, cpre, cend);
+  else
+    combine<KIND, NW_MAX>(P, p, r);
 }
 
 // which boundary treatments a kernel instance has to handle
@@ -440,17 +482,14 @@ enum { HAS_SURF = 1, HAS_PML = 2 };
 // sources, RK combine.  X/Y/Z: inner-operator derivatives of the 9 fields.
 // FLAGS prunes the surface and PML code from instances whose points never
 // need them (the result is the same either way).
+// finish_point with the point's metric m, medium md and slowness already
+// loaded
 template <int DX, int DY, int DZ, int KIND, int MT, int FLAGS = HAS_SURF | HAS_PML>
-__device__ __forceinline__ void finish_point(const StageParams& P, int i, int j, int k, long p,
-                                             const float* X, const float* Y, const float* Z)
+__device__ __forceinline__ void finish_point_loaded(const StageParams& P, int i, int j, int k,
+                                                    long p, const float* X, const float* Y,
+                                                    const float* Z, const float* m,
+                                                    const Medium<MT>& md, float slw)
 {
-  const long V = P.vol;
-  float m[9];
-#pragma unroll
-  for (int q = 0; q < 9; q++) m[q] = ld_stream(P.met + (1 + q) * V + p);
-  Medium<MT> md;
-  md.load(P.med, V, p);
-  const float slw = ld_stream(P.slw + p);
   float r[MT == MEDIUM_VISCO ? NW_MAX : 9];
   momentum(r, m, slw, X, Y, Z);
   if ((FLAGS & HAS_SURF) && P.free_top && k >= P.nk2 - (DZ == 0 ? 3 : 1)) {
@@ -474,6 +513,20 @@ __device__ __forceinline__ void finish_point(const StageParams& P, int i, int j,
   if constexpr (MT == MEDIUM_VISCO) atten(P, p, md.lam, md.mu, r);
   if (P.nsrc > 0) add_sources(P, i, j, k, p, r);
   combine<KIND, MT == MEDIUM_VISCO ? NW_MAX : 9>(P, p, r);
+}
+
+template <int DX, int DY, int DZ, int KIND, int MT, int FLAGS = HAS_SURF | HAS_PML>
+__device__ __forceinline__ void finish_point(const StageParams& P, int i, int j, int k, long p,
+                                             const float* X, const float* Y, const float* Z)
+{
+  const long V = P.vol;
+  float m[9];
+#pragma unroll
+  for (int q = 0; q < 9; q++) m[q] = ld_stream(P.met + (1 + q) * V + p);
+  Medium<MT> md;
+  md.load(P.med, V, p);
+  const float slw = ld_stream(P.slw + p);
+  finish_point_loaded<DX, DY, DZ, KIND, MT, FLAGS>(P, i, j, k, p, X, Y, Z, m, md, slw);
 }
 
 }  // namespace cgfdh

@@ -279,6 +279,192 @@ static void launch_interior_point(hipStream_t st, const StageParams& P, BoxParam
 #undef CGFDH_IP
 }
 
+// the per-point kernel in two phases for the elastic media: every operand a
+// point needs (the 13 stencil points of the 9 fields, metric, medium, the RK
+// combine inputs) is requested first, then the arithmetic runs.  A schedule
+// barrier keeps the compiler from interleaving loads and waits, so each
+// wave has all its loads in flight at once (fewer waves, far more memory
+// requests outstanding per wave).
+template <int DX, int DY, int DZ, int KIND, int MT, int BY, bool PRE>
+__global__ void __launch_bounds__(64 * BY)
+stage_interior_phased_kernel(StageParams P, BoxParams B)
+{
+  static_assert(MT != MEDIUM_VISCO, "elastic media only");
+  int bx, by, k;
+  if (!grouped_block(B, bx, by, k)) return;
+  const int i = B.i0 + bx * 64 + threadIdx.x;
+  const int j = B.j0 + by * BY + threadIdx.y;
+  if (i >= B.i0 + B.ni || j >= B.j0 + B.nj) return;
+  const long p = (long)i + (long)j * P.nx + (long)k * P.ss;
+  const long V = P.vol;
+  constexpr int xl = Win<DX>::lo, yl = Win<DY>::lo, zl = Win<DZ>::lo;
+  float ax[9][5], ay[9][5], az[9][5];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    const float* v = P.wcur + c * V;
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+      ax[c][n] = v[p + (xl + n)];
+      ay[c][n] = v[p + (long)(yl + n) * P.nx];
+      az[c][n] = v[p + (long)(zl + n) * P.ss];
+    }
+  }
+  float m[9];
+#pragma unroll
+  for (int n = 0; n < 9; n++) m[n] = ld_stream(P.met + (1 + n) * V + p);
+  float mv[Medium<MT>::ncoef];
+#pragma unroll
+  for (int n = 0; n < Medium<MT>::ncoef; n++) mv[n] = ld_stream(P.med + n * V + p);
+  const float slw = ld_stream(P.slw + p);
+  // PRE: the RK combine inputs join the first phase too (more registers)
+  float cpre[9], cend[9];
+  if (PRE) combine_load<KIND>(P, p, cpre, cend);
+  __builtin_amdgcn_sched_barrier(0);
+  float X[9], Y[9], Z[9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    X[c] = d5v<DX>(ax[c][0], ax[c][1], ax[c][2], ax[c][3], ax[c][4]);
+    Y[c] = d5v<DY>(ay[c][0], ay[c][1], ay[c][2], ay[c][3], ay[c][4]);
+    Z[c] = d5v<DZ>(az[c][0], az[c][1], az[c][2], az[c][3], az[c][4]);
+  }
+  Medium<MT> md;
+  md.from(mv);
+  float r[9];
+  momentum(r, m, slw, X, Y, Z);
+  float Xv[3] = {X[VX], X[VY], X[VZ]}, Yv[3] = {Y[VX], Y[VY], Y[VZ]},
+        Zv[3] = {Z[VX], Z[VY], Z[VZ]};
+  md.hooke(r, m, Xv, Yv, Zv);
+  if (!PRE) combine_load<KIND>(P, p, cpre, cend);
+  if (P.nsrc > 0) add_sources(P, i, j, k, p, r);
+  combine_store<KIND>(P, p, r, cpre, cend);
+}
+
+template <int KIND, int MT, int BY, bool PRE>
+static void launch_interior_phased(hipStream_t st, const StageParams& P, BoxParams B, int code)
+{
+  const int grid = grouped_grid(B, BY);
+  dim3 blk(64, BY, 1);
+#define CGFDH_IF(c, DX, DY, DZ)                                                            \
+  case c:                                                                                  \
+    hipLaunchKernelGGL((stage_interior_phased_kernel<DX, DY, DZ, KIND, MT, BY, PRE>), dim3(grid), \
+                       blk, 0, st, P, B);                                                  \
+    break;
+  switch (code) {
+    CGFDH_IF(0, 0, 0, 0) CGFDH_IF(1, 0, 0, 1) CGFDH_IF(2, 0, 1, 0) CGFDH_IF(3, 0, 1, 1)
+    CGFDH_IF(4, 1, 0, 0) CGFDH_IF(5, 1, 0, 1) CGFDH_IF(6, 1, 1, 0) CGFDH_IF(7, 1, 1, 1)
+  }
+#undef CGFDH_IF
+}
+
+// the LDS-tiled plane kernel in phases for the elastic media: the tile loads
+// first, then every other operand of the point (z stencil, metric, medium,
+// RK combine inputs); the LDS stores wait only for the tile loads, so the
+// other requests stay in flight across the barrier.
+template <int DX, int DY, int DZ, int KIND, int MT, int TY>
+__global__ void __launch_bounds__(64 * TY)
+stage_interior_ldsp_kernel(StageParams P, BoxParams B)
+{
+  static_assert(MT != MEDIUM_VISCO, "elastic media only");
+  using T = LdsTile<TY>;
+  constexpr int NE = (T::PLANE + 64 * TY - 1) / (64 * TY);   // tile entries per thread
+  __shared__ float sm[9 * T::PLANE];
+  int bx, by, k;
+  // block-uniform exit: every thread of a block takes the same branch
+  if (!grouped_block(B, bx, by, k)) return;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i0 = B.i0 + bx * 64, j0 = B.j0 + by * TY;
+  const int i = i0 + tx, j = j0 + ty;
+  const int nx = P.nx;
+  const int ny = (int)(P.ss / nx);
+  const long V = P.vol, ss = P.ss;
+  const long pk = (long)k * ss;
+  constexpr int xl = -Win<DX>::lo;
+  constexpr int yl = -Win<DY>::lo;
+  constexpr int zl = Win<DZ>::lo;
+  const bool valid = i < B.i0 + B.ni && j < B.j0 + B.nj;
+  // out-of-box threads read the box's first point instead (in bounds, unused)
+  const long p = valid ? (long)i + (long)j * nx + pk : (long)B.i0 + (long)B.j0 * nx + pk;
+  float tv[NE][9];
+#pragma unroll
+  for (int q = 0; q < NE; q++) {
+    const int e = ty * 64 + tx + q * 64 * TY;
+    const int r = e / T::W, col = e - r * T::W;
+    const int gi = i0 - xl + col, gj = j0 - yl + r;
+    const bool ok = e < T::PLANE && gi < nx && gj < ny;
+    const long g = ok ? (long)gi + (long)gj * nx + pk : p;
+#pragma unroll
+    for (int c = 0; c < 9; c++) tv[q][c] = P.wcur[c * V + g];
+    if (!ok) {
+#pragma unroll
+      for (int c = 0; c < 9; c++) tv[q][c] = 0.0f;
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  float az[9][5];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+#pragma unroll
+    for (int n = 0; n < 5; n++) az[c][n] = P.wcur[c * V + p + (long)(zl + n) * ss];
+  }
+  float m[9];
+#pragma unroll
+  for (int n = 0; n < 9; n++) m[n] = ld_stream(P.met + (1 + n) * V + p);
+  float mv[Medium<MT>::ncoef];
+#pragma unroll
+  for (int n = 0; n < Medium<MT>::ncoef; n++) mv[n] = ld_stream(P.med + n * V + p);
+  const float slw = ld_stream(P.slw + p);
+  float cpre[9], cend[9];
+  combine_load<KIND>(P, p, cpre, cend);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int q = 0; q < NE; q++) {
+    const int e = ty * 64 + tx + q * 64 * TY;
+    if (e < T::PLANE) {
+#pragma unroll
+      for (int c = 0; c < 9; c++) sm[c * T::PLANE + e] = tv[q][c];
+    }
+  }
+  __syncthreads();
+  if (!valid) return;
+  float X[9], Y[9], Z[9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    const float* s = sm + c * T::PLANE;
+    const int rowb = (ty + yl) * T::W + tx;
+    X[c] = d5v<DX>(s[rowb], s[rowb + 1], s[rowb + 2], s[rowb + 3], s[rowb + 4]);
+    const int colb = ty * T::W + tx + xl;
+    Y[c] = d5v<DY>(s[colb], s[colb + T::W], s[colb + 2 * T::W], s[colb + 3 * T::W],
+                   s[colb + 4 * T::W]);
+    Z[c] = d5v<DZ>(az[c][0], az[c][1], az[c][2], az[c][3], az[c][4]);
+  }
+  Medium<MT> md;
+  md.from(mv);
+  float r[9];
+  momentum(r, m, slw, X, Y, Z);
+  float Xv[3] = {X[VX], X[VY], X[VZ]}, Yv[3] = {Y[VX], Y[VY], Y[VZ]},
+        Zv[3] = {Z[VX], Z[VY], Z[VZ]};
+  md.hooke(r, m, Xv, Yv, Zv);
+  if (P.nsrc > 0) add_sources(P, i, j, k, p, r);
+  combine_store<KIND>(P, p, r, cpre, cend);
+}
+
+template <int KIND, int MT, int TY>
+static void launch_interior_ldsp(hipStream_t st, const StageParams& P, BoxParams B, int code)
+{
+  const int grid = grouped_grid(B, TY);
+  dim3 blk(64, TY, 1);
+#define CGFDH_IP2(c, DX, DY, DZ)                                                            \
+  case c:                                                                                   \
+    hipLaunchKernelGGL((stage_interior_ldsp_kernel<DX, DY, DZ, KIND, MT, TY>), dim3(grid), blk, \
+                       0, st, P, B);                                                        \
+    break;
+  switch (code) {
+    CGFDH_IP2(0, 0, 0, 0) CGFDH_IP2(1, 0, 0, 1) CGFDH_IP2(2, 0, 1, 0) CGFDH_IP2(3, 0, 1, 1)
+    CGFDH_IP2(4, 1, 0, 0) CGFDH_IP2(5, 1, 0, 1) CGFDH_IP2(6, 1, 1, 0) CGFDH_IP2(7, 1, 1, 1)
+  }
+#undef CGFDH_IP2
+}
+
 // the per-point kernel marching KZ consecutive planes per block, so that the
 // z neighbours one plane reads are read again by the same block moments
 // later (L1/L2 hits) instead of by blocks far apart in time.  The box's nk

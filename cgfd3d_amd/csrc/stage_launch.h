@@ -13,9 +13,12 @@
 // threads / 4 points per thread on 64x2 threads) plus the shell; 8/9 the
 // LDS-tiled interior kernel (one 64x8 / 64x16 plane tile per block, x/y
 // operands from LDS) plus the shell; 10/11 the per-point interior kernel
-// marching 2 / 4 planes per block plus the shell.  The VTI and anisotropic
-// media build variants 0 and 4..11, the visco-elastic medium all but 1..3,
-// 6 and 7.
+// marching 2 / 4 planes per block plus the shell; 12 / 13 the two-phase
+// per-point interior kernel (all loads, then the arithmetic; 13 loads the RK
+// combine inputs after the arithmetic) plus the shell; 14 the LDS-tiled
+// plane kernel (64x8) with the same phased loads plus the shell.  The VTI
+// and anisotropic media build variants 0 and 4..14, the visco-elastic medium
+// all but 1..3, 6, 7 and 12..14.
 #pragma once
 #include <algorithm>
 #include <cstdio>
@@ -123,7 +126,16 @@ static void launch_kind(const LaunchCtx& L, const StageParams& P, int dx, int dy
     if (in.ni > vb.ni)
       launch_box<KIND, MT, HAS_PML>(st, P, box(ilo + vb.ni, ihi, jlo, jhi, klo, khi), code);
   } else if (in.ni > 0 && in.nj > 0 && in.nk > 0) {
-    if (L.variant == 10) {
+    if (L.variant == 14) {
+      if constexpr (MT != MEDIUM_VISCO) launch_interior_ldsp<KIND, MT, 8>(st, P, in, code);
+    } else if (L.variant == 12 || L.variant == 13) {
+      if constexpr (MT != MEDIUM_VISCO) {
+        if (L.variant == 12)
+          launch_interior_phased<KIND, MT, 4, true>(st, P, in, code);
+        else
+          launch_interior_phased<KIND, MT, 4, false>(st, P, in, code);
+      }
+    } else if (L.variant == 10) {
       launch_interior_pointz<KIND, MT, 4, 2>(st, P, in, code);
     } else if (L.variant == 11) {
       launch_interior_pointz<KIND, MT, 4, 4>(st, P, in, code);

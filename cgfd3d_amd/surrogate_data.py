@@ -161,7 +161,7 @@ class FdGenerator:
     Solvers are built once per batch slot and reused: before each batch the
     wavefield and PML memory are reset and the sources replaced."""
 
-    def __init__(self, grid, batch, variant=4, autotune=True, progress=False):
+    def __init__(self, grid, batch, variant=None, autotune=True, progress=False):
         self.grid, self.batch, self.variant, self.autotune = grid, batch, variant, autotune
         self.progress = progress
         self.solvers = []
@@ -213,6 +213,25 @@ class FdGenerator:
         for s in self.solvers:
             s.close()
         self.solvers = []
+
+
+def compress_runs(seis, dt, fmax=8.0):
+    """Band-limited storage of FD seismograms [n, nrecv, 3, nt]: the real
+    FFT of every trace up to fmax Hz as float16 (real and imaginary parts).
+    The sources' Ricker wavelets (f0 <= 3 Hz) carry a fraction of 1e-4 of
+    their energy above 8 Hz."""
+    n, nr, nc, nt = seis.shape
+    nb = int(fmax * nt * dt) + 1
+    F = np.fft.rfft(seis.astype(np.float64), axis=3)[..., :nb]
+    return np.stack([F.real, F.imag], axis=-1).astype(np.float16), nt
+
+
+def decompress_runs(z, nt):
+    """Inverse of compress_runs: [n, nrecv, 3, nt] float32."""
+    F = z[..., 0].astype(np.float64) + 1j * z[..., 1].astype(np.float64)
+    full = np.zeros(F.shape[:-1] + (nt // 2 + 1,), np.complex128)
+    full[..., :F.shape[-1]] = F
+    return np.fft.irfft(full, n=nt, axis=3).astype(np.float32)
 
 
 def relative_misfit(pred, ref):

@@ -120,6 +120,13 @@ def main():
     ap.add_argument("--sweep", type=str, default="",
                     help="extra trainings on the same data, ';'-separated key=value lists "
                          "over lr, epochs, batch, width (e.g. 'lr=3e-3;width=2,lr=1e-3')")
+    ap.add_argument("--generate", type=int, default=0, metavar="N",
+                    help="only run N training sources (seed --seed) and store them, band-"
+                         "limited, in --store; prints the FD rate and the storage error")
+    ap.add_argument("--store", type=str, default="")
+    ap.add_argument("--extra", type=str, default="",
+                    help="comma-separated stores of earlier --generate jobs, added to the "
+                         "training runs")
     ap.add_argument("--no-mt", action="store_true",
                     help="plain output layer (no moment-tensor contraction)")
     ap.add_argument("--small", action="store_true",
@@ -135,6 +142,20 @@ def main():
     else:
         grid = sd.SurrogateGrid()
     log(f"grid {grid.wl.name} built in {time.perf_counter() - t:.1f} s")
+    if args.generate:
+        ptr, ijk_tr = grid.sample_sources(args.generate, args.seed)
+        gen = sd.FdGenerator(grid, args.fd_batch, progress=True)
+        seis = gen.run(ptr, ijk_tr)
+        gen.close()
+        z, nt = sd.compress_runs(seis, grid.dt)
+        err = sd.relative_misfit(sd.decompress_runs(z, nt).reshape(len(seis), -1),
+                                 seis.reshape(len(seis), -1))
+        np.savez(args.store, params=ptr, ijk=ijk_tr, z=z, nt=nt, dt=grid.dt, seed=args.seed)
+        print(json.dumps({"generated": args.generate, "seed": args.seed,
+                          "fd_sets_per_s": gen.runs / gen.fd_seconds,
+                          "store_bytes": int(z.nbytes), "storage_misfit_median": float(np.median(err)),
+                          "storage_misfit_max": float(err.max())}), flush=True)
+        return
     ptr, ijk_tr = grid.sample_sources(args.train, args.seed)
     pte, ijk_te = grid.sample_sources(args.test, args.seed + 1000)
     gen = sd.FdGenerator(grid, args.fd_batch, progress=True)
@@ -149,6 +170,14 @@ def main():
     gen.close()
     log(f"{args.test} held-out runs in {time.perf_counter() - t:.1f} s")
     fd_rate = fd_runs / fd_dev_s
+    n_fd_here = len(ptr)
+    for path in [v for v in args.extra.split(",") if v]:
+        d = np.load(path)
+        if int(d["nt"]) != grid.nt:
+            raise ValueError(f"{path}: {int(d['nt'])} steps, the grid runs {grid.nt}")
+        ptr = np.concatenate([d["params"], ptr])
+        seis_tr = np.concatenate([sd.decompress_runs(d["z"], int(d["nt"])), seis_tr])
+        log(f"added {len(d['params'])} stored runs from {path} (seed {int(d['seed'])})")
 
     mt = 0 if args.no_mt else 6
     xtr, xte = sd.features(ptr, grid, not mt), sd.features(pte, grid, not mt)
@@ -214,7 +243,7 @@ def main():
         "surrogate_sets_per_s": rate, "surrogate_ms_per_256": ms,
         "fd_sets_per_s": fd_rate, "speedup": rate / fd_rate,
         "grid": [grid.ni, grid.nj, grid.nk], "nt": grid.nt, "nrecv": grid.nrecv,
-        "test_runs": args.test, **rec,
+        "test_runs": args.test, "fd_runs_this_job": n_fd_here, **rec,
         "nearest_neighbour_misfit_median": float(np.median(nn)),
         "subsets": subsets, "sweep": sweep,
         "params": net.n_params(), "moment_tensor_output": bool(mt),

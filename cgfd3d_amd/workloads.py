@@ -223,12 +223,15 @@ class Workload:
         surf = 352.0 + (36.0 if self.medium_type == MEDIUM_VISCO_ISO else 0.0)
         return self.bytes_per_point() * self.npoints + 576.0 * self.n_aux() + surf * nsurf
 
-    def build_solver(self, variant=4):
+    def build_solver(self, variant=None):
+        """A configured solver; variant None keeps the engine's default
+        interior kernel for the medium."""
         L = _setup_sigs()
         s = Solver(self.ni, self.nj, self.nk, self.dt, self.nt_total, free_top=self.free_top,
                    pml_on=self.pml_on, pml_layers=self.pml_layers,
                    medium_type=self.medium_type, nmaxwell=self.nmaxwell)
-        s.set_kernel_variant(variant)
+        if variant is not None:
+            s.set_kernel_variant(variant)
         check(L.cgfdh_compute_metric(s.h, _p(self.xyz)), "compute_metric")
         if self.medium_type == MEDIUM_ELASTIC_ISO:
             s.set_medium_iso(self.lam, self.mu, self.slw)
