@@ -8,7 +8,8 @@ import os
 
 _P = C.c_void_p
 _PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_PKG, "libcgfdh.so")
+# CGFDH_LIB may name another build of the same library (A/B timing runs)
+LIB_PATH = os.environ.get("CGFDH_LIB") or os.path.join(_PKG, "libcgfdh.so")
 
 _lib = None
 

@@ -490,6 +490,10 @@ __device__ __forceinline__ void finish_point_loaded(const StageParams& P, int i,
                                                     const float* Z, const float* m,
                                                     const Medium<MT>& md, float slw)
 {
+  // the elastic media's RK combine inputs are requested before the
+  // arithmetic and the boundary branches
+  float cpre[9], cend[9];
+  if constexpr (MT != MEDIUM_VISCO) combine_load<KIND>(P, p, cpre, cend);
   float r[MT == MEDIUM_VISCO ? NW_MAX : 9];
   momentum(r, m, slw, X, Y, Z);
   if ((FLAGS & HAS_SURF) && P.free_top && k >= P.nk2 - (DZ == 0 ? 3 : 1)) {
@@ -512,7 +516,10 @@ __device__ __forceinline__ void finish_point_loaded(const StageParams& P, int i,
   }
   if constexpr (MT == MEDIUM_VISCO) atten(P, p, md.lam, md.mu, r);
   if (P.nsrc > 0) add_sources(P, i, j, k, p, r);
-  combine<KIND, MT == MEDIUM_VISCO ? NW_MAX : 9>(P, p, r);
+  if constexpr (MT != MEDIUM_VISCO)
+    combine_store<KIND>(P, p, r, cpre, cend);
+  else
+    combine<KIND, NW_MAX>(P, p, r);
 }
 
 template <int DX, int DY, int DZ, int KIND, int MT, int FLAGS = HAS_SURF | HAS_PML>

@@ -15,7 +15,8 @@
  *   sv_curv_col_vis_iso_onestage(...)   (forward/sv_curv_col_vis_iso.h:17-33).
  * This ABI carries the same data with plain pointers and sizes, so a C shim
  * that unpacks those structs into these calls replaces the reference's time
- * loop (the shim is shown in INTEGRATION.md).  No torch or HIP types appear.
+ * loop (cgfd3d_amd/integration/drv_rk_curv_col_cgfdh.c is that shim).  No
+ * torch or HIP types appear.
  *
  * Conventions (all follow the reference):
  *  - grids carry 3 ghost points per side: nx=ni+6, ny=nj+6, nz=nk+6;

@@ -4,7 +4,7 @@
  * No counterpart in the reference: the surrogate maps source parameters
  * (x, y, depth, strike, dip, rake, f0) to 3-component surface seismograms for
  * one fixed grid and medium, trained on runs of the batched forward solver
- * (cgfdh_batch_*), which stays the fallback and the judge of its misfit.
+ * (cgfdh_batch_*), which stays the fallback and the measure of its misfit.
  *
  * Storage conventions: bf16 is the upper 16 bits of an IEEE binary32 value
  * (conversion rounds to nearest, ties to even); matrices are row-major with

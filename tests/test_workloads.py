@@ -1,5 +1,5 @@
 """CPU checks of the benchmark workload builders: algorithmic bytes per point
-against the hand count of SURVEY-style stage traffic, medium volumes (VTI
+against a hand count of the stage traffic, medium volumes (VTI
 from Thomsen parameters, a rotated VTI tensor keeps its eigenvalues, GMB
 coefficients from the host builder), and the batched configuration's
 sizes."""
@@ -70,3 +70,27 @@ def test_batch_configuration_sizes():
     assert len({x.src["ijk"][0] * 1000 + x.src["ijk"][1] for x in wls}) > 1
     total = 24 * 128 ** 3 + 24 * 192 * 192 * 128 + 16 * 256 * 256 * 160
     assert total == 331_350_016     # 331.35 M points in the default batch
+
+
+def test_band_limited_store_round_trip():
+    """Stored FD runs keep everything below 8 Hz: a trace made of sinusoids
+    under 8 Hz comes back to float16 precision, one at 12 Hz is dropped."""
+    from cgfd3d_amd import surrogate_data as sd
+    dt, nt = 0.008, 1024
+    t = np.arange(nt) * dt
+    rng = np.random.default_rng(3)
+    seis = np.zeros((2, 4, 3, nt), np.float32)
+    for n in range(2):
+        for r in range(4):
+            for c in range(3):
+                # a whole number of periods over the window: one FFT bin
+                f = np.floor(rng.uniform(0.5, 7.0) * nt * dt) / (nt * dt)
+                seis[n, r, c] = np.sin(2 * np.pi * f * t + r)
+    z, nt2 = sd.compress_runs(seis, dt)
+    assert z.dtype == np.float16 and z.shape[3] == int(8.0 * nt * dt) + 1
+    back = sd.decompress_runs(z, nt2)
+    assert np.max(sd.relative_misfit(back.reshape(2, -1), seis.reshape(2, -1))) < 2e-3
+    f12 = np.floor(12.0 * nt * dt) / (nt * dt)
+    hi = np.sin(2 * np.pi * f12 * t).astype(np.float32).reshape(1, 1, 1, nt)
+    z, _ = sd.compress_runs(hi, dt)
+    assert np.abs(sd.decompress_runs(z, nt)).max() < 1e-2
