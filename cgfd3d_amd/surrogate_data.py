@@ -143,6 +143,58 @@ def features(params, grid, mechanism=True):
     return x
 
 
+def receiver_xy(grid):
+    """[nrecv, 2] horizontal coordinates of the surface receivers."""
+    nx, ny = grid.ni + 6, grid.nj + 6
+    xyz = grid.wl.xyz.reshape(3, grid.nk + 6, ny, nx)
+    out = np.zeros((grid.nrecv, 2))
+    for r, idx in enumerate(grid.wl.recv_idx):
+        p = int(idx[0])
+        i, j, k = p % nx, (p // nx) % ny, p // (nx * ny)
+        out[r] = xyz[0, k, j, i], xyz[1, k, j, i]
+    return out
+
+
+def receiver_features(params, grid):
+    """Input rows of the per-receiver network, one per (source, receiver):
+    source x, y, depth, f0 and receiver x, y scaled to [-1, 1]; the offset
+    (receiver minus source) and its length scaled by the grid's extent; and
+    sines and cosines of the offsets and depth at 1, 2 and 4 half-periods
+    over their range.  [n * nrecv, 32] float32, receivers fastest."""
+    p = np.asarray(params, np.float64)
+    n = len(p)
+    rxy = receiver_xy(grid)
+    nr = len(rxy)
+    Lx = (grid.ni - 1) * grid.dh
+    Ly = (grid.nj - 1) * grid.dh
+
+    def sc(v, lo, hi):
+        return 2.0 * (v - lo) / (hi - lo) - 1.0
+    xs = np.repeat(p[:, 0], nr)
+    ys = np.repeat(p[:, 1], nr)
+    dep = np.repeat(sc(p[:, 2], *grid.depth_range), nr)
+    f0 = np.repeat(sc(p[:, 6], *RANGES["f0"]), nr)
+    xr = np.tile(rxy[:, 0], n)
+    yr = np.tile(rxy[:, 1], n)
+    dx, dy = (xr - xs) / Lx, (yr - ys) / Ly
+    dist = np.sqrt(dx * dx + dy * dy)
+    base = [sc(xs, *grid.x_range), sc(ys, *grid.y_range), dep, f0, sc(xr, 0, Lx),
+            sc(yr, 0, Ly), dx, dy, dist]
+    enc = []
+    for f in (1.0, 2.0, 4.0):
+        for v in (dx, dy, dep):
+            enc += [np.sin(np.pi * f * v), np.cos(np.pi * f * v)]
+    x = np.zeros((n * nr, 32), np.float32)
+    x[:, :27] = np.stack(base + enc, axis=1)
+    return x
+
+
+def to_receiver_layout(seis):
+    """[n, nrecv, 3, nt] -> [n * nrecv, nt, 3] (one row block per trace set)."""
+    n, nr, nc, nt = seis.shape
+    return np.ascontiguousarray(seis.transpose(0, 1, 3, 2).reshape(n * nr, nt, nc))
+
+
 def to_target_layout(seis):
     """[n, nrecv, 3, nt] solver seismograms -> [n, nt, 3 * nrecv] (the
     network's channels-last output: channel = component * nrecv + receiver)."""

@@ -94,3 +94,23 @@ def test_band_limited_store_round_trip():
     hi = np.sin(2 * np.pi * f12 * t).astype(np.float32).reshape(1, 1, 1, nt)
     z, _ = sd.compress_runs(hi, dt)
     assert np.abs(sd.decompress_runs(z, nt)).max() < 1e-2
+
+
+def test_receiver_rows_line_up_with_traces():
+    """Per-receiver rows: features and traces of (source, receiver) pairs in
+    the same order (receivers fastest), offsets = receiver minus source."""
+    from cgfd3d_amd import surrogate_data as sd
+    g = sd.SurrogateGrid(ni=48, nj=46, nk=30, nt=128, n_side=2, pml=5)
+    p, _ = g.sample_sources(3, 2)
+    x = sd.receiver_features(p, g)
+    rxy = sd.receiver_xy(g)
+    assert x.shape == (3 * 4, 32)
+    Lx, Ly = (g.ni - 1) * g.dh, (g.nj - 1) * g.dh
+    for s in range(3):
+        for r in range(4):
+            row = x[s * 4 + r]
+            assert np.isclose(row[6], (rxy[r, 0] - p[s, 0]) / Lx, atol=1e-6)
+            assert np.isclose(row[7], (rxy[r, 1] - p[s, 1]) / Ly, atol=1e-6)
+    seis = np.random.default_rng(0).random((3, 4, 3, 128)).astype(np.float32)
+    rows = sd.to_receiver_layout(seis)
+    assert np.array_equal(rows[2 * 4 + 1], seis[2, 1].T)
