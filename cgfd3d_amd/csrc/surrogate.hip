@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g)
   __shared__ __attribute__((aligned(16))) uint16_t As[2][BM][LDK];
   __shared__ __attribute__((aligned(16))) uint16_t Bs[2][BN][LDK];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
   const int kb = blockIdx.z * g.kchunk;
   const int ke = min(g.K, kb + g.kchunk);
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
@@ -354,9 +354,10 @@ extern "C" int cgfds_gemm_bf16(int M, int N, int K, const void* A, long lda, int
              (const uint16_t*)mask, ldm, C32, ldc32, (uint16_t*)C16, ldc16, kchunk,
              S > 1 ? work : nullptr};
   hipStream_t st = (hipStream_t)stream;
-  dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, S);
+  // row tiles on x (up to 2^31 - 1 blocks), column tiles on y
+  dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, S);
   if (grid.y > 65535)
-    return sfail(CGFDH_EINVAL, "cgfds_gemm_bf16: M above 65535*64 rows");
+    return sfail(CGFDH_EINVAL, "cgfds_gemm_bf16: N above 65535*64 columns");
   if (!transA && !transB) hipLaunchKernelGGL((gemm_bf16_kernel<0, 0>), grid, dim3(256), 0, st, g);
   if (!transA && transB) hipLaunchKernelGGL((gemm_bf16_kernel<0, 1>), grid, dim3(256), 0, st, g);
   if (transA && !transB) hipLaunchKernelGGL((gemm_bf16_kernel<1, 0>), grid, dim3(256), 0, st, g);

@@ -4,6 +4,9 @@ it on held-out sources against FD.
 
     python -m cgfd3d_amd.surrogate_experiment --train 768 --test 128
 
+The default form is the per-receiver network (one row per source-receiver
+pair); --per-source selects one row per source with all receivers' traces.
+
 Prints one JSON line: FD and surrogate seismogram sets per second (a set is
 the 3 x nrecv x nt traces of one source), held-out relative L2 misfits of
 the surrogate and of the baselines without it (nearest training source in
@@ -197,9 +200,10 @@ def main():
     ap.add_argument("--extra", type=str, default="",
                     help="comma-separated stores of earlier --generate jobs, added to the "
                          "training runs")
-    ap.add_argument("--per-receiver", action="store_true",
-                    help="one shared decoder per receiver trace: the MLP also takes the "
-                         "receiver position and the source-receiver offset")
+    ap.add_argument("--per-source", action="store_true",
+                    help="one decoder output per source with every receiver's traces (the "
+                         "default form runs one shared decoder per receiver trace, the MLP "
+                         "also taking the receiver position and the source-receiver offset)")
     ap.add_argument("--no-mt", action="store_true",
                     help="plain output layer (no moment-tensor contraction)")
     ap.add_argument("--small", action="store_true",
@@ -252,7 +256,7 @@ def main():
         seis_tr = np.concatenate([sd.decompress_runs(d["z"], int(d["nt"])), seis_tr])
         log(f"added {len(d['params'])} stored runs from {path} (seed {int(d['seed'])})")
 
-    if args.per_receiver:
+    if not args.per_source:
         return per_receiver(args, grid, ptr, pte, seis_tr, seis_te, fd_rate, n_fd_here)
     mt = 0 if args.no_mt else 6
     xtr, xte = sd.features(ptr, grid, not mt), sd.features(pte, grid, not mt)

@@ -241,3 +241,20 @@ def test_mt_network_forward_backward_match_oracle():
         eb = rel_l2(_host(net.gb[li]), grads[li][1])
         print(f"layer {li}: dW rel L2 {ew:.1e}, db rel L2 {eb:.1e}")
         assert ew <= 2e-3 and eb <= 2e-3
+
+
+def test_gemm_with_more_than_65535_row_tiles():
+    """Row tiles run on the grid's x dimension: 4.2 M rows (65,600 tiles of
+    64) work, checked on sampled rows."""
+    from cgfd3d_amd import surrogate as sg
+    torch = _torch()
+    M, N, K = 65600 * 64, 8, 32
+    rng = np.random.default_rng(13)
+    a = so.bf16_round(rng.standard_normal((M, K)).astype(np.float32))
+    b = so.bf16_round(rng.standard_normal((K, N)).astype(np.float32))
+    c = torch.empty((M, N), device="cuda:0")
+    sg.gemm(M, N, K, _bf(a), K, 0, _bf(b), N, 0, C32=c, ldc32=N)
+    rows = np.concatenate([np.arange(64), rng.integers(0, M, 512), np.arange(M - 64, M)])
+    got = _host(c)[rows]
+    want = a[rows].astype(np.float64) @ b.astype(np.float64)
+    assert rel_l2(got, want) <= 1e-5
