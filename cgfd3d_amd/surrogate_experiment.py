@@ -131,11 +131,11 @@ def per_receiver(args, grid, ptr, pte, seis_tr, seis_te, fd_rate, n_fd_here):
         return sd.to_target_layout(rows.reshape(len(pte), nr, nt, 3).transpose(0, 1, 3, 2)) \
             .reshape(len(pte), -1)
 
-    def fit(width):
+    def fit(width, epochs=args.epochs, lr=args.lr):
         layers = sg.default_layers(1, nt, 6, width)
         net = sg.SurrogateNet(layers, seed=args.seed, mt_terms=6, out_shape=(nt, 3))
         t = time.perf_counter()
-        hist = train(net, xtr, yd, tn2, args.epochs, args.batch * nr, args.lr, args.seed,
+        hist = train(net, xtr, yd, tn2, epochs, args.batch * nr, lr, args.seed,
                      time_budget=args.train_seconds, log_every=50, m=mtr)
         torch.cuda.synchronize()
         pred = predict(net, xte, mte, nsamp_per_call=64 * nr) / scale
@@ -145,26 +145,28 @@ def per_receiver(args, grid, ptr, pte, seis_tr, seis_te, fd_rate, n_fd_here):
         mis_tr = sd.relative_misfit(
             sd.to_target_layout(ptr_pred.reshape(ntr, nr, nt, 3).transpose(0, 1, 3, 2))
             .reshape(ntr, -1), sd.to_target_layout(seis_tr[:ntr]).reshape(ntr, -1))
-        rec = dict(width=width, train_runs=len(ptr), heldout_misfit_median=float(np.median(mis)),
+        rec = dict(width=width, epochs=epochs, lr=lr, train_runs=len(ptr),
+                   heldout_misfit_median=float(np.median(mis)),
                    heldout_misfit_mean=float(mis.mean()),
                    heldout_frac_le_5pct=float((mis <= 0.05).mean()),
                    heldout_frac_le_50pct=float((mis <= 0.5).mean()),
                    train_misfit_median=float(np.median(mis_tr)), epochs_run=len(hist),
                    train_seconds=time.perf_counter() - t,
                    final_train_loss=hist[-1] if hist else None)
-        log(f"per-receiver width {width}: held-out median {rec['heldout_misfit_median']:.3f}, "
-            f"train {rec['train_misfit_median']:.3f}")
+        log(f"per-receiver width {width}, {epochs} epochs, lr {lr}: held-out median "
+            f"{rec['heldout_misfit_median']:.3f}, train {rec['train_misfit_median']:.3f}")
         return net, rec, mis
 
     sweep = []
     for item in [v for v in args.sweep.split(";") if v]:
         kw = dict(kv.split("=") for kv in item.split(","))
-        sweep.append(fit(int(kw.get("width", args.width)))[1])
+        sweep.append(fit(int(kw.get("width", args.width)), int(kw.get("epochs", args.epochs)),
+                         float(kw.get("lr", args.lr)))[1])
     net, rec, mis = fit(args.width)
     rate, ms = inference_rate(net, xte[:256 * nr], mte[:256 * nr], nsamp=256 * nr, reps=5)
     out = {"metric": "surrogate vs FD seismogram sets/s", "form": "per-receiver decoder",
            "surrogate_sets_per_s": rate / nr, "surrogate_ms_per_256": ms,
-           "fd_sets_per_s": fd_rate, "speedup": rate / nr / fd_rate,
+           "fd_sets_per_s": fd_rate, "speedup": rate / nr / fd_rate if fd_rate else None,
            "grid": [grid.ni, grid.nj, grid.nk], "nt": nt, "nrecv": nr,
            "test_runs": len(pte), "fd_runs_this_job": n_fd_here, **rec,
            "nearest_neighbour_misfit_median": float(np.median(nn)), "sweep": sweep,
@@ -197,6 +199,8 @@ def main():
                     help="only run N training sources (seed --seed) and store them, band-"
                          "limited, in --store; prints the FD rate and the storage error")
     ap.add_argument("--store", type=str, default="")
+    ap.add_argument("--test-store", type=str, default="",
+                    help="held-out runs from a store of a --generate job instead of new FD runs")
     ap.add_argument("--extra", type=str, default="",
                     help="comma-separated stores of earlier --generate jobs, added to the "
                          "training runs")
@@ -234,19 +238,24 @@ def main():
                           "storage_misfit_max": float(err.max())}), flush=True)
         return
     ptr, ijk_tr = grid.sample_sources(args.train, args.seed)
-    pte, ijk_te = grid.sample_sources(args.test, args.seed + 1000)
     gen = sd.FdGenerator(grid, args.fd_batch, progress=True)
     t = time.perf_counter()
     seis_tr = gen.run(ptr, ijk_tr)
     log(f"{args.train} training runs in {time.perf_counter() - t:.1f} s "
         f"(device {gen.fd_seconds:.1f} s)")
-    fd_dev_s, fd_runs = gen.fd_seconds, gen.runs
-    t = time.perf_counter()
-    seis_te = gen.run(pte, ijk_te)
-    fd_dev_s, fd_runs = gen.fd_seconds, gen.runs
+    if args.test_store:
+        d = np.load(args.test_store)
+        pte = d["params"]
+        seis_te = sd.decompress_runs(d["z"], int(d["nt"]))
+        log(f"{len(pte)} held-out runs from {args.test_store} (seed {int(d['seed'])})")
+    else:
+        pte, ijk_te = grid.sample_sources(args.test, args.seed + 1000)
+        t = time.perf_counter()
+        seis_te = gen.run(pte, ijk_te)
+        log(f"{args.test} held-out runs in {time.perf_counter() - t:.1f} s")
     gen.close()
-    log(f"{args.test} held-out runs in {time.perf_counter() - t:.1f} s")
-    fd_rate = fd_runs / fd_dev_s
+    # FD rate of this job's runs (None when every run came from stores)
+    fd_rate = gen.runs / gen.fd_seconds if gen.runs else None
     n_fd_here = len(ptr)
     for path in [v for v in args.extra.split(",") if v]:
         d = np.load(path)
@@ -320,9 +329,9 @@ def main():
     out = {
         "metric": "surrogate vs FD seismogram sets/s",
         "surrogate_sets_per_s": rate, "surrogate_ms_per_256": ms,
-        "fd_sets_per_s": fd_rate, "speedup": rate / fd_rate,
+        "fd_sets_per_s": fd_rate, "speedup": rate / fd_rate if fd_rate else None,
         "grid": [grid.ni, grid.nj, grid.nk], "nt": grid.nt, "nrecv": grid.nrecv,
-        "test_runs": args.test, "fd_runs_this_job": n_fd_here, **rec,
+        "test_runs": len(pte), "fd_runs_this_job": n_fd_here, **rec,
         "nearest_neighbour_misfit_median": float(np.median(nn)),
         "subsets": subsets, "sweep": sweep,
         "params": net.n_params(), "moment_tensor_output": bool(mt),
