@@ -55,6 +55,8 @@ KERNELS = [
     ("stage_interior_phased_late", "stage_interior_phased_kernel<BY=4, combine loads last> + "
                                    "shell kernels"),
     ("stage_interior_ldsp8", "stage_interior_ldsp_kernel<TY=8> + shell kernels"),
+    ("stage_interior_phased_w3", "stage_interior_phased_kernel<BY=4, 3 waves/SIMD> + "
+                                 "shell kernels"),
 ]
 
 
@@ -201,7 +203,8 @@ def main():
                          "LDS-tiled interior (64x8 / 64x16 plane tiles) + shell, 10/11 "
                          "per-point interior marching 2 / 4 planes per block + shell, 12 "
                          "two-phase per-point interior + shell, 13 the same with the RK "
-                         "combine loads last, 14 LDS-tiled interior with phased loads + shell")
+                         "combine loads last, 14 LDS-tiled interior with phased loads + shell, "
+                         "15 variant 12 built for 3 waves per SIMD")
     ap.add_argument("--workload", default="iso", choices=["iso", "vti", "aniso", "visco", "batch"])
     ap.add_argument("--size", type=str, default=None,
                     help="grid NIxNJxNK (default 600x600x300 iso, 400x400x200 media)")

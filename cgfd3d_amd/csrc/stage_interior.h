@@ -285,8 +285,8 @@ static void launch_interior_point(hipStream_t st, const StageParams& P, BoxParam
 // barrier keeps the compiler from interleaving loads and waits, so each
 // wave has all its loads in flight at once (fewer waves, far more memory
 // requests outstanding per wave).
-template <int DX, int DY, int DZ, int KIND, int MT, int BY, bool PRE>
-__global__ void __launch_bounds__(64 * BY)
+template <int DX, int DY, int DZ, int KIND, int MT, int BY, bool PRE, int MINB = 1>
+__global__ void __launch_bounds__(64 * BY, MINB)
 stage_interior_phased_kernel(StageParams P, BoxParams B)
 {
   static_assert(MT != MEDIUM_VISCO, "elastic media only");
@@ -339,15 +339,15 @@ stage_interior_phased_kernel(StageParams P, BoxParams B)
   combine_store<KIND>(P, p, r, cpre, cend);
 }
 
-template <int KIND, int MT, int BY, bool PRE>
+template <int KIND, int MT, int BY, bool PRE, int MINB = 1>
 static void launch_interior_phased(hipStream_t st, const StageParams& P, BoxParams B, int code)
 {
   const int grid = grouped_grid(B, BY);
   dim3 blk(64, BY, 1);
 #define CGFDH_IF(c, DX, DY, DZ)                                                            \
   case c:                                                                                  \
-    hipLaunchKernelGGL((stage_interior_phased_kernel<DX, DY, DZ, KIND, MT, BY, PRE>), dim3(grid), \
-                       blk, 0, st, P, B);                                                  \
+    hipLaunchKernelGGL((stage_interior_phased_kernel<DX, DY, DZ, KIND, MT, BY, PRE, MINB>),  \
+                       dim3(grid), blk, 0, st, P, B);                                      \
     break;
   switch (code) {
     CGFDH_IF(0, 0, 0, 0) CGFDH_IF(1, 0, 0, 1) CGFDH_IF(2, 0, 1, 0) CGFDH_IF(3, 0, 1, 1)

@@ -16,9 +16,10 @@
 // marching 2 / 4 planes per block plus the shell; 12 / 13 the two-phase
 // per-point interior kernel (all loads, then the arithmetic; 13 loads the RK
 // combine inputs after the arithmetic) plus the shell; 14 the LDS-tiled
-// plane kernel (64x8) with the same phased loads plus the shell.  The VTI
-// and anisotropic media build variants 0 and 4..14, the visco-elastic medium
-// all but 1..3, 6, 7 and 12..14.
+// plane kernel (64x8) with the same phased loads plus the shell; 15 variant
+// 12 compiled for 3 waves per SIMD (a few registers spill) plus the shell.
+// The VTI and anisotropic media build variants 0 and 4..15, the
+// visco-elastic medium all but 1..3, 6, 7 and 12..15.
 #pragma once
 #include <algorithm>
 #include <cstdio>
@@ -126,7 +127,9 @@ static void launch_kind(const LaunchCtx& L, const StageParams& P, int dx, int dy
     if (in.ni > vb.ni)
       launch_box<KIND, MT, HAS_PML>(st, P, box(ilo + vb.ni, ihi, jlo, jhi, klo, khi), code);
   } else if (in.ni > 0 && in.nj > 0 && in.nk > 0) {
-    if (L.variant == 14) {
+    if (L.variant == 15) {
+      if constexpr (MT != MEDIUM_VISCO) launch_interior_phased<KIND, MT, 4, true, 3>(st, P, in, code);
+    } else if (L.variant == 14) {
       if constexpr (MT != MEDIUM_VISCO) launch_interior_ldsp<KIND, MT, 8>(st, P, in, code);
     } else if (L.variant == 12 || L.variant == 13) {
       if constexpr (MT != MEDIUM_VISCO) {

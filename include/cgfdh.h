@@ -186,8 +186,9 @@ int cgfdh_last_timing(cgfdh_solver *s, double *run_ms, double *stage_ms,
  * the interior box (64x8 / 64x16 tiles), 10 / 11 = per-point kernel
  * marching 2 / 4 planes per block, 12 / 13 = two-phase per-point kernel
  * (all loads, then the arithmetic; 13 loads the RK combine inputs last),
- * 14 = LDS-tiled plane kernel with the same phased loads; 2..14 run the
- * per-point kernel on the PML and free-surface shell.  Default 12 for the
+ * 14 = LDS-tiled plane kernel with the same phased loads, 15 = variant 12
+ * compiled for 3 waves per SIMD; 2..15 run the per-point kernel on the PML
+ * and free-surface shell.  Default 12 for the
  * elastic media, 4 for the visco-elastic medium.  Variants 1..3 exist for the
  * isotropic medium only, 6, 7 and 12..14 not for the visco-elastic medium.
  * All variants produce identical bits. */
