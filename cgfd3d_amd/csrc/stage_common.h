@@ -223,15 +223,15 @@ __device__ void vlow_ht(const StageParams& P, int i, int j, int k, long p, const
 // RK combine of its auxiliary variables
 template <int KIND, int MT>
 __device__ void pml_face(const StageParams& P, const PmlDev& F, int fslot, int i, int j, int k,
-                         long p, const float* Dd, const Medium<MT>& md, float slw, float* r)
+                         long p, const float* Dd, const Medium<MT>& md, float slw,
+                         const float* m, float* r)
 {
-  const long V = P.vol;
+  // m: the point's metric xi_x..zeta_z, already loaded by the caller
   const int ic = F.idim == 0 ? i - F.i1 : (F.idim == 1 ? j - F.j1 : k - F.k1);
   const float cD = F.D[ic], cA = F.A[ic], cB = F.B[ic];
   const float cBm1 = (float)((double)cB - 1.0);
-  const float* M = P.met;
-  const int e = 1 + 3 * F.idim;
-  const float e1 = M[e * V + p], e2 = M[(e + 1) * V + p], e3 = M[(e + 2) * V + p];
+  const int e = 3 * F.idim;
+  const float e1 = m[e], e2 = m[e + 1], e3 = m[e + 2];
   float rr[9];
   rr[VX] = slw * (e1 * Dd[TXX] + e2 * Dd[TXY] + e3 * Dd[TXZ]);
   rr[VY] = slw * (e1 * Dd[TXY] + e2 * Dd[TYY] + e3 * Dd[TYZ]);
@@ -254,7 +254,7 @@ __device__ void pml_face(const StageParams& P, const PmlDev& F, int fslot, int i
     for (int q = 0; q < 3; q++)
       Dz[q] = mm[ij + 3 * q + 0] * Dd[VX] + mm[ij + 3 * q + 1] * Dd[VY] +
               mm[ij + 3 * q + 2] * Dd[VZ];
-    const float ztx = M[7 * V + p], zty = M[8 * V + p], ztz = M[9 * V + p];
+    const float ztx = m[6], zty = m[7], ztz = m[8];
     float s[9];
     md.surf(s, ztx, zty, ztz, Dz);
 #pragma unroll
@@ -511,7 +511,7 @@ __device__ __forceinline__ void finish_point_loaded(const StageParams& P, int i,
     const PmlDev& F = P.pml[f];
     if (i >= F.i1 && i <= F.i2 && j >= F.j1 && j <= F.j2 && k >= F.k1 && k <= F.k2) {
       const float* Dd = F.idim == 0 ? X : (F.idim == 1 ? Y : Z);
-      pml_face<KIND, MT>(P, F, f, i, j, k, p, Dd, md, slw, r);
+      pml_face<KIND, MT>(P, F, f, i, j, k, p, Dd, md, slw, m, r);
     }
   }
   if constexpr (MT == MEDIUM_VISCO) atten(P, p, md.lam, md.mu, r);
