@@ -232,15 +232,22 @@ __device__ void pml_face(const StageParams& P, const PmlDev& F, int fslot, int i
   const float cBm1 = (float)((double)cB - 1.0);
   const int e = 3 * F.idim;
   const float e1 = m[e], e2 = m[e + 1], e3 = m[e + 2];
+  // this face's memory variables: the current level and the levels the RK
+  // combine reads, requested before the arithmetic
+  const long ia = (long)(i - F.i1) + (long)(j - F.j1) * F.fnx + (long)(k - F.k1) * F.fnx * F.fny;
+  float a[9], apre[9], aend[9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    a[c] = ld_stream(F.aux_cur + c * F.fn + ia);
+    if (KIND == KIND_FIRST || KIND == KIND_MID) apre[c] = ld_stream(F.aux_pre + c * F.fn + ia);
+    if (KIND == KIND_MID || KIND == KIND_LAST) aend[c] = ld_stream(F.aux_end + c * F.fn + ia);
+  }
   float rr[9];
   rr[VX] = slw * (e1 * Dd[TXX] + e2 * Dd[TXY] + e3 * Dd[TXZ]);
   rr[VY] = slw * (e1 * Dd[TXY] + e2 * Dd[TYY] + e3 * Dd[TYZ]);
   rr[VZ] = slw * (e1 * Dd[TXZ] + e2 * Dd[TYZ] + e3 * Dd[TZZ]);
   md.one_dir(rr, e1, e2, e3, Dd);
-  const long ia = (long)(i - F.i1) + (long)(j - F.j1) * F.fnx + (long)(k - F.k1) * F.fnx * F.fny;
-  float a[9], ah[9];
-#pragma unroll
-  for (int c = 0; c < 9; c++) a[c] = ld_stream(F.aux_cur + c * F.fn + ia);
+  float ah[9];
 #pragma unroll
   for (int c = 0; c < 9; c++) {
     r[c] += cBm1 * rr[c] - cB * a[c];
@@ -269,24 +276,18 @@ __device__ void pml_face(const StageParams& P, const PmlDev& F, int fslot, int i
   } else if (KIND == KIND_FIRST) {
 #pragma unroll
     for (int c = 0; c < 9; c++) {
-      const float pre = ld_stream(F.aux_pre + c * F.fn + ia);
-      st_stream(F.aux_tmp + c * F.fn + ia, pre + P.ca * ah[c]);
-      st_stream(F.aux_end + c * F.fn + ia, pre + P.cb * ah[c]);
+      st_stream(F.aux_tmp + c * F.fn + ia, apre[c] + P.ca * ah[c]);
+      st_stream(F.aux_end + c * F.fn + ia, apre[c] + P.cb * ah[c]);
     }
   } else if (KIND == KIND_MID) {
 #pragma unroll
     for (int c = 0; c < 9; c++) {
-      const float pre = ld_stream(F.aux_pre + c * F.fn + ia);
-      const float end = ld_stream(F.aux_end + c * F.fn + ia);
-      st_stream(F.aux_tmp + c * F.fn + ia, pre + P.ca * ah[c]);
-      st_stream(F.aux_end + c * F.fn + ia, end + P.cb * ah[c]);
+      st_stream(F.aux_tmp + c * F.fn + ia, apre[c] + P.ca * ah[c]);
+      st_stream(F.aux_end + c * F.fn + ia, aend[c] + P.cb * ah[c]);
     }
   } else {
 #pragma unroll
-    for (int c = 0; c < 9; c++) {
-      const float end = ld_stream(F.aux_end + c * F.fn + ia);
-      st_stream(F.aux_end + c * F.fn + ia, end + P.cb * ah[c]);
-    }
+    for (int c = 0; c < 9; c++) st_stream(F.aux_end + c * F.fn + ia, aend[c] + P.cb * ah[c]);
   }
 }
 
