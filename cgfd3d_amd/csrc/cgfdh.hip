@@ -205,6 +205,8 @@ struct cgfdh_solver {
   float* recv_w = nullptr;
   float* seismo = nullptr;
   float* pg = nullptr;           // 15 peak maps + 3 displacement maps, [ny][nx] each
+  float* outbuf = nullptr;       // staging for packed output sub-volumes (grown on demand)
+  size_t outbuf_n = 0;
   int variant = 12;              // set per medium in cgfdh_create
   int tgx = 0, tgy = 0;          // tile group (0: the default)
   bool metric_set = false, medium_set = false, mats_set = false;
@@ -243,7 +245,7 @@ static void free_all(cgfdh_solver* s)
     for (auto& a : f.aux) F(a);
   }
   F(s->src); F(s->mij); F(s->src_coef);
-  F(s->recv_idx); F(s->recv_w); F(s->seismo); F(s->pg);
+  F(s->recv_idx); F(s->recv_w); F(s->seismo); F(s->pg); F(s->outbuf);
 }
 
 extern "C" int cgfdh_create(const cgfdh_desc* d, cgfdh_solver** out)
@@ -939,6 +941,148 @@ extern "C" int cgfdh_get_peak_ground_motion(cgfdh_solver* s, float* pg, float* d
   if (pg) HIPCHK(hipMemcpy(pg, s->pg, 15 * s->ss * sizeof(float), hipMemcpyDeviceToHost));
   if (dis)
     HIPCHK(hipMemcpy(dis, s->pg + 15 * s->ss, 3 * s->ss * sizeof(float), hipMemcpyDeviceToHost));
+  return CGFDH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// output sub-volumes: snapshots and slices packed on the device, so only the
+// packed points cross PCIe.  box = {i1, ni, di, j1, nj, dj, k1, nk, dk} in
+// array indices (ghosts included); the packed order is i fastest, then j,
+// then k, as io_snap_pack_buff (io_funcs.c:1525-1556) writes.
+// ---------------------------------------------------------------------------
+
+struct SubBox {
+  int i1, ni, di, j1, nj, dj, k1, nk, dk;
+};
+struct CompList {
+  int c[NW_MAX];
+};
+
+__device__ inline long subbox_point(const SubBox& b, long t, int nx, long ss)
+{
+  const long r = t / b.ni;
+  const int n1 = (int)(t - r * b.ni);
+  const int n3 = (int)(r / b.nj);
+  const int n2 = (int)(r - (long)n3 * b.nj);
+  return (long)(b.i1 + n1 * b.di) + (long)(b.j1 + n2 * b.dj) * nx + (long)(b.k1 + n3 * b.dk) * ss;
+}
+
+// out[c][t] = w[comp c][point t of the box]; blockIdx.y is the component
+__global__ void subvolume_kernel(const float* __restrict__ w, long vol, int nx, long ss, SubBox b,
+                                 long npt, CompList cl, float* __restrict__ out)
+{
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npt) return;
+  const int c = blockIdx.y;
+  out[(long)c * npt + t] = w[(long)cl.c[c] * vol + subbox_point(b, t, nx, ss)];
+}
+
+// strains from the stresses and the isotropic lambda, mu, with the rounding
+// of io_snap_stress_to_strain_eliso (io_funcs.c:1456-1522): the compliance
+// factors in double rounded to float, the normal strains in float, the
+// shear strains as 0.5 * E3 * T in double.  Output order Exx, Eyy, Ezz,
+// Exz, Eyz, Exy: the order of the strain variables io_snap_nc_put writes
+// (io_funcs.c:1226-1252).
+__global__ void strain_kernel(const float* __restrict__ w, const float* __restrict__ med, long vol,
+                              int nx, long ss, SubBox b, long npt, float* __restrict__ out)
+{
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npt) return;
+  const long p = subbox_point(b, t, nx, ss);
+  const float lam = med[p], mu = med[vol + p];
+  const float Txx = w[3 * vol + p], Tyy = w[4 * vol + p], Tzz = w[5 * vol + p];
+  const float Tyz = w[6 * vol + p], Txz = w[7 * vol + p], Txy = w[8 * vol + p];
+  const double den = 3.0 * (double)lam + 2.0 * (double)mu;
+  const float E1 = (float)((double)(lam + mu) / ((double)mu * den));
+  const float E2 = (float)((double)(-lam) / (2.0 * (double)mu * den));
+  const float E3 = (float)(1.0 / (double)mu);
+  const float E0 = E2 * (Txx + Tyy + Tzz);
+  const float d = E2 - E1;
+  out[t] = E0 - d * Txx;
+  out[npt + t] = E0 - d * Tyy;
+  out[2 * npt + t] = E0 - d * Tzz;
+  out[3 * npt + t] = (float)(0.5 * (double)E3 * (double)Txz);
+  out[4 * npt + t] = (float)(0.5 * (double)E3 * (double)Tyz);
+  out[5 * npt + t] = (float)(0.5 * (double)E3 * (double)Txy);
+}
+
+static int check_box(cgfdh_solver* s, const int* box, const char* who, SubBox* b, long* npt)
+{
+  if (!box) return fail(CGFDH_EINVAL, "%s: null box", who);
+  const int dims[3] = {s->nx, s->ny, s->nz};
+  for (int a = 0; a < 3; a++) {
+    const int o = box[3 * a], n = box[3 * a + 1], d = box[3 * a + 2];
+    if (n < 1 || d < 1 || o < 0 || (long)o + (long)(n - 1) * d >= dims[a])
+      return fail(CGFDH_EINVAL, "%s: box axis %d (start %d, count %d, stride %d) outside 0..%d",
+                  who, a, o, n, d, dims[a] - 1);
+  }
+  *b = SubBox{box[0], box[1], box[2], box[3], box[4], box[5], box[6], box[7], box[8]};
+  *npt = (long)b->ni * b->nj * b->nk;
+  return CGFDH_OK;
+}
+
+static int outbuf_reserve(cgfdh_solver* s, size_t n)
+{
+  if (n <= s->outbuf_n) return CGFDH_OK;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (s->outbuf) HIPCHK(hipFree(s->outbuf));
+  s->outbuf = nullptr;
+  s->outbuf_n = 0;
+  HIPCHK(hipMalloc(&s->outbuf, n * sizeof(float)));
+  s->outbuf_n = n;
+  return CGFDH_OK;
+}
+
+static dim3 subvolume_grid(long npt, int ny)
+{
+  return dim3((unsigned)((npt + 255) / 256), (unsigned)ny, 1);
+}
+
+extern "C" int cgfdh_get_subvolume(cgfdh_solver* s, const int* box, int ncomp, const int* comps,
+                                   float* out)
+{
+  if (!s || !comps || !out) return fail(CGFDH_EINVAL, "cgfdh_get_subvolume: null argument");
+  if (ncomp < 1 || ncomp > s->nw)
+    return fail(CGFDH_EINVAL, "cgfdh_get_subvolume: ncomp %d outside 1..%d", ncomp, s->nw);
+  CompList cl{};
+  for (int c = 0; c < ncomp; c++) {
+    if (comps[c] < 0 || comps[c] >= s->nw)
+      return fail(CGFDH_EINVAL, "cgfdh_get_subvolume: component %d outside 0..%d", comps[c],
+                  s->nw - 1);
+    cl.c[c] = comps[c];
+  }
+  SubBox b;
+  long npt;
+  int rc = check_box(s, box, "cgfdh_get_subvolume", &b, &npt);
+  if (!rc) rc = outbuf_reserve(s, (size_t)ncomp * npt);
+  if (rc) return rc;
+  hipLaunchKernelGGL(subvolume_kernel, subvolume_grid(npt, ncomp), dim3(256), 0, s->stream,
+                     s->lvl[0], s->vol, s->nx, s->ss, b, npt, cl, s->outbuf);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, s->outbuf, (size_t)ncomp * npt * sizeof(float),
+                        hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_get_strain_subvolume(cgfdh_solver* s, const int* box, float* out)
+{
+  if (!s || !out) return fail(CGFDH_EINVAL, "cgfdh_get_strain_subvolume: null argument");
+  if (s->d.medium_type != CGFDH_MEDIUM_ELASTIC_ISO && s->d.medium_type != CGFDH_MEDIUM_VISCO_ISO)
+    return fail(CGFDH_ESTATE, "cgfdh_get_strain_subvolume: needs an isotropic lambda, mu "
+                "(medium_type %d)", s->d.medium_type);
+  if (!s->medium_set) return fail(CGFDH_ESTATE, "cgfdh_get_strain_subvolume: medium not set");
+  SubBox b;
+  long npt;
+  int rc = check_box(s, box, "cgfdh_get_strain_subvolume", &b, &npt);
+  if (!rc) rc = outbuf_reserve(s, 6 * (size_t)npt);
+  if (rc) return rc;
+  hipLaunchKernelGGL(strain_kernel, subvolume_grid(npt, 1), dim3(256), 0, s->stream, s->lvl[0],
+                     s->med, s->vol, s->nx, s->ss, b, npt, s->outbuf);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, s->outbuf, 6 * (size_t)npt * sizeof(float), hipMemcpyDeviceToHost,
+                        s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
   return CGFDH_OK;
 }
 

@@ -7,10 +7,14 @@
  *
  * Covered: isotropic, VTI and general anisotropic elastic media and the GMB
  * visco-elastic medium, CFS-PML faces, traction-image free surface, point
- * and Gaussian moment sources, receivers.  Anything else the par file asks
- * for (acoustic medium, Graves Qs, force sources, dd sources, receiver lines,
- * slices, snapshots, ablexp, several MPI ranks) stops the run with a message,
- * the reference's fatal-error convention.
+ * and Gaussian moment sources, receivers, slices, snapshots (velocity,
+ * stress, strain) and the peak-ground-motion maps.  Slices and snapshots
+ * are packed on the GPU; only their points are copied into the host
+ * wavefield that the reference's own writers (io_slice_nc_put,
+ * io_snap_nc_put) then read, so the files are the reference's.  Anything
+ * else the par file asks for (acoustic medium, Graves Qs, force sources, dd
+ * sources, receiver lines, ablexp, output_all dumps, several MPI ranks)
+ * stops the run with a message, the reference's fatal-error convention.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -27,6 +31,34 @@ static void cgfdh_fatal(const char *what, int rc)
 }
 
 #define CHK(call) do { int rc_ = (call); if (rc_) cgfdh_fatal(#call, rc_); } while (0)
+
+/* copy the packed box {i1,ni,di, j1,nj,dj, k1,nk,dk} of components comps
+ * from the engine into the host wavefield w4d at the same points */
+static void fill_box(cgfdh_solver *s, const gd_t *gd, size_t siz_icmp, float *w4d,
+                     const int box[9], int ncomp, const int *comps, float *pack)
+{
+  CHK(cgfdh_get_subvolume(s, box, ncomp, comps, pack));
+  size_t t = 0;
+  for (int c = 0; c < ncomp; c++) {
+    float *dst = w4d + (size_t)comps[c] * siz_icmp;
+    for (int n3 = 0; n3 < box[7]; n3++)
+      for (int n2 = 0; n2 < box[4]; n2++)
+        for (int n1 = 0; n1 < box[1]; n1++, t++) {
+          size_t i = box[0] + (size_t)n1 * box[2];
+          size_t j = box[3] + (size_t)n2 * box[5];
+          size_t k = box[6] + (size_t)n3 * box[8];
+          dst[i + j * gd->siz_iy + k * gd->siz_iz] = pack[t];
+        }
+  }
+}
+
+/* the condition under which io_snap_nc_put writes snapshot n at step it
+ * (io_funcs.c:1148-1154) */
+static int snap_due(const iosnap_t *sn, int n, int it, int nt_total)
+{
+  int it1 = sn->it1[n], dit = sn->dit[n];
+  return it >= it1 && (it - it1) % dit == 0 && (it - it1) / dit <= (nt_total - it1) / dit;
+}
 
 void
 drv_rk_curv_col_allstep(
@@ -60,8 +92,7 @@ drv_rk_curv_col_allstep(
        mt != CONST_MEDIUM_ELASTIC_ANISO && !visco) ||
       (visco && md->visco_type != CONST_VISCO_GMB) || src->force_actived ||
       src->dd_is_valid || ioline->num_of_lines > 0 || bdry->is_enable_ablexp ||
-      ioslice->num_of_slice_x + ioslice->num_of_slice_y + ioslice->num_of_slice_z > 0 ||
-      iosnap->num_of_snap > 0 || output_all) {
+      output_all) {
     fprintf(stderr, "ERROR: par file asks for a feature the GPU time loop lacks\n");
     MPI_Abort(MPI_COMM_WORLD, 1);
   }
@@ -153,7 +184,75 @@ drv_rk_curv_col_allstep(
     free(idx); free(dijk);
   }
 
-  CHK(cgfdh_run(s, 0, nt_total));
+  /* slices every step, snapshots at their own steps (drv_rk_curv_col.c:503-509):
+   * the engine runs up to each output step, then the packed points go into
+   * level 0 of the reference's host wavefield (wav->v5d), level 1 serving as
+   * the writers' buffer as w_rhs does in the reference */
+  int *topoid = mympi->topoid;
+  ioslice_nc_t ioslice_nc;
+  io_slice_nc_create(ioslice, wav->ncmp, wav->visco_type, wav->cmp_name, gd->ni, gd->nj,
+                     gd->nk, topoid, &ioslice_nc);
+  iosnap_nc_t iosnap_nc;
+  io_snap_nc_create(iosnap, &iosnap_nc, topoid);
+  int nslice[3] = {ioslice->num_of_slice_x, ioslice->num_of_slice_y, ioslice->num_of_slice_z};
+  int has_slice = nslice[0] + nslice[1] + nslice[2] > 0;
+  size_t pack_n = 9 * (size_t)gd->nx * gd->ny;
+  if (9 * (size_t)gd->nx * gd->nz > pack_n) pack_n = 9 * (size_t)gd->nx * gd->nz;
+  if (9 * (size_t)gd->ny * gd->nz > pack_n) pack_n = 9 * (size_t)gd->ny * gd->nz;
+  for (int n = 0; n < iosnap->num_of_snap; n++) {
+    size_t m = 6 * (size_t)iosnap->ni[n] * iosnap->nj[n] * iosnap->nk[n];
+    if (m > pack_n) pack_n = m;
+  }
+  float *pack = (has_slice || iosnap->num_of_snap > 0) ? malloc(pack_n * sizeof(float)) : NULL;
+  float *w4d = wav->v5d;
+  float *wbuf = wav->v5d + wav->siz_ilevel;
+  const int cmp9[9] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
+  int it_run = 0;
+  for (int it = 0; it < nt_total; it++) {
+    int due = has_slice;
+    for (int n = 0; n < iosnap->num_of_snap && !due; n++) due = snap_due(iosnap, n, it, nt_total);
+    if (!due) continue;
+    CHK(cgfdh_run(s, it_run, it + 1 - it_run));
+    it_run = it + 1;
+    float t_cur = it * dt + t0;
+    float t_end = t_cur + dt;
+    /* slices: components 0..8 (all of them elastic, the first 9 for GMB) */
+    for (int n = 0; n < nslice[0]; n++) {
+      int box[9] = {ioslice->slice_x_indx[n], 1, 1, gd->nj1, gd->nj, 1, gd->nk1, gd->nk, 1};
+      fill_box(s, gd, wav->siz_icmp, w4d, box, 9, cmp9, pack);
+    }
+    for (int n = 0; n < nslice[1]; n++) {
+      int box[9] = {gd->ni1, gd->ni, 1, ioslice->slice_y_indx[n], 1, 1, gd->nk1, gd->nk, 1};
+      fill_box(s, gd, wav->siz_icmp, w4d, box, 9, cmp9, pack);
+    }
+    for (int n = 0; n < nslice[2]; n++) {
+      int box[9] = {gd->ni1, gd->ni, 1, gd->nj1, gd->nj, 1, ioslice->slice_z_indx[n], 1, 1};
+      fill_box(s, gd, wav->siz_icmp, w4d, box, 9, cmp9, pack);
+    }
+    /* snapshots: velocities, and the stresses the stress and strain outputs read */
+    for (int n = 0; n < iosnap->num_of_snap; n++) {
+      if (!snap_due(iosnap, n, it, nt_total)) continue;
+      int box[9] = {iosnap->i1[n], iosnap->ni[n], iosnap->di[n], iosnap->j1[n], iosnap->nj[n],
+                    iosnap->dj[n], iosnap->k1[n], iosnap->nk[n], iosnap->dk[n]};
+      if (iosnap->out_vel[n]) fill_box(s, gd, wav->siz_icmp, w4d, box, 3, cmp9, pack);
+      if (iosnap->out_stress[n] || iosnap->out_strain[n])
+        fill_box(s, gd, wav->siz_icmp, w4d, box, 6, cmp9 + 3, pack);
+    }
+    io_slice_nc_put(ioslice, &ioslice_nc, gd, w4d, wbuf, it, t_end, 0, wav->ncmp - 1,
+                    wav->visco_type);
+    io_snap_nc_put(iosnap, &iosnap_nc, gd, md, wav, w4d, wbuf, nt_total, it, t_end, 1, 1, 1);
+  }
+  if (it_run < nt_total) CHK(cgfdh_run(s, it_run, nt_total - it_run));
+  free(pack);
+  io_slice_nc_close(&ioslice_nc);
+  io_snap_nc_close(&iosnap_nc);
+
+  if (d.free_top) {   /* PG_slice_output (drv_rk_curv_col.c:546-550): 15 maps [ny][nx] */
+    float *pg = malloc(15 * (size_t)gd->nx * gd->ny * sizeof(float));
+    CHK(cgfdh_get_peak_ground_motion(s, pg, NULL));
+    PG_slice_output(pg, gd, output_dir, output_fname_part, topoid);
+    free(pg);
+  }
 
   if (nr > 0) {   /* iorecv->recvone[r].seismo[icmp*max_nt + it] */
     int nc = wav->ncmp;            /* 9, or 9 + 6 nmaxwell (wav_t.c:26-33) */

@@ -178,6 +178,29 @@ class Solver:
         check(lib().cgfdh_get_peak_ground_motion(self.h, _p(pg), _p(dis)), "peak_ground_motion")
         return pg, dis
 
+    # ---- output sub-volumes (snapshots, slices) ----
+    @staticmethod
+    def _box(box):
+        b = np.ascontiguousarray(np.asarray(box, dtype=np.int32).reshape(9))
+        return b, (int(b[1]), int(b[4]), int(b[7]))
+
+    def subvolume(self, box, comps):
+        """Strided box (i1, ni, di, j1, nj, dj, k1, nk, dk) of the given
+        wavefield components, packed on the device: (ncomp, nk, nj, ni)."""
+        b, (ni, nj, nk) = self._box(box)
+        c = np.ascontiguousarray(np.asarray(comps, dtype=np.int32).reshape(-1))
+        out = np.empty((max(c.size, 1), max(nk, 1), max(nj, 1), max(ni, 1)), np.float32)
+        check(lib().cgfdh_get_subvolume(self.h, _p(b), int(c.size), _p(c), _p(out)),
+              "get_subvolume")
+        return out
+
+    def strain_subvolume(self, box):
+        """Exx, Eyy, Ezz, Exz, Eyz, Exy of the box: (6, nk, nj, ni)."""
+        b, (ni, nj, nk) = self._box(box)
+        out = np.empty((6, max(nk, 1), max(nj, 1), max(ni, 1)), np.float32)
+        check(lib().cgfdh_get_strain_subvolume(self.h, _p(b), _p(out)), "get_strain_subvolume")
+        return out
+
     def onestage(self, it, ipair, istage, w_cur, aux_cur=None):
         w = _f32(w_cur)
         rhs = np.empty(self.ncomp * self.vol, np.float32)

@@ -166,6 +166,26 @@ int cgfdh_get_seismograms(cgfdh_solver *s, float *seismo);
  * dis = the 3 accumulated displacement maps.  Either pointer may be NULL. */
 int cgfdh_get_peak_ground_motion(cgfdh_solver *s, float *pg, float *dis);
 
+/* Output sub-volumes of the wavefield after the last step, packed on the
+ * device so that only the packed points are copied to the host.
+ * box = {i1, ni, di, j1, nj, dj, k1, nk, dk}: start, count and stride per
+ * axis in array indices (ghosts included), every point inside the array.
+ *
+ * cgfdh_get_subvolume: out[c][n3][n2][n1] = component comps[c] at
+ * (i1 + n1*di, j1 + n2*dj, k1 + n3*dk), i fastest: io_snap_pack_buff
+ * (io_funcs.c:1525-1556) for each velocity/stress snapshot component
+ * (io_funcs.c:1162-1222), and the x/y/z slices of io_slice_nc_put
+ * (io_funcs.c:1022-1108) as boxes one point thick with unit strides.
+ *
+ * cgfdh_get_strain_subvolume: out [6][nk][nj][ni] = Exx, Eyy, Ezz, Exz,
+ * Eyz, Exy from the stresses and lambda, mu, with the rounding of
+ * io_snap_stress_to_strain_eliso (io_funcs.c:1456-1522), in the order
+ * io_snap_nc_put writes the strain variables (io_funcs.c:1226-1252).
+ * Isotropic elastic and visco-elastic media only. */
+int cgfdh_get_subvolume(cgfdh_solver *s, const int *box, int ncomp, const int *comps,
+                        float *out);
+int cgfdh_get_strain_subvolume(cgfdh_solver *s, const int *box, float *out);
+
 /* One RHS evaluation without the RK combine (sv_curv_col_el_iso_onestage):
  * rhs and aux_rhs receive what the reference leaves in w_rhs and the PML
  * aux rhs level.  Host buffers; for parity checks and drop-in use. */
