@@ -48,6 +48,8 @@ SIGNATURES = [
     ("cgfdh_get_peak_ground_motion", C.c_int, [_P, _P, _P]),
     ("cgfdh_get_subvolume", C.c_int, [_P, _P, C.c_int, _P, _P]),
     ("cgfdh_get_strain_subvolume", C.c_int, [_P, _P, _P]),
+    ("cgfdh_estimate_dt", C.c_int, [C.c_int, C.c_int, C.c_int, _P, C.c_int, _P, C.c_int,
+                                    C.c_float, _P, _P, _P, _P]),
     ("cgfdh_onestage", C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
     ("cgfdh_last_timing", C.c_int, [_P, _P, _P, _P]),
     ("cgfdh_set_kernel_variant", C.c_int, [_P, C.c_int]),

@@ -1087,6 +1087,204 @@ extern "C" int cgfdh_get_strain_subvolume(cgfdh_solver* s, const int* box, float
 }
 
 // ---------------------------------------------------------------------------
+// stable time step, blk_dt_esti_curv (blk_t.c:2840-2930): per physical point
+// cfl / Vp * L, L the smallest distance from the point to the 8 planes through
+// its diagonal neighbour triples (fdlib_math_dist_point2plane,
+// fdlib_math.c:52-89), float arithmetic with the reference's double sqrt and
+// literals.  The minimum and its point come from one 64-bit key per point,
+// (dt bits << 32) | iptr: non-negative floats order like their bits, and the
+// smaller iptr wins a tie, which is the point the reference's k-j-i loop
+// keeps (it replaces only on a strictly smaller value).
+// ---------------------------------------------------------------------------
+
+__device__ inline float dist_point2plane(const float* p0, const float* p1, const float* p2,
+                                         const float* p3)
+{
+  float a[3], b[3], n[3];
+  for (int c = 0; c < 3; c++) {
+    a[c] = p2[c] - p1[c];
+    b[c] = p3[c] - p1[c];
+  }
+  n[0] = a[1] * b[2] - a[2] * b[1];
+  n[1] = a[2] * b[0] - a[0] * b[2];
+  n[2] = a[0] * b[1] - a[1] * b[0];
+  float d = 0.0f, e = 0.0f, nn = 0.0f;
+  for (int c = 0; c < 3; c++) d += n[c] * p1[c];
+  for (int c = 0; c < 3; c++) e += n[c] * p0[c];
+  for (int c = 0; c < 3; c++) nn += n[c] * n[c];
+  const float L = fabsf(e - d);
+  return L / (float)sqrt((double)nn);     // sqrtf, correctly rounded
+}
+
+struct DtGrid {
+  const float *x, *y, *z, *vols;
+  long vol, ss;
+  int nx, ni, nj, nk, mt, nmaxwell;
+  float cfl;
+};
+
+__device__ inline void dt_at(const DtGrid& G, long p, float* dt, float* vp_out, float* L_out)
+{
+  const float* v = G.vols;
+  const long V = G.vol;
+  float vp;
+  if (G.mt == CGFDH_MEDIUM_ELASTIC_VTI) {
+    const float vpv = (float)sqrt((double)(v[V + p] / v[2 * V + p]));
+    const float vph = (float)sqrt((double)(v[p] / v[2 * V + p]));
+    vp = vph > vpv ? vph : vpv;
+  } else if (G.mt == CGFDH_MEDIUM_ELASTIC_ANISO) {
+    vp = (float)sqrt((double)(v[p] / v[V + p]));
+  } else {
+    vp = (float)sqrt(((double)v[p] + 2.0 * (double)v[V + p]) / (double)v[2 * V + p]);
+    if (G.mt == CGFDH_MEDIUM_VISCO_ISO) {
+      const float* y = v + 3 * V;       // Ylam_n, Ymu_n interleaved per body
+      float ymax = y[p];
+      for (int n = 0; n < G.nmaxwell; n++) {
+        const float yl = y[2 * n * V + p], ym = y[(2 * n + 1) * V + p];
+        ymax = ymax > yl ? ymax : yl;
+        ymax = ymax > ym ? ymax : ym;
+      }
+      const float vvis = (float)((double)vp * (1.0 + sqrt((double)((float)G.nmaxwell * ymax))));
+      vp = vvis > vp ? vvis : vp;
+    }
+  }
+  float L = 1.0e20f;
+  const float p0[3] = {G.x[p], G.y[p], G.z[p]};
+  for (int kk = -1; kk <= 1; kk += 2)
+    for (int jj = -1; jj <= 1; jj += 2)
+      for (int ii = -1; ii <= 1; ii += 2) {
+        const long q1 = p - ii, q2 = p - (long)jj * G.nx, q3 = p - (long)kk * G.ss;
+        const float p1[3] = {G.x[q1], G.y[q1], G.z[q1]};
+        const float p2[3] = {G.x[q2], G.y[q2], G.z[q2]};
+        const float p3[3] = {G.x[q3], G.y[q3], G.z[q3]};
+        const float l = dist_point2plane(p0, p1, p2, p3);
+        if (L > l) L = l;
+      }
+  *dt = G.cfl / vp * L;
+  *vp_out = vp;
+  *L_out = L;
+}
+
+__global__ void __launch_bounds__(256) dt_min_kernel(DtGrid G, unsigned long long* key)
+{
+  __shared__ unsigned long long red[256];
+  const long npt = (long)G.ni * G.nj * G.nk;
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  unsigned long long k = ~0ull;
+  if (t < npt) {
+    const long r = t / G.ni;
+    const int i = 3 + (int)(t - r * G.ni);
+    const int kz = 3 + (int)(r / G.nj);
+    const int j = 3 + (int)(r - (long)(kz - 3) * G.nj);
+    const long p = (long)i + (long)j * G.nx + (long)kz * G.ss;
+    float dt, vp, L;
+    dt_at(G, p, &dt, &vp, &L);
+    if (dt < 1.0e10f)   // the reference's starting value; NaN never wins
+      k = ((unsigned long long)__float_as_uint(dt) << 32) | (unsigned long long)(unsigned)p;
+  }
+  red[threadIdx.x] = k;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const unsigned long long o = red[threadIdx.x + s];
+      if (o < red[threadIdx.x]) red[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && red[0] != ~0ull) atomicMin(key, red[0]);
+}
+
+// res = {dt, vp, L} and ijk of the winning point; ijk = -1 if none
+__global__ void dt_finish_kernel(DtGrid G, const unsigned long long* key, float* res, int* ijk)
+{
+  const unsigned long long k = *key;
+  if (k == ~0ull) {
+    res[0] = 1.0e10f; res[1] = 0.0f; res[2] = 0.0f;
+    ijk[0] = ijk[1] = ijk[2] = -1;
+    return;
+  }
+  const long p = (long)(k & 0xffffffffull);
+  dt_at(G, p, &res[0], &res[1], &res[2]);
+  const long kz = p / G.ss;
+  const long rem = p - kz * G.ss;
+  ijk[2] = (int)kz;
+  ijk[1] = (int)(rem / G.nx);
+  ijk[0] = (int)(rem - (long)ijk[1] * G.nx);
+}
+
+extern "C" int cgfdh_estimate_dt(int ni, int nj, int nk, const float* xyz, int medium_type,
+                                 const float* vols, int nmaxwell, float cfl, float* dtmax,
+                                 float* vpmax, float* lmin, int* ijk)
+{
+  if (!xyz || !vols || !dtmax || !vpmax || !lmin || !ijk)
+    return fail(CGFDH_EINVAL, "cgfdh_estimate_dt: null argument");
+  if (ni < 1 || nj < 1 || nk < 1)
+    return fail(CGFDH_EINVAL, "cgfdh_estimate_dt: grid %dx%dx%d", ni, nj, nk);
+  if (!(cfl > 0.0f)) return fail(CGFDH_EINVAL, "cgfdh_estimate_dt: cfl %g", (double)cfl);
+  int nv;
+  switch (medium_type) {
+    case CGFDH_MEDIUM_ELASTIC_ISO: nv = 3; break;
+    case CGFDH_MEDIUM_ELASTIC_VTI: nv = 3; break;
+    case CGFDH_MEDIUM_ELASTIC_ANISO: nv = 2; break;
+    case CGFDH_MEDIUM_VISCO_ISO:
+      if (nmaxwell < 1 || nmaxwell > MAXWELL_MAX)
+        return fail(CGFDH_EINVAL, "cgfdh_estimate_dt: nmaxwell %d outside 1..%d", nmaxwell,
+                    MAXWELL_MAX);
+      nv = 3 + 2 * nmaxwell;
+      break;
+    default: return fail(CGFDH_EINVAL, "cgfdh_estimate_dt: medium_type %d", medium_type);
+  }
+  const int nx = ni + 6, ny = nj + 6, nz = nk + 6;
+  const long vol = (long)nx * ny * nz;
+  if (vol > 0xffffffffL)
+    return fail(CGFDH_EINVAL, "cgfdh_estimate_dt: %ld points exceed the 32-bit point index", vol);
+  float *dx = nullptr, *dv = nullptr, *dres = nullptr;
+  unsigned long long* dkey = nullptr;
+  int* dijk = nullptr;
+  int rc = CGFDH_OK;
+  const unsigned long long none = ~0ull;
+  float res[3];
+  DtGrid G;
+  const long npt = (long)ni * nj * nk;
+#define DTCHK(x)                                                                              \
+  do {                                                                                       \
+    hipError_t e_ = (x);                                                                     \
+    if (e_ != hipSuccess) {                                                                  \
+      rc = fail(CGFDH_EDEVICE, "cgfdh_estimate_dt: %s: %s", #x, hipGetErrorString(e_));          \
+      goto done;                                                                             \
+    }                                                                                        \
+  } while (0)
+  DTCHK(hipMalloc(&dx, 3 * (size_t)vol * sizeof(float)));
+  DTCHK(hipMalloc(&dv, (size_t)nv * vol * sizeof(float)));
+  DTCHK(hipMalloc(&dres, 3 * sizeof(float)));
+  DTCHK(hipMalloc(&dkey, sizeof(unsigned long long)));
+  DTCHK(hipMalloc(&dijk, 3 * sizeof(int)));
+  DTCHK(hipMemcpy(dx, xyz, 3 * (size_t)vol * sizeof(float), hipMemcpyHostToDevice));
+  DTCHK(hipMemcpy(dv, vols, (size_t)nv * vol * sizeof(float), hipMemcpyHostToDevice));
+  DTCHK(hipMemcpy(dkey, &none, sizeof none, hipMemcpyHostToDevice));
+  G = DtGrid{dx, dx + vol, dx + 2 * vol, dv, vol, (long)nx * ny, nx, ni, nj, nk, medium_type,
+             medium_type == CGFDH_MEDIUM_VISCO_ISO ? nmaxwell : 0, cfl};
+  hipLaunchKernelGGL(dt_min_kernel, dim3((unsigned)((npt + 255) / 256)), dim3(256), 0, 0, G,
+                     dkey);
+  DTCHK(hipGetLastError());
+  hipLaunchKernelGGL(dt_finish_kernel, dim3(1), dim3(1), 0, 0, G, dkey, dres, dijk);
+  DTCHK(hipGetLastError());
+  DTCHK(hipMemcpy(res, dres, sizeof res, hipMemcpyDeviceToHost));
+  DTCHK(hipMemcpy(ijk, dijk, 3 * sizeof(int), hipMemcpyDeviceToHost));
+  *dtmax = res[0];
+  *vpmax = res[1];
+  *lmin = res[2];
+#undef DTCHK
+done:
+  (void)hipFree(dx);
+  (void)hipFree(dv);
+  (void)hipFree(dres);
+  (void)hipFree(dkey);
+  (void)hipFree(dijk);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
 // batched runs: independent solvers advanced together.  A cycle of 8 steps
 // (one full operator-pair schedule, an even number of level swaps, so every
 // pointer returns to where it started) of all solvers is captured once into

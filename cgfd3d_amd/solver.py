@@ -26,6 +26,22 @@ def _f32(a):
     return np.ascontiguousarray(a, dtype=np.float32)
 
 
+def estimate_dt(xyz, medium_type, vols, cfl, nmaxwell=0):
+    """Stable time step of a curvilinear grid on the GPU (blk_dt_esti_curv,
+    blk_t.c:2840-2930).  xyz: (3, nz, ny, nx) coordinates with ghosts; vols:
+    the medium volumes cgfdh_estimate_dt takes for medium_type.  Returns
+    (dtmax, vp, L, (i, j, k))."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float32)
+    vols = np.ascontiguousarray(vols, dtype=np.float32)
+    nz, ny, nx = xyz.shape[-3:]
+    out = np.zeros(3, np.float32)
+    ijk = np.zeros(3, np.int32)
+    check(lib().cgfdh_estimate_dt(nx - 6, ny - 6, nz - 6, _p(xyz), int(medium_type), _p(vols),
+                                  int(nmaxwell), float(cfl), _p(out[0:]), _p(out[1:]),
+                                  _p(out[2:]), _p(ijk)), "estimate_dt")
+    return float(out[0]), float(out[1]), float(out[2]), tuple(int(v) for v in ijk)
+
+
 class Solver:
     """Engine for one grid block: ni x nj x nk physical points."""
 

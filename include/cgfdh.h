@@ -186,6 +186,22 @@ int cgfdh_get_subvolume(cgfdh_solver *s, const int *box, int ncomp, const int *c
                         float *out);
 int cgfdh_get_strain_subvolume(cgfdh_solver *s, const int *box, float *out);
 
+/* Stable time step of a curvilinear grid, blk_dt_esti_curv
+ * (blk_t.c:2840-2930), computed on the GPU: the minimum over physical points
+ * of cfl / Vp * L, L the smallest distance from the point to the 8 planes
+ * through its diagonal neighbour triples (fdlib_math_dist_point2plane,
+ * fdlib_math.c:71-89), Vp per medium as the reference takes it.  Host
+ * arrays, ghosts included: xyz = x3d, y3d, z3d; vols = lambda, mu, rho
+ * (isotropic), c11, c33, rho (VTI), c11, rho (anisotropic), or lambda, mu,
+ * rho, then Ylam_n, Ymu_n interleaved per Maxwell body as for
+ * cgfdh_set_medium_visco (visco-elastic, n = nmaxwell).  Outputs the
+ * reference's dtmax, dtmaxVp, dtmaxL and dtmaxi/j/k (the first point in its
+ * k-j-i loop order on a tie); if no point gives a step below 1e10 (the
+ * reference's starting value), dtmax = 1e10, Vp = L = 0 and ijk = -1. */
+int cgfdh_estimate_dt(int ni, int nj, int nk, const float *xyz, int medium_type,
+                      const float *vols, int nmaxwell, float cfl, float *dtmax,
+                      float *vpmax, float *lmin, int *ijk);
+
 /* One RHS evaluation without the RK combine (sv_curv_col_el_iso_onestage):
  * rhs and aux_rhs receive what the reference leaves in w_rhs and the PML
  * aux rhs level.  Host buffers; for parity checks and drop-in use. */

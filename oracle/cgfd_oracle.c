@@ -532,6 +532,95 @@ void orc_dvh2dvz_iso(int ni, int nj, int nk, const float *metric10,
 }
 
 /* ------------------------------------------------------------------------ */
+/* setup: stable time step, blk_dt_esti_curv (blk_t.c:2840-2930) with the   */
+/* point-to-plane distance of fdlib_math.c:52-89; float throughout except   */
+/* where the reference calls the double sqrt or writes double literals      */
+/* ------------------------------------------------------------------------ */
+
+float orc_dist_point2plane(const float x0[3], const float x1[3], const float x2[3],
+                           const float x3[3])
+{
+  float a[3], b[3], p[3];
+  for (int c = 0; c < 3; c++) {
+    a[c] = x2[c] - x1[c];
+    b[c] = x3[c] - x1[c];
+  }
+  p[0] = a[1] * b[2] - a[2] * b[1];
+  p[1] = a[2] * b[0] - a[0] * b[2];
+  p[2] = a[0] * b[1] - a[1] * b[0];
+  float d = 0.0f, e = 0.0f, n = 0.0f;
+  for (int c = 0; c < 3; c++) d += p[c] * x1[c];
+  for (int c = 0; c < 3; c++) e += p[c] * x0[c];
+  for (int c = 0; c < 3; c++) n += p[c] * p[c];
+  float L = (float)fabs(e - d);
+  return L / sqrtf(n);
+}
+
+/* vols by medium: iso and visco lambda, mu, rho (visco then Ylam_n, Ymu_n
+ * interleaved per Maxwell body, the md_t order);
+ * VTI c11, c33, rho; aniso c11, rho.  ijk = -1 and vp = L = 0 if no point
+ * beats the reference's starting value 1e10. */
+void orc_dt_estimate(int ni, int nj, int nk, const float *xyz, int medium_type,
+                     const float *vols, int nmaxwell, float cfl, float *dtmax,
+                     float *vpmax, float *lmin, int *ijk)
+{
+  size_t nx = ni + 6, ny = nj + 6, nz = nk + 6;
+  size_t vol = nx * ny * nz, sy = nx, sz = nx * ny;
+  const float *x = xyz, *y = xyz + vol, *z = xyz + 2 * vol;
+  float best = 1.0e10f;
+  ijk[0] = ijk[1] = ijk[2] = -1;
+  *vpmax = 0.0f;
+  *lmin = 0.0f;
+  for (int k = 3; k < 3 + nk; k++)
+    for (int j = 3; j < 3 + nj; j++)
+      for (int i = 3; i < 3 + ni; i++) {
+        size_t p = i + j * sy + k * sz;
+        float vp;
+        if (medium_type == ORC_MEDIUM_VTI) {
+          float vpv = (float)sqrt(vols[vol + p] / vols[2 * vol + p]);
+          float vph = (float)sqrt(vols[p] / vols[2 * vol + p]);
+          vp = vph > vpv ? vph : vpv;
+        } else if (medium_type == ORC_MEDIUM_ANISO) {
+          vp = (float)sqrt(vols[p] / vols[vol + p]);
+        } else {
+          vp = (float)sqrt(((double)vols[p] + 2.0 * (double)vols[vol + p]) /
+                           (double)vols[2 * vol + p]);
+          if (medium_type == ORC_MEDIUM_VISCO) {
+            const float *yv = vols + 3 * vol;   /* Ylam_n, Ymu_n interleaved per body */
+            float ymax = yv[p];
+            for (int n = 0; n < nmaxwell; n++) {
+              float yl = yv[2 * n * vol + p], ym = yv[(2 * n + 1) * vol + p];
+              ymax = ymax > yl ? ymax : yl;
+              ymax = ymax > ym ? ymax : ym;
+            }
+            float vvis = (float)((double)vp * (1.0 + sqrt((double)(nmaxwell * ymax))));
+            vp = vvis > vp ? vvis : vp;
+          }
+        }
+        float L = 1.0e20f;
+        float p0[3] = {x[p], y[p], z[p]};
+        for (int kk = -1; kk <= 1; kk += 2)
+          for (int jj = -1; jj <= 1; jj += 2)
+            for (int ii = -1; ii <= 1; ii += 2) {
+              size_t q1 = p - ii, q2 = p - jj * sy, q3 = p - kk * sz;
+              float p1[3] = {x[q1], y[q1], z[q1]};
+              float p2[3] = {x[q2], y[q2], z[q2]};
+              float p3[3] = {x[q3], y[q3], z[q3]};
+              float l = orc_dist_point2plane(p0, p1, p2, p3);
+              if (L > l) L = l;
+            }
+        float dt = cfl / vp * L;
+        if (dt < best) {
+          best = dt;
+          ijk[0] = i; ijk[1] = j; ijk[2] = k;
+          *vpmax = vp;
+          *lmin = L;
+        }
+      }
+  *dtmax = best;
+}
+
+/* ------------------------------------------------------------------------ */
 /* setup: metric, gd_t.c:191-402 (7-point centred op, cross products,       */
 /* ghost mirroring in x1,x2,y1,y2,z1,z2 order)                              */
 /* ------------------------------------------------------------------------ */

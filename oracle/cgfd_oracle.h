@@ -106,6 +106,11 @@ void orc_visco_gmb(long npts, int nmaxwell, float freq, float fmin, float fmax,
                    float *Ylam, float *Ymu, float *wl);
 void orc_metric(int ni, int nj, int nk, const float *xyz /* 3 volumes */,
                 float *metric10);
+float orc_dist_point2plane(const float x0[3], const float x1[3], const float x2[3],
+                           const float x3[3]);
+void orc_dt_estimate(int ni, int nj, int nk, const float *xyz, int medium_type,
+                     const float *vols, int nmaxwell, float cfl, float *dtmax,
+                     float *vpmax, float *lmin, int *ijk);
 void orc_gauss_weights(float *w, float x0, float y0, float z0, float a,
                        int half, int half_below);
 float orc_ricker(float t, float fc, float t0);

@@ -2,7 +2,7 @@
  * ref_harness.c -- test-only driver around the reference's own hot-path
  * sources (fd_t.c, sv_curv_col_el.c, sv_curv_col_el_iso.c,
  * sv_curv_col_el_vti.c, sv_curv_col_el_aniso.c, sv_curv_col_vis_iso.c, bdry_t.c, wav_t.c,
- * fdlib_math.c, fdlib_mem.c), compiled unchanged from /root/reference by
+ * blk_t.c, fdlib_math.c, fdlib_mem.c), compiled unchanged from /root/reference by
  * oracle/Makefile into oracle/_ref/libcgfd3d_ref.so.
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product links or loads this.
@@ -37,6 +37,7 @@
 #include "sv_curv_col_el_vti.h"
 #include "sv_curv_col_el_aniso.h"
 #include "sv_curv_col_vis_iso.h"
+#include "blk_t.h"
 
 typedef struct {
   int ni, nj, nk;
@@ -666,4 +667,36 @@ int refh_run(const refh_cfg *cfg, float *coords, float *metric10, float *med3,
   }
   teardown(&c);
   return 0;
+}
+
+/* blk_dt_esti_curv (blk_t.c:2840-2930) itself on filled gd_t / md_t structs.
+ * vols: lambda, mu, rho (iso; visco then Ylam_n, Ymu_n per body), c11, c33,
+ * rho (VTI), c11, rho (aniso).  out3 = dtmax, Vp, L; ijk stays -1 when the
+ * reference keeps no point. */
+void refh_dt_esti(int ni, int nj, int nk, float *xyz, int medium_type, float *vols,
+                  int nmaxwell, float cfl, float *out3, int *ijk)
+{
+  size_t vol = (size_t)(ni + 6) * (nj + 6) * (nk + 6);
+  gd_t gd;
+  fill_gd(&gd, ni, nj, nk, xyz, xyz + vol, xyz + 2 * vol);
+  md_t md;
+  memset(&md, 0, sizeof md);
+  md.medium_type = medium_type;
+  md.nmaxwell = nmaxwell;
+  float *Ylam[16], *Ymu[16];
+  if (medium_type == CONST_MEDIUM_ELASTIC_VTI) {
+    md.c11 = vols; md.c33 = vols + vol; md.rho = vols + 2 * vol;
+  } else if (medium_type == CONST_MEDIUM_ELASTIC_ANISO) {
+    md.c11 = vols; md.rho = vols + vol;
+  } else {
+    md.lambda = vols; md.mu = vols + vol; md.rho = vols + 2 * vol;
+    for (int n = 0; n < nmaxwell && n < 16; n++) {
+      Ylam[n] = vols + (3 + 2 * (size_t)n) * vol;
+      Ymu[n] = vols + (4 + 2 * (size_t)n) * vol;
+    }
+    md.Ylam = Ylam; md.Ymu = Ymu;
+  }
+  out3[0] = out3[1] = out3[2] = 0.0f;
+  ijk[0] = ijk[1] = ijk[2] = -1;
+  blk_dt_esti_curv(&gd, &md, cfl, &out3[0], &out3[1], &out3[2], &ijk[0], &ijk[1], &ijk[2]);
 }
