@@ -428,3 +428,38 @@ def workload_case(wl):
         src.update(nsrc=1, inc=[0, 0, 0], gauss=0)
         case["src"] = src
     return case
+
+
+DT_MEDIA = ("iso", "vti", "aniso", "visco", "cartesian")
+
+
+def dt_case(medium, ni=14, nj=12, nk=10):
+    """Inputs of the stable-time-step estimate (blk_dt_esti_curv): coordinates
+    (3, nz, ny, nx), the medium name the estimate takes, its volumes in the
+    order cgfdh_estimate_dt takes them, and nmaxwell.  'cartesian' is a
+    homogeneous isotropic medium on an exact 64 m Cartesian grid, where every
+    point ties."""
+    xyz = curvilinear_coords(ni, nj, nk, seed=5, hill=0.4)
+    nmw = 0
+    if medium in ("iso", "cartesian"):
+        lam, mu, slw = iso_medium(ni, nj, nk, seed=6)
+        vols = np.stack([lam, mu, (1.0 / slw).astype(np.float32)])
+        if medium == "cartesian":
+            nx, ny, nz = dims(ni, nj, nk)
+            z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+            xyz = np.stack([x, y, z]).astype(np.float32) * np.float32(64.0)
+            vols = np.stack([np.full_like(lam, 3.0e10), np.full_like(lam, 2.5e10),
+                             np.full_like(lam, 2500.0)])
+        medium = "iso"
+    elif medium == "vti":
+        v = vti_medium(ni, nj, nk, seed=6)
+        vols = np.stack([v[0], v[2], (1.0 / v[5]).astype(np.float32)])
+    elif medium == "aniso":
+        v = aniso_medium(ni, nj, nk, seed=6)
+        vols = np.stack([v[0], (1.0 / v[21]).astype(np.float32)])
+    else:
+        v, wl = visco_medium(ni, nj, nk, seed=6)
+        vols = np.concatenate([v[:2], (1.0 / v[2]).astype(np.float32)[None], v[3:]])
+        nmw = len(wl)
+    return (np.ascontiguousarray(xyz, np.float32), medium,
+            np.ascontiguousarray(vols, np.float32), nmw)

@@ -12,6 +12,9 @@ from seeds by tests/cases.py) and the reference outputs:
   run_<name>.npz       seismograms of surface and buried receivers and the
                        final surface plane after an RK4 run with a point
                        moment source, CFS-PML and the free surface
+  dt_estimate.npz      blk_dt_esti_curv's dtmax, Vp, L and point for the
+                       inputs of tests/cases.py dt_case, every medium
+                       (`python tests/golden/make_golden.py dt` writes only it)
 The reference's surface row of hV is re-evaluated by the harness on stages
 with the forward z operator (the reference reads out of bounds there; see
 oracle/ref_harness.c).
@@ -62,9 +65,30 @@ def run_receivers(case):
     return np.concatenate([ridx, bi]), np.concatenate([rw, bw])
 
 
+def write_dt_golden():
+    ref = po.load_ref()
+    mt = {"iso": po.MEDIUM_ISO, "vti": po.MEDIUM_VTI, "aniso": po.MEDIUM_ANISO,
+          "visco": po.MEDIUM_VISCO}
+    arrays = {}
+    for name in cs.DT_MEDIA:
+        xyz, medium, vols, nmw = cs.dt_case(name)
+        nz, ny, nx = xyz.shape[1:]
+        out = np.zeros(3, np.float32)
+        ijk = np.full(3, -1, np.int32)
+        ref.refh_dt_esti(nx - 6, ny - 6, nz - 6, po.ptr(xyz), mt[medium], po.ptr(vols), nmw,
+                         1.3, po.ptr(out), po.ptr(ijk))
+        arrays[f"{name}_out"] = out
+        arrays[f"{name}_ijk"] = ijk
+        arrays[f"{name}_input_sha256"] = checksum(xyz, vols)
+    np.savez_compressed(os.path.join(HERE, "dt_estimate.npz"), **arrays)
+
+
 def main():
     if not po.ref_available():
         sys.exit("oracle/_ref/libcgfd3d_ref.so not built (needs the reference tree)")
+    if sys.argv[1:] == ["dt"]:
+        write_dt_golden()
+        return
     ref = po.load_ref()
     for name, kw in ONESTAGE_CASES.items():
         case = cs.make_case(**kw)

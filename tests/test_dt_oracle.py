@@ -108,6 +108,16 @@ def test_vti_aniso_visco_velocities():
     assert vp == pytest.approx(6000.0 * (1 + (2 * 0.04) ** 0.5), rel=1e-6)
 
 
+def _ref_dt(xyz, mt, vols, nmw, cfl):
+    ref = po.load_ref()
+    nz, ny, nx = xyz.shape[1:]
+    out = np.zeros(3, np.float32)
+    ijk = np.zeros(3, np.int32)
+    ref.refh_dt_esti(nx - 6, ny - 6, nz - 6, po.ptr(xyz), MT[mt], po.ptr(vols), nmw, cfl,
+                     po.ptr(out), po.ptr(ijk))
+    return out, ijk
+
+
 @pytest.mark.skipif(not po.ref_available(), reason="reference build absent")
 @pytest.mark.parametrize("medium", ["iso", "vti", "aniso", "visco", "cartesian"])
 def test_estimate_matches_reference(medium):
@@ -115,35 +125,25 @@ def test_estimate_matches_reference(medium):
     blk_t.c: dtmax, Vp, L bit for bit and the same point, on curvilinear grids
     for every medium and on a Cartesian grid full of ties."""
     from tests import cases as cs
-    ni, nj, nk = 14, 12, 10
-    xyz = cs.curvilinear_coords(ni, nj, nk, seed=5, hill=0.4)
-    mt, nmw = medium, 0
-    if medium in ("iso", "cartesian"):
-        lam, mu, slw = cs.iso_medium(ni, nj, nk, seed=6)
-        vols = np.stack([lam, mu, (1.0 / slw).astype(np.float32)])
-        mt = "iso"
-        if medium == "cartesian":
-            xyz = _cart(ni, nj, nk, 64.0)
-            vols = np.stack([np.full_like(lam, 3.0e10), np.full_like(lam, 2.5e10),
-                             np.full_like(lam, 2500.0)])
-    elif medium == "vti":
-        v = cs.vti_medium(ni, nj, nk, seed=6)
-        vols = np.stack([v[0], v[2], (1.0 / v[5]).astype(np.float32)])
-    elif medium == "aniso":
-        v = cs.aniso_medium(ni, nj, nk, seed=6)
-        vols = np.stack([v[0], (1.0 / v[21]).astype(np.float32)])
-    else:
-        v, wl = cs.visco_medium(ni, nj, nk, seed=6)
-        vols = np.concatenate([v[:2], (1.0 / v[2]).astype(np.float32)[None], v[3:]])
-        nmw = len(wl)
-    vols = np.ascontiguousarray(vols, np.float32)
-    xyz = np.ascontiguousarray(xyz, np.float32)
-    ref = po.load_ref()
-    out = np.zeros(3, np.float32)
-    ijk = np.zeros(3, np.int32)
-    ref.refh_dt_esti(ni, nj, nk, po.ptr(xyz), MT[mt], po.ptr(vols), nmw, 1.3, po.ptr(out),
-                     po.ptr(ijk))
+    xyz, mt, vols, nmw = cs.dt_case(medium)
+    out, ijk = _ref_dt(xyz, mt, vols, nmw, 1.3)
     got = dt_estimate(xyz, mt, vols, 1.3, nmaxwell=nmw)
     assert [np.float32(g).view(np.uint32) for g in got[:3]] == list(out.view(np.uint32))
     assert got[3] == tuple(int(v) for v in ijk)
     assert 0 < out[0] < 1
+
+
+@pytest.mark.parametrize("medium", ["iso", "vti", "aniso", "visco", "cartesian"])
+def test_estimate_matches_golden(medium):
+    """The same comparison against the reference's outputs stored in
+    tests/golden/dt_estimate.npz (written by tests/golden/make_golden.py dt)."""
+    from tests import cases as cs
+    from tests.golden.make_golden import checksum
+    g = np.load(po.os.path.join(po.os.path.dirname(po.HERE), "tests", "golden",
+                                "dt_estimate.npz"), allow_pickle=False)
+    xyz, mt, vols, nmw = cs.dt_case(medium)
+    assert checksum(xyz, vols) == str(g[f"{medium}_input_sha256"])
+    got = dt_estimate(xyz, mt, vols, 1.3, nmaxwell=nmw)
+    assert np.array_equal(np.array(got[:3], np.float32).view(np.uint32),
+                          g[f"{medium}_out"].view(np.uint32))
+    assert got[3] == tuple(int(v) for v in g[f"{medium}_ijk"])
