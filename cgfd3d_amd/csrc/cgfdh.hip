@@ -206,6 +206,9 @@ struct cgfdh_solver {
   float* seismo = nullptr;
   float* pg = nullptr;           // 15 peak maps + 3 displacement maps, [ny][nx] each
   float* outbuf = nullptr;       // staging for packed output sub-volumes (grown on demand)
+  float* abl = nullptr;          // ablexp masks Ex[nx], Ey[ny], Ez[nz]
+  int abl_blk[6][6] = {};        // enabled ablexp blocks: i1, i2, j1, j2, k1, k2
+  int abl_n = 0;
   size_t outbuf_n = 0;
   int variant = 12;              // set per medium in cgfdh_create
   int tgx = 0, tgy = 0;          // tile group (0: the default)
@@ -245,7 +248,7 @@ static void free_all(cgfdh_solver* s)
     for (auto& a : f.aux) F(a);
   }
   F(s->src); F(s->mij); F(s->src_coef);
-  F(s->recv_idx); F(s->recv_w); F(s->seismo); F(s->pg); F(s->outbuf);
+  F(s->recv_idx); F(s->recv_w); F(s->seismo); F(s->pg); F(s->outbuf); F(s->abl);
 }
 
 extern "C" int cgfdh_create(const cgfdh_desc* d, cgfdh_solver** out)
@@ -424,6 +427,34 @@ extern "C" int cgfdh_set_free_surface_matD(cgfdh_solver* s, const float* matD)
 {
   if (!s || !matD) return fail(CGFDH_EINVAL, "cgfdh_set_free_surface_matD: null argument");
   return h2d(s, s->mats + 27 * s->ss, matD, 9 * (size_t)s->ss);
+}
+
+extern "C" int cgfdh_set_ablexp(cgfdh_solver* s, const float* Ex, const float* Ey,
+                                const float* Ez, const int* blocks)
+{
+  if (!s || !Ex || !Ey || !Ez || !blocks)
+    return fail(CGFDH_EINVAL, "cgfdh_set_ablexp: null argument");
+  int blk[6][6] = {}, n = 0;
+  const int hi[3] = {s->nx - 1, s->ny - 1, s->nz - 1};
+  for (int q = 0; q < 6; q++) {
+    const int* b = blocks + 7 * q;
+    if (!b[0]) continue;
+    for (int a = 0; a < 3; a++)
+      if (b[1 + 2 * a] < 0 || b[2 + 2 * a] < b[1 + 2 * a] || b[2 + 2 * a] > hi[a])
+        return fail(CGFDH_EINVAL, "cgfdh_set_ablexp: block %d axis %d range %d..%d outside 0..%d",
+                    q, a, b[1 + 2 * a], b[2 + 2 * a], hi[a]);
+    for (int c = 0; c < 6; c++) blk[n][c] = b[1 + c];
+    n++;
+  }
+  const size_t len = (size_t)s->nx + s->ny + s->nz;
+  if (!s->abl) HIPCHK(hipMalloc(&s->abl, len * sizeof(float)));
+  int rc = h2d(s, s->abl, Ex, s->nx);
+  if (!rc) rc = h2d(s, s->abl + s->nx, Ey, s->ny);
+  if (!rc) rc = h2d(s, s->abl + s->nx + s->ny, Ez, s->nz);
+  if (rc) return rc;
+  memcpy(s->abl_blk, blk, sizeof blk);
+  s->abl_n = n;
+  return CGFDH_OK;
 }
 
 extern "C" int cgfdh_set_pml_face(cgfdh_solver* s, int f, const float* A, const float* B,
@@ -739,6 +770,42 @@ static void launch_stage(cgfdh_solver* s, hipStream_t stream, const StageParams&
 // then the pre/end swap.  ev (optional) gets a record before and after each
 // stage; it_off (optional) is a device step offset added to it, for graph
 // replays of a captured cycle.
+// ablexp sponge, bdry_ablexp_apply (bdry_t.c:840-890): every component of
+// w_end at the points of one block times min(Ex[i], Ey[j], Ez[k]); the blocks
+// are launched one after another in the reference's order, so a point that
+// two blocks share is damped twice, as there
+__global__ void ablexp_kernel(float* __restrict__ w, long vol, int nw, int nx, long ss,
+                              const float* __restrict__ Ex, const float* __restrict__ Ey,
+                              const float* __restrict__ Ez, int i1, int ni, int j1, int nj,
+                              int k1, long npt)
+{
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npt) return;
+  const long r = t / ni;
+  const int i = i1 + (int)(t - r * ni);
+  const int k = k1 + (int)(r / nj);
+  const int j = j1 + (int)(r - (long)(k - k1) * nj);
+  float mask = Ex[i] < Ey[j] ? Ex[i] : Ey[j];
+  if (mask > Ez[k]) mask = Ez[k];
+  const long p = (long)i + (long)j * nx + (long)k * ss;
+  for (int c = 0; c < nw; c++) w[c * vol + p] *= mask;
+}
+
+static void enqueue_ablexp(cgfdh_solver* s, hipStream_t st)
+{
+  const float* Ex = s->abl;
+  const float* Ey = s->abl + s->nx;
+  const float* Ez = s->abl + s->nx + s->ny;
+  for (int n = 0; n < s->abl_n; n++) {
+    const int* b = s->abl_blk[n];
+    const int ni = b[1] - b[0] + 1, nj = b[3] - b[2] + 1, nk = b[5] - b[4] + 1;
+    const long npt = (long)ni * nj * nk;
+    hipLaunchKernelGGL(ablexp_kernel, dim3((unsigned)((npt + 255) / 256)), dim3(256), 0, st,
+                       s->lvl[3], s->vol, s->nw, s->nx, s->ss, Ex, Ey, Ez, b[0], ni, b[2], nj,
+                       b[4], npt);
+  }
+}
+
 static int enqueue_step(cgfdh_solver* s, hipStream_t st, int it, const int* it_off,
                         hipEvent_t* ev)
 {
@@ -759,6 +826,7 @@ static int enqueue_step(cgfdh_solver* s, hipStream_t st, int it, const int* it_o
     hipLaunchKernelGGL(vis_free_kernel, dim3((s->d.ni + 255) / 256, s->d.nj), dim3(256), 0, st,
                        s->lvl[3], s->mats + 27 * s->ss, s->d.ni, s->d.nj, s->nx, s->ss, s->vol,
                        s->nk2);
+  if (s->abl_n) enqueue_ablexp(s, st);   // drv_rk_curv_col.c:482-486
   if (s->pg)   // drv_rk_curv_col.c:492-495, before the pre/end swap
     hipLaunchKernelGGL(pg_kernel, dim3((s->d.ni + 255) / 256, s->d.nj), dim3(256), 0, st,
                        s->lvl[3], s->lvl[0], s->pg, s->pg + 15 * s->ss, s->d.ni, s->d.nj, s->nx,

@@ -8,13 +8,13 @@
  * Covered: isotropic, VTI and general anisotropic elastic media and the GMB
  * visco-elastic medium, CFS-PML faces, traction-image free surface, point
  * and Gaussian moment sources, receivers, slices, snapshots (velocity,
- * stress, strain) and the peak-ground-motion maps.  Slices and snapshots
- * are packed on the GPU; only their points are copied into the host
- * wavefield that the reference's own writers (io_slice_nc_put,
+ * stress, strain), the peak-ground-motion maps and the ablexp sponge.
+ * Slices and snapshots are packed on the GPU; only their points are copied
+ * into the host wavefield that the reference's own writers (io_slice_nc_put,
  * io_snap_nc_put) then read, so the files are the reference's.  Anything
  * else the par file asks for (acoustic medium, Graves Qs, force sources, dd
- * sources, receiver lines, ablexp, output_all dumps, several MPI ranks)
- * stops the run with a message, the reference's fatal-error convention.
+ * sources, receiver lines, output_all dumps, several MPI ranks) stops the
+ * run with a message, the reference's fatal-error convention.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -91,8 +91,7 @@ drv_rk_curv_col_allstep(
   if ((mt != CONST_MEDIUM_ELASTIC_ISO && mt != CONST_MEDIUM_ELASTIC_VTI &&
        mt != CONST_MEDIUM_ELASTIC_ANISO && !visco) ||
       (visco && md->visco_type != CONST_VISCO_GMB) || src->force_actived ||
-      src->dd_is_valid || ioline->num_of_lines > 0 || bdry->is_enable_ablexp ||
-      output_all) {
+      src->dd_is_valid || ioline->num_of_lines > 0 || output_all) {
     fprintf(stderr, "ERROR: par file asks for a feature the GPU time loop lacks\n");
     MPI_Abort(MPI_COMM_WORLD, 1);
   }
@@ -182,6 +181,19 @@ drv_rk_curv_col_allstep(
     }
     CHK(cgfdh_set_receivers(s, nr, idx, dijk));
     free(idx); free(dijk);
+  }
+
+  if (bdry->is_enable_ablexp) {   /* masks and blocks of bdry_ablexp_set (bdry_t.c:513-820) */
+    int blk[6 * 7];
+    for (int n = 0; n < 6; n++) {
+      const bdry_block_t *D = bdry->bdry_blk + n;
+      int *b = blk + 7 * n;
+      b[0] = D->enable;
+      b[1] = D->ni1; b[2] = D->ni2;
+      b[3] = D->nj1; b[4] = D->nj2;
+      b[5] = D->nk1; b[6] = D->nk2;
+    }
+    CHK(cgfdh_set_ablexp(s, bdry->ablexp_Ex, bdry->ablexp_Ey, bdry->ablexp_Ez, blk));
   }
 
   /* slices every step, snapshots at their own steps (drv_rk_curv_col.c:503-509):

@@ -194,6 +194,14 @@ class Solver:
         check(lib().cgfdh_get_peak_ground_motion(self.h, _p(pg), _p(dis)), "peak_ground_motion")
         return pg, dis
 
+    def set_ablexp(self, Ex, Ey, Ez, blocks):
+        """ablexp sponge: masks Ex[nx], Ey[ny], Ez[nz] and 6 blocks
+        (enable, ni1, ni2, nj1, nj2, nk1, nk2), applied after every step."""
+        ex, ey, ez = _f32(Ex), _f32(Ey), _f32(Ez)
+        assert ex.size == self.nx and ey.size == self.ny and ez.size == self.nz
+        b = np.ascontiguousarray(np.asarray(blocks, dtype=np.int32).reshape(6, 7))
+        check(lib().cgfdh_set_ablexp(self.h, _p(ex), _p(ey), _p(ez), _p(b)), "set_ablexp")
+
     # ---- output sub-volumes (snapshots, slices) ----
     @staticmethod
     def _box(box):

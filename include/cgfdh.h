@@ -125,6 +125,16 @@ int cgfdh_set_free_surface(cgfdh_solver *s, const float *matVx2Vz2,
 int cgfdh_set_free_surface_matD(cgfdh_solver *s, const float *matD);
 int cgfdh_get_free_surface_matD(cgfdh_solver *s, float *matD);
 
+/* The ablexp sponge as bdry_ablexp_set leaves it (bdry_t.c:513-820): masks
+ * Ex[nx], Ey[ny], Ez[nz] (bdry->ablexp_Ex/Ey/Ez) and the 6 blocks of
+ * bdry->bdry_blk as {enable, ni1, ni2, nj1, nj2, nk1, nk2} each (array
+ * indices, inclusive).  After every step each enabled block's points, all
+ * ncomp components, are multiplied by min(Ex[i], Ey[j], Ez[k]) as
+ * bdry_ablexp_apply does (bdry_t.c:840-890, drv_rk_curv_col.c:482-486),
+ * before peak ground motion and receivers.  No enabled block turns it off. */
+int cgfdh_set_ablexp(cgfdh_solver *s, const float *Ex, const float *Ey, const float *Ez,
+                     const int *blocks);
+
 /* Moment sources as src_t holds them (src_t.h; src_t.c:985-1174):
  *  ijk      local indices with ghosts (si,sj,sk), 3 per source;
  *  it_begin/it_end per source; max_nt;

@@ -532,6 +532,30 @@ void orc_dvh2dvz_iso(int ni, int nj, int nk, const float *metric10,
 }
 
 /* ------------------------------------------------------------------------ */
+/* ablexp sponge, bdry_ablexp_apply (bdry_t.c:840-890): blocks in order,    */
+/* every component times min(Ex[i], Ey[j], Ez[k]); blocks as {enable, ni1, */
+/* ni2, nj1, nj2, nk1, nk2}                                                 */
+/* ------------------------------------------------------------------------ */
+
+void orc_ablexp_apply(int nx, int ny, int nz, int ncmp, float *w, const float *Ex,
+                      const float *Ey, const float *Ez, const int *blocks)
+{
+  size_t vol = (size_t)nx * ny * nz;
+  for (int c = 0; c < ncmp; c++)
+    for (int n = 0; n < 6; n++) {
+      const int *b = blocks + 7 * n;
+      if (!b[0]) continue;
+      for (int k = b[5]; k <= b[6]; k++)
+        for (int j = b[3]; j <= b[4]; j++)
+          for (int i = b[1]; i <= b[2]; i++) {
+            float m = Ex[i] < Ey[j] ? Ex[i] : Ey[j];
+            if (m > Ez[k]) m = Ez[k];
+            w[c * vol + i + (size_t)j * nx + (size_t)k * nx * ny] *= m;
+          }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* setup: stable time step, blk_dt_esti_curv (blk_t.c:2840-2930) with the   */
 /* point-to-plane distance of fdlib_math.c:52-89; float throughout except   */
 /* where the reference calls the double sqrt or writes double literals      */

@@ -700,3 +700,23 @@ void refh_dt_esti(int ni, int nj, int nk, float *xyz, int medium_type, float *vo
   ijk[0] = ijk[1] = ijk[2] = -1;
   blk_dt_esti_curv(&gd, &md, cfl, &out3[0], &out3[1], &out3[2], &ijk[0], &ijk[1], &ijk[2]);
 }
+
+/* bdry_ablexp_apply (bdry_t.c:840-890) itself on a filled bdry_t: masks
+ * Ex[nx], Ey[ny], Ez[nz], blocks as {enable, ni1, ni2, nj1, nj2, nk1, nk2}. */
+void refh_ablexp_apply(int nx, int ny, int nz, int ncmp, float *w, float *Ex, float *Ey,
+                       float *Ez, const int *blocks)
+{
+  bdry_t bdry;
+  memset(&bdry, 0, sizeof bdry);
+  bdry.nx = nx; bdry.ny = ny; bdry.nz = nz;
+  bdry.ablexp_Ex = Ex; bdry.ablexp_Ey = Ey; bdry.ablexp_Ez = Ez;
+  for (int n = 0; n < CONST_NDIM_2; n++) {
+    const int *b = blocks + 7 * n;
+    bdry_block_t *D = bdry.bdry_blk + n;
+    D->enable = b[0];
+    D->ni1 = b[1]; D->ni2 = b[2]; D->ni = b[2] - b[1] + 1;
+    D->nj1 = b[3]; D->nj2 = b[4]; D->nj = b[4] - b[3] + 1;
+    D->nk1 = b[5]; D->nk2 = b[6]; D->nk = b[6] - b[5] + 1;
+  }
+  bdry_ablexp_apply(&bdry, w, ncmp, (size_t)nx * ny * nz);
+}
