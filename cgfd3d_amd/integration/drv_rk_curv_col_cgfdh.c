@@ -7,14 +7,14 @@
  *
  * Covered: isotropic, VTI and general anisotropic elastic media and the GMB
  * visco-elastic medium, CFS-PML faces, traction-image free surface, point
- * and Gaussian moment sources, receivers, slices, snapshots (velocity,
- * stress, strain), the peak-ground-motion maps and the ablexp sponge.
- * Slices and snapshots are packed on the GPU; only their points are copied
- * into the host wavefield that the reference's own writers (io_slice_nc_put,
- * io_snap_nc_put) then read, so the files are the reference's.  Anything
- * else the par file asks for (acoustic medium, Graves Qs, force sources, dd
- * sources, receiver lines, output_all dumps, several MPI ranks) stops the
- * run with a message, the reference's fatal-error convention.
+ * and Gaussian moment sources, receivers and receiver lines, slices,
+ * snapshots (velocity, stress, strain), the peak-ground-motion maps and the
+ * ablexp sponge.  Slices and snapshots are packed on the GPU; only their
+ * points are copied into the host wavefield that the reference's own writers
+ * (io_slice_nc_put, io_snap_nc_put) then read, so the files are the
+ * reference's.  Anything else the par file asks for (acoustic medium, Graves
+ * Qs, force sources, dd sources, output_all dumps, several MPI ranks) stops
+ * the run with a message, the reference's fatal-error convention.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -91,7 +91,7 @@ drv_rk_curv_col_allstep(
   if ((mt != CONST_MEDIUM_ELASTIC_ISO && mt != CONST_MEDIUM_ELASTIC_VTI &&
        mt != CONST_MEDIUM_ELASTIC_ANISO && !visco) ||
       (visco && md->visco_type != CONST_VISCO_GMB) || src->force_actived ||
-      src->dd_is_valid || ioline->num_of_lines > 0 || output_all) {
+      src->dd_is_valid || output_all) {
     fprintf(stderr, "ERROR: par file asks for a feature the GPU time loop lacks\n");
     MPI_Abort(MPI_COMM_WORLD, 1);
   }
@@ -170,16 +170,28 @@ drv_rk_curv_col_allstep(
     free(ijk); free(mij); free(w);
   }
 
+  /* receivers, then the points of the receiver lines as receivers with zero
+   * interpolation weights on their own grid point: the engine's trilinear
+   * sum (io_funcs.c:1598-1612) then returns the point's value, which is what
+   * io_line_keep records (io_funcs.c:1626-1645) */
   int nr = iorecv->total_number;
-  if (nr > 0) {
-    int *idx = malloc(8 * nr * sizeof(int));
-    float *dijk = malloc(3 * nr * sizeof(float));
+  int nl = 0;
+  for (int n = 0; n < ioline->num_of_lines; n++) nl += ioline->line_nr[n];
+  if (nr + nl > 0) {
+    int *idx = malloc(8 * (size_t)(nr + nl) * sizeof(int));
+    float *dijk = malloc(3 * (size_t)(nr + nl) * sizeof(float));
     for (int r = 0; r < nr; r++) {
       iorecv_one_t *o = iorecv->recvone + r;
       memcpy(idx + 8 * r, o->indx1d, 8 * sizeof(int));
       dijk[3 * r] = o->di; dijk[3 * r + 1] = o->dj; dijk[3 * r + 2] = o->dk;
     }
-    CHK(cgfdh_set_receivers(s, nr, idx, dijk));
+    int q = nr;
+    for (int n = 0; n < ioline->num_of_lines; n++)
+      for (int ir = 0; ir < ioline->line_nr[n]; ir++, q++) {
+        for (int c = 0; c < 8; c++) idx[8 * q + c] = ioline->recv_iptr[n][ir];
+        dijk[3 * q] = dijk[3 * q + 1] = dijk[3 * q + 2] = 0.0f;
+      }
+    CHK(cgfdh_set_receivers(s, nr + nl, idx, dijk));
     free(idx); free(dijk);
   }
 
@@ -266,13 +278,20 @@ drv_rk_curv_col_allstep(
     free(pg);
   }
 
-  if (nr > 0) {   /* iorecv->recvone[r].seismo[icmp*max_nt + it] */
+  if (nr + nl > 0) {   /* recvone[r].seismo and line seismo, [icmp*max_nt + it] */
     int nc = wav->ncmp;            /* 9, or 9 + 6 nmaxwell (wav_t.c:26-33) */
-    float *seis = malloc((size_t)nr * nc * nt_total * sizeof(float));
+    float *seis = malloc((size_t)(nr + nl) * nc * nt_total * sizeof(float));
     CHK(cgfdh_get_seismograms(s, seis));
     for (int r = 0; r < nr; r++)
-      memcpy(iorecv->recvone[r].seismo, seis + (size_t)r * nc * nt_total,
-             (size_t)nc * nt_total * sizeof(float));
+      for (int c = 0; c < nc; c++)
+        memcpy(iorecv->recvone[r].seismo + (size_t)c * iorecv->max_nt,
+               seis + ((size_t)r * nc + c) * nt_total, (size_t)nt_total * sizeof(float));
+    int q = nr;
+    for (int n = 0; n < ioline->num_of_lines; n++)
+      for (int ir = 0; ir < ioline->line_nr[n]; ir++, q++)
+        for (int c = 0; c < nc; c++)
+          memcpy(ioline->recv_seismo[n] + ((size_t)ir * nc + c) * ioline->max_nt,
+                 seis + ((size_t)q * nc + c) * nt_total, (size_t)nt_total * sizeof(float));
     free(seis);
   }
   CHK(cgfdh_destroy(s));
