@@ -12,9 +12,10 @@
  * ablexp sponge.  Slices and snapshots are packed on the GPU; only their
  * points are copied into the host wavefield that the reference's own writers
  * (io_slice_nc_put, io_snap_nc_put) then read, so the files are the
- * reference's.  Anything else the par file asks for (acoustic medium, Graves
- * Qs, force sources, dd sources, output_all dumps, several MPI ranks) stops
- * the run with a message, the reference's fatal-error convention.
+ * reference's; the output_all debug dump copies the whole wavefield back each
+ * step.  Anything else the par file asks for (acoustic medium, Graves Qs,
+ * force sources, dd sources, several MPI ranks) stops the run with a
+ * message, the reference's fatal-error convention.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -91,7 +92,7 @@ drv_rk_curv_col_allstep(
   if ((mt != CONST_MEDIUM_ELASTIC_ISO && mt != CONST_MEDIUM_ELASTIC_VTI &&
        mt != CONST_MEDIUM_ELASTIC_ANISO && !visco) ||
       (visco && md->visco_type != CONST_VISCO_GMB) || src->force_actived ||
-      src->dd_is_valid || output_all) {
+      src->dd_is_valid) {
     fprintf(stderr, "ERROR: par file asks for a feature the GPU time loop lacks\n");
     MPI_Abort(MPI_COMM_WORLD, 1);
   }
@@ -232,8 +233,9 @@ drv_rk_curv_col_allstep(
   float *wbuf = wav->v5d + wav->siz_ilevel;
   const int cmp9[9] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
   int it_run = 0;
+  char ou_file[CONST_MAX_STRLEN];
   for (int it = 0; it < nt_total; it++) {
-    int due = has_slice;
+    int due = has_slice || output_all;
     for (int n = 0; n < iosnap->num_of_snap && !due; n++) due = snap_due(iosnap, n, it, nt_total);
     if (!due) continue;
     CHK(cgfdh_run(s, it_run, it + 1 - it_run));
@@ -265,6 +267,12 @@ drv_rk_curv_col_allstep(
     io_slice_nc_put(ioslice, &ioslice_nc, gd, w4d, wbuf, it, t_end, 0, wav->ncmp - 1,
                     wav->visco_type);
     io_snap_nc_put(iosnap, &iosnap_nc, gd, md, wav, w4d, wbuf, nt_total, it, t_end, 1, 1, 1);
+    if (output_all) {   /* debug dump of the whole w_end (drv_rk_curv_col.c:513-525) */
+      CHK(cgfdh_get_wavefield(s, w4d));
+      io_build_fname_time(output_dir, "w3d", ".nc", topoid, it, ou_file);
+      io_var3d_export_nc(ou_file, w4d, wav->cmp_pos, wav->cmp_name, wav->ncmp, gd->index_name,
+                         gd->nx, gd->ny, gd->nz);
+    }
   }
   if (it_run < nt_total) CHK(cgfdh_run(s, it_run, nt_total - it_run));
   free(pack);
