@@ -96,6 +96,11 @@ drv_rk_curv_col_allstep(
     fprintf(stderr, "ERROR: par file asks for a feature the GPU time loop lacks\n");
     MPI_Abort(MPI_COMM_WORLD, 1);
   }
+  for (int n = 0; n < iosnap->num_of_snap; n++)
+    if (iosnap->dit[n] <= 0) {   /* io_snap_nc_put divides by it (io_funcs.c:1148-1150) */
+      fprintf(stderr, "ERROR: snapshot %d has time increment %d\n", n, iosnap->dit[n]);
+      MPI_Abort(MPI_COMM_WORLD, 1);
+    }
 
   cgfdh_desc d;
   memset(&d, 0, sizeof d);

@@ -76,6 +76,26 @@ def test_bad_descriptor_rejected_before_the_device():
     assert lib().cgfdh_create(C.byref(d), C.byref(h)) == -1
 
 
+def test_dt_estimate_rejects_bad_arguments_before_the_device():
+    """cgfdh_estimate_dt checks its arguments on the host before any copy."""
+    from cgfd3d_amd._lib import lib
+    xyz = np.zeros((3, 14, 14, 14), np.float32)
+    vols = np.ones((3, 14, 14, 14), np.float32)
+    out = np.zeros(3, np.float32)
+    ijk = np.zeros(3, np.int32)
+    call = lambda ni, mt, nmw, cfl: lib().cgfdh_estimate_dt(
+        ni, 8, 8, po.ptr(xyz), mt, po.ptr(vols), nmw, cfl, po.ptr(out[0:]), po.ptr(out[1:]),
+        po.ptr(out[2:]), po.ptr(ijk))
+    assert call(8, 2, 0, 0.0) == -1 and b"cfl" in lib().cgfdh_last_error()
+    assert call(8, 2, 0, float("nan")) == -1
+    assert call(0, 2, 0, 1.0) == -1 and b"grid" in lib().cgfdh_last_error()
+    assert call(8, 7, 0, 1.0) == -1 and b"medium_type" in lib().cgfdh_last_error()
+    assert call(8, 5, 0, 1.0) == -1 and b"nmaxwell" in lib().cgfdh_last_error()
+    assert call(8, 5, 5, 1.0) == -1
+    assert lib().cgfdh_estimate_dt(8, 8, 8, None, 2, po.ptr(vols), 0, 1.0, po.ptr(out[0:]),
+                                   po.ptr(out[1:]), po.ptr(out[2:]), po.ptr(ijk)) == -1
+
+
 @pytest.mark.skipif(gpu_available(), reason="checks the no-device error path")
 def test_create_without_device_fails_loudly():
     from cgfd3d_amd import CgfdhError, Solver
