@@ -532,6 +532,28 @@ void orc_dvh2dvz_iso(int ni, int nj, int nk, const float *metric10,
 }
 
 /* ------------------------------------------------------------------------ */
+/* Graves Qs attenuation, sv_curv_col_el_graves_Qs (sv_curv_col_el.c:638-667),*/
+/* applied to w_end after the last stage's combine (drv_rk_curv_col.c:414-416):*/
+/* every component at physical points times expf(coef / Qs), coef =          */
+/* -PI * freq * dt evaluated in double (PI is a double literal) and stored   */
+/* as float                                                                  */
+/* ------------------------------------------------------------------------ */
+
+void orc_graves_qs(int ni, int nj, int nk, int ncmp, float *w, const float *Qs, float freq,
+                   float dt)
+{
+  size_t nx = ni + 6, ny = nj + 6, nz = nk + 6, vol = nx * ny * nz;
+  float coef = (float)(-3.14159265358979323846 * (double)freq * (double)dt);
+  for (int c = 0; c < ncmp; c++)
+    for (int k = 3; k < 3 + nk; k++)
+      for (int j = 3; j < 3 + nj; j++)
+        for (int i = 3; i < 3 + ni; i++) {
+          size_t p = i + j * nx + k * nx * ny;
+          w[c * vol + p] *= expf(coef / Qs[p]);
+        }
+}
+
+/* ------------------------------------------------------------------------ */
 /* ablexp sponge, bdry_ablexp_apply (bdry_t.c:840-890): blocks in order,    */
 /* every component times min(Ex[i], Ey[j], Ez[k]); blocks as {enable, ni1, */
 /* ni2, nj1, nj2, nk1, nk2}                                                 */

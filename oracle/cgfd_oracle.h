@@ -106,6 +106,8 @@ void orc_visco_gmb(long npts, int nmaxwell, float freq, float fmin, float fmax,
                    float *Ylam, float *Ymu, float *wl);
 void orc_metric(int ni, int nj, int nk, const float *xyz /* 3 volumes */,
                 float *metric10);
+void orc_graves_qs(int ni, int nj, int nk, int ncmp, float *w, const float *Qs, float freq,
+                   float dt);
 void orc_ablexp_apply(int nx, int ny, int nz, int ncmp, float *w, const float *Ex,
                       const float *Ey, const float *Ez, const int *blocks);
 float orc_dist_point2plane(const float x0[3], const float x1[3], const float x2[3],

@@ -69,6 +69,8 @@ def load_oracle(fp64=False):
     lib.orc_ncomp.restype = C.c_int
     lib.orc_ncomp.argtypes = [_P]
     lib.orc_metric.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P]
+    lib.orc_graves_qs.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, C.c_float,
+                                  C.c_float]
     lib.orc_ablexp_apply.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P]
     lib.orc_dist_point2plane.restype = C.c_float
     lib.orc_dist_point2plane.argtypes = [_P, _P, _P, _P]
@@ -97,6 +99,8 @@ def load_ref():
     lib.refh_onestage.argtypes = [_P, _P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]
     lib.refh_run.argtypes = [_P, _P, _P, _P, C.c_float, C.c_int, _P, C.c_int, _P, _P,
                              _P, _P, _P, _P]
+    lib.refh_graves_qs.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, C.c_float,
+                                   C.c_float]
     lib.refh_ablexp_apply.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P]
     lib.refh_dt_esti.argtypes = [C.c_int, C.c_int, C.c_int, _P, C.c_int, _P, C.c_int, C.c_float,
                                  _P, _P]

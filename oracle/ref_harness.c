@@ -720,3 +720,16 @@ void refh_ablexp_apply(int nx, int ny, int nz, int ncmp, float *w, float *Ex, fl
   }
   bdry_ablexp_apply(&bdry, w, ncmp, (size_t)nx * ny * nz);
 }
+
+/* sv_curv_col_el_graves_Qs (sv_curv_col_el.c:638-667) itself on filled gd_t /
+ * md_t structs. */
+void refh_graves_qs(int ni, int nj, int nk, int ncmp, float *w, float *Qs, float freq, float dt)
+{
+  gd_t gd;
+  fill_gd(&gd, ni, nj, nk, NULL, NULL, NULL);
+  md_t md;
+  memset(&md, 0, sizeof md);
+  md.Qs = Qs;
+  md.visco_Qs_freq = freq;
+  sv_curv_col_el_graves_Qs(w, ncmp, dt, &gd, &md);
+}
