@@ -809,7 +809,8 @@ struct orc_s {
   int region[6][6];
   size_t aux_off[6], aux_n[6], aux_total;
   real_t *pA[6], *pB[6], *pD[6];
-  real_t *srcm;             /* 6 x nsrc*max_nt*4 */
+  real_t *srcm;             /* 6 x nsrc*max_nt*4, NULL without moment */
+  real_t *srcf;             /* 3 x nsrc*max_nt*4, NULL without force */
   real_t *pg, *dis;         /* peak ground motion maps, free surface only */
 };
 
@@ -875,10 +876,15 @@ orc_t *orc_create(const orc_cfg *cfg, const float *metric10, const float *med)
     }
   }
   o->aux_total = off;
-  if (cfg->nsrc > 0) {
+  if (cfg->nsrc > 0 && cfg->src_mij) {
     size_t n = (size_t)cfg->nsrc * cfg->src_max_nt * 4;
     o->srcm = (real_t *)malloc(6 * n * sizeof(real_t));
     for (size_t p = 0; p < 6 * n; p++) o->srcm[p] = cfg->src_mij[p];
+  }
+  if (cfg->nsrc > 0 && cfg->src_fxyz) {
+    size_t n = (size_t)cfg->nsrc * cfg->src_max_nt * 4;
+    o->srcf = (real_t *)malloc(3 * n * sizeof(real_t));
+    for (size_t p = 0; p < 3 * n; p++) o->srcf[p] = cfg->src_fxyz[p];
   }
   return o;
 }
@@ -886,7 +892,8 @@ orc_t *orc_create(const orc_cfg *cfg, const float *metric10, const float *med)
 void orc_destroy(orc_t *o)
 {
   if (!o) return;
-  free(o->metric); free(o->med); free(o->mats); free(o->srcm); free(o->pg); free(o->dis);
+  free(o->metric); free(o->med); free(o->mats); free(o->srcm); free(o->srcf); free(o->pg);
+  free(o->dis);
   for (int f = 0; f < 6; f++) { free(o->pA[f]); free(o->pB[f]); free(o->pD[f]); }
   free(o);
 }
@@ -1205,7 +1212,18 @@ static void rhs_pml(orc_t *o, const real_t *w, real_t *h, const real_t *aux,
   }
 }
 
-/* moment sources, sv_curv_col_el.c:312-479 (moment part) */
+/* moment and force sources, sv_curv_col_el.c:312-479: a force adds
+ * f * (coef) * slw / jac to the velocity rates (float division first), a
+ * moment subtracts M * (coef) / jac from the stress rates */
+static void add_force(orc_t *o, const real_t *F, size_t p, real_t coef, real_t *h)
+{
+  size_t V = o->vol;
+  real_t Vf = coef * o->slw[p] / o->metric[p];
+  h[VX * V + p] += F[0] * Vf;
+  h[VY * V + p] += F[1] * Vf;
+  h[VZ * V + p] += F[2] * Vf;
+}
+
 static void rhs_src(orc_t *o, int it, int istage, real_t *h)
 {
   const orc_cfg *c = &o->cfg;
@@ -1218,15 +1236,26 @@ static void rhs_src(orc_t *o, int it, int istage, real_t *h)
   for (int is = 0; is < c->nsrc; is++) {
     if (!(it >= c->src_it_begin[is] && it <= c->src_it_end[is])) continue;
     size_t ip = (size_t)is * c->src_max_nt * 4 + (size_t)(it - c->src_it_begin[is]) * 4 + istage;
-    real_t Mv[6];
-    for (int m = 0; m < 6; m++) Mv[m] = o->srcm[m * nper + ip];
+    real_t Mv[6] = {0, 0, 0, 0, 0, 0}, Fv[3] = {0, 0, 0};
+    if (o->srcm)
+      for (int m = 0; m < 6; m++) Mv[m] = o->srcm[m * nper + ip];
+    if (o->srcf)
+      for (int m = 0; m < 3; m++) Fv[m] = o->srcf[m * nper + ip];
     int si = c->src_ijk[3 * is], sj = c->src_ijk[3 * is + 1], sk = c->src_ijk[3 * is + 2];
     /* M order Mxx,Myy,Mzz,Mxz,Myz,Mxy onto slots Txx,Tyy,Tzz,Txz,Tyz,Txy */
     static const int slot[6] = {TXX, TYY, TZZ, TXZ, TYZ, TXY};
     if (!c->src_gauss) {
       size_t p = si + sj * o->sl + sk * o->ss;
-      real_t rjac = 1.0 / jac[p];
-      for (int m = 0; m < 6; m++) h[slot[m] * V + p] -= Mv[m] * rjac;
+      if (o->srcf) {
+        real_t Vf = o->slw[p] / jac[p];
+        h[VX * V + p] += Fv[0] * Vf;
+        h[VY * V + p] += Fv[1] * Vf;
+        h[VZ * V + p] += Fv[2] * Vf;
+      }
+      if (o->srcm) {
+        real_t rjac = 1.0 / jac[p];
+        for (int m = 0; m < 6; m++) h[slot[m] * V + p] -= Mv[m] * rjac;
+      }
     } else {
       int kb = (sk + half < o->nk2) ? half : o->nk2 - sk;
       orc_gauss_weights(ext, c->src_inc[3 * is], c->src_inc[3 * is + 1], c->src_inc[3 * is + 2],
@@ -1240,8 +1269,11 @@ static void rhs_src(orc_t *o, int it, int istage, real_t *h)
               continue;
             size_t p = i + j * o->sl + k * o->ss;
             real_t coef = ext[ie];
-            real_t rjac = coef / jac[p];
-            for (int m = 0; m < 6; m++) h[slot[m] * V + p] -= Mv[m] * rjac;
+            if (o->srcf) add_force(o, Fv, p, coef, h);
+            if (o->srcm) {
+              real_t rjac = coef / jac[p];
+              for (int m = 0; m < 6; m++) h[slot[m] * V + p] -= Mv[m] * rjac;
+            }
           }
     }
   }

@@ -68,6 +68,10 @@ typedef struct {
   int nmaxwell;            /* memory-variable bodies */
   const float *visco_wl;   /* relaxation frequencies, nmaxwell values */
   const float *xyz;        /* coordinates (3 volumes), for the surface matD */
+  /* force sources (sv_curv_col_el.c:356-364, 440-452), not the strict surface
+   * force: Fx,Fy,Fz, each [nsrc][max_nt][4]; NULL for none.  src_mij may be
+   * NULL for force-only sources. */
+  const float *src_fxyz;
 } orc_cfg;
 
 typedef struct orc_s orc_t;

@@ -26,7 +26,7 @@ class OrcCfg(C.Structure):
                 ("src_ext_coef", C.c_float),
                 ("src_ijk", _P), ("src_inc", _P), ("src_it_begin", _P), ("src_it_end", _P),
                 ("src_max_nt", C.c_int), ("src_mij", _P), ("medium_type", C.c_int),
-                ("nmaxwell", C.c_int), ("visco_wl", _P), ("xyz", _P)]
+                ("nmaxwell", C.c_int), ("visco_wl", _P), ("xyz", _P), ("src_fxyz", _P)]
 
 
 MEDIUM_ISO, MEDIUM_VTI, MEDIUM_ANISO, MEDIUM_VISCO = 2, 3, 4, 5
@@ -39,7 +39,8 @@ class RefCfg(C.Structure):
                 ("pml_vel", C.c_float * 6),
                 ("nsrc", C.c_int), ("src_max_nt", C.c_int), ("src_ijk", _P),
                 ("src_it_begin", _P), ("src_it_end", _P), ("src_mij", _P),
-                ("medium_type", C.c_int), ("nmaxwell", C.c_int), ("visco_wl", _P)]
+                ("medium_type", C.c_int), ("nmaxwell", C.c_int), ("visco_wl", _P),
+                ("src_fxyz", _P)]
 
 
 def load_oracle(fp64=False):
@@ -150,15 +151,19 @@ class Oracle:
                         inc=np.ascontiguousarray(src.get("inc", np.zeros(3 * src["nsrc"])),
                                                  np.float32),
                         itb=np.ascontiguousarray(src["it_begin"], np.int32),
-                        ite=np.ascontiguousarray(src["it_end"], np.int32),
-                        mij=np.ascontiguousarray(src["mij"], np.float32))
+                        ite=np.ascontiguousarray(src["it_end"], np.int32))
+            if src.get("mij") is not None:
+                arrs["mij"] = np.ascontiguousarray(src["mij"], np.float32)
+            if src.get("fxyz") is not None:     # force sources: Fx, Fy, Fz tables
+                arrs["fxyz"] = np.ascontiguousarray(src["fxyz"], np.float32)
             self._keep += list(arrs.values())
             cfg.src_ijk = arrs["ijk"].ctypes.data
             cfg.src_inc = arrs["inc"].ctypes.data
             cfg.src_it_begin = arrs["itb"].ctypes.data
             cfg.src_it_end = arrs["ite"].ctypes.data
             cfg.src_max_nt = int(src["max_nt"])
-            cfg.src_mij = arrs["mij"].ctypes.data
+            cfg.src_mij = arrs["mij"].ctypes.data if "mij" in arrs else None
+            cfg.src_fxyz = arrs["fxyz"].ctypes.data if "fxyz" in arrs else None
         self.cfg = cfg
         self.metric = np.ascontiguousarray(case["metric"], np.float32)
         self.med = np.ascontiguousarray(case["med"], np.float32)

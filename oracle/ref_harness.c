@@ -62,6 +62,9 @@ typedef struct {
    * Maxwell body; wl the bodies' relaxation frequencies */
   int nmaxwell;
   const float *visco_wl;
+  /* point force sources: Fx, Fy, Fz back to back, same layout as src_mij;
+   * NULL for none (src_mij may then be NULL too) */
+  const float *src_fxyz;
 } refh_cfg;
 
 typedef struct {
@@ -178,8 +181,8 @@ static void setup(refh_ctx *c, const refh_cfg *cfg, float *coords,
     s->max_nt = cfg->src_max_nt;
     s->max_stage = 4;
     s->itype_spatial_ext = CONST_SRC_SPATIAL_POINT;
-    s->force_actived = 0;
-    s->moment_actived = 1;
+    s->force_actived = cfg->src_fxyz != NULL;
+    s->moment_actived = cfg->src_mij != NULL;
     s->is_surface_force_strict = 0;
     s->si = (int *)malloc(cfg->nsrc * sizeof(int));
     s->sj = (int *)malloc(cfg->nsrc * sizeof(int));
@@ -194,9 +197,15 @@ static void setup(refh_ctx *c, const refh_cfg *cfg, float *coords,
       s->it_end[is] = cfg->src_it_end[is];
     }
     size_t n = (size_t)cfg->nsrc * cfg->src_max_nt * 4;
-    float *mij = (float *)cfg->src_mij;
-    s->Mxx = mij + 0 * n; s->Myy = mij + 1 * n; s->Mzz = mij + 2 * n;
-    s->Mxz = mij + 3 * n; s->Myz = mij + 4 * n; s->Mxy = mij + 5 * n;
+    if (cfg->src_mij) {
+      float *mij = (float *)cfg->src_mij;
+      s->Mxx = mij + 0 * n; s->Myy = mij + 1 * n; s->Mzz = mij + 2 * n;
+      s->Mxz = mij + 3 * n; s->Myz = mij + 4 * n; s->Mxy = mij + 5 * n;
+    }
+    if (cfg->src_fxyz) {
+      float *f = (float *)cfg->src_fxyz;
+      s->Fx = f; s->Fy = f + n; s->Fz = f + 2 * n;
+    }
   }
 }
 

@@ -321,12 +321,16 @@ def ref_cfg(case, src_arrays=None):
         assert not src.get("gauss"), "the reference harness runs point sources only"
         a = [np.ascontiguousarray(src["ijk"], np.int32),
              np.ascontiguousarray(src["it_begin"], np.int32),
-             np.ascontiguousarray(src["it_end"], np.int32),
-             np.ascontiguousarray(src["mij"], np.float32)]
+             np.ascontiguousarray(src["it_end"], np.int32)]
         keep += a
         c.nsrc = src["nsrc"]
         c.src_max_nt = src["max_nt"]
-        c.src_ijk, c.src_it_begin, c.src_it_end, c.src_mij = [x.ctypes.data for x in a]
+        c.src_ijk, c.src_it_begin, c.src_it_end = [x.ctypes.data for x in a]
+        for key, fld in (("mij", "src_mij"), ("fxyz", "src_fxyz")):
+            if src.get(key) is not None:
+                t = np.ascontiguousarray(src[key], np.float32)
+                keep.append(t)
+                setattr(c, fld, t.ctypes.data)
     return c, keep
 
 
