@@ -15,6 +15,10 @@ from seeds by tests/cases.py) and the reference outputs:
   dt_estimate.npz      blk_dt_esti_curv's dtmax, Vp, L and point for the
                        inputs of tests/cases.py dt_case, every medium
                        (`python tests/golden/make_golden.py dt` writes only it)
+  extras.npz           checksums of the reference's bdry_ablexp_apply,
+                       sv_curv_col_el_graves_Qs and force-source RHS on the
+                       inputs of tests/golden/extras_cases.py
+                       (`python tests/golden/make_golden.py extras`)
 The reference's surface row of hV is re-evaluated by the harness on stages
 with the forward z operator (the reference reads out of bounds there; see
 oracle/ref_harness.c).
@@ -88,6 +92,12 @@ def main():
         sys.exit("oracle/_ref/libcgfd3d_ref.so not built (needs the reference tree)")
     if sys.argv[1:] == ["dt"]:
         write_dt_golden()
+        return
+    if sys.argv[1:] == ["extras"]:
+        from tests.golden import extras_cases as ec
+        ref = po.load_ref()
+        np.savez_compressed(os.path.join(HERE, "extras.npz"),
+                            **{name: checksum(out) for name, out in ec.run_all(ref).items()})
         return
     ref = po.load_ref()
     for name, kw in ONESTAGE_CASES.items():
