@@ -48,6 +48,7 @@ SIGNATURES = [
     ("cgfdh_get_peak_ground_motion", C.c_int, [_P, _P, _P]),
     ("cgfdh_get_subvolume", C.c_int, [_P, _P, C.c_int, _P, _P]),
     ("cgfdh_set_ablexp", C.c_int, [_P, _P, _P, _P, _P]),
+    ("cgfdh_set_graves_qs", C.c_int, [_P, _P, C.c_float]),
     ("cgfdh_get_strain_subvolume", C.c_int, [_P, _P, _P]),
     ("cgfdh_estimate_dt", C.c_int, [C.c_int, C.c_int, C.c_int, _P, C.c_int, _P, C.c_int,
                                     C.c_float, _P, _P, _P, _P]),

@@ -206,6 +206,8 @@ struct cgfdh_solver {
   float* seismo = nullptr;
   float* pg = nullptr;           // 15 peak maps + 3 displacement maps, [ny][nx] each
   float* outbuf = nullptr;       // staging for packed output sub-volumes (grown on demand)
+  float* qs = nullptr;           // Graves-Qs quality factor per point (NULL: off)
+  float qs_coef = 0.0f;          // -pi * freq * dt
   float* abl = nullptr;          // ablexp masks Ex[nx], Ey[ny], Ez[nz]
   int abl_blk[6][6] = {};        // enabled ablexp blocks: i1, i2, j1, j2, k1, k2
   int abl_n = 0;
@@ -249,6 +251,7 @@ static void free_all(cgfdh_solver* s)
   }
   F(s->src); F(s->mij); F(s->src_coef);
   F(s->recv_idx); F(s->recv_w); F(s->seismo); F(s->pg); F(s->outbuf); F(s->abl);
+  F(s->qs);
 }
 
 extern "C" int cgfdh_create(const cgfdh_desc* d, cgfdh_solver** out)
@@ -427,6 +430,20 @@ extern "C" int cgfdh_set_free_surface_matD(cgfdh_solver* s, const float* matD)
 {
   if (!s || !matD) return fail(CGFDH_EINVAL, "cgfdh_set_free_surface_matD: null argument");
   return h2d(s, s->mats + 27 * s->ss, matD, 9 * (size_t)s->ss);
+}
+
+extern "C" int cgfdh_set_graves_qs(cgfdh_solver* s, const float* Qs, float freq)
+{
+  if (!s || !Qs) return fail(CGFDH_EINVAL, "cgfdh_set_graves_qs: null argument");
+  if (s->d.medium_type != CGFDH_MEDIUM_ELASTIC_ISO)
+    return fail(CGFDH_ESTATE, "cgfdh_set_graves_qs: needs the isotropic elastic medium "
+                "(medium_type %d)", s->d.medium_type);
+  if (!(freq > 0.0f)) return fail(CGFDH_EINVAL, "cgfdh_set_graves_qs: freq %g", (double)freq);
+  if (!s->qs) HIPCHK(hipMalloc(&s->qs, (size_t)s->vol * sizeof(float)));
+  int rc = h2d(s, s->qs, Qs, s->vol);
+  if (rc) return rc;
+  s->qs_coef = (float)(-3.14159265358979323846 * (double)freq * (double)s->d.dt);
+  return CGFDH_OK;
 }
 
 extern "C" int cgfdh_set_ablexp(cgfdh_solver* s, const float* Ex, const float* Ey,
@@ -791,6 +808,26 @@ __global__ void ablexp_kernel(float* __restrict__ w, long vol, int nw, int nx, l
   for (int c = 0; c < nw; c++) w[c * vol + p] *= mask;
 }
 
+// Graves-Qs attenuation, sv_curv_col_el_graves_Qs (sv_curv_col_el.c:638-667):
+// every component at physical points times exp(coef / Qs), coef = -pi f dt.
+// The exponential is evaluated in double and rounded to float (correctly
+// rounded but for rare double-rounding cases); the reference's libm expf
+// differs from it by one ulp on a few inputs in 10^5.
+__global__ void graves_qs_kernel(float* __restrict__ w, long vol, int nw, int nx, long ss,
+                                 const float* __restrict__ qs, float coef, int ni, int nj,
+                                 long npt)
+{
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npt) return;
+  const long r = t / ni;
+  const int i = 3 + (int)(t - r * ni);
+  const int k = 3 + (int)(r / nj);
+  const int j = 3 + (int)(r - (long)(k - 3) * nj);
+  const long p = (long)i + (long)j * nx + (long)k * ss;
+  const float att = (float)exp((double)(coef / qs[p]));
+  for (int c = 0; c < nw; c++) w[c * vol + p] *= att;
+}
+
 static void enqueue_ablexp(cgfdh_solver* s, hipStream_t st)
 {
   const float* Ex = s->abl;
@@ -826,6 +863,12 @@ static int enqueue_step(cgfdh_solver* s, hipStream_t st, int it, const int* it_o
     hipLaunchKernelGGL(vis_free_kernel, dim3((s->d.ni + 255) / 256, s->d.nj), dim3(256), 0, st,
                        s->lvl[3], s->mats + 27 * s->ss, s->d.ni, s->d.nj, s->nx, s->ss, s->vol,
                        s->nk2);
+  if (s->qs) {   // after the last stage's combine, drv_rk_curv_col.c:414-416
+    const long npt = (long)s->d.ni * s->d.nj * s->d.nk;
+    hipLaunchKernelGGL(graves_qs_kernel, dim3((unsigned)((npt + 255) / 256)), dim3(256), 0, st,
+                       s->lvl[3], s->vol, s->nw, s->nx, s->ss, s->qs, s->qs_coef, s->d.ni,
+                       s->d.nj, npt);
+  }
   if (s->abl_n) enqueue_ablexp(s, st);   // drv_rk_curv_col.c:482-486
   if (s->pg)   // drv_rk_curv_col.c:492-495, before the pre/end swap
     hipLaunchKernelGGL(pg_kernel, dim3((s->d.ni + 255) / 256, s->d.nj), dim3(256), 0, st,

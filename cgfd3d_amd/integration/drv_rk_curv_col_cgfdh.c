@@ -8,14 +8,14 @@
  * Covered: isotropic, VTI and general anisotropic elastic media and the GMB
  * visco-elastic medium, CFS-PML faces, traction-image free surface, point
  * and Gaussian moment sources, receivers and receiver lines, slices,
- * snapshots (velocity, stress, strain), the peak-ground-motion maps and the
- * ablexp sponge.  Slices and snapshots are packed on the GPU; only their
- * points are copied into the host wavefield that the reference's own writers
+ * snapshots (velocity, stress, strain), the peak-ground-motion maps, the
+ * ablexp sponge and Graves-Qs attenuation with the isotropic medium.  Slices
+ * and snapshots are packed on the GPU; only their points are copied into the host wavefield that the reference's own writers
  * (io_slice_nc_put, io_snap_nc_put) then read, so the files are the
  * reference's; the output_all debug dump copies the whole wavefield back each
- * step.  Anything else the par file asks for (acoustic medium, Graves Qs,
- * force sources, dd sources, several MPI ranks) stops the run with a
- * message, the reference's fatal-error convention.
+ * step.  Anything else the par file asks for (acoustic medium, Graves Qs
+ * with an anisotropic medium, force sources, dd sources, several MPI ranks)
+ * stops the run with a message, the reference's fatal-error convention.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -91,7 +91,9 @@ drv_rk_curv_col_allstep(
   int visco = mt == CONST_MEDIUM_VISCOELASTIC_ISO;
   if ((mt != CONST_MEDIUM_ELASTIC_ISO && mt != CONST_MEDIUM_ELASTIC_VTI &&
        mt != CONST_MEDIUM_ELASTIC_ANISO && !visco) ||
-      (visco && md->visco_type != CONST_VISCO_GMB) || src->force_actived ||
+      (visco && md->visco_type != CONST_VISCO_GMB) ||
+      (!visco && md->visco_type == CONST_VISCO_GRAVES_QS && mt != CONST_MEDIUM_ELASTIC_ISO) ||
+      src->force_actived ||
       src->dd_is_valid) {
     fprintf(stderr, "ERROR: par file asks for a feature the GPU time loop lacks\n");
     MPI_Abort(MPI_COMM_WORLD, 1);
@@ -200,6 +202,9 @@ drv_rk_curv_col_allstep(
     CHK(cgfdh_set_receivers(s, nr + nl, idx, dijk));
     free(idx); free(dijk);
   }
+
+  if (!visco && md->visco_type == CONST_VISCO_GRAVES_QS)   /* drv_rk_curv_col.c:414-416 */
+    CHK(cgfdh_set_graves_qs(s, md->Qs, md->visco_Qs_freq));
 
   if (bdry->is_enable_ablexp) {   /* masks and blocks of bdry_ablexp_set (bdry_t.c:513-820) */
     int blk[6 * 7];

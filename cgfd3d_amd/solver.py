@@ -202,6 +202,12 @@ class Solver:
         b = np.ascontiguousarray(np.asarray(blocks, dtype=np.int32).reshape(6, 7))
         check(lib().cgfdh_set_ablexp(self.h, _p(ex), _p(ey), _p(ez), _p(b)), "set_ablexp")
 
+    def set_graves_qs(self, Qs, freq):
+        """Graves-Qs attenuation: quality factor per point, applied after every step."""
+        q = _f32(Qs)
+        assert q.size == self.vol
+        check(lib().cgfdh_set_graves_qs(self.h, _p(q), float(freq)), "set_graves_qs")
+
     # ---- output sub-volumes (snapshots, slices) ----
     @staticmethod
     def _box(box):

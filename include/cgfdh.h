@@ -125,6 +125,15 @@ int cgfdh_set_free_surface(cgfdh_solver *s, const float *matVx2Vz2,
 int cgfdh_set_free_surface_matD(cgfdh_solver *s, const float *matD);
 int cgfdh_get_free_surface_matD(cgfdh_solver *s, float *matD);
 
+/* Graves-Qs attenuation (md->visco_type CONST_VISCO_GRAVES_QS, isotropic
+ * elastic medium): md->Qs, one volume, and md->visco_Qs_freq.  After the
+ * last stage of every step each physical point, all components, is
+ * multiplied by exp(-pi * freq * dt / Qs) (sv_curv_col_el_graves_Qs,
+ * sv_curv_col_el.c:638-667, called at drv_rk_curv_col.c:414-416); the
+ * exponential is the double one rounded to float, within one ulp of the
+ * reference's expf. */
+int cgfdh_set_graves_qs(cgfdh_solver *s, const float *Qs, float freq);
+
 /* The ablexp sponge as bdry_ablexp_set leaves it (bdry_t.c:513-820): masks
  * Ex[nx], Ey[ny], Ez[nz] (bdry->ablexp_Ex/Ey/Ez) and the 6 blocks of
  * bdry->bdry_blk as {enable, ni1, ni2, nj1, nj2, nk1, nk2} each (array
