@@ -180,6 +180,146 @@ def test_gaussian_weights_normalised():
     assert abs((W.sum(axis=(0, 2)) * r).sum() + 0.2) < 0.02
 
 
+def _gauss_width_error(W, x0, y0, z0, a, half):
+    """Largest deviation of a (2h+1)^3 weight block from the separable form
+    exp(-((i-x0)^2 + (j-y0)^2 + (k-z0)^2) / a^2) that fun_gauss
+    (src_t.c:2179-2184) fixes: along each axis the log-ratio of adjacent
+    weights equals -(2 i + 1 - 2 x0) / a^2, whatever the normalisation."""
+    W = np.asarray(W, np.float64).reshape(2 * half + 1, 2 * half + 1, 2 * half + 1)
+    r = np.arange(-half, half)
+    err = 0.0
+    for axis, c0 in ((2, x0), (1, y0), (0, z0)):
+        lo = np.take(W, range(0, 2 * half), axis=axis)
+        hi = np.take(W, range(1, 2 * half + 1), axis=axis)
+        got = np.log(hi / lo)
+        shape = [1, 1, 1]
+        shape[axis] = 2 * half
+        want = (-(2 * r + 1 - 2 * c0) / (a * a)).reshape(shape)
+        err = max(err, float(np.abs(got - want).max()))
+    return err
+
+
+def _numpy_gauss_block(x0, y0, z0, a, half, half_below, form):
+    """float64 weight blocks for the width check's self-test: the right form
+    and the plausible mistakes it must reject."""
+    r = np.arange(-half, half + 1, dtype=np.float64)
+    f = {"right": lambda x: np.exp(-x * x / (a * a)),
+         "no_square": lambda x: np.exp(-x * x / a),
+         "half_width": lambda x: np.exp(-x * x / (a * a / 2)),
+         "double_width": lambda x: np.exp(-x * x / (2 * a) ** 2)}[form]
+    W = f(r - z0)[:, None, None] * f(r - y0)[None, :, None] * f(r - x0)[None, None, :]
+    return W / W[: half + 1 + half_below].sum()
+
+
+@pytest.mark.parametrize("shift", [(0.0, 0.0, 0.0), (0.3, -0.2, 0.1), (-0.45, 0.4, -0.35)])
+@pytest.mark.parametrize("half_below", [3, 1, 0])
+def test_gaussian_weights_width(shift, half_below):
+    """src_cal_norm_delt3d_z2fre with ext_func_coef 1.5 (src_t.c:41-46,
+    2111-2151): the adjacent-weight log-ratio pins the width a^2 and the
+    centre along all three axes, the surface truncation leaves the weights
+    above it in place (same normalisation) and the physical rows sum to 1."""
+    orc = po.load_oracle()
+    a, half = 1.5, 3
+    w = np.zeros(343, np.float32)
+    orc.orc_gauss_weights(po.ptr(w), *shift, a, half, half_below)
+    assert _gauss_width_error(w, *shift, a, half) < 2e-5
+    W = w.reshape(7, 7, 7).astype(np.float64)
+    assert abs(W[: half + 1 + half_below].sum() - 1) < 1e-6
+    ref = _numpy_gauss_block(*shift, a, half, half_below, "right")
+    assert np.allclose(W, ref, rtol=2e-6, atol=1e-12)
+
+
+@pytest.mark.parametrize("form", ["no_square", "half_width", "double_width"])
+def test_gaussian_width_check_rejects_mistakes(form):
+    """The width check fails on the plausible mistakes (exp(-x^2/a), a^2/2,
+    2a) that the sum, symmetry and centroid checks all let through."""
+    shift = (0.3, -0.2, 0.1)
+    bad = _numpy_gauss_block(*shift, 1.5, 3, 3, form)
+    assert _gauss_width_error(bad, *shift, 1.5, 3) > 0.05
+    good = _numpy_gauss_block(*shift, 1.5, 3, 3, "right")
+    assert _gauss_width_error(good, *shift, 1.5, 3) < 1e-9
+
+
+def _fault_moment_enu(strike, dip, rake, flip_xz_yz=False, swap_xy=False):
+    """Unit double couple M = n d^T + d n^T from the fault geometry of Aki &
+    Richards (Box 4.4: strike clockwise from north, dip, rake; fault normal n
+    and slip d in x north, y east, z down), rotated to the reference's frame
+    x east, y north, z up (src_t.c:2340-2342).  Returns Mxx, Myy, Mzz, Myz,
+    Mxz, Mxy in float64.  The flags build the plausible mistakes the check
+    must reject."""
+    f, dl, lm = np.radians([strike, dip, rake])
+    n = np.array([-np.sin(dl) * np.sin(f), np.sin(dl) * np.cos(f), -np.cos(dl)])
+    d = np.array([np.cos(lm) * np.cos(f) + np.cos(dl) * np.sin(lm) * np.sin(f),
+                  np.cos(lm) * np.sin(f) - np.cos(dl) * np.sin(lm) * np.cos(f),
+                  -np.sin(lm) * np.sin(dl)])
+    M = np.outer(n, d) + np.outer(d, n)
+    P = np.array([[0.0, 1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, -1.0]])   # NED -> ENU
+    if swap_xy:
+        P = np.array([[1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, -1.0]])
+    E = P @ M @ P.T
+    if flip_xz_yz:
+        D = np.diag([1.0, 1.0, -1.0])
+        E = D @ E @ D
+    return np.array([E[0, 0], E[1, 1], E[2, 2], E[1, 2], E[0, 2], E[0, 1]])
+
+
+_SDR = [(0.0, 45.0, 90.0), (90.0, 45.0, 90.0), (30.0, 60.0, -90.0), (0.0, 90.0, 0.0),
+        (211.0, 33.0, -47.0), (300.0, 75.0, 150.0), (45.0, 20.0, 60.0)]
+
+
+@pytest.mark.parametrize("sdr", _SDR)
+def test_angle2moment_matches_fault_geometry(sdr):
+    """angle2moment (src_t.c:2344-2383), every component, against the moment
+    tensor built from the fault normal and slip vectors and rotated into
+    x east, y north, z up."""
+    m = np.zeros(6, np.float32)
+    po.load_oracle().orc_angle2moment(*sdr, po.ptr(m))
+    want = _fault_moment_enu(*sdr)
+    assert np.allclose(m.astype(np.float64), want, atol=2e-6), (m, want)
+
+
+def test_angle2moment_named_mechanisms():
+    """Textbook mechanisms in x east, y north, z up: a 45-degree thrust has
+    its T axis vertical (Mzz = +1) and its P axis across strike (Mxx = -1 for
+    strike 0, Myy = -1 for strike 90); a 60-degree normal fault on strike 0
+    has Mzz < 0 and Mxx > 0; a vertical strike-slip fault striking north
+    with rake 0 is a pure Mxy couple."""
+    orc = po.load_oracle()
+    m = np.zeros(6, np.float32)
+
+    def mt(s, d, r):
+        orc.orc_angle2moment(s, d, r, po.ptr(m))
+        return m.astype(np.float64).copy()
+
+    mxx, myy, mzz, myz, mxz, mxy = mt(0.0, 45.0, 90.0)
+    assert abs(mzz - 1) < 1e-6 and abs(mxx + 1) < 1e-6 and abs(myy) < 1e-6
+    assert max(abs(myz), abs(mxz), abs(mxy)) < 1e-6
+    mxx, myy, mzz, myz, mxz, mxy = mt(90.0, 45.0, 90.0)
+    assert abs(mzz - 1) < 1e-6 and abs(myy + 1) < 1e-6 and abs(mxx) < 1e-6
+    mxx, myy, mzz, myz, mxz, mxy = mt(0.0, 60.0, -90.0)
+    s120 = np.sin(np.radians(120.0))
+    assert abs(mzz + s120) < 1e-6 and abs(mxx - s120) < 1e-6 and abs(myy) < 1e-6
+    # the fault dips east (to the right of north), so the east-up couple is
+    # cos(2 dip) = -0.5 with z up
+    assert abs(mxz - np.cos(np.radians(120.0))) < 1e-6
+    mxx, myy, mzz, myz, mxz, mxy = mt(0.0, 90.0, 0.0)
+    assert abs(mxy - 1) < 1e-6 and max(abs(mxx), abs(myy), abs(mzz), abs(mxz), abs(myz)) < 1e-6
+
+
+@pytest.mark.parametrize("mistake", ["flip_xz_yz", "swap_xy"])
+def test_fault_geometry_check_rejects_mistakes(mistake):
+    """A z-sign flip of Mxz/Myz, or x and y swapped, keeps the eigenvalues
+    (1, 0, -1) but fails the component check on these mechanisms."""
+    orc = po.load_oracle()
+    m = np.zeros(6, np.float32)
+    worst = 0.0
+    for sdr in _SDR:
+        orc.orc_angle2moment(*sdr, po.ptr(m))
+        bad = _fault_moment_enu(*sdr, **{mistake: True})
+        worst = max(worst, float(np.abs(m.astype(np.float64) - bad).max()))
+    assert worst > 0.3
+
+
 @pytest.mark.parametrize("sdr", [(30.0, 60.0, 90.0), (0.0, 90.0, 0.0), (211.0, 33.0, -47.0)])
 def test_double_couple_invariants(sdr):
     """Unit double couple: symmetric, traceless, eigenvalues (1, 0, -1)."""
