@@ -774,6 +774,13 @@ float orc_ricker(float t, float fc, float t0)
   return v;
 }
 
+float orc_ricker_deriv(float t, float fc, float t0)
+{
+  float u = (t - t0) * 2.0 * PI * fc;
+  float v = u * (-3 + 1.0 / 2.0 * u * u) * exp(-u * u / 4.0) * PI * fc;
+  return v;
+}
+
 void orc_angle2moment(float strike, float dip, float rake, float m[6])
 {
   float dp = dip / 180.0 * PI, st = strike / 180.0 * PI, rk = rake / 180.0 * PI;
@@ -811,6 +818,8 @@ struct orc_s {
   real_t *pA[6], *pB[6], *pD[6];
   real_t *srcm;             /* 6 x nsrc*max_nt*4, NULL without moment */
   real_t *srcf;             /* 3 x nsrc*max_nt*4, NULL without force */
+  real_t *srcfr;            /* 3 x nsurf*max_nt*4: surface forces' rates */
+  real_t *tsrc, *vsrc;      /* TxSrc..TzSrc, VxSrc..VzSrc: 3 x [ny][nx] each */
   real_t *pg, *dis;         /* peak ground motion maps, free surface only */
 };
 
@@ -886,6 +895,13 @@ orc_t *orc_create(const orc_cfg *cfg, const float *metric10, const float *med)
     o->srcf = (real_t *)malloc(3 * n * sizeof(real_t));
     for (size_t p = 0; p < 3 * n; p++) o->srcf[p] = cfg->src_fxyz[p];
   }
+  o->tsrc = (real_t *)calloc(3 * o->ss, sizeof(real_t));
+  o->vsrc = (real_t *)calloc(3 * o->ss, sizeof(real_t));
+  if (cfg->nsrc > 0 && cfg->src_fxyz && cfg->src_force_strict && cfg->nsurf > 0) {
+    size_t n = (size_t)cfg->nsurf * cfg->src_max_nt * 4;
+    o->srcfr = (real_t *)malloc(3 * n * sizeof(real_t));
+    for (size_t p = 0; p < 3 * n; p++) o->srcfr[p] = cfg->src_frate[p];
+  }
   return o;
 }
 
@@ -893,6 +909,7 @@ void orc_destroy(orc_t *o)
 {
   if (!o) return;
   free(o->metric); free(o->med); free(o->mats); free(o->srcm); free(o->srcf); free(o->pg);
+  free(o->srcfr); free(o->tsrc); free(o->vsrc);
   free(o->dis);
   for (int f = 0; f < 6; f++) { free(o->pA[f]); free(o->pB[f]); free(o->pD[f]); }
   free(o);
@@ -988,8 +1005,9 @@ static void rhs_inner(orc_t *o, const real_t *w, real_t *h,
       }
 }
 
-/* traction image for hV on the top layers, sv_curv_col_el.c:31-305.
- * Surface traction (TxSrc..) is zero: no surface-force sources. */
+/* traction image for hV on the top layers, sv_curv_col_el.c:31-305, with
+ * the surface traction TxSrc..TzSrc of the strict surface forces (zero
+ * without them) at the surface row and in the mirror */
 static void rhs_timg(orc_t *o, const real_t *w, real_t *h,
                      const op_t *ox, const op_t *oy, const op_t *oz, const int *zidx)
 {
@@ -1023,7 +1041,8 @@ static void rhs_timg(orc_t *o, const real_t *w, real_t *h,
             size_t q = p + oz->sh[n];
             vz[n] = jac[q] * (M[7 * V + q] * T0[q] + M[8 * V + q] * T1[q] + M[9 * V + q] * T2[q]);
           }
-          vz[n_free] = 0.0;  /* surface traction */
+          const real_t ts = o->tsrc[comp * o->ss + i + j * o->sl];
+          vz[n_free] = ts;   /* surface traction */
           for (int n = n_free + 1; n < zlen; n++) {
             int m = n_free - (n - n_free);
             real_t below;
@@ -1035,7 +1054,7 @@ static void rhs_timg(orc_t *o, const real_t *w, real_t *h,
               size_t q = p + (long)(zidx[n] - 2 * (n - n_free)) * (long)o->ss;
               below = jac[q] * (M[7 * V + q] * T0[q] + M[8 * V + q] * T1[q] + M[9 * V + q] * T2[q]);
             }
-            vz[n] = 2.0 * 0.0f - below;
+            vz[n] = 2.0 * ts - below;
           }
           real_t DxT = ox->c[0] * vx[0], DyT = oy->c[0] * vy[0], DzT = oz->c[0] * vz[0];
           for (int n = 1; n < ox->len; n++) DxT += ox->c[n] * vx[n];
@@ -1069,8 +1088,10 @@ static inline void hooke(real_t *h, size_t V, size_t p, const real_t *m9,
   h[TYZ * V + p] = mu * (xiz * X[1] + xiy * X[2] + etz * Y[1] + ety * Y[2] + ztz * Z[1] + zty * Z[2]);
 }
 
-/* hT on the top 3 layers, sv_curv_col_el_iso.c:452-634 (no surface-force
- * velocity term: VxSrc..VzSrc are zero without surface forces) */
+/* hT on the top 3 layers, sv_curv_col_el_iso.c:452-634; at the surface the
+ * isotropic forms add matF2Vz2 times the strict surface forces' velocity
+ * source VxSrc..VzSrc (sv_curv_col_el_iso.c:583-592; the VTI and anisotropic
+ * forms have no such term) */
 static void rhs_vlow(orc_t *o, const real_t *w, real_t *h,
                      const op_t *ox, const op_t *oy, const op_t ozl[4])
 {
@@ -1095,6 +1116,14 @@ static void rhs_vlow(orc_t *o, const real_t *w, real_t *h,
           for (int r = 0; r < 3; r++)
             Z[r] = mVx[ij + 3 * r + 0] * X[0] + mVx[ij + 3 * r + 1] * X[1] + mVx[ij + 3 * r + 2] * X[2]
                  + mVy[ij + 3 * r + 0] * Y[0] + mVy[ij + 3 * r + 1] * Y[1] + mVy[ij + 3 * r + 2] * Y[2];
+          if (ISO_LIKE(o)) {
+            const real_t *mF = o->mats + 2 * o->ss * 9;
+            size_t q = i + j * o->sl;
+            const real_t *vs = o->vsrc;
+            for (int r = 0; r < 3; r++)
+              Z[r] += mF[ij + 3 * r + 0] * vs[q] + mF[ij + 3 * r + 1] * vs[o->ss + q]
+                    + mF[ij + 3 * r + 2] * vs[2 * o->ss + q];
+          }
         } else {
           for (int c = 0; c < 3; c++) Z[c] = dop(&ozl[n], w + c * V, p);
         }
@@ -1246,7 +1275,7 @@ static void rhs_src(orc_t *o, int it, int istage, real_t *h)
     static const int slot[6] = {TXX, TYY, TZZ, TXZ, TYZ, TXY};
     if (!c->src_gauss) {
       size_t p = si + sj * o->sl + sk * o->ss;
-      if (o->srcf) {
+      if (o->srcf && (!c->src_force_strict || sk < o->nk2)) {
         real_t Vf = o->slw[p] / jac[p];
         h[VX * V + p] += Fv[0] * Vf;
         h[VY * V + p] += Fv[1] * Vf;
@@ -1269,7 +1298,7 @@ static void rhs_src(orc_t *o, int it, int istage, real_t *h)
               continue;
             size_t p = i + j * o->sl + k * o->ss;
             real_t coef = ext[ie];
-            if (o->srcf) add_force(o, Fv, p, coef, h);
+            if (o->srcf && (!c->src_force_strict || k < o->nk2)) add_force(o, Fv, p, coef, h);
             if (o->srcm) {
               real_t rjac = coef / jac[p];
               for (int m = 0; m < 6; m++) h[slot[m] * V + p] -= Mv[m] * rjac;
@@ -1389,10 +1418,74 @@ static void vis_free(orc_t *o, real_t *w)
     }
 }
 
+/* surface traction and velocity source of the strict surface forces for
+ * step it, stage istage, src_set_surface_layer_for_force (src_t.c:154-314):
+ * the slices start at zero; each listed source active at it adds its force
+ * (times its weight on the surface row, for the Gaussian) to TxSrc.. and its
+ * force rate (times the weight) over the surface jacobian to VxSrc.. */
+static void surface_layer(orc_t *o, int it, int istage)
+{
+  const orc_cfg *c = &o->cfg;
+  for (size_t q = 0; q < 3 * o->ss; q++) { o->tsrc[q] = 0; o->vsrc[q] = 0; }
+  if (!o->srcfr) return;
+  const real_t *jac = o->metric;
+  size_t nper = (size_t)c->nsrc * c->src_max_nt * 4;
+  size_t nperr = (size_t)c->nsurf * c->src_max_nt * 4;
+  int half = c->src_ext_half, len = 2 * half + 1;
+  float *ext = (float *)malloc(sizeof(float) * len * len * len);
+  for (int n = 0; n < c->nsurf; n++) {
+    int is = c->src_surf_idx[n];
+    if (it < c->src_it_begin[is] || it > c->src_it_end[is]) continue;
+    int si = c->src_ijk[3 * is], sj = c->src_ijk[3 * is + 1], sk = c->src_ijk[3 * is + 2];
+    size_t itr = (size_t)(it - c->src_it_begin[is]) * 4 + istage;
+    size_t ip = (size_t)is * c->src_max_nt * 4 + itr;
+    size_t ipr = (size_t)n * c->src_max_nt * 4 + itr;
+    real_t fv[3], fr[3];
+    for (int m = 0; m < 3; m++) {
+      fv[m] = o->srcf[m * nper + ip];
+      fr[m] = o->srcfr[m * nperr + ipr];
+    }
+    if (!c->src_gauss) {
+      size_t p = si + sj * o->sl + sk * o->ss, q = si + sj * o->sl;
+      for (int m = 0; m < 3; m++) {
+        o->tsrc[m * o->ss + q] += fv[m];
+        o->vsrc[m * o->ss + q] += fr[m] / jac[p];
+      }
+    } else {
+      int ke = o->nk2 - sk;
+      orc_gauss_weights(ext, c->src_inc[3 * is], c->src_inc[3 * is + 1], c->src_inc[3 * is + 2],
+                        c->src_ext_coef, half, ke);
+      int ie = (ke + half) * len * len;
+      int k = sk + ke;
+      for (int je = -half; je <= half; je++)
+        for (int iee = -half; iee <= half; iee++, ie++) {
+          int i = si + iee, j = sj + je;
+          real_t coef = ext[ie];
+          size_t p = i + j * o->sl + k * o->ss, q = i + j * o->sl;
+          for (int m = 0; m < 3; m++) {
+            o->tsrc[m * o->ss + q] += fv[m] * coef;
+            o->vsrc[m * o->ss + q] += fr[m] * coef / jac[p];
+          }
+        }
+    }
+  }
+  free(ext);
+}
+
+void orc_surface_force_layer(orc_t *o, int it, int istage, real_t *ts, real_t *vs)
+{
+  surface_layer(o, it, istage);
+  memcpy(ts, o->tsrc, 3 * o->ss * sizeof(real_t));
+  memcpy(vs, o->vsrc, 3 * o->ss * sizeof(real_t));
+}
+
 void orc_onestage(orc_t *o, int it, int ipair, int istage,
                   const real_t *w_cur, const real_t *aux_cur,
                   real_t *rhs, real_t *aux_rhs)
 {
+  /* the driver sets the surface-force slices before every stage
+   * (drv_rk_curv_col.c:227-230) */
+  surface_layer(o, it, istage);
   int dx = (pair_flags[ipair][0] + istage) % 2;
   int dy = (pair_flags[ipair][1] + istage) % 2;
   int dz = (pair_flags[ipair][2] + istage) % 2;

@@ -72,6 +72,17 @@ typedef struct {
    * force: Fx,Fy,Fz, each [nsrc][max_nt][4]; NULL for none.  src_mij may be
    * NULL for force-only sources. */
   const float *src_fxyz;
+  /* strict surface force (par_t.c:821-825, src_t.c:154-314, 1020-1170):
+   * with src_force_strict the force terms at k == nk2 are left out of the
+   * RHS (sv_curv_col_el.c:388, 447), and the nsurf sources listed in
+   * src_surf_idx (src->force_rate_indx: the force sources whose extent
+   * reaches the surface) set the surface traction TxSrc..TzSrc of the
+   * traction image and the velocity source VxSrc..VzSrc of vlow every stage;
+   * src_frate holds Fx_rate, Fy_rate, Fz_rate, each [nsurf][max_nt][4] */
+  int src_force_strict;
+  int nsurf;
+  const int *src_surf_idx;
+  const float *src_frate;
 } orc_cfg;
 
 typedef struct orc_s orc_t;
@@ -123,6 +134,8 @@ void orc_gauss_weights(float *w, float x0, float y0, float z0, float a,
                        int half, int half_below);
 float orc_ricker(float t, float fc, float t0);
 void orc_angle2moment(float strike, float dip, float rake, float m[6]);
+/* the derivative of fun_ricker, fun_ricker_deriv (src_t.c:2168-2176) */
+float orc_ricker_deriv(float t, float fc, float t0);
 
 /* ---- solver ---- */
 orc_t *orc_create(const orc_cfg *cfg, const float *metric10, const float *med);
@@ -139,6 +152,10 @@ void orc_onestage(orc_t *o, int it, int ipair, int istage,
 /* nt RK4 steps from level w (in/out) and aux (in/out, may be NULL if no PML);
  * seismo[(r*9+icmp)*nt_total + it0+n] sampled after each step when nrecv>0. */
 void orc_get_pg(const orc_t *o, real_t *pg, real_t *dis);
+/* the surface traction and velocity-source slices of step it, stage istage,
+ * src_set_surface_layer_for_force (src_t.c:154-314): ts = TxSrc,TySrc,TzSrc
+ * and vs = VxSrc,VySrc,VzSrc, [3][ny][nx] each (zero without surface forces) */
+void orc_surface_force_layer(orc_t *o, int it, int istage, real_t *ts, real_t *vs);
 void orc_run(orc_t *o, float dt, int it0, int nt, real_t *w, real_t *aux,
              int nrecv, const int *recv_idx, const float *recv_w,
              int nt_total, real_t *seismo);

@@ -26,7 +26,9 @@ class OrcCfg(C.Structure):
                 ("src_ext_coef", C.c_float),
                 ("src_ijk", _P), ("src_inc", _P), ("src_it_begin", _P), ("src_it_end", _P),
                 ("src_max_nt", C.c_int), ("src_mij", _P), ("medium_type", C.c_int),
-                ("nmaxwell", C.c_int), ("visco_wl", _P), ("xyz", _P), ("src_fxyz", _P)]
+                ("nmaxwell", C.c_int), ("visco_wl", _P), ("xyz", _P), ("src_fxyz", _P),
+                ("src_force_strict", C.c_int), ("nsurf", C.c_int), ("src_surf_idx", _P),
+                ("src_frate", _P)]
 
 
 MEDIUM_ISO, MEDIUM_VTI, MEDIUM_ANISO, MEDIUM_VISCO = 2, 3, 4, 5
@@ -40,7 +42,8 @@ class RefCfg(C.Structure):
                 ("nsrc", C.c_int), ("src_max_nt", C.c_int), ("src_ijk", _P),
                 ("src_it_begin", _P), ("src_it_end", _P), ("src_mij", _P),
                 ("medium_type", C.c_int), ("nmaxwell", C.c_int), ("visco_wl", _P),
-                ("src_fxyz", _P)]
+                ("src_fxyz", _P), ("src_force_strict", C.c_int), ("surf_slices", _P),
+                ("surf_nt", C.c_int)]
 
 
 def load_oracle(fp64=False):
@@ -82,6 +85,9 @@ def load_oracle(fp64=False):
     lib.orc_ricker.restype = C.c_float
     lib.orc_ricker.argtypes = [C.c_float, C.c_float, C.c_float]
     lib.orc_angle2moment.argtypes = [C.c_float, C.c_float, C.c_float, _P]
+    lib.orc_ricker_deriv.restype = C.c_float
+    lib.orc_ricker_deriv.argtypes = [C.c_float, C.c_float, C.c_float]
+    lib.orc_surface_force_layer.argtypes = [_P, C.c_int, C.c_int, _P, _P]
     return lib
 
 
@@ -164,6 +170,16 @@ class Oracle:
             cfg.src_max_nt = int(src["max_nt"])
             cfg.src_mij = arrs["mij"].ctypes.data if "mij" in arrs else None
             cfg.src_fxyz = arrs["fxyz"].ctypes.data if "fxyz" in arrs else None
+            # strict surface force: the sources listed in surf_idx carry rate
+            # tables frate (Fx_rate, Fy_rate, Fz_rate, [nsurf][max_nt][4])
+            cfg.src_force_strict = int(src.get("strict", 0))
+            if src.get("surf_idx") is not None and len(src["surf_idx"]) > 0:
+                si = np.ascontiguousarray(src["surf_idx"], np.int32)
+                fr = np.ascontiguousarray(src["frate"], np.float32)
+                self._keep += [si, fr]
+                cfg.nsurf = len(si)
+                cfg.src_surf_idx = si.ctypes.data
+                cfg.src_frate = fr.ctypes.data
         self.cfg = cfg
         self.metric = np.ascontiguousarray(case["metric"], np.float32)
         self.med = np.ascontiguousarray(case["med"], np.float32)
@@ -185,6 +201,27 @@ class Oracle:
         arhs = np.zeros(max(self.aux_len, 1), self.dtype)
         self.lib.orc_onestage(self.h, it, ipair, istage, ptr(w), ptr(aux), ptr(rhs), ptr(arhs))
         return rhs, arhs[: self.aux_len]
+
+    def surface_force_layer(self, it, istage):
+        """(TxSrc,TySrc,TzSrc) and (VxSrc,VySrc,VzSrc) of step it, stage
+        istage, each (3, ny, nx)."""
+        nx, ny = self.case["ni"] + 6, self.case["nj"] + 6
+        ts = np.zeros((3, ny, nx), self.dtype)
+        vs = np.zeros((3, ny, nx), self.dtype)
+        self.lib.orc_surface_force_layer(self.h, it, istage, ptr(ts), ptr(vs))
+        return ts, vs
+
+    def surface_slices(self, nt):
+        """The surface-force slices of steps 0..nt-1, every stage, in the
+        reference harness's layout [nt][4][6][ny][nx] (float32)."""
+        nx, ny = self.case["ni"] + 6, self.case["nj"] + 6
+        out = np.zeros((nt, 4, 6, ny, nx), np.float32)
+        for it in range(nt):
+            for s in range(4):
+                ts, vs = self.surface_force_layer(it, s)
+                out[it, s, :3] = ts
+                out[it, s, 3:] = vs
+        return out
 
     def run(self, dt, nt, w=None, aux=None, it0=0, recv_idx=None, recv_w=None, nt_total=None):
         w = (np.zeros(self.ncomp * self.vol, self.dtype) if w is None

@@ -65,6 +65,15 @@ typedef struct {
   /* point force sources: Fx, Fy, Fz back to back, same layout as src_mij;
    * NULL for none (src_mij may then be NULL too) */
   const float *src_fxyz;
+  /* strict surface force: src->is_surface_force_strict, and the surface
+   * traction / velocity-source slices TxSrc,TySrc,TzSrc,VxSrc,VySrc,VzSrc
+   * ([6][ny][nx]) of every step and stage, [surf_nt][4][6][ny][nx], as
+   * src_set_surface_layer_for_force (src_t.c:154-314, in a file that needs
+   * netCDF) leaves them before each stage (drv_rk_curv_col.c:227-230).
+   * refh_onestage reads entry 0.  NULL: all zero. */
+  int src_force_strict;
+  const float *surf_slices;
+  int surf_nt;
 } refh_cfg;
 
 typedef struct {
@@ -75,7 +84,9 @@ typedef struct {
   wav_t wav;
   bdry_t bdry;
   src_t src;
-  float *T0;   /* zero surface traction / velocity-source slices */
+  float *T0;   /* surface traction / velocity-source slices, 6 x [ny][nx] */
+  const float *surf;   /* the configured slices per step and stage, or NULL */
+  int surf_nt;
   float *Ylam[16], *Ymu[16], wl[16];
 } refh_ctx;
 
@@ -172,9 +183,12 @@ static void setup(refh_ctx *c, const refh_cfg *cfg, float *coords,
   bdry_pml_set(gd, &c->wav, &c->bdry, neighid, is_pml, nlay, amax, bmax, vel, 0);
 
   src_t *s = &c->src;
-  c->T0 = (float *)calloc(gd->siz_iz, sizeof(float));
-  s->TxSrc = s->TySrc = s->TzSrc = c->T0;
-  s->VxSrc = s->VySrc = s->VzSrc = c->T0;
+  c->T0 = (float *)calloc(6 * gd->siz_iz, sizeof(float));
+  s->TxSrc = c->T0; s->TySrc = c->T0 + gd->siz_iz; s->TzSrc = c->T0 + 2 * gd->siz_iz;
+  s->VxSrc = c->T0 + 3 * gd->siz_iz; s->VySrc = c->T0 + 4 * gd->siz_iz;
+  s->VzSrc = c->T0 + 5 * gd->siz_iz;
+  c->surf = cfg->surf_slices;
+  c->surf_nt = cfg->surf_nt;
   s->total_number = cfg->nsrc;
   s->dd_is_valid = 0;
   if (cfg->nsrc > 0) {
@@ -183,7 +197,7 @@ static void setup(refh_ctx *c, const refh_cfg *cfg, float *coords,
     s->itype_spatial_ext = CONST_SRC_SPATIAL_POINT;
     s->force_actived = cfg->src_fxyz != NULL;
     s->moment_actived = cfg->src_mij != NULL;
-    s->is_surface_force_strict = 0;
+    s->is_surface_force_strict = cfg->src_force_strict;
     s->si = (int *)malloc(cfg->nsrc * sizeof(int));
     s->sj = (int *)malloc(cfg->nsrc * sizeof(int));
     s->sk = (int *)malloc(cfg->nsrc * sizeof(int));
@@ -273,12 +287,14 @@ static void patch_timg_surface_row(refh_ctx *c, float *w_cur, float *rhs, fd_op_
           size_t q = p + fz->indx[n] * ss;
           vz[n] = m->jac[q] * (m->zeta_x[q] * T0[q] + m->zeta_y[q] * T1[q] + m->zeta_z[q] * T2[q]);
         }
-        vz[n_free] = 0.0f;
+        const float *TS = comp == 0 ? c->src.TxSrc : (comp == 1 ? c->src.TySrc : c->src.TzSrc);
+        const float ts = TS[i + j * sl];
+        vz[n_free] = ts;
         for (int n = n_free + 1; n < fz->total_len; n++) {
           long off = fz->indx[n] - 2 * (n - n_free);
           size_t q = p + off * (long)ss;
           float below = m->jac[q] * (m->zeta_x[q] * T0[q] + m->zeta_y[q] * T1[q] + m->zeta_z[q] * T2[q]);
-          vz[n] = 2.0 * 0.0f - below;
+          vz[n] = 2.0 * ts - below;
         }
         float DxT = fx->coef[0] * vx[0], DyT = fy->coef[0] * vy[0], DzT = fz->coef[0] * vz[0];
         for (int n = 1; n < fx->total_len; n++) DxT += fx->coef[n] * vx[n];
@@ -287,6 +303,17 @@ static void patch_timg_surface_row(refh_ctx *c, float *w_cur, float *rhs, fd_op_
         rhs[hpos[comp] + p] = (DxT + DyT + DzT) * slwjac;
       }
     }
+}
+
+/* the surface-force slices of step it, stage istage into src_t */
+static void load_surface_slices(refh_ctx *c, int it, int istage)
+{
+  size_t n = 6 * c->gd.siz_iz;
+  if (!c->surf || it >= c->surf_nt) {
+    memset(c->T0, 0, n * sizeof(float));
+    return;
+  }
+  memcpy(c->T0, c->surf + ((size_t)it * 4 + istage) * n, n * sizeof(float));
 }
 
 #define METRIC9(m) (m)->xi_x, (m)->xi_y, (m)->xi_z, (m)->eta_x, (m)->eta_y, (m)->eta_z, \
@@ -565,6 +592,7 @@ long refh_onestage(const refh_cfg *cfg, float *coords, float *metric10, float *m
   }
   c.src.it = it;
   c.src.istage = istage;
+  load_surface_slices(&c, 0, 0);
   harness_onestage(&c, wcur, rhs, ipair, istage, patch);
   memcpy(rhs_out, rhs, L * sizeof(float));
   off = 0;
@@ -614,6 +642,7 @@ int refh_run(const refh_cfg *cfg, float *coords, float *metric10, float *med3,
         a->cur = istage == 0 ? a->pre : a->tmp;
       }
       c.src.it = it; c.src.istage = istage;
+      load_surface_slices(&c, it, istage);
       harness_onestage(&c, w_cur, w_rhs, ipair, istage, 1);
       float ca = istage < 3 ? c.fd.rk_a[istage] * dt : 0.0f;
       float cb = c.fd.rk_b[istage] * dt;

@@ -267,6 +267,52 @@ def ricker_point_source(case, i, j, k, nt, dt, fc=2.0, t0=0.5, m0=1e16,
                 max_nt=nt, mij=mij.reshape(-1), gauss=int(gauss), ext_half=3, ext_coef=1.5)
 
 
+def ricker_force_source(case, ijk, nt, dt, fxyz=(1.0e12, -0.6e12, 2.0e12), fc=10.0, t0=0.05,
+                        gauss=False, inc=(0.0, 0.0, 0.0), strict=0, moment=None):
+    """Force sources with Ricker time functions, laid out as src_t.c:1047-1091:
+    F?[is][it][stage] = ricker(t) * f at t = it*dt + rk_t[stage]*dt.  ijk: one
+    (i, j, k) or a list of them.  strict: is_surface_force_strict; the sources
+    whose extent reaches the surface (sk + half >= nk2, src_t.c:361-371, half
+    0 for a point source, src_t.c:418) are listed in surf_idx with their rate
+    tables F?_rate = ricker'(t) * f (src_t.c:1082-1089).  moment: optional
+    (Mxx,..,Mxy) of a moment part with the same time function."""
+    orc = pyoracle.load_oracle()
+    ijk = np.array(ijk, np.int32).reshape(-1, 3)
+    ns = len(ijk)
+    rk_t = [0.0, 0.5, 0.5, 1.0]
+    stf = np.zeros((nt, 4), np.float32)
+    rate = np.zeros((nt, 4), np.float32)
+    for it in range(nt):
+        for s in range(4):
+            t = np.float32(np.float32(it * np.float32(dt)) + np.float32(np.float32(rk_t[s]) * np.float32(dt)))
+            stf[it, s] = orc.orc_ricker(float(t), fc, t0)
+            rate[it, s] = orc.orc_ricker_deriv(float(t), fc, t0)
+    f = np.array(fxyz, np.float32).reshape(-1, 3)
+    if len(f) == 1:
+        f = np.repeat(f, ns, axis=0)
+    fx = np.zeros((3, ns, nt, 4), np.float32)
+    for q in range(ns):
+        for m in range(3):
+            fx[m, q] = (stf * f[q, m]).astype(np.float32)
+    half = 3 if gauss else 0
+    nk2 = 3 + case["nk"] - 1
+    surf = [q for q in range(ns) if strict and ijk[q, 2] + half >= nk2]
+    fr = np.zeros((3, len(surf), nt, 4), np.float32)
+    for n, q in enumerate(surf):
+        for m in range(3):
+            fr[m, n] = (rate * f[q, m]).astype(np.float32)
+    src = dict(nsrc=ns, ijk=ijk.reshape(-1).tolist(), inc=list(inc) * ns if len(inc) == 3 else list(inc),
+               it_begin=[0] * ns, it_end=[nt - 1] * ns, max_nt=nt, mij=None,
+               fxyz=fx.reshape(-1), gauss=int(gauss), ext_half=3, ext_coef=1.5,
+               strict=int(strict), surf_idx=surf, frate=fr.reshape(-1))
+    if moment is not None:
+        mij = np.zeros((6, ns, nt, 4), np.float32)
+        for m in range(6):
+            mij[m] = (stf * np.float32(moment[m])).astype(np.float32)
+        src["mij"] = mij.reshape(-1)
+    return src
+
+
 def surface_receivers(case, n=4, seed=0):
     """Receivers on the free surface at grid nodes (dk forced to 0 there,
     io_funcs.c:366-368) with fractional di/dj."""
@@ -298,8 +344,10 @@ def corner_indices(i, j, k, nx, ny):
             i + (j + 1) * nx + (k + 1) * s, i + 1 + (j + 1) * nx + (k + 1) * s]
 
 
-def ref_cfg(case, src_arrays=None):
-    """The reference harness's config for the same case."""
+def ref_cfg(case, src_arrays=None, surf_slices=None):
+    """The reference harness's config for the same case; surf_slices: the
+    surface-force slices [nt][4][6][ny][nx] the harness hands to the
+    reference's traction image and vlow (Oracle.surface_slices)."""
     c = pyoracle.RefCfg()
     c.ni, c.nj, c.nk = case["ni"], case["nj"], case["nk"]
     c.free_top = int(case["free_top"])
@@ -331,6 +379,12 @@ def ref_cfg(case, src_arrays=None):
                 t = np.ascontiguousarray(src[key], np.float32)
                 keep.append(t)
                 setattr(c, fld, t.ctypes.data)
+        c.src_force_strict = int(src.get("strict", 0))
+    if surf_slices is not None:
+        t = np.ascontiguousarray(surf_slices, np.float32)
+        keep.append(t)
+        c.surf_slices = t.ctypes.data
+        c.surf_nt = t.shape[0]
     return c, keep
 
 
