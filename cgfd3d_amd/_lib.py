@@ -37,6 +37,9 @@ SIGNATURES = [
     ("cgfdh_set_free_surface", C.c_int, [_P, _P, _P, _P]),
     ("cgfdh_set_moment_sources", C.c_int,
      [_P, C.c_int, _P, _P, _P, C.c_int, _P, C.c_int, C.c_int, _P]),
+    ("cgfdh_set_sources", C.c_int,
+     [_P, C.c_int, _P, _P, _P, C.c_int, _P, _P, C.c_int, C.c_int, _P, C.c_int, C.c_int, _P,
+      _P]),
     ("cgfdh_set_receivers", C.c_int, [_P, C.c_int, _P, _P]),
     ("cgfdh_set_wavefield", C.c_int, [_P, _P]),
     ("cgfdh_get_wavefield", C.c_int, [_P, _P]),

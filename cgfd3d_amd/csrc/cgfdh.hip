@@ -196,10 +196,15 @@ struct cgfdh_solver {
   FaceHost face[6];
   int nsrc = 0;
   SrcDev* src = nullptr;
-  float* mij = nullptr;
+  float* mij = nullptr;          // moment tables (NULL: none)
+  float* fxyz = nullptr;         // force tables (NULL: none)
   long mij_stride = 0;
   int src_max_nt = 0, src_spatial = 1, src_half = 0;
   float* src_coef = nullptr;
+  int src_strict = 0;            // strict surface force
+  int nsurf = 0;                 // surface forces: indices and force-rate tables
+  int* surf_idx = nullptr;
+  float* frate = nullptr;
   int nrecv = 0;
   int* recv_idx = nullptr;
   float* recv_w = nullptr;
@@ -215,6 +220,9 @@ struct cgfdh_solver {
   int variant = 12;              // set per medium in cgfdh_create
   int tgx = 0, tgy = 0;          // tile group (0: the default)
   bool metric_set = false, medium_set = false, mats_set = false;
+  // configuration generation: every setter bumps it, so a batch graph
+  // captured under an older configuration is recaptured before its replay
+  unsigned long gen = 0;
   hipEvent_t ev_run[2];
   std::vector<hipEvent_t> ev_stage;
   double last_run_ms = 0, last_stage_ms = 0;
@@ -249,7 +257,7 @@ static void free_all(cgfdh_solver* s)
     F(f.A); F(f.B); F(f.D);
     for (auto& a : f.aux) F(a);
   }
-  F(s->src); F(s->mij); F(s->src_coef);
+  F(s->src); F(s->mij); F(s->fxyz); F(s->src_coef); F(s->surf_idx); F(s->frate);
   F(s->recv_idx); F(s->recv_w); F(s->seismo); F(s->pg); F(s->outbuf); F(s->abl);
   F(s->qs);
 }
@@ -363,6 +371,7 @@ static int h2d(cgfdh_solver* s, float* dst, const float* src, size_t n)
 
 extern "C" int cgfdh_set_metric(cgfdh_solver* s, const float* m)
 {
+  if (s) s->gen++;
   if (!s || !m) return fail(CGFDH_EINVAL, "cgfdh_set_metric: null argument");
   int rc = h2d(s, s->met, m, 10 * s->vol);
   if (rc == 0) s->metric_set = true;
@@ -372,6 +381,7 @@ extern "C" int cgfdh_set_metric(cgfdh_solver* s, const float* m)
 extern "C" int cgfdh_set_medium_iso(cgfdh_solver* s, const float* lam, const float* mu,
                                     const float* slw)
 {
+  if (s) s->gen++;
   if (!s || !lam || !mu || !slw) return fail(CGFDH_EINVAL, "cgfdh_set_medium_iso: null argument");
   if (s->d.medium_type != CGFDH_MEDIUM_ELASTIC_ISO)
     return fail(CGFDH_ESTATE, "cgfdh_set_medium_iso: solver medium_type is %d", s->d.medium_type);
@@ -386,6 +396,7 @@ extern "C" int cgfdh_set_medium_vti(cgfdh_solver* s, const float* c11, const flo
                                     const float* c33, const float* c55, const float* c66,
                                     const float* slw)
 {
+  if (s) s->gen++;
   if (!s || !c11 || !c13 || !c33 || !c55 || !c66 || !slw)
     return fail(CGFDH_EINVAL, "cgfdh_set_medium_vti: null argument");
   if (s->d.medium_type != CGFDH_MEDIUM_ELASTIC_VTI)
@@ -399,6 +410,7 @@ extern "C" int cgfdh_set_medium_vti(cgfdh_solver* s, const float* c11, const flo
 
 extern "C" int cgfdh_set_medium_aniso(cgfdh_solver* s, const float* cij, const float* slw)
 {
+  if (s) s->gen++;
   if (!s || !cij || !slw) return fail(CGFDH_EINVAL, "cgfdh_set_medium_aniso: null argument");
   if (s->d.medium_type != CGFDH_MEDIUM_ELASTIC_ANISO)
     return fail(CGFDH_ESTATE, "cgfdh_set_medium_aniso: solver medium_type is %d",
@@ -412,6 +424,7 @@ extern "C" int cgfdh_set_medium_aniso(cgfdh_solver* s, const float* cij, const f
 extern "C" int cgfdh_set_medium_visco(cgfdh_solver* s, const float* lambda, const float* mu,
                                       const float* slw, const float* ylam_ymu, const float* wl)
 {
+  if (s) s->gen++;
   if (!s || !lambda || !mu || !slw || !ylam_ymu || !wl)
     return fail(CGFDH_EINVAL, "cgfdh_set_medium_visco: null argument");
   if (s->d.medium_type != CGFDH_MEDIUM_VISCO_ISO)
@@ -428,15 +441,19 @@ extern "C" int cgfdh_set_medium_visco(cgfdh_solver* s, const float* lambda, cons
 
 extern "C" int cgfdh_set_free_surface_matD(cgfdh_solver* s, const float* matD)
 {
+  if (s) s->gen++;
   if (!s || !matD) return fail(CGFDH_EINVAL, "cgfdh_set_free_surface_matD: null argument");
   return h2d(s, s->mats + 27 * s->ss, matD, 9 * (size_t)s->ss);
 }
 
 extern "C" int cgfdh_set_graves_qs(cgfdh_solver* s, const float* Qs, float freq)
 {
+  if (s) s->gen++;
   if (!s || !Qs) return fail(CGFDH_EINVAL, "cgfdh_set_graves_qs: null argument");
-  if (s->d.medium_type != CGFDH_MEDIUM_ELASTIC_ISO)
-    return fail(CGFDH_ESTATE, "cgfdh_set_graves_qs: needs the isotropic elastic medium "
+  // any elastic medium (md->visco_type is Graves-Qs, drv_rk_curv_col.c:414-416);
+  // the GMB visco-elastic medium has its own attenuation
+  if (s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO)
+    return fail(CGFDH_ESTATE, "cgfdh_set_graves_qs: needs an elastic medium "
                 "(medium_type %d)", s->d.medium_type);
   if (!(freq > 0.0f)) return fail(CGFDH_EINVAL, "cgfdh_set_graves_qs: freq %g", (double)freq);
   if (!s->qs) HIPCHK(hipMalloc(&s->qs, (size_t)s->vol * sizeof(float)));
@@ -449,6 +466,7 @@ extern "C" int cgfdh_set_graves_qs(cgfdh_solver* s, const float* Qs, float freq)
 extern "C" int cgfdh_set_ablexp(cgfdh_solver* s, const float* Ex, const float* Ey,
                                 const float* Ez, const int* blocks)
 {
+  if (s) s->gen++;
   if (!s || !Ex || !Ey || !Ez || !blocks)
     return fail(CGFDH_EINVAL, "cgfdh_set_ablexp: null argument");
   int blk[6][6] = {}, n = 0;
@@ -477,6 +495,7 @@ extern "C" int cgfdh_set_ablexp(cgfdh_solver* s, const float* Ex, const float* E
 extern "C" int cgfdh_set_pml_face(cgfdh_solver* s, int f, const float* A, const float* B,
                                   const float* D)
 {
+  if (s) s->gen++;
   if (!s || f < 0 || f > 5 || !A || !B || !D)
     return fail(CGFDH_EINVAL, "cgfdh_set_pml_face: bad argument (face %d)", f);
   if (!s->face[f].on) return fail(CGFDH_EINVAL, "cgfdh_set_pml_face: face %d has no PML", f);
@@ -491,6 +510,7 @@ extern "C" int cgfdh_set_pml_face(cgfdh_solver* s, int f, const float* A, const 
 extern "C" int cgfdh_set_free_surface(cgfdh_solver* s, const float* mx, const float* my,
                                       const float* mf)
 {
+  if (s) s->gen++;
   if (!s || !mx || !my || !mf) return fail(CGFDH_EINVAL, "cgfdh_set_free_surface: null argument");
   size_t n = 9 * (size_t)s->ss;
   int rc = h2d(s, s->mats, mx, n);
@@ -500,24 +520,33 @@ extern "C" int cgfdh_set_free_surface(cgfdh_solver* s, const float* mx, const fl
   return rc;
 }
 
-extern "C" int cgfdh_set_moment_sources(cgfdh_solver* s, int nsrc, const int* ijk,
-                                        const int* it_begin, const int* it_end, int max_nt,
-                                        const float* mij, int spatial, int ext_half,
-                                        const float* ext_coefs)
+extern "C" int cgfdh_set_sources(cgfdh_solver* s, int nsrc, const int* ijk,
+                                 const int* it_begin, const int* it_end, int max_nt,
+                                 const float* mij, const float* fxyz, int spatial,
+                                 int ext_half, const float* ext_coefs, int strict, int nsurf,
+                                 const int* surf_idx, const float* frate)
 {
-  if (!s || nsrc < 0) return fail(CGFDH_EINVAL, "cgfdh_set_moment_sources: bad argument");
-  if (s->src) { (void)hipFree(s->src); s->src = nullptr; }
-  if (s->mij) { (void)hipFree(s->mij); s->mij = nullptr; }
-  if (s->src_coef) { (void)hipFree(s->src_coef); s->src_coef = nullptr; }
+  if (s) s->gen++;
+  if (!s || nsrc < 0 || nsurf < 0) return fail(CGFDH_EINVAL, "cgfdh_set_sources: bad argument");
+  auto F = [](auto*& p) { if (p) { (void)hipFree(p); p = nullptr; } };
+  F(s->src); F(s->mij); F(s->fxyz); F(s->src_coef); F(s->surf_idx); F(s->frate);
   s->nsrc = 0;
+  s->nsurf = 0;
+  s->src_strict = 0;
   if (nsrc == 0) return CGFDH_OK;
-  if (!ijk || !it_begin || !it_end || !mij || max_nt < 1)
-    return fail(CGFDH_EINVAL, "cgfdh_set_moment_sources: missing tables");
+  if (!ijk || !it_begin || !it_end || (!mij && !fxyz) || max_nt < 1)
+    return fail(CGFDH_EINVAL, "cgfdh_set_sources: missing tables");
   if (spatial != CGFDH_SRC_POINT && spatial != CGFDH_SRC_GAUSSIAN)
-    return fail(CGFDH_EINVAL, "cgfdh_set_moment_sources: spatial type %d", spatial);
+    return fail(CGFDH_EINVAL, "cgfdh_set_sources: spatial type %d", spatial);
   if (spatial == CGFDH_SRC_GAUSSIAN && (!ext_coefs || ext_half < 0 || ext_half > 8))
-    return fail(CGFDH_EINVAL, "cgfdh_set_moment_sources: Gaussian needs weights, half %d",
-                ext_half);
+    return fail(CGFDH_EINVAL, "cgfdh_set_sources: Gaussian needs weights, half %d", ext_half);
+  const int half = spatial == CGFDH_SRC_GAUSSIAN ? ext_half : 0;
+  if (nsurf > 0 && (!fxyz || !strict || !surf_idx || !frate))
+    return fail(CGFDH_EINVAL,
+                "cgfdh_set_sources: surface forces need force tables, strict mode, their "
+                "indices and rate tables");
+  if (nsurf > 0 && !s->d.free_top)
+    return fail(CGFDH_EINVAL, "cgfdh_set_sources: surface forces need the free surface");
   std::vector<SrcDev> hs(nsrc);
   int L = 2 * ext_half + 1;
   for (int is = 0; is < nsrc; is++) {
@@ -525,34 +554,77 @@ extern "C" int cgfdh_set_moment_sources(cgfdh_solver* s, int nsrc, const int* ij
     q.si = ijk[3 * is]; q.sj = ijk[3 * is + 1]; q.sk = ijk[3 * is + 2];
     if (q.si < s->ni1 || q.si > s->ni2 || q.sj < s->nj1 || q.sj > s->nj2 || q.sk < s->nk1 ||
         q.sk > s->nk2)
-      return fail(CGFDH_EINVAL, "cgfdh_set_moment_sources: source %d at (%d,%d,%d) outside the grid",
+      return fail(CGFDH_EINVAL, "cgfdh_set_sources: source %d at (%d,%d,%d) outside the grid",
                   is, q.si, q.sj, q.sk);
     q.it_begin = it_begin[is];
     q.it_end = it_end[is];
     if (q.it_end - q.it_begin + 1 > max_nt)
-      return fail(CGFDH_EINVAL, "cgfdh_set_moment_sources: source %d spans more than max_nt", is);
+      return fail(CGFDH_EINVAL, "cgfdh_set_sources: source %d spans more than max_nt", is);
     q.kb = (q.sk + ext_half < s->nk2) ? ext_half : s->nk2 - q.sk;
     q.coef_off = is * L * L * L;
+  }
+  // a surface force reaches the surface (src_glob_ext_reach_free,
+  // src_t.c:361-371): its point, or its Gaussian's top row, is on it
+  for (int n = 0; n < nsurf; n++) {
+    const int is = surf_idx[n];
+    if (is < 0 || is >= nsrc)
+      return fail(CGFDH_EINVAL, "cgfdh_set_sources: surface force %d names source %d", n, is);
+    if (hs[is].sk + half < s->nk2)
+      return fail(CGFDH_EINVAL,
+                  "cgfdh_set_sources: surface force %d (source %d, k %d) does not reach the "
+                  "surface",
+                  n, is, hs[is].sk);
+    if (spatial == CGFDH_SRC_GAUSSIAN &&
+        (hs[is].si - half < 0 || hs[is].si + half >= s->nx || hs[is].sj - half < 0 ||
+         hs[is].sj + half >= s->ny))
+      return fail(CGFDH_EINVAL, "cgfdh_set_sources: surface force %d overhangs the array", n);
   }
   HIPCHK(hipMalloc((void**)&s->src, nsrc * sizeof(SrcDev)));
   HIPCHK(hipMemcpy(s->src, hs.data(), nsrc * sizeof(SrcDev), hipMemcpyHostToDevice));
   s->mij_stride = (long)nsrc * max_nt * 4;
-  HIPCHK(hipMalloc((void**)&s->mij, 6 * s->mij_stride * sizeof(float)));
-  HIPCHK(hipMemcpy(s->mij, mij, 6 * s->mij_stride * sizeof(float), hipMemcpyHostToDevice));
+  if (mij) {
+    HIPCHK(hipMalloc((void**)&s->mij, 6 * s->mij_stride * sizeof(float)));
+    HIPCHK(hipMemcpy(s->mij, mij, 6 * s->mij_stride * sizeof(float), hipMemcpyHostToDevice));
+  }
+  if (fxyz) {
+    HIPCHK(hipMalloc((void**)&s->fxyz, 3 * s->mij_stride * sizeof(float)));
+    HIPCHK(hipMemcpy(s->fxyz, fxyz, 3 * s->mij_stride * sizeof(float), hipMemcpyHostToDevice));
+  }
   if (spatial == CGFDH_SRC_GAUSSIAN) {
     size_t n = (size_t)nsrc * L * L * L;
     HIPCHK(hipMalloc((void**)&s->src_coef, n * sizeof(float)));
     HIPCHK(hipMemcpy(s->src_coef, ext_coefs, n * sizeof(float), hipMemcpyHostToDevice));
   }
+  if (nsurf > 0) {
+    const size_t nr = 3 * (size_t)nsurf * max_nt * 4;
+    HIPCHK(hipMalloc((void**)&s->surf_idx, nsurf * sizeof(int)));
+    HIPCHK(hipMemcpy(s->surf_idx, surf_idx, nsurf * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc((void**)&s->frate, nr * sizeof(float)));
+    HIPCHK(hipMemcpy(s->frate, frate, nr * sizeof(float), hipMemcpyHostToDevice));
+  }
   s->nsrc = nsrc;
+  s->nsurf = nsurf;
+  s->src_strict = fxyz && strict ? 1 : 0;
   s->src_max_nt = max_nt;
   s->src_spatial = spatial;
-  s->src_half = spatial == CGFDH_SRC_GAUSSIAN ? ext_half : 0;
+  s->src_half = half;
   return CGFDH_OK;
+}
+
+extern "C" int cgfdh_set_moment_sources(cgfdh_solver* s, int nsrc, const int* ijk,
+                                        const int* it_begin, const int* it_end, int max_nt,
+                                        const float* mij, int spatial, int ext_half,
+                                        const float* ext_coefs)
+{
+  if (s) s->gen++;
+  if (nsrc > 0 && !mij) return fail(CGFDH_EINVAL, "cgfdh_set_moment_sources: missing tables");
+  return cgfdh_set_sources(s, nsrc, ijk, it_begin, it_end, max_nt, mij, nullptr, spatial,
+                           ext_half, ext_coefs, 0, 0, nullptr, nullptr);
 }
 
 extern "C" int cgfdh_set_receivers(cgfdh_solver* s, int nrecv, const int* idx, const float* dijk)
 {
+  if (s) s->gen++;
   if (!s || nrecv < 0) return fail(CGFDH_EINVAL, "cgfdh_set_receivers: bad argument");
   if (s->recv_idx) { (void)hipFree(s->recv_idx); s->recv_idx = nullptr; }
   if (s->recv_w) { (void)hipFree(s->recv_w); s->recv_w = nullptr; }
@@ -576,6 +648,7 @@ extern "C" int cgfdh_set_receivers(cgfdh_solver* s, int nrecv, const int* idx, c
 
 extern "C" int cgfdh_set_wavefield(cgfdh_solver* s, const float* w)
 {
+  if (s) s->gen++;
   if (!s || !w) return fail(CGFDH_EINVAL, "cgfdh_set_wavefield: null argument");
   return h2d(s, s->lvl[0], w, (size_t)s->nw * s->vol);
 }
@@ -616,6 +689,7 @@ static int aux_copy(cgfdh_solver* s, int level, float* host, bool to_dev)
 
 extern "C" int cgfdh_set_pml_aux(cgfdh_solver* s, const float* aux)
 {
+  if (s) s->gen++;
   if (!s || !aux) return fail(CGFDH_EINVAL, "cgfdh_set_pml_aux: null argument");
   return aux_copy(s, 0, (float*)aux, true);
 }
@@ -638,6 +712,7 @@ extern "C" int cgfdh_get_seismograms(cgfdh_solver* s, float* out)
 
 extern "C" int cgfdh_set_kernel_variant(cgfdh_solver* s, int v)
 {
+  if (s) s->gen++;
   if (!s || v < 0 || v > 15) return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: %d", v);
   if (s->d.medium_type != CGFDH_MEDIUM_ELASTIC_ISO && v >= 1 && v <= 3)
     return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: variant %d is built for the isotropic "
@@ -651,6 +726,7 @@ extern "C" int cgfdh_set_kernel_variant(cgfdh_solver* s, int v)
 
 extern "C" int cgfdh_set_tile_group(cgfdh_solver* s, int gx, int gy)
 {
+  if (s) s->gen++;
   if (!s || gx < 0 || gy < 0) return fail(CGFDH_EINVAL, "cgfdh_set_tile_group: bad argument");
   s->tgx = gx;
   s->tgy = gy;
@@ -706,9 +782,14 @@ static StageParams make_params(cgfdh_solver* s, int it, int istage, const float*
     D.aux_pre = F.aux[0]; D.aux_tmp = F.aux[1]; D.aux_end = F.aux[2];
   }
   P.npml = n;
-  P.nsrc = s->nsrc; P.src = s->src; P.mij = s->mij; P.mij_stride = s->mij_stride;
+  P.nsrc = s->nsrc; P.src = s->src; P.mij = s->mij; P.fxyz = s->fxyz;
+  P.mij_stride = s->mij_stride;
   P.src_max_nt = s->src_max_nt; P.src_spatial = s->src_spatial; P.src_half = s->src_half;
   P.src_coef = s->src_coef;
+  P.src_strict = s->src_strict;
+  P.nsurf = s->nsurf; P.surf_idx = s->surf_idx; P.frate = s->frate;
+  P.frate_stride = (long)s->nsurf * s->src_max_nt * 4;
+  P.matF = s->mats + 18 * s->ss;
   P.it = it; P.istage = istage;
   return P;
 }
@@ -969,6 +1050,7 @@ extern "C" int cgfdh_onestage(cgfdh_solver* s, int it, int ipair, int istage, co
 
 extern "C" int cgfdh_compute_metric(cgfdh_solver* s, const float* xyz)
 {
+  if (s) s->gen++;
   if (!s || !xyz) return fail(CGFDH_EINVAL, "cgfdh_compute_metric: null argument");
   float* d = nullptr;
   HIPCHK(hipMalloc((void**)&d, 3 * s->vol * sizeof(float)));
@@ -1004,6 +1086,7 @@ extern "C" int cgfdh_compute_metric(cgfdh_solver* s, const float* xyz)
 
 extern "C" int cgfdh_compute_free_surface(cgfdh_solver* s)
 {
+  if (s) s->gen++;
   if (!s) return fail(CGFDH_EINVAL, "cgfdh_compute_free_surface: null solver");
   if (!s->metric_set || !s->medium_set)
     return fail(CGFDH_ESTATE, "cgfdh_compute_free_surface: metric and medium first");
@@ -1406,8 +1489,34 @@ done:
 
 __global__ void set_int_kernel(int* p, int v) { *p = v; }
 
+// what a captured cycle of one member depends on beyond device memory
+// contents: its configuration generation and the level pointers it started
+// from (eager steps in between swap them)
+struct CaptureKey {
+  unsigned long gen;
+  const float* lvl0;
+  const float* aux0[6];
+  bool operator==(const CaptureKey& o) const
+  {
+    if (gen != o.gen || lvl0 != o.lvl0) return false;
+    for (int f = 0; f < 6; f++)
+      if (aux0[f] != o.aux0[f]) return false;
+    return true;
+  }
+};
+
+static CaptureKey capture_key(const cgfdh_solver* s)
+{
+  CaptureKey k;
+  k.gen = s->gen;
+  k.lvl0 = s->lvl[0];
+  for (int f = 0; f < 6; f++) k.aux0[f] = s->face[f].aux[0];
+  return k;
+}
+
 struct cgfdh_batch {
   std::vector<cgfdh_solver*> sv;
+  std::vector<CaptureKey> keys;     // members' keys when the graph was captured
   hipStream_t stream = nullptr;
   int* it_off = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1465,6 +1574,8 @@ extern "C" int cgfdh_batch_destroy(cgfdh_batch* b)
 static int batch_capture(cgfdh_batch* b, int it0)
 {
   batch_drop_graph(b);
+  b->keys.clear();
+  for (auto s : b->sv) b->keys.push_back(capture_key(s));
   hipGraph_t g = nullptr;
   HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeGlobal));
   int rc = CGFDH_OK;
@@ -1507,8 +1618,12 @@ extern "C" int cgfdh_batch_run(cgfdh_batch* b, int it0, int nsteps)
   int it = it0;
   const int cycles = nsteps / 8;
   if (cycles > 0) {
-    // a captured cycle serves every start step congruent to it mod 8
-    if (!b->exec || (it - b->cap_it0) % 8 != 0 || it < b->cap_it0) {
+    // a captured cycle serves every start step congruent to it mod 8, while
+    // no member's configuration or level pointers have changed since
+    bool stale = false;
+    for (size_t i = 0; i < b->sv.size() && b->exec && !stale; i++)
+      stale = !(capture_key(b->sv[i]) == b->keys[i]);
+    if (!b->exec || stale || (it - b->cap_it0) % 8 != 0 || it < b->cap_it0) {
       int rc = batch_capture(b, it);
       if (rc) return rc;
     }

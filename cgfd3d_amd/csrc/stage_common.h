@@ -92,12 +92,23 @@ struct StageParams {
   float* aux_rhs_out[6];    // KIND_RHS only, per enabled face
   int nsrc;
   const SrcDev* src;
-  const float* mij;         // 6 arrays of mij_stride
+  const float* mij;         // 6 arrays of mij_stride (NULL: no moment sources)
+  const float* fxyz;        // 3 arrays of mij_stride (NULL: no force sources)
   long mij_stride;
   int src_max_nt;
   int src_spatial;          // 1 point, 2 gaussian
   int src_half;
   const float* src_coef;
+  // strict surface force: forces on the surface row are left to the free
+  // surface, which takes the listed sources' forces (fxyz) as surface
+  // traction and their force rates (frate, 3 arrays of frate_stride) as a
+  // velocity source
+  int src_strict;
+  int nsurf;
+  const int* surf_idx;
+  const float* frate;
+  long frate_stride;
+  const float* matF;        // matF2Vz2 [ny][nx][3][3]
   int it, istage;
   const int* it_off;        // if set, a device step offset added to it (graph replays)
   // visco-elastic GMB: bodies, wavefield components (9 + 6 nmax), relaxation
@@ -137,10 +148,56 @@ __device__ __forceinline__ float tvec(const StageParams& P, long q, int e, const
                  M[(e + 2) * P.vol + q] * T2[q]);
 }
 
-// traction-image hV on the top rows (sv_curv_col_el.c:31-305), zero surface
-// traction.  Mirror values come from the grid points they stand for.
+// the surface-force slices at surface point (i, j) for the current step and
+// stage, src_set_surface_layer_for_force (src_t.c:154-314): ts = TxSrc,
+// TySrc, TzSrc (the listed sources' forces, times the Gaussian weight of
+// the surface row) and vs = VxSrc, VySrc, VzSrc (their force rates times the
+// weight, over the jacobian at the source's surface point), summed in list
+// order from zero
+__device__ __forceinline__ void surface_force(const StageParams& P, int i, int j, float* ts,
+                                              float* vs)
+{
+  ts[0] = ts[1] = ts[2] = 0.0f;
+  vs[0] = vs[1] = vs[2] = 0.0f;
+  const int it = P.it + (P.it_off ? *P.it_off : 0);
+  for (int n = 0; n < P.nsurf; n++) {
+    const int is = P.surf_idx[n];
+    const SrcDev S = P.src[is];
+    if (it < S.it_begin || it > S.it_end) continue;
+    const long itr = (long)(it - S.it_begin) * 4 + P.istage;
+    const long ip = (long)is * P.src_max_nt * 4 + itr;
+    const long ipr = (long)n * P.src_max_nt * 4 + itr;
+    if (P.src_spatial == 1) {
+      if (i != S.si || j != S.sj) continue;
+      const float jac = P.met[(long)S.si + (long)S.sj * P.nx + (long)S.sk * P.ss];
+#pragma unroll
+      for (int m = 0; m < 3; m++) {
+        ts[m] += P.fxyz[m * P.mij_stride + ip];
+        vs[m] += P.frate[m * P.frate_stride + ipr] / jac;
+      }
+    } else {
+      const int h = P.src_half;
+      const int di = i - S.si, dj = j - S.sj;
+      if (di < -h || di > h || dj < -h || dj > h) continue;
+      const int L = 2 * h + 1;
+      const int ke = P.nk2 - S.sk;
+      const float coef = P.src_coef[S.coef_off + ((ke + h) * L + (dj + h)) * L + (di + h)];
+      const float jac = P.met[(long)i + (long)j * P.nx + (long)P.nk2 * P.ss];
+#pragma unroll
+      for (int m = 0; m < 3; m++) {
+        ts[m] += P.fxyz[m * P.mij_stride + ip] * coef;
+        vs[m] += P.frate[m * P.frate_stride + ipr] * coef / jac;
+      }
+    }
+  }
+}
+
+// traction-image hV on the top rows (sv_curv_col_el.c:31-305): the surface
+// traction (zero, or the strict surface forces' TxSrc..TzSrc) at the
+// surface row and mirrored above it.  Mirror values come from the grid
+// points they stand for.
 template <int DX, int DY, int DZ>
-__device__ void timg_hv(const StageParams& P, int k, long p, float* r)
+__device__ void timg_hv(const StageParams& P, int i, int j, int k, long p, float* r)
 {
   const float* w = P.wcur;
   const long V = P.vol, sl = P.nx, ss = P.ss;
@@ -148,6 +205,11 @@ __device__ void timg_hv(const StageParams& P, int k, long p, float* r)
   const int n_free = P.nk2 - k - zlo;
   const float slwjac = P.slw[p] / P.met[p];
   const int rows[3][3] = {{TXX, TXY, TXZ}, {TXY, TYY, TYZ}, {TXZ, TYZ, TZZ}};
+  float ts[3] = {0.0f, 0.0f, 0.0f};
+  if (P.nsurf > 0) {
+    float vs[3];
+    surface_force(P, i, j, ts, vs);
+  }
 #pragma unroll
   for (int comp = 0; comp < 3; comp++) {
     const float* T0 = w + rows[comp][0] * V;
@@ -164,11 +226,11 @@ __device__ void timg_hv(const StageParams& P, int k, long p, float* r)
       if (n < n_free) {
         vz[n] = tvec(P, p + (long)(zlo + n) * ss, 7, T0, T1, T2);
       } else if (n == n_free) {
-        vz[n] = 0.0f;
+        vz[n] = ts[comp];
       } else {
         const long off = (zlo + n) - 2 * (n - n_free);
         const float below = tvec(P, p + off * ss, 7, T0, T1, T2);
-        vz[n] = (float)(2.0 * 0.0f - (double)below);
+        vz[n] = (float)(2.0 * (double)ts[comp] - (double)below);
       }
     }
     const float DxT = d5v<DX>(vx[0], vx[1], vx[2], vx[3], vx[4]);
@@ -180,7 +242,10 @@ __device__ void timg_hv(const StageParams& P, int k, long p, float* r)
 
 // hT on the top 3 rows (sv_curv_col_el_iso.c:452-634 and the VTI / general
 // anisotropic equivalents): matrices at the surface, the 2- and 3-point
-// one-sided z operators below it
+// one-sided z operators below it.  The isotropic forms (elastic and
+// visco-elastic) add matF2Vz2 times the strict surface forces' velocity
+// source at the surface (sv_curv_col_el_iso.c:583-592); with no such force
+// that term is zero and skipped.
 template <int DZ, int MT>
 __device__ void vlow_ht(const StageParams& P, int i, int j, int k, long p, const float* m,
                         const Medium<MT>& md, const float* X, const float* Y, float* r)
@@ -194,6 +259,14 @@ __device__ void vlow_ht(const StageParams& P, int i, int j, int k, long p, const
       Z[q] = P.matVx[ij + 3 * q + 0] * X[0] + P.matVx[ij + 3 * q + 1] * X[1] +
              P.matVx[ij + 3 * q + 2] * X[2] + P.matVy[ij + 3 * q + 0] * Y[0] +
              P.matVy[ij + 3 * q + 1] * Y[1] + P.matVy[ij + 3 * q + 2] * Y[2];
+    if ((MT == MEDIUM_ISO || MT == MEDIUM_VISCO) && P.nsurf > 0) {
+      float ts[3], vs[3];
+      surface_force(P, i, j, ts, vs);
+#pragma unroll
+      for (int q = 0; q < 3; q++)
+        Z[q] += P.matF[ij + 3 * q + 0] * vs[0] + P.matF[ij + 3 * q + 1] * vs[1] +
+                P.matF[ij + 3 * q + 2] * vs[2];
+    }
   } else if (k == P.nk2 - 1) {
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -291,7 +364,10 @@ __device__ void pml_face(const StageParams& P, const PmlDev& F, int fslot, int i
   }
 }
 
-// moment sources (sv_curv_col_el.c:312-479, moment part)
+// moment and force sources (sv_curv_col_el.c:312-479): a moment subtracts
+// M * coef / jac from the stress rates, a force adds f * (coef * slw / jac)
+// to the velocity rates, except on the surface row when the surface force
+// is strict (the free surface takes it there)
 __device__ __forceinline__ void add_sources(const StageParams& P, int i, int j, int k, long p,
                                             float* r)
 {
@@ -299,12 +375,14 @@ __device__ __forceinline__ void add_sources(const StageParams& P, int i, int j, 
   for (int is = 0; is < P.nsrc; is++) {
     const SrcDev S = P.src[is];
     if (it < S.it_begin || it > S.it_end) continue;
-    float coefv;
+    float coefv, forcev;
     bool hit;
+    const bool force = P.fxyz != nullptr && (!P.src_strict || k < P.nk2);
     if (P.src_spatial == 1) {
       hit = (i == S.si && j == S.sj && k == S.sk);
       if (!hit) continue;
       coefv = (float)(1.0 / (double)P.met[p]);
+      if (force) forcev = P.slw[p] / P.met[p];
     } else {
       const int h = P.src_half;
       const int di = i - S.si, dj = j - S.sj, dk = k - S.sk;
@@ -313,11 +391,18 @@ __device__ __forceinline__ void add_sources(const StageParams& P, int i, int j, 
       const int L = 2 * h + 1;
       const float coef = P.src_coef[S.coef_off + ((dk + h) * L + (dj + h)) * L + (di + h)];
       coefv = coef / P.met[p];
+      if (force) forcev = coef * P.slw[p] / P.met[p];
     }
     const long ip = (long)is * P.src_max_nt * 4 + (long)(it - S.it_begin) * 4 + P.istage;
-    const int slot[6] = {TXX, TYY, TZZ, TXZ, TYZ, TXY};
+    if (force) {
 #pragma unroll
-    for (int m = 0; m < 6; m++) r[slot[m]] -= P.mij[m * P.mij_stride + ip] * coefv;
+      for (int m = 0; m < 3; m++) r[VX + m] += P.fxyz[m * P.mij_stride + ip] * forcev;
+    }
+    if (P.mij != nullptr) {
+      const int slot[6] = {TXX, TYY, TZZ, TXZ, TYZ, TXY};
+#pragma unroll
+      for (int m = 0; m < 6; m++) r[slot[m]] -= P.mij[m * P.mij_stride + ip] * coefv;
+    }
   }
 }
 
@@ -499,7 +584,7 @@ __device__ __forceinline__ void finish_point_loaded(const StageParams& P, int i,
   momentum(r, m, slw, X, Y, Z);
   if ((FLAGS & HAS_SURF) && P.free_top && k >= P.nk2 - (DZ == 0 ? 3 : 1)) {
     // hV: traction image on rows nk2-3..nk2 (forward z op) or nk2-1..nk2
-    timg_hv<DX, DY, DZ>(P, k, p, r);
+    timg_hv<DX, DY, DZ>(P, i, j, k, p, r);
   }
   float Xv[3] = {X[VX], X[VY], X[VZ]}, Yv[3] = {Y[VX], Y[VY], Y[VZ]};
   if ((FLAGS & HAS_SURF) && P.free_top && k >= P.nk2 - 2) {

@@ -7,15 +7,16 @@
  *
  * Covered: isotropic, VTI and general anisotropic elastic media and the GMB
  * visco-elastic medium, CFS-PML faces, traction-image free surface, point
- * and Gaussian moment sources, receivers and receiver lines, slices,
- * snapshots (velocity, stress, strain), the peak-ground-motion maps, the
- * ablexp sponge and Graves-Qs attenuation with the isotropic medium.  Slices
+ * and Gaussian moment and force sources (the strict surface force
+ * included), receivers and receiver lines, slices, snapshots (velocity,
+ * stress, strain), the peak-ground-motion maps, the ablexp sponge and
+ * Graves-Qs attenuation with any elastic medium.  Slices
  * and snapshots are packed on the GPU; only their points are copied into the host wavefield that the reference's own writers
  * (io_slice_nc_put, io_snap_nc_put) then read, so the files are the
  * reference's; the output_all debug dump copies the whole wavefield back each
- * step.  Anything else the par file asks for (acoustic medium, Graves Qs
- * with an anisotropic medium, force sources, dd sources, several MPI ranks)
- * stops the run with a message, the reference's fatal-error convention.
+ * step.  Anything else the par file asks for (acoustic medium, dd sources,
+ * several MPI ranks) stops the run with a message, the reference's
+ * fatal-error convention.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -92,8 +93,6 @@ drv_rk_curv_col_allstep(
   if ((mt != CONST_MEDIUM_ELASTIC_ISO && mt != CONST_MEDIUM_ELASTIC_VTI &&
        mt != CONST_MEDIUM_ELASTIC_ANISO && !visco) ||
       (visco && md->visco_type != CONST_VISCO_GMB) ||
-      (!visco && md->visco_type == CONST_VISCO_GRAVES_QS && mt != CONST_MEDIUM_ELASTIC_ISO) ||
-      src->force_actived ||
       src->dd_is_valid) {
     fprintf(stderr, "ERROR: par file asks for a feature the GPU time loop lacks\n");
     MPI_Abort(MPI_COMM_WORLD, 1);
@@ -152,19 +151,39 @@ drv_rk_curv_col_allstep(
   if (d.free_top) CHK(cgfdh_compute_free_surface(s));   /* drv_rk_curv_col.c:132-159 */
 
   int ns = src->total_number;
-  if (ns > 0 && src->moment_actived) {
+  if (ns > 0 && (src->moment_actived || src->force_actived)) {
     size_t per = (size_t)ns * src->max_nt * src->max_stage;
     int gauss = src->itype_spatial_ext != CONST_SRC_SPATIAL_POINT;
     int half = src->ext_half_npoint, len = 2 * half + 1;
     int *ijk = malloc(3 * ns * sizeof(int));
-    float *mij = malloc(6 * per * sizeof(float));
+    float *mij = src->moment_actived ? malloc(6 * per * sizeof(float)) : NULL;
+    float *fxyz = src->force_actived ? malloc(3 * per * sizeof(float)) : NULL;
     float *w = gauss ? malloc((size_t)ns * len * len * len * sizeof(float)) : NULL;
-    memcpy(mij + 0 * per, src->Mxx, per * sizeof(float));
-    memcpy(mij + 1 * per, src->Myy, per * sizeof(float));
-    memcpy(mij + 2 * per, src->Mzz, per * sizeof(float));
-    memcpy(mij + 3 * per, src->Mxz, per * sizeof(float));
-    memcpy(mij + 4 * per, src->Myz, per * sizeof(float));
-    memcpy(mij + 5 * per, src->Mxy, per * sizeof(float));
+    if (mij) {
+      memcpy(mij + 0 * per, src->Mxx, per * sizeof(float));
+      memcpy(mij + 1 * per, src->Myy, per * sizeof(float));
+      memcpy(mij + 2 * per, src->Mzz, per * sizeof(float));
+      memcpy(mij + 3 * per, src->Mxz, per * sizeof(float));
+      memcpy(mij + 4 * per, src->Myz, per * sizeof(float));
+      memcpy(mij + 5 * per, src->Mxy, per * sizeof(float));
+    }
+    if (fxyz) {
+      memcpy(fxyz + 0 * per, src->Fx, per * sizeof(float));
+      memcpy(fxyz + 1 * per, src->Fy, per * sizeof(float));
+      memcpy(fxyz + 2 * per, src->Fz, per * sizeof(float));
+    }
+    /* strict surface forces: indices and force-rate tables (src_t.c:76-95,
+     * 1020-1028, 1080-1090); the engine forms TxSrc.. / VxSrc.. from them
+     * before every stage as src_set_surface_layer_for_force does */
+    int nsurf = src->force_actived ? src->total_number_surface_force : 0;
+    float *frate = NULL;
+    if (nsurf > 0) {
+      size_t perr = (size_t)nsurf * src->max_nt * src->max_stage;
+      frate = malloc(3 * perr * sizeof(float));
+      memcpy(frate + 0 * perr, src->Fx_rate, perr * sizeof(float));
+      memcpy(frate + 1 * perr, src->Fy_rate, perr * sizeof(float));
+      memcpy(frate + 2 * perr, src->Fz_rate, perr * sizeof(float));
+    }
     for (int is = 0; is < ns; is++) {
       ijk[3 * is] = src->si[is]; ijk[3 * is + 1] = src->sj[is]; ijk[3 * is + 2] = src->sk[is];
       if (gauss) {   /* the weights sv_curv_col_el_rhs_src computes per stage */
@@ -173,9 +192,10 @@ drv_rk_curv_col_allstep(
                                 src->sj_inc[is], src->sk_inc[is], src->ext_func_coef, half, kb));
       }
     }
-    CHK(cgfdh_set_moment_sources(s, ns, ijk, src->it_begin, src->it_end, src->max_nt, mij,
-                                 gauss ? CGFDH_SRC_GAUSSIAN : CGFDH_SRC_POINT, half, w));
-    free(ijk); free(mij); free(w);
+    CHK(cgfdh_set_sources(s, ns, ijk, src->it_begin, src->it_end, src->max_nt, mij, fxyz,
+                          gauss ? CGFDH_SRC_GAUSSIAN : CGFDH_SRC_POINT, half, w,
+                          src->is_surface_force_strict, nsurf, src->force_rate_indx, frate));
+    free(ijk); free(mij); free(fxyz); free(frate); free(w);
   }
 
   /* receivers, then the points of the receiver lines as receivers with zero

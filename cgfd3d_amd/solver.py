@@ -149,6 +149,27 @@ class Solver:
                                              None if ec is None else _p(ec)),
               "set_moment_sources")
 
+    def set_sources(self, ijk, it_begin, it_end, max_nt, mij=None, fxyz=None,
+                    spatial=SRC_POINT, ext_half=0, ext_coefs=None, strict=False,
+                    surf_idx=None, frate=None):
+        """Force and/or moment sources (cgfdh_set_sources): mij the 6
+        moment tables, fxyz the 3 force tables ([nsrc][max_nt][4] each);
+        strict with surf_idx / frate the strict surface forces."""
+        ijk = np.ascontiguousarray(ijk, np.int32).reshape(-1)
+        nsrc = ijk.size // 3
+        itb = np.ascontiguousarray(it_begin, np.int32)
+        ite = np.ascontiguousarray(it_end, np.int32)
+        m = None if mij is None else _f32(mij)
+        f = None if fxyz is None else _f32(fxyz)
+        ec = None if ext_coefs is None else _f32(ext_coefs)
+        si = None if surf_idx is None else np.ascontiguousarray(surf_idx, np.int32)
+        fr = None if frate is None else _f32(frate)
+        nsurf = 0 if si is None else si.size
+        pp = lambda a: None if a is None else _p(a)  # noqa: E731
+        check(lib().cgfdh_set_sources(self.h, nsrc, _p(ijk), _p(itb), _p(ite), max_nt, pp(m),
+                                      pp(f), spatial, ext_half, pp(ec), int(bool(strict)),
+                                      nsurf, pp(si), pp(fr)), "set_sources")
+
     def set_receivers(self, indx1d, dijk):
         idx = np.ascontiguousarray(indx1d, np.int32).reshape(-1)
         w = _f32(dijk).reshape(-1)

@@ -272,6 +272,39 @@ def moment_source(ni, nj, nk, i, j, k, dt, t0, mech6, fc, tpeak, stf_length, tst
                 mij=table.reshape(-1), spatial=SRC_POINT)
 
 
+def iso_hill_explosive(ni=100, nj=100, nk=60, dh=100.0, dt=0.005, nt_total=1001,
+                       hill_height=500.0, hill_sigma=1500.0, bottom=-5900.0, src_depth=2000.0,
+                       m0=1.0e16, fc=2.0, tpeak=0.5, stf_length=2.0, pml=10, n_side=4):
+    """Isotropic elastic homogeneous half-space under one Gaussian hill: an
+    ni x nj x nk curvilinear grid, dx = dy = dh, columns from a flat bottom at
+    z = bottom up to the surface (cartesian origin -5900 m with 100 m spacing
+    in the reference's example, example/cgfd3d.example.sh:235-236), Vp 6000,
+    Vs 3464 m/s, rho 2670 kg/m^3, an explosive point source (Mxx = Myy = Mzz
+    = m0) src_depth below the surface of the hill's centre column at the
+    nearest grid node, a Ricker rate with dominant frequency fc and peak time
+    tpeak, CFS-PML of pml layers on 5 faces (alpha_max 3.14, beta_max 2,
+    ref_vel 7000, as in the example's cfspml blocks), the traction-image free
+    surface and n_side^2 surface receivers."""
+    nx, ny = ni + 6, nj + 6
+    x = (np.arange(nx) - 3.0) * dh
+    y = (np.arange(ny) - 3.0) * dh
+    ci, cj = ni // 2, nj // 2
+    cx, cy = ci * dh, cj * dh
+    X, Y = np.meshgrid(x, y)
+    top = hill_height * np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / (2.0 * hill_sigma ** 2))
+    xyz = layered_coords(ni, nj, nk, dh, top, -bottom)
+    layers = [(1e9, 6000.0, 3464.0, 2670.0)]
+    lam, mu, slw, _ = layered_medium(xyz[2], layers, None)
+    si, sj = 3 + ci, 3 + cj
+    zcol = xyz[2, 3:3 + nk, sj, si].astype(np.float64)
+    sk = 3 + int(np.argmin(np.abs(zcol - (zcol[-1] - src_depth))))
+    m6 = np.array([m0, m0, m0, 0.0, 0.0, 0.0], np.float32)
+    src = moment_source(ni, nj, nk, si, sj, sk, dt, 0.0, m6, fc, tpeak, stf_length)
+    ridx, rw = surface_receiver_grid(ni, nj, nk, n_side)
+    return Workload(f"iso_hill{ni}x{nj}x{nk}", ni, nj, nk, dt, nt_total, xyz, lam, mu, slw,
+                    [pml] * 5 + [0], 7000.0, 3.14, 2.0, True, src, ridx, rw)
+
+
 def iso_topo_grf(ni=600, nj=600, nk=300, dh=50.0, nt_total=2001, dt=0.004, seed=2024,
                  nhills=20, pml=20, n_side=16):
     """Isotropic elastic, ni x nj x nk curvilinear grid over a sum of seeded

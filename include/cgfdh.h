@@ -125,8 +125,9 @@ int cgfdh_set_free_surface(cgfdh_solver *s, const float *matVx2Vz2,
 int cgfdh_set_free_surface_matD(cgfdh_solver *s, const float *matD);
 int cgfdh_get_free_surface_matD(cgfdh_solver *s, float *matD);
 
-/* Graves-Qs attenuation (md->visco_type CONST_VISCO_GRAVES_QS, isotropic
- * elastic medium): md->Qs, one volume, and md->visco_Qs_freq.  After the
+/* Graves-Qs attenuation (md->visco_type CONST_VISCO_GRAVES_QS, any elastic
+ * medium: isotropic, VTI or anisotropic): md->Qs, one volume, and
+ * md->visco_Qs_freq.  After the
  * last stage of every step each physical point, all components, is
  * multiplied by exp(-pi * freq * dt / Qs) (sv_curv_col_el_graves_Qs,
  * sv_curv_col_el.c:638-667, called at drv_rk_curv_col.c:414-416); the
@@ -156,6 +157,33 @@ int cgfdh_set_moment_sources(cgfdh_solver *s, int nsrc, const int *ijk,
                              const int *it_begin, const int *it_end, int max_nt,
                              const float *mij, int spatial, int ext_half,
                              const float *ext_coefs);
+
+/* Force and moment sources together, as src_t holds them (src_t.h;
+ * src_t.c:26-100, 985-1174), replacing cgfdh_set_moment_sources' set:
+ *  ijk, it_begin, it_end, max_nt, spatial, ext_half, ext_coefs as there;
+ *  mij      the 6 moment tables, or NULL (src->moment_actived 0);
+ *  fxyz     Fx, Fy, Fz, each [nsrc][max_nt][4 stages], or NULL
+ *           (src->force_actived 0); a force adds f * coef * slw / jac to
+ *           the velocity rates (sv_curv_col_el.c:356-364, 440-452);
+ *  strict   src->is_surface_force_strict: forces on the surface row k = nk2
+ *           are then left to the free surface (sv_curv_col_el.c:388, 447);
+ *  nsurf    src->total_number_surface_force: the strict forces that reach
+ *           the surface (src_glob_ext_reach_free, src_t.c:361-371: the
+ *           point on k = nk2, or the Gaussian's top row on it), named by
+ *  surf_idx src->force_rate_indx (source numbers), with
+ *  frate    Fx_rate, Fy_rate, Fz_rate, each [nsurf][max_nt][4]
+ *           (src_t.c:1080-1090).  Before every stage the engine forms the
+ *           reference's surface-force slices from them
+ *           (src_set_surface_layer_for_force, src_t.c:154-314): TxSrc..TzSrc
+ *           become the surface traction of the traction image
+ *           (sv_curv_col_el.c:147-163) and VxSrc..VzSrc enter the surface
+ *           velocity derivatives through matF2Vz2 (isotropic media,
+ *           sv_curv_col_el_iso.c:583-592).
+ * nsurf > 0 needs fxyz, strict and the free surface. */
+int cgfdh_set_sources(cgfdh_solver *s, int nsrc, const int *ijk, const int *it_begin,
+                      const int *it_end, int max_nt, const float *mij, const float *fxyz,
+                      int spatial, int ext_half, const float *ext_coefs, int strict,
+                      int nsurf, const int *surf_idx, const float *frate);
 
 /* Receivers as iorecv_one_t keeps them: indx1d[8] and di,dj,dk
  * (io_funcs.c:358-389); samples follow io_recv_keep (io_funcs.c:1585-1623). */

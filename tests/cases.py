@@ -405,7 +405,19 @@ def make_solver(case, dt, nt_total, variant=4):
     if case["free_top"]:
         s.set_free_surface(free_surface_mats(case))
     src = case.get("src")
-    if src is not None and src["nsrc"] > 0:
+    if src is not None and src["nsrc"] > 0 and src.get("fxyz") is not None:
+        # force sources (with or without a moment part): cgfdh_set_sources
+        gauss = bool(src.get("gauss"))
+        surf = src.get("surf_idx")
+        surf = None if surf is None or len(surf) == 0 else surf
+        s.set_sources(src["ijk"], src["it_begin"], src["it_end"], src["max_nt"],
+                      mij=src.get("mij"), fxyz=src["fxyz"],
+                      spatial=SRC_GAUSSIAN if gauss else SRC_POINT,
+                      ext_half=src.get("ext_half", 3) if gauss else 0,
+                      ext_coefs=gauss_coefs(case) if gauss else None,
+                      strict=bool(src.get("strict", 0)), surf_idx=surf,
+                      frate=None if surf is None else src["frate"])
+    elif src is not None and src["nsrc"] > 0:
         if src.get("gauss"):
             coefs = gauss_coefs(case)
             s.set_moment_sources(src["ijk"], src["it_begin"], src["it_end"], src["max_nt"],
@@ -478,7 +490,8 @@ def workload_case(wl):
                                np.float32)
     case = dict(ni=ni, nj=nj, nk=nk, metric=metric, med=med, medium_type=wl.medium_type,
                 free_top=int(wl.free_top), pml_on=list(wl.pml_on), pml_layers=list(wl.pml_layers),
-                pml_A=A, pml_B=B, pml_D=D, pml_region=region, coords=xyz)
+                pml_A=A, pml_B=B, pml_D=D, pml_region=region, coords=xyz,
+                pml_vel=wl.pml_vel, pml_alpha=wl.pml_alpha, pml_beta=wl.pml_beta)
     if wl.wl is not None:
         case["visco_wl"] = np.asarray(wl.wl, np.float32)
     if wl.src is not None:

@@ -15,6 +15,15 @@ from seeds by tests/cases.py) and the reference outputs:
   dt_estimate.npz      blk_dt_esti_curv's dtmax, Vp, L and point for the
                        inputs of tests/cases.py dt_case, every medium
                        (`python tests/golden/make_golden.py dt` writes only it)
+  config1_iso_hill.npz the reference's own full run of the 100x100x60
+                       isotropic hill workload (cgfd3d_amd.workloads.
+                       iso_hill_explosive: explosive source, CFS-PML 10,
+                       free surface, 1001 steps): seismograms of the 16
+                       surface receivers, all 9 components, and the final
+                       surface plane (`python tests/golden/make_golden.py
+                       config1`; about 6 minutes on one core), and
+                       config1_iso_hill_fp64.npz, the same from the
+                       oracle built in FP64 (the error reference)
   extras.npz           checksums of the reference's bdry_ablexp_apply,
                        sv_curv_col_el_graves_Qs and force-source RHS on the
                        inputs of tests/golden/extras_cases.py
@@ -87,11 +96,58 @@ def write_dt_golden():
     np.savez_compressed(os.path.join(HERE, "dt_estimate.npz"), **arrays)
 
 
+def write_config1_golden():
+    """The reference's RHS, PML set-up and free-surface matrices driven over
+    the whole isotropic hill run; the metric is the oracle's (gd_t.c needs
+    netCDF), the medium, source table and receivers are the workload's own."""
+    import time
+    from cgfd3d_amd import workloads as wk
+    ref = po.load_ref()
+    wl = wk.iso_hill_explosive()
+    case = cs.workload_case(wl)
+    ni, nj, nk = wl.ni, wl.nj, wl.nk
+    nx, ny, nz = ni + 6, nj + 6, nk + 6
+    nt = wl.nt_total
+    cfg, keep = cs.ref_cfg(case)
+    co = np.ascontiguousarray(case["coords"])
+    ridx = np.ascontiguousarray(wl.recv_idx, np.int32)
+    rw = np.ascontiguousarray(wl.recv_w, np.float32)
+    wfin = np.zeros(9 * nx * ny * nz, np.float32)
+    seis = np.zeros((len(ridx), 9, nt), np.float32)
+    t = time.time()
+    ref.refh_run(C.byref(cfg), po.ptr(co), po.ptr(case["metric"]), po.ptr(case["med"]),
+                 wl.dt, nt, None, len(ridx), po.ptr(ridx), po.ptr(rw), po.ptr(wfin),
+                 po.ptr(seis), None, None)
+    el = time.time() - t
+    W = wfin.reshape(9, nz, ny, nx)
+    np.savez_compressed(os.path.join(HERE, "config1_iso_hill.npz"), seismo=seis,
+                        surface=W[:, 3 + nk - 1, 3:3 + nj, 3:3 + ni],
+                        wave_sha256=checksum(wfin), input_sha256=input_checksum(case),
+                        src_sha256=checksum(np.asarray(wl.src["mij"], np.float32)),
+                        recv_sha256=checksum(ridx, rw), nt=nt, dt=wl.dt,
+                        ref_seconds=el)
+    print(f"wrote config1_iso_hill ({nt} steps in {el:.0f} s)")
+    # the same scheme in FP64 (the oracle built with double), stored rounded
+    # to float: the FP32 results' error against it is ~1e-6, far above the
+    # storage rounding
+    orc = po.Oracle(case, fp64=True)
+    w64, _, s64 = orc.run(wl.dt, nt, recv_idx=ridx, recv_w=rw, nt_total=nt)
+    W64 = np.asarray(w64).reshape(9, nz, ny, nx)
+    np.savez_compressed(os.path.join(HERE, "config1_iso_hill_fp64.npz"),
+                        seismo=np.asarray(s64, np.float32),
+                        surface=W64[:, 3 + nk - 1, 3:3 + nj, 3:3 + ni].astype(np.float32),
+                        input_sha256=input_checksum(case))
+    print("wrote config1_iso_hill_fp64")
+
+
 def main():
     if not po.ref_available():
         sys.exit("oracle/_ref/libcgfd3d_ref.so not built (needs the reference tree)")
     if sys.argv[1:] == ["dt"]:
         write_dt_golden()
+        return
+    if sys.argv[1:] == ["config1"]:
+        write_config1_golden()
         return
     if sys.argv[1:] == ["extras"]:
         from tests.golden import extras_cases as ec

@@ -107,3 +107,51 @@ def test_batch_autotune_leaves_the_state():
     b.close()
     for s in sv:
         s.close()
+
+
+def test_batch_recaptures_after_reconfiguration():
+    """Setters between two batch runs (new sources and receivers, Graves-Qs
+    switched on, another kernel variant) and eager steps of one member in
+    between (which swap its levels): the batch's next replay follows the new
+    configuration, equal to the members' own cgfdh_run bit for bit."""
+    from cgfd3d_amd import Batch
+    cases = _cases()[:2]
+    rng = np.random.default_rng(5)
+    qs = [rng.uniform(10.0, 100.0, (c["nk"] + 6, c["nj"] + 6, c["ni"] + 6)).astype(np.float32)
+          for c, _, _ in cases]
+
+    def reconfigure(s, n, case):
+        ni, nj, nk = case["ni"], case["nj"], case["nk"]
+        src = cs.ricker_point_source(case, 3 + ni // 3, 3 + nj // 3, 3 + nk // 3, NT, DT,
+                                     fc=8.0, t0=0.12, mech=(0, 0, 0, 1, 1, 0))
+        s.set_moment_sources(src["ijk"], src["it_begin"], src["it_end"], src["max_nt"],
+                             src["mij"])
+        ridx, rw = cs.surface_receivers(case, n=2, seed=11 + n)
+        s.set_receivers(ridx, rw)
+        s.set_graves_qs(qs[n], 2.0)
+        s.set_kernel_variant(12)
+
+    ref = []
+    for n, (case, ridx, rw) in enumerate(cases):
+        s = _solver(case, ridx, rw)
+        s.run(0, 16)
+        reconfigure(s, n, case)
+        s.run(16, 3)
+        s.run(19, 16)
+        ref.append((s.wavefield(), s.pml_aux(), s.seismograms()))
+        s.close()
+    sv = [_solver(case, ridx, rw) for case, ridx, rw in cases]
+    b = Batch(sv)
+    b.run(0, 16)
+    for n, (case, _, _) in enumerate(cases):
+        reconfigure(sv[n], n, case)
+    sv[0].run(16, 3)           # eager steps of one member swap its levels
+    sv[1].run(16, 3)
+    b.run(19, 16)
+    for n, s in enumerate(sv):
+        assert np.array_equal(s.wavefield(), ref[n][0]), n
+        assert np.array_equal(s.pml_aux(), ref[n][1]), n
+        assert np.array_equal(s.seismograms(), ref[n][2]), n
+    b.close()
+    for s in sv:
+        s.close()

@@ -173,9 +173,17 @@ def test_reference_shim_compiles():
     """The drop-in drv_rk_curv_col_allstep compiles against the reference's
     own headers (the structs it unpacks)."""
     import subprocess
+    import tempfile
     shim = os.path.join(ROOT, "cgfd3d_amd", "integration", "drv_rk_curv_col_cgfdh.c")
-    r = subprocess.run(["gcc", "-fsyntax-only", "-std=gnu99", "-Wall", "-Werror",
-                        "-I/opt/conda/include", "-I/root/reference/lib",
-                        "-I/root/reference/forward", "-I" + os.path.join(ROOT, "include"), shim],
+    inc = ["-I/opt/conda/include", "-I/root/reference/lib", "-I/root/reference/forward",
+           "-I" + os.path.join(ROOT, "include")]
+    r = subprocess.run(["gcc", "-fsyntax-only", "-std=gnu99", "-Wall", "-Werror"] + inc + [shim],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+    # and to an object, optimised, with every warning an error
+    with tempfile.TemporaryDirectory() as d:
+        r = subprocess.run(["gcc", "-c", "-O2", "-std=gnu99", "-Wall", "-Wextra",
+                            "-Wno-unused-parameter", "-Werror"] + inc +
+                           [shim, "-o", os.path.join(d, "shim.o")],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
