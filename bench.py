@@ -57,6 +57,8 @@ KERNELS = [
     ("stage_interior_ldsp8", "stage_interior_ldsp_kernel<TY=8> + shell kernels"),
     ("stage_interior_phased_w3", "stage_interior_phased_kernel<BY=4, 3 waves/SIMD> + "
                                  "shell kernels"),
+    ("stage_rows_phased", "stage_rows_phased_kernel<BY=4> (whole rows, PML folded in) + "
+                          "free-surface rows"),
 ]
 
 

@@ -80,8 +80,24 @@ def build_oracle(verbose=False):
     subprocess.run(cmd, check=True, stdout=None if verbose else subprocess.DEVNULL)
 
 
+def build_c_caller(verbose=False):
+    """tests/c/abi_replay: a C program built only from include/cgfdh.h that
+    drives libcgfdh.so in the drop-in shim's call order (a GPU test runs it)."""
+    src = os.path.join(ROOT, "tests", "c", "abi_replay.c")
+    if not os.path.exists(src):
+        return
+    cmd = ["gcc", "-O2", "-std=gnu99", "-Wall", "-Wextra", "-Werror",
+           "-I" + os.path.join(ROOT, "include"), src, "-L" + PKG, "-l:libcgfdh.so",
+           "-Wl,-rpath,$ORIGIN/../../cgfd3d_amd", "-o", os.path.join(ROOT, "tests", "c",
+                                                                    "abi_replay")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
 def build_all(verbose=False):
     build_engine(verbose)
+    build_c_caller(verbose)
     build_oracle(verbose)
 
 

@@ -293,18 +293,18 @@ __device__ void vlow_ht(const StageParams& P, int i, int j, int k, long p, const
 }
 
 // CFS-PML correction of one face (sv_curv_col_el_iso.c:645-1146) and the
-// RK combine of its auxiliary variables
-template <int KIND, int MT>
-__device__ void pml_face(const StageParams& P, const PmlDev& F, int fslot, int i, int j, int k,
-                         long p, const float* Dd, const Medium<MT>& md, float slw,
-                         const float* m, float* r)
+// RK combine of its auxiliary variables; IDIM the face normal (0 x, 1 y,
+// 2 z) as a template argument, so that no register array is indexed by it
+template <int KIND, int MT, int IDIM>
+__device__ void pml_face_d(const StageParams& P, const PmlDev& F, int fslot, int i, int j,
+                           int k, long p, const float* Dd, const Medium<MT>& md, float slw,
+                           const float* m, float* r)
 {
   // m: the point's metric xi_x..zeta_z, already loaded by the caller
-  const int ic = F.idim == 0 ? i - F.i1 : (F.idim == 1 ? j - F.j1 : k - F.k1);
+  const int ic = IDIM == 0 ? i - F.i1 : (IDIM == 1 ? j - F.j1 : k - F.k1);
   const float cD = F.D[ic], cA = F.A[ic], cB = F.B[ic];
   const float cBm1 = (float)((double)cB - 1.0);
-  const int e = 3 * F.idim;
-  const float e1 = m[e], e2 = m[e + 1], e3 = m[e + 2];
+  const float e1 = m[3 * IDIM], e2 = m[3 * IDIM + 1], e3 = m[3 * IDIM + 2];
   // this face's memory variables: the current level and the levels the RK
   // combine reads, requested before the arithmetic
   const long ia = (long)(i - F.i1) + (long)(j - F.j1) * F.fnx + (long)(k - F.k1) * F.fnx * F.fny;
@@ -326,9 +326,9 @@ __device__ void pml_face(const StageParams& P, const PmlDev& F, int fslot, int i
     r[c] += cBm1 * rr[c] - cB * a[c];
     ah[c] = cD * rr[c] - cA * a[c];
   }
-  if (F.idim < 2 && P.free_top && k == P.nk2) {
+  if (IDIM < 2 && P.free_top && k == P.nk2) {
     const long ij = ((long)i + (long)j * P.nx) * 9;
-    const float* mm = F.idim == 0 ? P.matVx : P.matVy;
+    const float* mm = IDIM == 0 ? P.matVx : P.matVy;
     float Dz[3];
 #pragma unroll
     for (int q = 0; q < 3; q++)
@@ -362,6 +362,21 @@ __device__ void pml_face(const StageParams& P, const PmlDev& F, int fslot, int i
 #pragma unroll
     for (int c = 0; c < 9; c++) st_stream(F.aux_end + c * F.fn + ia, aend[c] + P.cb * ah[c]);
   }
+}
+
+// pml_face_d for face F, with the derivatives along its normal (X, Y or Z)
+template <int KIND, int MT>
+__device__ __forceinline__ void pml_face(const StageParams& P, const PmlDev& F, int fslot, int i,
+                                         int j, int k, long p, const float* X, const float* Y,
+                                         const float* Z, const Medium<MT>& md, float slw,
+                                         const float* m, float* r)
+{
+  if (F.idim == 0)
+    pml_face_d<KIND, MT, 0>(P, F, fslot, i, j, k, p, X, md, slw, m, r);
+  else if (F.idim == 1)
+    pml_face_d<KIND, MT, 1>(P, F, fslot, i, j, k, p, Y, md, slw, m, r);
+  else
+    pml_face_d<KIND, MT, 2>(P, F, fslot, i, j, k, p, Z, md, slw, m, r);
 }
 
 // moment and force sources (sv_curv_col_el.c:312-479): a moment subtracts
@@ -596,8 +611,7 @@ __device__ __forceinline__ void finish_point_loaded(const StageParams& P, int i,
   for (int f = 0; (FLAGS & HAS_PML) && f < P.npml; f++) {
     const PmlDev& F = P.pml[f];
     if (i >= F.i1 && i <= F.i2 && j >= F.j1 && j <= F.j2 && k >= F.k1 && k <= F.k2) {
-      const float* Dd = F.idim == 0 ? X : (F.idim == 1 ? Y : Z);
-      pml_face<KIND, MT>(P, F, f, i, j, k, p, Dd, md, slw, m, r);
+      pml_face<KIND, MT>(P, F, f, i, j, k, p, X, Y, Z, md, slw, m, r);
     }
   }
   if constexpr (MT == MEDIUM_VISCO) atten(P, p, md.lam, md.mu, r);

@@ -356,6 +356,104 @@ static void launch_interior_phased(hipStream_t st, const StageParams& P, BoxPara
 #undef CGFDH_IF
 }
 
+// the two-phase per-point kernel over whole rows: every point of the grid
+// except the free-surface rows, PML faces included.  Tiles are 64 points of
+// one row segment starting at the first physical column, so rows are read
+// in full and the x-face PML columns share their cache lines with the
+// neighbouring interior points in the same launch.  A block whose tile meets
+// no PML face runs the interior form; a tile that meets one applies the
+// face corrections (and their memory-variable RK combine) to its PML lanes
+// after the common arithmetic.  The grouped block order spans whole rows.
+template <int DX, int DY, int DZ, int KIND, int MT, int BY>
+__global__ void __launch_bounds__(64 * BY)
+stage_rows_phased_kernel(StageParams P, BoxParams B)
+{
+  static_assert(MT != MEDIUM_VISCO, "elastic media only");
+  int bx, by, k;
+  if (!grouped_block(B, bx, by, k)) return;
+  // does this tile meet a PML face (block-uniform)?
+  const int ti0 = B.i0 + bx * 64, ti1 = min(ti0 + 63, B.i0 + B.ni - 1);
+  const int tj0 = B.j0 + by * BY, tj1 = min(tj0 + BY - 1, B.j0 + B.nj - 1);
+  bool tile_pml = false;
+  for (int f = 0; f < P.npml; f++) {
+    const PmlDev& F = P.pml[f];
+    tile_pml |= ti1 >= F.i1 && ti0 <= F.i2 && tj1 >= F.j1 && tj0 <= F.j2 && k >= F.k1 &&
+                k <= F.k2;
+  }
+  const int i = B.i0 + bx * 64 + threadIdx.x;
+  const int j = B.j0 + by * BY + threadIdx.y;
+  if (i >= B.i0 + B.ni || j >= B.j0 + B.nj) return;
+  const long p = (long)i + (long)j * P.nx + (long)k * P.ss;
+  const long V = P.vol;
+  constexpr int xl = Win<DX>::lo, yl = Win<DY>::lo, zl = Win<DZ>::lo;
+  float ax[9][5], ay[9][5], az[9][5];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    const float* v = P.wcur + c * V;
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+      ax[c][n] = v[p + (xl + n)];
+      ay[c][n] = v[p + (long)(yl + n) * P.nx];
+      az[c][n] = v[p + (long)(zl + n) * P.ss];
+    }
+  }
+  float m[9];
+#pragma unroll
+  for (int n = 0; n < 9; n++) m[n] = ld_stream(P.met + (1 + n) * V + p);
+  float mv[Medium<MT>::ncoef];
+#pragma unroll
+  for (int n = 0; n < Medium<MT>::ncoef; n++) mv[n] = ld_stream(P.med + n * V + p);
+  const float slw = ld_stream(P.slw + p);
+  float cpre[9], cend[9];
+  combine_load<KIND>(P, p, cpre, cend);
+  __builtin_amdgcn_sched_barrier(0);
+  float X[9], Y[9], Z[9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    X[c] = d5v<DX>(ax[c][0], ax[c][1], ax[c][2], ax[c][3], ax[c][4]);
+    Y[c] = d5v<DY>(ay[c][0], ay[c][1], ay[c][2], ay[c][3], ay[c][4]);
+    Z[c] = d5v<DZ>(az[c][0], az[c][1], az[c][2], az[c][3], az[c][4]);
+  }
+  Medium<MT> md;
+  md.from(mv);
+  float r[9];
+  momentum(r, m, slw, X, Y, Z);
+  float Xv[3] = {X[VX], X[VY], X[VZ]}, Yv[3] = {Y[VX], Y[VY], Y[VZ]},
+        Zv[3] = {Z[VX], Z[VY], Z[VZ]};
+  md.hooke(r, m, Xv, Yv, Zv);
+  if (tile_pml) {
+    for (int f = 0; f < P.npml; f++) {
+      const PmlDev& F = P.pml[f];
+      if (i >= F.i1 && i <= F.i2 && j >= F.j1 && j <= F.j2 && k >= F.k1 && k <= F.k2) {
+        pml_face<KIND, MT>(P, F, f, i, j, k, p, X, Y, Z, md, slw, m, r);
+      }
+    }
+  }
+  if (P.nsrc > 0) add_sources(P, i, j, k, p, r);
+  combine_store<KIND>(P, p, r, cpre, cend);
+}
+
+template <int KIND, int MT, int BY>
+static void launch_rows_phased(hipStream_t st, const StageParams& P, BoxParams B, int code)
+{
+  // groups of whole rows: every tile of a row in one group
+  const int tiles_x = (B.ni + 63) / 64;
+  if (B.gx <= 0) B.gx = tiles_x;
+  if (B.gy <= 0) B.gy = 2;
+  const int grid = grouped_grid(B, BY);
+  dim3 blk(64, BY, 1);
+#define CGFDH_RF(c, DX, DY, DZ)                                                            \
+  case c:                                                                                  \
+    hipLaunchKernelGGL((stage_rows_phased_kernel<DX, DY, DZ, KIND, MT, BY>), dim3(grid),   \
+                       blk, 0, st, P, B);                                                  \
+    break;
+  switch (code) {
+    CGFDH_RF(0, 0, 0, 0) CGFDH_RF(1, 0, 0, 1) CGFDH_RF(2, 0, 1, 0) CGFDH_RF(3, 0, 1, 1)
+    CGFDH_RF(4, 1, 0, 0) CGFDH_RF(5, 1, 0, 1) CGFDH_RF(6, 1, 1, 0) CGFDH_RF(7, 1, 1, 1)
+  }
+#undef CGFDH_RF
+}
+
 // the LDS-tiled plane kernel in phases for the elastic media: the tile loads
 // first, then every other operand of the point (z stencil, metric, medium,
 // RK combine inputs); the LDS stores wait only for the tile loads, so the

@@ -17,9 +17,11 @@
 // per-point interior kernel (all loads, then the arithmetic; 13 loads the RK
 // combine inputs after the arithmetic) plus the shell; 14 the LDS-tiled
 // plane kernel (64x8) with the same phased loads plus the shell; 15 variant
-// 12 compiled for 3 waves per SIMD (a few registers spill) plus the shell.
-// The VTI and anisotropic media build variants 0 and 4..15, the
-// visco-elastic medium all but 1..3, 6, 7 and 12..15.
+// 12 compiled for 3 waves per SIMD (a few registers spill) plus the shell;
+// 16 the two-phase kernel over whole rows with the PML faces folded in, the
+// free-surface rows on the general kernel.
+// The VTI and anisotropic media build variants 0 and 4..16, the
+// visco-elastic medium all but 1..3, 6, 7 and 12..16.
 #pragma once
 #include <algorithm>
 #include <cstdio>
@@ -108,6 +110,21 @@ static void launch_kind(const LaunchCtx& L, const StageParams& P, int dx, int dy
     b.ni = i1 - i0 + 1; b.nj = j1 - j0 + 1; b.nk = k1 - k0 + 1;
     return b;
   };
+  if constexpr (MT != MEDIUM_VISCO) {
+    if (L.variant == 16) {
+      // whole rows, PML faces included, on the two-phase kernel; the
+      // free-surface rows on the general kernel
+      const int kmain = L.free_top ? L.nk2 - 4 : L.nk2;
+      launch_rows_phased<KIND, MT, 4>(st, P, box(L.ni1, L.ni2, L.nj1, L.nj2, L.nk1, kmain),
+                                      code);
+      if (L.free_top)
+        launch_box<KIND, MT, HAS_SURF | HAS_PML>(L.shell_stream, P,
+                                                 box(L.ni1, L.ni2, L.nj1, L.nj2, kmain + 1,
+                                                     L.nk2),
+                                                 code);
+      return;
+    }
+  }
   const BoxParams in = box(ilo, ihi, jlo, jhi, klo, khi);
   if (MT != MEDIUM_VISCO && in.ni > 0 && in.nj > 0 && in.nk > 0 &&
       (L.variant == 6 || L.variant == 7)) {

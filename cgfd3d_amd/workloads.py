@@ -168,6 +168,41 @@ def surface_receiver_grid(ni, nj, nk, n_side):
     return np.array(idx, np.int32), np.array(w, np.float32)
 
 
+def keep_two_digi(dt):
+    """blk_keep_two_digi (blk_t.c:3004-3017): the time step cut to its two
+    leading significant digits, as the reference prints it with %9.7e, zeroes
+    the mantissa digits after the first decimal and reads it back."""
+    s = list("%9.7e" % np.float32(dt))
+    for i in range(3, 9):
+        s[i] = "0"
+    return float(np.float32("".join(s)))
+
+
+def stable_dt(xyz, medium_type, vols, nmaxwell=0, cfl=1.3):
+    """The reference's check_stability path (main_curv_col_el_3d.c:617-668):
+    dtmax from blk_dt_esti_curv, computed on the GPU by cgfdh_estimate_dt
+    (cfl = fd->CFL, fd_t.c:55), and the step it sets when none is given,
+    blk_keep_two_digi(dtmax).  vols: the estimate's medium volumes (lambda,
+    mu, rho for the isotropic medium).  Returns (dtmax, dt)."""
+    from .solver import estimate_dt
+    dtmax = estimate_dt(xyz, medium_type, vols, cfl, nmaxwell)[0]
+    if not dtmax > 0.0:
+        raise ValueError(f"maximum dt {dtmax} <= 0")
+    return dtmax, keep_two_digi(dtmax)
+
+
+def _check_or_set_dt(dt, xyz, medium_type, vols, nmaxwell=0):
+    """dt None: the stable step blk_keep_two_digi(dtmax); a given dt above
+    dtmax stops, as the reference does (main_curv_col_el_3d.c:663-667)."""
+    dtmax, auto = stable_dt(xyz, medium_type, np.stack([np.asarray(v, np.float32)
+                                                         for v in vols]), nmaxwell)
+    if dt is None:
+        return auto, dtmax
+    if dtmax < dt:
+        raise ValueError(f"dt={dt} > dtmax={dtmax}: the run would be unstable")
+    return dt, dtmax
+
+
 class Workload:
     """Everything one run of the engine needs, as host arrays."""
 
@@ -187,6 +222,7 @@ class Workload:
         self.free_top = free_top
         self.src = src
         self.recv_idx, self.recv_w = recv_idx, recv_w
+        self.dtmax = None          # the stable-step estimate, when it was made
 
     @property
     def npoints(self):
@@ -274,7 +310,8 @@ def moment_source(ni, nj, nk, i, j, k, dt, t0, mech6, fc, tpeak, stf_length, tst
 
 def iso_hill_explosive(ni=100, nj=100, nk=60, dh=100.0, dt=0.005, nt_total=1001,
                        hill_height=500.0, hill_sigma=1500.0, bottom=-5900.0, src_depth=2000.0,
-                       m0=1.0e16, fc=2.0, tpeak=0.5, stf_length=2.0, pml=10, n_side=4):
+                       m0=1.0e16, fc=2.0, tpeak=0.5, stf_length=2.0, pml=10, n_side=4,
+                       check_dt=False):
     """Isotropic elastic homogeneous half-space under one Gaussian hill: an
     ni x nj x nk curvilinear grid, dx = dy = dh, columns from a flat bottom at
     z = bottom up to the surface (cartesian origin -5900 m with 100 m spacing
@@ -284,7 +321,8 @@ def iso_hill_explosive(ni=100, nj=100, nk=60, dh=100.0, dt=0.005, nt_total=1001,
     nearest grid node, a Ricker rate with dominant frequency fc and peak time
     tpeak, CFS-PML of pml layers on 5 faces (alpha_max 3.14, beta_max 2,
     ref_vel 7000, as in the example's cfspml blocks), the traction-image free
-    surface and n_side^2 surface receivers."""
+    surface and n_side^2 surface receivers.  check_dt: stop if dt exceeds the
+    stable-step estimate (GPU), as the reference does."""
     nx, ny = ni + 6, nj + 6
     x = (np.arange(nx) - 3.0) * dh
     y = (np.arange(ny) - 3.0) * dh
@@ -295,31 +333,47 @@ def iso_hill_explosive(ni=100, nj=100, nk=60, dh=100.0, dt=0.005, nt_total=1001,
     xyz = layered_coords(ni, nj, nk, dh, top, -bottom)
     layers = [(1e9, 6000.0, 3464.0, 2670.0)]
     lam, mu, slw, _ = layered_medium(xyz[2], layers, None)
+    dtmax = None
+    if check_dt:
+        rho = np.full(lam.shape, 2670.0, np.float32)
+        dt, dtmax = _check_or_set_dt(dt, xyz, MEDIUM_ELASTIC_ISO, (lam, mu, rho))
     si, sj = 3 + ci, 3 + cj
     zcol = xyz[2, 3:3 + nk, sj, si].astype(np.float64)
     sk = 3 + int(np.argmin(np.abs(zcol - (zcol[-1] - src_depth))))
     m6 = np.array([m0, m0, m0, 0.0, 0.0, 0.0], np.float32)
     src = moment_source(ni, nj, nk, si, sj, sk, dt, 0.0, m6, fc, tpeak, stf_length)
     ridx, rw = surface_receiver_grid(ni, nj, nk, n_side)
-    return Workload(f"iso_hill{ni}x{nj}x{nk}", ni, nj, nk, dt, nt_total, xyz, lam, mu, slw,
-                    [pml] * 5 + [0], 7000.0, 3.14, 2.0, True, src, ridx, rw)
+    wlo = Workload(f"iso_hill{ni}x{nj}x{nk}", ni, nj, nk, dt, nt_total, xyz, lam, mu, slw,
+                   [pml] * 5 + [0], 7000.0, 3.14, 2.0, True, src, ridx, rw)
+    wlo.dtmax = dtmax
+    return wlo
 
 
-def iso_topo_grf(ni=600, nj=600, nk=300, dh=50.0, nt_total=2001, dt=0.004, seed=2024,
-                 nhills=20, pml=20, n_side=16):
+def iso_topo_grf(ni=600, nj=600, nk=300, dh=50.0, nt_total=2001, dt=None, seed=2024,
+                 nhills=20, pml=20, n_side=16, check_dt=True):
     """Isotropic elastic, ni x nj x nk curvilinear grid over a sum of seeded
     Gaussian hills (amplitude <= 1 km, sigma 1-5 km), 3-layer model times a
     Gaussian random field (5 % std, 500 m correlation), double-couple source
     (seeded strike/dip/rake) with a 2 Hz Ricker, CFS-PML on 5 faces, traction
-    free surface, n_side^2 surface receivers."""
+    free surface, n_side^2 surface receivers.  dt None: the reference's
+    stable step blk_keep_two_digi(dtmax) (estimated on the GPU); a given dt
+    is checked against dtmax unless check_dt is False."""
     top = topography(ni, nj, dh, nhills, seed)
     depth = (nk - 1) * dh
     xyz = layered_coords(ni, nj, nk, dh, top, depth)
     grf = gaussian_random_field((nk + 6, nj + 6, ni + 6), dh, 500.0, 0.05, seed + 1)
     layers = [(1e9, 4500.0, 2600.0, 2400.0), (-1500.0, 6000.0, 3464.0, 2670.0),
               (-6000.0, 6800.0, 3900.0, 2900.0)]
-    lam, mu, slw, vpmax = layered_medium(xyz[2], layers, grf)
+    vp, vs, rho = layered_velocity(xyz[2], layers, grf)
     del grf
+    mu = (rho * vs * vs).astype(np.float32)
+    lam = (rho * vp * vp - 2.0 * mu).astype(np.float32)
+    slw = np.where(rho > 1e-10, 1.0 / rho.astype(np.float64), 0.0).astype(np.float32)
+    del vp, vs
+    dtmax = None
+    if dt is None or check_dt:
+        dt, dtmax = _check_or_set_dt(dt, xyz, MEDIUM_ELASTIC_ISO, (lam, mu, rho))
+    del rho
     rng = np.random.default_rng(seed + 2)
     strike, dip, rake = rng.uniform(0, 360), rng.uniform(10, 90), rng.uniform(-180, 180)
     L = _setup_sigs()
@@ -329,8 +383,10 @@ def iso_topo_grf(ni=600, nj=600, nk=300, dh=50.0, nt_total=2001, dt=0.004, seed=
     sk = 3 + nk - 1 - min(int(round(5000.0 / dh)), int(0.4 * nk))
     src = moment_source(ni, nj, nk, si, sj, sk, dt, 0.0, m6, 2.0, 0.5, 2.0)
     ridx, rw = surface_receiver_grid(ni, nj, nk, n_side)
-    return Workload(f"iso{ni}x{nj}x{nk}", ni, nj, nk, dt, nt_total, xyz, lam, mu, slw,
-                    [pml, pml, pml, pml, pml, 0], 7000.0, 3.14, 2.0, True, src, ridx, rw)
+    wlo = Workload(f"iso{ni}x{nj}x{nk}", ni, nj, nk, dt, nt_total, xyz, lam, mu, slw,
+                   [pml, pml, pml, pml, pml, 0], 7000.0, 3.14, 2.0, True, src, ridx, rw)
+    wlo.dtmax = dtmax
+    return wlo
 
 
 def _layer_index(z3d, layers):
@@ -364,14 +420,15 @@ def _random_rotation(rng):
     return q
 
 
-def media_topo_grf(kind, ni=400, nj=400, nk=200, dh=50.0, nt_total=1501, dt=0.004, seed=2025,
+def media_topo_grf(kind, ni=400, nj=400, nk=200, dh=50.0, nt_total=1501, dt=None, seed=2025,
                    nhills=20, pml=20, n_side=16, nmaxwell=3, qp=100.0, qs=50.0,
-                   fmin=0.5, fmax=5.0, fref=1.0):
+                   fmin=0.5, fmax=5.0, fref=1.0, check_dt=True):
     """The anisotropy / attenuation workloads on the topography + layered GRF
     model of iso_topo_grf: kind 'vti' (Thomsen eps, delta, gamma seeded per
     layer in [0, 0.2]), 'aniso' (the VTI stiffness under one seeded random
     rotation: 21 Cij) or 'visco' (GMB with nmaxwell bodies, constant Qp, Qs
-    over fmin..fmax Hz)."""
+    over fmin..fmax Hz).  dt as for iso_topo_grf, from each medium's own
+    stable-step estimate."""
     top = topography(ni, nj, dh, nhills, seed)
     depth = (nk - 1) * dh
     xyz = layered_coords(ni, nj, nk, dh, top, depth)
@@ -436,7 +493,19 @@ def media_topo_grf(kind, ni=400, nj=400, nk=200, dh=50.0, nt_total=1501, dt=0.00
         mtype = MEDIUM_VISCO_ISO
     else:
         raise ValueError(f"unknown medium kind {kind!r}")
-    vpmax = float(vp.max())
+    dtmax = None
+    if dt is None or check_dt:
+        # the estimate's volumes per medium (cgfdh_estimate_dt): c11, c33, rho
+        # (VTI), c11, rho (anisotropic), lambda, mu, rho, Ylam_n, Ymu_n (GMB)
+        if mtype == MEDIUM_ELASTIC_VTI:
+            ev = (vols[0], vols[2], rho)
+        elif mtype == MEDIUM_ELASTIC_ANISO:
+            ev = (vols[0], rho)
+        else:
+            ev = tuple(vols[:2]) + (rho,) + tuple(vols[3:])
+        dt, dtmax = _check_or_set_dt(dt, xyz, mtype, ev,
+                                     nmaxwell if mtype == MEDIUM_VISCO_ISO else 0)
+        del ev
     del vp, vs, rho
     strike, dip, rake = rng.uniform(0, 360), rng.uniform(10, 90), rng.uniform(-180, 180)
     L = _setup_sigs()
@@ -450,16 +519,18 @@ def media_topo_grf(kind, ni=400, nj=400, nk=200, dh=50.0, nt_total=1501, dt=0.00
                    [pml, pml, pml, pml, pml, 0], 7000.0, 3.14, 2.0, True, src, ridx, rw,
                    medium_type=mtype, med_vols=vols, wl=wl,
                    nmaxwell=nmaxwell if mtype == MEDIUM_VISCO_ISO else 0)
-    _ = vpmax
+    wlo.dtmax = dtmax
     return wlo
 
 
-def batch_workloads(seed=4242, dh=50.0, dt=0.004, nt_total=1001, pml=20, n_side=4,
-                    sizes=((24, (128, 128, 128)), (24, (192, 192, 128)), (16, (256, 256, 160)))):
+def batch_workloads(seed=4242, dh=50.0, dt=None, nt_total=1001, pml=20, n_side=4,
+                    sizes=((24, (128, 128, 128)), (24, (192, 192, 128)), (16, (256, 256, 160))),
+                    check_dt=True):
     """Yields the independent runs of the batched configuration one at a time
     (64 by default: 24 at 128^3, 24 at 192x192x128, 16 at 256x256x160), each
     with its own seeded topography, GRF medium, source position, mechanism
-    and dominant frequency in 1-3 Hz."""
+    and dominant frequency in 1-3 Hz, and its own stable time step (dt None)
+    or the given dt checked against its dtmax."""
     rng = np.random.default_rng(seed)
     n = 0
     for count, (ni, nj, nk) in sizes:
@@ -473,8 +544,16 @@ def batch_workloads(seed=4242, dh=50.0, dt=0.004, nt_total=1001, pml=20, n_side=
             grf = gaussian_random_field((nk + 6, nj + 6, ni + 6), dh, 500.0, 0.05, s + 1)
             layers = [(1e9, 4500.0, 2600.0, 2400.0), (-1500.0, 6000.0, 3464.0, 2670.0),
                       (-4000.0, 6800.0, 3900.0, 2900.0)]
-            lam, mu, slw, _ = layered_medium(xyz[2], layers, grf)
+            vp, vs, rho = layered_velocity(xyz[2], layers, grf)
             del grf
+            mu = (rho * vs * vs).astype(np.float32)
+            lam = (rho * vp * vp - 2.0 * mu).astype(np.float32)
+            slw = np.where(rho > 1e-10, 1.0 / rho.astype(np.float64), 0.0).astype(np.float32)
+            del vp, vs
+            dti, dtmax = dt, None
+            if dt is None or check_dt:
+                dti, dtmax = _check_or_set_dt(dt, xyz, MEDIUM_ELASTIC_ISO, (lam, mu, rho))
+            del rho
             strike, dip, rake = r.uniform(0, 360), r.uniform(10, 90), r.uniform(-180, 180)
             fc = r.uniform(1.0, 3.0)
             L = _setup_sigs()
@@ -485,8 +564,10 @@ def batch_workloads(seed=4242, dh=50.0, dt=0.004, nt_total=1001, pml=20, n_side=
             sj = 3 + int(r.integers(pml + 3, max(pml + 4, nj - pml - 3)))
             sk = 3 + int(r.integers(pml + 3, max(pml + 4, nk - 4)))
             tpeak = 1.2 / fc
-            src = moment_source(ni, nj, nk, si, sj, sk, dt, 0.0, m6, fc, tpeak, 2.0 * tpeak)
+            src = moment_source(ni, nj, nk, si, sj, sk, dti, 0.0, m6, fc, tpeak, 2.0 * tpeak)
             ridx, rw = surface_receiver_grid(ni, nj, nk, n_side)
-            yield Workload(f"batch{n}_{ni}x{nj}x{nk}", ni, nj, nk, dt, nt_total, xyz, lam, mu,
+            wlo = Workload(f"batch{n}_{ni}x{nj}x{nk}", ni, nj, nk, dti, nt_total, xyz, lam, mu,
                            slw, [pml] * 5 + [0], 7000.0, 3.14, 2.0, True, src, ridx, rw)
+            wlo.dtmax = dtmax
+            yield wlo
             n += 1

@@ -69,7 +69,9 @@ def test_oracle_prefix_equals_reference():
 
 @pytest.mark.gpu
 def test_gpu_config1_full_run_equals_reference():
-    wl = _workload()
+    from cgfd3d_amd import workloads as wk
+    wl = wk.iso_hill_explosive(check_dt=True)     # dt 5 ms within the stable step
+    assert wl.dt <= wl.dtmax
     g, g64 = np.load(GOLD), np.load(GOLD64)
     s = wl.build_solver()
     s.run(0, wl.nt_total)
@@ -80,7 +82,7 @@ def test_gpu_config1_full_run_equals_reference():
     e_s, e_p = cs.rel_l2(seis, g["seismo"]), cs.rel_l2(surf, g["surface"])
     e64_s, e64_p = cs.rel_l2(seis, g64["seismo"]), cs.rel_l2(surf, g64["surface"])
     r64_s = cs.rel_l2(g["seismo"], g64["seismo"])
-    print(f"config1 1001 steps: GPU vs reference rel L2 seismograms {e_s:.2e}, surface "
+    print(f"config1 1001 steps (dt {wl.dt}, dtmax {wl.dtmax:.5f}): GPU vs reference rel L2 seismograms {e_s:.2e}, surface "
           f"{e_p:.2e}, bit-identical {bool(np.array_equal(seis, g['seismo']))}; "
           f"GPU vs FP64 seismograms {e64_s:.2e}, surface {e64_p:.2e} "
           f"(reference vs FP64 {r64_s:.2e})", flush=True)

@@ -7,6 +7,10 @@ import numpy as np
 import pytest
 
 from cgfd3d_amd import workloads as w
+from oracle import pyoracle as po
+
+# the stable-step estimate runs on the GPU: CPU checks pass a step and skip it
+FIXED_DT = dict(dt=0.004, check_dt=False)
 
 
 def _bytes(nw, ncoef):
@@ -19,8 +23,8 @@ def _bytes(nw, ncoef):
                                                  ("aniso", 9, 22, 1072),
                                                  ("visco", 27, 9, 2016)])
 def test_bytes_per_point(kind, nw, ncoef, total):
-    wl = (w.iso_topo_grf(ni=24, nj=20, nk=16, pml=4, n_side=2) if kind == "iso"
-          else w.media_topo_grf(kind, ni=24, nj=20, nk=16, pml=4, n_side=2))
+    wl = (w.iso_topo_grf(ni=24, nj=20, nk=16, pml=4, n_side=2, **FIXED_DT) if kind == "iso"
+          else w.media_topo_grf(kind, ni=24, nj=20, nk=16, pml=4, n_side=2, **FIXED_DT))
     assert wl.bytes_per_point() == _bytes(nw, ncoef) == total
     nsurf = 24 * 20
     n_aux = 2 * 5 * 20 * 16 + 2 * 24 * 5 * 16 + 24 * 20 * 5
@@ -30,8 +34,8 @@ def test_bytes_per_point(kind, nw, ncoef, total):
 
 
 def test_vti_from_thomsen_and_rotation_invariants():
-    v = w.media_topo_grf("vti", ni=20, nj=18, nk=16, pml=4, n_side=2)
-    a = w.media_topo_grf("aniso", ni=20, nj=18, nk=16, pml=4, n_side=2)
+    v = w.media_topo_grf("vti", ni=20, nj=18, nk=16, pml=4, n_side=2, **FIXED_DT)
+    a = w.media_topo_grf("aniso", ni=20, nj=18, nk=16, pml=4, n_side=2, **FIXED_DT)
     c11, c13, c33, c55, c66, slw = [np.asarray(x, np.float64) for x in v.med_vols]
     assert (c11 >= c33 * (1 - 1e-6)).all() and (c66 >= c55 * (1 - 1e-6)).all()   # eps, gamma >= 0
     assert np.array_equal(v.med_vols[5], a.med_vols[21])
@@ -56,7 +60,7 @@ def test_vti_from_thomsen_and_rotation_invariants():
 
 
 def test_visco_workload_uses_the_host_gmb_builder():
-    wl = w.media_topo_grf("visco", ni=20, nj=18, nk=16, pml=4, n_side=2)
+    wl = w.media_topo_grf("visco", ni=20, nj=18, nk=16, pml=4, n_side=2, **FIXED_DT)
     assert wl.medium_type == w.MEDIUM_VISCO_ISO and wl.nmaxwell == 3
     assert len(wl.med_vols) == 3 + 6 and len(wl.wl) == 3
     # relaxation frequencies log-spaced over 0.5-5 Hz (2 pi f)
@@ -65,7 +69,7 @@ def test_visco_workload_uses_the_host_gmb_builder():
 
 def test_batch_configuration_sizes():
     sizes = ((2, (24, 24, 20)), (1, (28, 24, 20)))
-    wls = list(w.batch_workloads(sizes=sizes, pml=4, n_side=2))
+    wls = list(w.batch_workloads(sizes=sizes, pml=4, n_side=2, **FIXED_DT))
     assert [x.name.split("_")[1] for x in wls] == ["24x24x20", "24x24x20", "28x24x20"]
     assert len({x.src["ijk"][0] * 1000 + x.src["ijk"][1] for x in wls}) > 1
     total = 24 * 128 ** 3 + 24 * 192 * 192 * 128 + 16 * 256 * 256 * 160
@@ -114,3 +118,25 @@ def test_receiver_rows_line_up_with_traces():
     seis = np.random.default_rng(0).random((3, 4, 3, 128)).astype(np.float32)
     rows = sd.to_receiver_layout(seis)
     assert np.array_equal(rows[2 * 4 + 1], seis[2, 1].T)
+
+
+def test_keep_two_digi_closed_forms():
+    """blk_keep_two_digi keeps the two leading significant digits of the
+    float it is given, truncated: 0.005 as a float prints 4.9999999e-03 and
+    becomes 0.0049."""
+    for x, want in [(0.0048372918, 0.0048), (0.012345, 0.012), (0.0099999, 0.0099),
+                    (0.005, 0.0049), (0.0047, 0.0046), (0.00625, 0.0062), (3.14159, 3.1)]:
+        assert w.keep_two_digi(x) == float(np.float32(want)), x
+
+
+@pytest.mark.skipif(not po.ref_available(), reason="reference build absent")
+def test_keep_two_digi_matches_reference():
+    """Against the reference's own blk_keep_two_digi (blk_t.c:3004-3017,
+    compiled into oracle/_ref) on 2000 log-spaced steps."""
+    import ctypes as C
+    ref = po.load_ref()
+    f = ref.blk_keep_two_digi
+    f.argtypes = [C.c_float]
+    f.restype = C.c_float
+    for x in np.geomspace(1e-5, 0.5, 2000).astype(np.float32):
+        assert w.keep_two_digi(float(x)) == f(float(x)), x
