@@ -59,6 +59,8 @@ KERNELS = [
                                  "shell kernels"),
     ("stage_rows_phased", "stage_rows_phased_kernel<BY=4> (whole rows, PML folded in) + "
                           "free-surface rows"),
+    ("stage_interior_phased_shellp", "stage_interior_phased_kernel<BY=4> + "
+                                     "stage_shellp_kernel (phased PML slabs) + free-surface rows"),
 ]
 
 

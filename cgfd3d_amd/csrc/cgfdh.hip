@@ -713,10 +713,10 @@ extern "C" int cgfdh_get_seismograms(cgfdh_solver* s, float* out)
 extern "C" int cgfdh_set_kernel_variant(cgfdh_solver* s, int v)
 {
   if (s) s->gen++;
-  if (!s || v < 0 || v > 16) return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: %d", v);
+  if (!s || v < 0 || v > 17) return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: %d", v);
   if (s->d.medium_type != CGFDH_MEDIUM_ELASTIC_ISO && v >= 1 && v <= 3)
     return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: variant %d is built for the isotropic "
-                "medium only (VTI / anisotropic: 0, 4..16; visco-elastic: 0, 4, 5, 8..11)", v);
+                "medium only (VTI / anisotropic: 0, 4..17; visco-elastic: 0, 4, 5, 8..11)", v);
   if (s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO && (v == 6 || v == 7 || v >= 12))
     return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: variant %d is not built for the "
                 "visco-elastic medium (0, 4, 5, 8..11)", v);

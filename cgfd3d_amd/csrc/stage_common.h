@@ -292,29 +292,41 @@ __device__ void vlow_ht(const StageParams& P, int i, int j, int k, long p, const
   md.hooke(r, m, X, Y, Z);
 }
 
-// CFS-PML correction of one face (sv_curv_col_el_iso.c:645-1146) and the
-// RK combine of its auxiliary variables; IDIM the face normal (0 x, 1 y,
-// 2 z) as a template argument, so that no register array is indexed by it
-template <int KIND, int MT, int IDIM>
-__device__ void pml_face_d(const StageParams& P, const PmlDev& F, int fslot, int i, int j,
-                           int k, long p, const float* Dd, const Medium<MT>& md, float slw,
-                           const float* m, float* r)
+// offset of point (i, j, k) in face F's memory-variable arrays
+__device__ __forceinline__ long pml_aux_index(const PmlDev& F, int i, int j, int k)
 {
-  // m: the point's metric xi_x..zeta_z, already loaded by the caller
-  const int ic = IDIM == 0 ? i - F.i1 : (IDIM == 1 ? j - F.j1 : k - F.k1);
-  const float cD = F.D[ic], cA = F.A[ic], cB = F.B[ic];
-  const float cBm1 = (float)((double)cB - 1.0);
-  const float e1 = m[3 * IDIM], e2 = m[3 * IDIM + 1], e3 = m[3 * IDIM + 2];
-  // this face's memory variables: the current level and the levels the RK
-  // combine reads, requested before the arithmetic
-  const long ia = (long)(i - F.i1) + (long)(j - F.j1) * F.fnx + (long)(k - F.k1) * F.fnx * F.fny;
-  float a[9], apre[9], aend[9];
+  return (long)(i - F.i1) + (long)(j - F.j1) * F.fnx + (long)(k - F.k1) * F.fnx * F.fny;
+}
+
+// face F's memory variables at the point: the current level and the levels
+// the RK combine of this stage kind reads
+template <int KIND>
+__device__ __forceinline__ void pml_aux_load(const PmlDev& F, long ia, float* a, float* apre,
+                                             float* aend)
+{
 #pragma unroll
   for (int c = 0; c < 9; c++) {
     a[c] = ld_stream(F.aux_cur + c * F.fn + ia);
     if (KIND == KIND_FIRST || KIND == KIND_MID) apre[c] = ld_stream(F.aux_pre + c * F.fn + ia);
     if (KIND == KIND_MID || KIND == KIND_LAST) aend[c] = ld_stream(F.aux_end + c * F.fn + ia);
   }
+}
+
+// CFS-PML correction of one face (sv_curv_col_el_iso.c:645-1146) and the
+// RK combine of its auxiliary variables, from memory variables already
+// loaded; IDIM the face normal (0 x, 1 y, 2 z) as a template argument, so
+// that no register array is indexed by it
+template <int KIND, int MT, int IDIM>
+__device__ void pml_face_apply(const StageParams& P, const PmlDev& F, int fslot, int i, int j,
+                               int k, long ia, const float* Dd, const Medium<MT>& md, float slw,
+                               const float* m, float* r, const float* a, const float* apre,
+                               const float* aend)
+{
+  // m: the point's metric xi_x..zeta_z, already loaded by the caller
+  const int ic = IDIM == 0 ? i - F.i1 : (IDIM == 1 ? j - F.j1 : k - F.k1);
+  const float cD = F.D[ic], cA = F.A[ic], cB = F.B[ic];
+  const float cBm1 = (float)((double)cB - 1.0);
+  const float e1 = m[3 * IDIM], e2 = m[3 * IDIM + 1], e3 = m[3 * IDIM + 2];
   float rr[9];
   rr[VX] = slw * (e1 * Dd[TXX] + e2 * Dd[TXY] + e3 * Dd[TXZ]);
   rr[VY] = slw * (e1 * Dd[TXY] + e2 * Dd[TYY] + e3 * Dd[TYZ]);
@@ -362,6 +374,18 @@ __device__ void pml_face_d(const StageParams& P, const PmlDev& F, int fslot, int
 #pragma unroll
     for (int c = 0; c < 9; c++) st_stream(F.aux_end + c * F.fn + ia, aend[c] + P.cb * ah[c]);
   }
+}
+
+// the same with the memory variables loaded here, ahead of the arithmetic
+template <int KIND, int MT, int IDIM>
+__device__ void pml_face_d(const StageParams& P, const PmlDev& F, int fslot, int i, int j,
+                           int k, long p, const float* Dd, const Medium<MT>& md, float slw,
+                           const float* m, float* r)
+{
+  const long ia = pml_aux_index(F, i, j, k);
+  float a[9], apre[9], aend[9];
+  pml_aux_load<KIND>(F, ia, a, apre, aend);
+  pml_face_apply<KIND, MT, IDIM>(P, F, fslot, i, j, k, ia, Dd, md, slw, m, r, a, apre, aend);
 }
 
 // pml_face_d for face F, with the derivatives along its normal (X, Y or Z)

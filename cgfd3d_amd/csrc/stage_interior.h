@@ -16,6 +16,8 @@
 namespace cgfdh {
 
 struct BoxParams {
+  int fslot;                // phased shell kernel: the PML face slot whose memory
+                            // variables are requested with the other operands (-1 none)
   int i0, j0, k0;           // first point of the box
   int ni, nj, nk;           // extent
   int tiles_x, tiles_y, nchunk, kchunk;
@@ -718,7 +720,7 @@ __global__ void __launch_bounds__(256) stage_box_kernel(StageParams P, BoxParams
 
 // several shell boxes of one FLAGS class in one launch: block ranges
 // start[q] .. start[q+1]-1 (multiples of 8) belong to box q
-constexpr int SHELL_MAX = 5;
+constexpr int SHELL_MAX = 6;
 struct ShellBoxes {
   BoxParams b[SHELL_MAX];
   int start[SHELL_MAX + 1];
@@ -752,6 +754,128 @@ __global__ void __launch_bounds__(256) stage_shell_kernel(StageParams P, ShellBo
     Z[c] = d5<DZ>(v, p, P.ss);
   }
   finish_point<DX, DY, DZ, KIND, MT, FLAGS>(P, i, j, k, p, X, Y, Z);
+}
+
+// one point of the phased shell kernel: for a point inside the box's own
+// PML face (IDIM its normal, -1 none) that face's memory variables are
+// requested first, so their HBM latency overlaps the stencil that follows;
+// then the stencil, metric and medium (loads and arithmetic interleaved by
+// the compiler), the PML corrections in face order (the box's own face from
+// the values already loaded, any other face holding the point with its own
+// loads), sources and the RK combine.  No free-surface rows.
+template <int DX, int DY, int DZ, int KIND, int MT, int IDIM>
+__device__ __forceinline__ void shellp_point(const StageParams& P, int fslot, int i, int j, int k,
+                                             long p)
+{
+  const long V = P.vol;
+  float a[9], apre[9], aend[9];
+  long ia = 0;
+  if constexpr (IDIM >= 0) {
+    const PmlDev& F = P.pml[fslot];
+    if (i >= F.i1 && i <= F.i2 && j >= F.j1 && j <= F.j2 && k >= F.k1 && k <= F.k2) {
+      ia = pml_aux_index(F, i, j, k);
+      pml_aux_load<KIND>(F, ia, a, apre, aend);
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  float X[9], Y[9], Z[9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    const float* v = P.wcur + c * V;
+    X[c] = d5<DX>(v, p, 1);
+    Y[c] = d5<DY>(v, p, P.nx);
+    Z[c] = d5<DZ>(v, p, P.ss);
+  }
+  float m[9];
+#pragma unroll
+  for (int n = 0; n < 9; n++) m[n] = ld_stream(P.met + (1 + n) * V + p);
+  Medium<MT> md;
+  md.load(P.med, V, p);
+  const float slw = ld_stream(P.slw + p);
+  float cpre[9], cend[9];
+  combine_load<KIND>(P, p, cpre, cend);
+  float r[9];
+  momentum(r, m, slw, X, Y, Z);
+  float Xv[3] = {X[VX], X[VY], X[VZ]}, Yv[3] = {Y[VX], Y[VY], Y[VZ]},
+        Zv[3] = {Z[VX], Z[VY], Z[VZ]};
+  md.hooke(r, m, Xv, Yv, Zv);
+  for (int f = 0; f < P.npml; f++) {
+    const PmlDev& F = P.pml[f];
+    if (!(i >= F.i1 && i <= F.i2 && j >= F.j1 && j <= F.j2 && k >= F.k1 && k <= F.k2)) continue;
+    if (IDIM >= 0 && f == fslot) {
+      if constexpr (IDIM >= 0)
+        pml_face_apply<KIND, MT, (IDIM < 0 ? 0 : IDIM)>(
+            P, F, f, i, j, k, ia, IDIM == 0 ? X : (IDIM == 1 ? Y : Z), md, slw, m, r, a, apre,
+            aend);
+    } else {
+      pml_face<KIND, MT>(P, F, f, i, j, k, p, X, Y, Z, md, slw, m, r);
+    }
+  }
+  if (P.nsrc > 0) add_sources(P, i, j, k, p, r);
+  combine_store<KIND>(P, p, r, cpre, cend);
+}
+
+// the PML shell (no free-surface rows) of the elastic media in one launch:
+// boxes of 64 x 4 tiles in the grouped block order, each naming its own PML
+// face; block ranges start[q] .. start[q+1]-1 belong to box q
+template <int DX, int DY, int DZ, int KIND, int MT>
+__global__ void __launch_bounds__(256) stage_shellp_kernel(StageParams P, ShellBoxes S)
+{
+  int q = 0;
+  while (q + 1 < S.n && (int)blockIdx.x >= S.start[q + 1]) ++q;
+  const BoxParams& B = S.b[q];
+  int bx, by, k;
+  if (!grouped_block_at(B, blockIdx.x - S.start[q], S.start[q + 1] - S.start[q], bx, by, k))
+    return;
+  const int i = B.i0 + bx * 64 + (int)(threadIdx.x & 63);
+  const int j = B.j0 + by * 4 + (int)(threadIdx.x >> 6);
+  if (i >= B.i0 + B.ni || j >= B.j0 + B.nj) return;
+  const long p = (long)i + (long)j * P.nx + (long)k * P.ss;
+  const int fs = B.fslot;
+  const int idim = fs >= 0 ? P.pml[fs].idim : -1;
+  if (idim == 0)
+    shellp_point<DX, DY, DZ, KIND, MT, 0>(P, fs, i, j, k, p);
+  else if (idim == 1)
+    shellp_point<DX, DY, DZ, KIND, MT, 1>(P, fs, i, j, k, p);
+  else if (idim == 2)
+    shellp_point<DX, DY, DZ, KIND, MT, 2>(P, fs, i, j, k, p);
+  else
+    shellp_point<DX, DY, DZ, KIND, MT, -1>(P, fs, i, j, k, p);
+}
+
+template <int KIND, int MT>
+static void launch_shellp(hipStream_t st, const StageParams& P, const BoxParams* boxes, int n,
+                          int code)
+{
+  ShellBoxes S;
+  memset(&S, 0, sizeof S);
+  int total = 0;
+  for (int q = 0; q < n && S.n < SHELL_MAX; q++) {
+    BoxParams B = boxes[q];
+    if (B.ni <= 0 || B.nj <= 0 || B.nk <= 0) continue;
+    // groups of whole box rows (at most 16 tiles wide), two tile rows deep
+    const int tiles_x = (B.ni + 63) / 64;
+    if (B.gx <= 0) B.gx = std::min(tiles_x, 16);
+    if (B.gy <= 0) B.gy = 2;
+    const int g = grouped_grid(B, 4, 64);
+    S.b[S.n] = B;
+    S.start[S.n] = total;
+    total += g;
+    S.n++;
+  }
+  if (!S.n) return;
+  S.start[S.n] = total;
+  dim3 blk(256, 1, 1);
+#define CGFDH_SP(c, DX, DY, DZ)                                                                \
+  case c:                                                                                      \
+    hipLaunchKernelGGL((stage_shellp_kernel<DX, DY, DZ, KIND, MT>), dim3(total), blk, 0, st, P, \
+                       S);                                                                     \
+    break;
+  switch (code) {
+    CGFDH_SP(0, 0, 0, 0) CGFDH_SP(1, 0, 0, 1) CGFDH_SP(2, 0, 1, 0) CGFDH_SP(3, 0, 1, 1)
+    CGFDH_SP(4, 1, 0, 0) CGFDH_SP(5, 1, 0, 1) CGFDH_SP(6, 1, 1, 0) CGFDH_SP(7, 1, 1, 1)
+  }
+#undef CGFDH_SP
 }
 
 template <int KIND, int MT, int FLAGS>

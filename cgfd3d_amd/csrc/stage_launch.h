@@ -19,9 +19,12 @@
 // plane kernel (64x8) with the same phased loads plus the shell; 15 variant
 // 12 compiled for 3 waves per SIMD (a few registers spill) plus the shell;
 // 16 the two-phase kernel over whole rows with the PML faces folded in, the
+// free-surface rows on the general kernel; 17 the two-phase interior kernel
+// on the tiles clear of every PML face and the phased shell kernel on the
+// PML slabs (x edges in whole 64-point tiles, y and z slabs in whole rows),
 // free-surface rows on the general kernel.
-// The VTI and anisotropic media build variants 0 and 4..16, the
-// visco-elastic medium all but 1..3, 6, 7 and 12..16.
+// The VTI and anisotropic media build variants 0 and 4..17, the
+// visco-elastic medium all but 1..3, 6, 7 and 12..17.
 #pragma once
 #include <algorithm>
 #include <cstdio>
@@ -120,6 +123,45 @@ static void launch_kind(const LaunchCtx& L, const StageParams& P, int dx, int dy
       if (L.free_top)
         launch_box<KIND, MT, HAS_SURF | HAS_PML>(L.shell_stream, P,
                                                  box(L.ni1, L.ni2, L.nj1, L.nj2, kmain + 1,
+                                                     L.nk2),
+                                                 code);
+      return;
+    }
+  }
+  if constexpr (MT != MEDIUM_VISCO) {
+    if (L.variant == 17) {
+      // one 64-point tile grid along x from the first physical column: the
+      // interior box holds the tiles clear of the x faces' PML, the tiles
+      // that meet them form the x edge boxes; the y and z PML slabs span
+      // whole rows.  Interior on the two-phase kernel; the PML slabs on the
+      // phased shell kernel (each box requests its own face's memory
+      // variables with its other operands); free-surface rows on the
+      // general kernel.
+      int slot[6], ns = 0;
+      for (int f = 0; f < 6; f++) slot[f] = L.pml_on[f] ? ns++ : -1;
+      int xa = L.ni1, xb = L.ni2 + 1;   // interior x range [xa, xb)
+      if (L.pml_on[0]) xa = std::min(L.ni2 + 1, L.ni1 + (L.pml_layers[0] + 1 + 63) / 64 * 64);
+      if (L.pml_on[1]) xb = L.ni1 + (L.ni2 - L.pml_layers[1] - L.ni1) / 64 * 64;
+      if (xb < xa) xb = xa;
+      const BoxParams inner = box(xa, xb - 1, jlo, jhi, klo, khi);
+      if (inner.ni > 0 && inner.nj > 0 && inner.nk > 0)
+        launch_interior_phased<KIND, MT, 4, true>(st, P, inner, code);
+      BoxParams sh[6];
+      int nsh = 0;
+      auto add = [&](BoxParams b, int face) {
+        b.fslot = face >= 0 ? slot[face] : -1;
+        if (b.ni > 0 && b.nj > 0 && b.nk > 0) sh[nsh++] = b;
+      };
+      if (L.pml_on[4]) add(box(L.ni1, L.ni2, L.nj1, L.nj2, L.nk1, klo - 1), 4);
+      if (!L.free_top && L.pml_on[5]) add(box(L.ni1, L.ni2, L.nj1, L.nj2, khi + 1, L.nk2), 5);
+      if (L.pml_on[2]) add(box(L.ni1, L.ni2, L.nj1, jlo - 1, klo, khi), 2);
+      if (L.pml_on[3]) add(box(L.ni1, L.ni2, jhi + 1, L.nj2, klo, khi), 3);
+      if (xa > L.ni1) add(box(L.ni1, xa - 1, jlo, jhi, klo, khi), L.pml_on[0] ? 0 : -1);
+      if (xb <= L.ni2) add(box(xb, L.ni2, jlo, jhi, klo, khi), L.pml_on[1] ? 1 : -1);
+      launch_shellp<KIND, MT>(L.shell_stream, P, sh, nsh, code);
+      if (L.free_top)
+        launch_box<KIND, MT, HAS_SURF | HAS_PML>(L.shell_stream, P,
+                                                 box(L.ni1, L.ni2, L.nj1, L.nj2, khi + 1,
                                                      L.nk2),
                                                  code);
       return;

@@ -23,9 +23,9 @@ CONFIGS = [
 # per-point, lean-interior and vectorised variants of the VTI and
 # anisotropic media, the per-point and lean-interior ones of the
 # visco-elastic medium
-MEDIUM_VARIANTS = ([("iso", v) for v in range(17)] +
+MEDIUM_VARIANTS = ([("iso", v) for v in range(18)] +
                    [(m, v) for m in ("vti", "aniso")
-                    for v in (0, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16)] +
+                    for v in (0, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17)] +
                    [("visco", v) for v in (0, 4, 5, 8, 9, 10, 11)])
 
 
@@ -53,8 +53,8 @@ def test_onestage_all_pairs(name, free, pml, medium, variant):
     assert mism == 0, "expected bit-identical RHS"
 
 
-RUN_VARIANTS = ([("iso", v) for v in range(17)] +
-                [(m, v) for m in ("vti", "aniso") for v in (0, 4, 6, 8, 11, 12, 16)] +
+RUN_VARIANTS = ([("iso", v) for v in range(18)] +
+                [(m, v) for m in ("vti", "aniso") for v in (0, 4, 6, 8, 11, 12, 16, 17)] +
                 [("visco", v) for v in (0, 4, 8, 11)])
 
 
@@ -142,7 +142,7 @@ def test_engine_matches_reference_golden():
     assert np.array_equal(w[:, 3 + nk - 1, 3:3 + nj, 3:3 + ni], g["surface"])
 
 
-@pytest.mark.parametrize("variant", [4, 8, 9, 10, 11, 12, 13, 14, 15, 16])
+@pytest.mark.parametrize("variant", [4, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17])
 def test_midsize_run_matches_oracle(variant):
     """120x110x80 grid, 12 steps of the benchmark's kind of workload (topography,
     random medium, double couple, PML 10 on 5 faces), engine vs oracle; the
@@ -214,3 +214,22 @@ def test_full_size_linearity():
     s.run(0, 3)
     assert not np.any(s.wavefield())
     s.close()
+
+
+@pytest.mark.parametrize("variant", [16, 17])
+def test_wide_run_matches_oracle(variant):
+    """A grid wide enough for every box of the whole-row (16) and phased-shell
+    (17) layouts: x edge tiles, a 64-point interior, y and z PML slabs in
+    whole rows, free-surface rows; 10 steps against the oracle."""
+    from cgfd3d_amd import workloads
+    wl = workloads.iso_topo_grf(ni=200, nj=48, nk=36, nt_total=20, dt=0.004, pml=10, n_side=4)
+    s = wl.build_solver(variant=variant)
+    s.run(0, 10)
+    w_g, a_g, seis_g = s.wavefield(), s.pml_aux(), s.seismograms()
+    s.close()
+    case = cs.workload_case(wl)
+    w_o, a_o, s_o = po.Oracle(case).run(wl.dt, 10, recv_idx=wl.recv_idx, recv_w=wl.recv_w,
+                                        nt_total=wl.nt_total)
+    print(f"wide variant {variant}: rel L2 wavefield {cs.rel_l2(w_g, w_o):.1e}", flush=True)
+    assert np.abs(w_o).max() > 0
+    assert np.array_equal(w_g, w_o) and np.array_equal(a_g, a_o) and np.array_equal(seis_g, s_o)
