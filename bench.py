@@ -83,9 +83,12 @@ def measured_traffic(kernel, workload_name):
 
 
 def cpu_baseline(wl, budget_s=15.0):
-    """The oracle on a sub-block cut from the same workload (top of the grid,
-    free surface, PML on 5 faces, the source inside, the workload's medium),
-    timed on the host cores."""
+    """The CPU path on a sub-block cut from the same workload (top of the
+    grid, free surface, PML on 5 faces, the source inside, the workload's
+    medium), timed on the host cores: the reference's own RHS compiled from
+    its sources (oracle/_ref, one process per core) where that build is
+    present, else the oracle (OpenMP over the cores) with the measured
+    speed ratio of the two attached."""
     import numpy as np
     from oracle import pyoracle as po
     from tests import cases as cs
@@ -116,6 +119,12 @@ def cpu_baseline(wl, budget_s=15.0):
     src = dict(wl.src)
     src.update(nsrc=1, ijk=[3 + ni // 2, 3 + nj // 2, 3 + nk - 1 - 20], inc=[0, 0, 0], gauss=0)
     case["src"] = src
+    case.update(pml_vel=wl.pml_vel, pml_alpha=wl.pml_alpha, pml_beta=wl.pml_beta)
+    if po.ref_available() and wl.wl is None:
+        try:
+            return reference_baseline(case, wl, budget_s)
+        except Exception as e:      # the port below still gives a baseline
+            log(f"reference baseline unavailable ({e}); timing the oracle instead")
     o = po.Oracle(case)
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     t = time.perf_counter()
@@ -126,12 +135,45 @@ def cpu_baseline(wl, budget_s=15.0):
     o.run(wl.dt, n, w=w, aux=aux, it0=1)
     tn = time.perf_counter() - t
     value = ni * nj * nk * n / tn
-    return {"value": value, "unit": "grid-point RK4 updates/s", "cores": threads,
+    out = {"value": value, "unit": "grid-point RK4 updates/s", "cores": threads,
             "kind": "port",
             "sample": f"oracle (C restatement, OpenMP {threads} threads) on a {ni}x{nj}x{nk} "
                       f"{wl.name.rstrip('0123456789x')} "
                       f"sub-block of the same workload with free surface, 10-layer PML on 5 "
                       f"faces and the source, {n} RK4 steps in {tn:.1f} s"}
+    cal = os.path.join(ROOT, "oracle", "calibration.json")
+    if os.path.exists(cal):
+        with open(cal) as f:
+            c = json.load(f)
+        out["calibration"] = {"restatement_over_reference": c["restatement_over_reference"],
+                              "measured": c["case"] + ", one thread each, on " + c["cpu"]}
+    return out
+
+
+def reference_baseline(case, wl, budget_s):
+    """The reference's own RHS (oracle/_ref, compiled from its sources) as
+    one single-threaded process per host core, each advancing its own copy
+    of the sub-block from rest with the source -- the reference's MPI layout
+    of one rank per core without its halo exchange."""
+    import tempfile
+    from oracle import ref_baseline as rb
+    ni, nj, nk = case["ni"], case["nj"], case["nk"]
+    nproc = max(1, min(16, os.cpu_count() or 1))
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "case.npz")
+        rb.save_case(path, case, wl.dt)
+        t1 = rb.run_parallel(path, 1, 1)[0]
+        n = max(1, min(50, int(budget_s / max(t1, 1e-3))))
+        times = rb.run_parallel(path, nproc, n)
+    tmax = max(times)
+    return {"value": nproc * ni * nj * nk * n / tmax, "unit": "grid-point RK4 updates/s",
+            "cores": nproc, "kind": "reference",
+            "sample": f"the reference's own RHS (forward/sv_curv_col_el*.c, bdry_t.c, fd_t.c, "
+                      f"compiled -O3) in {nproc} single-threaded processes at once, one per "
+                      f"core, each running its own copy of a {ni}x{nj}x{nk} "
+                      f"{wl.name.rstrip('0123456789x')} sub-block of the same workload (free "
+                      f"surface, 10-layer CFS-PML on 5 faces, the source) for {n} RK4 steps "
+                      f"from rest; slowest process {tmax:.1f} s"}
 
 
 MEDIUM_NAMES = {2: "iso elastic", 3: "VTI elastic (Thomsen eps/delta/gamma in [0, 0.2])",

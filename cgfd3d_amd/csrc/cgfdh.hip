@@ -324,9 +324,12 @@ extern "C" int cgfdh_create(const cgfdh_desc* d, cgfdh_solver** out)
     bad |= alloc(&F.A, np) | alloc(&F.B, np) | alloc(&F.D, np);
     for (auto& a : F.aux) bad |= alloc(&a, 9 * F.fn);
   }
-    // default interior kernel: the two-phase per-point kernel for the elastic
-  // media, the per-point kernel for the visco-elastic medium
-  s->variant = d->medium_type == CGFDH_MEDIUM_VISCO_ISO ? 4 : 12;
+  // default stage kernels: the whole-row two-phase kernel with the PML faces
+  // folded in for the isotropic medium, the two-phase interior kernel for
+  // VTI and anisotropic media, the per-point kernel for the visco-elastic one
+  s->variant = d->medium_type == CGFDH_MEDIUM_VISCO_ISO   ? 4
+               : d->medium_type == CGFDH_MEDIUM_ELASTIC_ISO ? 16
+                                                          : 12;
 if (!bad && hipStreamCreate(&s->stream) != hipSuccess) bad = 1;
   if (!bad && (hipStreamCreate(&s->stream2) != hipSuccess ||
                hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||

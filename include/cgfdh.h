@@ -271,9 +271,13 @@ int cgfdh_last_timing(cgfdh_solver *s, double *run_ms, double *stage_ms,
  * (all loads, then the arithmetic; 13 loads the RK combine inputs last),
  * 14 = LDS-tiled plane kernel with the same phased loads, 15 = variant 12
  * compiled for 3 waves per SIMD; 2..15 run the per-point kernel on the PML
- * and free-surface shell.  Default 12 for the
- * elastic media, 4 for the visco-elastic medium.  Variants 1..3 exist for the
- * isotropic medium only, 6, 7 and 12..14 not for the visco-elastic medium.
+ * and free-surface shell; 16 = the two-phase kernel over whole rows with the
+ * PML faces folded in (free-surface rows on the per-point kernel); 17 = the
+ * two-phase interior kernel on the tiles clear of every PML face plus a
+ * shell kernel that requests each PML slab's memory variables first.
+ * Default 16 for the isotropic medium, 12 for the VTI and anisotropic
+ * media, 4 for the visco-elastic medium.  Variants 1..3 exist for the
+ * isotropic medium only, 6, 7 and 12..17 not for the visco-elastic medium.
  * All variants produce identical bits. */
 int cgfdh_set_kernel_variant(cgfdh_solver *s, int variant);
 

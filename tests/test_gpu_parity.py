@@ -233,3 +233,39 @@ def test_wide_run_matches_oracle(variant):
     print(f"wide variant {variant}: rel L2 wavefield {cs.rel_l2(w_g, w_o):.1e}", flush=True)
     assert np.abs(w_o).max() > 0
     assert np.array_equal(w_g, w_o) and np.array_equal(a_g, a_o) and np.array_equal(seis_g, s_o)
+
+
+def test_full_size_values_match_oracle():
+    """600x600x300 benchmark workload at its full size: from a random state
+    on every physical point and every PML memory variable (so every point,
+    face and byte offset of the 4 GB levels carries data), two RK4 steps on
+    the GPU equal the oracle's two steps on the host bit for bit, wavefield
+    and PML memory variables."""
+    from cgfd3d_amd import workloads
+    wl = workloads.iso_topo_grf(nt_total=10)
+    print("full-size workload built, dt", wl.dt, flush=True)
+    ni, nj, nk = wl.ni, wl.nj, wl.nk
+    nx, ny, nz = ni + 6, nj + 6, nk + 6
+    rng = np.random.default_rng(42)
+    w0 = np.zeros((9, nz, ny, nx), np.float32)
+    for c in range(9):
+        w0[c, 3:3 + nk, 3:3 + nj, 3:3 + ni] = rng.standard_normal((nk, nj, ni), np.float32)
+    s = wl.build_solver()
+    try:
+        aux0 = rng.standard_normal(s.aux_len(), np.float32)
+        s.set_wavefield(w0)
+        s.set_pml_aux(aux0)
+        s.run(0, 2)
+        w_g, a_g = s.wavefield(), s.pml_aux()
+    finally:
+        s.close()
+    print("GPU: 2 steps done", flush=True)
+    case = cs.workload_case(wl)
+    w_o, a_o, _ = po.Oracle(case).run(wl.dt, 2, w=w0.reshape(-1), aux=aux0,
+                                      nt_total=wl.nt_total)
+    print("oracle: 2 steps done", flush=True)
+    e_w, e_a = cs.rel_l2(w_g, w_o), cs.rel_l2(a_g, a_o)
+    print(f"full size: rel L2 wavefield {e_w:.2e}, PML memory {e_a:.2e}, bit-identical "
+          f"{bool(np.array_equal(w_g, w_o))} / {bool(np.array_equal(a_g, a_o))}", flush=True)
+    assert e_w <= 1e-5 and e_a <= 1e-5
+    assert np.array_equal(w_g, w_o) and np.array_equal(a_g, a_o)

@@ -1,7 +1,8 @@
 """GPU: the benchmark workloads, built through their own solver set-up path
 (device metric, medium setters, PML tables, device free-surface matrices),
 advance bit-identically to the oracle on small grids of every medium, and a
-small batch of the batched configuration equals its members' own runs."""
+small batch of the batched configuration equals its members' own runs and,
+for one member, the oracle."""
 import numpy as np
 import pytest
 
@@ -48,6 +49,13 @@ def test_small_batch_configuration_equals_member_runs():
     for n, s in enumerate(sv):
         assert np.array_equal(s.wavefield(), ref[n][0]), n
         assert np.array_equal(s.seismograms(), ref[n][1]), n
+    # and one batched member against the oracle directly
+    case = cs.workload_case(wls[2])
+    w_o, _, s_o = po.Oracle(case).run(wls[2].dt, 20, recv_idx=wls[2].recv_idx,
+                                      recv_w=wls[2].recv_w, nt_total=wls[2].nt_total)
+    assert np.abs(w_o).max() > 0
+    assert np.array_equal(sv[2].wavefield(), w_o)
+    assert np.array_equal(sv[2].seismograms(), s_o)
     b.close()
     for s in sv:
         s.close()
