@@ -44,12 +44,15 @@ def build_engine(verbose=False):
     inc = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
     objdir = os.path.join(CSRC, "_obj")
     os.makedirs(objdir, exist_ok=True)
-    obj = os.path.join(objdir, "host_setup.o")
-    # host tables: plain C, no FMA contraction (the reference's x86-64 rounding)
-    ccmd = ["gcc", "-O2", "-fPIC", "-std=gnu99", "-ffp-contract=off", "-c",
-            os.path.join(CSRC, "host_setup.c"), "-o", obj] + inc
-    cmds = [ccmd]
-    objs = [obj]
+    cmds, objs = [], []
+    # host tables and output files: plain C, no FMA contraction (the
+    # reference's x86-64 rounding)
+    for src in ("host_setup.c", "host_io.c"):
+        obj = os.path.join(objdir, src.replace(".c", ".o"))
+        cmds.append(["gcc", "-O2", "-fPIC", "-std=gnu99", "-ffp-contract=off", "-Wall",
+                     "-Wno-unknown-pragmas", "-Werror", "-c", os.path.join(CSRC, src), "-o",
+                     obj] + inc)
+        objs.append(obj)
     for u in UNITS:
         o = os.path.join(objdir, u.replace(".hip", ".o"))
         cmds.append([hipcc()] + HIPCC_FLAGS + inc + ["-c", os.path.join(CSRC, u), "-o", o])

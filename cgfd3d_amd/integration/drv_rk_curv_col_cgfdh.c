@@ -25,6 +25,7 @@
 
 #include "drv_rk_curv_col.h"
 #include "cgfdh.h"
+#include "cgfdh_io.h"
 
 static void cgfdh_fatal(const char *what, int rc)
 {
@@ -34,25 +35,13 @@ static void cgfdh_fatal(const char *what, int rc)
 
 #define CHK(call) do { int rc_ = (call); if (rc_) cgfdh_fatal(#call, rc_); } while (0)
 
-/* copy the packed box {i1,ni,di, j1,nj,dj, k1,nk,dk} of components comps
- * from the engine into the host wavefield w4d at the same points */
-static void fill_box(cgfdh_solver *s, const gd_t *gd, size_t siz_icmp, float *w4d,
-                     const int box[9], int ncomp, const int *comps, float *pack)
+static void cgfdh_io_fatal(const char *what)
 {
-  CHK(cgfdh_get_subvolume(s, box, ncomp, comps, pack));
-  size_t t = 0;
-  for (int c = 0; c < ncomp; c++) {
-    float *dst = w4d + (size_t)comps[c] * siz_icmp;
-    for (int n3 = 0; n3 < box[7]; n3++)
-      for (int n2 = 0; n2 < box[4]; n2++)
-        for (int n1 = 0; n1 < box[1]; n1++, t++) {
-          size_t i = box[0] + (size_t)n1 * box[2];
-          size_t j = box[3] + (size_t)n2 * box[5];
-          size_t k = box[6] + (size_t)n3 * box[8];
-          dst[i + j * gd->siz_iy + k * gd->siz_iz] = pack[t];
-        }
-  }
+  fprintf(stderr, "ERROR: %s failed: %s\n", what, cgfdh_io_last_error());
+  MPI_Abort(MPI_COMM_WORLD, 1);
 }
+
+#define CHKIO(call) do { if ((call) != 0) cgfdh_io_fatal(#call); } while (0)
 
 /* the condition under which io_snap_nc_put writes snapshot n at step it
  * (io_funcs.c:1148-1154) */
@@ -239,80 +228,112 @@ drv_rk_curv_col_allstep(
     CHK(cgfdh_set_ablexp(s, bdry->ablexp_Ex, bdry->ablexp_Ey, bdry->ablexp_Ez, blk));
   }
 
-  /* slices every step, snapshots at their own steps (drv_rk_curv_col.c:503-509):
-   * the engine runs up to each output step, then the packed points go into
-   * level 0 of the reference's host wavefield (wav->v5d), level 1 serving as
-   * the writers' buffer as w_rhs does in the reference */
+  /* slices every step, snapshots at their own steps, the debug dump of
+   * output_all (drv_rk_curv_col.c:503-525): the engine runs up to each output
+   * step, packs the output points on the GPU, and the library's own netCDF
+   * writer puts them into files laid out as io_slice_nc_create /
+   * io_snap_nc_create / io_var3d_export_nc lay them out (cgfdh_io.h) */
   int *topoid = mympi->topoid;
-  ioslice_nc_t ioslice_nc;
-  io_slice_nc_create(ioslice, wav->ncmp, wav->visco_type, wav->cmp_name, gd->ni, gd->nj,
-                     gd->nk, topoid, &ioslice_nc);
-  iosnap_nc_t iosnap_nc;
-  io_snap_nc_create(iosnap, &iosnap_nc, topoid);
+  int nvs = visco ? 9 : wav->ncmp;   /* io_slice_nc_create: GMB slices hold 9 */
+  const char *const *vnames = (const char *const *)wav->cmp_name;
   int nslice[3] = {ioslice->num_of_slice_x, ioslice->num_of_slice_y, ioslice->num_of_slice_z};
-  int has_slice = nslice[0] + nslice[1] + nslice[2] > 0;
-  size_t pack_n = 9 * (size_t)gd->nx * gd->ny;
-  if (9 * (size_t)gd->nx * gd->nz > pack_n) pack_n = 9 * (size_t)gd->nx * gd->nz;
-  if (9 * (size_t)gd->ny * gd->nz > pack_n) pack_n = 9 * (size_t)gd->ny * gd->nz;
-  for (int n = 0; n < iosnap->num_of_snap; n++) {
-    size_t m = 6 * (size_t)iosnap->ni[n] * iosnap->nj[n] * iosnap->nk[n];
+  int nsl = nslice[0] + nslice[1] + nslice[2];
+  cgfdh_nc **slf = nsl ? calloc(nsl, sizeof(cgfdh_nc *)) : NULL;
+  int q = 0;
+  for (int n = 0; n < nslice[0]; n++, q++)
+    CHKIO(cgfdh_io_slice_create(ioslice->slice_x_fname[n], 'x', gd->nk, gd->nj,
+                                ioslice->slice_x_indx[n], topoid, nvs, vnames, &slf[q]));
+  for (int n = 0; n < nslice[1]; n++, q++)
+    CHKIO(cgfdh_io_slice_create(ioslice->slice_y_fname[n], 'y', gd->nk, gd->ni,
+                                ioslice->slice_y_indx[n], topoid, nvs, vnames, &slf[q]));
+  for (int n = 0; n < nslice[2]; n++, q++)
+    CHKIO(cgfdh_io_slice_create(ioslice->slice_z_fname[n], 'z', gd->nj, gd->ni,
+                                ioslice->slice_z_indx[n], topoid, nvs, vnames, &slf[q]));
+  int nsn = iosnap->num_of_snap;
+  cgfdh_nc **snf = nsn ? calloc(nsn, sizeof(cgfdh_nc *)) : NULL;
+  int *snrec = nsn ? calloc(nsn, sizeof(int)) : NULL;
+  for (int n = 0; n < nsn; n++) {
+    if (iosnap->out_strain[n] && mt != CONST_MEDIUM_ELASTIC_ISO && !visco) {
+      fprintf(stderr, "ERROR: strain snapshots need an isotropic medium\n");
+      MPI_Abort(MPI_COMM_WORLD, 1);
+    }
+    const int g_start[3] = {iosnap->i1_to_glob[n], iosnap->j1_to_glob[n], iosnap->k1_to_glob[n]};
+    const int l_start[3] = {iosnap->i1[n], iosnap->j1[n], iosnap->k1[n]};
+    const int stride[3] = {iosnap->di[n], iosnap->dj[n], iosnap->dk[n]};
+    CHKIO(cgfdh_io_snap_create(iosnap->fname[n], iosnap->nk[n], iosnap->nj[n], iosnap->ni[n],
+                               iosnap->out_vel[n], iosnap->out_stress[n], iosnap->out_strain[n],
+                               g_start, l_start, stride, topoid, &snf[n]));
+  }
+  size_t pack_n = (size_t)nvs * gd->nj * gd->nk;
+  if ((size_t)nvs * gd->ni * gd->nk > pack_n) pack_n = (size_t)nvs * gd->ni * gd->nk;
+  if ((size_t)nvs * gd->ni * gd->nj > pack_n) pack_n = (size_t)nvs * gd->ni * gd->nj;
+  for (int n = 0; n < nsn; n++) {
+    size_t m = 15 * (size_t)iosnap->ni[n] * iosnap->nj[n] * iosnap->nk[n];
     if (m > pack_n) pack_n = m;
   }
-  float *pack = (has_slice || iosnap->num_of_snap > 0) ? malloc(pack_n * sizeof(float)) : NULL;
+  float *pack = (nsl || nsn) ? malloc(pack_n * sizeof(float)) : NULL;
   float *w4d = wav->v5d;
-  float *wbuf = wav->v5d + wav->siz_ilevel;
-  const int cmp9[9] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
+  int cmps[CGFDH_MAXWELL_MAX * 6 + 9];
+  for (int c = 0; c < nvs; c++) cmps[c] = c;
+  /* stresses in the snapshot file's order Txx Tyy Tzz Txz Tyz Txy (wavefield
+   * slots 3 4 5 7 6 8) */
+  const int stress_cmps[6] = {3, 4, 5, 7, 6, 8};
   int it_run = 0;
-  char ou_file[CONST_MAX_STRLEN];
   for (int it = 0; it < nt_total; it++) {
-    int due = has_slice || output_all;
-    for (int n = 0; n < iosnap->num_of_snap && !due; n++) due = snap_due(iosnap, n, it, nt_total);
+    int due = nsl > 0 || output_all;
+    for (int n = 0; n < nsn && !due; n++) due = snap_due(iosnap, n, it, nt_total);
     if (!due) continue;
     CHK(cgfdh_run(s, it_run, it + 1 - it_run));
     it_run = it + 1;
     float t_cur = it * dt + t0;
     float t_end = t_cur + dt;
-    /* slices: components 0..8 (all of them elastic, the first 9 for GMB) */
-    for (int n = 0; n < nslice[0]; n++) {
+    q = 0;
+    for (int n = 0; n < nslice[0]; n++, q++) {
       int box[9] = {ioslice->slice_x_indx[n], 1, 1, gd->nj1, gd->nj, 1, gd->nk1, gd->nk, 1};
-      fill_box(s, gd, wav->siz_icmp, w4d, box, 9, cmp9, pack);
+      CHK(cgfdh_get_subvolume(s, box, nvs, cmps, pack));
+      CHKIO(cgfdh_io_slice_put(slf[q], it, t_end, nvs, pack));
     }
-    for (int n = 0; n < nslice[1]; n++) {
+    for (int n = 0; n < nslice[1]; n++, q++) {
       int box[9] = {gd->ni1, gd->ni, 1, ioslice->slice_y_indx[n], 1, 1, gd->nk1, gd->nk, 1};
-      fill_box(s, gd, wav->siz_icmp, w4d, box, 9, cmp9, pack);
+      CHK(cgfdh_get_subvolume(s, box, nvs, cmps, pack));
+      CHKIO(cgfdh_io_slice_put(slf[q], it, t_end, nvs, pack));
     }
-    for (int n = 0; n < nslice[2]; n++) {
+    for (int n = 0; n < nslice[2]; n++, q++) {
       int box[9] = {gd->ni1, gd->ni, 1, gd->nj1, gd->nj, 1, ioslice->slice_z_indx[n], 1, 1};
-      fill_box(s, gd, wav->siz_icmp, w4d, box, 9, cmp9, pack);
+      CHK(cgfdh_get_subvolume(s, box, nvs, cmps, pack));
+      CHKIO(cgfdh_io_slice_put(slf[q], it, t_end, nvs, pack));
     }
-    /* snapshots: velocities, and the stresses the stress and strain outputs read */
-    for (int n = 0; n < iosnap->num_of_snap; n++) {
+    for (int n = 0; n < nsn; n++) {
       if (!snap_due(iosnap, n, it, nt_total)) continue;
       int box[9] = {iosnap->i1[n], iosnap->ni[n], iosnap->di[n], iosnap->j1[n], iosnap->nj[n],
                     iosnap->dj[n], iosnap->k1[n], iosnap->nk[n], iosnap->dk[n]};
-      if (iosnap->out_vel[n]) fill_box(s, gd, wav->siz_icmp, w4d, box, 3, cmp9, pack);
-      if (iosnap->out_stress[n] || iosnap->out_strain[n])
-        fill_box(s, gd, wav->siz_icmp, w4d, box, 6, cmp9 + 3, pack);
+      size_t npt = (size_t)box[1] * box[4] * box[7];
+      float *v = iosnap->out_vel[n] ? pack : NULL;
+      float *t = iosnap->out_stress[n] ? pack + 3 * npt : NULL;
+      float *e = iosnap->out_strain[n] ? pack + 9 * npt : NULL;
+      if (v) CHK(cgfdh_get_subvolume(s, box, 3, cmps, v));
+      if (t) CHK(cgfdh_get_subvolume(s, box, 6, stress_cmps, t));
+      if (e) CHK(cgfdh_get_strain_subvolume(s, box, e));
+      CHKIO(cgfdh_io_snap_put(snf[n], snrec[n]++, t_end, v, t, e));
     }
-    io_slice_nc_put(ioslice, &ioslice_nc, gd, w4d, wbuf, it, t_end, 0, wav->ncmp - 1,
-                    wav->visco_type);
-    io_snap_nc_put(iosnap, &iosnap_nc, gd, md, wav, w4d, wbuf, nt_total, it, t_end, 1, 1, 1);
     if (output_all) {   /* debug dump of the whole w_end (drv_rk_curv_col.c:513-525) */
       CHK(cgfdh_get_wavefield(s, w4d));
-      io_build_fname_time(output_dir, "w3d", ".nc", topoid, it, ou_file);
-      io_var3d_export_nc(ou_file, w4d, wav->cmp_pos, wav->cmp_name, wav->ncmp, gd->index_name,
-                         gd->nx, gd->ny, gd->nz);
+      CHKIO(cgfdh_io_var3d_export(output_dir, "w3d", topoid, it, w4d, wav->cmp_pos,
+                                  vnames, wav->ncmp, (const char *const *)gd->index_name,
+                                  gd->nx, gd->ny, gd->nz));
     }
   }
   if (it_run < nt_total) CHK(cgfdh_run(s, it_run, nt_total - it_run));
   free(pack);
-  io_slice_nc_close(&ioslice_nc);
-  io_snap_nc_close(&iosnap_nc);
+  for (q = 0; q < nsl; q++) CHKIO(cgfdh_nc_close(slf[q]));
+  for (int n = 0; n < nsn; n++) CHKIO(cgfdh_nc_close(snf[n]));
+  free(slf); free(snf); free(snrec);
 
   if (d.free_top) {   /* PG_slice_output (drv_rk_curv_col.c:546-550): 15 maps [ny][nx] */
     float *pg = malloc(15 * (size_t)gd->nx * gd->ny * sizeof(float));
     CHK(cgfdh_get_peak_ground_motion(s, pg, NULL));
-    PG_slice_output(pg, gd, output_dir, output_fname_part, topoid);
+    CHKIO(cgfdh_io_pg_output(output_dir, output_fname_part, gd->nx, gd->ny, gd->ni, gd->nj,
+                             gd->gni1, gd->gnj1, topoid, pg));
     free(pg);
   }
 

@@ -52,7 +52,7 @@ def test_configuration_network_forward_backward_match_oracle():
     from cgfd3d_amd import surrogate as sg
     layers = sg.default_layers(64, 1024, 6)
     nt, nc = 1024, 3 * 64
-    ns = 4
+    ns = 8     # the network runs multiples of 8 samples per call
     net = sg.SurrogateNet(layers, seed=5, mt_terms=6, out_shape=(nt, nc))
     rng = np.random.default_rng(13)
     x = rng.standard_normal((ns, layers[0][0])).astype(np.float32)

@@ -30,7 +30,7 @@ def _declared(header):
 def test_library_exports_every_declared_symbol():
     from cgfd3d_amd._lib import LIB_PATH
     headers = sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h"))
-    assert headers == ["cgfdh.h", "cgfds.h"]
+    assert headers == ["cgfdh.h", "cgfdh_io.h", "cgfds.h"]
     names = [n for h in headers for n in _declared(h)]
     assert len(names) >= 30
     lib = C.CDLL(LIB_PATH)
