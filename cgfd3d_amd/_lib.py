@@ -50,6 +50,7 @@ SIGNATURES = [
     ("cgfdh_get_seismograms", C.c_int, [_P, _P]),
     ("cgfdh_get_peak_ground_motion", C.c_int, [_P, _P, _P]),
     ("cgfdh_get_subvolume", C.c_int, [_P, _P, C.c_int, _P, _P]),
+    ("cgfdh_get_subvolume_noalloc", C.c_int, [_P, _P, C.c_int, _P, _P]),
     ("cgfdh_set_ablexp", C.c_int, [_P, _P, _P, _P, _P]),
     ("cgfdh_set_graves_qs", C.c_int, [_P, _P, C.c_float]),
     ("cgfdh_get_strain_subvolume", C.c_int, [_P, _P, _P]),

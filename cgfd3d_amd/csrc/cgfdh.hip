@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "cgfdh.h"
@@ -211,6 +212,7 @@ struct cgfdh_solver {
   float* seismo = nullptr;
   float* pg = nullptr;           // 15 peak maps + 3 displacement maps, [ny][nx] each
   float* outbuf = nullptr;       // staging for packed output sub-volumes (grown on demand)
+  float* scratch = nullptr;      // device address of the static sub-volume buffer
   float* qs = nullptr;           // Graves-Qs quality factor per point (NULL: off)
   float qs_coef = 0.0f;          // -pi * freq * dt
   float* abl = nullptr;          // ablexp masks Ex[nx], Ey[ny], Ez[nz]
@@ -261,6 +263,8 @@ static void free_all(cgfdh_solver* s)
   F(s->recv_idx); F(s->recv_w); F(s->seismo); F(s->pg); F(s->outbuf); F(s->abl);
   F(s->qs);
 }
+
+static int scratch_resolve(float** out);
 
 extern "C" int cgfdh_create(const cgfdh_desc* d, cgfdh_solver** out)
 {
@@ -344,6 +348,9 @@ if (!bad && hipStreamCreate(&s->stream) != hipSuccess) bad = 1;
     return fail(CGFDH_ENOMEM, "cgfdh_create: device allocation failed for %dx%dx%d", d->ni,
                 d->nj, d->nk);
   }
+  // the static sub-volume buffer's address on this device (its entry point
+  // reports a failure here; the solver works without it)
+  if (scratch_resolve(&s->scratch) != CGFDH_OK) s->scratch = nullptr;
   *out = s;
   return CGFDH_OK;
 }
@@ -1257,6 +1264,72 @@ extern "C" int cgfdh_get_subvolume(cgfdh_solver* s, const int* box, int ncomp, c
                      s->lvl[0], s->vol, s->nx, s->ss, b, npt, cl, s->outbuf);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, s->outbuf, (size_t)ncomp * npt * sizeof(float),
+                        hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return CGFDH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// cgfdh_get_subvolume without any device allocation: the packed points go
+// through a fixed file-scope device array instead of the solver's staging
+// buffer.  Each GPU holds its own copy of the array from the code object's
+// load until the process ends; host code reaches it only through the device
+// address hipGetSymbolAddress gives for that device (kept per device at
+// solver creation, with the device current), never through the array's name.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) float scratch[CGFDH_SUBVOLUME_STATIC_MAX];
+
+constexpr int kScratchDevices = 64;
+static float* g_scratch_ptr[kScratchDevices];
+static std::mutex g_scratch_mu;
+
+// the scratch array's device address on the current device, resolved once
+static int scratch_resolve(float** out)
+{
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchDevices)
+    return fail(CGFDH_EDEVICE, "scratch: device %d beyond %d", dev, kScratchDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(CGFDH_EDEVICE, "scratch: hipGetSymbolAddress failed: %s",
+                  hipGetErrorString(e));
+    g_scratch_ptr[dev] = (float*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_get_subvolume_noalloc(cgfdh_solver* s, const int* box, int ncomp,
+                                           const int* comps, float* out)
+{
+  if (!s || !comps || !out) return fail(CGFDH_EINVAL, "cgfdh_get_subvolume_noalloc: null argument");
+  if (ncomp < 1 || ncomp > s->nw)
+    return fail(CGFDH_EINVAL, "cgfdh_get_subvolume_noalloc: ncomp %d outside 1..%d", ncomp, s->nw);
+  CompList cl{};
+  for (int c = 0; c < ncomp; c++) {
+    if (comps[c] < 0 || comps[c] >= s->nw)
+      return fail(CGFDH_EINVAL, "cgfdh_get_subvolume_noalloc: component %d outside 0..%d",
+                  comps[c], s->nw - 1);
+    cl.c[c] = comps[c];
+  }
+  SubBox b;
+  long npt;
+  int rc = check_box(s, box, "cgfdh_get_subvolume_noalloc", &b, &npt);
+  if (rc) return rc;
+  if ((size_t)ncomp * (size_t)npt > (size_t)CGFDH_SUBVOLUME_STATIC_MAX)
+    return fail(CGFDH_EINVAL,
+                "cgfdh_get_subvolume_noalloc: %d components x %ld points exceed the %d-value "
+                "static buffer (use cgfdh_get_subvolume)",
+                ncomp, npt, CGFDH_SUBVOLUME_STATIC_MAX);
+  if (!s->scratch) return fail(CGFDH_ESTATE, "cgfdh_get_subvolume_noalloc: no scratch address");
+  hipLaunchKernelGGL(subvolume_kernel, subvolume_grid(npt, ncomp), dim3(256), 0, s->stream,
+                     s->lvl[0], s->vol, s->nx, s->ss, b, npt, cl, s->scratch);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, s->scratch, (size_t)ncomp * npt * sizeof(float),
                         hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   return CGFDH_OK;

@@ -231,6 +231,19 @@ int cgfdh_get_peak_ground_motion(cgfdh_solver *s, float *pg, float *dis);
  * Isotropic elastic and visco-elastic media only. */
 int cgfdh_get_subvolume(cgfdh_solver *s, const int *box, int ncomp, const int *comps,
                         float *out);
+/* cgfdh_get_subvolume for callers that cannot allocate (code under stream
+ * capture, tight serving loops, harnesses that forbid hipMalloc); it has no
+ * counterpart in the reference.  The same packed output, through a fixed
+ * device buffer of CGFDH_SUBVOLUME_STATIC_MAX floats (32 MiB per GPU, held
+ * from the library's load until the process ends) instead of the solver's
+ * growing staging buffer: no hipMalloc, hipFree or other allocating call.
+ * ncomp times the box's point count above the limit is rejected with
+ * CGFDH_EINVAL.  Every call on one GPU shares that buffer, which keeps the
+ * last call's packed values: calls that overlap on one GPU (several host
+ * threads, or solvers on different streams of one device at once) race. */
+#define CGFDH_SUBVOLUME_STATIC_MAX (8 * 1024 * 1024)
+int cgfdh_get_subvolume_noalloc(cgfdh_solver *s, const int *box, int ncomp, const int *comps,
+                                float *out);
 int cgfdh_get_strain_subvolume(cgfdh_solver *s, const int *box, float *out);
 
 /* Stable time step of a curvilinear grid, blk_dt_esti_curv
