@@ -40,7 +40,10 @@ SIGNATURES = [
     ("cgfdh_set_sources", C.c_int,
      [_P, C.c_int, _P, _P, _P, C.c_int, _P, _P, C.c_int, C.c_int, _P, C.c_int, C.c_int, _P,
       _P]),
+    ("cgfdh_set_dd_sources", C.c_int, [_P, C.c_int, _P, C.c_int, C.c_int, C.c_int, C.c_int]),
+    ("cgfdh_put_dd_values", C.c_int, [_P, C.c_int, C.c_int, _P, _P]),
     ("cgfdh_set_receivers", C.c_int, [_P, C.c_int, _P, _P]),
+    ("cgfdh_check_nan", C.c_int, [_P, _P, _P]),
     ("cgfdh_set_wavefield", C.c_int, [_P, _P]),
     ("cgfdh_get_wavefield", C.c_int, [_P, _P]),
     ("cgfdh_pml_aux_len", C.c_size_t, [_P]),
@@ -91,6 +94,16 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise CgfdhError(f"{LIB_PATH} is missing: build it with "
                          "`python -m cgfd3d_amd.build` (hipcc, gfx950)")
+    # One HIP runtime per process: torch ships its own libamdhip64 and loads
+    # it by a name that does not match an already loaded /opt/rocm runtime,
+    # so a process that loaded this library first and imports torch later
+    # ends up with two runtimes, and torch then sees no device.  Loading
+    # torch first makes this library bind to torch's runtime (same soname),
+    # so device pointers and streams are shared with it.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     for name, res, args in SIGNATURES:
         f = getattr(L, name)

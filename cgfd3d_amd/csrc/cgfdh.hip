@@ -8,6 +8,7 @@
 // replaces.  PML aux needs only pre/tmp/end: it is read at its own point.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -206,6 +207,16 @@ struct cgfdh_solver {
   int nsurf = 0;                 // surface forces: indices and force-rate tables
   int* surf_idx = nullptr;
   float* frate = nullptr;
+  // dd sources at grid points: marks, sorted indices and their input
+  // positions on the device, a dd_cap-row ring of value tables, and per row
+  // the step whose values it holds (-1: none)
+  int ndd = 0, dd_max_nt = 0, dd_cap = 0;
+  unsigned* dd_mask = nullptr;
+  long* dd_idx = nullptr;
+  int* dd_perm = nullptr;
+  float* dd_vi = nullptr;
+  float* dd_mij = nullptr;
+  std::vector<int> dd_row_it;
   int nrecv = 0;
   int* recv_idx = nullptr;
   float* recv_w = nullptr;
@@ -213,6 +224,7 @@ struct cgfdh_solver {
   float* pg = nullptr;           // 15 peak maps + 3 displacement maps, [ny][nx] each
   float* outbuf = nullptr;       // staging for packed output sub-volumes (grown on demand)
   float* scratch = nullptr;      // device address of the static sub-volume buffer
+  unsigned long long* nan_key = nullptr;   // result word of the NaN scan
   float* qs = nullptr;           // Graves-Qs quality factor per point (NULL: off)
   float qs_coef = 0.0f;          // -pi * freq * dt
   float* abl = nullptr;          // ablexp masks Ex[nx], Ey[ny], Ez[nz]
@@ -262,6 +274,8 @@ static void free_all(cgfdh_solver* s)
   F(s->src); F(s->mij); F(s->fxyz); F(s->src_coef); F(s->surf_idx); F(s->frate);
   F(s->recv_idx); F(s->recv_w); F(s->seismo); F(s->pg); F(s->outbuf); F(s->abl);
   F(s->qs);
+  F(s->dd_mask); F(s->dd_idx); F(s->dd_perm); F(s->dd_vi); F(s->dd_mij);
+  F(s->nan_key);
 }
 
 static int scratch_resolve(float** out);
@@ -621,6 +635,99 @@ extern "C" int cgfdh_set_sources(cgfdh_solver* s, int nsrc, const int* ijk,
   return CGFDH_OK;
 }
 
+extern "C" int cgfdh_set_dd_sources(cgfdh_solver* s, int ndd, const size_t* indx, int max_nt,
+                                    int nt_cap, int has_vi, int has_mij)
+{
+  if (s) s->gen++;
+  if (!s || ndd < 0) return fail(CGFDH_EINVAL, "cgfdh_set_dd_sources: bad argument");
+  auto F = [](auto*& p) { if (p) { (void)hipFree(p); p = nullptr; } };
+  F(s->dd_mask); F(s->dd_idx); F(s->dd_perm); F(s->dd_vi); F(s->dd_mij);
+  s->ndd = 0;
+  s->dd_row_it.clear();
+  if (ndd == 0) return CGFDH_OK;
+  const int mt = s->d.medium_type;
+  if (mt != CGFDH_MEDIUM_ELASTIC_ISO && mt != CGFDH_MEDIUM_VISCO_ISO)
+    return fail(CGFDH_EINVAL,
+                "cgfdh_set_dd_sources: dd sources need the isotropic or GMB medium (medium "
+                "type %d)",
+                mt);
+  if (!indx || max_nt < 1 || nt_cap < 1 || (!has_vi && !has_mij))
+    return fail(CGFDH_EINVAL, "cgfdh_set_dd_sources: missing points, steps or value kinds");
+  // the points, sorted by index with equal indices kept in input order
+  std::vector<int> perm(ndd);
+  for (int q = 0; q < ndd; q++) {
+    const long p = (long)indx[q];
+    const long i = p % s->nx, j = p / s->nx % s->ny, k = p / s->ss;
+    if (indx[q] >= (size_t)s->vol || i < s->ni1 || i > s->ni2 || j < s->nj1 || j > s->nj2 ||
+        k < s->nk1 || k > s->nk2)
+      return fail(CGFDH_EINVAL,
+                  "cgfdh_set_dd_sources: source %d at index %zu is not a physical grid point", q,
+                  indx[q]);
+    perm[q] = q;
+  }
+  std::stable_sort(perm.begin(), perm.end(),
+                   [&](int a, int b) { return indx[a] < indx[b]; });
+  std::vector<long> sorted(ndd);
+  std::vector<unsigned> mask(((size_t)s->vol + 31) / 32, 0u);
+  for (int q = 0; q < ndd; q++) {
+    sorted[q] = (long)indx[perm[q]];
+    mask[sorted[q] >> 5] |= 1u << (sorted[q] & 31);
+  }
+  const int cap = nt_cap < max_nt ? nt_cap : max_nt;
+  const size_t rows = (size_t)cap * 4 * ndd;
+  HIPCHK(hipMalloc((void**)&s->dd_mask, mask.size() * sizeof(unsigned)));
+  HIPCHK(hipMemcpy(s->dd_mask, mask.data(), mask.size() * sizeof(unsigned),
+                   hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc((void**)&s->dd_idx, ndd * sizeof(long)));
+  HIPCHK(hipMemcpy(s->dd_idx, sorted.data(), ndd * sizeof(long), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc((void**)&s->dd_perm, ndd * sizeof(int)));
+  HIPCHK(hipMemcpy(s->dd_perm, perm.data(), ndd * sizeof(int), hipMemcpyHostToDevice));
+  if (has_vi) HIPCHK(hipMalloc((void**)&s->dd_vi, 3 * rows * sizeof(float)));
+  if (has_mij) HIPCHK(hipMalloc((void**)&s->dd_mij, 6 * rows * sizeof(float)));
+  s->ndd = ndd;
+  s->dd_max_nt = max_nt;
+  s->dd_cap = cap;
+  s->dd_row_it.assign(cap, -1);
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_put_dd_values(cgfdh_solver* s, int it0, int nt, const float* vi,
+                                   const float* mij)
+{
+  if (!s || it0 < 0 || nt < 0) return fail(CGFDH_EINVAL, "cgfdh_put_dd_values: bad argument");
+  if (s->ndd == 0) return fail(CGFDH_EINVAL, "cgfdh_put_dd_values: no dd sources set");
+  if (nt > s->dd_cap || it0 + nt > s->dd_max_nt)
+    return fail(CGFDH_EINVAL,
+                "cgfdh_put_dd_values: steps %d..%d exceed the %d-row table or max_nt %d", it0,
+                it0 + nt - 1, s->dd_cap, s->dd_max_nt);
+  if ((s->dd_vi && !vi) || (s->dd_mij && !mij))
+    return fail(CGFDH_EINVAL, "cgfdh_put_dd_values: missing value tables");
+  // rows in flight on the device are read by enqueued stages: wait for them
+  HIPCHK(hipStreamSynchronize(s->stream));
+  const size_t per = (size_t)4 * s->ndd;
+  for (int n = 0; n < nt; n++) {
+    const int row = (it0 + n) % s->dd_cap;
+    if (s->dd_vi)
+      HIPCHK(hipMemcpy(s->dd_vi + 3 * per * row, vi + 3 * per * n, 3 * per * sizeof(float),
+                       hipMemcpyHostToDevice));
+    if (s->dd_mij)
+      HIPCHK(hipMemcpy(s->dd_mij + 6 * per * row, mij + 6 * per * n, 6 * per * sizeof(float),
+                       hipMemcpyHostToDevice));
+    s->dd_row_it[row] = it0 + n;
+  }
+  return CGFDH_OK;
+}
+
+// the value rows of steps it0 .. it0 + nsteps - 1 that add dd sources are loaded
+static int dd_check(const cgfdh_solver* s, int it0, int nsteps, const char* who)
+{
+  if (s->ndd == 0) return CGFDH_OK;
+  for (int it = it0; it < it0 + nsteps && it < s->dd_max_nt; it++)
+    if (s->dd_row_it[it % s->dd_cap] != it)
+      return fail(CGFDH_EINVAL, "%s: dd source values of step %d are not loaded", who, it);
+  return CGFDH_OK;
+}
+
 extern "C" int cgfdh_set_moment_sources(cgfdh_solver* s, int nsrc, const int* ijk,
                                         const int* it_begin, const int* it_end, int max_nt,
                                         const float* mij, int spatial, int ext_half,
@@ -669,6 +776,56 @@ extern "C" int cgfdh_get_wavefield(cgfdh_solver* s, float* w)
   HIPCHK(hipMemcpyAsync(w, s->lvl[0], (size_t)s->nw * s->vol * sizeof(float), hipMemcpyDeviceToHost,
                         s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
+  return CGFDH_OK;
+}
+
+__global__ void set_u64_kernel(unsigned long long* p, unsigned long long v) { *p = v; }
+
+// the first NaN of a level in wav_check_value's scan order (wav_t.c:253-273:
+// component after component, array index within each): the smallest flat
+// index icmp * vol + iptr that holds a NaN.  Grid-stride, so a thread's
+// first hit is its smallest; block minimum, then one atomicMin per block.
+__global__ void __launch_bounds__(256) nan_scan_kernel(const float* w, long n,
+                                                       unsigned long long* key)
+{
+  __shared__ unsigned long long red[256];
+  unsigned long long k = ~0ull;
+  const long stride = (long)gridDim.x * 256;
+  for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < n; q += stride)
+    if (w[q] != w[q]) {
+      k = (unsigned long long)q;
+      break;
+    }
+  red[threadIdx.x] = k;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h && red[threadIdx.x + h] < red[threadIdx.x])
+      red[threadIdx.x] = red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && red[0] != ~0ull) atomicMin(key, red[0]);
+}
+
+extern "C" int cgfdh_check_nan(cgfdh_solver* s, int* icmp, long* iptr)
+{
+  if (!s || !icmp || !iptr) return fail(CGFDH_EINVAL, "cgfdh_check_nan: null argument");
+  if (!s->nan_key) HIPCHK(hipMalloc((void**)&s->nan_key, sizeof(unsigned long long)));
+  const long n = (long)s->nw * s->vol;
+  hipLaunchKernelGGL(set_u64_kernel, dim3(1), dim3(1), 0, s->stream, s->nan_key, ~0ull);
+  const long blocks = std::min<long>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(nan_scan_kernel, dim3((unsigned)blocks), dim3(256), 0, s->stream, s->lvl[0],
+                     n, s->nan_key);
+  HIPCHK(hipGetLastError());
+  unsigned long long k = 0;
+  HIPCHK(hipMemcpyAsync(&k, s->nan_key, sizeof k, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (k == ~0ull) {
+    *icmp = -1;
+    *iptr = -1;
+  } else {
+    *icmp = (int)((long)k / s->vol);
+    *iptr = (long)k % s->vol;
+  }
   return CGFDH_OK;
 }
 
@@ -800,6 +957,9 @@ static StageParams make_params(cgfdh_solver* s, int it, int istage, const float*
   P.nsurf = s->nsurf; P.surf_idx = s->surf_idx; P.frate = s->frate;
   P.frate_stride = (long)s->nsurf * s->src_max_nt * 4;
   P.matF = s->mats + 18 * s->ss;
+  P.ndd = s->ndd; P.dd_max_nt = s->dd_max_nt; P.dd_cap = s->dd_cap;
+  P.dd_mask = s->dd_mask; P.dd_idx = s->dd_idx; P.dd_perm = s->dd_perm;
+  P.dd_vi = s->dd_vi; P.dd_mij = s->dd_mij;
   P.it = it; P.istage = istage;
   return P;
 }
@@ -985,6 +1145,8 @@ extern "C" int cgfdh_run(cgfdh_solver* s, int it0, int nsteps)
   if (it0 < 0 || nsteps < 0 || it0 + nsteps > s->d.nt_total)
     return fail(CGFDH_EINVAL, "cgfdh_run: steps %d..%d outside nt_total %d", it0, it0 + nsteps,
                 s->d.nt_total);
+  rc = dd_check(s, it0, nsteps, "cgfdh_run");
+  if (rc) return rc;
   const int nev = 8 * nsteps;
   while ((int)s->ev_stage.size() < nev) {
     hipEvent_t e;
@@ -1030,6 +1192,7 @@ extern "C" int cgfdh_onestage(cgfdh_solver* s, int it, int ipair, int istage, co
   if (ipair < 0 || ipair > 7 || istage < 0 || istage > 3)
     return fail(CGFDH_EINVAL, "cgfdh_onestage: pair %d stage %d", ipair, istage);
   int rc = ready(s, "cgfdh_onestage");
+  if (!rc) rc = dd_check(s, it, 1, "cgfdh_onestage");
   if (rc) return rc;
   // inputs go to tmpA / aux tmp, the rhs to tmpB / aux end
   const size_t lv = (size_t)s->nw * s->vol;
@@ -1684,6 +1847,7 @@ extern "C" int cgfdh_batch_run(cgfdh_batch* b, int it0, int nsteps)
   for (size_t i = 0; i < b->sv.size(); i++) {
     cgfdh_solver* s = b->sv[i];
     int rc = ready(s, "cgfdh_batch_run");
+    if (!rc) rc = dd_check(s, it0, nsteps, "cgfdh_batch_run");
     if (rc) return rc;
     if (it0 + nsteps > s->d.nt_total)
       return fail(CGFDH_EINVAL, "cgfdh_batch_run: steps %d..%d outside solver %zu's nt_total %d",

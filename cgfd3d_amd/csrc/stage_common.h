@@ -109,6 +109,15 @@ struct StageParams {
   const float* frate;
   long frate_stride;
   const float* matF;        // matF2Vz2 [ny][nx][3][3]
+  // dd sources at grid points: one bit per point marks them, dd_idx holds
+  // their indices sorted (equal indices in input order) and dd_perm each
+  // one's input position; step it reads row it % dd_cap of the value tables
+  int ndd, dd_max_nt, dd_cap;
+  const unsigned* dd_mask;
+  const long* dd_idx;
+  const int* dd_perm;
+  const float* dd_vi;       // [dd_cap][4][ndd][3] (NULL: none)
+  const float* dd_mij;      // [dd_cap][4][ndd][6] (NULL: none)
   int it, istage;
   const int* it_off;        // if set, a device step offset added to it (graph replays)
   // visco-elastic GMB: bodies, wavefield components (9 + 6 nmax), relaxation
@@ -445,6 +454,47 @@ __device__ __forceinline__ void add_sources(const StageParams& P, int i, int j, 
   }
 }
 
+// dd sources at grid points (sv_curv_col_el.c:486-555): every source at
+// this point, in input order, adds vi * (slw / jac) to the velocity rates
+// and subtracts mij * (1 / jac) (Mxx,Myy,Mzz,Myz,Mxz,Mxy) from the stress
+// rates; nothing from step dd_max_nt on
+__device__ __forceinline__ void add_dd(const StageParams& P, long p, float* r)
+{
+  if (!((P.dd_mask[p >> 5] >> (p & 31)) & 1u)) return;
+  const int it = P.it + (P.it_off ? *P.it_off : 0);
+  if (it >= P.dd_max_nt) return;
+  int lo = 0, hi = P.ndd;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (P.dd_idx[mid] < p) lo = mid + 1;
+    else hi = mid;
+  }
+  const long row = ((long)(it % P.dd_cap) * 4 + P.istage) * P.ndd;
+  const float V = P.slw[p] / P.met[p];
+  const float rjac = (float)(1.0 / (double)P.met[p]);
+  for (int q = lo; q < P.ndd && P.dd_idx[q] == p; q++) {
+    const long e = row + P.dd_perm[q];
+    if (P.dd_vi != nullptr) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) r[VX + c] += P.dd_vi[3 * e + c] * V;
+    }
+    if (P.dd_mij != nullptr) {
+      const int slot[6] = {TXX, TYY, TZZ, TYZ, TXZ, TXY};
+#pragma unroll
+      for (int c = 0; c < 6; c++) r[slot[c]] -= P.dd_mij[6 * e + c] * rjac;
+    }
+  }
+}
+
+// the reference adds dd sources in the isotropic and GMB right-hand sides
+// only (sv_curv_col_el_iso.c:193-199, sv_curv_col_vis_iso.c:235-241)
+template <int MT>
+__device__ __forceinline__ void add_dd_for(const StageParams& P, long p, float* r)
+{
+  if constexpr (MT == MEDIUM_ISO || MT == MEDIUM_VISCO)
+    if (P.ndd > 0) add_dd(P, p, r);
+}
+
 // RK combine of the wavefield (drv_rk_curv_col.c:291-446); NW > 9 when the
 // visco-elastic memory variables follow (P.nw of them are live)
 template <int KIND, int NW = 9>
@@ -594,6 +644,7 @@ __device__ __forceinline__ void interior_finish(const StageParams& P, int i, int
   md.hooke(r, m, Xv, Yv, Zv);
   if constexpr (MT == MEDIUM_VISCO) atten(P, p, md.lam, md.mu, r);
   if (P.nsrc > 0) add_sources(P, i, j, k, p, r);
+  add_dd_for<MT>(P, p, r);
   if constexpr (MT != MEDIUM_VISCO)
     combine_store<KIND>(P, p, r, cpre, cend);
   else
@@ -640,6 +691,7 @@ __device__ __forceinline__ void finish_point_loaded(const StageParams& P, int i,
   }
   if constexpr (MT == MEDIUM_VISCO) atten(P, p, md.lam, md.mu, r);
   if (P.nsrc > 0) add_sources(P, i, j, k, p, r);
+  add_dd_for<MT>(P, p, r);
   if constexpr (MT != MEDIUM_VISCO)
     combine_store<KIND>(P, p, r, cpre, cend);
   else

@@ -338,6 +338,7 @@ stage_interior_phased_kernel(StageParams P, BoxParams B)
   md.hooke(r, m, Xv, Yv, Zv);
   if (!PRE) combine_load<KIND>(P, p, cpre, cend);
   if (P.nsrc > 0) add_sources(P, i, j, k, p, r);
+  add_dd_for<MT>(P, p, r);
   combine_store<KIND>(P, p, r, cpre, cend);
 }
 
@@ -432,6 +433,7 @@ stage_rows_phased_kernel(StageParams P, BoxParams B)
     }
   }
   if (P.nsrc > 0) add_sources(P, i, j, k, p, r);
+  add_dd_for<MT>(P, p, r);
   combine_store<KIND>(P, p, r, cpre, cend);
 }
 
@@ -545,6 +547,7 @@ stage_interior_ldsp_kernel(StageParams P, BoxParams B)
         Zv[3] = {Z[VX], Z[VY], Z[VZ]};
   md.hooke(r, m, Xv, Yv, Zv);
   if (P.nsrc > 0) add_sources(P, i, j, k, p, r);
+  add_dd_for<MT>(P, p, r);
   combine_store<KIND>(P, p, r, cpre, cend);
 }
 
@@ -812,6 +815,7 @@ __device__ __forceinline__ void shellp_point(const StageParams& P, int fslot, in
     }
   }
   if (P.nsrc > 0) add_sources(P, i, j, k, p, r);
+  add_dd_for<MT>(P, p, r);
   combine_store<KIND>(P, p, r, cpre, cend);
 }
 

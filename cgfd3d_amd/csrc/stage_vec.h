@@ -120,7 +120,8 @@ __device__ __forceinline__ void combine_q(const StageParams& P, const float* wcu
 // law from the velocity derivatives (field by field) and the stress
 // combine; then the momentum sums one direction at a time, folded in the
 // reference's order (xi terms, eta terms, zeta terms), and the velocity
-// combine.  Moment sources only touch stresses, so they join the first phase.
+// combine.  The sources' moment terms join the first phase, their force
+// terms the second.
 template <int DX, int DY, int DZ, int KIND, int MT, int BY, int NQ>
 __global__ void __launch_bounds__(64 * BY, NQ == 2 ? 4 : 2)
 stage_interior_vec_kernel(StageParams P, BoxParams B)
@@ -164,7 +165,11 @@ stage_interior_vec_kernel(StageParams P, BoxParams B)
       const float Yq[3] = {Y[0][q], Y[1][q], Y[2][q]};
       const float Zq[3] = {Z[0][q], Z[1][q], Z[2][q]};
       md.hooke(r, m, Xq, Yq, Zq);
+      // the moment terms of the sources here, the force terms in the
+      // velocity pass below
+      r[VX] = r[VY] = r[VZ] = 0.0f;
       if (P.nsrc > 0) add_sources(P, i + q, j, k, p + q, r);
+      add_dd_for<MT>(P, p + q, r);
 #pragma unroll
       for (int t = 0; t < 6; t++) rT[t][q] = r[TXX + t];
     }
@@ -200,13 +205,23 @@ stage_interior_vec_kernel(StageParams P, BoxParams B)
     }
     float s4[NQ];
     ldq_stream<NQ>(P.slw + p, s4);
+    float rv[3][NQ];
 #pragma unroll
-    for (int v = 0; v < 3; v++) {
-      float rv[NQ];
+    for (int v = 0; v < 3; v++)
 #pragma unroll
-      for (int q = 0; q < NQ; q++) rv[q] = s4[q] * acc[v][q];
-      combine_q<KIND, NQ>(P, w, p, VX + v, rv);
+      for (int q = 0; q < NQ; q++) rv[v][q] = s4[q] * acc[v][q];
+    if ((P.nsrc > 0 && P.fxyz != nullptr) || P.ndd > 0) {
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        float r[9] = {rv[0][q], rv[1][q], rv[2][q], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        if (P.nsrc > 0 && P.fxyz != nullptr) add_sources(P, i + q, j, k, p + q, r);
+        add_dd_for<MT>(P, p + q, r);
+#pragma unroll
+        for (int v = 0; v < 3; v++) rv[v][q] = r[VX + v];
+      }
     }
+#pragma unroll
+    for (int v = 0; v < 3; v++) combine_q<KIND, NQ>(P, w, p, VX + v, rv[v]);
   }
 }
 

@@ -8,14 +8,17 @@
  * Covered: isotropic, VTI and general anisotropic elastic media and the GMB
  * visco-elastic medium, CFS-PML faces, traction-image free surface, point
  * and Gaussian moment and force sources (the strict surface force
- * included), receivers and receiver lines, slices, snapshots (velocity,
- * stress, strain), the peak-ground-motion maps, the ablexp sponge and
- * Graves-Qs attenuation with any elastic medium.  Slices
- * and snapshots are packed on the GPU; only their points are copied into the host wavefield that the reference's own writers
- * (io_slice_nc_put, io_snap_nc_put) then read, so the files are the
- * reference's; the output_all debug dump copies the whole wavefield back each
- * step.  Anything else the par file asks for (acoustic medium, dd sources,
- * several MPI ranks) stops the run with a message, the reference's
+ * included), dd sources at points (their value chunks streamed to the
+ * device as src_dd_accit_loadstf reads them), receivers and receiver lines,
+ * slices, snapshots (velocity, stress, strain), the peak-ground-motion maps,
+ * the ablexp sponge and Graves-Qs attenuation with any elastic medium, and
+ * the NaN check every qc_check_nan_num_of_step steps (wav_check_value; the
+ * reference's driver has that call commented out, the shim makes it).
+ * Slices and snapshots are packed on the GPU and written by the library's
+ * own netCDF writer (include/cgfdh_io.h) in the reference's file layouts;
+ * the output_all debug dump copies the whole wavefield back each step.
+ * Anything else the par file asks for (acoustic medium, smoothed dd
+ * sources, several MPI ranks) stops the run with a message, the reference's
  * fatal-error convention.
  */
 #include <stdio.h>
@@ -79,10 +82,17 @@ drv_rk_curv_col_allstep(
     }
   int mt = md->medium_type;
   int visco = mt == CONST_MEDIUM_VISCOELASTIC_ISO;
+  /* the isotropic and GMB right-hand sides add dd sources
+   * (sv_curv_col_el_iso.c:193-199, sv_curv_col_vis_iso.c:235-241), at points
+   * only (sv_curv_col_el.c:572-577 stops on smoothed ones) */
+  int dd_on = src->dd_is_valid == 1 && (mt == CONST_MEDIUM_ELASTIC_ISO || visco);
+  if (dd_on && src->dd_is_add_at_point != 1) {
+    fprintf(stderr, "ERROR: ddsrc with smoooth not implemented\n");
+    MPI_Abort(MPI_COMM_WORLD, 1);
+  }
   if ((mt != CONST_MEDIUM_ELASTIC_ISO && mt != CONST_MEDIUM_ELASTIC_VTI &&
        mt != CONST_MEDIUM_ELASTIC_ANISO && !visco) ||
-      (visco && md->visco_type != CONST_VISCO_GMB) ||
-      src->dd_is_valid) {
+      (visco && md->visco_type != CONST_VISCO_GMB)) {
     fprintf(stderr, "ERROR: par file asks for a feature the GPU time loop lacks\n");
     MPI_Abort(MPI_COMM_WORLD, 1);
   }
@@ -187,6 +197,17 @@ drv_rk_curv_col_allstep(
     free(ijk); free(mij); free(fxyz); free(frate); free(w);
   }
 
+  /* dd sources: the points and the chunk src_dd_read2local read first
+   * (dd_nt_per_read steps, src_t.c:1838-1900) */
+  const float *dd_vi = NULL, *dd_mij = NULL;
+  if (dd_on) {
+    dd_vi = src->dd_vi_actived ? src->dd_vi : NULL;
+    dd_mij = src->dd_mij_actived ? src->dd_mij : NULL;
+    CHK(cgfdh_set_dd_sources(s, src->dd_total_number, src->dd_indx, src->dd_max_nt,
+                             src->dd_nt_per_read, dd_vi != NULL, dd_mij != NULL));
+    CHK(cgfdh_put_dd_values(s, 0, src->dd_nt_per_read, dd_vi, dd_mij));
+  }
+
   /* receivers, then the points of the receiver lines as receivers with zero
    * interpolation weights on their own grid point: the engine's trilinear
    * sum (io_funcs.c:1598-1612) then returns the point's value, which is what
@@ -280,11 +301,34 @@ drv_rk_curv_col_allstep(
   const int stress_cmps[6] = {3, 4, 5, 7, 6, 8};
   int it_run = 0;
   for (int it = 0; it < nt_total; it++) {
-    int due = nsl > 0 || output_all;
+    /* drv_rk_curv_col.c:179-181; after a new chunk is read the steps before
+     * it run on the old one, then the new one goes to the device */
+    if (src->dd_is_valid == 1) {
+      src_dd_accit_loadstf(src, it, mympi->myid);
+      if (dd_on && it > 0 && src->dd_is_valid == 1 && src->dd_it_here == 0) {
+        if (it_run < it) CHK(cgfdh_run(s, it_run, it - it_run));
+        it_run = it;
+        CHK(cgfdh_put_dd_values(s, it, src->dd_nt_this_read, dd_vi, dd_mij));
+      }
+    }
+    /* the value check of check_nan_every_nummber_of_steps
+     * (drv_rk_curv_col.c:476-479), wav_check_value's message and exit */
+    int nan_due = qc_check_nan_num_of_step > 0 && it % qc_check_nan_num_of_step == 0;
+    int due = nsl > 0 || output_all || nan_due;
     for (int n = 0; n < nsn && !due; n++) due = snap_due(iosnap, n, it, nt_total);
     if (!due) continue;
     CHK(cgfdh_run(s, it_run, it + 1 - it_run));
     it_run = it + 1;
+    if (nan_due) {
+      int nan_c;
+      long nan_p;
+      CHK(cgfdh_check_nan(s, &nan_c, &nan_p));
+      if (nan_c >= 0) {
+        fprintf(stderr, "ERROR: NaN occurs at iptr=%ld icmp=%d\n", nan_p, nan_c);
+        fflush(stderr);
+        exit(-1);
+      }
+    }
     float t_cur = it * dt + t0;
     float t_end = t_cur + dt;
     q = 0;

@@ -170,6 +170,25 @@ class Solver:
                                       pp(f), spatial, ext_half, pp(ec), int(bool(strict)),
                                       nsurf, pp(si), pp(fr)), "set_sources")
 
+    def set_dd_sources(self, indx, max_nt, nt_cap=None, vi=True, mij=True):
+        """dd sources at the 1-D array indices indx (cgfdh_set_dd_sources);
+        nt_cap steps of values on the device (default max_nt)."""
+        idx = np.ascontiguousarray(indx, np.uint64).reshape(-1)
+        self.ndd = idx.size
+        cap = max_nt if nt_cap is None else nt_cap
+        check(lib().cgfdh_set_dd_sources(self.h, idx.size, _p(idx), max_nt, cap, int(bool(vi)),
+                                         int(bool(mij))), "set_dd_sources")
+
+    def put_dd_values(self, it0, vi=None, mij=None):
+        """Values of steps it0.. (cgfdh_put_dd_values): vi [nt][4][ndd][3],
+        mij [nt][4][ndd][6]."""
+        v = None if vi is None else _f32(vi)
+        m = None if mij is None else _f32(mij)
+        per = 4 * self.ndd
+        nt = v.size // (3 * per) if v is not None else m.size // (6 * per)
+        pp = lambda a: None if a is None else _p(a)  # noqa: E731
+        check(lib().cgfdh_put_dd_values(self.h, it0, nt, pp(v), pp(m)), "put_dd_values")
+
     def set_receivers(self, indx1d, dijk):
         idx = np.ascontiguousarray(indx1d, np.int32).reshape(-1)
         w = _f32(dijk).reshape(-1)
@@ -186,6 +205,13 @@ class Solver:
         w = np.empty(self.ncomp * self.vol, np.float32)
         check(lib().cgfdh_get_wavefield(self.h, _p(w)), "get_wavefield")
         return w
+
+    def check_nan(self):
+        """(icmp, iptr) of the first NaN of the current state, or None
+        (cgfdh_check_nan)."""
+        c, p = C.c_int(), C.c_long()
+        check(lib().cgfdh_check_nan(self.h, C.byref(c), C.byref(p)), "check_nan")
+        return None if c.value < 0 else (c.value, p.value)
 
     def aux_len(self):
         return int(lib().cgfdh_pml_aux_len(self.h))

@@ -185,6 +185,26 @@ int cgfdh_set_sources(cgfdh_solver *s, int nsrc, const int *ijk, const int *it_b
                       int spatial, int ext_half, const float *ext_coefs, int strict,
                       int nsurf, const int *surf_idx, const float *frate);
 
+/* Distributed (dd) sources added at grid points, the src_t dd_* members that
+ * src_dd_read2local fills (src_t.c:1180-1900, dd_is_add_at_point = 1) and
+ * sv_curv_col_el_rhs_srcdd adds (sv_curv_col_el.c:486-555, called from the
+ * isotropic and GMB right-hand sides only): ndd points at the 1-D array
+ * indices indx (src->dd_indx, physical points, duplicates allowed and added
+ * in input order), steps 0 .. max_nt-1 (src->dd_max_nt), a device table of
+ * nt_cap steps (src->dd_nt_per_read), velocity values (has_vi,
+ * src->dd_vi_actived) and/or moment values (has_mij, src->dd_mij_actived).
+ * ndd = 0 removes them.  Isotropic and GMB media only. */
+int cgfdh_set_dd_sources(cgfdh_solver *s, int ndd, const size_t *indx, int max_nt, int nt_cap,
+                         int has_vi, int has_mij);
+/* Values of steps it0 .. it0+nt-1 (nt <= nt_cap), as the reference's chunk
+ * holds them after src_dd_accit_loadstf (src_t.c:1929-2005):
+ * vi[nt][4][ndd][3] and mij[nt][4][ndd][6] (Mxx,Myy,Mzz,Myz,Mxz,Mxy), each
+ * [stage][source] row the value of that RK stage.  Step it occupies table
+ * row it % nt_cap, replacing what it held.  A step below max_nt whose values
+ * are not in the table makes cgfdh_run / cgfdh_onestage / cgfdh_batch_run
+ * fail; from step max_nt on nothing is added. */
+int cgfdh_put_dd_values(cgfdh_solver *s, int it0, int nt, const float *vi, const float *mij);
+
 /* Receivers as iorecv_one_t keeps them: indx1d[8] and di,dj,dk
  * (io_funcs.c:358-389); samples follow io_recv_keep (io_funcs.c:1585-1623). */
 int cgfdh_set_receivers(cgfdh_solver *s, int nrecv, const int *indx1d,
@@ -194,6 +214,12 @@ int cgfdh_set_receivers(cgfdh_solver *s, int nrecv, const int *indx1d,
  * ncomp volumes. */
 int cgfdh_set_wavefield(cgfdh_solver *s, const float *w9);
 int cgfdh_get_wavefield(cgfdh_solver *s, float *w9);
+/* The value check of wav_check_value (wav_t.c:253-273) on the current state,
+ * as the par file's check_nan_every_nummber_of_steps asks for it
+ * (drv_rk_curv_col.c:476-479): *icmp and *iptr of the first NaN in its scan
+ * order (component by component, array index within each), or -1 and -1
+ * when there is none. */
+int cgfdh_check_nan(cgfdh_solver *s, int *icmp, long *iptr);
 size_t cgfdh_pml_aux_len(cgfdh_solver *s);   /* floats, all enabled faces */
 int cgfdh_set_pml_aux(cgfdh_solver *s, const float *aux);
 int cgfdh_get_pml_aux(cgfdh_solver *s, float *aux);
@@ -231,6 +257,7 @@ int cgfdh_get_peak_ground_motion(cgfdh_solver *s, float *pg, float *dis);
  * Isotropic elastic and visco-elastic media only. */
 int cgfdh_get_subvolume(cgfdh_solver *s, const int *box, int ncomp, const int *comps,
                         float *out);
+
 /* cgfdh_get_subvolume for callers that cannot allocate (code under stream
  * capture, tight serving loops, harnesses that forbid hipMalloc); it has no
  * counterpart in the reference.  The same packed output, through a fixed

@@ -1309,6 +1309,41 @@ static void rhs_src(orc_t *o, int it, int istage, real_t *h)
   free(ext);
 }
 
+/* dd sources at grid points, sv_curv_col_el_rhs_srcdd (sv_curv_col_el.c:
+ * 486-555): for step it < dd_max_nt, each source in order adds vi * (slw /
+ * jac) to the velocity rates and subtracts mij * (1 / jac), the reciprocal
+ * in double, from the stress rates (mij order Mxx,Myy,Mzz,Myz,Mxz,Mxy) */
+static void rhs_srcdd(orc_t *o, int it, int istage, real_t *h)
+{
+  const orc_cfg *c = &o->cfg;
+  if (c->ndd <= 0 || it >= c->dd_max_nt) return;
+  size_t V = o->vol;
+  const real_t *jac = o->metric;
+  if (c->dd_vi) {
+    const float *vi = c->dd_vi + ((size_t)it * 4 + istage) * c->ndd * 3;
+    for (int is = 0; is < c->ndd; is++) {
+      size_t p = (size_t)c->dd_indx[is];
+      real_t Vf = o->slw[p] / jac[p];
+      h[VX * V + p] += vi[3 * is + 0] * Vf;
+      h[VY * V + p] += vi[3 * is + 1] * Vf;
+      h[VZ * V + p] += vi[3 * is + 2] * Vf;
+    }
+  }
+  if (c->dd_mij) {
+    const float *m = c->dd_mij + ((size_t)it * 4 + istage) * c->ndd * 6;
+    for (int is = 0; is < c->ndd; is++) {
+      size_t p = (size_t)c->dd_indx[is];
+      real_t rjac = 1.0 / jac[p];
+      h[TXX * V + p] -= m[6 * is + 0] * rjac;
+      h[TYY * V + p] -= m[6 * is + 1] * rjac;
+      h[TZZ * V + p] -= m[6 * is + 2] * rjac;
+      h[TYZ * V + p] -= m[6 * is + 3] * rjac;
+      h[TXZ * V + p] -= m[6 * is + 4] * rjac;
+      h[TXY * V + p] -= m[6 * is + 5] * rjac;
+    }
+  }
+}
+
 /* GMB attenuation after the PML terms, sv_curv_col_vis_iso.c:251-342: memory
  * variable rates and the memory terms subtracted from the stress rates.
  * J slots of body n: 9+6n + {Jxx, Jyy, Jzz, Jyz, Jxz, Jxy} */
@@ -1502,6 +1537,7 @@ void orc_onestage(orc_t *o, int it, int ipair, int istage,
   if (o->aux_total) rhs_pml(o, w_cur, rhs, aux_cur, aux_rhs, &ox, &oy, &oz);
   if (o->mt == ORC_MEDIUM_VISCO) rhs_atten(o, w_cur, rhs);
   if (o->cfg.nsrc > 0) rhs_src(o, it, istage, rhs);
+  if (ISO_LIKE(o)) rhs_srcdd(o, it, istage, rhs);
 }
 
 /* peak ground motion on the free surface, PG_calcu (wav_t.c:380-455):

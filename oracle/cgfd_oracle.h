@@ -83,6 +83,18 @@ typedef struct {
   int nsurf;
   const int *src_surf_idx;
   const float *src_frate;
+  /* distributed (dd) sources added at grid points (sv_curv_col_el_rhs_srcdd,
+   * sv_curv_col_el.c:486-555; called by the isotropic and visco-elastic
+   * RHS only, sv_curv_col_el_iso.c:194-199, sv_curv_col_vis_iso.c:236-241):
+   * dd_indx the points' array indices; dd_vi [dd_max_nt][4][ndd][3] and
+   * dd_mij [dd_max_nt][4][ndd][6] (Mxx,Myy,Mzz,Myz,Mxz,Mxy) the values of
+   * step it and stage (NULL for none); nothing is added from step
+   * dd_max_nt on (src_dd_accit_loadstf, src_t.c:1929-1944) */
+  int ndd;
+  int dd_max_nt;
+  const long *dd_indx;
+  const float *dd_vi;
+  const float *dd_mij;
 } orc_cfg;
 
 typedef struct orc_s orc_t;

@@ -28,7 +28,8 @@ class OrcCfg(C.Structure):
                 ("src_max_nt", C.c_int), ("src_mij", _P), ("medium_type", C.c_int),
                 ("nmaxwell", C.c_int), ("visco_wl", _P), ("xyz", _P), ("src_fxyz", _P),
                 ("src_force_strict", C.c_int), ("nsurf", C.c_int), ("src_surf_idx", _P),
-                ("src_frate", _P)]
+                ("src_frate", _P), ("ndd", C.c_int), ("dd_max_nt", C.c_int),
+                ("dd_indx", _P), ("dd_vi", _P), ("dd_mij", _P)]
 
 
 MEDIUM_ISO, MEDIUM_VTI, MEDIUM_ANISO, MEDIUM_VISCO = 2, 3, 4, 5
@@ -43,7 +44,8 @@ class RefCfg(C.Structure):
                 ("src_it_begin", _P), ("src_it_end", _P), ("src_mij", _P),
                 ("medium_type", C.c_int), ("nmaxwell", C.c_int), ("visco_wl", _P),
                 ("src_fxyz", _P), ("src_force_strict", C.c_int), ("surf_slices", _P),
-                ("surf_nt", C.c_int)]
+                ("surf_nt", C.c_int), ("ndd", C.c_int), ("dd_max_nt", C.c_int),
+                ("dd_indx", _P), ("dd_vi", _P), ("dd_mij", _P)]
 
 
 def load_oracle(fp64=False):
@@ -114,6 +116,24 @@ def load_ref():
     return lib
 
 
+def set_dd_fields(cfg, dd, keep):
+    """dd sources at points into an OrcCfg or RefCfg: dd holds indx (1-D
+    indices), max_nt and the tables vi [max_nt][4][ndd][3] and/or mij
+    [max_nt][4][ndd][6]."""
+    if dd is None or len(dd["indx"]) == 0:
+        return
+    idx = np.ascontiguousarray(dd["indx"], np.int64)
+    keep.append(idx)
+    cfg.ndd = idx.size
+    cfg.dd_max_nt = int(dd["max_nt"])
+    cfg.dd_indx = idx.ctypes.data
+    for key in ("vi", "mij"):
+        if dd.get(key) is not None:
+            t = np.ascontiguousarray(dd[key], np.float32)
+            keep.append(t)
+            setattr(cfg, "dd_" + key, t.ctypes.data)
+
+
 class Oracle:
     """One oracle solver instance over a fixed grid, medium and boundary set.
 
@@ -180,6 +200,7 @@ class Oracle:
                 cfg.nsurf = len(si)
                 cfg.src_surf_idx = si.ctypes.data
                 cfg.src_frate = fr.ctypes.data
+        set_dd_fields(cfg, case.get("dd"), self._keep)
         self.cfg = cfg
         self.metric = np.ascontiguousarray(case["metric"], np.float32)
         self.med = np.ascontiguousarray(case["med"], np.float32)

@@ -74,6 +74,14 @@ typedef struct {
   int src_force_strict;
   const float *surf_slices;
   int surf_nt;
+  /* dd sources at points (src_t dd_* members): dd_indx the 1-D indices,
+   * dd_vi [dd_max_nt][4][ndd][3] and dd_mij [dd_max_nt][4][ndd][6] the whole
+   * value tables (one read of dd_max_nt steps), NULL for a kind not given */
+  int ndd;
+  int dd_max_nt;
+  const size_t *dd_indx;
+  const float *dd_vi;
+  const float *dd_mij;
 } refh_cfg;
 
 typedef struct {
@@ -190,7 +198,20 @@ static void setup(refh_ctx *c, const refh_cfg *cfg, float *coords,
   c->surf = cfg->surf_slices;
   c->surf_nt = cfg->surf_nt;
   s->total_number = cfg->nsrc;
-  s->dd_is_valid = 0;
+  s->max_stage = 4;
+  s->dd_is_valid = cfg->ndd > 0;
+  if (cfg->ndd > 0) {
+    s->dd_is_add_at_point = 1;
+    s->dd_total_number = cfg->ndd;
+    s->dd_max_nt = cfg->dd_max_nt;
+    s->dd_nt_per_read = cfg->dd_max_nt;
+    s->dd_it_here = 0;
+    s->dd_indx = (size_t *)cfg->dd_indx;
+    s->dd_vi = (float *)cfg->dd_vi;
+    s->dd_mij = (float *)cfg->dd_mij;
+    s->dd_vi_actived = cfg->dd_vi != NULL;
+    s->dd_mij_actived = cfg->dd_mij != NULL;
+  }
   if (cfg->nsrc > 0) {
     s->max_nt = cfg->src_max_nt;
     s->max_stage = 4;
@@ -447,6 +468,11 @@ static void harness_onestage(refh_ctx *c, float *w_cur, float *rhs, int ipair, i
   if (c->src.total_number > 0)
     sv_curv_col_el_rhs_src(hVx, hVy, hVz, hTxx, hTyy, hTzz, hTxz, hTyz, hTxy,
                            m->jac, md->rho, gd, &c->src, 0, 0);
+  /* sv_curv_col_el_iso.c:193-199, sv_curv_col_vis_iso.c:235-241 */
+  if (c->src.dd_is_valid == 1 &&
+      (mt == CONST_MEDIUM_ELASTIC_ISO || mt == CONST_MEDIUM_VISCOELASTIC_ISO))
+    sv_curv_col_el_rhs_srcdd(hVx, hVy, hVz, hTxx, hTyy, hTzz, hTxz, hTyz, hTxy,
+                             m->jac, md->rho, &c->src, 0, 0);
 }
 
 /* ---- operator tables: [pair 8][stage 4][dim 3 (z has 4 layers)] -------- */
@@ -592,6 +618,11 @@ long refh_onestage(const refh_cfg *cfg, float *coords, float *metric10, float *m
   }
   c.src.it = it;
   c.src.istage = istage;
+  /* one read holds every step: row it, none from dd_max_nt on */
+  if (cfg->ndd > 0) {
+    c.src.dd_it_here = it;
+    c.src.dd_is_valid = it < cfg->dd_max_nt;
+  }
   load_surface_slices(&c, 0, 0);
   harness_onestage(&c, wcur, rhs, ipair, istage, patch);
   memcpy(rhs_out, rhs, L * sizeof(float));
@@ -634,6 +665,13 @@ int refh_run(const refh_cfg *cfg, float *coords, float *metric10, float *med3,
   }
   for (int it = 0; it < nt; it++) {
     int ipair = it % c.fd.num_of_pairs;
+    /* the step bookkeeping of src_dd_accit_loadstf (src_t.c:1929-1952, in a
+     * file that needs netCDF) for one read of every step: row it, none from
+     * dd_max_nt on */
+    if (cfg->ndd > 0) {
+      c.src.dd_it_here = it;
+      c.src.dd_is_valid = it < cfg->dd_max_nt;
+    }
     for (int istage = 0; istage < 4; istage++) {
       w_cur = istage == 0 ? w_pre : w_tmp;
       for (int f = 0; f < 6; f++) {

@@ -313,6 +313,31 @@ def ricker_force_source(case, ijk, nt, dt, fxyz=(1.0e12, -0.6e12, 2.0e12), fc=10
     return src
 
 
+def dd_sources(case, ndd, max_nt, seed=0, vi=True, mij=True, dup=2, scale=1.0e3,
+               nt_cap=None, surface=True):
+    """dd sources at ndd random physical points (the last `dup` repeat
+    earlier ones, as overlapping input points do) with random value tables
+    vi [max_nt][4][ndd][3] and mij [max_nt][4][ndd][6]; with surface, one
+    point sits on the top row."""
+    rng = np.random.default_rng(seed + 101)
+    ni, nj, nk = case["ni"], case["nj"], case["nk"]
+    nx, ny, _ = dims(ni, nj, nk)
+    n0 = ndd - dup
+    i = rng.integers(3, 3 + ni, n0)
+    j = rng.integers(3, 3 + nj, n0)
+    k = rng.integers(3, 3 + nk, n0)
+    if surface and n0 > 0:
+        k[0] = 3 + nk - 1
+    idx = i + j * nx + k * nx * ny
+    idx = np.concatenate([idx, idx[rng.integers(0, n0, dup)]]).astype(np.int64)
+    dd = dict(indx=idx, max_nt=max_nt, nt_cap=nt_cap, vi=None, mij=None)
+    if vi:
+        dd["vi"] = (scale * rng.standard_normal((max_nt, 4, ndd, 3))).astype(np.float32)
+    if mij:
+        dd["mij"] = (scale * 1.0e9 * rng.standard_normal((max_nt, 4, ndd, 6))).astype(np.float32)
+    return dd
+
+
 def surface_receivers(case, n=4, seed=0):
     """Receivers on the free surface at grid nodes (dk forced to 0 there,
     io_funcs.c:366-368) with fractional di/dj."""
@@ -385,6 +410,7 @@ def ref_cfg(case, src_arrays=None, surf_slices=None):
         keep.append(t)
         c.surf_slices = t.ctypes.data
         c.surf_nt = t.shape[0]
+    pyoracle.set_dd_fields(c, case.get("dd"), keep)
     return c, keep
 
 
@@ -425,6 +451,12 @@ def make_solver(case, dt, nt_total, variant=4):
         else:
             s.set_moment_sources(src["ijk"], src["it_begin"], src["it_end"], src["max_nt"],
                                  src["mij"], SRC_POINT)
+    dd = case.get("dd")
+    if dd is not None and len(dd["indx"]) > 0:
+        s.set_dd_sources(dd["indx"], dd["max_nt"], nt_cap=dd.get("nt_cap"),
+                         vi=dd.get("vi") is not None, mij=dd.get("mij") is not None)
+        if dd.get("nt_cap") is None:
+            s.put_dd_values(0, dd.get("vi"), dd.get("mij"))
     return s
 
 

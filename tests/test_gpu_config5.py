@@ -8,9 +8,12 @@ from, against the oracle: 16 steps of two sources, seismograms bit-identical.
 parameters -> MLP 512 x 4 -> 1-D transposed-convolution decoder to 3
 components x 64 receivers x 1024 samples, moment-tensor output) on the
 library's BF16 MFMA kernels against the CPU oracle (oracle/surrogate_oracle.py,
-bf16 storage emulated at the device's rounding points); tolerance rel L2 <=
-2e-3 as for the other network checks (an accumulation-order difference can
-flip an intermediate bf16 rounding)."""
+bf16 storage emulated at the device's rounding points).  Tolerance: the
+forward output rel L2 <= 2e-3 as for the other network checks; the
+gradients rel L2 <= 2^-8, the unit roundoff of bf16 (an accumulation-order
+difference can flip an intermediate bf16 rounding, which moves that value by
+at most one bf16 step, and the backward pass through six layers carries such
+flips into every gradient; a wrong term moves them by O(1))."""
 import numpy as np
 import pytest
 
@@ -78,4 +81,4 @@ def test_configuration_network_forward_backward_match_oracle():
         ew = cs.rel_l2(net.gw[li].float().cpu().numpy(), grads[li][0])
         eb = cs.rel_l2(net.gb[li].float().cpu().numpy(), grads[li][1])
         print(f"layer {li} {layers[li]}: dW rel L2 {ew:.1e}, db rel L2 {eb:.1e}", flush=True)
-        assert ew <= 2e-3 and eb <= 2e-3
+        assert ew <= 2.0 ** -8 and eb <= 2.0 ** -8

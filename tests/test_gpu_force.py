@@ -55,8 +55,9 @@ def test_force_onestage_matches_oracle(medium, gauss, strict, depth):
         s.close()
 
 
-@pytest.mark.parametrize("medium,variant", [("iso", 12), ("iso", 4), ("vti", 12),
-                                            ("visco", 4)])
+@pytest.mark.parametrize("medium,variant", [("iso", 12), ("iso", 4), ("iso", 6), ("iso", 7),
+                                            ("iso", 16), ("iso", 17), ("vti", 12), ("vti", 7),
+                                            ("aniso", 6), ("visco", 4)])
 @pytest.mark.parametrize("gauss", [False, True])
 def test_strict_surface_force_run_matches_oracle(medium, variant, gauss):
     """Whole RK4 steps with strict surface forces (on the surface, and for
