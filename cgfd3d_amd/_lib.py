@@ -44,6 +44,8 @@ SIGNATURES = [
     ("cgfdh_put_dd_values", C.c_int, [_P, C.c_int, C.c_int, _P, _P]),
     ("cgfdh_set_receivers", C.c_int, [_P, C.c_int, _P, _P]),
     ("cgfdh_check_nan", C.c_int, [_P, _P, _P]),
+    ("cgfdh_locate_points", C.c_int, [C.c_int] * 3 + [_P, C.c_int, _P, _P, _P, _P]),
+    ("cgfdh_depth_to_axis", C.c_int, [C.c_int] * 3 + [_P, C.c_int, _P, _P, _P]),
     ("cgfdh_set_wavefield", C.c_int, [_P, _P]),
     ("cgfdh_get_wavefield", C.c_int, [_P, _P]),
     ("cgfdh_pml_aux_len", C.c_size_t, [_P]),

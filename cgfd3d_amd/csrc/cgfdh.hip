@@ -141,6 +141,7 @@ __global__ void pg_kernel(const float* __restrict__ w1, const float* __restrict_
 }
 
 #include "setup_kernels.h"
+#include "locate_kernels.h"
 
 // ---------------------------------------------------------------------------
 // host side
@@ -1715,6 +1716,199 @@ done:
   (void)hipFree(dkey);
   (void)hipFree(dijk);
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// point location (gd_curv_coord_to_local_indx, gd_curv_depth_to_axis)
+// ---------------------------------------------------------------------------
+
+namespace {
+// a device buffer freed when it goes out of scope
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  template <class T> T* as() { return (T*)p; }
+};
+
+// the tile ranges of gd_curv_set_minmax (gd_t.c:1808-1850): the physical
+// points of an axis split evenly, the first (n % 4) tiles one point longer
+void loc_tile_ranges(int n, int* s, int* e)
+{
+  const int avg = n / LOC_TILES, left = n % LOC_TILES;
+  for (int t = 0; t < LOC_TILES; t++) {
+    s[t] = t == 0 ? 3 : e[t - 1] + 1;
+    e[t] = s[t] + avg - 1 + (t < left ? 1 : 0);
+  }
+}
+
+// coordinates on the device and the 64 tile boxes on the host
+int loc_setup(const char* who, int ni, int nj, int nk, const float* xyz, int npts,
+              const float* pts, DevBuf& dxyz, DevBuf& dpts, LocGrid& G, std::vector<float>& box)
+{
+  if (!xyz || (npts > 0 && !pts)) return fail(CGFDH_EINVAL, "%s: null argument", who);
+  if (ni < 1 || nj < 1 || nk < 1 || npts < 0)
+    return fail(CGFDH_EINVAL, "%s: grid %dx%dx%d, %d points", who, ni, nj, nk, npts);
+  const long nx = ni + 6, ny = nj + 6, nz = nk + 6, vol = nx * ny * nz;
+  HIPCHK(hipMalloc(&dxyz.p, 3 * (size_t)vol * sizeof(float)));
+  HIPCHK(hipMemcpy(dxyz.p, xyz, 3 * (size_t)vol * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&dpts.p, 3 * (size_t)std::max(npts, 1) * sizeof(float)));
+  if (npts > 0)
+    HIPCHK(hipMemcpy(dpts.p, pts, 3 * (size_t)npts * sizeof(float), hipMemcpyHostToDevice));
+  G.x = dxyz.as<float>(); G.y = G.x + vol; G.z = G.x + 2 * vol;
+  G.nx = (int)nx; G.ny = (int)ny; G.ss = nx * ny;
+  loc_tile_ranges(ni, G.ts[0], G.te[0]);
+  loc_tile_ranges(nj, G.ts[1], G.te[1]);
+  loc_tile_ranges(nk, G.ts[2], G.te[2]);
+  constexpr int NT = LOC_TILES * LOC_TILES * LOC_TILES;
+  DevBuf dbox;
+  HIPCHK(hipMalloc(&dbox.p, NT * 6 * sizeof(float)));
+  hipLaunchKernelGGL(tile_box_kernel, dim3(NT), dim3(256), 0, 0, G, dbox.as<float>());
+  HIPCHK(hipGetLastError());
+  box.resize(NT * 6);
+  HIPCHK(hipMemcpy(box.data(), dbox.p, NT * 6 * sizeof(float), hipMemcpyDeviceToHost));
+  return CGFDH_OK;
+}
+
+// one search kernel over (point, tile) pairs, in launches of at most 65535
+template <class K>
+int loc_search(K kernel, const LocGrid& G, const float* dpts, const std::vector<int>& pairs,
+               long max_cells, unsigned long long* dkey)
+{
+  const long npair = (long)pairs.size() / 2;
+  if (npair == 0) return CGFDH_OK;
+  DevBuf dpairs;
+  HIPCHK(hipMalloc(&dpairs.p, pairs.size() * sizeof(int)));
+  HIPCHK(hipMemcpy(dpairs.p, pairs.data(), pairs.size() * sizeof(int), hipMemcpyHostToDevice));
+  const unsigned bx = (unsigned)std::min<long>(std::max<long>((max_cells + 255) / 256, 1), 64);
+  for (long p0 = 0; p0 < npair; p0 += 65535) {
+    const unsigned ny = (unsigned)std::min<long>(65535, npair - p0);
+    hipLaunchKernelGGL(kernel, dim3(bx, ny), dim3(256), 0, 0, G, dpts,
+                       dpairs.as<int>() + 2 * p0, dkey);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipDeviceSynchronize());
+  return CGFDH_OK;
+}
+}  // namespace
+
+extern "C" int cgfdh_locate_points(int ni, int nj, int nk, const float* xyz, int npts,
+                                   const float* pts, int* is_here, int* ijk, float* inc)
+{
+  const char* who = "cgfdh_locate_points";
+  if (npts > 0 && (!is_here || !ijk || !inc)) return fail(CGFDH_EINVAL, "%s: null argument", who);
+  DevBuf dxyz, dpts, dkey, dcell;
+  LocGrid G;
+  std::vector<float> box;
+  int rc = loc_setup(who, ni, nj, nk, xyz, npts, pts, dxyz, dpts, G, box);
+  if (rc || npts == 0) return rc;
+  // candidate tiles: the tile's box holds the point (gd_t.c:2050-2078)
+  std::vector<int> pairs;
+  long max_cells = 0;
+  for (int t = 0; t < LOC_TILES * LOC_TILES * LOC_TILES; t++) {
+    const int ti = t % LOC_TILES, tj = t / LOC_TILES % LOC_TILES, tk = t / (LOC_TILES * LOC_TILES);
+    const long n = (long)(G.te[0][ti] - G.ts[0][ti] + 1) * (G.te[1][tj] - G.ts[1][tj] + 1) *
+                   (G.te[2][tk] - G.ts[2][tk] + 1);
+    max_cells = std::max(max_cells, n);
+  }
+  for (int q = 0; q < npts; q++) {
+    const float* p = pts + 3 * q;
+    for (int t = 0; t < LOC_TILES * LOC_TILES * LOC_TILES; t++) {
+      const float* b = box.data() + 6 * t;
+      if (p[0] < b[0] || p[0] > b[1] || p[1] < b[2] || p[1] > b[3] || p[2] < b[4] || p[2] > b[5])
+        continue;
+      pairs.push_back(q);
+      pairs.push_back(t);
+    }
+  }
+  std::vector<unsigned long long> key(npts, ~0ull);
+  HIPCHK(hipMalloc(&dkey.p, npts * sizeof(unsigned long long)));
+  HIPCHK(hipMemcpy(dkey.p, key.data(), npts * sizeof(unsigned long long), hipMemcpyHostToDevice));
+  rc = loc_search(cell_search_kernel, G, dpts.as<float>(), pairs, max_cells,
+                  dkey.as<unsigned long long>());
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(key.data(), dkey.p, npts * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  // the holding cells, then the nearest sample in each
+  std::vector<int> cell(3 * (size_t)npts, -1);
+  for (int q = 0; q < npts; q++) {
+    if (key[q] == ~0ull) continue;
+    const int t = (int)(key[q] >> 40);
+    const long c = (long)(key[q] & ((1ull << 40) - 1));
+    const int ti = t % LOC_TILES, tj = t / LOC_TILES % LOC_TILES, tk = t / (LOC_TILES * LOC_TILES);
+    const int nti = G.te[0][ti] - G.ts[0][ti] + 1, ntj = G.te[1][tj] - G.ts[1][tj] + 1;
+    cell[3 * q] = G.ts[0][ti] + (int)(c % nti);
+    cell[3 * q + 1] = G.ts[1][tj] + (int)(c / nti % ntj);
+    cell[3 * q + 2] = G.ts[2][tk] + (int)(c / nti / ntj);
+  }
+  HIPCHK(hipMalloc(&dcell.p, cell.size() * sizeof(int)));
+  HIPCHK(hipMemcpy(dcell.p, cell.data(), cell.size() * sizeof(int), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(sample_shift_kernel, dim3(npts), dim3(256), 0, 0, G, dpts.as<float>(),
+                     dcell.as<int>(), dkey.as<unsigned long long>());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(key.data(), dkey.p, npts * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  constexpr int NS = LOC_SAMPLES + 1;
+  for (int q = 0; q < npts; q++) {
+    is_here[q] = cell[3 * q] >= 0;
+    for (int d = 0; d < 3; d++) {
+      ijk[3 * q + d] = cell[3 * q + d];
+      inc[3 * q + d] = 0.0f;
+    }
+    if (!is_here[q]) continue;
+    const int s = (int)(key[q] & 0xffffffffull);
+    const int n[3] = {s % NS, s / NS % NS, s / (NS * NS)};
+    for (int d = 0; d < 3; d++) {
+      float f = (float)n[d] / (float)LOC_SAMPLES;
+      // past one half the nearest point is the next one (gd_t.c:2146-2157)
+      if (f > 0.5) {
+        ijk[3 * q + d] += 1;
+        f = (float)((double)f - 1.0);
+      }
+      inc[3 * q + d] = f;
+    }
+  }
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_depth_to_axis(int ni, int nj, int nk, const float* xyz, int npts,
+                                   const float* xyd, int* found, float* z)
+{
+  const char* who = "cgfdh_depth_to_axis";
+  if (npts > 0 && (!found || !z)) return fail(CGFDH_EINVAL, "%s: null argument", who);
+  DevBuf dxyz, dpts, dkey, dz, dfound;
+  LocGrid G;
+  std::vector<float> box;
+  int rc = loc_setup(who, ni, nj, nk, xyz, npts, xyd, dxyz, dpts, G, box);
+  if (rc || npts == 0) return rc;
+  // candidate tiles of the top tile layer by their x-y box (gd_t.c:2205-2216)
+  std::vector<int> pairs;
+  long max_cells = 0;
+  const int tk = LOC_TILES - 1;
+  for (int t = 0; t < LOC_TILES * LOC_TILES; t++)
+    max_cells = std::max<long>(max_cells, (long)(G.te[0][t % LOC_TILES] - G.ts[0][t % LOC_TILES] + 1) *
+                                              (G.te[1][t / LOC_TILES] - G.ts[1][t / LOC_TILES] + 1));
+  for (int q = 0; q < npts; q++) {
+    const float* p = xyd + 3 * q;
+    for (int t = 0; t < LOC_TILES * LOC_TILES; t++) {
+      const float* b = box.data() + 6 * (tk * LOC_TILES * LOC_TILES + t);
+      if (p[0] < b[0] || p[0] > b[1] || p[1] < b[2] || p[1] > b[3]) continue;
+      pairs.push_back(q);
+      pairs.push_back(t);
+    }
+  }
+  std::vector<unsigned long long> key(npts, ~0ull);
+  HIPCHK(hipMalloc(&dkey.p, npts * sizeof(unsigned long long)));
+  HIPCHK(hipMemcpy(dkey.p, key.data(), npts * sizeof(unsigned long long), hipMemcpyHostToDevice));
+  rc = loc_search(surface_search_kernel, G, dpts.as<float>(), pairs, max_cells,
+                  dkey.as<unsigned long long>());
+  if (rc) return rc;
+  HIPCHK(hipMalloc(&dz.p, npts * sizeof(float)));
+  HIPCHK(hipMalloc(&dfound.p, npts * sizeof(int)));
+  hipLaunchKernelGGL(surface_finish_kernel, dim3((npts + 255) / 256), dim3(256), 0, 0, G,
+                     dpts.as<float>(), npts, dkey.as<unsigned long long>(), dz.as<float>(),
+                     dfound.as<int>());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(z, dz.p, npts * sizeof(float), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(found, dfound.p, npts * sizeof(int), hipMemcpyDeviceToHost));
+  return CGFDH_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -42,6 +42,36 @@ def estimate_dt(xyz, medium_type, vols, cfl, nmaxwell=0):
     return float(out[0]), float(out[1]), float(out[2]), tuple(int(v) for v in ijk)
 
 
+def locate_points(xyz, pts):
+    """Point location on the GPU (cgfdh_locate_points): xyz (3, nz, ny, nx)
+    coordinates with ghosts, pts (n, 3).  Returns (is_here[n], ijk[n, 3],
+    inc[n, 3])."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float32)
+    pts = np.ascontiguousarray(pts, dtype=np.float32).reshape(-1, 3)
+    nz, ny, nx = xyz.shape[-3:]
+    n = len(pts)
+    here = np.zeros(n, np.int32)
+    ijk = np.zeros((n, 3), np.int32)
+    inc = np.zeros((n, 3), np.float32)
+    check(lib().cgfdh_locate_points(nx - 6, ny - 6, nz - 6, _p(xyz), n, _p(pts), _p(here),
+                                    _p(ijk), _p(inc)), "locate_points")
+    return here, ijk, inc
+
+
+def depth_to_axis(xyz, xyd):
+    """Depth below the top surface to z on the GPU (cgfdh_depth_to_axis):
+    xyd (n, 3) = x, y, depth.  Returns (found[n], z[n])."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float32)
+    xyd = np.ascontiguousarray(xyd, dtype=np.float32).reshape(-1, 3)
+    nz, ny, nx = xyz.shape[-3:]
+    n = len(xyd)
+    found = np.zeros(n, np.int32)
+    z = np.zeros(n, np.float32)
+    check(lib().cgfdh_depth_to_axis(nx - 6, ny - 6, nz - 6, _p(xyz), n, _p(xyd), _p(found),
+                                    _p(z)), "depth_to_axis")
+    return found, z
+
+
 class Solver:
     """Engine for one grid block: ni x nj x nk physical points."""
 

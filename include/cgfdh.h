@@ -273,6 +273,29 @@ int cgfdh_get_subvolume_noalloc(cgfdh_solver *s, const int *box, int ncomp, cons
                                 float *out);
 int cgfdh_get_strain_subvolume(cgfdh_solver *s, const int *box, float *out);
 
+/* Point location on a curvilinear grid, computed on the GPU, for npts
+ * points at once: gd_curv_coord_to_local_indx (gd_t.c:1998-2173) with its
+ * cell boxes and 4x4x4 tiles (gd_curv_set_minmax, gd_t.c:1708-1880), the
+ * point-in-cell test isPointInHexahedron (media_geometry3d.cpp:198-270) and
+ * the 101^3-sample shift of gd_curv_coord2shift_sample (gd_t.c:2304-2378).
+ * xyz = x3d, y3d, z3d with ghosts as gd_t holds them; pts[n][3] the x, y, z
+ * of each point.  Per point: is_here (1 when a physical cell holds it, the
+ * first in the reference's tile and cell order), ijk (index with ghosts, the
+ * cell's corner or the next point where the shift passed one half) and inc
+ * (the shift, in (-0.5, 0.5]); ijk = -1 and inc = 0 when not here.  The
+ * global index of a one-block grid is ijk - 3 (gd_curv_coord_to_glob_indx). */
+int cgfdh_locate_points(int ni, int nj, int nk, const float *xyz, int npts, const float *pts,
+                        int *is_here, int *ijk, float *inc);
+
+/* gd_curv_depth_to_axis (gd_t.c:2179-2296) for npts points at once: xyd[n]
+ * = x, y, depth below the top surface; z[n] = the surface's z at (x, y) from
+ * the two triangles of its surface cell (fdlib_math_Point2InWhichTri,
+ * fdlib_math_computeZfromTri) minus the depth, found[n] = 1; where no
+ * surface cell lies over (x, y), found = 0 and z keeps the depth, as the
+ * reference leaves it. */
+int cgfdh_depth_to_axis(int ni, int nj, int nk, const float *xyz, int npts, const float *xyd,
+                        int *found, float *z);
+
 /* Stable time step of a curvilinear grid, blk_dt_esti_curv
  * (blk_t.c:2840-2930), computed on the GPU: the minimum over physical points
  * of cfl / Vp * L, L the smallest distance from the point to the 8 planes

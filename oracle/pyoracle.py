@@ -90,7 +90,54 @@ def load_oracle(fp64=False):
     lib.orc_ricker_deriv.restype = C.c_float
     lib.orc_ricker_deriv.argtypes = [C.c_float, C.c_float, C.c_float]
     lib.orc_surface_force_layer.argtypes = [_P, C.c_int, C.c_int, _P, _P]
+    lib.orc_point_in_hexa.restype = C.c_int
+    lib.orc_point_in_hexa.argtypes = [C.c_float] * 3 + [_P] * 3
+    lib.orc_sample_shift.argtypes = [C.c_float] * 3 + [_P] * 3 + [C.c_int] * 3 + [_P]
+    lib.orc_locate_point.restype = C.c_int
+    lib.orc_locate_point.argtypes = [C.c_int] * 3 + [_P] * 3 + [C.c_float] * 3 + [_P, _P, _P]
+    lib.orc_depth_to_axis.restype = C.c_int
+    lib.orc_depth_to_axis.argtypes = [C.c_int] * 3 + [_P] * 3 + [C.c_float] * 2 + [_P]
     return lib
+
+
+def locate_points(coords, ni, nj, nk, pts, cells=False):
+    """orc_locate_point for each point: (found[n], ijk[n,3], inc[n,3]) and,
+    with cells, the holding cells [n,3] too."""
+    lib = load_oracle()
+    co = np.ascontiguousarray(coords, np.float32).reshape(3, -1)
+    pts = np.asarray(pts, np.float32).reshape(-1, 3)
+    n = len(pts)
+    found = np.zeros(n, np.int32)
+    ijk = np.full((n, 3), -1, np.int32)
+    cell = np.full((n, 3), -1, np.int32)
+    inc = np.zeros((n, 3), np.float32)
+    xs, ys, zs = (np.ascontiguousarray(co[d]) for d in range(3))
+    for q in range(n):
+        a = np.zeros(3, np.int32)
+        b = np.zeros(3, np.float32)
+        c = np.zeros(3, np.int32)
+        found[q] = lib.orc_locate_point(ni, nj, nk, ptr(xs), ptr(ys), ptr(zs), float(pts[q, 0]),
+                                        float(pts[q, 1]), float(pts[q, 2]), ptr(a), ptr(b),
+                                        ptr(c))
+        if found[q]:
+            ijk[q], inc[q], cell[q] = a, b, c
+    return (found, ijk, inc, cell) if cells else (found, ijk, inc)
+
+
+def depth_to_axis(coords, ni, nj, nk, xyd):
+    """orc_depth_to_axis for each (x, y, depth): (found[n], z[n])."""
+    lib = load_oracle()
+    co = np.ascontiguousarray(coords, np.float32).reshape(3, -1)
+    xyd = np.asarray(xyd, np.float32).reshape(-1, 3)
+    xs, ys, zs = (np.ascontiguousarray(co[d]) for d in range(3))
+    found = np.zeros(len(xyd), np.int32)
+    z = np.empty(len(xyd), np.float32)
+    for q in range(len(xyd)):
+        v = np.array([xyd[q, 2]], np.float32)
+        found[q] = lib.orc_depth_to_axis(ni, nj, nk, ptr(xs), ptr(ys), ptr(zs), float(xyd[q, 0]),
+                                         float(xyd[q, 1]), ptr(v))
+        z[q] = v[0]
+    return found, z
 
 
 def ref_available():
