@@ -120,7 +120,7 @@ def cpu_baseline(wl, budget_s=15.0):
     src.update(nsrc=1, ijk=[3 + ni // 2, 3 + nj // 2, 3 + nk - 1 - 20], inc=[0, 0, 0], gauss=0)
     case["src"] = src
     case.update(pml_vel=wl.pml_vel, pml_alpha=wl.pml_alpha, pml_beta=wl.pml_beta)
-    if po.ref_available() and wl.wl is None:
+    if po.ref_available():
         try:
             return reference_baseline(case, wl, budget_s)
         except Exception as e:      # the port below still gives a baseline
@@ -236,27 +236,111 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_outputs(args, wl, s):
+    """The workload with output every step: K z slices (all 9 components)
+    and the free surface's velocities, packed on the device and copied to
+    host buffers; async overlaps packing and copies with the following
+    steps (two sets of pinned buffers in turn), sync waits for each step's
+    output before the next step.  Wall clock over the timed steps, with the
+    host reading every output (a checksum standing in for the file write)."""
+    import numpy as np
+    import torch
+    from cgfd3d_amd import PinnedBuffer
+    nx, ny = wl.ni + 6, wl.nj + 6
+    ks = np.linspace(3, 3 + wl.nk - 1, args.outputs + 2).astype(int)[1:-1]
+    boxes = [((0, nx, 1, 0, ny, 1, int(k), 1, 1), list(range(9))) for k in ks]
+    boxes.append(((3, wl.ni, 1, 3, wl.nj, 1, 3 + wl.nk - 1, 1, 1), [0, 1, 2]))
+    shapes = [(len(c), b[7], b[4], b[1]) for b, c in boxes]
+    nbytes = sum(4 * int(np.prod(sh)) for sh in shapes)
+    s.run(0, args.warmup)
+    torch.cuda.synchronize()
+    check = 0.0
+    it = args.warmup
+    if args.output_mode == "async":
+        sets = [[PinnedBuffer(sh) for sh in shapes] for _ in range(2)]
+        pend = [None, None]
+        t0 = time.perf_counter()
+        for n in range(args.steps):
+            q = n % 2
+            if pend[q] is not None:          # this set's copies, two steps back
+                for t in pend[q]:
+                    s.wait(t)
+                check += sum(float(b.array[0].ravel()[0]) for b in sets[q])
+            s.run_async(it + n, 1)
+            pend[q] = [s.subvolume_async(b, c, sets[q][m].array)
+                       for m, (b, c) in enumerate(boxes)]
+        for q in range(2):
+            if pend[q] is not None:
+                for t in pend[q]:
+                    s.wait(t)
+                check += sum(float(b.array[0].ravel()[0]) for b in sets[q])
+        s.sync()
+        wall = time.perf_counter() - t0
+        for st in sets:
+            for b in st:
+                b.close()
+    else:
+        t0 = time.perf_counter()
+        for n in range(args.steps):
+            s.run(it + n, 1)
+            for b, c in boxes:
+                check += float(s.subvolume(b, c)[0].ravel()[0])
+        wall = time.perf_counter() - t0
+    s.close()
+    if not np.isfinite(check):
+        raise RuntimeError("non-finite output")
+    N = wl.npoints
+    bstep = wl.algorithmic_bytes_per_step()
+    out = {
+        "metric": "grid-point RK4 updates/s", "value": N * args.steps / wall,
+        "unit": "grid-point RK4 updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{wl.name} with output every step: {args.outputs} z slices "
+                               f"of 9 components + surface velocities "
+                               f"({nbytes / 1e6:.1f} MB per step), {args.output_mode}",
+                   "grid": [wl.ni, wl.nj, wl.nk], "dt": wl.dt, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(bstep / (wall / args.steps) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(bstep / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "all kernels of a step, wall clock with output"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--variant", type=int, default=None,
-                    help="stage kernels (default: the engine's choice for the medium, 12 "
-                         "elastic, 4 visco-elastic): 0 per-point, 1 tiled, 2/3 z-marching interior + "
+                    help="stage kernels (default: the engine's choice for the medium, 16 "
+                         "elastic, 4 visco-elastic): 0 per-point, 1 "
+                         "tiled, 2/3 z-marching interior + "
                          "shell, 4/5 per-point interior (64x4 / 64x2) + shell, 6/7 "
                          "x-vectorised interior (2 / 4 points per thread) + shell, 8/9 "
                          "LDS-tiled interior (64x8 / 64x16 plane tiles) + shell, 10/11 "
                          "per-point interior marching 2 / 4 planes per block + shell, 12 "
                          "two-phase per-point interior + shell, 13 the same with the RK "
                          "combine loads last, 14 LDS-tiled interior with phased loads + shell, "
-                         "15 variant 12 built for 3 waves per SIMD")
+                         "15 variant 12 built for 3 waves per SIMD, 16 whole-row two-phase "
+                         "kernel with the PML faces folded in, 17 variant 12 with a two-phase "
+                         "shell")
     ap.add_argument("--workload", default="iso", choices=["iso", "vti", "aniso", "visco", "batch"])
     ap.add_argument("--size", type=str, default=None,
                     help="grid NIxNJxNK (default 600x600x300 iso, 400x400x200 media)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device copies of MIB MiB (PMC counter calibration)")
+    ap.add_argument("--outputs", type=int, default=0, metavar="K",
+                    help="after every timed step also write out K z slices of all 9 "
+                         "components and the velocities of the free surface (the "
+                         "reference's slice and snapshot output), timed on the wall clock")
+    ap.add_argument("--output-mode", default="async", choices=["async", "sync"],
+                    help="with --outputs: packed and copied while the next steps run "
+                         "(async, pinned buffers) or between steps (sync)")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -292,6 +376,8 @@ def main():
     t = time.perf_counter()
     s = wl.build_solver(variant=args.variant)
     log(f"engine set up in {time.perf_counter() - t:.1f} s")
+    if args.outputs:
+        return bench_outputs(args, wl, s)
     s.run(0, args.warmup)
     torch.cuda.synchronize()
     log(f"warmup {args.warmup} steps done")

@@ -3,5 +3,6 @@ collocated-grid elastic wave solver (the drv_rk_curv_col / sv_curv_col_el_{iso,
 vti,aniso} path of CGFD3D), behind the C ABI in include/cgfdh.h."""
 from ._lib import CgfdhError, LIB_PATH  # noqa: F401
 from .solver import (MEDIUM_ELASTIC_ANISO, MEDIUM_ELASTIC_ISO, MEDIUM_ELASTIC_VTI,  # noqa: F401
-                     MEDIUM_VISCO_ISO, SRC_GAUSSIAN, SRC_POINT, Batch, Solver, depth_to_axis,
-                     estimate_dt, locate_points)
+                     MEDIUM_VISCO_ISO, SRC_GAUSSIAN, SRC_POINT, Batch, PinnedBuffer, Solver,
+                     depth_to_axis,
+                     estimate_dt, gen_grid_layer, layer2model, locate_points)

@@ -46,6 +46,10 @@ SIGNATURES = [
     ("cgfdh_check_nan", C.c_int, [_P, _P, _P]),
     ("cgfdh_locate_points", C.c_int, [C.c_int] * 3 + [_P, C.c_int, _P, _P, _P, _P]),
     ("cgfdh_depth_to_axis", C.c_int, [C.c_int] * 3 + [_P, C.c_int, _P, _P, _P]),
+    ("cgfdh_layer2model", C.c_int,
+     [C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, C.c_long, _P, _P, _P, _P, _P, _P]),
+    ("cgfdh_gen_grid_layer", C.c_int,
+     [C.c_int, _P, _P, C.c_int, C.c_int, _P, _P, _P] + [C.c_int] * 9 + [_P]),
     ("cgfdh_set_wavefield", C.c_int, [_P, _P]),
     ("cgfdh_get_wavefield", C.c_int, [_P, _P]),
     ("cgfdh_pml_aux_len", C.c_size_t, [_P]),
@@ -59,6 +63,12 @@ SIGNATURES = [
     ("cgfdh_set_ablexp", C.c_int, [_P, _P, _P, _P, _P]),
     ("cgfdh_set_graves_qs", C.c_int, [_P, _P, C.c_float]),
     ("cgfdh_get_strain_subvolume", C.c_int, [_P, _P, _P]),
+    ("cgfdh_run_async", C.c_int, [_P, C.c_int, C.c_int]),
+    ("cgfdh_subvolume_async", C.c_int, [_P, _P, C.c_int, _P, _P, _P]),
+    ("cgfdh_wait", C.c_int, [_P, C.c_int]),
+    ("cgfdh_sync", C.c_int, [_P]),
+    ("cgfdh_host_alloc", C.c_int, [C.c_size_t, _P]),
+    ("cgfdh_host_free", C.c_int, [_P]),
     ("cgfdh_estimate_dt", C.c_int, [C.c_int, C.c_int, C.c_int, _P, C.c_int, _P, C.c_int,
                                     C.c_float, _P, _P, _P, _P]),
     ("cgfdh_onestage", C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),

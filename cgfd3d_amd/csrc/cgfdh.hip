@@ -142,6 +142,8 @@ __global__ void pg_kernel(const float* __restrict__ w1, const float* __restrict_
 
 #include "setup_kernels.h"
 #include "locate_kernels.h"
+#include "grid_kernels.h"
+#include "media_kernels.h"
 
 // ---------------------------------------------------------------------------
 // host side
@@ -226,6 +228,18 @@ struct cgfdh_solver {
   float* outbuf = nullptr;       // staging for packed output sub-volumes (grown on demand)
   float* scratch = nullptr;      // device address of the static sub-volume buffer
   unsigned long long* nan_key = nullptr;   // result word of the NaN scan
+  // asynchronous output: a copy stream and device staging slots, each
+  // packed on the solver's stream and copied out on the copy stream
+  hipStream_t copy_stream = nullptr;
+  struct OutSlot {
+    float* d = nullptr;
+    size_t n = 0;
+    hipEvent_t packed = nullptr, done = nullptr;
+    unsigned long gen = 0;
+  };
+  std::vector<OutSlot> oslots;
+  std::vector<std::pair<int, unsigned long>> tickets;   // (slot, generation)
+  unsigned long out_gen = 0;
   float* qs = nullptr;           // Graves-Qs quality factor per point (NULL: off)
   float qs_coef = 0.0f;          // -pi * freq * dt
   float* abl = nullptr;          // ablexp masks Ex[nx], Ey[ny], Ez[nz]
@@ -344,12 +358,10 @@ extern "C" int cgfdh_create(const cgfdh_desc* d, cgfdh_solver** out)
     for (auto& a : F.aux) bad |= alloc(&a, 9 * F.fn);
   }
   // default stage kernels: the whole-row two-phase kernel with the PML faces
-  // folded in for the isotropic medium, the two-phase interior kernel for
-  // VTI and anisotropic media, the per-point kernel for the visco-elastic one
-  s->variant = d->medium_type == CGFDH_MEDIUM_VISCO_ISO   ? 4
-               : d->medium_type == CGFDH_MEDIUM_ELASTIC_ISO ? 16
-                                                          : 12;
-if (!bad && hipStreamCreate(&s->stream) != hipSuccess) bad = 1;
+  // folded in for the elastic media, the per-point kernel for the
+  // visco-elastic one
+  s->variant = d->medium_type == CGFDH_MEDIUM_VISCO_ISO ? 4 : 16;
+  if (!bad && hipStreamCreate(&s->stream) != hipSuccess) bad = 1;
   if (!bad && (hipStreamCreate(&s->stream2) != hipSuccess ||
                hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
                hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming) != hipSuccess))
@@ -374,6 +386,15 @@ extern "C" int cgfdh_destroy(cgfdh_solver* s)
 {
   if (!s) return CGFDH_OK;
   (void)hipStreamSynchronize(s->stream);
+  if (s->copy_stream) {
+    (void)hipStreamSynchronize(s->copy_stream);
+    for (auto& o : s->oslots) {
+      if (o.d) (void)hipFree(o.d);
+      if (o.packed) (void)hipEventDestroy(o.packed);
+      if (o.done) (void)hipEventDestroy(o.done);
+    }
+    (void)hipStreamDestroy(s->copy_stream);
+  }
   free_all(s);
   for (auto e : s->ev_stage) (void)hipEventDestroy(e);
   (void)hipEventDestroy(s->ev_run[0]);
@@ -1434,6 +1455,136 @@ extern "C" int cgfdh_get_subvolume(cgfdh_solver* s, const int* box, int ncomp, c
 }
 
 // ---------------------------------------------------------------------------
+// asynchronous steps and output: the steps and the packing run in order on
+// the solver's stream, each packed box is copied to the host on a second
+// stream, so the copy (and the caller's use of earlier output) overlaps the
+// steps enqueued after it
+// ---------------------------------------------------------------------------
+
+extern "C" int cgfdh_run_async(cgfdh_solver* s, int it0, int nsteps)
+{
+  if (!s) return fail(CGFDH_EINVAL, "cgfdh_run_async: null solver");
+  int rc = ready(s, "cgfdh_run_async");
+  if (rc) return rc;
+  if (it0 < 0 || nsteps < 0 || it0 + nsteps > s->d.nt_total)
+    return fail(CGFDH_EINVAL, "cgfdh_run_async: steps %d..%d outside nt_total %d", it0,
+                it0 + nsteps, s->d.nt_total);
+  rc = dd_check(s, it0, nsteps, "cgfdh_run_async");
+  if (rc) return rc;
+  for (int n = 0; n < nsteps; n++) {
+    rc = enqueue_step(s, s->stream, it0 + n, nullptr, nullptr);
+    if (rc) return rc;
+  }
+  HIPCHK(hipGetLastError());
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_sync(cgfdh_solver* s)
+{
+  if (!s) return fail(CGFDH_EINVAL, "cgfdh_sync: null solver");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (s->copy_stream) HIPCHK(hipStreamSynchronize(s->copy_stream));
+  return CGFDH_OK;
+}
+
+// a staging slot of at least n floats whose last copy has finished: a free
+// one, a new one (at most kOutSlots), or the oldest after its copy
+static int out_slot(cgfdh_solver* s, size_t n, int* out)
+{
+  constexpr int kOutSlots = 16;
+  if (!s->copy_stream) HIPCHK(hipStreamCreate(&s->copy_stream));
+  int pick = -1;
+  for (int i = 0; i < (int)s->oslots.size(); i++) {
+    hipError_t q = hipEventQuery(s->oslots[i].done);
+    if (q == hipErrorNotReady) continue;
+    if (q != hipSuccess) HIPCHK(q);
+    if (pick < 0 || (s->oslots[i].n >= n && s->oslots[pick].n < n)) pick = i;
+  }
+  if (pick < 0 && (int)s->oslots.size() < kOutSlots) {
+    cgfdh_solver::OutSlot o;
+    HIPCHK(hipEventCreateWithFlags(&o.packed, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&o.done, hipEventDisableTiming));
+    s->oslots.push_back(o);
+    pick = (int)s->oslots.size() - 1;
+  }
+  if (pick < 0) {
+    pick = 0;
+    for (int i = 1; i < (int)s->oslots.size(); i++)
+      if (s->oslots[i].gen < s->oslots[pick].gen) pick = i;
+    HIPCHK(hipEventSynchronize(s->oslots[pick].done));
+  }
+  auto& o = s->oslots[pick];
+  if (o.n < n) {
+    if (o.d) HIPCHK(hipFree(o.d));
+    o.d = nullptr;
+    o.n = 0;
+    HIPCHK(hipMalloc(&o.d, n * sizeof(float)));
+    o.n = n;
+  }
+  *out = pick;
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_subvolume_async(cgfdh_solver* s, const int* box, int ncomp,
+                                     const int* comps, float* out, int* ticket)
+{
+  const char* who = "cgfdh_subvolume_async";
+  if (!s || !comps || !out || !ticket) return fail(CGFDH_EINVAL, "%s: null argument", who);
+  if (ncomp < 1 || ncomp > s->nw)
+    return fail(CGFDH_EINVAL, "%s: ncomp %d outside 1..%d", who, ncomp, s->nw);
+  CompList cl{};
+  for (int c = 0; c < ncomp; c++) {
+    if (comps[c] < 0 || comps[c] >= s->nw)
+      return fail(CGFDH_EINVAL, "%s: component %d outside 0..%d", who, comps[c], s->nw - 1);
+    cl.c[c] = comps[c];
+  }
+  SubBox b;
+  long npt;
+  int rc = check_box(s, box, who, &b, &npt);
+  int k = -1;
+  if (!rc) rc = out_slot(s, (size_t)ncomp * npt, &k);
+  if (rc) return rc;
+  auto& o = s->oslots[k];
+  hipLaunchKernelGGL(subvolume_kernel, subvolume_grid(npt, ncomp), dim3(256), 0, s->stream,
+                     s->lvl[0], s->vol, s->nx, s->ss, b, npt, cl, o.d);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(o.packed, s->stream));
+  HIPCHK(hipStreamWaitEvent(s->copy_stream, o.packed, 0));
+  HIPCHK(hipMemcpyAsync(out, o.d, (size_t)ncomp * npt * sizeof(float), hipMemcpyDeviceToHost,
+                        s->copy_stream));
+  HIPCHK(hipEventRecord(o.done, s->copy_stream));
+  o.gen = ++s->out_gen;
+  s->tickets.push_back({k, o.gen});
+  *ticket = (int)s->tickets.size() - 1;
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_wait(cgfdh_solver* s, int ticket)
+{
+  if (!s || ticket < 0 || ticket >= (int)s->tickets.size())
+    return fail(CGFDH_EINVAL, "cgfdh_wait: no ticket %d", ticket);
+  const auto t = s->tickets[ticket];
+  // a slot is reused only after its copy has finished, so a newer generation
+  // means this ticket's copy is done
+  if (s->oslots[t.first].gen == t.second) HIPCHK(hipEventSynchronize(s->oslots[t.first].done));
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_host_alloc(size_t bytes, void** p)
+{
+  if (!p) return fail(CGFDH_EINVAL, "cgfdh_host_alloc: null argument");
+  *p = nullptr;
+  HIPCHK(hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault));
+  return CGFDH_OK;
+}
+
+extern "C" int cgfdh_host_free(void* p)
+{
+  if (p) HIPCHK(hipHostFree(p));
+  return CGFDH_OK;
+}
+
+// ---------------------------------------------------------------------------
 // cgfdh_get_subvolume without any device allocation: the packed points go
 // through a fixed file-scope device array instead of the solver's staging
 // buffer.  Each GPU holds its own copy of the array from the code object's
@@ -1908,6 +2059,206 @@ extern "C" int cgfdh_depth_to_axis(int ni, int nj, int nk, const float* xyz, int
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpy(z, dz.p, npts * sizeof(float), hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(found, dfound.p, npts * sizeof(int), hipMemcpyDeviceToHost));
+  return CGFDH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// grid generation from a layer file (gd_curv_gen_layer)
+// ---------------------------------------------------------------------------
+
+namespace {
+// the spline lines of one resampling pass: bases in (input, output) pairs
+int spline_pass(const float* src, float* dst, const std::vector<long>& base, long sin, long sout,
+                int n, int ni, int f, int* dbad)
+{
+  const int nline = (int)(base.size() / 2);
+  if (nline == 0) return CGFDH_OK;
+  DevBuf dbase, dwork;
+  HIPCHK(hipMalloc(&dbase.p, base.size() * sizeof(long)));
+  HIPCHK(hipMemcpy(dbase.p, base.data(), base.size() * sizeof(long), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&dwork.p, (size_t)nline * 5 * n * sizeof(float)));
+  SplineLines S{src, dst, dbase.as<long>(), sin, sout, n, ni, f, nline, dwork.as<float>(), dbad};
+  hipLaunchKernelGGL(spline_lines_kernel, dim3((nline + 63) / 64), dim3(64), 0, 0, S);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  return CGFDH_OK;
+}
+}  // namespace
+
+extern "C" int cgfdh_gen_grid_layer(int n_if, const int* ncell_in, const int* equal, int nx_l,
+                                    int ny_l, const float* layer_xyz, const int* resample,
+                                    const int* start, int ntx, int nty, int ntz, int ni, int nj,
+                                    int nk, int gni1, int gnj1, int gnk1, float* xyz)
+{
+  const char* who = "cgfdh_gen_grid_layer";
+  if (!ncell_in || !equal || !layer_xyz || !resample || !start || !xyz)
+    return fail(CGFDH_EINVAL, "%s: null argument", who);
+  const int ng = 3, nlay = n_if - 1;
+  if (nlay < 1 || n_if > GRID_MAX_IF)
+    return fail(CGFDH_EINVAL, "%s: %d interfaces outside 2..%d", who, n_if, GRID_MAX_IF);
+  if (nx_l < 2 || ny_l < 2 || ni < 1 || nj < 1 || nk < 1)
+    return fail(CGFDH_EINVAL, "%s: %dx%d samples, block %dx%dx%d", who, nx_l, ny_l, ni, nj, nk);
+  const int fx = std::abs(resample[0]), fy = std::abs(resample[1]), fz = std::abs(resample[2]);
+  if (fx < 1 || fy < 1 || fz < 1)
+    return fail(CGFDH_EINVAL, "%s: resample factors %d %d %d", who, resample[0], resample[1],
+                resample[2]);
+  // the resampled sizes and the block's first indices (gd_t.c:1066-1112)
+  const int nxi = resample[0] < 0 ? (nx_l + 1) / fx : (nx_l - 1) * fx + 1;
+  const int nx_first = resample[0] < 0 ? (start[0] + 1) / fx : (start[0] - 1) * fx + 1;
+  const int nyi = resample[1] < 0 ? (ny_l + 1) / fy : (ny_l - 1) * fy + 1;
+  const int ny_first = resample[1] < 0 ? (start[1] + 1) / fy : (start[1] - 1) * fy + 1;
+  GridColumns C{};
+  int nzi = 1;
+  for (int k = 0; k < nlay; k++) {
+    C.ncell[k] = resample[2] < 0 ? ncell_in[k] / fz : ncell_in[k] * fz;
+    C.equal[k] = equal[k];
+    nzi += C.ncell[k];
+    // an unequal middle layer with one cell would read before the
+    // reference's spacing buffer
+    if (C.ncell[k] < 1 || (equal[k] < 1 && k > 0 && k < nlay - 1 && C.ncell[k] < 2))
+      return fail(CGFDH_EINVAL, "%s: layer %d has %d cells", who, k, C.ncell[k]);
+  }
+  if (nzi < 4) return fail(CGFDH_EINVAL, "%s: %d grid levels, 4 needed", who, nzi);
+  const int xg = nx_first + gni1 - ng, yg = ny_first + gnj1 - ng;
+  const int zg = nzi - start[2] - ntz + gnk1 - ng;
+  // the reference's parameter checks (gd_t.c:1151-1177)
+  if (nx_first - ng < 0 || ny_first - ng < 0)
+    return fail(CGFDH_EINVAL, "%s: horizontal start index %d %d leaves no room for ghosts", who,
+                start[0], start[1]);
+  if (nx_first + ntx + ng > nxi || ny_first + nty + ng > nyi)
+    return fail(CGFDH_EINVAL, "%s: %dx%d points from %d,%d exceed the %dx%d resampled layers",
+                who, ntx, nty, nx_first, ny_first, nxi, nyi);
+  if (zg < 0) return fail(CGFDH_EINVAL, "%s: vertical start index %d too deep", who, start[2]);
+  if (xg + ni + 2 * ng > nxi || yg + nj + 2 * ng > nyi || zg + nk + 2 * ng > nzi + ng)
+    return fail(CGFDH_EINVAL, "%s: the block reaches past the resampled layers", who);
+
+  const long vin = (long)nx_l * ny_l * n_if, vit = (long)nxi * nyi * n_if;
+  DevBuf din, dL, dout, dbad;
+  HIPCHK(hipMalloc(&din.p, 3 * vin * sizeof(float)));
+  HIPCHK(hipMemcpy(din.p, layer_xyz, 3 * vin * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&dL.p, 3 * vit * sizeof(float)));
+  HIPCHK(hipMalloc(&dbad.p, sizeof(int)));
+  const int zero = 0;
+  HIPCHK(hipMemcpy(dbad.p, &zero, sizeof(int), hipMemcpyHostToDevice));
+  const float* In = din.as<float>();
+  float* L = dL.as<float>();
+  int* bad = dbad.as<int>();
+  int rc = CGFDH_OK;
+  std::vector<long> base;
+  if (resample[0] < 2 && resample[1] < 2) {
+    const long n = 3 * vit;
+    hipLaunchKernelGGL(sample_layers_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0,
+                       In, L, nx_l, ny_l, n_if, nxi, nyi, fx, fy);
+    HIPCHK(hipGetLastError());
+  } else if (resample[0] < 2) {          // every fx-th in x, a spline along y
+    for (int c = 0; c < 3; c++)
+      for (int kk = 0; kk < n_if; kk++)
+        for (int ii = 0; ii < nxi; ii++) {
+          base.push_back(c * vin + (long)ii * fx + (long)kk * nx_l * ny_l);
+          base.push_back(c * vit + ii + (long)kk * nxi * nyi);
+        }
+    rc = spline_pass(In, L, base, nx_l, nxi, ny_l, nyi, fy, bad);
+  } else if (resample[1] < 2) {          // a spline along x, every fy-th in y
+    for (int c = 0; c < 3; c++)
+      for (int kk = 0; kk < n_if; kk++)
+        for (int jj = 0; jj < nyi; jj++) {
+          base.push_back(c * vin + (long)jj * fy * nx_l + (long)kk * nx_l * ny_l);
+          base.push_back(c * vit + (long)jj * nxi + (long)kk * nxi * nyi);
+        }
+    rc = spline_pass(In, L, base, 1, 1, nx_l, nxi, fx, bad);
+  } else {                               // along x on the input rows, then along y
+    for (int c = 0; c < 3; c++)
+      for (int kk = 0; kk < n_if; kk++)
+        for (int jj = 0; jj < ny_l; jj++) {
+          base.push_back(c * vin + (long)jj * nx_l + (long)kk * nx_l * ny_l);
+          base.push_back(c * vit + (long)jj * fy * nxi + (long)kk * nxi * nyi);
+        }
+    rc = spline_pass(In, L, base, 1, 1, nx_l, nxi, fx, bad);
+    base.clear();
+    for (int c = 0; c < 3 && !rc; c++)
+      for (int kk = 0; kk < n_if; kk++)
+        for (int ii = 0; ii < nxi; ii++) {
+          base.push_back(c * vit + ii + (long)kk * nxi * nyi);
+          base.push_back(c * vit + ii + (long)kk * nxi * nyi);
+        }
+    if (!rc) rc = spline_pass(L, L, base, (long)fy * nxi, nxi, ny_l, nyi, fy, bad);
+  }
+  if (rc) return rc;
+  const int nx = ni + 2 * ng, ny = nj + 2 * ng, nz = nk + 2 * ng;
+  const long vol = (long)nx * ny * nz;
+  HIPCHK(hipMalloc(&dout.p, 3 * vol * sizeof(float)));
+  C.L = L; C.nxi = nxi; C.nyi = nyi; C.nif = n_if;
+  C.xg = xg; C.yg = yg; C.zg = zg;
+  C.nx = nx; C.ny = ny; C.nz = nz;
+  C.x3d = dout.as<float>(); C.y3d = C.x3d + vol; C.z3d = C.x3d + 2 * vol;
+  C.bad = bad;
+  hipLaunchKernelGGL(grid_columns_kernel, dim3((nx * ny + 63) / 64), dim3(64), 0, 0, C);
+  HIPCHK(hipGetLastError());
+  int hbad = 0;
+  HIPCHK(hipMemcpy(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost));
+  if (hbad)
+    return fail(CGFDH_EINVAL, "%s: interface samples or heights are not strictly increasing",
+                who);
+  HIPCHK(hipMemcpy(xyz, dout.p, 3 * vol * sizeof(float), hipMemcpyDeviceToHost));
+  return CGFDH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// layered model onto the grid (media_layer2model, local values)
+// ---------------------------------------------------------------------------
+
+extern "C" int cgfdh_layer2model(int nprop, int n_if, int mx, int my, const float* geo,
+                                 const float* fields, long npts, const float* x, const float* y,
+                                 const float* z, float* a, float* b, float* c)
+{
+  const char* who = "cgfdh_layer2model";
+  if (!geo || !fields || !x || !y || !z || !a || (nprop == 3 && (!b || !c)))
+    return fail(CGFDH_EINVAL, "%s: null argument", who);
+  if (nprop != 1 && nprop != 3) return fail(CGFDH_EINVAL, "%s: nprop %d (1 or 3)", who, nprop);
+  if (n_if < 1 || n_if > MEDIA_MAX_IF || mx < 2 || my < 2 || npts < 0)
+    return fail(CGFDH_EINVAL, "%s: %d interfaces of %dx%d samples, %ld points", who, n_if, mx,
+                my, npts);
+  if (npts == 0) return CGFDH_OK;
+  const size_t nf = 1 + 3 * (size_t)nprop, vol = (size_t)n_if * mx * my;
+  DevBuf df, dx, dout, dbad;
+  HIPCHK(hipMalloc(&df.p, nf * vol * sizeof(float)));
+  HIPCHK(hipMemcpy(df.p, fields, nf * vol * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&dx.p, 3 * (size_t)npts * sizeof(float)));
+  float* d = dx.as<float>();
+  HIPCHK(hipMemcpy(d, x, npts * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d + npts, y, npts * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d + 2 * npts, z, npts * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&dout.p, (size_t)nprop * npts * sizeof(float)));
+  HIPCHK(hipMalloc(&dbad.p, sizeof(unsigned long long)));
+  const unsigned long long none = ~0ull;
+  HIPCHK(hipMemcpy(dbad.p, &none, sizeof none, hipMemcpyHostToDevice));
+  LayerModel M;
+  M.f = df.as<float>();
+  M.ni = n_if; M.mx = mx; M.my = my; M.nprop = nprop;
+  M.minx = geo[0]; M.miny = geo[1]; M.dx = geo[2]; M.dy = geo[3];
+  M.x = d; M.y = d + npts; M.z = d + 2 * npts;
+  M.n = npts;
+  M.a = dout.as<float>();
+  M.b = nprop == 3 ? M.a + npts : nullptr;
+  M.c = nprop == 3 ? M.a + 2 * npts : nullptr;
+  M.bad = dbad.as<unsigned long long>();
+  hipLaunchKernelGGL(layer2model_kernel, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, 0,
+                     M);
+  HIPCHK(hipGetLastError());
+  unsigned long long bad = 0;
+  HIPCHK(hipMemcpy(&bad, dbad.p, sizeof bad, hipMemcpyDeviceToHost));
+  if (bad != none) {
+    const long p = (long)(bad & ((1ull << 48) - 1));
+    if ((bad >> 48) == 1)
+      return fail(CGFDH_EINVAL, "%s: point %ld (%g, %g) is outside the interface mesh", who, p,
+                  (double)x[p], (double)y[p]);
+    return fail(CGFDH_EINVAL, "%s: point %ld has vp <= sqrt(2) vs", who, p);
+  }
+  HIPCHK(hipMemcpy(a, M.a, npts * sizeof(float), hipMemcpyDeviceToHost));
+  if (nprop == 3) {
+    HIPCHK(hipMemcpy(b, M.b, npts * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(c, M.c, npts * sizeof(float), hipMemcpyDeviceToHost));
+  }
   return CGFDH_OK;
 }
 

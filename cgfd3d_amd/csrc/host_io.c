@@ -656,3 +656,159 @@ int cgfdh_io_recv_output_sac(const char *output_dir, const char *evtnm, int nrec
     }
   return 0;
 }
+
+/* ---- the grid layer file of gd_curv_gen_layer (gd_t.c:1031-1060) ---------
+ * whitespace-separated text: the number of interfaces n; n-1 cell counts;
+ * n-1 equal-spacing flags; nx and ny of the samples; then x y z of every
+ * sample, x fastest, then y, then interface (bottom first). */
+static int layer_head(FILE *fp, const char *path, int *n_if, int *ncell, int *equal, int *nx,
+                      int *ny)
+{
+  if (fscanf(fp, "%d", n_if) != 1 || *n_if < 2)
+    return io_fail("grid layer file %s: bad number of interfaces", path);
+  for (int i = 0; i < *n_if - 1; i++) {
+    int v;
+    if (fscanf(fp, "%d", &v) != 1) return io_fail("grid layer file %s: cell counts", path);
+    if (ncell) ncell[i] = v;
+  }
+  for (int i = 0; i < *n_if - 1; i++) {
+    int v;
+    if (fscanf(fp, "%d", &v) != 1) return io_fail("grid layer file %s: spacing flags", path);
+    if (equal) equal[i] = v;
+  }
+  if (fscanf(fp, "%d %d", nx, ny) != 2 || *nx < 1 || *ny < 1)
+    return io_fail("grid layer file %s: sample counts", path);
+  return 0;
+}
+
+int cgfdh_io_grid_layer_dims(const char *path, int *n_interfaces, int *nx_layers,
+                             int *ny_layers)
+{
+  FILE *fp = fopen(path, "r");
+  if (!fp) return io_fail("cgfdh_io_grid_layer_dims: %s: %s", path, strerror(errno));
+  int rc = layer_head(fp, path, n_interfaces, NULL, NULL, nx_layers, ny_layers);
+  fclose(fp);
+  return rc;
+}
+
+int cgfdh_io_read_grid_layer(const char *path, int *ncell, int *equal, float *layer_xyz)
+{
+  FILE *fp = fopen(path, "r");
+  if (!fp) return io_fail("cgfdh_io_read_grid_layer: %s: %s", path, strerror(errno));
+  int n_if, nx, ny;
+  int rc = layer_head(fp, path, &n_if, ncell, equal, &nx, &ny);
+  const size_t n = (size_t)nx * ny * n_if;
+  for (size_t i = 0; i < n && !rc; i++)
+    if (fscanf(fp, "%f %f %f", &layer_xyz[i], &layer_xyz[i + n], &layer_xyz[i + 2 * n]) != 3)
+      rc = io_fail("grid layer file %s: sample %zu of %zu", path, i, n);
+  fclose(fp);
+  return rc;
+}
+
+/* ---- the layered-model file of media_layer2model (.md3lay) ---------------
+ * read_interface_file (media_read_file.cpp:22-420): lines starting with '#'
+ * or empty are skipped; then the media type, the number of interfaces, the
+ * mesh (nx ny minx miny dx dy) and per interface, per sample (x fastest) the
+ * elevation followed by value, gradient and power of each property (one
+ * component: var; elastic isotropic: rho, vp, vs). */
+typedef struct {
+  char *text;
+  char *pos;
+} md_tokens;
+
+static int md_load(const char *path, md_tokens *t)
+{
+  FILE *fp = fopen(path, "r");
+  if (!fp) return io_fail("md3lay file %s: %s", path, strerror(errno));
+  size_t cap = 1 << 16, n = 0;
+  char *buf = malloc(cap), line[8192];
+  if (!buf) { fclose(fp); return io_fail("md3lay file %s: out of memory", path); }
+  while (fgets(line, sizeof line, fp)) {
+    if (line[0] == '#' || line[0] == '\n') continue;
+    size_t l = strlen(line);
+    if (n + l + 2 > cap) {
+      while (n + l + 2 > cap) cap *= 2;
+      char *nb = realloc(buf, cap);
+      if (!nb) { free(buf); fclose(fp); return io_fail("md3lay file %s: out of memory", path); }
+      buf = nb;
+    }
+    memcpy(buf + n, line, l);
+    n += l;
+    buf[n++] = ' ';
+  }
+  buf[n] = 0;
+  fclose(fp);
+  t->text = t->pos = buf;
+  return 0;
+}
+
+static int md_next(md_tokens *t, char *tok, size_t cap)
+{
+  while (*t->pos && (*t->pos == ' ' || *t->pos == '\t' || *t->pos == '\n' || *t->pos == '\r'))
+    t->pos++;
+  if (!*t->pos) return 0;
+  size_t k = 0;
+  while (*t->pos && !(*t->pos == ' ' || *t->pos == '\t' || *t->pos == '\n' || *t->pos == '\r')) {
+    if (k + 1 < cap) tok[k++] = *t->pos;
+    t->pos++;
+  }
+  tok[k] = 0;
+  return 1;
+}
+
+static int md_head(const char *path, md_tokens *t, int *nprop, int *n_if, int *mx, int *my,
+                   float *geo)
+{
+  char tok[256];
+  if (!md_next(t, tok, sizeof tok)) return io_fail("md3lay file %s: no media type", path);
+  if (strcmp(tok, "one_component") == 0) *nprop = 1;
+  else if (strcmp(tok, "elastic_isotropic") == 0) *nprop = 3;
+  else return io_fail("md3lay file %s: media type %s not supported here", path, tok);
+  int iv[3];
+  for (int k = 0; k < 3; k++) {
+    char *e;
+    if (!md_next(t, tok, sizeof tok)) return io_fail("md3lay file %s: short header", path);
+    iv[k] = (int)strtol(tok, &e, 10);
+  }
+  *n_if = iv[0]; *mx = iv[1]; *my = iv[2];
+  for (int k = 0; k < 4; k++) {
+    if (!md_next(t, tok, sizeof tok)) return io_fail("md3lay file %s: short header", path);
+    geo[k] = strtof(tok, NULL);
+  }
+  if (*n_if < 1 || *mx < 2 || *my < 2)
+    return io_fail("md3lay file %s: %d interfaces of %d x %d samples", path, *n_if, *mx, *my);
+  return 0;
+}
+
+int cgfdh_io_md3lay_dims(const char *path, int *nprop, int *n_interfaces, int *nx, int *ny,
+                         float *geo)
+{
+  md_tokens t;
+  int rc = md_load(path, &t);
+  if (rc) return rc;
+  rc = md_head(path, &t, nprop, n_interfaces, nx, ny, geo);
+  free(t.text);
+  return rc;
+}
+
+int cgfdh_io_read_md3lay(const char *path, float *fields)
+{
+  md_tokens t;
+  int rc = md_load(path, &t);
+  if (rc) return rc;
+  int nprop, n_if, mx, my;
+  float geo[4];
+  rc = md_head(path, &t, &nprop, &n_if, &mx, &my, geo);
+  const int nf = 1 + 3 * nprop;
+  const size_t vol = (size_t)n_if * mx * my;
+  char tok[256];
+  for (size_t p = 0; p < vol && !rc; p++)
+    for (int f = 0; f < nf && !rc; f++) {
+      if (!md_next(&t, tok, sizeof tok))
+        rc = io_fail("md3lay file %s: insufficient data at sample %zu", path, p);
+      else
+        fields[f * vol + p] = strtof(tok, NULL);
+    }
+  free(t.text);
+  return rc;
+}

@@ -285,90 +285,154 @@ drv_rk_curv_col_allstep(
                                iosnap->out_vel[n], iosnap->out_stress[n], iosnap->out_strain[n],
                                g_start, l_start, stride, topoid, &snf[n]));
   }
-  size_t pack_n = (size_t)nvs * gd->nj * gd->nk;
-  if ((size_t)nvs * gd->ni * gd->nk > pack_n) pack_n = (size_t)nvs * gd->ni * gd->nk;
-  if ((size_t)nvs * gd->ni * gd->nj > pack_n) pack_n = (size_t)nvs * gd->ni * gd->nj;
-  for (int n = 0; n < nsn; n++) {
-    size_t m = 15 * (size_t)iosnap->ni[n] * iosnap->nj[n] * iosnap->nk[n];
-    if (m > pack_n) pack_n = m;
+  /* output jobs (the slices, then the snapshots), each with two pinned host
+   * buffers used in turn: a due step's boxes are packed behind its step and
+   * copied out on the engine's copy stream while the next steps run, and
+   * written to their files at the next due step (or the end), in order */
+  int njob = nsl + nsn;
+  size_t *jn = njob ? calloc(njob, sizeof(size_t)) : NULL;
+  for (q = 0; q < nsl; q++) {
+    int a = q < nslice[0] ? 0 : (q < nslice[0] + nslice[1] ? 1 : 2);
+    size_t n1 = a == 0 ? gd->nk : (a == 1 ? gd->nk : gd->nj);
+    size_t n2 = a == 0 ? gd->nj : gd->ni;
+    jn[q] = (size_t)nvs * n1 * n2;
   }
-  float *pack = (nsl || nsn) ? malloc(pack_n * sizeof(float)) : NULL;
+  for (int n = 0; n < nsn; n++)
+    jn[nsl + n] = 15 * (size_t)iosnap->ni[n] * iosnap->nj[n] * iosnap->nk[n];
+  float **hb[2] = {njob ? calloc(njob, sizeof(float *)) : NULL,
+                   njob ? calloc(njob, sizeof(float *)) : NULL};
+  for (int h = 0; h < 2; h++)
+    for (q = 0; q < njob; q++) CHK(cgfdh_host_alloc(jn[q] * sizeof(float), (void **)&hb[h][q]));
+  typedef struct {
+    int live, it;
+    float t_end;
+    int *tk;     /* two tickets per job: slice / snapshot velocities, stresses */
+    int *rec;    /* per snapshot: its record, or -1 when not due */
+  } out_set;
+  out_set os[2];
+  for (int h = 0; h < 2; h++) {
+    os[h].live = 0;
+    os[h].tk = calloc(2 * (size_t)(njob ? njob : 1), sizeof(int));
+    os[h].rec = calloc(nsn ? nsn : 1, sizeof(int));
+  }
   float *w4d = wav->v5d;
   int cmps[CGFDH_MAXWELL_MAX * 6 + 9];
   for (int c = 0; c < nvs; c++) cmps[c] = c;
   /* stresses in the snapshot file's order Txx Tyy Tzz Txz Tyz Txy (wavefield
    * slots 3 4 5 7 6 8) */
   const int stress_cmps[6] = {3, 4, 5, 7, 6, 8};
-  int it_run = 0;
-  for (int it = 0; it < nt_total; it++) {
-    /* drv_rk_curv_col.c:179-181; after a new chunk is read the steps before
-     * it run on the old one, then the new one goes to the device */
-    if (src->dd_is_valid == 1) {
-      src_dd_accit_loadstf(src, it, mympi->myid);
-      if (dd_on && it > 0 && src->dd_is_valid == 1 && src->dd_it_here == 0) {
-        if (it_run < it) CHK(cgfdh_run(s, it_run, it - it_run));
-        it_run = it;
-        CHK(cgfdh_put_dd_values(s, it, src->dd_nt_this_read, dd_vi, dd_mij));
+  int it_run = 0, cur = 0;
+  for (int it = 0; it <= nt_total; it++) {
+    int last = it == nt_total;
+    int nan_due = 0, due = last;
+    if (!last) {
+      /* drv_rk_curv_col.c:179-181; after a new chunk is read the steps
+       * before it run on the old one, then the new one goes to the device */
+      if (src->dd_is_valid == 1) {
+        src_dd_accit_loadstf(src, it, mympi->myid);
+        if (dd_on && it > 0 && src->dd_is_valid == 1 && src->dd_it_here == 0) {
+          if (it_run < it) CHK(cgfdh_run_async(s, it_run, it - it_run));
+          it_run = it;
+          CHK(cgfdh_put_dd_values(s, it, src->dd_nt_this_read, dd_vi, dd_mij));
+        }
       }
+      /* the value check of check_nan_every_nummber_of_steps
+       * (drv_rk_curv_col.c:476-479), wav_check_value's message and exit */
+      nan_due = qc_check_nan_num_of_step > 0 && it % qc_check_nan_num_of_step == 0;
+      due = nsl > 0 || output_all || nan_due;
+      for (int n = 0; n < nsn && !due; n++) due = snap_due(iosnap, n, it, nt_total);
     }
-    /* the value check of check_nan_every_nummber_of_steps
-     * (drv_rk_curv_col.c:476-479), wav_check_value's message and exit */
-    int nan_due = qc_check_nan_num_of_step > 0 && it % qc_check_nan_num_of_step == 0;
-    int due = nsl > 0 || output_all || nan_due;
-    for (int n = 0; n < nsn && !due; n++) due = snap_due(iosnap, n, it, nt_total);
     if (!due) continue;
-    CHK(cgfdh_run(s, it_run, it + 1 - it_run));
-    it_run = it + 1;
-    if (nan_due) {
-      int nan_c;
-      long nan_p;
-      CHK(cgfdh_check_nan(s, &nan_c, &nan_p));
-      if (nan_c >= 0) {
-        fprintf(stderr, "ERROR: NaN occurs at iptr=%ld icmp=%d\n", nan_p, nan_c);
-        fflush(stderr);
-        exit(-1);
+    if (it_run < (last ? nt_total : it + 1)) {
+      CHK(cgfdh_run_async(s, it_run, (last ? nt_total : it + 1) - it_run));
+      it_run = last ? nt_total : it + 1;
+    }
+    out_set *o = &os[cur];
+    if (!last) {
+      if (nan_due) {
+        int nan_c;
+        long nan_p;
+        CHK(cgfdh_check_nan(s, &nan_c, &nan_p));
+        if (nan_c >= 0) {
+          fprintf(stderr, "ERROR: NaN occurs at iptr=%ld icmp=%d\n", nan_p, nan_c);
+          fflush(stderr);
+          exit(-1);
+        }
+      }
+      /* this step's boxes, packed behind it (a set is reused two due steps
+       * later, after its files were written) */
+      o->live = 1;
+      o->it = it;
+      o->t_end = it * dt + t0 + dt;
+      q = 0;
+      for (int n = 0; n < nslice[0]; n++, q++) {
+        int box[9] = {ioslice->slice_x_indx[n], 1, 1, gd->nj1, gd->nj, 1, gd->nk1, gd->nk, 1};
+        CHK(cgfdh_subvolume_async(s, box, nvs, cmps, hb[cur][q], &o->tk[2 * q]));
+      }
+      for (int n = 0; n < nslice[1]; n++, q++) {
+        int box[9] = {gd->ni1, gd->ni, 1, ioslice->slice_y_indx[n], 1, 1, gd->nk1, gd->nk, 1};
+        CHK(cgfdh_subvolume_async(s, box, nvs, cmps, hb[cur][q], &o->tk[2 * q]));
+      }
+      for (int n = 0; n < nslice[2]; n++, q++) {
+        int box[9] = {gd->ni1, gd->ni, 1, gd->nj1, gd->nj, 1, ioslice->slice_z_indx[n], 1, 1};
+        CHK(cgfdh_subvolume_async(s, box, nvs, cmps, hb[cur][q], &o->tk[2 * q]));
+      }
+      for (int n = 0; n < nsn; n++) {
+        int j = nsl + n;
+        o->rec[n] = -1;
+        o->tk[2 * j] = o->tk[2 * j + 1] = -1;
+        if (!snap_due(iosnap, n, it, nt_total)) continue;
+        int box[9] = {iosnap->i1[n], iosnap->ni[n], iosnap->di[n], iosnap->j1[n], iosnap->nj[n],
+                      iosnap->dj[n], iosnap->k1[n], iosnap->nk[n], iosnap->dk[n]};
+        size_t npt = (size_t)box[1] * box[4] * box[7];
+        if (iosnap->out_vel[n])
+          CHK(cgfdh_subvolume_async(s, box, 3, cmps, hb[cur][j], &o->tk[2 * j]));
+        if (iosnap->out_stress[n])
+          CHK(cgfdh_subvolume_async(s, box, 6, stress_cmps, hb[cur][j] + 3 * npt,
+                                    &o->tk[2 * j + 1]));
+        if (iosnap->out_strain[n]) CHK(cgfdh_get_strain_subvolume(s, box, hb[cur][j] + 9 * npt));
+        o->rec[n] = snrec[n]++;
+      }
+      if (output_all) {   /* debug dump of the whole w_end (drv_rk_curv_col.c:513-525) */
+        CHK(cgfdh_get_wavefield(s, w4d));
+        CHKIO(cgfdh_io_var3d_export(output_dir, "w3d", topoid, it, w4d, wav->cmp_pos,
+                                    vnames, wav->ncmp, (const char *const *)gd->index_name,
+                                    gd->nx, gd->ny, gd->nz));
       }
     }
-    float t_cur = it * dt + t0;
-    float t_end = t_cur + dt;
-    q = 0;
-    for (int n = 0; n < nslice[0]; n++, q++) {
-      int box[9] = {ioslice->slice_x_indx[n], 1, 1, gd->nj1, gd->nj, 1, gd->nk1, gd->nk, 1};
-      CHK(cgfdh_get_subvolume(s, box, nvs, cmps, pack));
-      CHKIO(cgfdh_io_slice_put(slf[q], it, t_end, nvs, pack));
+    /* the previous due step's files, while this step runs on the GPU; at the
+     * end both sets, older first */
+    for (int pass = 0; pass < (last ? 2 : 1); pass++) {
+      int h = last ? (cur + 1 + pass) % 2 : 1 - cur;
+      out_set *p = &os[h];
+      if (!p->live) continue;
+      for (q = 0; q < nsl; q++) {
+        CHK(cgfdh_wait(s, p->tk[2 * q]));
+        CHKIO(cgfdh_io_slice_put(slf[q], p->it, p->t_end, nvs, hb[h][q]));
+      }
+      for (int n = 0; n < nsn; n++) {
+        int j = nsl + n;
+        if (p->rec[n] < 0) continue;
+        size_t npt = (size_t)iosnap->ni[n] * iosnap->nj[n] * iosnap->nk[n];
+        if (p->tk[2 * j] >= 0) CHK(cgfdh_wait(s, p->tk[2 * j]));
+        if (p->tk[2 * j + 1] >= 0) CHK(cgfdh_wait(s, p->tk[2 * j + 1]));
+        CHKIO(cgfdh_io_snap_put(snf[n], p->rec[n], p->t_end,
+                                iosnap->out_vel[n] ? hb[h][j] : NULL,
+                                iosnap->out_stress[n] ? hb[h][j] + 3 * npt : NULL,
+                                iosnap->out_strain[n] ? hb[h][j] + 9 * npt : NULL));
+      }
+      p->live = 0;
     }
-    for (int n = 0; n < nslice[1]; n++, q++) {
-      int box[9] = {gd->ni1, gd->ni, 1, ioslice->slice_y_indx[n], 1, 1, gd->nk1, gd->nk, 1};
-      CHK(cgfdh_get_subvolume(s, box, nvs, cmps, pack));
-      CHKIO(cgfdh_io_slice_put(slf[q], it, t_end, nvs, pack));
-    }
-    for (int n = 0; n < nslice[2]; n++, q++) {
-      int box[9] = {gd->ni1, gd->ni, 1, gd->nj1, gd->nj, 1, ioslice->slice_z_indx[n], 1, 1};
-      CHK(cgfdh_get_subvolume(s, box, nvs, cmps, pack));
-      CHKIO(cgfdh_io_slice_put(slf[q], it, t_end, nvs, pack));
-    }
-    for (int n = 0; n < nsn; n++) {
-      if (!snap_due(iosnap, n, it, nt_total)) continue;
-      int box[9] = {iosnap->i1[n], iosnap->ni[n], iosnap->di[n], iosnap->j1[n], iosnap->nj[n],
-                    iosnap->dj[n], iosnap->k1[n], iosnap->nk[n], iosnap->dk[n]};
-      size_t npt = (size_t)box[1] * box[4] * box[7];
-      float *v = iosnap->out_vel[n] ? pack : NULL;
-      float *t = iosnap->out_stress[n] ? pack + 3 * npt : NULL;
-      float *e = iosnap->out_strain[n] ? pack + 9 * npt : NULL;
-      if (v) CHK(cgfdh_get_subvolume(s, box, 3, cmps, v));
-      if (t) CHK(cgfdh_get_subvolume(s, box, 6, stress_cmps, t));
-      if (e) CHK(cgfdh_get_strain_subvolume(s, box, e));
-      CHKIO(cgfdh_io_snap_put(snf[n], snrec[n]++, t_end, v, t, e));
-    }
-    if (output_all) {   /* debug dump of the whole w_end (drv_rk_curv_col.c:513-525) */
-      CHK(cgfdh_get_wavefield(s, w4d));
-      CHKIO(cgfdh_io_var3d_export(output_dir, "w3d", topoid, it, w4d, wav->cmp_pos,
-                                  vnames, wav->ncmp, (const char *const *)gd->index_name,
-                                  gd->nx, gd->ny, gd->nz));
-    }
+    cur = 1 - cur;
   }
-  if (it_run < nt_total) CHK(cgfdh_run(s, it_run, nt_total - it_run));
-  free(pack);
+  CHK(cgfdh_sync(s));
+  for (int h = 0; h < 2; h++) {
+    for (q = 0; q < njob; q++) CHK(cgfdh_host_free(hb[h][q]));
+    free(hb[h]);
+    free(os[h].tk);
+    free(os[h].rec);
+  }
+  free(jn);
   for (q = 0; q < nsl; q++) CHKIO(cgfdh_nc_close(slf[q]));
   for (int n = 0; n < nsn; n++) CHKIO(cgfdh_nc_close(snf[n]));
   free(slf); free(snf); free(snrec);

@@ -31,6 +31,10 @@ def _io():
         L.cgfdh_sac_export_1c1r.argtypes = [C.c_char_p, _P] + [C.c_float] * 9 + [C.c_int]
         L.cgfdh_io_recv_output_sac.argtypes = [C.c_char_p, C.c_char_p, C.c_int, _P, _P, _P,
                                                C.c_int, C.c_int, _P, C.c_int, C.c_float]
+        L.cgfdh_io_grid_layer_dims.argtypes = [C.c_char_p, _P, _P, _P]
+        L.cgfdh_io_read_grid_layer.argtypes = [C.c_char_p, _P, _P, _P]
+        L.cgfdh_io_md3lay_dims.argtypes = [C.c_char_p, _P, _P, _P, _P, _P]
+        L.cgfdh_io_read_md3lay.argtypes = [C.c_char_p, _P]
         L._io_sigs = True
     return L
 
@@ -142,3 +146,31 @@ def write_receivers_sac(output_dir, evtnm, names, xyz, seismo, cmp_names, dt, nc
     _check(_io().cgfdh_io_recv_output_sac(output_dir.encode(), evtnm.encode(), nr, C.cast(na, _P),
                                           _f32(xyz).ctypes.data_as(_P), s.ctypes.data_as(_P),
                                           nca, nc, C.cast(cn, _P), nt, dt), "receiver SAC")
+
+
+def read_grid_layer(path):
+    """The grid layer file gd_curv_gen_layer reads: (ncell, equal,
+    layer_xyz (3, n_interfaces, ny, nx))."""
+    n_if, nx, ny = C.c_int(), C.c_int(), C.c_int()
+    _check(_io().cgfdh_io_grid_layer_dims(path.encode(), C.byref(n_if), C.byref(nx),
+                                          C.byref(ny)), "grid layer file")
+    ncell = np.zeros(n_if.value - 1, np.int32)
+    equal = np.zeros(n_if.value - 1, np.int32)
+    xyz = np.zeros((3, n_if.value, ny.value, nx.value), np.float32)
+    _check(_io().cgfdh_io_read_grid_layer(path.encode(), ncell.ctypes.data_as(_P),
+                                          equal.ctypes.data_as(_P), xyz.ctypes.data_as(_P)),
+           "grid layer file")
+    return ncell, equal, xyz
+
+
+def read_md3lay(path):
+    """The layered-model file media_layer2model reads (one_component or
+    elastic_isotropic): (nprop, fields [1+3 nprop][n_interfaces][ny][nx],
+    geo = minx, miny, dx, dy)."""
+    nprop, n_if, nx, ny = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    geo = np.zeros(4, np.float32)
+    _check(_io().cgfdh_io_md3lay_dims(path.encode(), C.byref(nprop), C.byref(n_if), C.byref(nx),
+                                      C.byref(ny), geo.ctypes.data_as(_P)), "md3lay file")
+    fields = np.zeros((1 + 3 * nprop.value, n_if.value, ny.value, nx.value), np.float32)
+    _check(_io().cgfdh_io_read_md3lay(path.encode(), fields.ctypes.data_as(_P)), "md3lay file")
+    return nprop.value, fields, geo

@@ -72,6 +72,57 @@ def depth_to_axis(xyz, xyd):
     return found, z
 
 
+def gen_grid_layer(ncell, equal, layer_xyz, resample, start, ni, nj, nk):
+    """A one-block grid from interface samples on the GPU
+    (cgfdh_gen_grid_layer): layer_xyz (3, n_if, ny_l, nx_l).  Returns the
+    coordinates (3, nk+6, nj+6, ni+6)."""
+    L = np.ascontiguousarray(layer_xyz, np.float32)
+    n_if, ny_l, nx_l = L.shape[1:]
+    nc = np.ascontiguousarray(ncell, np.int32)
+    eq = np.ascontiguousarray(equal, np.int32)
+    rs = np.ascontiguousarray(resample, np.int32)
+    st = np.ascontiguousarray(start, np.int32)
+    out = np.zeros((3, nk + 6, nj + 6, ni + 6), np.float32)
+    check(lib().cgfdh_gen_grid_layer(n_if, _p(nc), _p(eq), nx_l, ny_l, _p(L), _p(rs), _p(st),
+                                     ni, nj, nk, ni, nj, nk, 0, 0, 0, _p(out)), "gen_grid_layer")
+    return out
+
+
+def layer2model(nprop, fields, geo, xyz):
+    """A layered model onto grid points on the GPU with local values
+    (cgfdh_layer2model): fields [1+3 nprop][n_if][ny][nx] and geo from
+    output.read_md3lay, xyz (3, ...) the points.  Returns [var] or
+    [lambda, mu, rho], each of the points' shape."""
+    f = np.ascontiguousarray(fields, np.float32)
+    g = np.ascontiguousarray(geo, np.float32)
+    x, y, z = (np.ascontiguousarray(xyz[d], np.float32) for d in range(3))
+    n = x.size
+    outs = [np.zeros(x.shape, np.float32) for _ in range(nprop)]
+    pp = lambda k: _p(outs[k]) if k < nprop else None  # noqa: E731
+    check(lib().cgfdh_layer2model(nprop, f.shape[1], f.shape[3], f.shape[2], _p(g), _p(f), n,
+                                  _p(x), _p(y), _p(z), pp(0), pp(1), pp(2)), "layer2model")
+    return outs
+
+
+class PinnedBuffer:
+    """Page-locked host memory (cgfdh_host_alloc) viewed as a float32 array,
+    the destination that lets cgfdh_subvolume_async copies overlap steps."""
+
+    def __init__(self, shape):
+        n = int(np.prod(shape))
+        p = C.c_void_p()
+        check(lib().cgfdh_host_alloc(max(n, 1) * 4, C.byref(p)), "host_alloc")
+        self.ptr = p
+        buf = (C.c_float * max(n, 1)).from_address(p.value)
+        self.array = np.frombuffer(buf, np.float32, count=n).reshape(shape)
+
+    def close(self):
+        if self.ptr:
+            self.array = None
+            lib().cgfdh_host_free(self.ptr)
+            self.ptr = None
+
+
 class Solver:
     """Engine for one grid block: ni x nj x nk physical points."""
 
@@ -300,6 +351,30 @@ class Solver:
         check(lib().cgfdh_get_subvolume(self.h, _p(b), int(c.size), _p(c), _p(out)),
               "get_subvolume")
         return out
+
+    def run_async(self, it0, nsteps):
+        """Enqueue steps without waiting (cgfdh_run_async)."""
+        check(lib().cgfdh_run_async(self.h, it0, nsteps), "run_async")
+
+    def subvolume_async(self, box, comps, out):
+        """Pack a box of the state after every enqueued step into out (an
+        array of the subvolume's shape, kept alive by the caller, best in
+        pinned memory from PinnedBuffer) on a copy stream; returns the
+        ticket for wait (cgfdh_subvolume_async)."""
+        b, (ni, nj, nk) = self._box(box)
+        c = np.ascontiguousarray(np.asarray(comps, dtype=np.int32).reshape(-1))
+        assert out.dtype == np.float32 and out.flags.c_contiguous
+        assert out.size == c.size * ni * nj * nk
+        t = C.c_int()
+        check(lib().cgfdh_subvolume_async(self.h, _p(b), int(c.size), _p(c), _p(out),
+                                          C.byref(t)), "subvolume_async")
+        return t.value
+
+    def wait(self, ticket):
+        check(lib().cgfdh_wait(self.h, int(ticket)), "wait")
+
+    def sync(self):
+        check(lib().cgfdh_sync(self.h), "sync")
 
     def strain_subvolume(self, box):
         """Exx, Eyy, Ezz, Exz, Eyz, Exy of the box: (6, nk, nj, ni)."""

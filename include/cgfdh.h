@@ -273,6 +273,25 @@ int cgfdh_get_subvolume_noalloc(cgfdh_solver *s, const int *box, int ncomp, cons
                                 float *out);
 int cgfdh_get_strain_subvolume(cgfdh_solver *s, const int *box, float *out);
 
+/* Asynchronous steps and output, so that packing and the copy to the host
+ * overlap the steps after them (the reference's loop writes its slices and
+ * snapshots between steps, drv_rk_curv_col.c:497-525).
+ * cgfdh_run_async enqueues steps it0 .. it0+nsteps-1 and returns;
+ * cgfdh_subvolume_async packs a box of the state left by every step
+ * enqueued so far (cgfdh_get_subvolume's layout) on the solver's stream and
+ * copies it to out on a copy stream, returning a ticket; cgfdh_wait(ticket)
+ * returns once out holds the values, cgfdh_sync once everything enqueued
+ * has finished.  out must stay valid until then; it overlaps with the steps
+ * when it is pinned memory from cgfdh_host_alloc.  Up to 16 boxes can be in
+ * flight per solver (a 17th request waits for the oldest copy). */
+int cgfdh_run_async(cgfdh_solver *s, int it0, int nsteps);
+int cgfdh_subvolume_async(cgfdh_solver *s, const int *box, int ncomp, const int *comps,
+                          float *out, int *ticket);
+int cgfdh_wait(cgfdh_solver *s, int ticket);
+int cgfdh_sync(cgfdh_solver *s);
+int cgfdh_host_alloc(size_t bytes, void **p);
+int cgfdh_host_free(void *p);
+
 /* Point location on a curvilinear grid, computed on the GPU, for npts
  * points at once: gd_curv_coord_to_local_indx (gd_t.c:1998-2173) with its
  * cell boxes and 4x4x4 tiles (gd_curv_set_minmax, gd_t.c:1708-1880), the
@@ -295,6 +314,42 @@ int cgfdh_locate_points(int ni, int nj, int nk, const float *xyz, int npts, cons
  * reference leaves it. */
 int cgfdh_depth_to_axis(int ni, int nj, int nk, const float *xyz, int npts, const float *xyd,
                         int *found, float *z);
+
+/* The curvilinear grid of one block generated from a layer file on the GPU,
+ * gd_curv_gen_layer (gd_t.c:987-1480) after the file is read
+ * (cgfdh_io_read_grid_layer reads it): n_if interfaces with ncell cells and
+ * an equal-spacing flag per layer (n_if - 1 each), layer_xyz the x, y, z of
+ * every interface sample ([3][n_if][ny_layers][nx_layers], bottom interface
+ * first); resample and start the par file's refine factors and start
+ * indices (x, y, z); nt* the total physical points and g*1 the block's first
+ * global index (0 for one block).  The interfaces are resampled in x and y
+ * (every |f|-th sample, or the reference's cubic spline), every column gets
+ * its z values through the layers (gd_grid_z_interp, graded spacing in
+ * unequal middle layers) and x, y as cubic splines of z, and three ghost
+ * points are mirrored above the top.  xyz receives x3d, y3d, z3d of the
+ * block with ghosts.  The reference's parameter checks (gd_t.c:1151-1177)
+ * and unordered samples or heights are errors. */
+int cgfdh_gen_grid_layer(int n_if, const int *ncell, const int *equal, int nx_layers,
+                         int ny_layers, const float *layer_xyz, const int *resample,
+                         const int *start, int ntx, int nty, int ntz, int ni, int nj, int nk,
+                         int gni1, int gnj1, int gnk1, float *xyz);
+
+/* A layered model discretised onto grid points on the GPU with local values:
+ * media_layer2model_onecmp (nprop 1) and media_layer2model_el_iso (nprop 3)
+ * with equivalent_medium_method "loc" (media_layer2model.cpp:47-190,
+ * 422-560, 1239-1383).  The interface mesh has n_if interfaces of mx x my
+ * samples at geo = minx, miny, dx, dy; fields as cgfdh_io_read_md3lay gives
+ * them.  Per point (x, y, z of npts points, e.g. x3d, y3d, z3d with ghosts):
+ * each interface's elevation interpolated bilinearly, the layer holding the
+ * point, value + dz^pow * grad of each property (the mean of both layers'
+ * values on an interface, the top layer's values above the surface).
+ * Outputs var into a, or lambda, mu, rho into a, b, c (mu = vs^2 rho,
+ * lambda = vp^2 rho - 2 mu).  A point outside the mesh, or with
+ * vp <= sqrt(2) vs, is an error (the reference stops there too).  The
+ * harmonic / arithmetic averaging methods are not provided. */
+int cgfdh_layer2model(int nprop, int n_if, int mx, int my, const float *geo,
+                      const float *fields, long npts, const float *x, const float *y,
+                      const float *z, float *a, float *b, float *c);
 
 /* Stable time step of a curvilinear grid, blk_dt_esti_curv
  * (blk_t.c:2840-2930), computed on the GPU: the minimum over physical points
@@ -338,8 +393,8 @@ int cgfdh_last_timing(cgfdh_solver *s, double *run_ms, double *stage_ms,
  * PML faces folded in (free-surface rows on the per-point kernel); 17 = the
  * two-phase interior kernel on the tiles clear of every PML face plus a
  * shell kernel that requests each PML slab's memory variables first.
- * Default 16 for the isotropic medium, 12 for the VTI and anisotropic
- * media, 4 for the visco-elastic medium.  Variants 1..3 exist for the
+ * Default 16 for the elastic media (isotropic, VTI, anisotropic), 4 for the
+ * visco-elastic medium.  Variants 1..3 exist for the
  * isotropic medium only, 6, 7 and 12..17 not for the visco-elastic medium.
  * All variants produce identical bits. */
 int cgfdh_set_kernel_variant(cgfdh_solver *s, int variant);

@@ -118,6 +118,25 @@ int cgfdh_io_recv_output_sac(const char *output_dir, const char *evtnm, int nrec
                              int ncomp_all, int ncomp, const char *const *cmp_names, int nt,
                              float dt);
 
+/* The grid layer file gd_curv_gen_layer reads (in_grid_layer_file,
+ * gd_t.c:1031-1060): first its sizes, then its contents into caller arrays:
+ * ncell and equal (n_interfaces - 1 each) and layer_xyz = x, then y, then z
+ * of every sample ([3][n_interfaces][ny_layers][nx_layers]), the input of
+ * cgfdh_gen_grid_layer. */
+int cgfdh_io_grid_layer_dims(const char *path, int *n_interfaces, int *nx_layers,
+                             int *ny_layers);
+int cgfdh_io_read_grid_layer(const char *path, int *ncell, int *equal, float *layer_xyz);
+
+/* The layered-model file of media_layer2model (.md3lay, read_interface_file,
+ * media_read_file.cpp:22-420) for the one_component and elastic_isotropic
+ * types: first its header (nprop 1 or 3, interfaces, samples nx x ny, geo =
+ * minx, miny, dx, dy), then its samples into fields
+ * ([1 + 3 nprop][n_interfaces][ny][nx]: elevation, then value, gradient and
+ * power of var, or of rho, vp, vs), the input of cgfdh_layer2model. */
+int cgfdh_io_md3lay_dims(const char *path, int *nprop, int *n_interfaces, int *nx, int *ny,
+                         float *geo);
+int cgfdh_io_read_md3lay(const char *path, float *fields);
+
 #ifdef __cplusplus
 }
 #endif

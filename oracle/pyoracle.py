@@ -95,6 +95,16 @@ def load_oracle(fp64=False):
     lib.orc_sample_shift.argtypes = [C.c_float] * 3 + [_P] * 3 + [C.c_int] * 3 + [_P]
     lib.orc_locate_point.restype = C.c_int
     lib.orc_locate_point.argtypes = [C.c_int] * 3 + [_P] * 3 + [C.c_float] * 3 + [_P, _P, _P]
+    lib.orc_spline.restype = C.c_int
+    lib.orc_spline.argtypes = [C.c_int] + [_P] * 5
+    lib.orc_spline_at.restype = C.c_int
+    lib.orc_spline_at.argtypes = [C.c_int, _P, _P, C.c_int, _P, _P]
+    lib.orc_grid_z.argtypes = [_P, _P, _P, _P, C.c_int]
+    lib.orc_gen_layer.restype = C.c_int
+    lib.orc_gen_layer.argtypes = ([C.c_int, _P, _P, C.c_int, C.c_int, _P, _P, _P] +
+                                  [C.c_int] * 9 + [_P] * 3)
+    lib.orc_layer2model.restype = C.c_int
+    lib.orc_layer2model.argtypes = [C.c_int, _P, _P, _P, C.c_size_t] + [_P] * 6
     lib.orc_depth_to_axis.restype = C.c_int
     lib.orc_depth_to_axis.argtypes = [C.c_int] * 3 + [_P] * 3 + [C.c_float] * 2 + [_P]
     return lib
@@ -122,6 +132,22 @@ def locate_points(coords, ni, nj, nk, pts, cells=False):
         if found[q]:
             ijk[q], inc[q], cell[q] = a, b, c
     return (found, ijk, inc, cell) if cells else (found, ijk, inc)
+
+
+def gen_layer(ncell, equal, layer_xyz, resample, start, ni, nj, nk):
+    """orc_gen_layer for a one-block grid: layer_xyz (3, n_if, ny_l, nx_l).
+    Returns (rc, coords (3, nk+6, nj+6, ni+6))."""
+    lib = load_oracle()
+    L = np.ascontiguousarray(layer_xyz, np.float32)
+    n_if, ny_l, nx_l = L.shape[1:]
+    nc = np.ascontiguousarray(ncell, np.int32)
+    eq = np.ascontiguousarray(equal, np.int32)
+    rs = np.ascontiguousarray(resample, np.int32)
+    st = np.ascontiguousarray(start, np.int32)
+    out = np.zeros((3, nk + 6, nj + 6, ni + 6), np.float32)
+    rc = lib.orc_gen_layer(n_if, ptr(nc), ptr(eq), nx_l, ny_l, ptr(L), ptr(rs), ptr(st), ni, nj,
+                           nk, ni, nj, nk, 0, 0, 0, ptr(out[0]), ptr(out[1]), ptr(out[2]))
+    return rc, out
 
 
 def depth_to_axis(coords, ni, nj, nk, xyd):

@@ -33,7 +33,9 @@ def save_case(path, case, dt):
              free_top=case["free_top"], src_ijk=np.array(src["ijk"], np.int32),
              src_itb=np.array(src["it_begin"], np.int32),
              src_ite=np.array(src["it_end"], np.int32), src_max_nt=src["max_nt"],
-             src_mij=np.asarray(src["mij"], np.float32))
+             src_mij=np.asarray(src["mij"], np.float32),
+             medium_type=int(case.get("medium_type", 2)),
+             visco_wl=np.asarray(case.get("visco_wl", []), np.float32))
 
 
 def _load_case(path):
@@ -47,7 +49,10 @@ def _load_case(path):
                 pml_beta=float(z["pml_beta"]), free_top=int(z["free_top"]),
                 src=dict(nsrc=1, ijk=z["src_ijk"].tolist(), it_begin=z["src_itb"].tolist(),
                          it_end=z["src_ite"].tolist(), max_nt=int(z["src_max_nt"]),
-                         mij=z["src_mij"]))
+                         mij=z["src_mij"]),
+                medium_type=int(z["medium_type"]))
+    if z["visco_wl"].size:
+        case["visco_wl"] = z["visco_wl"]
     return case, float(z["dt"])
 
 
@@ -62,7 +67,7 @@ def worker(path, steps):
     cfg, keep = cs.ref_cfg(case)
     co = np.ascontiguousarray(case["coords"])
     nx, ny, nz = case["ni"] + 6, case["nj"] + 6, case["nk"] + 6
-    w = np.zeros(9 * nx * ny * nz, np.float32)
+    w = np.zeros(cs.ncomp(case) * nx * ny * nz, np.float32)
     t = time.perf_counter()
     ref.refh_run(C.byref(cfg), po.ptr(co), po.ptr(case["metric"]), po.ptr(case["med"]), dt,
                  steps, None, 0, None, None, po.ptr(w), None, None, None)
