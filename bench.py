@@ -187,12 +187,14 @@ def bench_batch(args):
     import torch
     from cgfd3d_amd import Batch, workloads
     t = time.perf_counter()
-    solvers, npts, bstep = [], 0, 0.0
+    solvers, npts, bstep, big = [], 0, 0.0, None
     nt_total = max(1001, args.steps + args.warmup + 16)
     for wl in workloads.batch_workloads(nt_total=nt_total):
         solvers.append(wl.build_solver(variant=args.variant))
         npts += wl.npoints
         bstep += wl.algorithmic_bytes_per_step()
+        if big is None or wl.npoints > big.npoints:
+            big = wl               # the CPU baseline's sub-block comes from the largest grid
         del wl
     log(f"{len(solvers)} grids, {npts / 1e6:.1f} M points, set up in "
         f"{time.perf_counter() - t:.1f} s")
@@ -231,7 +233,7 @@ def bench_batch(args):
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes_per_step": bstep,
                      "kernel": "all stage kernels of the batch (graph replay device time)"},
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(big),
     }
     print(json.dumps(out), flush=True)
 

@@ -48,6 +48,25 @@ SIGNATURES = [
     ("cgfdh_depth_to_axis", C.c_int, [C.c_int] * 3 + [_P, C.c_int, _P, _P, _P]),
     ("cgfdh_layer2model", C.c_int,
      [C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, C.c_long, _P, _P, _P, _P, _P, _P]),
+    ("cgfdh_layer2model_grid", C.c_int, [C.c_int] * 6 + [_P, _P] + [C.c_int] * 3 + [_P] * 4),
+    ("cgfdh_media_layer2model_onecmp", C.c_int,
+     [_P] * 4 + [C.c_size_t] * 3 + [C.c_int, C.c_char_p, C.c_char_p, C.c_int]),
+    ("cgfdh_media_layer2model_el_iso", C.c_int,
+     [_P] * 6 + [C.c_size_t] * 3 + [C.c_int, C.c_char_p, C.c_char_p, C.c_int]),
+    ("cgfdh_media_layer2model_el_vti", C.c_int,
+     [_P] * 9 + [C.c_size_t] * 3 + [C.c_int, C.c_char_p, C.c_char_p, C.c_int]),
+    ("cgfdh_media_layer2model_el_aniso", C.c_int,
+     [_P] * 25 + [C.c_size_t] * 3 + [C.c_int, C.c_char_p, C.c_char_p, C.c_int]),
+    ("cgfdh_grid2model_grid", C.c_int,
+     [C.c_int] * 4 + [_P, C.c_int, C.c_int, _P, _P] + [C.c_int] * 3 + [_P] * 4),
+    ("cgfdh_media_grid2model_onecmp", C.c_int,
+     [_P] * 4 + [C.c_int] * 3 + [C.c_float] * 4 + [C.c_int, C.c_char_p, C.c_char_p, C.c_int]),
+    ("cgfdh_media_grid2model_el_iso", C.c_int,
+     [_P] * 6 + [C.c_int] * 3 + [C.c_float] * 4 + [C.c_int, C.c_char_p, C.c_char_p, C.c_int]),
+    ("cgfdh_media_grid2model_el_vti", C.c_int,
+     [_P] * 9 + [C.c_int] * 3 + [C.c_float] * 4 + [C.c_int, C.c_char_p, C.c_char_p, C.c_int]),
+    ("cgfdh_media_grid2model_el_aniso", C.c_int,
+     [_P] * 25 + [C.c_int] * 3 + [C.c_float] * 4 + [C.c_int, C.c_char_p, C.c_char_p, C.c_int]),
     ("cgfdh_gen_grid_layer", C.c_int,
      [C.c_int, _P, _P, C.c_int, C.c_int, _P, _P, _P] + [C.c_int] * 9 + [_P]),
     ("cgfdh_set_wavefield", C.c_int, [_P, _P]),

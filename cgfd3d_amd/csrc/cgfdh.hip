@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -17,6 +18,7 @@
 #include <vector>
 
 #include "cgfdh.h"
+#include "cgfdh_io.h"
 #include "stage_launch.h"
 
 using namespace cgfdh;
@@ -2207,59 +2209,378 @@ extern "C" int cgfdh_gen_grid_layer(int n_if, const int* ncell_in, const int* eq
 // layered model onto the grid (media_layer2model, local values)
 // ---------------------------------------------------------------------------
 
+// the native coefficients' places among a target's outputs, in the
+// reference functions' argument order: onecmp var | el_iso lambda mu rho |
+// el_vti rho c11 c33 c55 c66 c13 | el_aniso rho c11 c12 .. c66; -1 when the
+// target does not take the media type
+static int layer_slots(int mt, int target, int gridmodel, int* slot)
+{
+  static const int iso_in_grid_iso[3] = {1, 2, 0};   // media_grid2model_el_iso: rho lambda mu
+  static const int vti_in_vti[6] = {1, 2, 3, 4, 5, 0};
+  static const int iso_in_aniso[3] = {3, 16, 0};
+  static const int vti_in_aniso[6] = {1, 12, 19, 21, 3, 0};
+  const bool vti = mt == MD_PREM || mt == MD_THOMSEN || mt == MD_VCIJ;
+  const bool full = mt == MD_TTHOMSEN || mt == MD_TBOND || mt == MD_ACIJ;
+  if (target == CGFDH_L2M_ONECMP && mt == MD_ONE) { slot[0] = 0; return 1; }
+  if (target == CGFDH_L2M_EL_ISO && mt == MD_ISO) {
+    for (int q = 0; q < 3; q++) slot[q] = gridmodel ? iso_in_grid_iso[q] : q;
+    return 3;
+  }
+  if (target == CGFDH_L2M_EL_VTI && vti) {
+    for (int q = 0; q < 6; q++) slot[q] = vti_in_vti[q];
+    return 6;
+  }
+  if (target == CGFDH_L2M_EL_ANISO) {
+    if (mt == MD_ISO) { for (int q = 0; q < 3; q++) slot[q] = iso_in_aniso[q]; return 3; }
+    if (vti) { for (int q = 0; q < 6; q++) slot[q] = vti_in_aniso[q]; return 6; }
+    if (full) {
+      for (int q = 0; q < 21; q++) slot[q] = 1 + q;
+      slot[21] = 0;
+      return 22;
+    }
+  }
+  return -1;
+}
+
+static int layer_nout(int target)
+{
+  return target == CGFDH_L2M_ONECMP ? 1 : target == CGFDH_L2M_EL_ISO ? 3
+                                        : target == CGFDH_L2M_EL_VTI ? 6 : 22;
+}
+
+// the layered model (nl = 0) or the grid model (nl layers of ngz[l]
+// surfaces, n_if in all) on points (method loc; grid = 0) or on a
+// curvilinear grid of nx*ny*nz points (any method)
+static int layer2model_run(const char* who, int mt, int target, int method, int n_if, int mx,
+                           int my, const float* geo, const float* fields, int grid, int nx,
+                           int ny, int nz, long npts, const float* x, const float* y,
+                           const float* z, float* const* out, int nl = 0,
+                           const int* ngz = nullptr)
+{
+  if (!geo || !fields || !x || !y || !z || !out) return fail(CGFDH_EINVAL, "%s: null argument", who);
+  if (target < CGFDH_L2M_ONECMP || target > CGFDH_L2M_EL_ANISO)
+    return fail(CGFDH_EINVAL, "%s: target %d", who, target);
+  if (method < CGFDH_L2M_LOC || method > CGFDH_L2M_ARI || (!grid && method != CGFDH_L2M_LOC))
+    return fail(CGFDH_EINVAL, "%s: method %d (loc 0, har 1, ari 2; averaging needs a grid)",
+                who, method);
+  LayerOut O;
+  O.nn = layer_slots(mt, target, nl > 0, O.slot);
+  if (O.nn < 0) return fail(CGFDH_EINVAL, "%s: media type %d is not supported by this target", who, mt);
+  if (n_if < 1 || n_if > (nl > 0 ? MEDIA_MAX_SURF : MEDIA_MAX_IF) || mx < 2 || my < 2 || npts < 0)
+    return fail(CGFDH_EINVAL, "%s: %d %s of %dx%d samples, %ld points", who, n_if,
+                nl > 0 ? "surfaces" : "interfaces", mx, my, npts);
+  LayerModel M;
+  memset(&M, 0, sizeof M);
+  if (nl > 0) {
+    if (!ngz || nl > MEDIA_MAX_LAYERS) return fail(CGFDH_EINVAL, "%s: %d layers", who, nl);
+    int acc = 0;
+    for (int l = 0; l < nl; l++) {
+      if (ngz[l] < 1) return fail(CGFDH_EINVAL, "%s: layer %d has %d surfaces", who, l, ngz[l]);
+      M.first[l] = acc;
+      acc += ngz[l];
+      if (l < nl - 1 && acc < n_if) M.bound[acc >> 5] |= 1u << (acc & 31);
+    }
+    if (acc != n_if)
+      return fail(CGFDH_EINVAL, "%s: the layers hold %d surfaces, not %d", who, acc, n_if);
+    M.grid = 1;
+    M.nl = nl;
+  }
+  if (grid && (nx < 1 || ny < 1 || nz < 1 || (long)nx * ny * nz != npts || npts >= INT_MAX))
+    return fail(CGFDH_EINVAL, "%s: grid %dx%dx%d", who, nx, ny, nz);
+  const int nout = layer_nout(target);
+  for (int q = 0; q < nout; q++)
+    if (!out[q]) return fail(CGFDH_EINVAL, "%s: null output %d", who, q);
+  if (npts == 0) return CGFDH_OK;
+  const int nprop = media_nprop(mt);
+  const size_t nf = (nl > 0 ? 1 : 1 + 2 * (size_t)nprop) + nprop, vol = (size_t)n_if * mx * my;
+  const size_t n = npts;
+  DevBuf df, dx, dout, dbad;
+  HIPCHK(hipMalloc(&df.p, nf * vol * sizeof(float)));
+  HIPCHK(hipMemcpy(df.p, fields, nf * vol * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&dx.p, 3 * n * sizeof(float)));
+  float* d = dx.as<float>();
+  HIPCHK(hipMemcpy(d, x, n * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d + n, y, n * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d + 2 * n, z, n * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&dout.p, (size_t)nout * n * sizeof(float)));
+  HIPCHK(hipMalloc(&dbad.p, sizeof(unsigned long long)));
+  const unsigned long long none = ~0ull;
+  HIPCHK(hipMemcpy(dbad.p, &none, sizeof none, hipMemcpyHostToDevice));
+  M.f = df.as<float>();
+  M.ni = n_if; M.mx = mx; M.my = my; M.nprop = nprop; M.mt = mt;
+  M.minx = geo[0]; M.miny = geo[1]; M.dx = geo[2]; M.dy = geo[3];
+  M.bad = dbad.as<unsigned long long>();
+  O.out = dout.as<float>();
+  O.n = npts;
+  const dim3 g((unsigned)((n + 255) / 256));
+  auto report = [&](unsigned long long bad, bool sub) {
+    const long p = (long)(bad & ((1ull << 48) - 1));
+    if ((bad >> 48) == 1)
+      return fail(CGFDH_EINVAL, "%s: point %ld (%g, %g, %g)%s is outside the interface mesh", who,
+                  p, (double)x[p], (double)y[p], (double)z[p],
+                  sub ? ": a sub-point of its half cell" : "");
+    return fail(CGFDH_EINVAL, "%s: point %ld%s has vp <= sqrt(2) vs", who, p,
+                sub ? " (a sub-point of its half cell)" : "");
+  };
+  hipLaunchKernelGGL(layer_loc_kernel, g, dim3(256), 0, 0, M, d, d + n, d + 2 * n, O);
+  HIPCHK(hipGetLastError());
+  unsigned long long bad = 0;
+  HIPCHK(hipMemcpy(&bad, dbad.p, sizeof bad, hipMemcpyDeviceToHost));
+  if (bad != none) return report(bad, false);
+  // the grid model averages nothing with one layer (the reference keeps
+  // the local values then)
+  if (method != CGFDH_L2M_LOC && nl != 1) {
+    DevBuf dh, dmat, dlist, dcnt;
+    HIPCHK(hipMalloc(&dh.p, 3 * n * sizeof(float)));
+    HIPCHK(hipMalloc(&dmat.p, n * sizeof(int)));
+    HIPCHK(hipMalloc(&dlist.p, n * sizeof(int)));
+    HIPCHK(hipMalloc(&dcnt.p, sizeof(unsigned)));
+    HIPCHK(hipMemset(dcnt.p, 0, sizeof(unsigned)));
+    hipLaunchKernelGGL(half_grid_kernel, g, dim3(256), 0, 0, d, dh.as<float>(), nx, ny, nz);
+    hipLaunchKernelGGL(layer_mark_kernel, g, dim3(256), 0, 0, M, dh.as<float>(), npts,
+                       dmat.as<int>());
+    hipLaunchKernelGGL(layer_mixed_kernel, g, dim3(256), 0, 0, dmat.as<int>(), nx, ny, nz,
+                       dlist.as<int>(), dcnt.as<unsigned>());
+    HIPCHK(hipGetLastError());
+    unsigned cnt = 0;
+    HIPCHK(hipMemcpy(&cnt, dcnt.p, sizeof cnt, hipMemcpyDeviceToHost));
+    const int har = method == CGFDH_L2M_HAR;
+    if (cnt > 0) {
+#define CGFDH_AVG(NN)                                                                         \
+  hipLaunchKernelGGL(layer_average_kernel<NN>, dim3(cnt), dim3(256), 0, 0, M, dh.as<float>(), \
+                     dlist.as<int>(), har, O, nx, ny)
+      switch (O.nn) {
+      case 1: CGFDH_AVG(1); break;
+      case 3: CGFDH_AVG(3); break;
+      case 6: CGFDH_AVG(6); break;
+      default: CGFDH_AVG(22); break;
+      }
+#undef CGFDH_AVG
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpy(&bad, dbad.p, sizeof bad, hipMemcpyDeviceToHost));
+      if (bad != none) return report(bad, true);
+    }
+  }
+  if (target == CGFDH_L2M_EL_ANISO && O.nn < 22) {
+    hipLaunchKernelGGL(layer_expand_kernel, g, dim3(256), 0, 0, O.nn == 6 ? 1 : 0, O.out, npts);
+    HIPCHK(hipGetLastError());
+  }
+  for (int q = 0; q < nout; q++)
+    HIPCHK(hipMemcpy(out[q], O.out + (size_t)q * n, n * sizeof(float), hipMemcpyDeviceToHost));
+  return CGFDH_OK;
+}
+
 extern "C" int cgfdh_layer2model(int nprop, int n_if, int mx, int my, const float* geo,
                                  const float* fields, long npts, const float* x, const float* y,
                                  const float* z, float* a, float* b, float* c)
 {
-  const char* who = "cgfdh_layer2model";
-  if (!geo || !fields || !x || !y || !z || !a || (nprop == 3 && (!b || !c)))
-    return fail(CGFDH_EINVAL, "%s: null argument", who);
-  if (nprop != 1 && nprop != 3) return fail(CGFDH_EINVAL, "%s: nprop %d (1 or 3)", who, nprop);
-  if (n_if < 1 || n_if > MEDIA_MAX_IF || mx < 2 || my < 2 || npts < 0)
-    return fail(CGFDH_EINVAL, "%s: %d interfaces of %dx%d samples, %ld points", who, n_if, mx,
-                my, npts);
-  if (npts == 0) return CGFDH_OK;
-  const size_t nf = 1 + 3 * (size_t)nprop, vol = (size_t)n_if * mx * my;
-  DevBuf df, dx, dout, dbad;
-  HIPCHK(hipMalloc(&df.p, nf * vol * sizeof(float)));
-  HIPCHK(hipMemcpy(df.p, fields, nf * vol * sizeof(float), hipMemcpyHostToDevice));
-  HIPCHK(hipMalloc(&dx.p, 3 * (size_t)npts * sizeof(float)));
-  float* d = dx.as<float>();
-  HIPCHK(hipMemcpy(d, x, npts * sizeof(float), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(d + npts, y, npts * sizeof(float), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(d + 2 * npts, z, npts * sizeof(float), hipMemcpyHostToDevice));
-  HIPCHK(hipMalloc(&dout.p, (size_t)nprop * npts * sizeof(float)));
-  HIPCHK(hipMalloc(&dbad.p, sizeof(unsigned long long)));
-  const unsigned long long none = ~0ull;
-  HIPCHK(hipMemcpy(dbad.p, &none, sizeof none, hipMemcpyHostToDevice));
-  LayerModel M;
-  M.f = df.as<float>();
-  M.ni = n_if; M.mx = mx; M.my = my; M.nprop = nprop;
-  M.minx = geo[0]; M.miny = geo[1]; M.dx = geo[2]; M.dy = geo[3];
-  M.x = d; M.y = d + npts; M.z = d + 2 * npts;
-  M.n = npts;
-  M.a = dout.as<float>();
-  M.b = nprop == 3 ? M.a + npts : nullptr;
-  M.c = nprop == 3 ? M.a + 2 * npts : nullptr;
-  M.bad = dbad.as<unsigned long long>();
-  hipLaunchKernelGGL(layer2model_kernel, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, 0,
-                     M);
-  HIPCHK(hipGetLastError());
-  unsigned long long bad = 0;
-  HIPCHK(hipMemcpy(&bad, dbad.p, sizeof bad, hipMemcpyDeviceToHost));
-  if (bad != none) {
-    const long p = (long)(bad & ((1ull << 48) - 1));
-    if ((bad >> 48) == 1)
-      return fail(CGFDH_EINVAL, "%s: point %ld (%g, %g) is outside the interface mesh", who, p,
-                  (double)x[p], (double)y[p]);
-    return fail(CGFDH_EINVAL, "%s: point %ld has vp <= sqrt(2) vs", who, p);
-  }
-  HIPCHK(hipMemcpy(a, M.a, npts * sizeof(float), hipMemcpyDeviceToHost));
-  if (nprop == 3) {
-    HIPCHK(hipMemcpy(b, M.b, npts * sizeof(float), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(c, M.c, npts * sizeof(float), hipMemcpyDeviceToHost));
-  }
-  return CGFDH_OK;
+  if (nprop != 1 && nprop != 3)
+    return fail(CGFDH_EINVAL, "cgfdh_layer2model: nprop %d (1 or 3)", nprop);
+  float* out[3] = {a, b, c};
+  return layer2model_run("cgfdh_layer2model", nprop == 1 ? MD_ONE : MD_ISO,
+                         nprop == 1 ? CGFDH_L2M_ONECMP : CGFDH_L2M_EL_ISO, CGFDH_L2M_LOC, n_if,
+                         mx, my, geo, fields, 0, 0, 0, 0, npts, x, y, z, out);
+}
+
+extern "C" int cgfdh_layer2model_grid(int media_type, int target, int method, int n_if, int mx,
+                                      int my, const float* geo, const float* fields, int nx,
+                                      int ny, int nz, const float* x3d, const float* y3d,
+                                      const float* z3d, float* const* out)
+{
+  return layer2model_run("cgfdh_layer2model_grid", media_type, target, method, n_if, mx, my,
+                         geo, fields, 1, nx, ny, nz, (long)nx * ny * nz, x3d, y3d, z3d, out);
+}
+
+// the reference's file-level entry points: read the .md3lay file, check
+// the grid type and method names, run on the device
+static int layer2model_file(const char* who, int target, float* const* out, const float* x3d,
+                            const float* y3d, const float* z3d, size_t nx, size_t ny, size_t nz,
+                            int grid_type, const char* path, const char* method_name)
+{
+  if (!path || !method_name) return fail(CGFDH_EINVAL, "%s: null argument", who);
+  if (grid_type != 3)
+    return fail(CGFDH_EINVAL, "%s: grid type %d (only the curvilinear grid, 3, is provided)", who,
+                grid_type);
+  int method;
+  if (strcmp(method_name, "loc") == 0) method = CGFDH_L2M_LOC;
+  else if (strcmp(method_name, "har") == 0) method = CGFDH_L2M_HAR;
+  else if (strcmp(method_name, "ari") == 0) method = CGFDH_L2M_ARI;
+  else return fail(CGFDH_EINVAL, "%s: equivalent medium method %s is not provided (loc, har, ari)",
+                   who, method_name);
+  if (nx > (size_t)INT_MAX || ny > (size_t)INT_MAX || nz > (size_t)INT_MAX)
+    return fail(CGFDH_EINVAL, "%s: grid too large", who);
+  int mt, nprop, n_if, mx, my;
+  float geo[4];
+  if (cgfdh_io_md3lay_info(path, &mt, &nprop, &n_if, &mx, &my, geo))
+    return fail(CGFDH_EINVAL, "%s: %s", who, cgfdh_io_last_error());
+  int slot[22];
+  if (layer_slots(mt, target, 0, slot) < 0)
+    return fail(CGFDH_EINVAL, "%s: media type %d of %s is not supported by this function", who,
+                mt, path);
+  std::vector<float> fields((size_t)(1 + 3 * nprop) * n_if * mx * my);
+  if (cgfdh_io_read_md3lay(path, fields.data()))
+    return fail(CGFDH_EINVAL, "%s: %s", who, cgfdh_io_last_error());
+  return layer2model_run(who, mt, target, method, n_if, mx, my, geo, fields.data(), 1, (int)nx,
+                         (int)ny, (int)nz, (long)(nx * ny * nz), x3d, y3d, z3d, out);
+}
+
+extern "C" int cgfdh_media_layer2model_onecmp(float* var3d, const float* x3d, const float* y3d,
+                                              const float* z3d, size_t nx, size_t ny, size_t nz,
+                                              int grid_type, const char* in_var_file,
+                                              const char* average_method, int myid)
+{
+  (void)myid;
+  float* out[1] = {var3d};
+  return layer2model_file("cgfdh_media_layer2model_onecmp", CGFDH_L2M_ONECMP, out, x3d, y3d, z3d,
+                          nx, ny, nz, grid_type, in_var_file, average_method);
+}
+
+extern "C" int cgfdh_media_layer2model_el_iso(float* lam3d, float* mu3d, float* rho3d,
+                                              const float* x3d, const float* y3d,
+                                              const float* z3d, size_t nx, size_t ny, size_t nz,
+                                              int grid_type, const char* in_3lay_file,
+                                              const char* equivalent_medium_method, int myid)
+{
+  (void)myid;
+  float* out[3] = {lam3d, mu3d, rho3d};
+  return layer2model_file("cgfdh_media_layer2model_el_iso", CGFDH_L2M_EL_ISO, out, x3d, y3d, z3d,
+                          nx, ny, nz, grid_type, in_3lay_file, equivalent_medium_method);
+}
+
+extern "C" int cgfdh_media_layer2model_el_vti(float* rho, float* c11, float* c33, float* c55,
+                                              float* c66, float* c13, const float* x3d,
+                                              const float* y3d, const float* z3d, size_t nx,
+                                              size_t ny, size_t nz, int grid_type,
+                                              const char* in_3lay_file,
+                                              const char* equivalent_medium_method, int myid)
+{
+  (void)myid;
+  float* out[6] = {rho, c11, c33, c55, c66, c13};
+  return layer2model_file("cgfdh_media_layer2model_el_vti", CGFDH_L2M_EL_VTI, out, x3d, y3d, z3d,
+                          nx, ny, nz, grid_type, in_3lay_file, equivalent_medium_method);
+}
+
+extern "C" int cgfdh_media_layer2model_el_aniso(
+    float* rho, float* c11, float* c12, float* c13, float* c14, float* c15, float* c16,
+    float* c22, float* c23, float* c24, float* c25, float* c26, float* c33, float* c34,
+    float* c35, float* c36, float* c44, float* c45, float* c46, float* c55, float* c56,
+    float* c66, const float* x3d, const float* y3d, const float* z3d, size_t nx, size_t ny,
+    size_t nz, int grid_type, const char* in_3lay_file, const char* equivalent_medium_method,
+    int myid)
+{
+  (void)myid;
+  float* out[22] = {rho, c11, c12, c13, c14, c15, c16, c22, c23, c24, c25,
+                    c26, c33, c34, c35, c36, c44, c45, c46, c55, c56, c66};
+  return layer2model_file("cgfdh_media_layer2model_el_aniso", CGFDH_L2M_EL_ANISO, out, x3d, y3d,
+                          z3d, nx, ny, nz, grid_type, in_3lay_file, equivalent_medium_method);
+}
+
+extern "C" int cgfdh_grid2model_grid(int media_type, int target, int method, int n_layers,
+                                     const int* ngz, int mx, int my, const float* geo,
+                                     const float* fields, int nx, int ny, int nz,
+                                     const float* x3d, const float* y3d, const float* z3d,
+                                     float* const* out)
+{
+  const char* who = "cgfdh_grid2model_grid";
+  if (!ngz || n_layers < 1) return fail(CGFDH_EINVAL, "%s: %d layers", who, n_layers);
+  long ns = 0;
+  for (int l = 0; l < n_layers; l++) ns += ngz[l];
+  if (ns < 1 || ns > MEDIA_MAX_SURF) return fail(CGFDH_EINVAL, "%s: %ld surfaces", who, ns);
+  return layer2model_run(who, media_type, target, method, (int)ns, mx, my, geo, fields, 1, nx, ny,
+                         nz, (long)nx * ny * nz, x3d, y3d, z3d, out, n_layers, ngz);
+}
+
+// the reference's grid-model file entry points: read the .md3grd window
+// covering [Xmin, Xmax] x [Ymin, Ymax], check the grid type and method
+// names, run on the device
+static int grid2model_file(const char* who, int target, float* const* out, const float* x3d,
+                           const float* y3d, const float* z3d, int nx, int ny, int nz,
+                           float xmin, float xmax, float ymin, float ymax, int grid_type,
+                           const char* path, const char* method_name)
+{
+  if (!path || !method_name) return fail(CGFDH_EINVAL, "%s: null argument", who);
+  if (grid_type != 3)
+    return fail(CGFDH_EINVAL, "%s: grid type %d (only the curvilinear grid, 3, is provided)", who,
+                grid_type);
+  int method;
+  if (strcmp(method_name, "loc") == 0) method = CGFDH_L2M_LOC;
+  else if (strcmp(method_name, "har") == 0) method = CGFDH_L2M_HAR;
+  else if (strcmp(method_name, "ari") == 0) method = CGFDH_L2M_ARI;
+  else return fail(CGFDH_EINVAL, "%s: equivalent medium method %s is not provided (loc, har, ari)",
+                   who, method_name);
+  int mt, nprop, nl, ns, mx, my;
+  std::vector<int> ngz(MEDIA_MAX_LAYERS);
+  float geo[4];
+  if (cgfdh_io_md3grd_info(path, xmin, xmax, ymin, ymax, &mt, &nprop, &nl, ngz.data(),
+                           MEDIA_MAX_LAYERS, &ns, &mx, &my, geo))
+    return fail(CGFDH_EINVAL, "%s: %s", who, cgfdh_io_last_error());
+  int slot[22];
+  if (layer_slots(mt, target, 1, slot) < 0)
+    return fail(CGFDH_EINVAL, "%s: media type %d of %s is not supported by this function", who,
+                mt, path);
+  std::vector<float> fields((size_t)(1 + nprop) * ns * mx * my);
+  if (cgfdh_io_read_md3grd(path, xmin, xmax, ymin, ymax, fields.data()))
+    return fail(CGFDH_EINVAL, "%s: %s", who, cgfdh_io_last_error());
+  if (nx < 1 || ny < 1 || nz < 1) return fail(CGFDH_EINVAL, "%s: grid %dx%dx%d", who, nx, ny, nz);
+  return layer2model_run(who, mt, target, method, ns, mx, my, geo, fields.data(), 1, nx, ny, nz,
+                         (long)nx * ny * nz, x3d, y3d, z3d, out, nl, ngz.data());
+}
+
+extern "C" int cgfdh_media_grid2model_onecmp(float* var3d, const float* x3d, const float* y3d,
+                                             const float* z3d, int nx, int ny, int nz, float Xmin,
+                                             float Xmax, float Ymin, float Ymax, int grid_type,
+                                             const char* in_media_file,
+                                             const char* equivalent_medium_method, int myid)
+{
+  (void)myid;
+  float* out[1] = {var3d};
+  return grid2model_file("cgfdh_media_grid2model_onecmp", CGFDH_L2M_ONECMP, out, x3d, y3d, z3d,
+                         nx, ny, nz, Xmin, Xmax, Ymin, Ymax, grid_type, in_media_file,
+                         equivalent_medium_method);
+}
+
+extern "C" int cgfdh_media_grid2model_el_iso(float* rho3d, float* lam3d, float* mu3d,
+                                             const float* x3d, const float* y3d,
+                                             const float* z3d, int nx, int ny, int nz, float Xmin,
+                                             float Xmax, float Ymin, float Ymax, int grid_type,
+                                             const char* in_media_file,
+                                             const char* equivalent_medium_method, int myid)
+{
+  (void)myid;
+  float* out[3] = {rho3d, lam3d, mu3d};
+  return grid2model_file("cgfdh_media_grid2model_el_iso", CGFDH_L2M_EL_ISO, out, x3d, y3d, z3d,
+                         nx, ny, nz, Xmin, Xmax, Ymin, Ymax, grid_type, in_media_file,
+                         equivalent_medium_method);
+}
+
+extern "C" int cgfdh_media_grid2model_el_vti(float* rho, float* c11, float* c33, float* c55,
+                                             float* c66, float* c13, const float* x3d,
+                                             const float* y3d, const float* z3d, int nx, int ny,
+                                             int nz, float Xmin, float Xmax, float Ymin,
+                                             float Ymax, int grid_type, const char* in_media_file,
+                                             const char* equivalent_medium_method, int myid)
+{
+  (void)myid;
+  float* out[6] = {rho, c11, c33, c55, c66, c13};
+  return grid2model_file("cgfdh_media_grid2model_el_vti", CGFDH_L2M_EL_VTI, out, x3d, y3d, z3d,
+                         nx, ny, nz, Xmin, Xmax, Ymin, Ymax, grid_type, in_media_file,
+                         equivalent_medium_method);
+}
+
+extern "C" int cgfdh_media_grid2model_el_aniso(
+    float* rho, float* c11, float* c12, float* c13, float* c14, float* c15, float* c16,
+    float* c22, float* c23, float* c24, float* c25, float* c26, float* c33, float* c34,
+    float* c35, float* c36, float* c44, float* c45, float* c46, float* c55, float* c56,
+    float* c66, const float* x3d, const float* y3d, const float* z3d, int nx, int ny, int nz,
+    float Xmin, float Xmax, float Ymin, float Ymax, int grid_type, const char* in_media_file,
+    const char* equivalent_medium_method, int myid)
+{
+  (void)myid;
+  float* out[22] = {rho, c11, c12, c13, c14, c15, c16, c22, c23, c24, c25,
+                    c26, c33, c34, c35, c36, c44, c45, c46, c55, c56, c66};
+  return grid2model_file("cgfdh_media_grid2model_el_aniso", CGFDH_L2M_EL_ANISO, out, x3d, y3d,
+                         z3d, nx, ny, nz, Xmin, Xmax, Ymin, Ymax, grid_type, in_media_file,
+                         equivalent_medium_method);
 }
 
 // ---------------------------------------------------------------------------
@@ -2483,7 +2804,7 @@ extern "C" int cgfdh_batch_autotune(cgfdh_batch* b, int it0)
     const bool iso = s->d.medium_type == CGFDH_MEDIUM_ELASTIC_ISO;
     const bool visco = s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO;
     std::vector<int> variants = visco ? std::vector<int>{4, 5, 8, 9}
-                                      : std::vector<int>{4, 6, 7, 8, 12, 14};
+                                      : std::vector<int>{4, 6, 7, 8, 12, 14, 16};
     if (iso) variants.push_back(3);
     const int groups[][2] = {{9, 2}, {3, 4}, {9, 8}};
     float best = 1e30f;

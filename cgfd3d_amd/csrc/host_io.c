@@ -13,6 +13,7 @@
  * in variable order.
  */
 #include <errno.h>
+#include <math.h>
 #include <stdarg.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -709,8 +710,9 @@ int cgfdh_io_read_grid_layer(const char *path, int *ncell, int *equal, float *la
  * read_interface_file (media_read_file.cpp:22-420): lines starting with '#'
  * or empty are skipped; then the media type, the number of interfaces, the
  * mesh (nx ny minx miny dx dy) and per interface, per sample (x fastest) the
- * elevation followed by value, gradient and power of each property (one
- * component: var; elastic isotropic: rho, vp, vs). */
+ * elevation followed by value, gradient and power of each property of the
+ * type (media_read_file.cpp:110-620; the type names and codes of
+ * media_utility.hpp:37-45). */
 typedef struct {
   char *text;
   char *pos;
@@ -756,14 +758,27 @@ static int md_next(md_tokens *t, char *tok, size_t cap)
   return 1;
 }
 
-static int md_head(const char *path, md_tokens *t, int *nprop, int *n_if, int *mx, int *my,
-                   float *geo)
+static const struct {
+  const char *name;
+  int code, nprop;
+} md_types[] = {{"one_component", 0, 1},        {"acoustic_isotropic", 10, 2},
+                {"elastic_isotropic", 21, 3},    {"elastic_vti_prem", 22, 6},
+                {"elastic_vti_thomsen", 23, 6},  {"elastic_vti_cij", 24, 6},
+                {"elastic_tti_thomsen", 25, 8},  {"elastic_tti_bond", 26, 8},
+                {"elastic_aniso_cij", 27, 22}};
+
+static int md_head(const char *path, md_tokens *t, int *mtype, int *nprop, int *n_if, int *mx,
+                   int *my, float *geo)
 {
   char tok[256];
   if (!md_next(t, tok, sizeof tok)) return io_fail("md3lay file %s: no media type", path);
-  if (strcmp(tok, "one_component") == 0) *nprop = 1;
-  else if (strcmp(tok, "elastic_isotropic") == 0) *nprop = 3;
-  else return io_fail("md3lay file %s: media type %s not supported here", path, tok);
+  *nprop = 0;
+  for (size_t q = 0; q < sizeof md_types / sizeof md_types[0]; q++)
+    if (strcmp(tok, md_types[q].name) == 0) {
+      *mtype = md_types[q].code;
+      *nprop = md_types[q].nprop;
+    }
+  if (!*nprop) return io_fail("md3lay file %s: media type %s is not supported", path, tok);
   int iv[3];
   for (int k = 0; k < 3; k++) {
     char *e;
@@ -780,15 +795,22 @@ static int md_head(const char *path, md_tokens *t, int *nprop, int *n_if, int *m
   return 0;
 }
 
-int cgfdh_io_md3lay_dims(const char *path, int *nprop, int *n_interfaces, int *nx, int *ny,
-                         float *geo)
+int cgfdh_io_md3lay_info(const char *path, int *media_type, int *nprop, int *n_interfaces,
+                         int *nx, int *ny, float *geo)
 {
   md_tokens t;
   int rc = md_load(path, &t);
   if (rc) return rc;
-  rc = md_head(path, &t, nprop, n_interfaces, nx, ny, geo);
+  rc = md_head(path, &t, media_type, nprop, n_interfaces, nx, ny, geo);
   free(t.text);
   return rc;
+}
+
+int cgfdh_io_md3lay_dims(const char *path, int *nprop, int *n_interfaces, int *nx, int *ny,
+                         float *geo)
+{
+  int mt;
+  return cgfdh_io_md3lay_info(path, &mt, nprop, n_interfaces, nx, ny, geo);
 }
 
 int cgfdh_io_read_md3lay(const char *path, float *fields)
@@ -796,9 +818,9 @@ int cgfdh_io_read_md3lay(const char *path, float *fields)
   md_tokens t;
   int rc = md_load(path, &t);
   if (rc) return rc;
-  int nprop, n_if, mx, my;
+  int mt, nprop, n_if, mx, my;
   float geo[4];
-  rc = md_head(path, &t, &nprop, &n_if, &mx, &my, geo);
+  rc = md_head(path, &t, &mt, &nprop, &n_if, &mx, &my, geo);
   const int nf = 1 + 3 * nprop;
   const size_t vol = (size_t)n_if * mx * my;
   char tok[256];
@@ -810,5 +832,135 @@ int cgfdh_io_read_md3lay(const char *path, float *fields)
         fields[f * vol + p] = strtof(tok, NULL);
     }
   free(t.text);
+  return rc;
+}
+
+/* ---- the grid-model file of media_grid2model (.md3grd) -------------------
+ * read_grid_file (media_read_file.cpp:670-1320): the media type, the number
+ * of layers NL and the surfaces of each, the mesh (NX NY MINX MINY DX DY),
+ * then per surface, per sample (x fastest) the elevation followed by the
+ * property values.  Only the samples of the window covering [xmin, xmax] x
+ * [ymin, ymax] are kept: ix0 = (xmin - MINX) / DX truncated, ix1 =
+ * ceil((xmax - MINX) / DX), the kept mesh starting at ix0 * DX + MINX (the
+ * reference's float arithmetic, so interpolation on the kept mesh rounds
+ * as the reference's does).  A window reaching past the last sample is an
+ * error (the reference would read samples it never filled). */
+#define MG_MAX_LAYERS 4096
+
+static int mg_head(const char *path, md_tokens *t, float xmin, float xmax, float ymin, float ymax,
+                   int *mtype, int *nprop, int *nl, int *ngz, int ngz_cap, int *ni, int *full,
+                   int *win, float *geo)
+{
+  char tok[256];
+  if (!md_next(t, tok, sizeof tok)) return io_fail("md3grd file %s: no media type", path);
+  *nprop = 0;
+  for (size_t q = 0; q < sizeof md_types / sizeof md_types[0]; q++)
+    if (strcmp(tok, md_types[q].name) == 0) {
+      *mtype = md_types[q].code;
+      *nprop = md_types[q].nprop;
+    }
+  if (!*nprop) return io_fail("md3grd file %s: media type %s is not supported", path, tok);
+  if (!md_next(t, tok, sizeof tok)) return io_fail("md3grd file %s: short header", path);
+  *nl = (int)strtol(tok, NULL, 10);
+  if (*nl < 1) return io_fail("md3grd file %s: %d layers", path, *nl);
+  if (ngz && *nl > ngz_cap)
+    return io_fail("md3grd file %s: %d layers, room for %d", path, *nl, ngz_cap);
+  *ni = 0;
+  for (int l = 0; l < *nl; l++) {
+    if (!md_next(t, tok, sizeof tok)) return io_fail("md3grd file %s: short header", path);
+    const int g = (int)strtol(tok, NULL, 10);
+    if (g < 1) return io_fail("md3grd file %s: layer %d has %d surfaces", path, l, g);
+    if (ngz) ngz[l] = g;
+    *ni += g;
+  }
+  int iv[2];
+  float fv[4];
+  for (int k = 0; k < 2; k++) {
+    if (!md_next(t, tok, sizeof tok)) return io_fail("md3grd file %s: short header", path);
+    iv[k] = (int)strtol(tok, NULL, 10);
+  }
+  for (int k = 0; k < 4; k++) {
+    if (!md_next(t, tok, sizeof tok)) return io_fail("md3grd file %s: short header", path);
+    fv[k] = strtof(tok, NULL);
+  }
+  const int NX = iv[0], NY = iv[1];
+  const float MINX = fv[0], MINY = fv[1], DX = fv[2], DY = fv[3];
+  if (NX < 2 || NY < 2) return io_fail("md3grd file %s: %d x %d samples", path, NX, NY);
+  const float fx0 = (xmin - MINX) / DX, fy0 = (ymin - MINY) / DY;
+  const double cx1 = ceil((xmax - MINX) / DX), cy1 = ceil((ymax - MINY) / DY);
+  if (fx0 < 0 || cx1 > NX - 1)
+    return io_fail("md3grd file %s: the media mesh does not cover the grid in x", path);
+  if (fy0 < 0 || cy1 > NY - 1)
+    return io_fail("md3grd file %s: the media mesh does not cover the grid in y", path);
+  const size_t ix0 = (size_t)fx0, iy0 = (size_t)fy0, ix1 = (size_t)cx1, iy1 = (size_t)cy1;
+  full[0] = NX;
+  full[1] = NY;
+  win[0] = (int)ix0; win[1] = (int)ix1; win[2] = (int)iy0; win[3] = (int)iy1;
+  geo[0] = ix0 * DX + MINX;
+  geo[1] = iy0 * DY + MINY;
+  geo[2] = DX;
+  geo[3] = DY;
+  return 0;
+}
+
+int cgfdh_io_md3grd_info(const char *path, float xmin, float xmax, float ymin, float ymax,
+                         int *media_type, int *nprop, int *n_layers, int *ngz, int ngz_cap,
+                         int *n_surfaces, int *nx, int *ny, float *geo)
+{
+  md_tokens t;
+  int rc = md_load(path, &t);
+  if (rc) return rc;
+  int full[2], win[4];
+  rc = mg_head(path, &t, xmin, xmax, ymin, ymax, media_type, nprop, n_layers, ngz, ngz_cap,
+               n_surfaces, full, win, geo);
+  if (!rc) {
+    *nx = win[1] - win[0] + 1;
+    *ny = win[3] - win[2] + 1;
+  }
+  free(t.text);
+  return rc;
+}
+
+int cgfdh_io_read_md3grd(const char *path, float xmin, float xmax, float ymin, float ymax,
+                         float *fields)
+{
+  md_tokens t;
+  int rc = md_load(path, &t);
+  if (rc) return rc;
+  int mt, nprop, nl, ni, full[2], win[4];
+  int ngz[MG_MAX_LAYERS];
+  float geo[4];
+  rc = mg_head(path, &t, xmin, xmax, ymin, ymax, &mt, &nprop, &nl, ngz, MG_MAX_LAYERS, &ni, full,
+               win, geo);
+  const int nf = 1 + nprop, nx = win[1] - win[0] + 1, ny = win[3] - win[2] + 1;
+  const size_t vol = (size_t)ni * nx * ny;
+  char tok[256];
+  for (int s = 0; s < ni && !rc; s++)
+    for (int iy = 0; iy < full[1] && !rc; iy++)
+      for (int ix = 0; ix < full[0] && !rc; ix++) {
+        const int keep = ix >= win[0] && ix <= win[1] && iy >= win[2] && iy <= win[3];
+        const size_t p = (size_t)s * nx * ny + (size_t)(iy - win[2]) * nx + (ix - win[0]);
+        for (int f = 0; f < nf && !rc; f++) {
+          if (!md_next(&t, tok, sizeof tok))
+            rc = io_fail("md3grd file %s: insufficient data at surface %d sample (%d, %d)", path,
+                         s, ix, iy);
+          else if (keep)
+            fields[f * vol + p] = strtof(tok, NULL);
+        }
+      }
+  free(t.text);
+  /* checkGridData (media_read_file.cpp:1242-1265): each layer's last
+   * surface lies on the next layer's first */
+  if (!rc && nl > 1) {
+    const size_t sl = (size_t)nx * ny;
+    int s = 0;
+    for (int l = 0; l < nl - 1 && !rc; l++) {
+      s += ngz[l];
+      for (size_t q = 0; q < sl && !rc; q++)
+        if (fields[(size_t)(s - 1) * sl + q] != fields[(size_t)s * sl + q])
+          rc = io_fail("md3grd file %s: the last surface of layer %d is not the first of layer %d",
+                       path, l, l + 1);
+    }
+  }
   return rc;
 }

@@ -34,6 +34,10 @@ def _io():
         L.cgfdh_io_grid_layer_dims.argtypes = [C.c_char_p, _P, _P, _P]
         L.cgfdh_io_read_grid_layer.argtypes = [C.c_char_p, _P, _P, _P]
         L.cgfdh_io_md3lay_dims.argtypes = [C.c_char_p, _P, _P, _P, _P, _P]
+        L.cgfdh_io_md3lay_info.argtypes = [C.c_char_p, _P, _P, _P, _P, _P, _P]
+        L.cgfdh_io_md3grd_info.argtypes = ([C.c_char_p] + [C.c_float] * 4 + [_P] * 4 +
+                                           [C.c_int] + [_P] * 4)
+        L.cgfdh_io_read_md3grd.argtypes = [C.c_char_p] + [C.c_float] * 4 + [_P]
         L.cgfdh_io_read_md3lay.argtypes = [C.c_char_p, _P]
         L._io_sigs = True
     return L
@@ -163,14 +167,35 @@ def read_grid_layer(path):
     return ncell, equal, xyz
 
 
-def read_md3lay(path):
-    """The layered-model file media_layer2model reads (one_component or
-    elastic_isotropic): (nprop, fields [1+3 nprop][n_interfaces][ny][nx],
-    geo = minx, miny, dx, dy)."""
-    nprop, n_if, nx, ny = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+def read_md3lay(path, with_type=False):
+    """The layered-model file media_layer2model reads (every media type):
+    (nprop, fields [1+3 nprop][n_interfaces][ny][nx], geo = minx, miny, dx,
+    dy), with the type code first when with_type."""
+    mt, nprop, n_if, nx, ny = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
     geo = np.zeros(4, np.float32)
-    _check(_io().cgfdh_io_md3lay_dims(path.encode(), C.byref(nprop), C.byref(n_if), C.byref(nx),
-                                      C.byref(ny), geo.ctypes.data_as(_P)), "md3lay file")
+    _check(_io().cgfdh_io_md3lay_info(path.encode(), C.byref(mt), C.byref(nprop), C.byref(n_if),
+                                      C.byref(nx), C.byref(ny), geo.ctypes.data_as(_P)),
+           "md3lay file")
     fields = np.zeros((1 + 3 * nprop.value, n_if.value, ny.value, nx.value), np.float32)
     _check(_io().cgfdh_io_read_md3lay(path.encode(), fields.ctypes.data_as(_P)), "md3lay file")
+    if with_type:
+        return mt.value, nprop.value, fields, geo
     return nprop.value, fields, geo
+
+
+def read_md3grd(path, xmin, xmax, ymin, ymax):
+    """The grid-model file media_grid2model reads, cropped to the window
+    covering [xmin, xmax] x [ymin, ymax] as the reference crops it: (type
+    code, nprop, surfaces per layer, fields [1+nprop][surfaces][ny][nx], geo
+    of the cropped mesh)."""
+    mt, nprop, nl, ns, nx, ny = (C.c_int() for _ in range(6))
+    ngz = np.zeros(4096, np.int32)
+    geo = np.zeros(4, np.float32)
+    _check(_io().cgfdh_io_md3grd_info(path.encode(), xmin, xmax, ymin, ymax, C.byref(mt),
+                                      C.byref(nprop), C.byref(nl), ngz.ctypes.data_as(_P),
+                                      len(ngz), C.byref(ns), C.byref(nx), C.byref(ny),
+                                      geo.ctypes.data_as(_P)), "md3grd file")
+    fields = np.zeros((1 + nprop.value, ns.value, ny.value, nx.value), np.float32)
+    _check(_io().cgfdh_io_read_md3grd(path.encode(), xmin, xmax, ymin, ymax,
+                                      fields.ctypes.data_as(_P)), "md3grd file")
+    return mt.value, nprop.value, [int(g) for g in ngz[:nl.value]], fields, geo

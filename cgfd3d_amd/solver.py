@@ -104,6 +104,85 @@ def layer2model(nprop, fields, geo, xyz):
     return outs
 
 
+# targets and methods of the layered-model discretisation
+L2M_ONECMP, L2M_EL_ISO, L2M_EL_VTI, L2M_EL_ANISO = 0, 1, 2, 3
+_L2M_NOUT = {L2M_ONECMP: 1, L2M_EL_ISO: 3, L2M_EL_VTI: 6, L2M_EL_ANISO: 22}
+_L2M_METHODS = {"loc": 0, "har": 1, "ari": 2}
+_G2M_FILE_FN = {L2M_ONECMP: "cgfdh_media_grid2model_onecmp",
+                L2M_EL_ISO: "cgfdh_media_grid2model_el_iso",
+                L2M_EL_VTI: "cgfdh_media_grid2model_el_vti",
+                L2M_EL_ANISO: "cgfdh_media_grid2model_el_aniso"}
+_L2M_FILE_FN = {L2M_ONECMP: "cgfdh_media_layer2model_onecmp",
+                L2M_EL_ISO: "cgfdh_media_layer2model_el_iso",
+                L2M_EL_VTI: "cgfdh_media_layer2model_el_vti",
+                L2M_EL_ANISO: "cgfdh_media_layer2model_el_aniso"}
+
+
+def layer2model_grid(media_type, target, method, fields, geo, xyz):
+    """A layered model onto a curvilinear grid on the GPU
+    (cgfdh_layer2model_grid): media_type, fields and geo as
+    output.read_md3lay(path, with_type=True) gives them, target one of
+    L2M_ONECMP / _EL_ISO / _EL_VTI / _EL_ANISO, method "loc", "har" or "ari",
+    xyz (3, nz, ny, nx).  Returns the target's arrays (var | lambda, mu, rho |
+    rho, c11, c33, c55, c66, c13 | rho, c11, c12, .., c66), each (nz, ny, nx)."""
+    f = np.ascontiguousarray(fields, np.float32)
+    g = np.ascontiguousarray(geo, np.float32)
+    x, y, z = (np.ascontiguousarray(xyz[d], np.float32) for d in range(3))
+    nz, ny, nx = x.shape
+    outs = [np.zeros(x.shape, np.float32) for _ in range(_L2M_NOUT[target])]
+    arr = (C.c_void_p * len(outs))(*[o.ctypes.data for o in outs])
+    check(lib().cgfdh_layer2model_grid(media_type, target, _L2M_METHODS.get(method, -1),
+                                       f.shape[1], f.shape[3], f.shape[2], _p(g), _p(f), nx, ny,
+                                       nz, _p(x), _p(y), _p(z), C.cast(arr, C.c_void_p)),
+          "layer2model_grid")
+    return outs
+
+
+def media_layer2model(target, path, xyz, method="loc", grid_type=3):
+    """The reference-shaped file entry points cgfdh_media_layer2model_onecmp /
+    _el_iso / _el_vti / _el_aniso: the .md3lay file at path onto the grid
+    xyz (3, nz, ny, nx).  Returns the target's arrays as layer2model_grid."""
+    x, y, z = (np.ascontiguousarray(xyz[d], np.float32) for d in range(3))
+    nz, ny, nx = x.shape
+    outs = [np.zeros(x.shape, np.float32) for _ in range(_L2M_NOUT[target])]
+    fn = getattr(lib(), _L2M_FILE_FN[target])
+    check(fn(*[_p(o) for o in outs], _p(x), _p(y), _p(z), nx, ny, nz, grid_type, path.encode(),
+             method.encode(), 0), "media_layer2model")
+    return outs
+
+
+def grid2model_grid(media_type, target, method, ngz, fields, geo, xyz):
+    """A grid model (.md3grd) onto a curvilinear grid on the GPU
+    (cgfdh_grid2model_grid): media_type, ngz, fields and geo as
+    output.read_md3grd gives them, target and method as layer2model_grid
+    (the el_iso target returns rho, lambda, mu here, the reference's order)."""
+    f = np.ascontiguousarray(fields, np.float32)
+    g = np.ascontiguousarray(geo, np.float32)
+    gz = np.ascontiguousarray(ngz, np.int32)
+    x, y, z = (np.ascontiguousarray(xyz[d], np.float32) for d in range(3))
+    nz, ny, nx = x.shape
+    outs = [np.zeros(x.shape, np.float32) for _ in range(_L2M_NOUT[target])]
+    arr = (C.c_void_p * len(outs))(*[o.ctypes.data for o in outs])
+    check(lib().cgfdh_grid2model_grid(media_type, target, _L2M_METHODS.get(method, -1), len(gz),
+                                      _p(gz), f.shape[3], f.shape[2], _p(g), _p(f), nx, ny, nz,
+                                      _p(x), _p(y), _p(z), C.cast(arr, C.c_void_p)),
+          "grid2model_grid")
+    return outs
+
+
+def media_grid2model(target, path, xyz, extent, method="loc", grid_type=3):
+    """The reference-shaped file entry points cgfdh_media_grid2model_*: the
+    .md3grd file at path onto the grid xyz (3, nz, ny, nx) whose x/y extent
+    is (xmin, xmax, ymin, ymax)."""
+    x, y, z = (np.ascontiguousarray(xyz[d], np.float32) for d in range(3))
+    nz, ny, nx = x.shape
+    outs = [np.zeros(x.shape, np.float32) for _ in range(_L2M_NOUT[target])]
+    fn = getattr(lib(), _G2M_FILE_FN[target])
+    check(fn(*[_p(o) for o in outs], _p(x), _p(y), _p(z), nx, ny, nz, *[float(e) for e in extent],
+             grid_type, path.encode(), method.encode(), 0), "media_grid2model")
+    return outs
+
+
 class PinnedBuffer:
     """Page-locked host memory (cgfdh_host_alloc) viewed as a float32 array,
     the destination that lets cgfdh_subvolume_async copies overlap steps."""

@@ -345,11 +345,124 @@ int cgfdh_gen_grid_layer(int n_if, const int *ncell, const int *equal, int nx_la
  * values on an interface, the top layer's values above the surface).
  * Outputs var into a, or lambda, mu, rho into a, b, c (mu = vs^2 rho,
  * lambda = vp^2 rho - 2 mu).  A point outside the mesh, or with
- * vp <= sqrt(2) vs, is an error (the reference stops there too).  The
- * harmonic / arithmetic averaging methods are not provided. */
+ * vp <= sqrt(2) vs, is an error (the reference stops there too).  Points
+ * need no grid here; the averaging methods and the other media types are
+ * cgfdh_layer2model_grid's. */
 int cgfdh_layer2model(int nprop, int n_if, int mx, int my, const float *geo,
                       const float *fields, long npts, const float *x, const float *y,
                       const float *z, float *a, float *b, float *c);
+
+/* targets of cgfdh_layer2model_grid: the reference entry point restated */
+#define CGFDH_L2M_ONECMP 0   /* media_layer2model_onecmp: var */
+#define CGFDH_L2M_EL_ISO 1   /* media_layer2model_el_iso: lambda, mu, rho */
+#define CGFDH_L2M_EL_VTI 2   /* media_layer2model_el_vti: rho, c11, c33, c55, c66, c13 */
+#define CGFDH_L2M_EL_ANISO 3 /* media_layer2model_el_aniso: rho, c11, c12, .. c66 (22) */
+/* equivalent medium methods */
+#define CGFDH_L2M_LOC 0
+#define CGFDH_L2M_HAR 1
+#define CGFDH_L2M_ARI 2
+
+/* A layered model discretised onto a curvilinear grid on the GPU, every
+ * media type and method of media_layer2model_{onecmp,el_iso,el_vti,el_aniso}
+ * (media_layer2model.cpp:47-420, with grid type GRID_CURV).  media_type is
+ * the file's type code (cgfdh_io_md3lay_info), fields its samples; the grid
+ * has nx*ny*nz points (x fastest; x3d, y3d, z3d with ghosts).  out holds
+ * the target's arrays in the reference function's argument order (1, 3, 6
+ * or 22 host arrays of nx*ny*nz floats).
+ *   loc: each point's own value (the point value of its layer, the mean of
+ *        both layers' on an interface, the top layer's above the surface),
+ *        converted to lambda/mu, VTI or rotated TTI stiffnesses (para2vti,
+ *        para2tti with BondTransform);
+ *   har, ari: a point (1..n-2 in each direction) whose surrounding half-grid
+ *        cell spans layers takes the harmonic (stiffness; kappa and mu for
+ *        the isotropic medium) or arithmetic average over the cell's 9^3
+ *        sub-points, density always arithmetic (the reference's NG = 8).
+ * el_aniso also takes isotropic and VTI files and fills the other
+ * stiffnesses as the reference does.  Errors (the reference exits): a
+ * media type the target does not take, a point or sub-point outside the
+ * interface mesh, an isotropic vp <= sqrt(2) vs.  The "tti" method is not
+ * provided (only the isotropic type has one in the reference). */
+int cgfdh_layer2model_grid(int media_type, int target, int method, int n_if, int mx, int my,
+                           const float *geo, const float *fields, int nx, int ny, int nz,
+                           const float *x3d, const float *y3d, const float *z3d,
+                           float *const *out);
+
+/* Drop-in replacements of the reference's media_layer2model_onecmp,
+ * _el_iso, _el_vti and _el_aniso (media_layer2model.cpp:47-420, same
+ * arguments): read the .md3lay file, then cgfdh_layer2model_grid.  grid_type
+ * must be 3 (GRID_CURV, what main_curv_col_el_3d.c:328-380 passes); the
+ * method is "loc", "har" or "ari"; myid is unused.  CGFDH_OK or an error
+ * code with the message in cgfdh_last_error (the reference exits instead). */
+int cgfdh_media_layer2model_onecmp(float *var3d, const float *x3d, const float *y3d,
+                                   const float *z3d, size_t nx, size_t ny, size_t nz,
+                                   int grid_type, const char *in_var_file,
+                                   const char *average_method, int myid);
+int cgfdh_media_layer2model_el_iso(float *lam3d, float *mu3d, float *rho3d, const float *x3d,
+                                   const float *y3d, const float *z3d, size_t nx, size_t ny,
+                                   size_t nz, int grid_type, const char *in_3lay_file,
+                                   const char *equivalent_medium_method, int myid);
+int cgfdh_media_layer2model_el_vti(float *rho, float *c11, float *c33, float *c55, float *c66,
+                                   float *c13, const float *x3d, const float *y3d,
+                                   const float *z3d, size_t nx, size_t ny, size_t nz,
+                                   int grid_type, const char *in_3lay_file,
+                                   const char *equivalent_medium_method, int myid);
+int cgfdh_media_layer2model_el_aniso(float *rho, float *c11, float *c12, float *c13, float *c14,
+                                     float *c15, float *c16, float *c22, float *c23, float *c24,
+                                     float *c25, float *c26, float *c33, float *c34, float *c35,
+                                     float *c36, float *c44, float *c45, float *c46, float *c55,
+                                     float *c56, float *c66, const float *x3d, const float *y3d,
+                                     const float *z3d, size_t nx, size_t ny, size_t nz,
+                                     int grid_type, const char *in_3lay_file,
+                                     const char *equivalent_medium_method, int myid);
+
+/* The grid model of media_grid2model (.md3grd) on a curvilinear grid on
+ * the GPU, every media type and method of media_grid2model_{onecmp,el_iso,
+ * el_vti,el_aniso} (media_grid2model.cpp:29-2800, GRID_CURV): n_layers
+ * layers of ngz[l] surfaces (at most 64 layers, 256 surfaces), fields
+ * [1 + nprop][surfaces][my][mx] and geo of the cropped mesh as
+ * cgfdh_io_read_md3grd gives them.  A point's values are interpolated
+ * linearly in z between the surfaces around it (the top surface's above,
+ * the last's below, the mean of both sides on a boundary between layers);
+ * har / ari as cgfdh_layer2model_grid except that sub-points on a layer
+ * boundary are left out and a single layer keeps the local values.
+ * Outputs in the reference functions' order; note el_iso's is rho, lambda,
+ * mu here (media_grid2model_el_iso:133-150).  Errors as
+ * cgfdh_layer2model_grid. */
+int cgfdh_grid2model_grid(int media_type, int target, int method, int n_layers, const int *ngz,
+                          int mx, int my, const float *geo, const float *fields, int nx, int ny,
+                          int nz, const float *x3d, const float *y3d, const float *z3d,
+                          float *const *out);
+
+/* Drop-in replacements of media_grid2model_onecmp, _el_iso, _el_vti and
+ * _el_aniso (media_grid2model.cpp:29-420, same arguments; Xmin..Ymax the
+ * grid's extent, main_curv_col_el_3d.c:388-460): read the window of the
+ * .md3grd file, then cgfdh_grid2model_grid.  grid_type must be 3; the
+ * method is "loc", "har" or "ari"; myid is unused.  CGFDH_OK or an error
+ * code with the message in cgfdh_last_error (the reference exits). */
+int cgfdh_media_grid2model_onecmp(float *var3d, const float *x3d, const float *y3d,
+                                  const float *z3d, int nx, int ny, int nz, float Xmin, float Xmax,
+                                  float Ymin, float Ymax, int grid_type, const char *in_media_file,
+                                  const char *equivalent_medium_method, int myid);
+int cgfdh_media_grid2model_el_iso(float *rho3d, float *lam3d, float *mu3d, const float *x3d,
+                                  const float *y3d, const float *z3d, int nx, int ny, int nz,
+                                  float Xmin, float Xmax, float Ymin, float Ymax, int grid_type,
+                                  const char *in_media_file, const char *equivalent_medium_method,
+                                  int myid);
+int cgfdh_media_grid2model_el_vti(float *rho, float *c11, float *c33, float *c55, float *c66,
+                                  float *c13, const float *x3d, const float *y3d,
+                                  const float *z3d, int nx, int ny, int nz, float Xmin,
+                                  float Xmax, float Ymin, float Ymax, int grid_type,
+                                  const char *in_media_file, const char *equivalent_medium_method,
+                                  int myid);
+int cgfdh_media_grid2model_el_aniso(float *rho, float *c11, float *c12, float *c13, float *c14,
+                                    float *c15, float *c16, float *c22, float *c23, float *c24,
+                                    float *c25, float *c26, float *c33, float *c34, float *c35,
+                                    float *c36, float *c44, float *c45, float *c46, float *c55,
+                                    float *c56, float *c66, const float *x3d, const float *y3d,
+                                    const float *z3d, int nx, int ny, int nz, float Xmin,
+                                    float Xmax, float Ymin, float Ymax, int grid_type,
+                                    const char *in_media_file,
+                                    const char *equivalent_medium_method, int myid);
 
 /* Stable time step of a curvilinear grid, blk_dt_esti_curv
  * (blk_t.c:2840-2930), computed on the GPU: the minimum over physical points

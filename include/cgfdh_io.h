@@ -128,14 +128,36 @@ int cgfdh_io_grid_layer_dims(const char *path, int *n_interfaces, int *nx_layers
 int cgfdh_io_read_grid_layer(const char *path, int *ncell, int *equal, float *layer_xyz);
 
 /* The layered-model file of media_layer2model (.md3lay, read_interface_file,
- * media_read_file.cpp:22-420) for the one_component and elastic_isotropic
- * types: first its header (nprop 1 or 3, interfaces, samples nx x ny, geo =
- * minx, miny, dx, dy), then its samples into fields
+ * media_read_file.cpp:22-620), every media type: first its header (the type
+ * code of media_utility.hpp:37-45 -- one_component 0, acoustic_isotropic 10,
+ * elastic_isotropic 21, elastic_vti_prem 22, _thomsen 23, _cij 24,
+ * elastic_tti_thomsen 25, elastic_tti_bond 26, elastic_aniso_cij 27 --, the
+ * properties per sample nprop (1, 2, 3, 6, 6, 6, 8, 8, 22), interfaces,
+ * samples nx x ny, geo = minx, miny, dx, dy), then its samples into fields
  * ([1 + 3 nprop][n_interfaces][ny][nx]: elevation, then value, gradient and
- * power of var, or of rho, vp, vs), the input of cgfdh_layer2model. */
+ * power of each property in the file's order), the input of
+ * cgfdh_layer2model_grid.  _dims is _info without the type. */
+int cgfdh_io_md3lay_info(const char *path, int *media_type, int *nprop, int *n_interfaces,
+                         int *nx, int *ny, float *geo);
 int cgfdh_io_md3lay_dims(const char *path, int *nprop, int *n_interfaces, int *nx, int *ny,
                          float *geo);
 int cgfdh_io_read_md3lay(const char *path, float *fields);
+
+/* The grid-model file of media_grid2model (.md3grd, read_grid_file,
+ * media_read_file.cpp:670-1320), every media type: the header (type code
+ * and nprop as for .md3lay, n_layers and the surfaces of each into ngz --
+ * at most ngz_cap --, n_surfaces their sum) and the samples of the window of
+ * the mesh that covers [xmin, xmax] x [ymin, ymax] (the grid's extent, as
+ * the reference takes it): nx x ny samples from geo = minx, miny, dx, dy,
+ * the reference's cropped mesh; fields [1 + nprop][n_surfaces][ny][nx]
+ * (elevation, then each property's value), the input of
+ * cgfdh_grid2model_grid.  A mesh that does not cover the window is an
+ * error. */
+int cgfdh_io_md3grd_info(const char *path, float xmin, float xmax, float ymin, float ymax,
+                         int *media_type, int *nprop, int *n_layers, int *ngz, int ngz_cap,
+                         int *n_surfaces, int *nx, int *ny, float *geo);
+int cgfdh_io_read_md3grd(const char *path, float xmin, float xmax, float ymin, float ymax,
+                         float *fields);
 
 #ifdef __cplusplus
 }

@@ -105,6 +105,11 @@ def load_oracle(fp64=False):
                                   [C.c_int] * 9 + [_P] * 3)
     lib.orc_layer2model.restype = C.c_int
     lib.orc_layer2model.argtypes = [C.c_int, _P, _P, _P, C.c_size_t] + [_P] * 6
+    lib.orc_layer2model_grid.restype = C.c_int
+    lib.orc_layer2model_grid.argtypes = [C.c_int] * 3 + [_P] * 3 + [C.c_int] * 3 + [_P] * 4
+    lib.orc_grid2model_grid.restype = C.c_int
+    lib.orc_grid2model_grid.argtypes = ([C.c_int] * 3 + [_P] * 2 + [C.c_int] + [_P] * 2 +
+                                        [C.c_int] * 3 + [_P] * 4)
     lib.orc_depth_to_axis.restype = C.c_int
     lib.orc_depth_to_axis.argtypes = [C.c_int] * 3 + [_P] * 3 + [C.c_float] * 2 + [_P]
     return lib

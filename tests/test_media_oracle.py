@@ -11,61 +11,22 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as po
-from tests import cases as cs
+from tests import media_models as mm
 
 needs_ref = pytest.mark.skipif(not po.ref_available(), reason="reference build absent")
 
 GRID_CURV = 3
 
 
-def md3lay_model(kind, seed=0, mx=31, my=27, dx=50.0, dy=60.0, x0=-200.0, y0=-300.0):
-    """Three interfaces (top with topography), per-layer values, gradients,
-    powers: fields [nf][3][my][mx], geo (minx, miny, dx, dy)."""
-    rng = np.random.default_rng(seed)
-    X, Y = np.meshgrid(x0 + dx * np.arange(mx), y0 + dy * np.arange(my))
-    top = 150.0 * np.exp(-((X - X.mean()) ** 2 + (Y - Y.mean()) ** 2) / (2 * 400.0 ** 2))
-    elev = np.stack([top + 5.0 * rng.standard_normal(X.shape), -600.0 + 0.05 * X,
-                     -1400.0 + 0.02 * Y])
-    nprop = 1 if kind == "one_component" else 3
-    f = np.zeros((1 + 3 * nprop, 3, my, mx), np.float32)
-    f[0] = elev
-    base = [[2200.0, 2500.0, 2800.0], [3000.0, 4500.0, 5600.0], [1700.0, 2500.0, 3200.0]]
-    grad = [0.2, 0.5, 0.28]                     # rho, vp, vs per metre (vp/vs stays > sqrt 2)
-    for q in range(nprop):
-        for L in range(3):
-            f[1 + 3 * q, L] = base[q][L] * (1 + 0.02 * rng.standard_normal(X.shape))
-            f[2 + 3 * q, L] = grad[q] * (1 + 0.05 * rng.standard_normal(X.shape))
-            f[3 + 3 * q, L] = 1.0 + 0.04 * L + 0.01 * rng.standard_normal(X.shape)
-    return nprop, f.astype(np.float32), (np.float32(x0), np.float32(y0), np.float32(dx),
-                                         np.float32(dy))
+def md3lay_model(kind, seed=0, **kw):
+    """(nprop, fields, geo) of a one-component or isotropic model
+    (tests/media_models.py)."""
+    code, f, geo = mm.model(kind, seed=seed, **kw)
+    return (f.shape[0] - 1) // 3, f, geo
 
 
-def write_md3lay(path, kind, f, geo):
-    nf, ni, my, mx = f.shape
-    with open(path, "w") as fp:
-        fp.write("# layered model written by the tests\n")
-        fp.write(f"{kind}\n{ni}\n{mx} {my} {geo[0]:.9g} {geo[1]:.9g} {geo[2]:.9g} {geo[3]:.9g}\n")
-        fp.write("# samples\n")
-        flat = f.reshape(nf, -1)
-        for p in range(flat.shape[1]):
-            fp.write(" ".join(f"{v:.9g}" for v in flat[:, p]) + "\n")
-
-
-def grid_points(f, geo, ni=18, nj=15, nk=16, seed=1):
-    """A curvilinear grid inside the mesh from 1700 m deep to above the top,
-    with one row of points exactly on the middle interface's samples."""
-    co = cs.curvilinear_coords(ni, nj, nk, dh=40.0, seed=seed, hill=0.2, wiggle=0.1)
-    co[0] += float(geo[0]) + 150.0
-    co[1] += float(geo[1]) + 150.0
-    z = co[2]
-    co[2] = -1700.0 + (z - z.min()) / (z.max() - z.min()) * 2000.0
-    # grid points placed on mesh samples of interface 1 (z exactly its elevation)
-    for n in range(6):
-        i, j = 4 + n, 3 + n
-        co[0, 5, 4, 5 + n] = geo[0] + geo[2] * i
-        co[1, 5, 4, 5 + n] = geo[1] + geo[3] * j
-        co[2, 5, 4, 5 + n] = f[0, 1, j, i]
-    return co.astype(np.float32)
+write_md3lay = mm.write_md3lay
+grid_points = mm.grid_points
 
 
 def _ref_run(kind, path, co):
