@@ -57,6 +57,13 @@ SIGNATURES = [
      [_P] * 9 + [C.c_size_t] * 3 + [C.c_int, C.c_char_p, C.c_char_p, C.c_int]),
     ("cgfdh_media_layer2model_el_aniso", C.c_int,
      [_P] * 25 + [C.c_size_t] * 3 + [C.c_int, C.c_char_p, C.c_char_p, C.c_int]),
+    ("cgfdh_bin2model_grid", C.c_int, [C.c_int] + [_P] * 5 + [C.c_long] + [_P] * 4),
+    ("cgfdh_media_bin2model_el_iso", C.c_int,
+     [_P] * 6 + [C.c_size_t] * 3 + [C.c_float] * 4 + [C.c_int] + [_P] * 4 +
+     [C.c_char_p] * 3),
+    ("cgfdh_media_bin2model_vis_iso", C.c_int,
+     [_P] * 5 + [C.c_size_t] * 3 + [C.c_float] * 4 + [C.c_int] + [_P] * 4 +
+     [C.c_char_p] * 2),
     ("cgfdh_grid2model_grid", C.c_int,
      [C.c_int] * 4 + [_P, C.c_int, C.c_int, _P, _P] + [C.c_int] * 3 + [_P] * 4),
     ("cgfdh_media_grid2model_onecmp", C.c_int,

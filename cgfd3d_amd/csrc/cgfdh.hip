@@ -2475,6 +2475,176 @@ extern "C" int cgfdh_media_layer2model_el_aniso(
                           z3d, nx, ny, nz, grid_type, in_3lay_file, equivalent_medium_method);
 }
 
+extern "C" int cgfdh_bin2model_grid(int nvol, const int* n3, const float* xv, const float* yv,
+                                    const float* zv, const float* vols, long npts, const float* x,
+                                    const float* y, const float* z, float* const* out)
+{
+  const char* who = "cgfdh_bin2model_grid";
+  if (!n3 || !xv || !yv || !zv || !vols || !x || !y || !z || !out)
+    return fail(CGFDH_EINVAL, "%s: null argument", who);
+  if ((nvol != 2 && nvol != 3) || n3[0] < 2 || n3[1] < 2 || n3[2] < 2 || npts < 0)
+    return fail(CGFDH_EINVAL, "%s: %d volumes of %dx%dx%d samples, %ld points (each axis needs "
+                "two samples)", who, nvol, n3[0], n3[1], n3[2], npts);
+  for (int q = 0; q < nvol; q++)
+    if (!out[q]) return fail(CGFDH_EINVAL, "%s: null output %d", who, q);
+  if (npts == 0) return CGFDH_OK;
+  const size_t v3 = (size_t)n3[0] * n3[1] * n3[2], n = npts;
+  DevBuf dvol, dvec, dx, dout;
+  HIPCHK(hipMalloc(&dvol.p, nvol * v3 * sizeof(float)));
+  HIPCHK(hipMemcpy(dvol.p, vols, nvol * v3 * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&dvec.p, ((size_t)n3[0] + n3[1] + n3[2]) * sizeof(float)));
+  float* vec = dvec.as<float>();
+  HIPCHK(hipMemcpy(vec, xv, n3[0] * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(vec + n3[0], yv, n3[1] * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(vec + n3[0] + n3[1], zv, n3[2] * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&dx.p, 3 * n * sizeof(float)));
+  float* d = dx.as<float>();
+  HIPCHK(hipMemcpy(d, x, n * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d + n, y, n * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d + 2 * n, z, n * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&dout.p, nvol * n * sizeof(float)));
+  BinModel B;
+  B.vol = dvol.as<float>();
+  B.xv = vec; B.yv = vec + n3[0]; B.zv = vec + n3[0] + n3[1];
+  B.nx = n3[0]; B.ny = n3[1]; B.nz = n3[2]; B.nvol = nvol;
+  hipLaunchKernelGGL(bin_loc_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, B, d,
+                     d + n, d + 2 * n, npts, dout.as<float>());
+  HIPCHK(hipGetLastError());
+  for (int q = 0; q < nvol; q++)
+    HIPCHK(hipMemcpy(out[q], dout.as<float>() + q * n, n * sizeof(float), hipMemcpyDeviceToHost));
+  return CGFDH_OK;
+}
+
+// the window and mesh vectors of a binary model over the grid's extent
+// (media_bin2model_el_iso :46-140): start / end per stored axis, vectors
+// x0 + (ix_start + i) dx (x, y windowed, z whole); false when the model
+// misses the grid (the reference stops)
+static bool bin_window(const int* order, const int* size, const float* spacing,
+                       const float* origin, float xmin, float xmax, float ymin, float ymax,
+                       float zmin, float zmax, int* start, int* end, std::vector<float>& xv,
+                       std::vector<float>& yv, std::vector<float>& zv, char* why, size_t nwhy)
+{
+  int dim[3] = {-1, -1, -1};
+  for (int a = 0; a < 3; a++) {
+    if (order[a] < 0 || order[a] > 2) {
+      snprintf(why, nwhy, "bin_order[%d] = %d (0, 1 or 2)", a, order[a]);
+      return false;
+    }
+    dim[order[a]] = a;
+  }
+  if (dim[0] < 0 || dim[1] < 0 || dim[2] < 0) {
+    snprintf(why, nwhy, "bin_order %d,%d,%d misses a direction", order[0], order[1], order[2]);
+    return false;
+  }
+  const float x0 = origin[dim[0]], y0 = origin[dim[1]], z0 = origin[dim[2]];
+  const float dx = spacing[dim[0]], dy = spacing[dim[1]], dz = spacing[dim[2]];
+  const int ix_start = (int)(xmin < x0 ? 0.0f : (xmin - x0) / dx);
+  if (ix_start >= size[dim[0]] || xmax < x0) {
+    snprintf(why, nwhy, "the model does not coincide with the grid in x");
+    return false;
+  }
+  int ix_end = (int)ceil((double)((xmax - x0) / dx));
+  if (ix_end >= size[dim[0]]) ix_end = size[dim[0]] - 1;
+  const int iy_start = (int)(ymin < y0 ? 0.0f : (ymin - y0) / dy);
+  if (iy_start >= size[dim[1]] || ymax < y0) {
+    snprintf(why, nwhy, "the model does not coincide with the grid in y");
+    return false;
+  }
+  int iy_end = (int)ceil((double)((ymax - y0) / dy));
+  if (iy_end >= size[dim[1]]) iy_end = size[dim[1]] - 1;
+  const float zn = z0 + dz * (size[dim[2]] - 1);
+  if (zmin >= std::max(zn, z0) || zmax <= std::min(z0, zn)) {
+    snprintf(why, nwhy, "the model does not coincide with the grid in z");
+    return false;
+  }
+  xv.resize(ix_end - ix_start + 1);
+  yv.resize(iy_end - iy_start + 1);
+  zv.resize(size[dim[2]]);
+  for (size_t i = 0; i < xv.size(); i++) xv[i] = x0 + (ix_start + (int)i) * dx;
+  for (size_t i = 0; i < yv.size(); i++) yv[i] = y0 + (iy_start + (int)i) * dy;
+  for (size_t i = 0; i < zv.size(); i++) zv[i] = z0 + (int)i * dz;
+  start[dim[0]] = ix_start; end[dim[0]] = ix_end;
+  start[dim[1]] = iy_start; end[dim[1]] = iy_end;
+  start[dim[2]] = 0; end[dim[2]] = size[dim[2]] - 1;
+  return true;
+}
+
+// media_bin2model_el_iso / _vis_iso: the z extent of the grid on the
+// device, the window, the volumes read, then cgfdh_bin2model_grid
+static int bin2model_file(const char* who, int nvol, float* const* out, const float* x3d,
+                          const float* y3d, const float* z3d, size_t nx, size_t ny, size_t nz,
+                          float xmin, float xmax, float ymin, float ymax, int grid_type,
+                          const int* order, const int* size, const float* spacing,
+                          const float* origin, const char* const* files)
+{
+  if (!x3d || !y3d || !z3d || !order || !size || !spacing || !origin)
+    return fail(CGFDH_EINVAL, "%s: null argument", who);
+  for (int q = 0; q < nvol; q++)
+    if (!files[q]) return fail(CGFDH_EINVAL, "%s: null file name", who);
+  if (grid_type != 3)
+    return fail(CGFDH_EINVAL, "%s: grid type %d (only the curvilinear grid, 3, is provided)", who,
+                grid_type);
+  const size_t n = nx * ny * nz;
+  if (n == 0) return CGFDH_OK;
+  unsigned mm[2] = {0xffffffffu, 0u};
+  {
+    DevBuf dz, dmm;
+    HIPCHK(hipMalloc(&dz.p, n * sizeof(float)));
+    HIPCHK(hipMemcpy(dz.p, z3d, n * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&dmm.p, sizeof mm));
+    HIPCHK(hipMemcpy(dmm.p, mm, sizeof mm, hipMemcpyHostToDevice));
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(minmax_kernel, dim3(blocks), dim3(256), 0, 0, dz.as<float>(), (long)n,
+                       dmm.as<unsigned>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(mm, dmm.p, sizeof mm, hipMemcpyDeviceToHost));
+  }
+  int start[3], end[3];
+  std::vector<float> xv, yv, zv;
+  char why[160];
+  if (!bin_window(order, size, spacing, origin, xmin, xmax, ymin, ymax, float_unkey(mm[0]),
+                  float_unkey(mm[1]), start, end, xv, yv, zv, why, sizeof why))
+    return fail(CGFDH_EINVAL, "%s: %s", who, why);
+  const int n3[3] = {(int)xv.size(), (int)yv.size(), (int)zv.size()};
+  const size_t v3 = (size_t)n3[0] * n3[1] * n3[2];
+  std::vector<float> vols(nvol * v3);
+  for (int q = 0; q < nvol; q++)
+    if (cgfdh_io_read_bin_window(files[q], order, size, start, end, vols.data() + q * v3))
+      return fail(CGFDH_EINVAL, "%s: %s", who, cgfdh_io_last_error());
+  return cgfdh_bin2model_grid(nvol, n3, xv.data(), yv.data(), zv.data(), vols.data(), (long)n,
+                              x3d, y3d, z3d, out);
+}
+
+extern "C" int cgfdh_media_bin2model_el_iso(float* rho3d, float* lam3d, float* mu3d,
+                                            const float* x3d, const float* y3d, const float* z3d,
+                                            size_t nx, size_t ny, size_t nz, float xmin,
+                                            float xmax, float ymin, float ymax, int grid_type,
+                                            int* bin_order, int* bin_size, float* bin_spacing,
+                                            float* bin_origin, const char* bin_file_rho,
+                                            const char* bin_file_vp, const char* bin_file_vs)
+{
+  float* out[3] = {rho3d, lam3d, mu3d};
+  const char* files[3] = {bin_file_rho, bin_file_vp, bin_file_vs};
+  return bin2model_file("cgfdh_media_bin2model_el_iso", 3, out, x3d, y3d, z3d, nx, ny, nz, xmin,
+                        xmax, ymin, ymax, grid_type, bin_order, bin_size, bin_spacing,
+                        bin_origin, files);
+}
+
+extern "C" int cgfdh_media_bin2model_vis_iso(float* Qp3d, float* Qs3d, const float* x3d,
+                                             const float* y3d, const float* z3d, size_t nx,
+                                             size_t ny, size_t nz, float xmin, float xmax,
+                                             float ymin, float ymax, int grid_type,
+                                             int* bin_order, int* bin_size, float* bin_spacing,
+                                             float* bin_origin, const char* bin_file_Qp,
+                                             const char* bin_file_Qs)
+{
+  float* out[2] = {Qp3d, Qs3d};
+  const char* files[2] = {bin_file_Qp, bin_file_Qs};
+  return bin2model_file("cgfdh_media_bin2model_vis_iso", 2, out, x3d, y3d, z3d, nx, ny, nz, xmin,
+                        xmax, ymin, ymax, grid_type, bin_order, bin_size, bin_spacing,
+                        bin_origin, files);
+}
+
 extern "C" int cgfdh_grid2model_grid(int media_type, int target, int method, int n_layers,
                                      const int* ngz, int mx, int my, const float* geo,
                                      const float* fields, int nx, int ny, int nz,

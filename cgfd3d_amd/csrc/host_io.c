@@ -964,3 +964,51 @@ int cgfdh_io_read_md3grd(const char *path, float xmin, float xmax, float ymin, f
   }
   return rc;
 }
+
+/* ---- binary volumes of media_bin2model ------------------------------------
+ * read_bin_file (media_read_file.cpp:1274-1320): size[0] x size[1] x size[2]
+ * float32 samples, the first axis fastest; axis a holds the grid direction
+ * order[a] (0 x, 1 y, 2 z).  The samples with start[a] <= i[a] <= end[a]
+ * are kept, laid out x fastest, then y, then z: out[(ix - sx) + (iy - sy) nx
+ * + (iz - sz) nx ny] (the reference always keeps every z sample, sz = 0). */
+int cgfdh_io_read_bin_window(const char *path, const int *order, const int *size,
+                             const int *start, const int *end, float *out)
+{
+  int dim[3] = {-1, -1, -1};
+  for (int a = 0; a < 3; a++) {
+    if (order[a] < 0 || order[a] > 2)
+      return io_fail("binary model %s: bin_order[%d] = %d (0, 1 or 2)", path, a, order[a]);
+    dim[order[a]] = a;
+  }
+  if (dim[0] < 0 || dim[1] < 0 || dim[2] < 0)
+    return io_fail("binary model %s: bin_order misses a direction", path);
+  for (int a = 0; a < 3; a++)
+    if (size[a] < 1 || start[a] < 0 || end[a] < start[a] || end[a] >= size[a])
+      return io_fail("binary model %s: window %d..%d of axis %d (%d samples)", path, start[a],
+                     end[a], a, size[a]);
+  FILE *fp = fopen(path, "rb");
+  if (!fp) return io_fail("binary model %s: %s", path, strerror(errno));
+  const size_t nx = end[dim[0]] - start[dim[0]] + 1, ny = end[dim[1]] - start[dim[1]] + 1;
+  const size_t line = nx, slice = nx * ny;
+  float *row = malloc((size_t)size[0] * sizeof(float));
+  if (!row) { fclose(fp); return io_fail("binary model %s: out of memory", path); }
+  int rc = 0;
+  int i[3];
+  for (i[2] = 0; i[2] < size[2] && !rc; i[2]++)
+    for (i[1] = 0; i[1] < size[1] && !rc; i[1]++) {
+      if (fread(row, sizeof(float), (size_t)size[0], fp) != (size_t)size[0]) {
+        rc = io_fail("binary model %s: insufficient data", path);
+        break;
+      }
+      if (i[1] < start[1] || i[1] > end[1] || i[2] < start[2] || i[2] > end[2]) continue;
+      for (i[0] = start[0]; i[0] <= end[0]; i[0]++) {
+        const size_t p = (size_t)(i[dim[0]] - start[dim[0]]) +
+                         (size_t)(i[dim[1]] - start[dim[1]]) * line +
+                         (size_t)(i[dim[2]] - start[dim[2]]) * slice;
+        out[p] = row[i[0]];
+      }
+    }
+  free(row);
+  fclose(fp);
+  return rc;
+}

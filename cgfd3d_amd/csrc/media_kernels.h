@@ -578,4 +578,128 @@ __global__ void __launch_bounds__(256) layer_expand_kernel(int vti, float* out, 
   for (int z = 0; z < 12; z++) c[zeros[z]][p] = 0.0f;
 }
 
+// ---- binary volumes (media_bin2model.cpp) --------------------------------
+struct BinModel {
+  const float* vol;          // [nvol][nz][ny][nx]
+  const float *xv, *yv, *zv; // the windowed mesh vectors (z may decrease)
+  int nx, ny, nz, nvol;
+};
+
+// TrilinearInterpolation (media_utility.cpp:304-401): the bracketing
+// samples by floor / ceil of the offset over the spacing, clamped to the
+// mesh, then linear in x, y, z in turn (a direction whose two samples
+// coincide taken as is)
+__device__ float bin_trilinear(const BinModel& B, const float* v, float xq, float yq, float zq)
+{
+  const float* x = B.xv;
+  const float* y = B.yv;
+  const float* z = B.zv;
+  const int nx = B.nx, ny = B.ny, nz = B.nz;
+  const float dx = x[1] - x[0], dy = y[1] - y[0], dz = z[1] - z[0];
+  int ix0 = (int)floor((double)((xq - x[0]) / dx)), iy0 = (int)floor((double)((yq - y[0]) / dy));
+  int ix2 = (int)ceil((double)((xq - x[0]) / dx)), iy2 = (int)ceil((double)((yq - y[0]) / dy));
+  ix0 = ix0 < 0 ? 0 : ix0;
+  ix2 = ix2 < 0 ? 0 : ix2;
+  ix0 = ix0 > nx - 1 ? nx - 1 : ix0;
+  ix2 = ix2 > nx - 1 ? nx - 1 : ix2;
+  iy0 = iy0 < 0 ? 0 : iy0;
+  iy2 = iy2 < 0 ? 0 : iy2;
+  iy0 = iy0 > ny - 1 ? ny - 1 : iy0;
+  iy2 = iy2 > ny - 1 ? ny - 1 : iy2;
+  int iz0 = (int)floor((double)((zq - z[0]) / dz)), iz2 = (int)ceil((double)((zq - z[0]) / dz));
+  iz0 = iz0 < 0 ? 0 : iz0;
+  iz0 = iz0 > nz - 1 ? nz - 1 : iz0;
+  iz2 = iz2 > nz - 1 ? nz - 1 : iz2;
+  iz2 = iz2 < 0 ? 0 : iz2;
+  const long line = nx, slice = (long)nx * ny;
+  if (ix0 == ix2 && iy0 == iy2 && iz0 == iz2) return v[ix0 + iy0 * line + iz0 * slice];
+  const float v000 = v[ix0 + iy0 * line + iz0 * slice], v002 = v[ix0 + iy0 * line + iz2 * slice];
+  const float v200 = v[ix2 + iy0 * line + iz0 * slice], v202 = v[ix2 + iy0 * line + iz2 * slice];
+  const float v020 = v[ix0 + iy2 * line + iz0 * slice], v022 = v[ix0 + iy2 * line + iz2 * slice];
+  const float v220 = v[ix2 + iy2 * line + iz0 * slice], v222 = v[ix2 + iy2 * line + iz2 * slice];
+  float v100, v102, v122, v120;
+  if (ix0 == ix2) {
+    v100 = v000; v102 = v002; v122 = v022; v120 = v020;
+  } else {
+    const float w = (xq - x[ix0]) / (x[ix2] - x[ix0]);
+    v100 = v000 + w * (v200 - v000);
+    v102 = v002 + w * (v202 - v002);
+    v122 = v022 + w * (v222 - v022);
+    v120 = v020 + w * (v220 - v020);
+  }
+  float v110, v112;
+  if (iy0 == iy2) {
+    v110 = v100; v112 = v102;
+  } else {
+    const float w = (yq - y[iy0]) / (y[iy2] - y[iy0]);
+    v110 = v100 + w * (v120 - v100);
+    v112 = v102 + w * (v122 - v102);
+  }
+  if (iz0 == iz2) return v110;
+  return v110 + (zq - z[iz0]) / (z[iz2] - z[iz0]) * (v112 - v110);
+}
+
+// every point's samples: rho, vp, vs into rho, lambda, mu
+// (parameterization_bin_el_iso_loc :185-234), or Qp, Qs as they are
+__global__ void __launch_bounds__(256)
+bin_loc_kernel(BinModel B, const float* x, const float* y, const float* z, long n, float* out)
+{
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  if (p >= n) return;
+  const long v3 = (long)B.nx * B.ny * B.nz;
+  float s[3] = {0.0f, 0.0f, 0.0f};
+  for (int q = 0; q < B.nvol; q++) s[q] = bin_trilinear(B, B.vol + q * v3, x[p], y[p], z[p]);
+  if (B.nvol == 3) {
+    const float rho = s[0], vp = s[1], vs = s[2];
+    const float mu = vs * vs * rho;
+    out[2 * n + p] = mu;
+    out[n + p] = (float)((double)(vp * vp * rho) - 2.0 * (double)mu);
+    out[p] = rho;
+  } else {
+    out[p] = s[0];
+    out[n + p] = s[1];
+  }
+}
+
+// the smallest and largest of n floats (ordered-integer keys, one
+// atomicMin / atomicMax per block)
+__device__ __forceinline__ unsigned float_key(float f)
+{
+  const unsigned b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__global__ void __launch_bounds__(256) minmax_kernel(const float* v, long n, unsigned* mm)
+{
+  __shared__ unsigned lo[256], hi[256];
+  unsigned a = 0xffffffffu, b = 0u;
+  for (long p = (long)blockIdx.x * 256 + threadIdx.x; p < n; p += (long)gridDim.x * 256) {
+    const unsigned k = float_key(v[p]);
+    a = min(a, k);
+    b = max(b, k);
+  }
+  lo[threadIdx.x] = a;
+  hi[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      lo[threadIdx.x] = min(lo[threadIdx.x], lo[threadIdx.x + s]);
+      hi[threadIdx.x] = max(hi[threadIdx.x], hi[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicMin(&mm[0], lo[0]);
+    atomicMax(&mm[1], hi[0]);
+  }
+}
+
+inline float float_unkey(unsigned k)
+{
+  const unsigned b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+  float f;
+  memcpy(&f, &b, sizeof f);
+  return f;
+}
+
 }  // namespace cgfdh

@@ -38,6 +38,7 @@ def _io():
         L.cgfdh_io_md3grd_info.argtypes = ([C.c_char_p] + [C.c_float] * 4 + [_P] * 4 +
                                            [C.c_int] + [_P] * 4)
         L.cgfdh_io_read_md3grd.argtypes = [C.c_char_p] + [C.c_float] * 4 + [_P]
+        L.cgfdh_io_read_bin_window.argtypes = [C.c_char_p] + [_P] * 5
         L.cgfdh_io_read_md3lay.argtypes = [C.c_char_p, _P]
         L._io_sigs = True
     return L
@@ -181,6 +182,22 @@ def read_md3lay(path, with_type=False):
     if with_type:
         return mt.value, nprop.value, fields, geo
     return nprop.value, fields, geo
+
+
+def read_bin_window(path, order, size, start, end):
+    """The samples start[a]..end[a] of each stored axis of a media_bin2model
+    volume (float32, the first axis fastest, axis a holding direction
+    order[a]), laid out (nz, ny, nx)."""
+    order, size, start, end = (np.ascontiguousarray(v, np.int32) for v in (order, size, start, end))
+    n = [0, 0, 0]
+    for a in range(3):
+        n[order[a]] = int(end[a] - start[a] + 1)
+    out_ = np.zeros((n[2], n[1], n[0]), np.float32)
+    _check(_io().cgfdh_io_read_bin_window(path.encode(), order.ctypes.data_as(_P),
+                                          size.ctypes.data_as(_P), start.ctypes.data_as(_P),
+                                          end.ctypes.data_as(_P), out_.ctypes.data_as(_P)),
+           "binary model")
+    return out_
 
 
 def read_md3grd(path, xmin, xmax, ymin, ymax):

@@ -151,6 +151,41 @@ def media_layer2model(target, path, xyz, method="loc", grid_type=3):
     return outs
 
 
+def bin2model_grid(vols, xv, yv, zv, xyz):
+    """Binary volumes sampled at grid points on the GPU
+    (cgfdh_bin2model_grid): vols (3 or 2, nz, ny, nx) -- rho, vp, vs giving
+    rho, lambda, mu, or Qp, Qs as they are -- on the mesh vectors xv, yv, zv;
+    xyz (3, ...) the points.  Returns the outputs, each of the points' shape."""
+    v = np.ascontiguousarray(vols, np.float32)
+    nvol = v.shape[0]
+    n3 = np.array([v.shape[3], v.shape[2], v.shape[1]], np.int32)
+    xv, yv, zv = (np.ascontiguousarray(a, np.float32) for a in (xv, yv, zv))
+    x, y, z = (np.ascontiguousarray(xyz[d], np.float32) for d in range(3))
+    outs = [np.zeros(x.shape, np.float32) for _ in range(nvol)]
+    arr = (C.c_void_p * nvol)(*[o.ctypes.data for o in outs])
+    check(lib().cgfdh_bin2model_grid(nvol, _p(n3), _p(xv), _p(yv), _p(zv), _p(v), x.size, _p(x),
+                                     _p(y), _p(z), C.cast(arr, C.c_void_p)), "bin2model_grid")
+    return outs
+
+
+def media_bin2model(files, xyz, extent, order, size, spacing, origin, grid_type=3):
+    """The reference-shaped entry points cgfdh_media_bin2model_el_iso (three
+    files rho, vp, vs: returns rho, lambda, mu) and _vis_iso (two files Qp,
+    Qs): the binary model onto the grid xyz (3, nz, ny, nx) of x/y extent
+    (xmin, xmax, ymin, ymax)."""
+    x, y, z = (np.ascontiguousarray(xyz[d], np.float32) for d in range(3))
+    nz, ny, nx = x.shape
+    outs = [np.zeros(x.shape, np.float32) for _ in files]
+    order, size = (np.ascontiguousarray(a, np.int32) for a in (order, size))
+    spacing, origin = (np.ascontiguousarray(a, np.float32) for a in (spacing, origin))
+    fn = lib().cgfdh_media_bin2model_el_iso if len(files) == 3 else \
+        lib().cgfdh_media_bin2model_vis_iso
+    check(fn(*[_p(o) for o in outs], _p(x), _p(y), _p(z), nx, ny, nz,
+             *[float(e) for e in extent], grid_type, _p(order), _p(size), _p(spacing),
+             _p(origin), *[f.encode() for f in files]), "media_bin2model")
+    return outs
+
+
 def grid2model_grid(media_type, target, method, ngz, fields, geo, xyz):
     """A grid model (.md3grd) onto a curvilinear grid on the GPU
     (cgfdh_grid2model_grid): media_type, ngz, fields and geo as

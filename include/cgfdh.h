@@ -433,6 +433,37 @@ int cgfdh_grid2model_grid(int media_type, int target, int method, int n_layers, 
                           int nz, const float *x3d, const float *y3d, const float *z3d,
                           float *const *out);
 
+/* Binary volumes sampled at the grid points on the GPU
+ * (parameterization_bin_el_iso_loc / _vis_iso_loc, media_bin2model.cpp:
+ * 185-432): nvol 3 volumes rho, vp, vs into rho, lambda, mu (out in that
+ * order), or nvol 2 volumes Qp, Qs as they are; each volume n3[0] x n3[1] x
+ * n3[2] samples (x fastest) on the mesh vectors xv, yv, zv (z may
+ * decrease), as cgfdh_io_read_bin_window cuts them; npts points x, y, z.
+ * TrilinearInterpolation (media_utility.cpp:304-401) per volume, points
+ * outside the mesh clamped to its edge. */
+int cgfdh_bin2model_grid(int nvol, const int *n3, const float *xv, const float *yv,
+                         const float *zv, const float *vols, long npts, const float *x,
+                         const float *y, const float *z, float *const *out);
+
+/* Drop-in replacements of media_bin2model_el_iso and media_bin2model_vis_iso
+ * (media_bin2model.cpp:27-183, 236-386, same arguments; main_curv_col_el_3d.c:
+ * 466-515): the window of the binary volumes over the grid's extent (x, y
+ * from xmin..ymax, z from z3d), read, sampled on the GPU.  grid_type must be
+ * 3.  CGFDH_OK or an error code (the reference exits) when bin_order is not
+ * a permutation of 0, 1, 2, the model misses the grid, or a file is short. */
+int cgfdh_media_bin2model_el_iso(float *rho3d, float *lam3d, float *mu3d, const float *x3d,
+                                 const float *y3d, const float *z3d, size_t nx, size_t ny,
+                                 size_t nz, float xmin, float xmax, float ymin, float ymax,
+                                 int grid_type, int *bin_order, int *bin_size, float *bin_spacing,
+                                 float *bin_origin, const char *bin_file_rho,
+                                 const char *bin_file_vp, const char *bin_file_vs);
+int cgfdh_media_bin2model_vis_iso(float *Qp3d, float *Qs3d, const float *x3d, const float *y3d,
+                                  const float *z3d, size_t nx, size_t ny, size_t nz, float xmin,
+                                  float xmax, float ymin, float ymax, int grid_type,
+                                  int *bin_order, int *bin_size, float *bin_spacing,
+                                  float *bin_origin, const char *bin_file_Qp,
+                                  const char *bin_file_Qs);
+
 /* Drop-in replacements of media_grid2model_onecmp, _el_iso, _el_vti and
  * _el_aniso (media_grid2model.cpp:29-420, same arguments; Xmin..Ymax the
  * grid's extent, main_curv_col_el_3d.c:388-460): read the window of the

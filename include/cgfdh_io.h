@@ -159,6 +159,14 @@ int cgfdh_io_md3grd_info(const char *path, float xmin, float xmax, float ymin, f
 int cgfdh_io_read_md3grd(const char *path, float xmin, float xmax, float ymin, float ymax,
                          float *fields);
 
+/* One float32 volume of media_bin2model (read_bin_file,
+ * media_read_file.cpp:1274-1320): size[0] x size[1] x size[2] samples, the
+ * first axis fastest, axis a holding direction order[a] (0 x, 1 y, 2 z);
+ * the samples start[a]..end[a] of each axis are kept, x fastest then y then
+ * z (the layout cgfdh_bin2model_grid takes). */
+int cgfdh_io_read_bin_window(const char *path, const int *order, const int *size,
+                             const int *start, const int *end, float *out);
+
 #ifdef __cplusplus
 }
 #endif

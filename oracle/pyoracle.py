@@ -107,6 +107,10 @@ def load_oracle(fp64=False):
     lib.orc_layer2model.argtypes = [C.c_int, _P, _P, _P, C.c_size_t] + [_P] * 6
     lib.orc_layer2model_grid.restype = C.c_int
     lib.orc_layer2model_grid.argtypes = [C.c_int] * 3 + [_P] * 3 + [C.c_int] * 3 + [_P] * 4
+    lib.orc_bin_window.restype = C.c_int
+    lib.orc_bin_window.argtypes = [_P] * 4 + [C.c_float] * 6 + [_P] * 6
+    lib.orc_bin2model_grid.restype = C.c_int
+    lib.orc_bin2model_grid.argtypes = [C.c_int] + [_P] * 5 + [C.c_size_t] + [_P] * 4
     lib.orc_grid2model_grid.restype = C.c_int
     lib.orc_grid2model_grid.argtypes = ([C.c_int] * 3 + [_P] * 2 + [C.c_int] + [_P] * 2 +
                                         [C.c_int] * 3 + [_P] * 4)

@@ -208,3 +208,65 @@ def oracle_grid2model(code, target, method, ngz, f, geo, co):
                                  po.ptr(gz), po.ptr(fc), nx, ny, nz, *[po.ptr(a) for a in xyz],
                                  C.cast(arr, C.c_void_p))
     return rc, outs
+
+
+# ---- binary volumes (media_bin2model) --------------------------------------
+
+def bin_model(order=(2, 0, 1), seed=0, n=(40, 33, 29), spacing=(50.0, 60.0, -70.0),
+              origin=(-200.0, -300.0, 300.0)):
+    """Smooth random volumes rho, vp, vs (and Qp, Qs) on a regular mesh of
+    n = (nx, ny, nz) samples, spacing and origin per direction (z may
+    decrease), stored with axis a holding direction order[a], the first
+    axis fastest.  Returns (vols (5, nz, ny, nx) in x-fastest layout, the
+    stored arrays (5, ...) to write, size, spacing and origin per stored
+    axis)."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = n
+    base = np.array([2500.0, 4500.0, 2500.0, 80.0, 40.0])
+    amp = np.array([200.0, 600.0, 300.0, 20.0, 10.0])
+    zz, yy, xx = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    vols = np.zeros((5, nz, ny, nx), np.float32)
+    for q in range(5):
+        ph = rng.uniform(0, 2 * np.pi, 3)
+        vols[q] = base[q] + amp[q] * (np.sin(0.21 * xx + ph[0]) * np.cos(0.17 * yy + ph[1]) +
+                                      0.5 * np.sin(0.13 * zz + ph[2]))
+    vols[1] = np.maximum(vols[1], 1.6 * vols[2])        # vp > sqrt 2 vs
+    # stored layout: axis a is direction order[a]; numpy's last axis fastest
+    perm = [2 - order[a] for a in (2, 1, 0)]             # (nz, ny, nx) axes -> stored
+    stored = np.stack([np.ascontiguousarray(np.transpose(v, perm)) for v in vols])
+    size = [n[order[a]] for a in range(3)]
+    sp = [spacing[order[a]] for a in range(3)]
+    org = [origin[order[a]] for a in range(3)]
+    return vols, stored, size, sp, org
+
+
+def write_bins(tmp_path, stored, names=("rho", "vp", "vs", "qp", "qs")):
+    paths = []
+    for v, nm in zip(stored, names):
+        p = str(tmp_path / f"{nm}.bin")
+        np.ascontiguousarray(v, np.float32).tofile(p)
+        paths.append(p)
+    return paths
+
+
+def oracle_bin(order, size, spacing, origin, extent, vols_window, co, nvol):
+    """orc_bin_window + orc_bin2model_grid: (rc, outputs)."""
+    lib = po.load_oracle()
+    x, y, z = (np.ascontiguousarray(co[d], np.float32).reshape(-1) for d in range(3))
+    start = np.zeros(3, np.int32); end = np.zeros(3, np.int32); n3 = np.zeros(3, np.int32)
+    xv = np.zeros(4096, np.float32); yv = np.zeros(4096, np.float32); zv = np.zeros(4096, np.float32)
+    o, s, sp, org = (np.ascontiguousarray(a, t) for a, t in ((order, np.int32), (size, np.int32),
+                                                             (spacing, np.float32),
+                                                             (origin, np.float32)))
+    rc = lib.orc_bin_window(po.ptr(o), po.ptr(s), po.ptr(sp), po.ptr(org), *map(float, extent),
+                            float(z.min()), float(z.max()), po.ptr(start), po.ptr(end),
+                            po.ptr(xv), po.ptr(yv), po.ptr(zv), po.ptr(n3))
+    if rc:
+        return rc, None, None
+    outs = [np.zeros(x.shape, np.float32) for _ in range(nvol)]
+    arr = (C.c_void_p * nvol)(*[a.ctypes.data for a in outs])
+    vw = np.ascontiguousarray(vols_window(start, end), np.float32)
+    rc = lib.orc_bin2model_grid(nvol, po.ptr(n3), po.ptr(xv), po.ptr(yv), po.ptr(zv), po.ptr(vw),
+                                x.size, po.ptr(x), po.ptr(y), po.ptr(z), C.cast(arr, C.c_void_p))
+    return rc, [a.reshape(co[0].shape) for a in outs], (start, end, xv[:n3[0]], yv[:n3[1]],
+                                                        zv[:n3[2]])
