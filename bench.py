@@ -61,6 +61,8 @@ KERNELS = [
                           "free-surface rows"),
     ("stage_interior_phased_shellp", "stage_interior_phased_kernel<BY=4> + "
                                      "stage_shellp_kernel (phased PML slabs) + free-surface rows"),
+    ("stage_rows_lds", "stage_rows_lds_kernel<BY=4> (whole rows, PML folded in, non-stencil "
+                       "operands loaded straight to LDS) + free-surface rows"),
 ]
 
 
@@ -200,9 +202,10 @@ def bench_batch(args):
         f"{time.perf_counter() - t:.1f} s")
     b = Batch(solvers)
     t = time.perf_counter()
-    b.autotune(0)
+    if args.variant is None:      # a variant given on the command line is kept for every grid
+        b.autotune(0)
     choices = collections.Counter(tuple(s.kernel_choice()) for s in solvers)
-    log(f"autotuned in {time.perf_counter() - t:.1f} s: {dict(choices)}")
+    log(f"kernel choice in {time.perf_counter() - t:.1f} s: {dict(choices)}")
     b.run(0, args.warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -319,7 +322,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--variant", type=int, default=None,
                     help="stage kernels (default: the engine's choice for the medium, 16 "
-                         "elastic, 4 visco-elastic): 0 per-point, 1 "
+                         "isotropic and VTI, 18 anisotropic, 4 visco-elastic): 0 per-point, 1 "
                          "tiled, 2/3 z-marching interior + "
                          "shell, 4/5 per-point interior (64x4 / 64x2) + shell, 6/7 "
                          "x-vectorised interior (2 / 4 points per thread) + shell, 8/9 "
@@ -329,7 +332,8 @@ def main():
                          "combine loads last, 14 LDS-tiled interior with phased loads + shell, "
                          "15 variant 12 built for 3 waves per SIMD, 16 whole-row two-phase "
                          "kernel with the PML faces folded in, 17 variant 12 with a two-phase "
-                         "shell")
+                         "shell, 18 variant 16 with the operands that have no stencil loaded "
+                         "straight to LDS (4 waves per SIMD)")
     ap.add_argument("--workload", default="iso", choices=["iso", "vti", "aniso", "visco", "batch"])
     ap.add_argument("--size", type=str, default=None,
                     help="grid NIxNJxNK (default 600x600x300 iso, 400x400x200 media)")

@@ -362,7 +362,8 @@ extern "C" int cgfdh_create(const cgfdh_desc* d, cgfdh_solver** out)
   // default stage kernels: the whole-row two-phase kernel with the PML faces
   // folded in for the elastic media, the per-point kernel for the
   // visco-elastic one
-  s->variant = d->medium_type == CGFDH_MEDIUM_VISCO_ISO ? 4 : 16;
+  s->variant = d->medium_type == CGFDH_MEDIUM_VISCO_ISO ? 4
+               : d->medium_type == CGFDH_MEDIUM_ELASTIC_ANISO ? 18 : 16;
   if (!bad && hipStreamCreate(&s->stream) != hipSuccess) bad = 1;
   if (!bad && (hipStreamCreate(&s->stream2) != hipSuccess ||
                hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
@@ -904,10 +905,10 @@ extern "C" int cgfdh_get_seismograms(cgfdh_solver* s, float* out)
 extern "C" int cgfdh_set_kernel_variant(cgfdh_solver* s, int v)
 {
   if (s) s->gen++;
-  if (!s || v < 0 || v > 17) return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: %d", v);
+  if (!s || v < 0 || v > 18) return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: %d", v);
   if (s->d.medium_type != CGFDH_MEDIUM_ELASTIC_ISO && v >= 1 && v <= 3)
     return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: variant %d is built for the isotropic "
-                "medium only (VTI / anisotropic: 0, 4..17; visco-elastic: 0, 4, 5, 8..11)", v);
+                "medium only (VTI / anisotropic: 0, 4..18; visco-elastic: 0, 4, 5, 8..11)", v);
   if (s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO && (v == 6 || v == 7 || v >= 12))
     return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: variant %d is not built for the "
                 "visco-elastic medium (0, 4, 5, 8..11)", v);
@@ -2974,7 +2975,7 @@ extern "C" int cgfdh_batch_autotune(cgfdh_batch* b, int it0)
     const bool iso = s->d.medium_type == CGFDH_MEDIUM_ELASTIC_ISO;
     const bool visco = s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO;
     std::vector<int> variants = visco ? std::vector<int>{4, 5, 8, 9}
-                                      : std::vector<int>{4, 6, 7, 8, 12, 14, 16};
+                                      : std::vector<int>{4, 6, 7, 8, 12, 14, 16, 18};
     if (iso) variants.push_back(3);
     const int groups[][2] = {{9, 2}, {3, 4}, {9, 8}};
     float best = 1e30f;

@@ -458,6 +458,143 @@ static void launch_rows_phased(hipStream_t st, const StageParams& P, BoxParams B
 #undef CGFDH_RF
 }
 
+// the whole-row kernel with the operands that have no stencil -- the
+// metric, the medium, the slowness and the RK combine inputs -- brought in
+// by direct-to-LDS loads (global_load_lds_dword): each wave's lanes land
+// in its own LDS slots (slot s of wave w holds the 64 lanes' values at
+// lds[w][s][lane]), so the loads stay in flight without holding VGPRs and
+// only the 135 stencil values occupy registers during the load phase.  The
+// values are read back by the same lanes after the loads have completed
+// (s_waitcnt vmcnt(0)); no barrier is needed.  Same arithmetic as
+// stage_rows_phased_kernel.
+template <int MT, int KIND> struct RowsLdsSlots {
+  static constexpr int ncomb = KIND == KIND_MID ? 18 : 9;
+  static constexpr int met = 0, med = 9, slw = 9 + Medium<MT>::ncoef;
+  static constexpr int comb = slw + 1;
+  static constexpr int n = comb + ncomb;
+};
+
+__device__ __forceinline__ void lds_dma_dword(const float* src, float* lds_wave_slot)
+{
+  __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                   (__attribute__((address_space(3))) void*)lds_wave_slot, 4, 0,
+                                   0);
+}
+
+template <int DX, int DY, int DZ, int KIND, int MT, int BY>
+__global__ void __launch_bounds__(64 * BY, 4)
+stage_rows_lds_kernel(StageParams P, BoxParams B)
+{
+  static_assert(MT != MEDIUM_VISCO, "elastic media only");
+  static_assert(KIND != KIND_RHS, "time-stepping stages only");
+  using S = RowsLdsSlots<MT, KIND>;
+  __shared__ float lds[BY][S::n][64];
+  int bx, by, k;
+  if (!grouped_block(B, bx, by, k)) return;
+  const int ti0 = B.i0 + bx * 64, ti1 = min(ti0 + 63, B.i0 + B.ni - 1);
+  const int tj0 = B.j0 + by * BY, tj1 = min(tj0 + BY - 1, B.j0 + B.nj - 1);
+  bool tile_pml = false;
+  for (int f = 0; f < P.npml; f++) {
+    const PmlDev& F = P.pml[f];
+    tile_pml |= ti1 >= F.i1 && ti0 <= F.i2 && tj1 >= F.j1 && tj0 <= F.j2 && k >= F.k1 &&
+                k <= F.k2;
+  }
+  const int i = B.i0 + bx * 64 + threadIdx.x;
+  const int j = B.j0 + by * BY + threadIdx.y;
+  if (i >= B.i0 + B.ni || j >= B.j0 + B.nj) return;
+  const long p = (long)i + (long)j * P.nx + (long)k * P.ss;
+  const long V = P.vol;
+  float (*L)[64] = lds[threadIdx.y];
+  // the operands without a stencil go straight to LDS
+#pragma unroll
+  for (int n = 0; n < 9; n++) lds_dma_dword(P.met + (1 + n) * V + p, L[S::met + n]);
+#pragma unroll
+  for (int n = 0; n < Medium<MT>::ncoef; n++) lds_dma_dword(P.med + n * V + p, L[S::med + n]);
+  lds_dma_dword(P.slw + p, L[S::slw]);
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    if (KIND == KIND_FIRST || KIND == KIND_MID) lds_dma_dword(P.wpre + c * V + p, L[S::comb + c]);
+    if (KIND == KIND_MID) lds_dma_dword(P.wend + c * V + p, L[S::comb + 9 + c]);
+    if (KIND == KIND_LAST) lds_dma_dword(P.wend + c * V + p, L[S::comb + c]);
+  }
+  constexpr int xl = Win<DX>::lo, yl = Win<DY>::lo, zl = Win<DZ>::lo;
+  float ax[9][5], ay[9][5], az[9][5];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    const float* v = P.wcur + c * V;
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+      ax[c][n] = v[p + (xl + n)];
+      ay[c][n] = v[p + (long)(yl + n) * P.nx];
+      az[c][n] = v[p + (long)(zl + n) * P.ss];
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  float X[9], Y[9], Z[9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    X[c] = d5v<DX>(ax[c][0], ax[c][1], ax[c][2], ax[c][3], ax[c][4]);
+    Y[c] = d5v<DY>(ay[c][0], ay[c][1], ay[c][2], ay[c][3], ay[c][4]);
+    Z[c] = d5v<DZ>(az[c][0], az[c][1], az[c][2], az[c][3], az[c][4]);
+  }
+  // every load of this lane has landed (the stencil values are in use)
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_sched_barrier(0);
+  const int t = threadIdx.x;
+  float m[9];
+#pragma unroll
+  for (int n = 0; n < 9; n++) m[n] = L[S::met + n][t];
+  float mv[Medium<MT>::ncoef];
+#pragma unroll
+  for (int n = 0; n < Medium<MT>::ncoef; n++) mv[n] = L[S::med + n][t];
+  const float slw = L[S::slw][t];
+  Medium<MT> md;
+  md.from(mv);
+  float r[9];
+  momentum(r, m, slw, X, Y, Z);
+  float Xv[3] = {X[VX], X[VY], X[VZ]}, Yv[3] = {Y[VX], Y[VY], Y[VZ]},
+        Zv[3] = {Z[VX], Z[VY], Z[VZ]};
+  md.hooke(r, m, Xv, Yv, Zv);
+  if (tile_pml) {
+    for (int f = 0; f < P.npml; f++) {
+      const PmlDev& F = P.pml[f];
+      if (i >= F.i1 && i <= F.i2 && j >= F.j1 && j <= F.j2 && k >= F.k1 && k <= F.k2) {
+        pml_face<KIND, MT>(P, F, f, i, j, k, p, X, Y, Z, md, slw, m, r);
+      }
+    }
+  }
+  if (P.nsrc > 0) add_sources(P, i, j, k, p, r);
+  add_dd_for<MT>(P, p, r);
+  float cpre[9], cend[9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    cpre[c] = (KIND == KIND_FIRST || KIND == KIND_MID) ? L[S::comb + c][t] : 0.0f;
+    cend[c] = KIND == KIND_MID ? L[S::comb + 9 + c][t] : (KIND == KIND_LAST ? L[S::comb + c][t]
+                                                                              : 0.0f);
+  }
+  combine_store<KIND>(P, p, r, cpre, cend);
+}
+
+template <int KIND, int MT, int BY>
+static void launch_rows_lds(hipStream_t st, const StageParams& P, BoxParams B, int code)
+{
+  const int tiles_x = (B.ni + 63) / 64;
+  if (B.gx <= 0) B.gx = tiles_x;
+  if (B.gy <= 0) B.gy = 2;
+  const int grid = grouped_grid(B, BY);
+  dim3 blk(64, BY, 1);
+#define CGFDH_RL(c, DX, DY, DZ)                                                          \
+  case c:                                                                                \
+    hipLaunchKernelGGL((stage_rows_lds_kernel<DX, DY, DZ, KIND, MT, BY>), dim3(grid), blk, \
+                       0, st, P, B);                                                     \
+    break;
+  switch (code) {
+    CGFDH_RL(0, 0, 0, 0) CGFDH_RL(1, 0, 0, 1) CGFDH_RL(2, 0, 1, 0) CGFDH_RL(3, 0, 1, 1)
+    CGFDH_RL(4, 1, 0, 0) CGFDH_RL(5, 1, 0, 1) CGFDH_RL(6, 1, 1, 0) CGFDH_RL(7, 1, 1, 1)
+  }
+#undef CGFDH_RL
+}
+
 // the LDS-tiled plane kernel in phases for the elastic media: the tile loads
 // first, then every other operand of the point (z stencil, metric, medium,
 // RK combine inputs); the LDS stores wait only for the tile loads, so the

@@ -536,10 +536,14 @@ int cgfdh_last_timing(cgfdh_solver *s, double *run_ms, double *stage_ms,
  * and free-surface shell; 16 = the two-phase kernel over whole rows with the
  * PML faces folded in (free-surface rows on the per-point kernel); 17 = the
  * two-phase interior kernel on the tiles clear of every PML face plus a
- * shell kernel that requests each PML slab's memory variables first.
- * Default 16 for the elastic media (isotropic, VTI, anisotropic), 4 for the
- * visco-elastic medium.  Variants 1..3 exist for the
- * isotropic medium only, 6, 7 and 12..17 not for the visco-elastic medium.
+ * shell kernel that requests each PML slab's memory variables first; 18 =
+ * variant 16 with the metric, medium, slowness and RK combine inputs loaded
+ * straight to LDS (global_load_lds), so the stencil values alone hold
+ * registers and four waves fit per SIMD.
+ * Default 16 for the isotropic and VTI media, 18 for the anisotropic medium
+ * (whose 21 coefficients otherwise hold registers), 4 for the visco-elastic
+ * medium.  Variants 1..3 exist for the
+ * isotropic medium only, 6, 7 and 12..18 not for the visco-elastic medium.
  * All variants produce identical bits. */
 int cgfdh_set_kernel_variant(cgfdh_solver *s, int variant);
 

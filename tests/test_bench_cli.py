@@ -17,9 +17,10 @@ def test_more_than_one_gpu_exits_with_a_message():
 def test_kernel_table_and_traffic_lookup():
     sys.path.insert(0, ROOT)
     import bench
-    assert len(bench.KERNELS) == 18          # one entry per kernel variant 0..17
+    assert len(bench.KERNELS) == 19          # one entry per kernel variant 0..18
     assert bench.KERNELS[12][0] == "stage_interior_phased"
     assert bench.KERNELS[16][0] == "stage_rows_phased"
+    assert bench.KERNELS[18][0] == "stage_rows_lds"
     t = bench.measured_traffic("stage_interior_phased", "iso600x600x300")
     assert t is not None and 2.4e10 < t < 4.0e10     # above the 24.0 GB algorithmic bytes
     assert bench.measured_traffic("no_such_kernel", "iso600x600x300") is None

@@ -23,9 +23,9 @@ CONFIGS = [
 # per-point, lean-interior and vectorised variants of the VTI and
 # anisotropic media, the per-point and lean-interior ones of the
 # visco-elastic medium
-MEDIUM_VARIANTS = ([("iso", v) for v in range(18)] +
+MEDIUM_VARIANTS = ([("iso", v) for v in range(19)] +
                    [(m, v) for m in ("vti", "aniso")
-                    for v in (0, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17)] +
+                    for v in (0, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18)] +
                    [("visco", v) for v in (0, 4, 5, 8, 9, 10, 11)])
 
 
@@ -53,8 +53,8 @@ def test_onestage_all_pairs(name, free, pml, medium, variant):
     assert mism == 0, "expected bit-identical RHS"
 
 
-RUN_VARIANTS = ([("iso", v) for v in range(18)] +
-                [(m, v) for m in ("vti", "aniso") for v in (0, 4, 6, 8, 11, 12, 16, 17)] +
+RUN_VARIANTS = ([("iso", v) for v in range(19)] +
+                [(m, v) for m in ("vti", "aniso") for v in (0, 4, 6, 8, 11, 12, 16, 17, 18)] +
                 [("visco", v) for v in (0, 4, 8, 11)])
 
 
@@ -142,7 +142,7 @@ def test_engine_matches_reference_golden():
     assert np.array_equal(w[:, 3 + nk - 1, 3:3 + nj, 3:3 + ni], g["surface"])
 
 
-@pytest.mark.parametrize("variant", [4, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17])
+@pytest.mark.parametrize("variant", [4, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18])
 def test_midsize_run_matches_oracle(variant):
     """120x110x80 grid, 12 steps of the benchmark's kind of workload (topography,
     random medium, double couple, PML 10 on 5 faces), engine vs oracle; the
@@ -216,7 +216,7 @@ def test_full_size_linearity():
     s.close()
 
 
-@pytest.mark.parametrize("variant", [16, 17])
+@pytest.mark.parametrize("variant", [16, 17, 18])
 def test_wide_run_matches_oracle(variant):
     """A grid wide enough for every box of the whole-row (16) and phased-shell
     (17) layouts: x edge tiles, a 64-point interior, y and z PML slabs in
