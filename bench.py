@@ -333,7 +333,8 @@ def main():
                          "15 variant 12 built for 3 waves per SIMD, 16 whole-row two-phase "
                          "kernel with the PML faces folded in, 17 variant 12 with a two-phase "
                          "shell, 18 variant 16 with the operands that have no stencil loaded "
-                         "straight to LDS (4 waves per SIMD)")
+                         "straight to LDS (4 waves per SIMD); with --workload batch a "
+                         "given variant serves every grid and the autotune is skipped")
     ap.add_argument("--workload", default="iso", choices=["iso", "vti", "aniso", "visco", "batch"])
     ap.add_argument("--size", type=str, default=None,
                     help="grid NIxNJxNK (default 600x600x300 iso, 400x400x200 media)")
