@@ -241,6 +241,121 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def _media_model_file(path, x0, y0, x1, y1, d=100.0):
+    """A three-layer isotropic .md3lay over [x0, x1] x [y0, y1]: a flat top
+    above the grid, two undulating interfaces at about -1.5 and -6 km, the
+    default workload's layer velocities with depth gradients."""
+    import numpy as np
+    xs = np.arange(x0, x1 + d, d, dtype=np.float64)
+    ys = np.arange(y0, y1 + d, d, dtype=np.float64)
+    X, Y = np.meshgrid(xs, ys)
+    elev = [np.full(X.shape, 2000.0), -1500.0 + 300.0 * np.sin(X / 4000.0) * np.cos(Y / 5000.0),
+            -6000.0 + 800.0 * np.sin(X / 7000.0 + 1.0) * np.sin(Y / 6000.0)]
+    base = [(2400.0, 4500.0, 2600.0), (2670.0, 6000.0, 3464.0), (2900.0, 6800.0, 3900.0)]
+    grads = (0.02, 0.05, 0.03)
+    with open(path, "w") as fp:
+        fp.write(f"elastic_isotropic\n3\n{len(xs)} {len(ys)} {x0:.9g} {y0:.9g} {d:.9g} {d:.9g}\n")
+        for L in range(3):
+            e = elev[L].reshape(-1)
+            cols = [e]
+            for q in range(3):
+                cols += [np.full(e.shape, base[L][q]), np.full(e.shape, grads[q]),
+                         np.full(e.shape, 1.0)]
+            fp.write("\n".join(" ".join(f"{v:.9g}" for v in row) for row in zip(*cols)) + "\n")
+    return len(xs), len(ys)
+
+
+def bench_media(args):
+    """The media discretisation of the default grid (600x600x300 with its
+    topography, ghosts included: 112 M points) from a three-layer .md3lay
+    with the harmonic averaging, on the GPU (cgfdh_media_layer2model_el_iso,
+    file read, upload, kernels, download); the reference's own
+    media_layer2model_el_iso (oracle/_ref, single-threaded) on a column of
+    the same grid through every depth, scaled by point count."""
+    import tempfile
+    import numpy as np
+    from cgfd3d_amd import L2M_EL_ISO, media_layer2model, workloads
+    ni, nj, nk = 600, 600, 300
+    dh = 50.0
+    top = workloads.topography(ni, nj, dh, 20, 2024)
+    xyz = workloads.layered_coords(ni, nj, nk, dh, top, (nk - 1) * dh)
+    n = xyz[0].size
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "layers.md3lay")
+        _media_model_file(path, -500.0, -500.0, 30500.0, 30500.0)
+        media_layer2model(L2M_EL_ISO, path, xyz[:, :8, :8, :8], args.media_method)   # warm-up
+        times = []
+        for _ in range(2):
+            t = time.perf_counter()
+            out = media_layer2model(L2M_EL_ISO, path, xyz, args.media_method)
+            times.append(time.perf_counter() - t)
+        gpu_s = min(times)
+        if not all(np.isfinite(o).all() for o in out):
+            raise RuntimeError("non-finite media values")
+        del out
+        cpu = None
+        if not args.no_cpu:
+            cpu = media_reference_baseline(path, xyz, args.media_method, n)
+    log(f"media {args.media_method}: {gpu_s * 1e3:.1f} ms on the GPU for {n / 1e6:.1f} M points")
+    print(json.dumps({
+        "metric": "media discretisation grid points/s", "value": n / gpu_s,
+        "unit": "grid points/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": gpu_s * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32/f64", "data": "synthetic",
+        "config": {"workload": f"media_layer2model_el_iso '{args.media_method}' on the 600x600x300 "
+                               "grid with topography (606x606x306 points with ghosts), 3-layer "
+                               ".md3lay with undulating interfaces and power-law gradients; "
+                               "file read + upload + kernels + download",
+                   "parallelism": "single GPU"},
+        "roofline": None, "cpu_baseline": cpu}), flush=True)
+
+
+def media_reference_baseline(path, xyz, method, n_full, ncol=64):
+    """The reference's media_layer2model_el_iso on an ncol x ncol column of
+    the grid through every depth (its stdout progress discarded), one core;
+    its file parse, timed on one point, is counted once, not per point."""
+    import ctypes as C
+    import numpy as np
+    from oracle import pyoracle as po
+    if not po.ref_available():
+        return None
+    ref = po.load_ref()
+    nz, ny, nx = xyz.shape[1:]
+    j0, i0 = ny // 2 - ncol // 2, nx // 2 - ncol // 2
+    sub = [np.ascontiguousarray(xyz[d][:, j0:j0 + ncol, i0:i0 + ncol]) for d in range(3)]
+    outs = [np.zeros(sub[0].shape, np.float32) for _ in range(3)]
+    fn = ref.media_layer2model_el_iso
+    fn.argtypes = [C.c_void_p] * 6 + [C.c_size_t] * 3 + [C.c_int, C.c_char_p, C.c_char_p, C.c_int]
+    sys.stdout.flush()
+    saved = os.dup(1)
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(devnull, 1)
+    try:
+        one = [np.ascontiguousarray(a[:1, :1, :1]) for a in sub]
+        o1 = [np.zeros(1, np.float32) for _ in range(3)]
+        t = time.perf_counter()
+        fn(*[o.ctypes.data for o in o1], *[a.ctypes.data for a in one], 1, 1, 1, 3,
+           path.encode(), b"loc", 1)
+        parse_s = time.perf_counter() - t
+        t = time.perf_counter()
+        fn(*[o.ctypes.data for o in outs], *[a.ctypes.data for a in sub], ncol, ncol, nz, 3,
+           path.encode(), method.encode(), 1)
+        cpu_s = time.perf_counter() - t
+    finally:
+        os.dup2(saved, 1)
+        os.close(saved)
+        os.close(devnull)
+    npts = sub[0].size
+    work_s = max(cpu_s - parse_s, 1e-9)
+    whole = parse_s + work_s * n_full / npts
+    return {"value": n_full / whole, "unit": "grid points/s", "cores": 1, "kind": "reference",
+            "sample": f"the reference's media_layer2model_el_iso ('{method}', compiled -O3, "
+                      f"one thread as the reference runs it) on a {ncol}x{ncol}x{nz} column of "
+                      f"the same grid through every depth: {cpu_s:.2f} s for {npts} points, of "
+                      f"which {parse_s:.2f} s read the file; the whole grid at that rate: "
+                      f"{whole:.0f} s"}
+
+
 def bench_outputs(args, wl, s):
     """The workload with output every step: K z slices (all 9 components)
     and the free surface's velocities, packed on the device and copied to
@@ -335,7 +450,10 @@ def main():
                          "shell, 18 variant 16 with the operands that have no stencil loaded "
                          "straight to LDS (4 waves per SIMD); with --workload batch a "
                          "given variant serves every grid and the autotune is skipped")
-    ap.add_argument("--workload", default="iso", choices=["iso", "vti", "aniso", "visco", "batch"])
+    ap.add_argument("--workload", default="iso",
+                    choices=["iso", "vti", "aniso", "visco", "batch", "media"])
+    ap.add_argument("--media-method", default="har", choices=["loc", "har", "ari"],
+                    help="with --workload media: the equivalent-medium method")
     ap.add_argument("--size", type=str, default=None,
                     help="grid NIxNJxNK (default 600x600x300 iso, 400x400x200 media)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -369,6 +487,8 @@ def main():
 
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "media":
+        return bench_media(args)
     size = args.size or ("600x600x300" if args.workload == "iso" else "400x400x200")
     ni, nj, nk = (int(v) for v in size.split("x"))
     t = time.perf_counter()
