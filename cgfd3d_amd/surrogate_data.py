@@ -189,6 +189,37 @@ def receiver_features(params, grid):
     return x
 
 
+def arrival_shifts(params, grid, vmax=8500.0, relief=600.0):
+    """Per (source, receiver) row, receivers fastest: a lower bound of the
+    first arrival in samples, floor(distance / vmax / dt), the distance
+    bounded below by the horizontal offset and the source depth less the
+    topography's relief (so the bound never passes the true P arrival of a
+    medium slower than vmax).  [n * nrecv] int."""
+    p = np.asarray(params, np.float64)
+    rxy = receiver_xy(grid)
+    dx = rxy[None, :, 0] - p[:, None, 0]
+    dy = rxy[None, :, 1] - p[:, None, 1]
+    dz = np.maximum(p[:, None, 2] - relief, 0.0)
+    dist = np.sqrt(dx * dx + dy * dy + dz * dz)
+    s = np.floor(dist / vmax / grid.dt).astype(np.int64)
+    return np.clip(s, 0, grid.nt - 1).reshape(-1)
+
+
+def shift_rows(rows, shifts, back=False):
+    """rows [m, nt, c] moved earlier by shifts[m] samples (zeros after the
+    end), or with back later by them (zeros before the start): the
+    arrival-aligned form and its inverse."""
+    out = np.zeros_like(rows)
+    nt = rows.shape[1]
+    for m, s in enumerate(shifts):
+        s = int(s)
+        if back:
+            out[m, s:] = rows[m, :nt - s]
+        else:
+            out[m, :nt - s] = rows[m, s:]
+    return out
+
+
 def to_receiver_layout(seis):
     """[n, nrecv, 3, nt] -> [n * nrecv, nt, 3] (one row block per trace set)."""
     n, nr, nc, nt = seis.shape
