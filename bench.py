@@ -437,7 +437,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--variant", type=int, default=None,
                     help="stage kernels (default: the engine's choice for the medium, 16 "
-                         "isotropic and VTI, 18 anisotropic, 4 visco-elastic): 0 per-point, 1 "
+                         "isotropic, VTI and visco-elastic, 18 anisotropic): 0 per-point, 1 "
                          "tiled, 2/3 z-marching interior + "
                          "shell, 4/5 per-point interior (64x4 / 64x2) + shell, 6/7 "
                          "x-vectorised interior (2 / 4 points per thread) + shell, 8/9 "

@@ -362,8 +362,7 @@ extern "C" int cgfdh_create(const cgfdh_desc* d, cgfdh_solver** out)
   // default stage kernels: the whole-row two-phase kernel with the PML faces
   // folded in for the elastic media, the per-point kernel for the
   // visco-elastic one
-  s->variant = d->medium_type == CGFDH_MEDIUM_VISCO_ISO ? 4
-               : d->medium_type == CGFDH_MEDIUM_ELASTIC_ANISO ? 18 : 16;
+  s->variant = d->medium_type == CGFDH_MEDIUM_ELASTIC_ANISO ? 18 : 16;
   if (!bad && hipStreamCreate(&s->stream) != hipSuccess) bad = 1;
   if (!bad && (hipStreamCreate(&s->stream2) != hipSuccess ||
                hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
@@ -908,10 +907,10 @@ extern "C" int cgfdh_set_kernel_variant(cgfdh_solver* s, int v)
   if (!s || v < 0 || v > 18) return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: %d", v);
   if (s->d.medium_type != CGFDH_MEDIUM_ELASTIC_ISO && v >= 1 && v <= 3)
     return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: variant %d is built for the isotropic "
-                "medium only (VTI / anisotropic: 0, 4..18; visco-elastic: 0, 4, 5, 8..11)", v);
-  if (s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO && (v == 6 || v == 7 || v >= 12))
+                "medium only (VTI / anisotropic: 0, 4..18; visco-elastic: 0, 4, 5, 8..11, 16)", v);
+  if (s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO && (v == 6 || v == 7 || (v >= 12 && v != 16)))
     return fail(CGFDH_EINVAL, "cgfdh_set_kernel_variant: variant %d is not built for the "
-                "visco-elastic medium (0, 4, 5, 8..11)", v);
+                "visco-elastic medium (0, 4, 5, 8..11, 16)", v);
   s->variant = v;
   return CGFDH_OK;
 }
@@ -2974,7 +2973,7 @@ extern "C" int cgfdh_batch_autotune(cgfdh_batch* b, int it0)
     hipError_t e = copy_state(true);
     const bool iso = s->d.medium_type == CGFDH_MEDIUM_ELASTIC_ISO;
     const bool visco = s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO;
-    std::vector<int> variants = visco ? std::vector<int>{4, 5, 8, 9}
+    std::vector<int> variants = visco ? std::vector<int>{4, 5, 8, 9, 16}
                                       : std::vector<int>{4, 6, 7, 8, 12, 14, 16, 18};
     if (iso) variants.push_back(3);
     const int groups[][2] = {{9, 2}, {3, 4}, {9, 8}};

@@ -371,7 +371,6 @@ template <int DX, int DY, int DZ, int KIND, int MT, int BY>
 __global__ void __launch_bounds__(64 * BY)
 stage_rows_phased_kernel(StageParams P, BoxParams B)
 {
-  static_assert(MT != MEDIUM_VISCO, "elastic media only");
   int bx, by, k;
   if (!grouped_block(B, bx, by, k)) return;
   // does this tile meet a PML face (block-uniform)?
@@ -407,8 +406,10 @@ stage_rows_phased_kernel(StageParams P, BoxParams B)
 #pragma unroll
   for (int n = 0; n < Medium<MT>::ncoef; n++) mv[n] = ld_stream(P.med + n * V + p);
   const float slw = ld_stream(P.slw + p);
+  // the elastic media's RK combine inputs with the other operands; the
+  // visco-elastic medium combines all its fields after the arithmetic
   float cpre[9], cend[9];
-  combine_load<KIND>(P, p, cpre, cend);
+  if constexpr (MT != MEDIUM_VISCO) combine_load<KIND>(P, p, cpre, cend);
   __builtin_amdgcn_sched_barrier(0);
   float X[9], Y[9], Z[9];
 #pragma unroll
@@ -419,7 +420,7 @@ stage_rows_phased_kernel(StageParams P, BoxParams B)
   }
   Medium<MT> md;
   md.from(mv);
-  float r[9];
+  float r[MT == MEDIUM_VISCO ? NW_MAX : 9];
   momentum(r, m, slw, X, Y, Z);
   float Xv[3] = {X[VX], X[VY], X[VZ]}, Yv[3] = {Y[VX], Y[VY], Y[VZ]},
         Zv[3] = {Z[VX], Z[VY], Z[VZ]};
@@ -432,9 +433,13 @@ stage_rows_phased_kernel(StageParams P, BoxParams B)
       }
     }
   }
+  if constexpr (MT == MEDIUM_VISCO) atten(P, p, md.lam, md.mu, r);
   if (P.nsrc > 0) add_sources(P, i, j, k, p, r);
   add_dd_for<MT>(P, p, r);
-  combine_store<KIND>(P, p, r, cpre, cend);
+  if constexpr (MT != MEDIUM_VISCO)
+    combine_store<KIND>(P, p, r, cpre, cend);
+  else
+    combine<KIND, NW_MAX>(P, p, r);
 }
 
 template <int KIND, int MT, int BY>

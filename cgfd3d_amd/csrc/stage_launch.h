@@ -113,28 +113,26 @@ static void launch_kind(const LaunchCtx& L, const StageParams& P, int dx, int dy
     b.ni = i1 - i0 + 1; b.nj = j1 - j0 + 1; b.nk = k1 - k0 + 1;
     return b;
   };
-  if constexpr (MT != MEDIUM_VISCO) {
-    if (L.variant == 16 || L.variant == 18) {
-      // whole rows, PML faces included, on the two-phase kernel (18: its
-      // operands without a stencil loaded straight to LDS); the
-      // free-surface rows on the general kernel
-      const int kmain = L.free_top ? L.nk2 - 4 : L.nk2;
-      const BoxParams rows = box(L.ni1, L.ni2, L.nj1, L.nj2, L.nk1, kmain);
-      bool done = false;
-      if constexpr (KIND != KIND_RHS) {
-        if (L.variant == 18) {
-          launch_rows_lds<KIND, MT, 4>(st, P, rows, code);
-          done = true;
-        }
+  if (L.variant == 16 || (MT != MEDIUM_VISCO && L.variant == 18)) {
+    // whole rows, PML faces included, on the two-phase kernel (18: its
+    // operands without a stencil loaded straight to LDS, elastic media);
+    // the free-surface rows on the general kernel
+    const int kmain = L.free_top ? L.nk2 - 4 : L.nk2;
+    const BoxParams rows = box(L.ni1, L.ni2, L.nj1, L.nj2, L.nk1, kmain);
+    bool done = false;
+    if constexpr (KIND != KIND_RHS && MT != MEDIUM_VISCO) {
+      if (L.variant == 18) {
+        launch_rows_lds<KIND, MT, 4>(st, P, rows, code);
+        done = true;
       }
-      if (!done) launch_rows_phased<KIND, MT, 4>(st, P, rows, code);
-      if (L.free_top)
-        launch_box<KIND, MT, HAS_SURF | HAS_PML>(L.shell_stream, P,
-                                                 box(L.ni1, L.ni2, L.nj1, L.nj2, kmain + 1,
-                                                     L.nk2),
-                                                 code);
-      return;
     }
+    if (!done) launch_rows_phased<KIND, MT, 4>(st, P, rows, code);
+    if (L.free_top)
+      launch_box<KIND, MT, HAS_SURF | HAS_PML>(L.shell_stream, P,
+                                               box(L.ni1, L.ni2, L.nj1, L.nj2, kmain + 1,
+                                                   L.nk2),
+                                               code);
+    return;
   }
   if constexpr (MT != MEDIUM_VISCO) {
     if (L.variant == 17) {

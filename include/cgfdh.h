@@ -540,10 +540,10 @@ int cgfdh_last_timing(cgfdh_solver *s, double *run_ms, double *stage_ms,
  * variant 16 with the metric, medium, slowness and RK combine inputs loaded
  * straight to LDS (global_load_lds), so the stencil values alone hold
  * registers and four waves fit per SIMD.
- * Default 16 for the isotropic and VTI media, 18 for the anisotropic medium
- * (whose 21 coefficients otherwise hold registers), 4 for the visco-elastic
- * medium.  Variants 1..3 exist for the
- * isotropic medium only, 6, 7 and 12..18 not for the visco-elastic medium.
+ * Default 16 for the isotropic, VTI and visco-elastic media, 18 for the
+ * anisotropic medium (whose 21 coefficients otherwise hold registers).
+ * Variants 1..3 exist for the isotropic medium only; the visco-elastic
+ * medium has 0, 4, 5, 8..11 and 16.
  * All variants produce identical bits. */
 int cgfdh_set_kernel_variant(cgfdh_solver *s, int variant);
 

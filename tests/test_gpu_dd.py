@@ -44,7 +44,7 @@ def test_dd_onestage_matches_oracle(medium):
         s.close()
 
 
-RUN_VARIANTS = [("iso", v) for v in range(19)] + [("visco", v) for v in (0, 4, 5, 8, 9, 10, 11)]
+RUN_VARIANTS = [("iso", v) for v in range(19)] + [("visco", v) for v in (0, 4, 5, 8, 9, 10, 11, 16)]
 
 
 @pytest.mark.parametrize("medium,variant", RUN_VARIANTS)

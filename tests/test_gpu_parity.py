@@ -26,7 +26,7 @@ CONFIGS = [
 MEDIUM_VARIANTS = ([("iso", v) for v in range(19)] +
                    [(m, v) for m in ("vti", "aniso")
                     for v in (0, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18)] +
-                   [("visco", v) for v in (0, 4, 5, 8, 9, 10, 11)])
+                   [("visco", v) for v in (0, 4, 5, 8, 9, 10, 11, 16)])
 
 
 @pytest.mark.parametrize("medium,variant", MEDIUM_VARIANTS)
@@ -55,7 +55,7 @@ def test_onestage_all_pairs(name, free, pml, medium, variant):
 
 RUN_VARIANTS = ([("iso", v) for v in range(19)] +
                 [(m, v) for m in ("vti", "aniso") for v in (0, 4, 6, 8, 11, 12, 16, 17, 18)] +
-                [("visco", v) for v in (0, 4, 8, 11)])
+                [("visco", v) for v in (0, 4, 8, 11, 16)])
 
 
 @pytest.mark.parametrize("medium,variant", RUN_VARIANTS)
