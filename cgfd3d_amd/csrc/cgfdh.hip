@@ -1013,12 +1013,8 @@ static int shell_stream()
   return v;
 }
 
-static void launch_stage(cgfdh_solver* s, hipStream_t stream, const StageParams& P, int kind,
-                         int ipair, int istage)
+static LaunchCtx make_ctx(const cgfdh_solver* s, hipStream_t stream)
 {
-  const int dx = (kPairFlags[ipair][0] + istage) % 2;
-  const int dy = (kPairFlags[ipair][1] + istage) % 2;
-  const int dz = (kPairFlags[ipair][2] + istage) % 2;
   LaunchCtx L;
   L.stream = stream;
   L.variant = s->variant;
@@ -1033,6 +1029,20 @@ static void launch_stage(cgfdh_solver* s, hipStream_t stream, const StageParams&
     L.pml_layers[f] = s->d.pml_layers[f];
   }
   L.shell_stream = stream;
+  L.slab_only = 0;
+  return L;
+}
+
+// slab_only: variant 16's free-surface rows alone (the batch's fused launch
+// runs the whole rows of all its grids)
+static void launch_stage(cgfdh_solver* s, hipStream_t stream, const StageParams& P, int kind,
+                         int ipair, int istage, bool slab_only = false)
+{
+  const int dx = (kPairFlags[ipair][0] + istage) % 2;
+  const int dy = (kPairFlags[ipair][1] + istage) % 2;
+  const int dz = (kPairFlags[ipair][2] + istage) % 2;
+  LaunchCtx L = make_ctx(s, stream);
+  L.slab_only = slab_only ? 1 : 0;
   bool two = false;
   if (shell_stream() && s->variant >= 2) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -1118,22 +1128,11 @@ static void enqueue_ablexp(cgfdh_solver* s, hipStream_t st)
   }
 }
 
-static int enqueue_step(cgfdh_solver* s, hipStream_t st, int it, const int* it_off,
-                        hipEvent_t* ev)
+// what follows the four stages of step it (drv_rk_curv_col.c:414-544): the
+// visco-elastic surface rotation, Graves-Qs, the sponge, peak ground
+// motion, receivers, then the pre/end swap
+static int enqueue_post_step(cgfdh_solver* s, hipStream_t st, int it, const int* it_off)
 {
-  const int ipair = it % 8;
-  float* tA = s->lvl[1];
-  float* tB = s->lvl[2];
-  for (int sg = 0; sg < 4; sg++) {
-    const float* wcur = sg == 0 ? s->lvl[0] : (sg % 2 == 1 ? tA : tB);
-    float* wtmp = sg % 2 == 0 ? tA : tB;
-    StageParams P = make_params(s, it, sg, wcur, wtmp, sg == 0 ? 0 : 1);
-    P.it_off = it_off;
-    const int kind = sg == 0 ? KIND_FIRST : (sg == 3 ? KIND_LAST : KIND_MID);
-    if (ev) HIPCHK(hipEventRecord(ev[2 * sg], st));
-    launch_stage(s, st, P, kind, ipair, sg);
-    if (ev) HIPCHK(hipEventRecord(ev[2 * sg + 1], st));
-  }
   if (s->d.medium_type == CGFDH_MEDIUM_VISCO_ISO && s->d.free_top)   // drv_rk_curv_col.c:440-445
     hipLaunchKernelGGL(vis_free_kernel, dim3((s->d.ni + 255) / 256, s->d.nj), dim3(256), 0, st,
                        s->lvl[3], s->mats + 27 * s->ss, s->d.ni, s->d.nj, s->nx, s->ss, s->vol,
@@ -1159,6 +1158,25 @@ static int enqueue_step(cgfdh_solver* s, hipStream_t st, int it, const int* it_o
   for (auto& f : s->face)
     if (f.on) std::swap(f.aux[0], f.aux[2]);
   return CGFDH_OK;
+}
+
+static int enqueue_step(cgfdh_solver* s, hipStream_t st, int it, const int* it_off,
+                        hipEvent_t* ev)
+{
+  const int ipair = it % 8;
+  float* tA = s->lvl[1];
+  float* tB = s->lvl[2];
+  for (int sg = 0; sg < 4; sg++) {
+    const float* wcur = sg == 0 ? s->lvl[0] : (sg % 2 == 1 ? tA : tB);
+    float* wtmp = sg % 2 == 0 ? tA : tB;
+    StageParams P = make_params(s, it, sg, wcur, wtmp, sg == 0 ? 0 : 1);
+    P.it_off = it_off;
+    const int kind = sg == 0 ? KIND_FIRST : (sg == 3 ? KIND_LAST : KIND_MID);
+    if (ev) HIPCHK(hipEventRecord(ev[2 * sg], st));
+    launch_stage(s, st, P, kind, ipair, sg);
+    if (ev) HIPCHK(hipEventRecord(ev[2 * sg + 1], st));
+  }
+  return enqueue_post_step(s, st, it, it_off);
 }
 
 extern "C" int cgfdh_run(cgfdh_solver* s, int it0, int nsteps)
@@ -2799,6 +2817,13 @@ struct cgfdh_batch {
   std::vector<hipEvent_t> ev;       // fork / join
   hipEvent_t t[2];
   double last_ms = 0;
+  // fused cycle: the whole-row stage of every grid in one launch; device
+  // copies of the 32 stages' parameters of every member, their row boxes
+  // and the block offsets of each member in the launch
+  bool fused = false;
+  StageParams* fz_p = nullptr;
+  BoxParams* fz_b = nullptr;
+  int* fz_start = nullptr;
 };
 
 static void batch_drop_graph(cgfdh_batch* b)
@@ -2840,6 +2865,9 @@ extern "C" int cgfdh_batch_destroy(cgfdh_batch* b)
   (void)hipEventDestroy(b->t[0]);
   (void)hipEventDestroy(b->t[1]);
   (void)hipFree(b->it_off);
+  (void)hipFree(b->fz_p);
+  (void)hipFree(b->fz_b);
+  (void)hipFree(b->fz_start);
   (void)hipStreamDestroy(b->stream);
   delete b;
   return CGFDH_OK;
@@ -2874,6 +2902,134 @@ static int batch_capture(cgfdh_batch* b, int it0)
     return fail(CGFDH_EDEVICE, "cgfdh_batch: graph capture failed: %s", hipGetErrorString(e));
   }
   b->cap_it0 = it0;
+  b->fused = false;
+  return CGFDH_OK;
+}
+
+// the fused cycle (opt-in: CGFDH_BATCH_FUSED=1) serves batches whose
+// members all run variant 16 on one isotropic, VTI or visco-elastic medium
+static bool batch_fusable(const cgfdh_batch* b)
+{
+  const char* e = getenv("CGFDH_BATCH_FUSED");
+  if (!e || e[0] != '1') return false;
+  const int mt = b->sv[0]->d.medium_type;
+  if (mt != CGFDH_MEDIUM_ELASTIC_ISO && mt != CGFDH_MEDIUM_ELASTIC_VTI &&
+      mt != CGFDH_MEDIUM_VISCO_ISO)
+    return false;
+  for (auto s : b->sv)
+    if (s->variant != 16 || s->d.medium_type != mt) return false;
+  return true;
+}
+
+// capture steps it0..it0+7 of every solver into one graph whose stages are
+// single launches over all grids' whole rows (a cross-grid tile queue:
+// stage_rows_multi_kernel); each member's free-surface rows and its
+// post-step work run on its own stream beside them
+static int batch_capture_fused(cgfdh_batch* b, int it0)
+{
+  batch_drop_graph(b);
+  const int n = (int)b->sv.size();
+  std::vector<BoxParams> boxes(n);
+  std::vector<int> start(n + 1, 0);
+  for (int i = 0; i < n; i++) {
+    const long nb = start[i] + (long)rows_box(make_ctx(b->sv[i], b->stream), boxes[i]);
+    if (nb > INT_MAX) return fail(CGFDH_EINVAL, "cgfdh_batch: fused launch too large");
+    start[i + 1] = (int)nb;
+  }
+  (void)hipFree(b->fz_p);
+  (void)hipFree(b->fz_b);
+  (void)hipFree(b->fz_start);
+  b->fz_p = nullptr; b->fz_b = nullptr; b->fz_start = nullptr;
+  HIPCHK(hipMalloc((void**)&b->fz_p, (size_t)32 * n * sizeof(StageParams)));
+  HIPCHK(hipMalloc((void**)&b->fz_b, (size_t)n * sizeof(BoxParams)));
+  HIPCHK(hipMalloc((void**)&b->fz_start, (size_t)(n + 1) * sizeof(int)));
+  HIPCHK(hipMemcpy(b->fz_b, boxes.data(), (size_t)n * sizeof(BoxParams), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->fz_start, start.data(), (size_t)(n + 1) * sizeof(int),
+                   hipMemcpyHostToDevice));
+  b->keys.clear();
+  for (auto s : b->sv) b->keys.push_back(capture_key(s));
+  // the member's level pointers as they stand before the cycle: capture
+  // swaps them step by step, as replays will
+  std::vector<StageParams> ph((size_t)32 * n);
+  const int mt = b->sv[0]->d.medium_type;
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeGlobal));
+  int rc = CGFDH_OK;
+  hipError_t e = hipSuccess;
+  auto fork = [&]() {
+    if (e == hipSuccess) e = hipEventRecord(b->ev[2 * n], b->stream);
+  };
+  auto join = [&](int i) {
+    if (e == hipSuccess) e = hipEventRecord(b->ev[i], b->sv[i]->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->ev[i], 0);
+  };
+  for (int q = 0; q < 8 && e == hipSuccess && !rc; q++) {
+    const int it = it0 + q;
+    const int ipair = it % 8;
+    for (int sg = 0; sg < 4 && e == hipSuccess; sg++) {
+      const int kind = sg == 0 ? KIND_FIRST : (sg == 3 ? KIND_LAST : KIND_MID);
+      StageParams* pq = ph.data() + (size_t)(q * 4 + sg) * n;
+      fork();
+      for (int i = 0; i < n && e == hipSuccess; i++) {
+        cgfdh_solver* s = b->sv[i];
+        const float* wcur = sg == 0 ? s->lvl[0] : (sg % 2 == 1 ? s->lvl[1] : s->lvl[2]);
+        float* wtmp = sg % 2 == 0 ? s->lvl[1] : s->lvl[2];
+        pq[i] = make_params(s, it, sg, wcur, wtmp, sg == 0 ? 0 : 1);
+        pq[i].it_off = b->it_off;
+        if (!s->d.free_top) continue;
+        e = hipStreamWaitEvent(s->stream, b->ev[2 * n], 0);
+        if (e == hipSuccess) launch_stage(s, s->stream, pq[i], kind, ipair, sg, true);
+        join(i);
+      }
+      if (e != hipSuccess) break;
+      RowsMulti R;
+      R.P = b->fz_p + (size_t)(q * 4 + sg) * n;
+      R.B = b->fz_b;
+      R.start = b->fz_start;
+      R.n = n;
+      const int dx = (kPairFlags[ipair][0] + sg) % 2;
+      const int dy = (kPairFlags[ipair][1] + sg) % 2;
+      const int dz = (kPairFlags[ipair][2] + sg) % 2;
+      if (mt == CGFDH_MEDIUM_ELASTIC_VTI)
+        launch_rows_multi_vti(b->stream, R, start[n], kind, dx, dy, dz);
+      else if (mt == CGFDH_MEDIUM_VISCO_ISO)
+        launch_rows_multi_visco(b->stream, R, start[n], kind, dx, dy, dz);
+      else
+        launch_rows_multi_iso(b->stream, R, start[n], kind, dx, dy, dz);
+    }
+    fork();
+    for (int i = 0; i < n && e == hipSuccess && !rc; i++) {
+      cgfdh_solver* s = b->sv[i];
+      e = hipStreamWaitEvent(s->stream, b->ev[2 * n], 0);
+      if (e == hipSuccess) rc = enqueue_post_step(s, s->stream, it, b->it_off);
+      join(i);
+    }
+  }
+  hipError_t ee = hipStreamEndCapture(b->stream, &g);
+  if (e == hipSuccess) e = ee;
+  if (e == hipSuccess && !rc) e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  if (e == hipSuccess && !rc)
+    e = hipMemcpy(b->fz_p, ph.data(), ph.size() * sizeof(StageParams), hipMemcpyHostToDevice);
+  if (rc || e != hipSuccess) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+    // capture swapped the members' level pointers as the steps went; the
+    // cycle never ran, so put back the pointers the keys describe
+    for (int i = 0; i < n; i++) {
+      cgfdh_solver* s = b->sv[i];
+      if (s->lvl[0] != b->keys[i].lvl0) std::swap(s->lvl[0], s->lvl[3]);
+      for (int f = 0; f < 6; f++)
+        if (s->face[f].on && s->face[f].aux[0] != b->keys[i].aux0[f])
+          std::swap(s->face[f].aux[0], s->face[f].aux[2]);
+    }
+    b->keys.clear();
+    return rc ? rc
+              : fail(CGFDH_EDEVICE, "cgfdh_batch: fused graph capture failed: %s",
+                     hipGetErrorString(e));
+  }
+  b->cap_it0 = it0;
+  b->fused = true;
   return CGFDH_OK;
 }
 
@@ -2896,11 +3052,12 @@ extern "C" int cgfdh_batch_run(cgfdh_batch* b, int it0, int nsteps)
   if (cycles > 0) {
     // a captured cycle serves every start step congruent to it mod 8, while
     // no member's configuration or level pointers have changed since
-    bool stale = false;
+    const bool fuse = batch_fusable(b);
+    bool stale = b->fused != fuse;
     for (size_t i = 0; i < b->sv.size() && b->exec && !stale; i++)
       stale = !(capture_key(b->sv[i]) == b->keys[i]);
     if (!b->exec || stale || (it - b->cap_it0) % 8 != 0 || it < b->cap_it0) {
-      int rc = batch_capture(b, it);
+      int rc = fuse ? batch_capture_fused(b, it) : batch_capture(b, it);
       if (rc) return rc;
     }
     for (int c = 0; c < cycles; c++, it += 8) {

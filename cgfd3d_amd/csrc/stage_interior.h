@@ -368,11 +368,9 @@ static void launch_interior_phased(hipStream_t st, const StageParams& P, BoxPara
 // face corrections (and their memory-variable RK combine) to its PML lanes
 // after the common arithmetic.  The grouped block order spans whole rows.
 template <int DX, int DY, int DZ, int KIND, int MT, int BY>
-__global__ void __launch_bounds__(64 * BY)
-stage_rows_phased_kernel(StageParams P, BoxParams B)
+__device__ __forceinline__ void rows_phased_tile(const StageParams& P, const BoxParams& B, int bx,
+                                                 int by, int k)
 {
-  int bx, by, k;
-  if (!grouped_block(B, bx, by, k)) return;
   // does this tile meet a PML face (block-uniform)?
   const int ti0 = B.i0 + bx * 64, ti1 = min(ti0 + 63, B.i0 + B.ni - 1);
   const int tj0 = B.j0 + by * BY, tj1 = min(tj0 + BY - 1, B.j0 + B.nj - 1);
@@ -440,6 +438,69 @@ stage_rows_phased_kernel(StageParams P, BoxParams B)
     combine_store<KIND>(P, p, r, cpre, cend);
   else
     combine<KIND, NW_MAX>(P, p, r);
+}
+
+template <int DX, int DY, int DZ, int KIND, int MT, int BY>
+__global__ void __launch_bounds__(64 * BY)
+stage_rows_phased_kernel(StageParams P, BoxParams B)
+{
+  int bx, by, k;
+  if (!grouped_block(B, bx, by, k)) return;
+  rows_phased_tile<DX, DY, DZ, KIND, MT, BY>(P, B, bx, by, k);
+}
+
+// the whole-row stage of several grids in one launch (the batch's
+// cross-grid tile queue): blocks start[q] .. start[q+1]-1 (multiples of 8)
+// belong to grid q, whose parameters and box sit in device memory; each
+// grid keeps its own XCD-grouped block order
+struct RowsMulti {
+  const StageParams* P;
+  const BoxParams* B;
+  const int* start;   // n + 1
+  int n;
+};
+
+template <int DX, int DY, int DZ, int KIND, int MT, int BY>
+__global__ void __launch_bounds__(64 * BY)
+__attribute__((amdgpu_waves_per_eu(KIND == KIND_MID ? 2 : 3))) stage_rows_multi_kernel(RowsMulti R)
+{
+  // the tables are read through the constant address space, as kernel
+  // arguments are: the kernel never writes them, so their fields load into
+  // scalar registers once instead of being re-read after every store; the
+  // grid index is wave-uniform
+  typedef const __attribute__((address_space(4))) StageParams CStageParams;
+  typedef const __attribute__((address_space(4))) BoxParams CBoxParams;
+  typedef const __attribute__((address_space(4))) int CInt;
+  CInt* start = (CInt*)R.start;
+  int lo = 0, hi = R.n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= (int)blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  lo = __builtin_amdgcn_readfirstlane(lo);
+  const BoxParams& B = *(const BoxParams*)((CBoxParams*)R.B + lo);
+  const int s0 = start[lo], s1 = start[lo + 1];
+  int bx, by, k;
+  if (!grouped_block_at(B, blockIdx.x - s0, s1 - s0, bx, by, k)) return;
+  rows_phased_tile<DX, DY, DZ, KIND, MT, BY>(*(const StageParams*)((CStageParams*)R.P + lo), B,
+                                             bx, by, k);
+}
+
+template <int KIND, int MT, int BY>
+static void launch_rows_multi(hipStream_t st, const RowsMulti& R, int nblocks, int code)
+{
+  dim3 blk(64, BY, 1);
+#define CGFDH_RM(c, DX, DY, DZ)                                                            \
+  case c:                                                                                  \
+    hipLaunchKernelGGL((stage_rows_multi_kernel<DX, DY, DZ, KIND, MT, BY>), dim3(nblocks),  \
+                       blk, 0, st, R);                                                     \
+    break;
+  switch (code) {
+    CGFDH_RM(0, 0, 0, 0) CGFDH_RM(1, 0, 0, 1) CGFDH_RM(2, 0, 1, 0) CGFDH_RM(3, 0, 1, 1)
+    CGFDH_RM(4, 1, 0, 0) CGFDH_RM(5, 1, 0, 1) CGFDH_RM(6, 1, 1, 0) CGFDH_RM(7, 1, 1, 1)
+  }
+#undef CGFDH_RM
 }
 
 template <int KIND, int MT, int BY>

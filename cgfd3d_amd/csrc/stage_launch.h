@@ -48,7 +48,24 @@ struct LaunchCtx {
   int tgx, tgy;            // tile group of the grouped block order (0: default)
   int merge_shell;         // 1: the five PML slabs of the shell in one launch
   hipStream_t shell_stream;  // stream of the shell boxes (= stream, or one beside it)
+  int slab_only;           // variant 16: launch only the free-surface rows (the batch's
+                           // fused launch runs the whole rows of every grid)
 };
+
+// the whole-row box of variant 16 with its block grouping resolved; returns
+// the launch's block count (a multiple of 8)
+inline int rows_box(const LaunchCtx& L, BoxParams& b)
+{
+  const int kmain = L.free_top ? L.nk2 - 4 : L.nk2;
+  memset(&b, 0, sizeof b);
+  b.gx = L.tgx; b.gy = L.tgy;
+  b.i0 = L.ni1; b.j0 = L.nj1; b.k0 = L.nk1;
+  b.ni = L.ni2 - L.ni1 + 1; b.nj = L.nj2 - L.nj1 + 1; b.nk = kmain - L.nk1 + 1;
+  const int tiles_x = (b.ni + 63) / 64;
+  if (b.gx <= 0) b.gx = tiles_x;
+  if (b.gy <= 0) b.gy = 2;
+  return grouped_grid(b, 4);
+}
 
 // per-point kernel: one thread per grid point, every stencil operand read
 // from global memory.  The plain reference form of the stage.
@@ -119,9 +136,9 @@ static void launch_kind(const LaunchCtx& L, const StageParams& P, int dx, int dy
     // the free-surface rows on the general kernel
     const int kmain = L.free_top ? L.nk2 - 4 : L.nk2;
     const BoxParams rows = box(L.ni1, L.ni2, L.nj1, L.nj2, L.nk1, kmain);
-    bool done = false;
+    bool done = L.slab_only != 0;
     if constexpr (KIND != KIND_RHS && MT != MEDIUM_VISCO) {
-      if (L.variant == 18) {
+      if (!done && L.variant == 18) {
         launch_rows_lds<KIND, MT, 4>(st, P, rows, code);
         done = true;
       }
@@ -254,6 +271,18 @@ void launch_stage_mt(const LaunchCtx& L, const StageParams& P, int kind, int dx,
   }
 }
 
+template <int MT>
+void launch_rows_multi_mt(hipStream_t st, const RowsMulti& R, int nblocks, int kind, int dx,
+                          int dy, int dz)
+{
+  const int code = dx * 4 + dy * 2 + dz;
+  switch (kind) {
+    case KIND_FIRST: launch_rows_multi<KIND_FIRST, MT, 4>(st, R, nblocks, code); break;
+    case KIND_MID: launch_rows_multi<KIND_MID, MT, 4>(st, R, nblocks, code); break;
+    default: launch_rows_multi<KIND_LAST, MT, 4>(st, R, nblocks, code); break;
+  }
+}
+
 // defined in stage_iso.hip, stage_vti.hip and stage_aniso.hip
 void launch_stage_iso(const LaunchCtx& L, const StageParams& P, int kind, int dx, int dy, int dz);
 void launch_stage_vti(const LaunchCtx& L, const StageParams& P, int kind, int dx, int dy, int dz);
@@ -261,5 +290,11 @@ void launch_stage_aniso(const LaunchCtx& L, const StageParams& P, int kind, int 
                         int dz);
 void launch_stage_visco(const LaunchCtx& L, const StageParams& P, int kind, int dx, int dy,
                         int dz);
+void launch_rows_multi_iso(hipStream_t st, const RowsMulti& R, int nblocks, int kind, int dx,
+                           int dy, int dz);
+void launch_rows_multi_vti(hipStream_t st, const RowsMulti& R, int nblocks, int kind, int dx,
+                           int dy, int dz);
+void launch_rows_multi_visco(hipStream_t st, const RowsMulti& R, int nblocks, int kind, int dx,
+                             int dy, int dz);
 
 }  // namespace cgfdh

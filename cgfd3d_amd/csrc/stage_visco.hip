@@ -10,4 +10,10 @@ void launch_stage_visco(const LaunchCtx& L, const StageParams& P, int kind, int 
   launch_stage_mt<MEDIUM_VISCO>(L, P, kind, dx, dy, dz);
 }
 
+void launch_rows_multi_visco(hipStream_t st, const RowsMulti& R, int nblocks, int kind, int dx,
+                          int dy, int dz)
+{
+  launch_rows_multi_mt<MEDIUM_VISCO>(st, R, nblocks, kind, dx, dy, dz);
+}
+
 }  // namespace cgfdh

@@ -558,6 +558,11 @@ int cgfdh_get_kernel_choice(cgfdh_solver *s, int *variant, int *gx, int *gy);
  * advances them all over the same steps, capturing 8-step cycles of all
  * solvers once into a hipGraph (one stream per solver, concurrent branches)
  * and replaying it; results equal cgfdh_run of each solver bit for bit.
+ * With the environment variable CGFDH_BATCH_FUSED=1, when every solver runs
+ * variant 16 on one isotropic, VTI or visco-elastic medium, each RK stage of
+ * the cycle is instead one launch over the whole rows of all grids (a
+ * cross-grid tile queue; each grid's free-surface rows and post-step work on
+ * its own stream beside it); same results.
  * cgfdh_batch_autotune times each solver's kernel variants and tile groups
  * for one step from the current state (restored afterwards) and keeps the
  * fastest. */

@@ -155,3 +155,62 @@ def test_batch_recaptures_after_reconfiguration():
     b.close()
     for s in sv:
         s.close()
+
+
+@pytest.mark.parametrize("medium", ["iso", "vti", "visco"])
+def test_fused_batch_equals_individual_runs(medium, monkeypatch):
+    """Members all on the whole-row kernel (variant 16) and one medium: the
+    batch's cycle runs each stage as one launch over every grid's rows (the
+    cross-grid tile queue).  Three uneven grids, one without a free surface,
+    one with Graves-Qs (elastic media): 19 steps with the fused cycle
+    (CGFDH_BATCH_FUSED=1), 8 more with the per-member cycle, 8 fused again
+    from another start step; bit-identical to each member's own cgfdh_run."""
+    from cgfd3d_amd import Batch
+    rng = np.random.default_rng(9)
+    cases = []
+    for n, (dims, pml, top) in enumerate([((70, 23, 19), (1, 1, 1, 1, 1, 0), True),
+                                          ((19, 22, 17), (1, 0, 1, 1, 1, 1), False),
+                                          ((24, 17, 21), (0, 1, 1, 0, 1, 0), True)]):
+        ni, nj, nk = dims
+        case = cs.make_case(ni, nj, nk, seed=40 + n, free_top=top, pml=pml, layers=3,
+                            medium=medium)
+        case["src"] = cs.ricker_point_source(case, 3 + ni // 2, 3 + nj // 2, 3 + nk // 2, NT,
+                                             DT, fc=10.0, t0=0.1)
+        ridx, rw = cs.surface_receivers(case, n=3, seed=n)
+        cases.append((case, ridx, rw))
+    qs = rng.uniform(10.0, 100.0, (cases[2][0]["nk"] + 6, cases[2][0]["nj"] + 6,
+                                   cases[2][0]["ni"] + 6)).astype(np.float32)
+
+    def state(s, n):
+        # peak ground motion exists on free-surface grids only
+        if cases[n][0]["free_top"]:
+            return _state(s)
+        return s.wavefield(), s.pml_aux(), s.seismograms()
+
+    def make(n):
+        case, ridx, rw = cases[n]
+        s = _solver(case, ridx, rw, variant=16)
+        if n == 2 and medium != "visco":     # Graves-Qs: elastic media only
+            s.set_graves_qs(qs, 2.0)
+        return s
+
+    ref = []
+    for n in range(3):
+        s = make(n)
+        s.run(0, 35)
+        ref.append(state(s, n))
+        s.close()
+    sv = [make(n) for n in range(3)]
+    b = Batch(sv)
+    monkeypatch.setenv("CGFDH_BATCH_FUSED", "1")
+    b.run(0, 19)
+    monkeypatch.setenv("CGFDH_BATCH_FUSED", "0")
+    b.run(19, 8)
+    monkeypatch.setenv("CGFDH_BATCH_FUSED", "1")
+    b.run(27, 8)
+    print(f"fused batch ({medium}): last call {b.last_ms():.2f} ms")
+    for n, s in enumerate(sv):
+        assert _same(state(s, n), ref[n]), f"solver {n}"
+    b.close()
+    for s in sv:
+        s.close()
